@@ -1,0 +1,264 @@
+"""Python host binding of the Aperture MI355X core (libaperture_hip.so).
+
+The core is a C ABI (include/aperture/ap_core.h); this module loads it with
+ctypes and mirrors the reference's host classes over it:
+
+* ``SceneRenderer`` -- the reference ``SceneRenderer`` (core/scene.hpp:19-154):
+  constructed from a SceneObject array, it owns the device scene.
+* ``Camera`` -- the reference ``Camera`` (core/camera.hpp:14-99).
+* ``trace_closest`` / ``trace_any`` -- the reference ``TraverseBVH<false/true>``
+  (core/render/intersection.cuh:235-397) as a batched operator over device
+  buffers.
+
+Device buffers are passed as torch tensors (``.data_ptr()``) or raw integer
+addresses; torch is only plumbing for device memory and streams. There is no
+CPU fallback: loading fails loudly when the shared library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaperture_hip.so")
+
+AP_OK = 0
+
+# ---------------------------------------------------------------------------
+# Layouts (include/aperture/ap_layout.h)
+# ---------------------------------------------------------------------------
+_F3 = ("<f4", (3,))
+SCENE_OBJECT_DTYPE = np.dtype(
+    {
+        "names": [
+            "center", "extent", "shading_model", "albedo", "emissive", "specular", "metallic",
+            "roughness", "ior", "sigma_a", "normal", "abbe_number", "tilt_number", "textures",
+            "world_to_object", "object_to_world", "type", "sdf_type", "sdf_info0",
+            "cached_volume_index", "v0_position", "v0_normal", "v0_uv", "v1_position", "v1_normal",
+            "v1_uv", "v2_position", "v2_normal", "v2_uv", "proxy_min", "proxy_max", "proxy_center",
+            "object_index", "light_index", "visible",
+        ],
+        "formats": [
+            _F3, _F3, "<i4", _F3, _F3, "<f4", "<f4", "<f4", "<f4", "<f4", _F3, "<f4", "<f4",
+            ("<u8", (4,)), ("<f4", (3, 3)), ("<f4", (3, 3)), "<i4", "<i4", ("<f4", (4,)), "<i4",
+            _F3, _F3, ("<f4", (2,)), _F3, _F3, ("<f4", (2,)), _F3, _F3, ("<f4", (2,)),
+            _F3, _F3, _F3, "<i4", "<i4", "u1",
+        ],
+        "offsets": [
+            0, 12, 32, 48, 60, 72, 76, 80, 84, 88, 92, 104, 108, 112, 160, 196, 232, 240, 256,
+            272, 288, 300, 312, 320, 332, 344, 352, 364, 376, 384, 396, 408, 420, 424, 428,
+        ],
+        "itemsize": 432,
+    }
+)
+RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("tmin", "<f4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("id", "<i4"), ("t", "<f4")])
+STATS_DTYPE = np.dtype([("nodes", "<u4"), ("prims", "<u4")])
+BVH_NODE_DTYPE = np.dtype(
+    {
+        "names": ["bound_min", "bound_max", "left", "right", "first", "count"],
+        "formats": [_F3, _F3, "<i4", "<i4", "<i4", "<i4"],
+        "offsets": [0, 12, 24, 28, 32, 36],
+        "itemsize": 48,
+    }
+)
+LIGHT_INFO_DTYPE = np.dtype([("object_index", "<i4"), ("importance_weight", "<f4"), ("select_prob", "<f4")])
+
+OBJ_CUBE, OBJ_SPHERE, OBJ_SDF, OBJ_TRIANGLE = 0, 1, 2, 3
+MAT_DIFFUSE, MAT_LIGHT, MAT_GLASS, MAT_GGX = 0, 1, 2, 3
+TRAVERSE_REFERENCE, TRAVERSE_CULLED = 0, 1
+
+
+class BuildParams(ctypes.Structure):
+    _fields_ = [("leaf_size", ctypes.c_int32), ("bin_count", ctypes.c_int32)]
+
+
+class BVHInfo(ctypes.Structure):
+    _fields_ = [
+        ("node_count", ctypes.c_int64),
+        ("interior_count", ctypes.c_int64),
+        ("prim_count", ctypes.c_int64),
+        ("max_depth", ctypes.c_int32),
+        ("stack_capacity", ctypes.c_int32),
+        ("device_bytes", ctypes.c_int64),
+        ("light_count", ctypes.c_int32),
+        ("traversal_mode", ctypes.c_int32),
+        ("line_margin", ctypes.c_float),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+class ApertureError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"aperture error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+# Every symbol include/aperture/ap_core.h and ap_synth.h declare.
+EXPORTED_SYMBOLS = (
+    "ap_scene_create", "ap_build_bvh", "ap_scene_destroy", "ap_scene_get_info", "ap_scene_set_traversal_mode",
+    "ap_scene_export_bvh",
+    "ap_trace_closest", "ap_trace_any", "ap_trace_closest_stats", "ap_last_error_string",
+    "ap_synth_triangle_soup", "ap_synth_camera_rays", "ap_synth_secondary_rays",
+)
+
+
+def lib():
+    """Loads libaperture_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+        L.ap_scene_create.argtypes = [vp, sz, ctypes.POINTER(BuildParams), ctypes.POINTER(vp)]
+        L.ap_scene_destroy.argtypes = [vp]
+        L.ap_build_bvh.argtypes = [vp, sz, ctypes.POINTER(BuildParams), vp, vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.ap_scene_get_info.argtypes = [vp, ctypes.POINTER(BVHInfo)]
+        L.ap_scene_set_traversal_mode.argtypes = [vp, i32]
+        L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
+        for name in ("ap_trace_closest", "ap_trace_any"):
+            getattr(L, name).argtypes = [vp, vp, sz, vp, vp]
+        L.ap_trace_closest_stats.argtypes = [vp, vp, sz, vp, vp, vp]
+        L.ap_last_error_string.restype = ctypes.c_char_p
+        L.ap_last_error_string.argtypes = []
+        L.ap_synth_triangle_soup.argtypes = [sz, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, vp]
+        L.ap_synth_camera_rays.argtypes = [i32, i32, vp, vp, ctypes.c_float, vp]
+        L.ap_synth_secondary_rays.argtypes = [vp, vp, vp, sz, ctypes.c_uint64, ctypes.c_float, vp,
+                                              ctypes.POINTER(ctypes.c_size_t)]
+        for name in EXPORTED_SYMBOLS:
+            if name != "ap_last_error_string":
+                getattr(L, name).restype = i32
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != AP_OK:
+        raise ApertureError(code, lib().ap_last_error_string().decode())
+    return code
+
+
+def _ptr(x):
+    """Address of a numpy array, torch tensor or integer."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    return x.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+# ---------------------------------------------------------------------------
+# Synthetic workloads (include/aperture/ap_synth.h)
+# ---------------------------------------------------------------------------
+def synth_triangle_soup(n, seed=2, edge_min=1e-3, edge_max=1e-2):
+    objs = np.zeros(n, dtype=SCENE_OBJECT_DTYPE)
+    check(lib().ap_synth_triangle_soup(n, seed, edge_min, edge_max, _ptr(objs)))
+    return objs
+
+
+def synth_camera_rays(width, height, origin, direction, fov_deg):
+    rays = np.zeros(width * height, dtype=RAY_DTYPE)
+    o = np.asarray(origin, dtype=np.float32)
+    d = np.asarray(direction, dtype=np.float32)
+    check(lib().ap_synth_camera_rays(width, height, _ptr(o), _ptr(d), fov_deg, _ptr(rays)))
+    return rays
+
+
+def build_bvh(scene_objects, leaf_size=8, bin_count=12):
+    """Host BVH build (BuildBVH, core/bvh/bvh.cuh:142-422): (nodes, object_indices)."""
+    objs = np.ascontiguousarray(scene_objects, dtype=SCENE_OBJECT_DTYPE)
+    n = len(objs)
+    nodes = np.zeros(max(2 * n, 1), dtype=BVH_NODE_DTYPE)
+    idx = np.zeros(max(n, 1), dtype=np.int32)
+    count = ctypes.c_size_t(0)
+    check(lib().ap_build_bvh(_ptr(objs), n, ctypes.byref(BuildParams(leaf_size, bin_count)), _ptr(nodes),
+                             _ptr(idx), ctypes.byref(count)))
+    return nodes[: count.value], idx[:n]
+
+
+class SceneRenderer:
+    """Owns one device scene; mirrors the reference SceneRenderer
+    (core/scene.hpp:113, core/scene.cu:1025-1118)."""
+
+    def __init__(self, scene_objects, leaf_size=8, bin_count=12):
+        objs = np.ascontiguousarray(scene_objects, dtype=SCENE_OBJECT_DTYPE)
+        self._objects = objs
+        self._handle = ctypes.c_void_p()
+        params = BuildParams(leaf_size, bin_count)
+        check(lib().ap_scene_create(_ptr(objs) if len(objs) else None, len(objs), ctypes.byref(params),
+                                    ctypes.byref(self._handle)))
+
+    @property
+    def handle(self):
+        return self._handle.value
+
+    def info(self):
+        out = BVHInfo()
+        check(lib().ap_scene_get_info(self.handle, ctypes.byref(out)))
+        return out
+
+    def set_traversal_mode(self, mode):
+        check(lib().ap_scene_set_traversal_mode(self.handle, mode))
+
+    def export_bvh(self):
+        inf = self.info()
+        nodes = np.zeros(inf.node_count, dtype=BVH_NODE_DTYPE)
+        idx = np.zeros(inf.prim_count, dtype=np.int32)
+        lights = np.zeros(inf.light_count, dtype=LIGHT_INFO_DTYPE)
+        check(lib().ap_scene_export_bvh(self.handle, _ptr(nodes), _ptr(idx), _ptr(lights)))
+        return nodes, idx, lights
+
+    def trace_closest(self, rays, hits, n=None, stream=None, stats=None):
+        """Closest hit of n device rays (ap_ray, 32 B) into device hits (ap_hit, 8 B)."""
+        n = n if n is not None else _numel_rays(rays)
+        if stats is None:
+            check(lib().ap_trace_closest(self.handle, _ptr(rays), n, _ptr(hits), _stream(stream)))
+        else:
+            check(lib().ap_trace_closest_stats(self.handle, _ptr(rays), n, _ptr(hits), _ptr(stats),
+                                               _stream(stream)))
+
+    def trace_any(self, rays, hits, n=None, stream=None):
+        n = n if n is not None else _numel_rays(rays)
+        check(lib().ap_trace_any(self.handle, _ptr(rays), n, _ptr(hits), _stream(stream)))
+
+    def secondary_rays(self, primaries, hits, seed=3, offset=1e-4):
+        """Host generator: one cosine-weighted ray per primary hit."""
+        primaries = np.ascontiguousarray(primaries, dtype=RAY_DTYPE)
+        hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+        out = np.zeros(len(primaries), dtype=RAY_DTYPE)
+        n_out = ctypes.c_size_t(0)
+        check(lib().ap_synth_secondary_rays(self.handle, _ptr(primaries), _ptr(hits), len(primaries), seed,
+                                            offset, _ptr(out), ctypes.byref(n_out)))
+        return out[: n_out.value]
+
+    def close(self):
+        if self._handle is not None and self._handle.value:
+            lib().ap_scene_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _numel_rays(rays):
+    if hasattr(rays, "nbytes"):
+        return int(rays.nbytes) // 32
+    return int(rays.numel() * rays.element_size()) // 32

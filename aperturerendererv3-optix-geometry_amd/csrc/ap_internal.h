@@ -1,0 +1,114 @@
+// Internal declarations of the Aperture core: device record layouts, the
+// scene object behind the C ABI handle, and error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "aperture/ap_core.h"
+#include "ap_vec.h"
+
+namespace ap {
+
+// ---------------------------------------------------------------------------
+// Device layouts of the acceleration structure.
+//
+// NodeRec (64 B, 64-B aligned): one per INTERIOR node of the reference BVH,
+// holding both children's boxes so that one 64-B read serves one expansion
+// (the reference reads the two child BVHNodes separately,
+// intersection.cuh:313-314). A child link >= 0 is the child's NodeRec index;
+// a link < 0 is a leaf whose primitives start at slot ~link.
+//
+// PrimRec (48 B): one per primitive slot of objectIndices_ (bvh.cuh:49),
+// slot-ordered so a leaf is a contiguous run; the last slot of each leaf
+// carries PRIM_LAST. Triangles store world-space p0 and the edges
+// e1 = p1 - p0, e2 = p2 - p0, computed once on the host with the same FP32
+// operations the reference performs per test (intersection.cuh:27-33).
+// ---------------------------------------------------------------------------
+struct alignas(64) NodeRec {
+    float lmin[3]; int32_t llink;
+    float lmax[3]; int32_t rlink;
+    float rmin[3]; int32_t pad0;
+    float rmax[3]; int32_t pad1;
+};
+static_assert(sizeof(NodeRec) == 64, "NodeRec is one 64-B line");
+
+enum : uint32_t {
+    PRIM_LAST = 1u,             // last primitive of its leaf
+    PRIM_KIND_SHIFT = 1u,
+    PRIM_KIND_TRIANGLE = 0u,
+    PRIM_KIND_NEVER = 1u,       // invisible or unsupported type: always a miss
+    PRIM_KIND_CUBE = 2u,        // analytic OBB (core/object/aabb.cuh:28-89)
+    PRIM_KIND_SPHERE = 3u,      // analytic sphere (core/object/sphere.cuh:37-70)
+};
+
+struct alignas(16) PrimRec {
+    float p0[3]; int32_t obj;     // obj = SceneObject index (the hit ID)
+    float e1[3]; uint32_t flags;  // PRIM_LAST | kind << PRIM_KIND_SHIFT
+    float e2[3]; int32_t aux;     // analytic record index for CUBE/SPHERE
+};
+static_assert(sizeof(PrimRec) == 48, "PrimRec is 48 B");
+
+// Transform data of an analytic (cube or sphere) primitive.
+struct alignas(16) AnalyticRec {
+    float center[3]; float pad0;
+    float extent[3]; float pad1;
+    float w2o[9];    float pad2[3];   // worldToObject_ rows
+    float o2w[9];    float pad3[3];   // objectToWorld_ rows
+};
+static_assert(sizeof(AnalyticRec) == 128, "AnalyticRec is 128 B");
+
+struct RootInfo {
+    float bmin[3];
+    float bmax[3];
+    int32_t link;
+    int32_t valid;   // 0 for an empty scene
+};
+
+struct DeviceScene {
+    const NodeRec* nodes;
+    const PrimRec* prims;
+    const AnalyticRec* analytic;
+    RootInfo root;
+    int32_t stack_cap;
+    float line_margin;   // world-space growth of boxes for the CULL line test
+    int32_t mode;        // AP_TRAVERSE_REFERENCE or AP_TRAVERSE_CULLED
+};
+
+// Reference-layout BVH build result (bvh.cuh:54-58, 32-bit node variant).
+struct HostBVH {
+    std::vector<ap_bvh_node> nodes;
+    std::vector<int32_t> indices;
+};
+
+HostBVH build_bvh(const std::vector<ap_primitive_proxy>& proxies, int leaf_size, int bin_count);
+int bvh_max_depth(const HostBVH& bvh);
+
+}  // namespace ap
+
+struct ap_scene {
+    int device = 0;
+    std::vector<ap_scene_object> objects;     // host copy, indices stamped
+    ap::HostBVH bvh;
+    std::vector<ap_light_info> lights;
+    int max_depth = 0;
+    int64_t interior_count = 0;
+    ap::DeviceScene dev{};
+    void* d_nodes = nullptr;
+    void* d_prims = nullptr;
+    void* d_analytic = nullptr;
+    int64_t device_bytes = 0;
+};
+
+namespace ap {
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// Kernel launchers (ap_trace.hip).
+hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
+                        ap_trace_stats* stats, bool any_hit, hipStream_t stream);
+}  // namespace ap

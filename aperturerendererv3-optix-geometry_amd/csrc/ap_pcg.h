@@ -1,0 +1,39 @@
+// PCG32 (XSH-RR variant, 64-bit state, O'Neill 2014, "PCG: A Family of
+// Simple Fast Space-Efficient Statistically Good Algorithms for Random
+// Number Generation"), host and device. The core uses it in place of the
+// reference's curand XORWOW streams (core/random.cuh:31-53): a stream per
+// pixel (or per ray) selected by the increment, seeded by the pixel index.
+#pragma once
+
+#include <cstdint>
+
+#include "ap_vec.h"
+
+namespace ap {
+
+struct Pcg32 {
+    uint64_t state;
+    uint64_t inc;
+};
+
+AP_HD uint32_t pcg32_next(Pcg32& r) {
+    const uint64_t old = r.state;
+    r.state = old * 6364136223846793005ULL + r.inc;
+    const uint32_t xorshifted = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = static_cast<uint32_t>(old >> 59u);
+    return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
+}
+
+// pcg32_srandom_r(initstate, initseq).
+AP_HD Pcg32 pcg32_seed(uint64_t initstate, uint64_t initseq) {
+    Pcg32 r{0u, (initseq << 1u) | 1u};
+    pcg32_next(r);
+    r.state += initstate;
+    pcg32_next(r);
+    return r;
+}
+
+// Uniform float in [0, 1) from the top 24 bits.
+AP_HD float pcg32_float(Pcg32& r) { return static_cast<float>(pcg32_next(r) >> 8) * 0x1p-24f; }
+
+}  // namespace ap

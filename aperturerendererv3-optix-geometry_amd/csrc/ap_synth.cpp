@@ -1,0 +1,184 @@
+// Synthetic benchmark scenes and ray sets (host C++).
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "aperture/ap_synth.h"
+#include "ap_internal.h"
+#include "ap_pcg.h"
+#include "ap_sample.h"
+
+namespace ap {
+
+// Default-constructed SceneObject with an identity transform
+// (SceneObject(center, extent, material, rotation = 0, type),
+// core/render/object.cuh:344-364; Material / GBuffer defaults,
+// object.cuh:60-91, 124-127).
+void init_object(ap_scene_object& o, f3 center, f3 extent, int type) {
+    std::memset(&o, 0, sizeof(o));
+    o.center = {center.x, center.y, center.z};
+    o.extent = {fabsf(extent.x), fabsf(extent.y), fabsf(extent.z)};
+    ap_gbuffer& g = o.material.texture.base;
+    o.material.shading_model = AP_MAT_DIFFUSE;
+    g.albedo = {1.0f, 1.0f, 1.0f};
+    g.emissive = {0.0f, 0.0f, 0.0f};
+    g.ior = 1.5f;
+    g.sigma_a = 100.0f;
+    g.normal = {0.0f, 0.0f, 1.0f};
+    g.abbe_number = 20.0f;
+    o.world_to_object = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    o.object_to_world = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    o.type = type;
+    o.additional.sdf_info.cached_volume_index = -1;
+    o.object_index = -1;
+    o.light_index = -1;
+    o.visible = 1;
+}
+
+// SceneObject::UpdateProxy (core/render/object.cuh:309-342).
+void update_proxy(ap_scene_object& o) {
+    const f3 c{o.center.x, o.center.y, o.center.z};
+    o.proxy.center = o.center;
+    const m3 m{{o.object_to_world.x.x, o.object_to_world.x.y, o.object_to_world.x.z},
+               {o.object_to_world.y.x, o.object_to_world.y.y, o.object_to_world.y.z},
+               {o.object_to_world.z.x, o.object_to_world.z.y, o.object_to_world.z.z}};
+    if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SDF) {
+        const f3 ax = mul(m, f3{1, 0, 0}), ay = mul(m, f3{0, 1, 0}), az = mul(m, f3{0, 0, 1});
+        const f3 e{o.extent.x, o.extent.y, o.extent.z};
+        f3 h;
+        h.x = fabsf(ax.x) * e.x + fabsf(ay.x) * e.y + fabsf(az.x) * e.z;
+        h.y = fabsf(ax.y) * e.x + fabsf(ay.y) * e.y + fabsf(az.y) * e.z;
+        h.z = fabsf(ax.z) * e.x + fabsf(ay.z) * e.y + fabsf(az.z) * e.z;
+        o.proxy.bound_min = {c.x - h.x, c.y - h.y, c.z - h.z};
+        o.proxy.bound_max = {c.x + h.x, c.y + h.y, c.z + h.z};
+    } else if (o.type == AP_OBJ_SPHERE) {
+        const float r = fmaxf(o.extent.x, fmaxf(o.extent.y, o.extent.z));
+        o.proxy.bound_min = {c.x - r, c.y - r, c.z - r};
+        o.proxy.bound_max = {c.x + r, c.y + r, c.z + r};
+    } else if (o.type == AP_OBJ_TRIANGLE) {
+        auto P = [&](const ap_vertex& v) { return mul(m, f3{v.position.x, v.position.y, v.position.z}) + c; };
+        const f3 p0 = P(o.additional.v0), p1 = P(o.additional.v1), p2 = P(o.additional.v2);
+        o.proxy.bound_min = {fminf(fminf(p0.x, p1.x), p2.x), fminf(fminf(p0.y, p1.y), p2.y),
+                             fminf(fminf(p0.z, p1.z), p2.z)};
+        o.proxy.bound_max = {fmaxf(fmaxf(p0.x, p1.x), p2.x), fmaxf(fmaxf(p0.y, p1.y), p2.y),
+                             fmaxf(fmaxf(p0.z, p1.z), p2.z)};
+    }
+}
+
+// Rotates v by the unit quaternion (x, y, z, w).
+static f3 qrot(const float q[4], f3 v) {
+    const f3 u{q[0], q[1], q[2]};
+    const float w = q[3];
+    const f3 t = 2.0f * cross(u, v);
+    return v + w * t + cross(u, t);
+}
+
+}  // namespace ap
+
+using namespace ap;
+
+extern "C" {
+
+int ap_synth_triangle_soup(size_t n, uint64_t seed, float edge_min, float edge_max, ap_scene_object* out) {
+    if (n > 0 && !out) return fail(AP_ERR_INVALID_ARG, "ap_synth_triangle_soup: NULL out");
+    if (!(edge_min > 0.0f) || !(edge_max >= edge_min))
+        return fail(AP_ERR_INVALID_ARG, "ap_synth_triangle_soup: bad edge range");
+    Pcg32 rng = pcg32_seed(seed, 0);
+    const double lmin = std::log(static_cast<double>(edge_min));
+    const double lmax = std::log(static_cast<double>(edge_max));
+    for (size_t i = 0; i < n; ++i) {
+        const f3 c{pcg32_float(rng), pcg32_float(rng), pcg32_float(rng)};
+        const float s = static_cast<float>(std::exp(lmin + (lmax - lmin) * pcg32_float(rng)));
+        // Uniform rotation (Shoemake 1992).
+        const double u1 = pcg32_float(rng), u2 = pcg32_float(rng), u3 = pcg32_float(rng);
+        const double a = std::sqrt(1.0 - u1), b = std::sqrt(u1);
+        const double tau = 6.283185307179586;
+        const float q[4] = {static_cast<float>(a * std::sin(tau * u2)), static_cast<float>(a * std::cos(tau * u2)),
+                            static_cast<float>(b * std::sin(tau * u3)), static_cast<float>(b * std::cos(tau * u3))};
+        const float R = s / std::sqrt(3.0f);
+        const f3 v0 = qrot(q, f3{0.0f, R, 0.0f});
+        const f3 v1 = qrot(q, f3{-0.5f * s, -0.5f * R, 0.0f});
+        const f3 v2 = qrot(q, f3{0.5f * s, -0.5f * R, 0.0f});
+        ap_scene_object& o = out[i];
+        init_object(o, c, f3{s, s, s}, AP_OBJ_TRIANGLE);
+        const f3 nrm = normalize(cross(v1 - v0, v2 - v0));
+        ap_vertex* vs[3] = {&o.additional.v0, &o.additional.v1, &o.additional.v2};
+        const f3 ps[3] = {v0, v1, v2};
+        for (int k = 0; k < 3; ++k) {
+            vs[k]->position = {ps[k].x, ps[k].y, ps[k].z};
+            vs[k]->normal = {nrm.x, nrm.y, nrm.z};
+            vs[k]->uv = {0.0f, 0.0f};
+        }
+        update_proxy(o);
+    }
+    return AP_OK;
+}
+
+int ap_synth_camera_rays(int width, int height, const float origin[3], const float direction[3], float fov_deg,
+                         ap_ray* out) {
+    if (width <= 0 || height <= 0 || !origin || !direction || !out)
+        return fail(AP_ERR_INVALID_ARG, "ap_synth_camera_rays: bad arguments");
+    const f3 o{origin[0], origin[1], origin[2]};
+    const f3 dir{direction[0], direction[1], direction[2]};
+    const float fov = fov_deg * 3.1415926f / 180.0f;
+    const float aspect = static_cast<float>(width) / fmaxf(1.0f, static_cast<float>(height));
+    const float tan_y = tanf(0.5f * fov);
+    const float tan_x = tan_y * aspect;
+    const f3 right = normalize(cross(dir, f3{0.0f, 1.0f, 0.0f}));
+    const f3 up = normalize(cross(dir, right));
+    const float inf = std::numeric_limits<float>::infinity();
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < height; ++j) {
+        for (int i = 0; i < width; ++i) {
+            const float u = (static_cast<float>(i) + 0.5f) / static_cast<float>(width);
+            const float v = (static_cast<float>(j) + 0.5f) / static_cast<float>(height);
+            const float nx = u * 2.0f - 1.0f;
+            const float ny = static_cast<float>((1.0 - static_cast<double>(v)) * 2.0f - 1.0f);
+            const f3 d = normalize(dir + right * (nx * tan_x) + up * (ny * tan_y));
+            ap_ray& r = out[static_cast<size_t>(j) * width + i];
+            r = ap_ray{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, inf};
+        }
+    }
+    return AP_OK;
+}
+
+int ap_synth_secondary_rays(const ap_scene* scene, const ap_ray* prim, const ap_hit* hits, size_t n,
+                            uint64_t seed, float offset, ap_ray* out, size_t* n_out) {
+    if (!scene || (n > 0 && (!prim || !hits || !out)) || !n_out)
+        return fail(AP_ERR_INVALID_ARG, "ap_synth_secondary_rays: bad arguments");
+    const float inf = std::numeric_limits<float>::infinity();
+    // Prefix positions so the output stays in primary order.
+    std::vector<size_t> pos(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) pos[i + 1] = pos[i] + (hits[i].id >= 0 ? 1 : 0);
+    const int64_t nn = static_cast<int64_t>(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < nn; ++ii) {
+        const size_t i = static_cast<size_t>(ii);
+        const int32_t id = hits[i].id;
+        if (id < 0) continue;
+        const ap_scene_object& ob = scene->objects[id];
+        const f3 o{prim[i].ox, prim[i].oy, prim[i].oz};
+        const f3 d{prim[i].dx, prim[i].dy, prim[i].dz};
+        f3 nrm{0.0f, 1.0f, 0.0f};
+        if (ob.type == AP_OBJ_TRIANGLE) {
+            const m3 m{{ob.object_to_world.x.x, ob.object_to_world.x.y, ob.object_to_world.x.z},
+                       {ob.object_to_world.y.x, ob.object_to_world.y.y, ob.object_to_world.y.z},
+                       {ob.object_to_world.z.x, ob.object_to_world.z.y, ob.object_to_world.z.z}};
+            const f3 c{ob.center.x, ob.center.y, ob.center.z};
+            auto P = [&](const ap_vertex& v) { return mul(m, f3{v.position.x, v.position.y, v.position.z}) + c; };
+            const f3 p0 = P(ob.additional.v0), p1 = P(ob.additional.v1), p2 = P(ob.additional.v2);
+            nrm = normalize(cross(p1 - p0, p2 - p0));
+        }
+        if (dot(nrm, d) > 0.0f) nrm = -nrm;
+        const f3 hp = o + d * hits[i].t;
+        const f3 org = hp + nrm * offset;
+        Pcg32 rng = pcg32_seed(seed, static_cast<uint64_t>(i));
+        const float e0 = pcg32_float(rng), e1 = pcg32_float(rng), e2 = pcg32_float(rng);
+        const DirSample s = cosine_hemisphere(f3{e0, e1, e2}, nrm);
+        out[pos[i]] = ap_ray{org.x, org.y, org.z, 0.0f, s.dir.x, s.dir.y, s.dir.z, inf};
+    }
+    *n_out = pos[n];
+    return AP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+/*
+ * Aperture MI355X core: C ABI.
+ *
+ * One shared library, libaperture_hip.so, exports these entry points. All
+ * return an int status (AP_OK = 0, negative on error), never throw and never
+ * abort; ap_last_error_string() gives the message of the calling thread's
+ * last failure. Device pointers are owned by the caller; host arrays are
+ * copied. Scene state is passed to every kernel by argument, so any number
+ * of scenes may coexist on one device (the reference keeps one scene per
+ * device in __constant__ symbols, core/render/render.cuh:24-56).
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef APERTURE_AP_CORE_H
+#define APERTURE_AP_CORE_H
+
+#include "ap_layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AP_OK 0
+#define AP_ERR_INVALID_ARG (-1)
+#define AP_ERR_HIP (-2)
+#define AP_ERR_OUT_OF_MEMORY (-3)
+#define AP_ERR_UNSUPPORTED (-4)
+#define AP_ERR_NO_DEVICE (-5)
+
+typedef struct ap_scene ap_scene;
+
+/* Traversal culling. REFERENCE reproduces the reference slab test literally
+ * (accept iff tExit >= max(0,tMin) && tEnter <= tMax, bvh.cuh:424-470, which
+ * never checks tEnter <= tExit, so every box in front of the ray passes until
+ * a hit shrinks tMax). CULLED (the default) also rejects boxes whose grown
+ * bounds the ray's line misses; its visit order is a subsequence of the
+ * reference order that keeps every box able to hold an accepted hit, so it
+ * returns the same ids and distances. */
+#define AP_TRAVERSE_REFERENCE 0
+#define AP_TRAVERSE_CULLED 1
+
+/* One ray, 32 B: origin, tmin, direction, tmax. The direction is used as
+ * given (TraverseBVH does not normalise it, intersection.cuh:235-246). */
+typedef struct ap_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} ap_ray;
+
+/* One hit, 8 B: objectIndex_ of the accepted SceneObject (-1 on a miss)
+ * and its distance_ (tmax on a miss, +inf for an empty scene), as
+ * IntersectionContext reports them (intersection.cuh:248-253, 160-169). */
+typedef struct ap_hit {
+    int32_t id;
+    float t;
+} ap_hit;
+
+/* Per-ray traversal work in the reference visit order: node records read
+ * (each popped node plus both children of every expanded interior node,
+ * intersection.cuh:273, 313-314) and primitive tests (:299-309). */
+typedef struct ap_trace_stats {
+    uint32_t nodes;
+    uint32_t prims;
+} ap_trace_stats;
+
+/* BuildBVH(proxy, leafSize = 8, binCount = 12) (core/bvh/bvh.cuh:142). */
+typedef struct ap_build_params {
+    int32_t leaf_size;
+    int32_t bin_count;
+} ap_build_params;
+
+typedef struct ap_bvh_info {
+    int64_t node_count;      /* reference BVHNode count (DFS pre-order, root 0) */
+    int64_t interior_count;  /* 64-B device records (one per interior node) */
+    int64_t prim_count;
+    int32_t max_depth;       /* root = depth 0 */
+    int32_t stack_capacity;  /* per-lane traversal stack entries used */
+    int64_t device_bytes;    /* node records + primitive records + analytic records */
+    int32_t light_count;
+    int32_t traversal_mode;  /* AP_TRAVERSE_* */
+    float line_margin;       /* world-space box growth of the CULLED line test */
+    int32_t pad;
+} ap_bvh_info;
+
+/* Replaces SceneRenderer::SceneRenderer(const std::vector<SceneObject>&)
+ * (core/scene.hpp:113) together with SceneRenderer::UpdateSceneToDeviceIfDirty
+ * (core/scene.cu:1025-1118): copies the objects, stamps objectIndex_ and
+ * lightIndex_, builds the binned-SAH BVH on the host (bvh.cuh:142-422), builds
+ * the light selection table (scene.cu:1056-1080) and uploads the device
+ * layouts to the current HIP device. Triangles stay in the traversal (the
+ * reference's CUDA path retypes them to 999 and never hits them,
+ * scene.cu:1040-1044). params may be NULL (defaults 8, 12). */
+int ap_scene_create(const ap_scene_object* objects, size_t count,
+                    const ap_build_params* params, ap_scene** out_scene);
+
+/* Replaces BuildBVH(const std::vector<PrimitiveProxy>&, leafSize, binCount)
+ * (core/bvh/bvh.cuh:142-422) over the objects' proxy_ fields, host only (no
+ * device needed). nodes must hold 2*count entries (a binary tree over count
+ * leaves has at most 2*count-1 nodes), indices count entries; *node_count
+ * receives the number of nodes written. */
+int ap_build_bvh(const ap_scene_object* objects, size_t count, const ap_build_params* params,
+                 ap_bvh_node* nodes, int32_t* indices, size_t* node_count);
+
+/* Replaces SceneRenderer::~SceneRenderer (core/scene.cu, FreeSceneRelatedDevicePtr). */
+int ap_scene_destroy(ap_scene* scene);
+
+int ap_scene_get_info(const ap_scene* scene, ap_bvh_info* out_info);
+
+/* Selects AP_TRAVERSE_REFERENCE or AP_TRAVERSE_CULLED for later traces. No
+ * reference counterpart (the reference has one compiled-in test). */
+int ap_scene_set_traversal_mode(ap_scene* scene, int mode);
+
+/* Copies out the reference-layout BVH (BVHHost::nodes_, objectIndices_,
+ * bvh.cuh:54-58) and light table (AdditionalLightInfo[], scene.cu:1056-1080)
+ * to host arrays sized from ap_scene_get_info. Any pointer may be NULL. */
+int ap_scene_export_bvh(const ap_scene* scene, ap_bvh_node* nodes, int32_t* object_indices,
+                        ap_light_info* lights);
+
+/* Replaces intersection::TraverseBVH<false> (core/render/intersection.cuh:235-370)
+ * over SceneObjectsIntersector (:140-172): closest hit for n rays, same
+ * primitive tests, same acceptance window [tmin, tmax), same near-first visit
+ * order with the left child first on ties. rays/hits are device pointers;
+ * stream is a hipStream_t (NULL = default stream). Asynchronous. */
+int ap_trace_closest(const ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hits,
+                     void* stream);
+
+/* Replaces intersection::TraverseBVH<true> as reached from AnyHitSegmentBVH
+ * (intersection.cuh:372-397): the first accepted primitive in visit order. */
+int ap_trace_any(const ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hits,
+                 void* stream);
+
+/* ap_trace_closest that also writes per-ray ap_trace_stats (device pointer).
+ * No reference counterpart: it instruments the same visit order for the
+ * algorithmic-bytes accounting of the traversal roofline. */
+int ap_trace_closest_stats(const ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hits,
+                           ap_trace_stats* stats, void* stream);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* ap_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APERTURE_AP_CORE_H */

@@ -1,0 +1,43 @@
+/*
+ * Synthetic workload generators of the Aperture core (host C++, exported
+ * from libaperture_hip.so). They have no counterpart in the reference, which
+ * loads OBJ meshes and hand-built scenes (core/helper/scene_helper.h); they
+ * build the benchmark scenes and ray sets deterministically from PCG32 seeds
+ * so that the GPU path and the CPU oracle consume identical inputs.
+ */
+#ifndef APERTURE_AP_SYNTH_H
+#define APERTURE_AP_SYNTH_H
+
+#include "ap_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Triangle soup: n SceneObjects of type OBJ_TRIANGLE. Centroids uniform in
+ * the unit cube, edge length log-uniform in [edge_min, edge_max], equilateral
+ * triangles with a uniformly random rotation (uniform unit quaternion).
+ * Each object's center_ is its centroid (the proxy centroid the reference
+ * builder splits on, core/render/object.cuh:309-341), objectToWorld_ is the
+ * identity and the vertices are relative to the centroid. Diffuse white. */
+int ap_synth_triangle_soup(size_t n, uint64_t seed, float edge_min, float edge_max,
+                           ap_scene_object* out);
+
+/* Pinhole rays through pixel centres with the reference camera basis
+ * (core/scene.cu:214-230; vertical FOV given in degrees instead of the fixed
+ * 60). tmin = 0, tmax = +inf. Row-major, index j*width + i. */
+int ap_synth_camera_rays(int width, int height, const float origin[3], const float direction[3],
+                         float fov_deg, ap_ray* out);
+
+/* One secondary ray per primary hit (in primary order): origin = hit point +
+ * offset * n with n the geometric normal turned toward the incoming ray,
+ * direction cosine-weighted about n (SampleCosineWeightedHemisphere,
+ * core/render/sample.cuh:88-108) from PCG32(seed, stream = primary index).
+ * Writes *n_out rays; out must hold n entries. Host arrays. */
+int ap_synth_secondary_rays(const ap_scene* scene, const ap_ray* primaries, const ap_hit* hits,
+                            size_t n, uint64_t seed, float offset, ap_ray* out, size_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APERTURE_AP_SYNTH_H */

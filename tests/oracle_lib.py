@@ -1,0 +1,98 @@
+"""ctypes loader of oracle/liboracle.so (the CPU restatement of the reference
+algorithms). Test infrastructure: used only as the checker."""
+import ctypes
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(REPO, "oracle", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle` or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+        L.or_bvh_build.restype = vp
+        L.or_bvh_build.argtypes = [vp, sz, i32, i32]
+        L.or_bvh_free.argtypes = [vp]
+        L.or_bvh_node_count.restype = sz
+        L.or_bvh_node_count.argtypes = [vp]
+        L.or_bvh_export.argtypes = [vp, vp, vp]
+        L.or_trace.argtypes = [vp, vp, sz, i32, i32, ctypes.c_float, vp, vp, vp, vp, i32]
+        L.or_brute_closest.argtypes = [vp, sz, vp, sz, vp, vp, i32]
+        L.or_object_t.restype = ctypes.c_float
+        L.or_object_t.argtypes = [vp, i32, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _objs(objs):
+    import aperture
+
+    return np.ascontiguousarray(objs, dtype=aperture.SCENE_OBJECT_DTYPE)
+
+
+def _rays(rays):
+    import aperture
+
+    return np.ascontiguousarray(rays, dtype=aperture.RAY_DTYPE)
+
+
+class OracleBVH:
+    """BuildBVH + TraverseBVH restated on the CPU over a borrowed SceneObject array."""
+
+    def __init__(self, objs, leaf_size=8, bin_count=12):
+        self.objs = _objs(objs)
+        self.h = lib().or_bvh_build(_p(self.objs), len(self.objs), leaf_size, bin_count)
+
+    def export(self):
+        import aperture
+
+        n = lib().or_bvh_node_count(self.h)
+        nodes = np.zeros(n, dtype=aperture.BVH_NODE_DTYPE)
+        idx = np.zeros(len(self.objs), dtype=np.int32)
+        lib().or_bvh_export(self.h, _p(nodes), _p(idx))
+        return nodes, idx
+
+    def trace(self, rays, any_hit=False, threads=None, stats=False, cull_margin=None):
+        """Reference traversal; with cull_margin set, the core's CULLED visit order."""
+        rays = _rays(rays)
+        n = len(rays)
+        ids = np.zeros(n, np.int32)
+        ts = np.zeros(n, np.float32)
+        nodes = np.zeros(n, np.uint32) if stats else None
+        prims = np.zeros(n, np.uint32) if stats else None
+        cull = cull_margin is not None
+        lib().or_trace(self.h, _p(rays), n, 1 if any_hit else 0, 1 if cull else 0, float(cull_margin or 0.0),
+                       _p(ids), _p(ts), _p(nodes), _p(prims), threads or os.cpu_count())
+        return (ids, ts, nodes, prims) if stats else (ids, ts)
+
+    def __del__(self):
+        try:
+            lib().or_bvh_free(self.h)
+        except Exception:
+            pass
+
+
+def brute_closest(objs, rays, threads=None):
+    objs = _objs(objs)
+    rays = _rays(rays)
+    ids = np.zeros(len(rays), np.int32)
+    ts = np.zeros(len(rays), np.float32)
+    lib().or_brute_closest(_p(objs), len(objs), _p(rays), len(rays), _p(ids), _p(ts), threads or os.cpu_count())
+    return ids, ts
+
+
+def object_t(objs, idx, ray):
+    r = _rays(np.asarray(ray).reshape(1))
+    return lib().or_object_t(_p(_objs(objs)), int(idx), _p(r))

@@ -1,0 +1,135 @@
+"""Small hand-built scenes for tests (numpy SceneObject arrays)."""
+import numpy as np
+
+import aperture as ap
+
+
+def concat(parts):
+    """np.concatenate that keeps the 432-byte SceneObject layout."""
+    return np.concatenate([np.asarray(p, dtype=ap.SCENE_OBJECT_DTYPE) for p in parts]).astype(ap.SCENE_OBJECT_DTYPE)
+
+
+def blank(n):
+    """n default SceneObjects (Material/GBuffer defaults, identity transforms)."""
+    o = np.zeros(n, dtype=ap.SCENE_OBJECT_DTYPE)
+    o["shading_model"] = ap.MAT_DIFFUSE
+    o["albedo"] = 1.0
+    o["ior"] = 1.5
+    o["sigma_a"] = 100.0
+    o["normal"] = (0.0, 0.0, 1.0)
+    o["abbe_number"] = 20.0
+    o["world_to_object"] = np.eye(3, dtype=np.float32)
+    o["object_to_world"] = np.eye(3, dtype=np.float32)
+    o["cached_volume_index"] = -1
+    o["object_index"] = -1
+    o["light_index"] = -1
+    o["visible"] = 1
+    return o
+
+
+def triangles(p0, p1, p2, center=None):
+    """Triangle SceneObjects from world-space vertex arrays (N,3). The centre
+    is the centroid unless given; local vertices are relative to it."""
+    p0, p1, p2 = (np.asarray(x, dtype=np.float32).reshape(-1, 3) for x in (p0, p1, p2))
+    n = len(p0)
+    c = ((p0 + p1 + p2) / np.float32(3.0)).astype(np.float32) if center is None else np.broadcast_to(
+        np.asarray(center, np.float32), (n, 3))
+    o = blank(n)
+    o["type"] = ap.OBJ_TRIANGLE
+    o["center"] = c
+    o["extent"] = 1.0
+    o["v0_position"] = p0 - c
+    o["v1_position"] = p1 - c
+    o["v2_position"] = p2 - c
+    nrm = np.cross(p1 - p0, p2 - p0)
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    for k in ("v0_normal", "v1_normal", "v2_normal"):
+        o[k] = nrm
+    update_proxy(o)
+    return o
+
+
+def update_proxy(o):
+    """SceneObject::UpdateProxy for identity-transform objects (object.cuh:309-342)."""
+    c = o["center"].astype(np.float32)
+    o["proxy_center"] = c
+    tri = o["type"] == ap.OBJ_TRIANGLE
+    if tri.any():
+        ps = [o[k][tri] + c[tri] for k in ("v0_position", "v1_position", "v2_position")]
+        o["proxy_min"][tri] = np.minimum(np.minimum(ps[0], ps[1]), ps[2])
+        o["proxy_max"][tri] = np.maximum(np.maximum(ps[0], ps[1]), ps[2])
+    cube = o["type"] == ap.OBJ_CUBE
+    o["proxy_min"][cube] = c[cube] - o["extent"][cube]
+    o["proxy_max"][cube] = c[cube] + o["extent"][cube]
+    sph = o["type"] == ap.OBJ_SPHERE
+    r = o["extent"][sph].max(axis=1, keepdims=True)
+    o["proxy_min"][sph] = c[sph] - r
+    o["proxy_max"][sph] = c[sph] + r
+    return o
+
+
+def cubes(centers, extents, emissive=None):
+    centers = np.asarray(centers, np.float32).reshape(-1, 3)
+    o = blank(len(centers))
+    o["type"] = ap.OBJ_CUBE
+    o["center"] = centers
+    o["extent"] = np.abs(np.broadcast_to(np.asarray(extents, np.float32), centers.shape))
+    if emissive is not None:
+        o["shading_model"] = ap.MAT_LIGHT
+        o["albedo"] = 0.0
+        o["emissive"] = emissive
+    return update_proxy(o)
+
+
+def spheres(centers, radii):
+    centers = np.asarray(centers, np.float32).reshape(-1, 3)
+    o = blank(len(centers))
+    o["type"] = ap.OBJ_SPHERE
+    o["center"] = centers
+    o["extent"] = np.asarray(radii, np.float32).reshape(-1, 1)
+    return update_proxy(o)
+
+
+def quad_grid(n, z=0.0, size=1.0):
+    """n x n axis-aligned quads (2 triangles each) on the plane z, spanning
+    [0,size]^2: every interior edge and vertex is shared."""
+    xs = np.linspace(0.0, size, n + 1, dtype=np.float32)
+    p0, p1, p2 = [], [], []
+    for j in range(n):
+        for i in range(n):
+            a = (xs[i], xs[j], z)
+            b = (xs[i + 1], xs[j], z)
+            c = (xs[i + 1], xs[j + 1], z)
+            d = (xs[i], xs[j + 1], z)
+            p0 += [a, a]
+            p1 += [b, c]
+            p2 += [c, d]
+    return triangles(p0, p1, p2)
+
+
+def rays(o, d, tmin=0.0, tmax=np.inf):
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    n = max(len(o), len(d))
+    r = np.zeros(n, dtype=ap.RAY_DTYPE)
+    r["o"] = np.broadcast_to(o, (n, 3))
+    r["d"] = np.broadcast_to(d, (n, 3))
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r
+
+
+def random_rays(n, seed, lo=-0.5, hi=1.5, normalize=True):
+    """Rays with origins in [lo,hi]^3 and random directions, including
+    axis-aligned and zero-component directions."""
+    g = np.random.default_rng(seed)
+    o = g.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = g.normal(size=(n, 3)).astype(np.float32)
+    k = n // 8
+    d[:k] = 0.0
+    d[np.arange(k), g.integers(0, 3, k)] = g.choice([-1.0, 1.0], k)  # axis aligned
+    d[k:2 * k, g.integers(0, 3)] = 0.0                                  # one zero component
+    d[2 * k:3 * k, 0] = -0.0
+    if normalize:
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return rays(o, d)
