@@ -10,9 +10,10 @@ ctypes and mirrors the reference's host classes over it:
   (core/render/intersection.cuh:235-397) as a batched operator over device
   buffers.
 
-Device buffers are passed as torch tensors (``.data_ptr()``) or raw integer
-addresses; torch is only plumbing for device memory and streams. There is no
-CPU fallback: loading fails loudly when the shared library is missing.
+Device memory, streams and timing events come from the library's own HIP
+runtime (include/aperture/ap_device.h, wrapped by ``DeviceBuffer``, ``Stream``
+and ``Event``), so a process never mixes two HIP runtimes. There is no CPU
+fallback: loading fails loudly when the shared library is missing.
 """
 from __future__ import annotations
 
@@ -104,6 +105,9 @@ EXPORTED_SYMBOLS = (
     "ap_scene_export_bvh",
     "ap_trace_closest", "ap_trace_any", "ap_trace_closest_stats", "ap_last_error_string",
     "ap_synth_triangle_soup", "ap_synth_camera_rays", "ap_synth_secondary_rays",
+    "ap_device_count", "ap_device_set", "ap_device_name", "ap_device_malloc", "ap_device_free", "ap_memcpy",
+    "ap_memset_async", "ap_stream_create", "ap_stream_destroy", "ap_stream_synchronize", "ap_device_synchronize",
+    "ap_event_create", "ap_event_destroy", "ap_event_record", "ap_event_elapsed_ms",
 )
 
 
@@ -130,6 +134,22 @@ def lib():
         L.ap_synth_camera_rays.argtypes = [i32, i32, vp, vp, ctypes.c_float, vp]
         L.ap_synth_secondary_rays.argtypes = [vp, vp, vp, sz, ctypes.c_uint64, ctypes.c_float, vp,
                                               ctypes.POINTER(ctypes.c_size_t)]
+        pvp = ctypes.POINTER(vp)
+        L.ap_device_count.argtypes = [ctypes.POINTER(i32)]
+        L.ap_device_set.argtypes = [i32]
+        L.ap_device_name.argtypes = [i32, ctypes.c_char_p, sz]
+        L.ap_device_malloc.argtypes = [sz, pvp]
+        L.ap_device_free.argtypes = [vp]
+        L.ap_memcpy.argtypes = [vp, vp, sz, i32]
+        L.ap_memset_async.argtypes = [vp, i32, sz, vp]
+        L.ap_stream_create.argtypes = [pvp]
+        L.ap_stream_destroy.argtypes = [vp]
+        L.ap_stream_synchronize.argtypes = [vp]
+        L.ap_device_synchronize.argtypes = []
+        L.ap_event_create.argtypes = [pvp]
+        L.ap_event_destroy.argtypes = [vp]
+        L.ap_event_record.argtypes = [vp, vp]
+        L.ap_event_elapsed_ms.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_float)]
         for name in EXPORTED_SYMBOLS:
             if name != "ap_last_error_string":
                 getattr(L, name).restype = i32
@@ -144,13 +164,15 @@ def check(code):
 
 
 def _ptr(x):
-    """Address of a numpy array, torch tensor or integer."""
+    """Address of a numpy array, DeviceBuffer or integer."""
     if x is None:
         return None
     if isinstance(x, int):
         return x
     if isinstance(x, np.ndarray):
         return x.ctypes.data
+    if isinstance(x, DeviceBuffer):
+        return x.ptr
     return x.data_ptr()
 
 
@@ -159,7 +181,108 @@ def _stream(stream):
         return None
     if isinstance(stream, int):
         return stream
+    if isinstance(stream, Stream):
+        return stream.handle
     return stream.cuda_stream
+
+
+# ---------------------------------------------------------------------------
+# Device plumbing (include/aperture/ap_device.h)
+# ---------------------------------------------------------------------------
+COPY_H2D, COPY_D2H, COPY_D2D = 1, 2, 3
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().ap_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_name(device=0):
+    buf = ctypes.create_string_buffer(128)
+    check(lib().ap_device_name(device, buf, 128))
+    return buf.value.decode()
+
+
+def synchronize():
+    check(lib().ap_device_synchronize())
+
+
+class DeviceBuffer:
+    """A device allocation of `nbytes` bytes (hipMalloc through the C ABI)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().ap_device_malloc(max(self.nbytes, 1), ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr):
+        a = np.ascontiguousarray(arr)
+        buf = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().ap_memcpy(buf.ptr, a.ctypes.data, a.nbytes, COPY_H2D))
+        return buf
+
+    def to_numpy(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else int(count)
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().ap_memcpy(out.ctypes.data, self.ptr, out.nbytes, COPY_D2H))
+        return out
+
+    def zero(self, stream=None):
+        check(lib().ap_memset_async(self.ptr, 0, self.nbytes, _stream(stream)))
+
+    def free(self):
+        if self.ptr:
+            lib().ap_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().ap_stream_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self):
+        check(lib().ap_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            lib().ap_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().ap_event_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        check(lib().ap_event_record(self.handle, _stream(stream)))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float(0.0)
+        check(lib().ap_event_elapsed_ms(self.handle, end.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().ap_event_destroy(self.handle)
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------
@@ -259,6 +382,4 @@ class SceneRenderer:
 
 
 def _numel_rays(rays):
-    if hasattr(rays, "nbytes"):
-        return int(rays.nbytes) // 32
-    return int(rays.numel() * rays.element_size()) // 32
+    return int(rays.nbytes) // 32

@@ -89,12 +89,13 @@ def main():
               "(see DESIGN.md for how it would shard); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import aperture as ap
 
-    torch.cuda.set_device(0)
-    stream = torch.cuda.current_stream()
+    if ap.device_count() < 1:
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    ap.check(ap.lib().ap_device_set(0))
+    stream = ap.Stream()
 
     t0 = time.perf_counter()
     objs = ap.synth_triangle_soup(args.triangles, seed=2, edge_min=1e-3, edge_max=1e-2)
@@ -103,32 +104,28 @@ def main():
     log(f"scene: {args.triangles} triangles, {info.node_count} nodes, depth {info.max_depth}, "
         f"stack {info.stack_capacity}, {info.device_bytes / 2**20:.1f} MiB on device ({time.perf_counter() - t0:.1f} s)")
 
-    def dev(a):
-        a = np.ascontiguousarray(a)
-        t = torch.empty(a.nbytes, dtype=torch.uint8, device="cuda")
-        t.copy_(torch.from_numpy(a.view(np.uint8).reshape(-1)))
-        return t
+    dev = ap.DeviceBuffer.from_numpy
 
     prim = ap.synth_camera_rays(args.res, args.res, [0.5, 0.5, -1.5], [0, 0, 1], 40.0)
     n_prim = len(prim)
     d_prim = dev(prim)
-    d_phit = torch.empty(n_prim * 8, dtype=torch.uint8, device="cuda")
+    d_phit = ap.DeviceBuffer(n_prim * 8)
     scene.trace_closest(d_prim, d_phit, n=n_prim, stream=stream)
-    torch.cuda.synchronize()
-    phit = d_phit.cpu().numpy().view(ap.HIT_DTYPE)
+    stream.synchronize()
+    phit = d_phit.to_numpy(ap.HIT_DTYPE, n_prim)
     sec = scene.secondary_rays(prim, phit, seed=3, offset=1e-4)
     n_sec = len(sec)
     d_sec = dev(sec)
-    d_shit = torch.empty(max(n_sec, 1) * 8, dtype=torch.uint8, device="cuda")
+    d_shit = ap.DeviceBuffer(max(n_sec, 1) * 8)
     log(f"rays: {n_prim} primaries ({(phit['id'] >= 0).mean():.3f} hit), {n_sec} secondaries")
 
     # Algorithmic bytes of one step, from the kernel's own visit counts.
-    d_stats = torch.empty(max(n_prim, n_sec) * 8, dtype=torch.uint8, device="cuda")
+    d_stats = ap.DeviceBuffer(max(n_prim, n_sec) * 8)
     algo = {}
     for name, d_r, d_h, n in (("primary", d_prim, d_phit, n_prim), ("secondary", d_sec, d_shit, n_sec)):
         scene.trace_closest(d_r, d_h, n=n, stream=stream, stats=d_stats)
-        torch.cuda.synchronize()
-        st = d_stats[: n * 8].cpu().numpy().view(ap.STATS_DTYPE)
+        stream.synchronize()
+        st = d_stats.to_numpy(ap.STATS_DTYPE, n)
         nodes = int(st["nodes"].astype(np.int64).sum())
         prims = int(st["prims"].astype(np.int64).sum())
         algo[name] = {"bytes": 40 * n + 32 * nodes + 36 * prims, "nodes_per_ray": nodes / n,
@@ -147,20 +144,20 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    stream.synchronize()
 
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    evs = [[ap.Event() for _ in range(3)] for _ in range(args.steps)]
+    start, end = ap.Event(), ap.Event()
+    ap.synchronize()
     start.record(stream)
     for k in range(args.steps):
         step(evs[k])
     end.record(stream)
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(end)
-    t_prim = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
-    t_sec = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    stream.synchronize()
+    ap.synchronize()
+    total_ms = start.elapsed_ms(end)
+    t_prim = sum(e[0].elapsed_ms(e[1]) for e in evs) / args.steps
+    t_sec = sum(e[1].elapsed_ms(e[2]) for e in evs) / args.steps
     ms_per_step = total_ms / args.steps
     rays_per_step = n_prim + n_sec
     value = rays_per_step / (ms_per_step / 1e3) / 1e6

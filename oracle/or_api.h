@@ -26,6 +26,20 @@ void or_brute_closest(const ap_scene_object* objs, size_t count, const ap_ray* r
                       int32_t* ids, float* ts, int threads);
 float or_object_t(const ap_scene_object* objs, int idx, const ap_ray* r);
 
+/* Spectral restatement (or_spectrum.cpp). Tables come from the core's data files. */
+int or_spectrum_load(const char* mlp_path, const char* cie_path);
+void or_rgb_encode(const float* rgb, size_t n, float* out);
+void or_lambda_encode(const float* lam, size_t n, float* out);
+float or_spectral_interact(const float* rgbK, const float* lamK);
+void or_query_rgb_basis(const float* rgb, size_t n, float* out);
+void or_query_lambda(const float* lam, size_t n, float* out);
+float or_ndtrif(float u);
+void or_sample_cie1931(const float* u3, size_t n, float* lambda, float* pdf);
+float or_pdf_cie1931(float lambda);
+float or_d65_norm(float lambda);
+void or_lambda_to_xyz(float lambda, float* xyz);
+void or_xyz_to_srgb(const float* xyz, float* rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,9 +31,11 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
-def torch_gpu():
-    import torch
+def gpu():
+    import aperture
 
-    if not torch.cuda.is_available():
+    if aperture.device_count() < 1:
         pytest.fail("a HIP device is required for -m gpu tests")
-    return torch
+    aperture.check(aperture.lib().ap_device_set(0))
+    print(f"[gpu] device 0: {aperture.device_name(0)}", flush=True)
+    return aperture

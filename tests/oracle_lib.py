@@ -28,6 +28,27 @@ def lib():
         L.or_brute_closest.argtypes = [vp, sz, vp, sz, vp, vp, i32]
         L.or_object_t.restype = ctypes.c_float
         L.or_object_t.argtypes = [vp, i32, vp]
+        f32 = ctypes.c_float
+        L.or_spectrum_load.restype = i32
+        L.or_spectrum_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        for name in ("or_rgb_encode", "or_lambda_encode", "or_query_rgb_basis", "or_query_lambda"):
+            getattr(L, name).argtypes = [vp, sz, vp]
+        L.or_spectral_interact.restype = f32
+        L.or_spectral_interact.argtypes = [vp, vp]
+        L.or_ndtrif.restype = f32
+        L.or_ndtrif.argtypes = [f32]
+        L.or_sample_cie1931.argtypes = [vp, sz, vp, vp]
+        L.or_pdf_cie1931.restype = f32
+        L.or_pdf_cie1931.argtypes = [f32]
+        L.or_d65_norm.restype = f32
+        L.or_d65_norm.argtypes = [f32]
+        L.or_lambda_to_xyz.argtypes = [f32, vp]
+        L.or_xyz_to_srgb.argtypes = [vp, vp]
+        data = os.path.join(REPO, "aperturerendererv3-optix-geometry_amd", "data")
+        rc = L.or_spectrum_load(os.path.join(data, "spectrum_mlp.bin").encode(),
+                                os.path.join(data, "cie_tables.bin").encode())
+        if rc != 0:
+            raise ImportError(f"oracle could not load spectral tables ({rc})")
         _lib = L
     return _lib
 
@@ -96,3 +117,73 @@ def brute_closest(objs, rays, threads=None):
 def object_t(objs, idx, ray):
     r = _rays(np.asarray(ray).reshape(1))
     return lib().or_object_t(_p(_objs(objs)), int(idx), _p(r))
+
+
+# ---- spectral restatement ----
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def rgb_encode(rgb):
+    rgb = _f32(rgb, (-1, 3))
+    out = np.zeros((len(rgb), 4), np.float32)
+    lib().or_rgb_encode(_p(rgb), len(rgb), _p(out))
+    return out
+
+
+def lambda_encode(lam):
+    lam = _f32(lam, (-1,))
+    out = np.zeros((len(lam), 4), np.float32)
+    lib().or_lambda_encode(_p(lam), len(lam), _p(out))
+    return out
+
+
+def spectral_interact(rk, lk):
+    return float(lib().or_spectral_interact(_p(_f32(rk)), _p(_f32(lk))))
+
+
+def query_rgb_basis(rgb):
+    rgb = _f32(rgb, (-1, 3))
+    out = np.zeros((len(rgb), 4), np.float32)
+    lib().or_query_rgb_basis(_p(rgb), len(rgb), _p(out))
+    return out
+
+
+def query_lambda(lam):
+    lam = _f32(lam, (-1,))
+    out = np.zeros((len(lam), 4), np.float32)
+    lib().or_query_lambda(_p(lam), len(lam), _p(out))
+    return out
+
+
+def ndtrif(u):
+    return float(lib().or_ndtrif(float(u)))
+
+
+def sample_cie1931(u3):
+    u3 = _f32(u3, (-1, 3))
+    lam = np.zeros(len(u3), np.float32)
+    pdf = np.zeros(len(u3), np.float32)
+    lib().or_sample_cie1931(_p(u3), len(u3), _p(lam), _p(pdf))
+    return lam, pdf
+
+
+def pdf_cie1931(lam):
+    return float(lib().or_pdf_cie1931(float(lam)))
+
+
+def d65_norm(lam):
+    return float(lib().or_d65_norm(float(lam)))
+
+
+def lambda_to_xyz(lam):
+    out = np.zeros(3, np.float32)
+    lib().or_lambda_to_xyz(float(lam), _p(out))
+    return out
+
+
+def xyz_to_srgb(xyz):
+    out = np.zeros(3, np.float32)
+    lib().or_xyz_to_srgb(_p(_f32(xyz)), _p(out))
+    return out

@@ -42,14 +42,14 @@ def check_parity(objs, rays, label):
     scene.close()
 
 
-def test_soup_camera_and_random_rays(torch_gpu):
+def test_soup_camera_and_random_rays(gpu):
     objs = ap.synth_triangle_soup(20000, seed=2)
     rays = np.concatenate([ap.synth_camera_rays(96, 96, [0.5, 0.5, -1.5], [0, 0, 1], 40.0),
                            scenes.random_rays(8192, seed=5)])
     check_parity(objs, rays, "soup20k")
 
 
-def test_secondary_rays(torch_gpu):
+def test_secondary_rays(gpu):
     import gpu_util
 
     objs = ap.synth_triangle_soup(20000, seed=2, edge_min=2e-3, edge_max=2e-2)
@@ -61,7 +61,7 @@ def test_secondary_rays(torch_gpu):
     check_parity(objs, sec, "secondaries")
 
 
-def test_shared_edges_and_vertices(torch_gpu):
+def test_shared_edges_and_vertices(gpu):
     grid = scenes.quad_grid(16)
     xs = np.linspace(0.0, 1.0, 17, dtype=np.float32)
     gx, gy = np.meshgrid(xs, xs)
@@ -71,7 +71,7 @@ def test_shared_edges_and_vertices(torch_gpu):
     check_parity(grid, r, "quad_grid")
 
 
-def test_cubes_spheres_and_invisible(torch_gpu):
+def test_cubes_spheres_and_invisible(gpu):
     grid = scenes.quad_grid(6)
     grid["visible"][::3] = 0
     objs = scenes.concat([grid, scenes.cubes([(0.5, 0.5, 1.0), (0.2, 0.2, 0.3)], (0.25, 0.01, 0.25), emissive=15.0),
@@ -79,7 +79,7 @@ def test_cubes_spheres_and_invisible(torch_gpu):
     check_parity(objs, scenes.random_rays(8192, seed=13), "mixed")
 
 
-def test_windows_and_degenerate_rays(torch_gpu):
+def test_windows_and_degenerate_rays(gpu):
     objs = ap.synth_triangle_soup(3000, seed=4, edge_min=0.02, edge_max=0.1)
     r = scenes.random_rays(4096, seed=21)
     g = np.random.default_rng(1)
@@ -90,7 +90,7 @@ def test_windows_and_degenerate_rays(torch_gpu):
     check_parity(objs, r, "windows")
 
 
-def test_empty_and_single(torch_gpu):
+def test_empty_and_single(gpu):
     import gpu_util
 
     empty = ap.SceneRenderer(np.zeros(0, dtype=ap.SCENE_OBJECT_DTYPE))
@@ -103,11 +103,10 @@ def test_empty_and_single(torch_gpu):
     assert len(h) == 0
 
 
-def test_benchmark_size_sampled(torch_gpu):
+def test_benchmark_size_sampled(gpu):
     """1M-triangle soup with 4096^2 primaries and their secondaries: sampled
     bit-exact parity plus whole-set properties."""
     import gpu_util
-    import torch
 
     objs = ap.synth_triangle_soup(1_000_000, seed=2)
     scene = ap.SceneRenderer(objs)
@@ -128,5 +127,4 @@ def test_benchmark_size_sampled(torch_gpu):
         anyh = gpu_util.trace(scene, rays, any_hit=True)
         assert np.array_equal(anyh["id"] >= 0, h["id"] >= 0), name
         assert (anyh["t"][h["id"] >= 0] >= h["t"][h["id"] >= 0]).all(), name
-    del torch
     scene.close()
