@@ -48,7 +48,7 @@ SCENE_OBJECT_DTYPE = np.dtype(
             _F3, _F3, _F3, "<i4", "<i4", "u1",
         ],
         "offsets": [
-            0, 12, 32, 48, 60, 72, 76, 80, 84, 88, 92, 104, 108, 112, 160, 196, 232, 240, 256,
+            0, 12, 32, 64, 76, 88, 92, 96, 100, 104, 108, 120, 124, 128, 160, 196, 232, 240, 256,
             272, 288, 300, 312, 320, 332, 344, 352, 364, 376, 384, 396, 408, 420, 424, 428,
         ],
         "itemsize": 432,
@@ -91,6 +91,40 @@ class BVHInfo(ctypes.Structure):
     ]
 
 
+class Camera(ctypes.Structure):
+    """ap_camera (include/aperture/ap_render.h)."""
+    _fields_ = [("origin", ctypes.c_float * 3), ("direction", ctypes.c_float * 3), ("fov_deg", ctypes.c_float),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+    @classmethod
+    def make(cls, origin, direction, fov_deg, width, height):
+        return cls((ctypes.c_float * 3)(*origin), (ctypes.c_float * 3)(*direction), fov_deg, width, height)
+
+
+class RenderParams(ctypes.Structure):
+    """ap_render_params; defaults are the reference's constants (config.cuh)."""
+    _fields_ = [("spp", ctypes.c_int32), ("frame_index", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("rr_min_depth", ctypes.c_int32), ("nee_max_depth", ctypes.c_int32), ("use_nee", ctypes.c_int32),
+                ("nee_always", ctypes.c_int32), ("sky", ctypes.c_int32), ("triangle_material", ctypes.c_int32),
+                ("use_guiding", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+    @classmethod
+    def defaults(cls, **kw):
+        p = cls()
+        lib().ap_render_params_default(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class Framebuffers(ctypes.Structure):
+    _fields_ = [("hdr", ctypes.c_void_p), ("odd", ctypes.c_void_p), ("even", ctypes.c_void_p),
+                ("sum_xyz", ctypes.c_void_p), ("primary_id", ctypes.c_void_p)]
+
+
+TRIANGLE_MATERIAL_OBJECT, TRIANGLE_MATERIAL_REFERENCE = 0, 1
+
+
 class ApertureError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(f"aperture error {code}: {message}")
@@ -108,6 +142,9 @@ EXPORTED_SYMBOLS = (
     "ap_device_count", "ap_device_set", "ap_device_name", "ap_device_malloc", "ap_device_free", "ap_memcpy",
     "ap_memset_async", "ap_stream_create", "ap_stream_destroy", "ap_stream_synchronize", "ap_device_synchronize",
     "ap_event_create", "ap_event_destroy", "ap_event_record", "ap_event_elapsed_ms",
+    "ap_spectrum_setup", "ap_spectrum_encode_rgb", "ap_spectrum_encode_lambda", "ap_spectrum_query_rgb",
+    "ap_spectrum_query_lambda", "ap_spectrum_interact", "ap_spectrum_sample_lambda",
+    "ap_render_params_default", "ap_render", "ap_render_offline", "ap_xyz_to_srgb", "ap_synth_box_scene",
 )
 
 
@@ -150,7 +187,22 @@ def lib():
         L.ap_event_destroy.argtypes = [vp]
         L.ap_event_record.argtypes = [vp, vp]
         L.ap_event_elapsed_ms.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_float)]
+        L.ap_spectrum_setup.argtypes = []
+        for name in ("ap_spectrum_encode_rgb", "ap_spectrum_encode_lambda", "ap_spectrum_query_rgb",
+                     "ap_spectrum_query_lambda"):
+            getattr(L, name).argtypes = [vp, sz, vp, vp]
+        L.ap_spectrum_interact.argtypes = [vp, vp, sz, vp, vp]
+        L.ap_spectrum_sample_lambda.argtypes = [vp, sz, vp, vp, vp]
+        L.ap_render_params_default.argtypes = [ctypes.POINTER(RenderParams)]
+        L.ap_render.argtypes = [vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams),
+                                ctypes.POINTER(Framebuffers), vp]
+        L.ap_render_offline.argtypes = [vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), vp]
+        L.ap_xyz_to_srgb.argtypes = [vp, sz, vp, vp]
+        L.ap_synth_box_scene.argtypes = [ctypes.c_uint64, i32, ctypes.c_float, vp, sz, ctypes.POINTER(sz)]
         for name in EXPORTED_SYMBOLS:
+            if name in ("ap_render_params_default",):
+                getattr(L, name).restype = None
+                continue
             if name != "ap_last_error_string":
                 getattr(L, name).restype = i32
         _lib = L
@@ -302,6 +354,73 @@ def synth_camera_rays(width, height, origin, direction, fov_deg):
     return rays
 
 
+def synth_box_scene(seed=1, sphere_subdiv=3, light=15.0):
+    """The closed-box workload (include/aperture/ap_synth.h)."""
+    n = ctypes.c_size_t(0)
+    check(lib().ap_synth_box_scene(seed, sphere_subdiv, light, None, 0, ctypes.byref(n)))
+    objs = np.zeros(n.value, dtype=SCENE_OBJECT_DTYPE)
+    check(lib().ap_synth_box_scene(seed, sphere_subdiv, light, _ptr(objs), n.value, ctypes.byref(n)))
+    return objs
+
+
+def spectrum_setup():
+    check(lib().ap_spectrum_setup())
+
+
+def _spec_call(name, host_in, width_in, width_out, extra_out=False):
+    a = np.ascontiguousarray(host_in, dtype=np.float32).reshape(-1)
+    n = a.size // width_in
+    d_in = DeviceBuffer.from_numpy(a)
+    d_out = DeviceBuffer(max(n * width_out * 4, 4))
+    if extra_out:
+        d_out2 = DeviceBuffer(max(n * 4, 4))
+        check(getattr(lib(), name)(d_in.ptr, n, d_out.ptr, d_out2.ptr, None))
+        synchronize()
+        return d_out.to_numpy(np.float32, n), d_out2.to_numpy(np.float32, n)
+    check(getattr(lib(), name)(d_in.ptr, n, d_out.ptr, None))
+    synchronize()
+    return d_out.to_numpy(np.float32, n * width_out).reshape(n, width_out)
+
+
+def spectrum_encode_rgb(rgb):
+    return _spec_call("ap_spectrum_encode_rgb", rgb, 3, 4)
+
+
+def spectrum_encode_lambda(lam):
+    return _spec_call("ap_spectrum_encode_lambda", lam, 1, 4)
+
+
+def spectrum_query_rgb(rgb):
+    return _spec_call("ap_spectrum_query_rgb", rgb, 3, 4)
+
+
+def spectrum_query_lambda(lam):
+    return _spec_call("ap_spectrum_query_lambda", lam, 1, 4)
+
+
+def spectrum_sample_lambda(u3):
+    return _spec_call("ap_spectrum_sample_lambda", u3, 3, 1, extra_out=True)
+
+
+def spectrum_interact(lam, rgb):
+    lam = np.ascontiguousarray(lam, dtype=np.float32).reshape(-1)
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32).reshape(-1, 3)
+    d_l, d_c, d_o = DeviceBuffer.from_numpy(lam), DeviceBuffer.from_numpy(rgb), DeviceBuffer(max(lam.size * 4, 4))
+    check(lib().ap_spectrum_interact(d_l.ptr, d_c.ptr, lam.size, d_o.ptr, None))
+    synchronize()
+    return d_o.to_numpy(np.float32, lam.size)
+
+
+def xyz_to_srgb(xyz):
+    """XYZ2SRGBLinearD65 on the device for an (..., 3) host array."""
+    a = np.ascontiguousarray(xyz, dtype=np.float32)
+    n = a.size // 3
+    d_in, d_out = DeviceBuffer.from_numpy(a), DeviceBuffer(max(a.nbytes, 4))
+    check(lib().ap_xyz_to_srgb(d_in.ptr, n, d_out.ptr, None))
+    synchronize()
+    return d_out.to_numpy(np.float32, a.size).reshape(a.shape)
+
+
 def build_bvh(scene_objects, leaf_size=8, bin_count=12):
     """Host BVH build (BuildBVH, core/bvh/bvh.cuh:142-422): (nodes, object_indices)."""
     objs = np.ascontiguousarray(scene_objects, dtype=SCENE_OBJECT_DTYPE)
@@ -368,6 +487,18 @@ class SceneRenderer:
         check(lib().ap_synth_secondary_rays(self.handle, _ptr(primaries), _ptr(hits), len(primaries), seed,
                                             offset, _ptr(out), ctypes.byref(n_out)))
         return out[: n_out.value]
+
+    def render(self, camera, params, hdr, odd=None, even=None, sum_xyz=None, primary_id=None, stream=None):
+        """SceneRenderer::Render into caller-owned device framebuffers (DeviceBuffer or addresses)."""
+        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id))
+        check(lib().ap_render(self.handle, ctypes.byref(camera), ctypes.byref(params), ctypes.byref(fb),
+                              _stream(stream)))
+
+    def render_offline(self, camera, params):
+        """SceneRenderer::Render(size, origin, dir, spp): host (H, W, 3) linear XYZ."""
+        out = np.zeros(camera.width * camera.height * 3, np.float32)
+        check(lib().ap_render_offline(self.handle, ctypes.byref(camera), ctypes.byref(params), _ptr(out)))
+        return out.reshape(camera.height, camera.width, 3)
 
     def close(self):
         if self._handle is not None and self._handle.value:

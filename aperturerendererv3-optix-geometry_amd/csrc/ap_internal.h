@@ -89,6 +89,11 @@ int bvh_max_depth(const HostBVH& bvh);
 
 }  // namespace ap
 
+namespace ap {
+struct RenderState;  // integrator buffers of a scene (ap_render.hip)
+void render_state_free(RenderState* r);
+}  // namespace ap
+
 struct ap_scene {
     int device = 0;
     std::vector<ap_scene_object> objects;     // host copy, indices stamped
@@ -101,6 +106,7 @@ struct ap_scene {
     void* d_prims = nullptr;
     void* d_analytic = nullptr;
     int64_t device_bytes = 0;
+    ap::RenderState* render = nullptr;
 };
 
 namespace ap {
@@ -108,7 +114,14 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
+// Spectral context of the current device, built on first use (ap_spectrum.hip).
+struct f4;
+int spectrum_get(const f4** rgb_lut, const f4** lambda_lut, const float** tables);
+
 // Kernel launchers (ap_trace.hip).
+// dyn_n (optional, device): the actual ray count, read by the kernel; n is
+// then the capacity the grid is sized for.
 hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
-                        ap_trace_stats* stats, bool any_hit, hipStream_t stream);
+                        ap_trace_stats* stats, bool any_hit, hipStream_t stream,
+                        const uint32_t* dyn_n = nullptr);
 }  // namespace ap

@@ -33,6 +33,24 @@ AP_HD Pcg32 pcg32_seed(uint64_t initstate, uint64_t initseq) {
     return r;
 }
 
+// SplitMix64 finaliser (Steele, Lea, Flood 2014): a bijective 64-bit mix.
+AP_HD uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// The stream of one path sample: stream `pixel`, start state hashed from
+// (seed, frame, pixel). Seeding consecutive frames with the raw frame number
+// (pcg32_seed(pixel, frame)) leaves their start states on an arithmetic
+// lattice, and the first three draws of neighbouring frames are then visibly
+// correlated; the hash removes that.
+AP_HD Pcg32 pcg32_path(uint64_t pixel, uint64_t frame, uint64_t seed) {
+    const uint64_t k = splitmix64(seed ^ splitmix64(frame));
+    return pcg32_seed(splitmix64(k ^ pixel), pixel);
+}
+
 // Uniform float in [0, 1) from the top 24 bits.
 AP_HD float pcg32_float(Pcg32& r) { return static_cast<float>(pcg32_next(r) >> 8) * 0x1p-24f; }
 

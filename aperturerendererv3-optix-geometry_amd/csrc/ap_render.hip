@@ -1,0 +1,953 @@
+// Wavefront path tracer (include/aperture/ap_render.h).
+//
+// The reference runs one thread per pixel through CalculateRadiance
+// (core/render/render.cuh:460-999), a loop of Trace -> guiding -> emission
+// -> NEE -> Russian roulette. Here each loop iteration becomes kernels over
+// compacted queues of live paths:
+//
+//   generate  camera ray, wavelength sample, path state      (scene.cu:206-230, render.cuh:466-540)
+//   extend    closest-hit traversal of the path queue          (render.cuh:75-108)
+//   shade     hit attributes, BSDF sample/eval, emission + MIS,
+//             NEE light sample -> shadow queue, RR, next ray    (render.cuh:109-163, 541-908)
+//   shadow    any-hit traversal of the shadow queue            (render.cuh:165-183, 254)
+//   resolve   NEE contribution of unoccluded samples           (render.cuh:255-343, 818)
+//   accumulate clamp, running means, FP64 sums                 (render.cuh:966-997, scene.cu:232-254)
+//
+// Every path owns a PCG32 stream seeded by (pixel, frame) and draws its
+// random numbers in the reference's order, so a path takes the same
+// decisions as the CPU oracle's per-pixel loop wherever their FP32 results
+// agree. Queues are appended to with one atomic per wavefront (ballot +
+// popcount). Several frames are traced as one batch and accumulated in frame
+// order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "aperture/ap_render.h"
+#include "ap_internal.h"
+#include "ap_pcg.h"
+#include "ap_shade.h"
+#include "ap_spectrum.h"
+
+namespace ap {
+
+// Per-object geometry and material index for shading.
+struct alignas(16) GeoRec {
+    float p0[3]; int32_t type;       // AP_OBJ_*
+    float e1[3]; int32_t mat;        // MatRec index
+    float e2[3]; int32_t analytic;   // AnalyticRec index (cube / sphere), else -1
+    float n0[3]; int32_t light_index;
+    float n1[3]; int32_t pad0;
+    float n2[3]; int32_t pad1;
+};
+static_assert(sizeof(GeoRec) == 96, "GeoRec is 96 B");
+
+// Material inputs before spectral conversion.
+struct MatIn {
+    f3 albedo, emissive;
+    float specular, metallic, roughness;
+    int32_t model, light_index;
+};
+
+struct alignas(16) PathState {
+    uint64_t rng_state, rng_inc;
+    f4 lambda_basis;
+    float xyz[3]; float lambda_pdf;
+    float radiance, mask, d65, last_pdf;
+    float last_pos[3]; int32_t ndd;    // nonDeltaDepth
+    uint32_t flags; int32_t pixel; int32_t frame; float lambda;
+};
+static_assert(sizeof(PathState) == 96, "PathState is 96 B");
+
+enum : uint32_t { PATH_PREV_NEE = 1u, PATH_INSIDE = 2u };
+
+struct alignas(16) ShadowPayload {
+    int32_t path;
+    float add;        // mask * NEE radiance * D65 norm (0 when the cosines fail)
+    uint32_t cos_ok;
+    uint32_t pad;
+};
+
+struct RenderState {
+    int device = -1;
+    int tri_mode = -1;
+    // scene shading data
+    GeoRec* geo = nullptr;
+    MatRec* mats = nullptr;
+    LightRec* lights = nullptr;
+    int32_t n_lights = 0;
+    int32_t n_mats = 0;
+    // path buffers
+    size_t capacity = 0;
+    PathState* paths = nullptr;
+    ap_ray* rays[2] = {nullptr, nullptr};
+    int32_t* path_of[2] = {nullptr, nullptr};
+    ap_hit* hits = nullptr;
+    ap_ray* shadow_rays = nullptr;
+    ShadowPayload* shadow_payload = nullptr;
+    ap_hit* shadow_hits = nullptr;
+    uint32_t* counters = nullptr;     // [0],[1] path queues, [2] shadow queue
+    float* eps = nullptr;             // INTERSECT_EPS(d) for d = 0 .. kEpsEntries-1
+    float* target_tmp = nullptr;      // offline render target
+    size_t target_cap = 0;
+};
+
+constexpr int kEpsEntries = 1024;
+
+void render_state_free(RenderState* r) {
+    if (!r) return;
+    void* ptrs[] = {r->geo, r->mats, r->lights, r->paths, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
+                    r->hits, r->shadow_rays, r->shadow_payload, r->shadow_hits, r->counters, r->eps,
+                    r->target_tmp};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete r;
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float rand1(Pcg32& r) { return pcg32_float(r); }
+
+// Appends the calling lanes with pred set to a queue: one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (mask == 0) return 0;
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    const uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64u - lane)));
+    return base + static_cast<uint32_t>(__popcll(below));
+}
+
+struct CamParams {
+    f3 origin, dir, right, up;
+    float tan_x, tan_y;
+    int32_t width, height;
+};
+
+struct FrameParams {
+    int32_t frame0;
+    int32_t frames;
+    int32_t n_pixels;
+    int32_t inside;
+    uint64_t seed;
+};
+
+__global__ void set_counters_kernel(uint32_t* c, uint32_t q0, uint32_t q1, uint32_t s) {
+    if (threadIdx.x == 0) { c[0] = q0; c[1] = q1; c[2] = s; }
+}
+
+__global__ void begin_bounce_kernel(uint32_t* c, int out) {
+    if (threadIdx.x == 0) { c[out] = 0; c[2] = 0; }
+}
+
+__global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FrameParams fp, const f4* __restrict__ lam_lut,
+                                                          const float* __restrict__ tables,
+                                                          const float* __restrict__ eps, PathState* __restrict__ paths,
+                                                          ap_ray* __restrict__ rays, int32_t* __restrict__ path_of) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t total = static_cast<uint32_t>(fp.n_pixels) * static_cast<uint32_t>(fp.frames);
+    if (i >= total) return;
+    const int32_t f = static_cast<int32_t>(i / fp.n_pixels);
+    const int32_t p = static_cast<int32_t>(i % fp.n_pixels);
+    const int32_t px = p % cam.width, py = p / cam.width;
+    const int32_t frame = fp.frame0 + f;
+    Pcg32 rng = pcg32_path(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), fp.seed);
+    // RenderCamera (scene.cu:214-230).
+    const float u = (static_cast<float>(px) + rand1(rng)) / static_cast<float>(cam.width);
+    const float v = (static_cast<float>(py) + rand1(rng)) / static_cast<float>(cam.height);
+    const float nx = u * 2.0f - 1.0f;
+    const float ny = static_cast<float>((1.0 - static_cast<double>(v)) * 2.0f - 1.0f);
+    f3 d = normalize(cam.dir + cam.right * (nx * cam.tan_x) + cam.up * (ny * cam.tan_y));
+    // CalculateRadiance prologue (render.cuh:467-519).
+    float lam, lpdf;
+    {
+        const float r0 = rand1(rng), r1 = rand1(rng), r2 = rand1(rng);
+        sample_lambda(r0, r1, r2, lam, lpdf);
+    }
+    d = normalize(d);
+    PathState s;
+    s.lambda_basis = query_lambda(lam_lut, lam);
+    const f3 xyz = lambda_to_xyz(tables, lam);
+    s.xyz[0] = xyz.x; s.xyz[1] = xyz.y; s.xyz[2] = xyz.z;
+    s.lambda_pdf = lpdf;
+    s.d65 = d65_norm(tables, lam);
+    s.radiance = 0.0f;
+    s.mask = 1.0f;
+    s.last_pdf = 0.0f;
+    s.last_pos[0] = s.last_pos[1] = s.last_pos[2] = 0.0f;
+    s.ndd = 0;
+    s.flags = fp.inside ? PATH_INSIDE : 0u;
+    s.pixel = p;
+    s.frame = frame;
+    s.lambda = lam;
+    s.rng_state = rng.state;
+    s.rng_inc = rng.inc;
+    paths[i] = s;
+    rays[i] = ap_ray{cam.origin.x, cam.origin.y, cam.origin.z, eps[0], d.x, d.y, d.z, __int_as_float(0x7f800000)};
+    path_of[i] = static_cast<int32_t>(i);
+}
+
+struct ShadeParams {
+    int32_t depth;
+    int32_t max_depth;
+    int32_t rr_min_depth;
+    int32_t nee_max_depth;
+    int32_t use_nee;
+    int32_t nee_always;
+    int32_t sky;
+    int32_t in_q;
+    int32_t out_q;
+    int32_t n_lights;
+};
+
+// Hit attributes as the reference intersectors report them.
+struct HitInfo {
+    f3 pos, normal;
+    int32_t mat, light_index;
+};
+
+__device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict__ an, f3 o, f3 d, float t) {
+    HitInfo h;
+    h.mat = g.mat;
+    h.light_index = g.light_index;
+    if (g.type == AP_OBJ_TRIANGLE) {
+        // IntersectTriangleObject (intersection.cuh:15-89): same MT, then the
+        // interpolated normal turned toward the ray.
+        float tt, u, v;
+        (void)tri_mt(o, d, {g.p0[0], g.p0[1], g.p0[2]}, {g.e1[0], g.e1[1], g.e1[2]}, {g.e2[0], g.e2[1], g.e2[2]}, tt,
+                     u, v);
+        const float w = 1.0f - u - v;
+        const f3 n0{g.n0[0], g.n0[1], g.n0[2]}, n1{g.n1[0], g.n1[1], g.n1[2]}, n2{g.n2[0], g.n2[1], g.n2[2]};
+        const f3 n = normalize(n0 * w + n1 * u + n2 * v);
+        h.pos = o + d * t;
+        h.normal = dot(d, n) > 0.0f ? -n : n;
+        return h;
+    }
+    const AnalyticRec& a = an[g.analytic];
+    const f3 c{a.center[0], a.center[1], a.center[2]}, ext{a.extent[0], a.extent[1], a.extent[2]};
+    const m3 w2o{{a.w2o[0], a.w2o[1], a.w2o[2]}, {a.w2o[3], a.w2o[4], a.w2o[5]}, {a.w2o[6], a.w2o[7], a.w2o[8]}};
+    const m3 o2w{{a.o2w[0], a.o2w[1], a.o2w[2]}, {a.o2w[3], a.o2w[4], a.o2w[5]}, {a.o2w[6], a.o2w[7], a.o2w[8]}};
+    const f3 lo = mul(w2o, o - c);
+    const f3 ld = normalize(mul(w2o, d));
+    if (g.type == AP_OBJ_CUBE) {
+        // IntersectAABBObject (aabb.cuh:28-89).
+        const f3 div = inv_safe(ld);
+        const f3 t1 = (-ext - lo) * div, t2 = (ext - lo) * div;
+        const f3 mn = fmin3(t1, t2), mx = fmax3(t1, t2);
+        const float te = fmaxf(fmaxf(mn.x, mn.y), mn.z);
+        const float tx = fminf(fminf(mx.x, mx.y), mx.z);
+        const bool inside = te < 0.0f;
+        const float th = inside ? tx : te;
+        const f3 lp = lo + ld * th;
+        const float rx = fabsf(lp.x) / ext.x, ry = fabsf(lp.y) / ext.y, rz = fabsf(lp.z) / ext.z;
+        f3 ln;
+        if (rx >= ry && rx >= rz) ln = f3{(lp.x > 0.0f) ? 1.0f : -1.0f, 0.0f, 0.0f};
+        else if (ry >= rz) ln = f3{0.0f, (lp.y > 0.0f) ? 1.0f : -1.0f, 0.0f};
+        else ln = f3{0.0f, 0.0f, (lp.z > 0.0f) ? 1.0f : -1.0f};
+        if (inside) ln = -ln;
+        h.pos = mul(o2w, lp) + c;
+        h.normal = normalize(mul(o2w, ln));
+        return h;
+    }
+    // IntersectSphereObject (sphere.cuh:10-70).
+    const float r = max3(ext);
+    const float b = dot(lo, ld);
+    const float cc = dot(lo, lo) - r * r;
+    const float s = sqrtf(b * b - 1.0f * cc);
+    const bool inside = (-b - s) <= 0.0f;
+    const f3 lp = lo + ld * t;
+    const f3 ln = normalize(lp) * (inside ? -1.0f : 1.0f);
+    h.pos = mul(o2w, lp) + c;
+    h.normal = normalize(mul(o2w, ln));
+    return h;
+}
+
+// NEESampleLight (nee.cuh:330-377) with the box-facet and sphere-cap samplers.
+__device__ LightSample sample_light(const LightRec* __restrict__ lights, int n, f3 pos, Pcg32& rng) {
+    LightSample ls{};
+    ls.valid = false;
+    if (n <= 0) return ls;
+    int pick = n - 1;
+    float pprob = 0.0f;
+    {
+        const float r = rand1(rng);
+        float acc = 0.0f;
+        for (int k = 0; k < n; ++k) {
+            const float pk = lights[k].select_prob;
+            acc += pk;
+            if (r < acc) { pick = k; pprob = pk; break; }
+        }
+    }
+    const LightRec& L = lights[pick];
+    if (L.type == AP_OBJ_CUBE) {
+        f3 fn[6], fc[6];
+        float s1[6], s2[6], area[6];
+        box_facets(L, fn, fc, s1, s2);
+        const f3 ux{L.ux[0], L.ux[1], L.ux[2]}, uy{L.uy[0], L.uy[1], L.uy[2]}, uz{L.uz[0], L.uz[1], L.uz[2]};
+        const f3 tang[6] = {uy, uy, ux, ux, ux, ux};
+        const f3 bitan[6] = {uz, uz, uz, uz, uy, uy};
+        for (int f = 0; f < 6; ++f) area[f] = 4.0f * s1[f] * s2[f];
+        int vidx[6];
+        float vw[6];
+        int vc = 0;
+        float vsum = 0.0f;
+        for (int f = 0; f < 6; ++f) {
+            const float w = facing_weight(fn[f], fc[f], pos, area[f]);
+            if (w > 0.0f) { vidx[vc] = f; vw[vc] = w; vsum += w; ++vc; }
+        }
+        if (vc == 0) return ls;
+        int slot = 0, face = vidx[0];
+        {
+            const float r = rand1(rng) * vsum;
+            float acc = 0.0f;
+            for (int k = 0; k < vc; ++k) {
+                acc += vw[k];
+                if (r <= acc) { slot = k; face = vidx[k]; break; }
+            }
+        }
+        const float u1 = 2.0f * rand1(rng) - 1.0f;
+        const float u2 = 2.0f * rand1(rng) - 1.0f;
+        ls.point = fc[face] + tang[face] * (u1 * s1[face]) + bitan[face] * (u2 * s2[face]);
+        ls.normal = fn[face];
+        ls.p_facet = 1.0f * (vw[slot] / vsum);
+        ls.p_area = 1.0f / area[face];
+        ls.valid = true;
+    } else if (L.type == AP_OBJ_SPHERE) {
+        const f3 sc{L.center[0], L.center[1], L.center[2]};
+        const float r = fmaxf(L.extent[0], fmaxf(L.extent[1], L.extent[2]));
+        const f3 pc = pos - sc;
+        const float dist = length(pc);
+        if (dist <= r) return ls;
+        const f3 axis = pc / dist;
+        const float cosC = clampf(r / dist, 0.0f, 1.0f);
+        const float cap = 2.0f * kPiF * r * r * (1.0f - cosC);
+        const float u = rand1(rng);
+        const float vphi = rand1(rng);
+        const float cosT = lerpf(cosC, 1.0f, u);
+        const float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
+        const float phi = 2.0f * kPiF * vphi;
+        f3 t, b;
+        tangent_basis(axis, t, b);
+        const f3 nn = t * (cosf(phi) * sinT) + b * (sinf(phi) * sinT) + axis * cosT;
+        ls.point = sc + nn * r;
+        ls.normal = nn;
+        ls.p_facet = 1.0f;
+        ls.p_area = 1.0f / cap;
+        ls.valid = true;
+    } else {
+        return ls;
+    }
+    ls.object = L.object;
+    ls.mat = L.mat;
+    ls.p_facet = ls.p_facet * pprob;
+    return ls;
+}
+
+__global__ __launch_bounds__(kBlock) void shade_kernel(
+    ShadeParams sp, const GeoRec* __restrict__ geo, const MatRec* __restrict__ mats,
+    const LightRec* __restrict__ lights, const AnalyticRec* __restrict__ analytic, const f4* __restrict__ rgb_lut,
+    const float* __restrict__ eps, PathState* __restrict__ paths, const ap_ray* __restrict__ rays_in,
+    const int32_t* __restrict__ path_in, const ap_hit* __restrict__ hits, ap_ray* __restrict__ rays_out,
+    int32_t* __restrict__ path_out, ap_ray* __restrict__ shadow_rays, ShadowPayload* __restrict__ shadow_pay,
+    uint32_t* __restrict__ counters, int32_t* __restrict__ primary_id, int32_t primary_frame) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = counters[sp.in_q];
+    if (i >= n) return;
+    const int32_t pid = path_in[i];
+    PathState S = paths[pid];
+    Pcg32 rng{S.rng_state, S.rng_inc};
+    const ap_ray R = rays_in[i];
+    const ap_hit H = hits[i];
+    const f3 o{R.ox, R.oy, R.oz}, d{R.dx, R.dy, R.dz};
+    const f4 lb = S.lambda_basis;
+    const bool inside = (S.flags & PATH_INSIDE) != 0u;
+    if (sp.depth == 0 && primary_id && S.frame == primary_frame) primary_id[S.pixel] = H.id;
+
+    bool emit_next = false, emit_shadow = false;
+    ap_ray next{}, shadow{};
+    ShadowPayload pay{pid, 0.0f, 0u, 0u};
+
+    if (H.id < 0) {
+        // Miss: SkyLighting and the offset draw of Trace, then the path ends.
+        (void)rand1(rng);
+        if (!inside) {
+            const float le = sp.sky ? interact(rgb_lut, lb, sky_lighting(d)) : 0.0f;
+            S.radiance = S.radiance + S.mask * le * S.d65;
+        }
+        S.flags &= ~PATH_PREV_NEE;
+    } else {
+        const GeoRec g = geo[H.id];
+        const HitInfo hi = hit_attributes(g, analytic, o, d, H.t);
+        const MatRec M = mats[hi.mat];
+        const f3 N = hi.normal;
+        f3 hp = hi.pos;
+        // SampleBSDF (material.cuh:177-330) for DIFFUSE / LIGHT / GGX.
+        f3 next_dir{0.0f, 0.0f, 0.0f};
+        float pdf = 1.0f;
+        bool terminate = false;
+        bool pure_light = false;
+        if (M.shading_model == SM_DIFFUSE) {
+            const float e0 = rand1(rng), e1 = rand1(rng), e2 = rand1(rng);
+            const DirSample ds = cosine_hemisphere(f3{e0, e1, e2}, N);
+            next_dir = ds.dir;
+            pdf = ds.pdf;
+        } else if (M.shading_model == SM_LIGHT) {
+            terminate = true;
+            pure_light = true;
+        } else {  // SM_GGX
+            const f3 V = -d;
+            const float NoV = fmaxf(0.0f, dot(N, V));
+            if (NoV <= 0.0f) {
+                terminate = true;
+            } else {
+                const float alpha = M.roughness * M.roughness;
+                const float F0 = lerpf(f0_ue(M.specular), albedo_response(M, lb), M.metallic);
+                const float pSpec = clampf(F0, 0.1f, 0.9f);
+                const float uu = rand1(rng);
+                if (uu < pSpec) {
+                    const float e0 = rand1(rng), e1 = rand1(rng), e2 = rand1(rng);
+                    const DirSample ds = sample_ggx_vndf(V, N, alpha, f3{e0, e1, e2});
+                    const float NoL = fmaxf(0.0f, dot(N, ds.dir));
+                    if (NoL <= 0.0f) {
+                        terminate = true;
+                    } else {
+                        next_dir = ds.dir;
+                        pdf = pSpec * ds.pdf + (1.0f - pSpec) * (NoL / kPiF);
+                    }
+                } else {
+                    const float e0 = rand1(rng), e1 = rand1(rng), e2 = rand1(rng);
+                    const DirSample ds = cosine_hemisphere(f3{e0, e1, e2}, N);
+                    next_dir = ds.dir;
+                    pdf = pSpec * ggx_vndf_pdf(V, N, next_dir, alpha) + (1.0f - pSpec) * ds.pdf;
+                }
+            }
+        }
+        const float bsdf_cos = eval_bsdf_cos(M, lb, d, next_dir, N);
+        // Trace's offset of the shading point (render.cuh:156-161).
+        {
+            const float off = eps[S.ndd] * fmaxf(1.0f, length(hp));
+            hp = hp + N * signf(dot(N, next_dir)) * off * (rand1(rng) + 0.5f);
+        }
+        // Emission of a light hit, MIS-weighted against the previous NEE (render.cuh:666-710).
+        if (hi.light_index >= 0) {
+            float mis = 1.0f;
+            if (sp.use_nee && sp.depth > 0 && (S.flags & PATH_PREV_NEE)) {
+                const float pB = fmaxf(S.last_pdf, 0.0f);
+                const float pL = nee_pdf_from_hit(lights[hi.light_index], {S.last_pos[0], S.last_pos[1], S.last_pos[2]},
+                                                  hp, N);
+                if (pL >= 0.0f && pB > 0.0f) mis = power_heuristic(pB, pL);
+            }
+            const float le = interact_basis(M.emissive_basis, M.emissive_scale, lb);
+            S.radiance = S.radiance + S.mask * le * S.d65 * mis;
+        }
+        (void)pure_light;
+        if (terminate) {
+            S.flags &= ~PATH_PREV_NEE;
+        } else {
+            // NEE (render.cuh:766-827, 206-343).
+            bool nee_valid = false;
+            bool nee_deferred = false;
+            const float rate = sp.nee_always ? 1.0f : sqrtf(saturatef(S.mask));
+            if (sp.use_nee && S.ndd < sp.nee_max_depth && !inside && rand1(rng) < rate) {
+                const LightSample ls = sample_light(lights, sp.n_lights, hp, rng);
+                if (ls.valid) {
+                    const float pat = fmaxf(1e-20f, ls.p_area * ls.p_facet);
+                    const f3 lu = ls.point - hp;
+                    const float d2 = fmaxf(dot(lu, lu), 1e-12f);
+                    const f3 L = lu * (1.0f / sqrtf(d2));
+                    const float cosX = fmaxf(0.0f, dot(N, L));
+                    const float cosY = fmaxf(0.0f, dot(ls.normal, -L));
+                    const bool cos_ok = !(cosX <= 0.0f || cosY <= 0.0f);
+                    float add = 0.0f;
+                    if (cos_ok) {
+                        const float bc = eval_bsdf_cos(M, lb, d, L, N);
+                        const float G = cosY / d2;
+                        const MatRec& LM = mats[ls.mat];
+                        const float resp = interact_basis(LM.emissive_basis, LM.emissive_scale, lb);
+                        const float contribution = bc * resp * (G / pat);
+                        const float pLo = area_to_solid_angle_pdf(hp, ls.point, ls.normal, ls.p_facet * ls.p_area);
+                        const float pBs = fmaxf(bsdf_pdf(M, lb, N, d, L), 0.0f);
+                        const float rad = contribution * power_heuristic(pLo, pBs);
+                        add = S.mask * rad * S.d65;
+                    }
+                    // AnyHitSegmentBVH (intersection.cuh:372-397).
+                    const f3 seg = ls.point - hp;
+                    const float dist = length(seg);
+                    bool traced = false;
+                    if (dist > 0.0f) {
+                        const f3 sd = seg / dist;
+                        const float seps = eps[sp.depth] * fmaxf(1.0f, fmaxf(length(hp), length(ls.point)));
+                        const float tmax = fmaxf(0.0f, dist - seps);
+                        if (tmax > 0.0f) {
+                            shadow = ap_ray{hp.x, hp.y, hp.z, 0.0f, sd.x, sd.y, sd.z, tmax};
+                            traced = true;
+                        }
+                    }
+                    if (traced) {
+                        emit_shadow = true;
+                        nee_deferred = true;
+                        pay.add = add;
+                        pay.cos_ok = cos_ok ? 1u : 0u;
+                    } else {
+                        nee_valid = cos_ok;
+                        if (cos_ok) S.radiance = S.radiance + add;
+                    }
+                }
+            }
+            if (!nee_deferred) {
+                if (nee_valid) S.flags |= PATH_PREV_NEE;
+                else S.flags &= ~PATH_PREV_NEE;
+            }
+            S.last_pdf = pdf;
+            S.last_pos[0] = hp.x; S.last_pos[1] = hp.y; S.last_pos[2] = hp.z;
+            const float contrib = bsdf_cos / fmaxf(pdf, 1e-12f);
+            // Russian roulette (render.cuh:838-859).
+            float p_rr = 1.0f;
+            bool survived = true;
+            if (S.ndd >= sp.rr_min_depth) {
+                const float basic = clampf(S.mask * 0.5f + S.mask * 0.5f, 0.05f, 0.95f);
+                p_rr = lerpf(basic, 1.0f, inside ? 0.99f : 0.0f);
+                if (rand1(rng) > p_rr) survived = false;
+            }
+            if (survived) {
+                S.mask = S.mask * contrib;
+                S.mask = S.mask / p_rr;
+                S.ndd += 1;  // no delta surfaces in this core
+                if (sp.depth + 1 < sp.max_depth) {
+                    emit_next = true;
+                    next = ap_ray{hp.x, hp.y, hp.z, eps[S.ndd], next_dir.x, next_dir.y, next_dir.z,
+                                  __int_as_float(0x7f800000)};
+                }
+            }
+        }
+    }
+    S.rng_state = rng.state;
+    S.rng_inc = rng.inc;
+    paths[pid] = S;
+    const uint32_t qi = wave_append(&counters[sp.out_q], emit_next);
+    if (emit_next) {
+        rays_out[qi] = next;
+        path_out[qi] = pid;
+    }
+    const uint32_t si = wave_append(&counters[2], emit_shadow);
+    if (emit_shadow) {
+        shadow_rays[si] = shadow;
+        shadow_pay[si] = pay;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void resolve_kernel(const uint32_t* __restrict__ counters,
+                                                         const ShadowPayload* __restrict__ pay,
+                                                         const ap_hit* __restrict__ shits,
+                                                         PathState* __restrict__ paths) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= counters[2]) return;
+    const ShadowPayload p = pay[i];
+    const bool occluded = shits[i].id >= 0;
+    PathState& S = paths[p.path];
+    const bool cos_ok = p.cos_ok != 0u;
+    if (!occluded && cos_ok) S.radiance = S.radiance + p.add;
+    if (occluded || cos_ok) S.flags |= PATH_PREV_NEE;
+    else S.flags &= ~PATH_PREV_NEE;
+}
+
+__device__ __forceinline__ float fix_nan_inf(float x) {
+    if (isnan(x)) return 0.0f;
+    if (isinf(x)) return 0.0f;
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, const PathState* __restrict__ paths,
+                                                            float* __restrict__ hdr, float* __restrict__ odd,
+                                                            float* __restrict__ even, double* __restrict__ sums) {
+    const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= fp.n_pixels) return;
+    f3 acc{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
+    f3 ao{0, 0, 0}, ae{0, 0, 0};
+    if (odd) ao = f3{odd[3 * p], odd[3 * p + 1], odd[3 * p + 2]};
+    if (even) ae = f3{even[3 * p], even[3 * p + 1], even[3 * p + 2]};
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int32_t f = 0; f < fp.frames; ++f) {
+        const PathState& S = paths[static_cast<size_t>(f) * fp.n_pixels + p];
+        const float rad = fix_nan_inf(S.radiance);
+        const f3 xyz{S.xyz[0], S.xyz[1], S.xyz[2]};
+        const f3 pre = (xyz * rad) / S.lambda_pdf;
+        f3 res{clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
+        res = fmax3(f3{0.0f, 0.0f, 0.0f}, res);
+        const int32_t frame = S.frame;
+        const float w = 1.0f / static_cast<float>(1 + frame);
+        acc = lerp3(acc, res, w);
+        if (odd && even) {
+            const float wh = 1.0f / static_cast<float>(1 + (frame >> 1));
+            if (frame % 2 == 1) ao = lerp3(ao, res, wh);
+            else ae = lerp3(ae, res, wh);
+        }
+        s0 += static_cast<double>(res.x);
+        s1 += static_cast<double>(res.y);
+        s2 += static_cast<double>(res.z);
+    }
+    hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
+    if (odd && even) {
+        odd[3 * p] = ao.x; odd[3 * p + 1] = ao.y; odd[3 * p + 2] = ao.z;
+        even[3 * p] = ae.x; even[3 * p + 1] = ae.y; even[3 * p + 2] = ae.z;
+    }
+    if (sums) {
+        sums[3 * p] += s0;
+        sums[3 * p + 1] += s1;
+        sums[3 * p + 2] += s2;
+    }
+}
+
+__global__ void build_mats_kernel(const MatIn* __restrict__ in, int n, const f4* __restrict__ lut,
+                                  MatRec* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const MatIn m = in[i];
+    MatRec r;
+    const float la = max3(m.albedo);
+    r.albedo_basis = query_rgb_basis(lut, la > 1.0f ? m.albedo / la : m.albedo);
+    r.albedo_scale = la;
+    const float le = max3(m.emissive);
+    r.emissive_basis = query_rgb_basis(lut, le > 1.0f ? m.emissive / le : m.emissive);
+    r.emissive_scale = le;
+    r.specular = m.specular;
+    r.metallic = m.metallic;
+    r.roughness = m.roughness;
+    r.shading_model = m.model;
+    r.light_index = m.light_index;
+    r.pad = 0;
+    out[i] = r;
+}
+
+__global__ void xyz_to_srgb_kernel(const float* __restrict__ xyz, size_t n, float* __restrict__ rgb) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 r = xyz_to_srgb(f3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]});
+    rgb[3 * i] = r.x; rgb[3 * i + 1] = r.y; rgb[3 * i + 2] = r.z;
+}
+
+inline dim3 grid_of(size_t n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
+
+template <typename T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return AP_OK;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    return e == hipSuccess ? AP_OK : hip_fail(e, "hipMalloc(render)");
+}
+
+template <typename T>
+int upload_vec(const std::vector<T>& v, T** p) {
+    int rc = dev_alloc(p, v.size());
+    if (rc != AP_OK || v.empty()) return rc;
+    const hipError_t e = hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "hipMemcpy(render)");
+}
+
+inline f3 F(const ap_float3& a) { return {a.x, a.y, a.z}; }
+inline m3 M3(const ap_float3x3& m) { return {F(m.x), F(m.y), F(m.z)}; }
+
+// Shading records of the scene for a triangle-material mode.
+int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
+    for (void* p : {static_cast<void*>(r->geo), static_cast<void*>(r->mats), static_cast<void*>(r->lights)})
+        if (p) (void)hipFree(p);
+    r->geo = nullptr; r->mats = nullptr; r->lights = nullptr;
+    const size_t n = s->objects.size();
+    std::vector<GeoRec> geo(n);
+    std::vector<MatIn> mats;
+    std::map<std::vector<uint32_t>, int32_t> mat_index;
+    auto intern = [&](const MatIn& m) {
+        std::vector<uint32_t> key(sizeof(MatIn) / 4);
+        std::memcpy(key.data(), &m, sizeof(MatIn));
+        auto it = mat_index.find(key);
+        if (it != mat_index.end()) return it->second;
+        const int32_t id = static_cast<int32_t>(mats.size());
+        mats.push_back(m);
+        mat_index.emplace(key, id);
+        return id;
+    };
+    // Analytic record index per object, in slot order as the scene built them.
+    std::vector<int32_t> analytic_of(n, -1);
+    {
+        int32_t k = 0;
+        for (size_t slot = 0; slot < s->bvh.indices.size(); ++slot) {
+            const ap_scene_object& o = s->objects[s->bvh.indices[slot]];
+            if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SPHERE) analytic_of[s->bvh.indices[slot]] = k++;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const ap_scene_object& o = s->objects[i];
+        if (o.material.shading_model == AP_MAT_GLASS)
+            return fail(AP_ERR_UNSUPPORTED, "ap_render: glass materials are not supported (object " + std::to_string(i) + ")");
+        const ap_gbuffer& gb = o.material.texture.base;
+        if (gb.albedo_tex || gb.normal_tex || gb.metallic_tex || gb.roughness_tex)
+            return fail(AP_ERR_UNSUPPORTED, "ap_render: textured materials are not supported (object " + std::to_string(i) + ")");
+        MatIn m;
+        const bool ref_tri = o.type == AP_OBJ_TRIANGLE && tri_mode == AP_TRIANGLE_MATERIAL_REFERENCE;
+        if (ref_tri) {  // default-constructed GBuffer (object.cuh:60-72)
+            m.albedo = {1.0f, 1.0f, 1.0f};
+            m.emissive = {0.0f, 0.0f, 0.0f};
+            m.specular = 0.0f; m.metallic = 0.0f; m.roughness = 0.0f;
+        } else {
+            m.albedo = F(gb.albedo);
+            m.emissive = F(gb.emissive);
+            m.specular = gb.specular; m.metallic = gb.metallic; m.roughness = gb.roughness;
+        }
+        m.model = o.material.shading_model;
+        m.light_index = o.light_index;
+        GeoRec& g = geo[i];
+        std::memset(&g, 0, sizeof(g));
+        g.type = o.type;
+        g.mat = intern(m);
+        g.analytic = analytic_of[i];
+        g.light_index = o.light_index;
+        if (o.type == AP_OBJ_TRIANGLE) {
+            const m3 o2w = M3(o.object_to_world);
+            const f3 c = F(o.center);
+            const f3 p0 = mul(o2w, F(o.additional.v0.position)) + c;
+            const f3 p1 = mul(o2w, F(o.additional.v1.position)) + c;
+            const f3 p2 = mul(o2w, F(o.additional.v2.position)) + c;
+            const f3 e1 = p1 - p0, e2 = p2 - p0;
+            const f3 n0 = mul(o2w, F(o.additional.v0.normal));
+            const f3 n1 = mul(o2w, F(o.additional.v1.normal));
+            const f3 n2 = mul(o2w, F(o.additional.v2.normal));
+            const float* src[6] = {&p0.x, &e1.x, &e2.x, &n0.x, &n1.x, &n2.x};
+            float* dst[6] = {g.p0, g.e1, g.e2, g.n0, g.n1, g.n2};
+            for (int k = 0; k < 6; ++k) std::memcpy(dst[k], src[k], 12);
+        }
+    }
+    // Lights (scene.cu:1056-1080) with the OBB axes of GetOBBAxes (nee.cuh:24-31).
+    std::vector<LightRec> lights(s->lights.size());
+    for (size_t k = 0; k < s->lights.size(); ++k) {
+        const ap_light_info& li = s->lights[k];
+        const ap_scene_object& o = s->objects[li.object_index];
+        LightRec& L = lights[k];
+        std::memset(&L, 0, sizeof(L));
+        L.type = (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SPHERE) ? o.type : -1;
+        L.object = li.object_index;
+        L.center[0] = o.center.x; L.center[1] = o.center.y; L.center[2] = o.center.z;
+        L.extent[0] = o.extent.x; L.extent[1] = o.extent.y; L.extent[2] = o.extent.z;
+        const ap_float3x3& Mw = o.object_to_world;
+        const f3 ux = normalize(f3{Mw.x.x, Mw.y.x, Mw.z.x});
+        const f3 uy = normalize(f3{Mw.x.y, Mw.y.y, Mw.z.y});
+        const f3 uz = normalize(f3{Mw.x.z, Mw.y.z, Mw.z.z});
+        std::memcpy(L.ux, &ux, 12); std::memcpy(L.uy, &uy, 12); std::memcpy(L.uz, &uz, 12);
+        L.select_prob = li.select_prob;
+        L.mat = geo[li.object_index].mat;
+    }
+    int rc;
+    MatIn* d_in = nullptr;
+    if ((rc = upload_vec(geo, &r->geo)) != AP_OK) return rc;
+    if ((rc = upload_vec(lights, &r->lights)) != AP_OK) return rc;
+    if ((rc = upload_vec(mats, &d_in)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->mats, std::max<size_t>(mats.size(), 1))) != AP_OK) { (void)hipFree(d_in); return rc; }
+    if (!mats.empty()) {
+        hipLaunchKernelGGL(build_mats_kernel, dim3((mats.size() + 127) / 128), dim3(128), 0, nullptr, d_in,
+                           static_cast<int>(mats.size()), lut, r->mats);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(d_in);
+        if (e != hipSuccess) return hip_fail(e, "build_mats_kernel");
+    }
+    r->n_lights = static_cast<int32_t>(lights.size());
+    r->n_mats = static_cast<int32_t>(mats.size());
+    r->tri_mode = tri_mode;
+    return AP_OK;
+}
+
+int ensure_paths(RenderState* r, size_t cap) {
+    if (r->capacity >= cap) return AP_OK;
+    void* old[] = {r->paths, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1], r->hits, r->shadow_rays,
+                   r->shadow_payload, r->shadow_hits};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    r->capacity = 0;
+    int rc;
+    if ((rc = dev_alloc(&r->paths, cap)) != AP_OK) return rc;
+    for (int q = 0; q < 2; ++q) {
+        if ((rc = dev_alloc(&r->rays[q], cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->path_of[q], cap)) != AP_OK) return rc;
+    }
+    if ((rc = dev_alloc(&r->hits, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->shadow_rays, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->shadow_payload, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->shadow_hits, cap)) != AP_OK) return rc;
+    r->capacity = cap;
+    return AP_OK;
+}
+
+// OverlapBase at the camera origin over every object (intersection.cuh:399-414,
+// aabb.cuh:91-106, sphere.cuh:179-192): whether the camera starts inside one.
+bool camera_inside(const ap_scene* s, f3 o) {
+    for (const ap_scene_object& ob : s->objects) {
+        if (ob.type != AP_OBJ_CUBE && ob.type != AP_OBJ_SPHERE) continue;
+        const f3 lo = mul(M3(ob.world_to_object), o - F(ob.center));
+        const f3 e = F(ob.extent);
+        if (ob.type == AP_OBJ_CUBE) {
+            if (-e.x < lo.x && -e.y < lo.y && -e.z < lo.z && e.x > lo.x && e.y > lo.y && e.z > lo.z) return true;
+        } else {
+            const float rr = max3(e);
+            if (dot(lo, lo) < rr * rr) return true;
+        }
+    }
+    return false;
+}
+
+}  // namespace
+
+int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm, const ap_framebuffers* fb,
+                  hipStream_t stream) {
+    if (!s || !cam || !prm || !fb || !fb->hdr) return fail(AP_ERR_INVALID_ARG, "ap_render: NULL argument");
+    if (cam->width <= 0 || cam->height <= 0) return fail(AP_ERR_INVALID_ARG, "ap_render: bad resolution");
+    if (prm->spp < 0 || prm->frame_index < 0 || prm->max_depth < 0 || prm->max_depth > kEpsEntries - 2)
+        return fail(AP_ERR_INVALID_ARG, "ap_render: bad spp, frame_index or max_depth");
+    if (prm->triangle_material != AP_TRIANGLE_MATERIAL_OBJECT && prm->triangle_material != AP_TRIANGLE_MATERIAL_REFERENCE)
+        return fail(AP_ERR_INVALID_ARG, "ap_render: bad triangle_material");
+    if (prm->use_guiding) return fail(AP_ERR_UNSUPPORTED, "ap_render: path guiding is not available in this build");
+    if (prm->spp == 0 || prm->max_depth == 0) return AP_OK;
+    const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
+    const float* tables = nullptr;
+    int rc = spectrum_get(&rgb_lut, &lam_lut, &tables);
+    if (rc != AP_OK) return rc;
+    if (!s->render) s->render = new RenderState();
+    RenderState* r = s->render;
+    if (!r->eps) {
+        // INTERSECT_EPS(depth) = 1e-3 * expf(min(depth / 2, 4)) (config.cuh:3), on the host.
+        std::vector<float> e(kEpsEntries);
+        for (int d = 0; d < kEpsEntries; ++d) e[d] = 1.0e-3f * std::exp(fminf(static_cast<float>(d) * 0.5f, 4.0f));
+        if ((rc = upload_vec(e, &r->eps)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->counters, 4)) != AP_OK) return rc;
+    }
+    if (r->tri_mode != prm->triangle_material) {
+        if ((rc = build_shading(s, r, prm->triangle_material, rgb_lut)) != AP_OK) return rc;
+    }
+    const int32_t n_pix = cam->width * cam->height;
+    // Frames per batch: about 4M paths in flight.
+    const int32_t per_batch = std::max(1, std::min(prm->spp, static_cast<int32_t>((4u << 20) / static_cast<uint32_t>(n_pix))));
+    if ((rc = ensure_paths(r, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
+
+    // Camera basis of RenderCamera (scene.cu:220-228), direction normalised
+    // as Camera::SetDirection does (camera.hpp:38-41).
+    CamParams cp;
+    cp.origin = {cam->origin[0], cam->origin[1], cam->origin[2]};
+    cp.dir = normalize(f3{cam->direction[0], cam->direction[1], cam->direction[2]});
+    const float fov = cam->fov_deg * 3.1415926f / 180.0f;
+    const float aspect = static_cast<float>(cam->width) / fmaxf(1.0f, static_cast<float>(cam->height));
+    cp.tan_y = tanf(0.5f * fov);
+    cp.tan_x = cp.tan_y * aspect;
+    cp.right = normalize(cross(cp.dir, f3{0.0f, 1.0f, 0.0f}));
+    cp.up = normalize(cross(cp.dir, cp.right));
+    cp.width = cam->width;
+    cp.height = cam->height;
+    const int32_t inside = camera_inside(s, cp.origin) ? 1 : 0;
+    const int32_t last_frame = prm->frame_index + prm->spp - 1;
+
+    for (int32_t f0 = 0; f0 < prm->spp; f0 += per_batch) {
+        const int32_t nf = std::min(per_batch, prm->spp - f0);
+        const uint32_t total = static_cast<uint32_t>(nf) * static_cast<uint32_t>(n_pix);
+        FrameParams fp{prm->frame_index + f0, nf, n_pix, inside, prm->seed};
+        hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, r->counters, total, 0u, 0u);
+        hipLaunchKernelGGL(generate_kernel, grid_of(total), dim3(kBlock), 0, stream, cp, fp, lam_lut, tables, r->eps,
+                           r->paths, r->rays[0], r->path_of[0]);
+        int in_q = 0;
+        for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
+            const int out_q = 1 - in_q;
+            hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, out_q);
+            hipError_t e = launch_trace(s->dev, r->rays[in_q], total, r->hits, nullptr, false, stream, &r->counters[in_q]);
+            if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
+            ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
+                           prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
+            hipLaunchKernelGGL(shade_kernel, grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats, r->lights,
+                               s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q], r->path_of[in_q], r->hits,
+                               r->rays[out_q], r->path_of[out_q], r->shadow_rays, r->shadow_payload, r->counters,
+                               fb->primary_id, last_frame);
+            e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
+            if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
+            hipLaunchKernelGGL(resolve_kernel, grid_of(total), dim3(kBlock), 0, stream, r->counters, r->shadow_payload,
+                               r->shadow_hits, r->paths);
+            in_q = out_q;
+            // Deep limits: stop once every path has ended.
+            if (prm->max_depth > 32 && (depth % 16) == 15) {
+                uint32_t live = 0;
+                e = hipMemcpyAsync(&live, &r->counters[in_q], sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) return hip_fail(e, "ap_render: live count");
+                if (live == 0) break;
+            }
+        }
+        hipLaunchKernelGGL(accumulate_kernel, grid_of(n_pix), dim3(kBlock), 0, stream, fp, r->paths, fb->hdr, fb->odd,
+                           fb->even, fb->sum_xyz);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
+    }
+    return AP_OK;
+}
+
+}  // namespace ap
+
+using namespace ap;
+
+extern "C" {
+
+void ap_render_params_default(ap_render_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->spp = 1;
+    p->frame_index = 0;
+    p->max_depth = 1000;
+    p->rr_min_depth = 3;
+    p->nee_max_depth = 10;
+    p->use_nee = 1;
+    p->nee_always = 0;
+    p->sky = 1;
+    p->triangle_material = AP_TRIANGLE_MATERIAL_OBJECT;
+    p->use_guiding = 0;
+    p->seed = 0;
+}
+
+int ap_render(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, const ap_framebuffers* fb,
+              void* stream) {
+    return render_frames(scene, camera, params, fb, static_cast<hipStream_t>(stream));
+}
+
+int ap_render_offline(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, float* host_xyz) {
+    if (!scene || !camera || !params || !host_xyz) return fail(AP_ERR_INVALID_ARG, "ap_render_offline: NULL argument");
+    if (camera->width <= 0 || camera->height <= 0) return fail(AP_ERR_INVALID_ARG, "ap_render_offline: bad resolution");
+    const size_t n = static_cast<size_t>(camera->width) * camera->height * 3;
+    float* target = nullptr;
+    hipError_t e = hipMalloc(&target, n * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(offline target)");
+    e = hipMemset(target, 0, n * sizeof(float));
+    int rc = e == hipSuccess ? AP_OK : hip_fail(e, "hipMemset");
+    if (rc == AP_OK) {
+        ap_render_params p = *params;
+        p.frame_index = 0;
+        ap_framebuffers fb{};
+        fb.hdr = target;
+        rc = render_frames(scene, camera, &p, &fb, nullptr);
+    }
+    if (rc == AP_OK) {
+        e = hipMemcpy(host_xyz, target, n * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(offline)");
+    }
+    (void)hipFree(target);
+    return rc;
+}
+
+int ap_xyz_to_srgb(const float* xyz, size_t n, float* rgb, void* stream) {
+    if (n == 0) return AP_OK;
+    if (!xyz || !rgb) return fail(AP_ERR_INVALID_ARG, "ap_xyz_to_srgb: NULL pointer");
+    hipLaunchKernelGGL(xyz_to_srgb_kernel, grid_of(n), dim3(kBlock), 0, static_cast<hipStream_t>(stream), xyz, n, rgb);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_xyz_to_srgb");
+}
+
+}  // extern "C"

@@ -12,8 +12,6 @@
 
 namespace ap {
 
-constexpr float kPiF = 3.14159265358979323846f;
-
 struct DirSample {
     f3 dir;
     float pdf;

@@ -3,6 +3,9 @@
 #include <cstring>
 #include <limits>
 
+#include <map>
+#include <utility>
+
 #include "aperture/ap_synth.h"
 #include "ap_internal.h"
 #include "ap_pcg.h"
@@ -73,6 +76,53 @@ static f3 qrot(const float q[4], f3 v) {
     return v + w * t + cross(u, t);
 }
 
+// Unit icosphere: 12 vertices, 20 faces, each subdivision splits a face in
+// four with normalised edge midpoints.
+static void icosphere(int subdiv, std::vector<f3>& verts, std::vector<int>& tris) {
+    const float t = (1.0f + std::sqrt(5.0f)) / 2.0f;
+    verts = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+             {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+    for (f3& v : verts) v = normalize(v);
+    tris = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6, 7, 1, 8,
+            3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+    for (int s = 0; s < subdiv; ++s) {
+        std::vector<int> next;
+        std::map<std::pair<int, int>, int> mid;
+        auto midpoint = [&](int a, int b) {
+            const std::pair<int, int> key{std::min(a, b), std::max(a, b)};
+            auto it = mid.find(key);
+            if (it != mid.end()) return it->second;
+            verts.push_back(normalize((verts[a] + verts[b]) * 0.5f));
+            const int id = static_cast<int>(verts.size()) - 1;
+            mid.emplace(key, id);
+            return id;
+        };
+        for (size_t k = 0; k < tris.size(); k += 3) {
+            const int a = tris[k], b = tris[k + 1], c = tris[k + 2];
+            const int ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+            const int nt[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+            next.insert(next.end(), nt, nt + 12);
+        }
+        tris.swap(next);
+    }
+}
+
+// One triangle object: centre = centroid, local vertices relative to it.
+static void triangle_object(ap_scene_object& o, f3 p0, f3 p1, f3 p2, f3 n0, f3 n1, f3 n2, f3 albedo) {
+    const f3 c = (p0 + p1 + p2) / 3.0f;
+    init_object(o, c, f3{1.0f, 1.0f, 1.0f}, AP_OBJ_TRIANGLE);
+    o.material.texture.base.albedo = {albedo.x, albedo.y, albedo.z};
+    ap_vertex* vs[3] = {&o.additional.v0, &o.additional.v1, &o.additional.v2};
+    const f3 ps[3] = {p0 - c, p1 - c, p2 - c};
+    const f3 ns[3] = {n0, n1, n2};
+    for (int k = 0; k < 3; ++k) {
+        vs[k]->position = {ps[k].x, ps[k].y, ps[k].z};
+        vs[k]->normal = {ns[k].x, ns[k].y, ns[k].z};
+        vs[k]->uv = {0.0f, 0.0f};
+    }
+    update_proxy(o);
+}
+
 }  // namespace ap
 
 using namespace ap;
@@ -110,6 +160,59 @@ int ap_synth_triangle_soup(size_t n, uint64_t seed, float edge_min, float edge_m
             vs[k]->uv = {0.0f, 0.0f};
         }
         update_proxy(o);
+    }
+    return AP_OK;
+}
+
+int ap_synth_box_scene(uint64_t seed, int sphere_subdiv, float light, ap_scene_object* out, size_t capacity,
+                       size_t* n_out) {
+    if (!n_out || sphere_subdiv < 0 || sphere_subdiv > 7) return fail(AP_ERR_INVALID_ARG, "ap_synth_box_scene: bad arguments");
+    std::vector<f3> sv;
+    std::vector<int> st;
+    icosphere(sphere_subdiv, sv, st);
+    const size_t n = 10 + 1 + 2 * (st.size() / 3);
+    *n_out = n;
+    if (!out) return AP_OK;
+    if (capacity < n) return fail(AP_ERR_INVALID_ARG, "ap_synth_box_scene: capacity too small");
+    Pcg32 rng = pcg32_seed(seed, 0);
+    auto albedo = [&]() {
+        const float r = 0.1f + 0.8f * pcg32_float(rng);
+        const float g = 0.1f + 0.8f * pcg32_float(rng);
+        const float b = 0.1f + 0.8f * pcg32_float(rng);
+        return f3{r, g, b};
+    };
+    size_t k = 0;
+    // Quads as (corner, edge u, edge v) with the inward normal u x v.
+    struct Quad { f3 c, u, v; };
+    const Quad quads[5] = {
+        {{-1, -1, -1}, {0, 0, 2}, {2, 0, 0}},   // floor y = -1, normal +y
+        {{-1, 1, -1}, {2, 0, 0}, {0, 0, 2}},    // ceiling y = +1, normal -y
+        {{-1, -1, -1}, {2, 0, 0}, {0, 2, 0}},   // back z = -1, normal +z
+        {{-1, -1, -1}, {0, 2, 0}, {0, 0, 2}},   // left x = -1, normal +x
+        {{1, -1, -1}, {0, 0, 2}, {0, 2, 0}},    // right x = +1, normal -x
+    };
+    for (const Quad& q : quads) {
+        const f3 a = albedo();
+        const f3 nrm = normalize(cross(q.u, q.v));
+        const f3 p00 = q.c, p10 = q.c + q.u, p11 = q.c + q.u + q.v, p01 = q.c + q.v;
+        triangle_object(out[k++], p00, p10, p11, nrm, nrm, nrm, a);
+        triangle_object(out[k++], p00, p11, p01, nrm, nrm, nrm, a);
+    }
+    {
+        ap_scene_object& L = out[k++];
+        init_object(L, f3{0.0f, 1.0f - 0.01f, 0.0f}, f3{0.25f, 0.01f, 0.25f}, AP_OBJ_CUBE);
+        L.material.shading_model = AP_MAT_LIGHT;
+        L.material.texture.base.albedo = {0.0f, 0.0f, 0.0f};
+        L.material.texture.base.emissive = {light, light, light};
+        update_proxy(L);
+    }
+    const f3 centres[2] = {{-0.45f, -0.7f, -0.3f}, {0.45f, -0.7f, 0.35f}};
+    for (const f3& c : centres) {
+        const f3 a = albedo();
+        for (size_t t = 0; t < st.size(); t += 3) {
+            const f3 n0 = sv[st[t]], n1 = sv[st[t + 1]], n2 = sv[st[t + 2]];
+            triangle_object(out[k++], c + n0 * 0.3f, c + n1 * 0.3f, c + n2 * 0.3f, n0, n1, n2, a);
+        }
     }
     return AP_OK;
 }

@@ -71,12 +71,15 @@ __device__ bool sphere_t(f3 o, f3 d, const AnalyticRec& a, float& t) {
 
 template <bool ANY, bool STATS, bool CULL, int CAP>
 __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray* __restrict__ rays,
-                                                     size_t n, ap_hit* __restrict__ hits,
+                                                     size_t n, const uint32_t* __restrict__ dyn_n,
+                                                     ap_hit* __restrict__ hits,
                                                      ap_trace_stats* __restrict__ stats) {
     __shared__ int32_t st_link[CAP * WAVE];
     __shared__ float st_te[CAP * WAVE];
     const int lane = threadIdx.x;
     const size_t i = static_cast<size_t>(blockIdx.x) * WAVE + lane;
+    // A device-side count (queues of the integrator) bounds a grid sized for capacity n.
+    if (dyn_n) n = *dyn_n;
     if (i >= n) return;
 
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
 }
 
 template <bool ANY, bool STATS, bool CULL>
-hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
+hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, size_t n, const uint32_t* dyn_n, ap_hit* hits,
                       ap_trace_stats* stats, hipStream_t stream) {
     const dim3 block(WAVE);
     const dim3 grid(static_cast<unsigned>((n + WAVE - 1) / WAVE));
@@ -189,7 +192,7 @@ hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit
 #define AP_CAP_CASE(C)                                                                   \
     if (cap <= C) {                                                                      \
         hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C>), grid, block, 0, stream, s, rays, n, \
-                           hits, stats);                                                 \
+                           dyn_n, hits, stats);                                                 \
         return hipGetLastError();                                                        \
     }
     AP_CAP_CASE(16)
@@ -206,11 +209,11 @@ hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit
 }  // namespace
 
 hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
-                        ap_trace_stats* stats, bool any_hit, hipStream_t stream) {
+                        ap_trace_stats* stats, bool any_hit, hipStream_t stream, const uint32_t* dyn_n) {
     if (n == 0) return hipSuccess;
     const bool cull = s.mode == AP_TRAVERSE_CULLED;
 #define AP_DISPATCH(A, S, C) \
-    if (any_hit == A && (stats != nullptr) == S && cull == C) return launch_cap<A, S, C>(s, rays, n, hits, stats, stream);
+    if (any_hit == A && (stats != nullptr) == S && cull == C) return launch_cap<A, S, C>(s, rays, n, dyn_n, hits, stats, stream);
     AP_DISPATCH(false, false, true)
     AP_DISPATCH(true, false, true)
     AP_DISPATCH(false, true, true)

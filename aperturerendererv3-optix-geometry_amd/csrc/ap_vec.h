@@ -25,6 +25,8 @@
 
 namespace ap {
 
+constexpr float kPiF = 3.14159265358979323846f;  // CUDART_PI_F rounds to the same float
+
 struct f3 { float x, y, z; };
 struct f2 { float x, y; };
 

@@ -1,0 +1,83 @@
+/*
+ * Path-tracing entry points of the Aperture core: the reference's
+ * RenderCamera -> CalculateRadiance megakernel (core/scene.cu:201-255,
+ * core/render/render.cuh:460-999) re-designed as a wavefront integrator
+ * (generate, extend, shade, shadow, resolve, accumulate) over compacted
+ * ray queues. Single-wavelength spectral transport with the reference's
+ * NEE, MIS, Russian roulette, epsilons and firefly clamp; the constants the
+ * reference hard-codes are parameters here, with its values as defaults.
+ */
+#ifndef APERTURE_AP_RENDER_H
+#define APERTURE_AP_RENDER_H
+
+#include "ap_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Camera (core/camera.hpp:56-63 + the pinhole of core/scene.cu:214-230).
+ * direction is normalised like Camera::SetDirection. */
+typedef struct ap_camera {
+    float origin[3];
+    float direction[3];
+    float fov_deg;      /* vertical FOV; the reference fixes 60 (scene.cu:220) */
+    int32_t width;
+    int32_t height;
+} ap_camera;
+
+/* AP_TRIANGLE_MATERIAL_* : which GBuffer a triangle hit carries. OBJECT
+ * uses the triangle's own material, as the cube and sphere intersectors do
+ * (aabb.cuh:86, sphere.cuh:174); REFERENCE keeps IntersectTriangleObject's
+ * default-constructed GBuffer (white albedo, roughness 0, no emission,
+ * intersection.cuh:17,63), the shading model still coming from the object. */
+#define AP_TRIANGLE_MATERIAL_OBJECT 0
+#define AP_TRIANGLE_MATERIAL_REFERENCE 1
+
+typedef struct ap_render_params {
+    int32_t spp;               /* samples per pixel added by this call (one frame each) */
+    int32_t frame_index;       /* FRAME_INDEX of the first sample: running-mean weight 1/(1+frame) */
+    int32_t max_depth;         /* MAX_TRACING_DEPTH (config.cuh:8) = 1000 */
+    int32_t rr_min_depth;      /* MIN_TRACING_DEPTH (config.cuh:7) = 3 */
+    int32_t nee_max_depth;     /* NEE_MAX_NON_DELTA_DEPTH (config.cuh:25) = 10 */
+    int32_t use_nee;           /* SceneSetting::bUseNEE_ = 1 */
+    int32_t nee_always;        /* 0: NEE with probability sqrt(saturate(mask)) (render.cuh:777); 1: always */
+    int32_t sky;               /* 1: SkyLighting on a miss (render.cuh:58-73); 0: black */
+    int32_t triangle_material; /* AP_TRIANGLE_MATERIAL_* (default OBJECT) */
+    int32_t use_guiding;       /* SceneSetting::bUsePathGuiding_ (probes, when built) */
+    uint64_t seed;             /* PCG32 seed: stream per (pixel, frame) */
+} ap_render_params;
+
+/* Caller-owned device framebuffers (GUI.hpp:232-274 owns them in the
+ * reference). hdr is required; every other pointer may be NULL. */
+typedef struct ap_framebuffers {
+    float* hdr;          /* W*H*3 running mean of linear XYZ (scene.cu:241-242) */
+    float* odd;          /* W*H*3 running mean of odd frames (scene.cu:243-253) */
+    float* even;         /* W*H*3 running mean of even frames */
+    double* sum_xyz;     /* W*H*3 FP64 per-pixel sums of the clamped XYZ samples (added) */
+    int32_t* primary_id; /* W*H closest-hit objectIndex_ of the camera ray of the last sample, -1 on miss */
+} ap_framebuffers;
+
+/* Reference defaults (config.cuh), spp = 1, frame 0, seed 0. */
+void ap_render_params_default(ap_render_params* params);
+
+/* Replaces SceneRenderer::Render(hdr, ldr, size, origin, dir, frameIndex, ...)'s
+ * RenderCamera dispatch (core/scene.cu:308-346, 257-274): adds params->spp
+ * frames (frame_index, frame_index+1, ...) to the framebuffers. Asynchronous
+ * on `stream`; concurrent renders of one scene race on its path buffers. */
+int ap_render(ap_scene* scene, const ap_camera* camera, const ap_render_params* params,
+              const ap_framebuffers* fb, void* stream);
+
+/* Replaces SceneRenderer::Render(size, origin, dir, sampleNum)
+ * (core/scene.cu:276-306): frames 0..spp-1 into a zero-initialised target,
+ * returned as host float[W*H*3] linear XYZ. Synchronous. */
+int ap_render_offline(ap_scene* scene, const ap_camera* camera, const ap_render_params* params,
+                      float* host_xyz);
+
+/* Replaces XYZ2SRGBLinearD65 (core/tonemap.cuh:116-126) over n device XYZ triples. */
+int ap_xyz_to_srgb(const float* xyz, size_t n, float* rgb, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APERTURE_AP_RENDER_H */

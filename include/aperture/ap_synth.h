@@ -37,6 +37,18 @@ int ap_synth_camera_rays(int width, int height, const float origin[3], const flo
 int ap_synth_secondary_rays(const ap_scene* scene, const ap_ray* primaries, const ap_hit* hits,
                             size_t n, uint64_t seed, float offset, ap_ray* out, size_t* n_out);
 
+/* Closed-box scene: five axis-aligned quads of [-1,1]^3 (floor, ceiling,
+ * back, left, right; the +z side open), two triangles each; a 0.5 x 0.5
+ * emissive quad under the ceiling as a thin OBJ_CUBE (centre (0, 0.99, 0),
+ * extent (0.25, 0.01, 0.25), radiance `light`, the NEE-capable light form of
+ * SceneHelper::CreateCornellBox, core/helper/scene_helper.h:14-72); two
+ * icospheres (subdivision `sphere_subdiv`, radius 0.3) at (-0.45,-0.7,-0.3) and
+ * (0.45,-0.7,0.35) with vertex normals. Diffuse albedo per mesh drawn
+ * U[0.1, 0.9] per channel from PCG32(seed). Writes up to `capacity` objects
+ * and the count to *n_out (10 + 1 + 2 * 20 * 4^subdiv). */
+int ap_synth_box_scene(uint64_t seed, int sphere_subdiv, float light, ap_scene_object* out, size_t capacity,
+                       size_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "aperture/ap_core.h"
+#include "aperture/ap_render.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -25,6 +26,12 @@ void or_trace(const or_bvh* h, const ap_ray* rays, size_t n, int any_hit, int cu
 void or_brute_closest(const ap_scene_object* objs, size_t count, const ap_ray* rays, size_t n,
                       int32_t* ids, float* ts, int threads);
 float or_object_t(const ap_scene_object* objs, int idx, const ap_ray* r);
+
+/* Integrator restatement (or_render.cpp): frames frame_index .. +spp-1 into
+ * hdr (running mean, in place), sums (added) and primary ids (last frame). */
+int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+              const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
+              int threads);
 
 /* Spectral restatement (or_spectrum.cpp). Tables come from the core's data files. */
 int or_spectrum_load(const char* mlp_path, const char* cie_path);

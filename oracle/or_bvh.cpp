@@ -416,3 +416,14 @@ float or_object_t(const ap_scene_object* objs, int idx, const ap_ray* r) {
 }
 
 }  // extern "C"
+
+namespace orc {
+// One traversal for the integrator restatement (or_render.cpp).
+void TraceOne(const or_bvh* h, float3 o, float3 d, float tMin, float tMax, bool anyHit, bool cull, float margin,
+              int* id, float* t) {
+    const TraceResult r =
+        TraverseBVH(anyHit, h->objs, h->bvh.nodes_, h->bvh.objectIndices_, o, d, tMin, tMax, cull, margin);
+    *id = r.id;
+    *t = r.t;
+}
+}  // namespace orc

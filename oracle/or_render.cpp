@@ -1,0 +1,768 @@
+// ORACLE — test infrastructure only: CPU restatement of the reference's
+// per-pixel integrator, the checker for the core's wavefront path tracer
+// (tests/ and bench.py's cpu_baseline leg; the product never links it).
+//
+//   RenderCamera (pinhole, running means)     core/scene.cu:201-255
+//   light table (area * luminance)             core/scene.cu:1056-1080
+//   Trace (closest hit, material, offset)       core/render/render.cuh:75-163
+//   NEE / SegmentTrace / PowerHeuristic         core/render/render.cuh:165-343
+//   CalculateRadiance (path loop)              core/render/render.cuh:460-999
+//   EvaluateMaterial / SampleBSDF / EvaluateBSDFWithCos / BSDF_PDF
+//                                              core/render/material.cuh:48-425
+//   brdf / sample helpers                      core/render/brdf.cuh, sample.cuh
+//   NEESampleLight, box / sphere samplers, NEEPDFOmegaFromHit
+//                                              core/nee/nee.cuh:24-377
+// Deviations shared with the core, each a parameter or a documented
+// restriction: PCG32 replaces curand (stream per pixel, start state hashed
+// from seed, frame and pixel with SplitMix64); rsqrtf is 1/sqrtf; triangles may carry their object's
+// material (param) instead of IntersectTriangleObject's default GBuffer;
+// glass, textures and path guiding are not restated; a path that misses while
+// "inside a medium" ends (the reference continues from an undefined point).
+// Parity pins: white-furnace and analytic-irradiance scenes
+// (tests/test_render_cpu.py), NEE pdf pairing and power-heuristic identities.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+#include <vector>
+
+#include "aperture/ap_render.h"
+#include "or_api.h"
+#include "or_math.h"
+#include "or_spectrum.h"
+
+namespace orc {
+
+// ---- PCG32 (O'Neill 2014, pcg32_random_r / pcg32_srandom_r) ----
+struct Rng {
+    uint64_t state, inc;
+    uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = static_cast<uint32_t>(old >> 59u);
+        return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    float rand1() { return static_cast<float>(next() >> 8) * 0x1p-24f; }
+    static Rng seeded(uint64_t initstate, uint64_t initseq) {
+        Rng r{0u, (initseq << 1u) | 1u};
+        r.next();
+        r.state += initstate;
+        r.next();
+        return r;
+    }
+};
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014).
+static inline uint64_t SplitMix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// Stream of one path sample: stream = pixel, start state hashed from
+// (seed, frame, pixel) so that consecutive frames are decorrelated.
+static inline Rng PathRng(uint64_t pixel, uint64_t frame, uint64_t seed) {
+    const uint64_t k = SplitMix64(seed ^ SplitMix64(frame));
+    return Rng::seeded(SplitMix64(k ^ pixel), pixel);
+}
+
+struct TraceResult2 { int id; float t; };
+
+// From or_bvh.cpp.
+extern bool IntersectTriangleT(float3 o, float3 d, const ap_scene_object& ob, float& t, float& uu, float& vv);
+extern bool IntersectAABBT(float3 o, float3 d, const ap_scene_object& ob, float& t, bool& inside);
+extern bool IntersectSphereT(float3 o, float3 d, const ap_scene_object& ob, float& t, bool& inside);
+
+inline float3 F3(const ap_float3& a) { return {a.x, a.y, a.z}; }
+inline float3x3 M3(const ap_float3x3& m) { return {F3(m.x), F3(m.y), F3(m.z)}; }
+inline float INTERSECT_EPS(int depth) { return 1.0e-3f * std::exp(std::fmin(static_cast<float>(depth) * 0.5f, 4.0f)); }
+
+// GBuffer subset the integrator reads.
+struct GB {
+    float3 albedo, emissive;
+    float specular, metallic, roughness;
+};
+
+struct Ctx {  // IntersectionContext subset
+    bool hit = false;
+    int objectIndex = -1, lightIndex = -1;
+    float distance = 0.0f;
+    float3 pos{0, 0, 0}, normal{0, 0, 0};
+    GB gb{};
+};
+
+struct Scene {
+    const ap_scene_object* objs;
+    size_t n;
+    const or_bvh* bvh;
+    std::vector<ap_light_info> lights;
+    std::vector<int> light_index;  // per object
+    int tri_mode;
+    float margin;                  // > 0: the core's CULLED traversal (same results, faster)
+};
+
+// The wavelength response cache of Interact: LUT cells computed on demand.
+struct Basis { float v[4]; };
+static thread_local std::unordered_map<uint64_t, Basis>* tl_cells = nullptr;
+
+static Basis Cell(int x, int y, int z) {
+    if (!tl_cells) tl_cells = new std::unordered_map<uint64_t, Basis>();
+    const uint64_t key = (static_cast<uint64_t>(z) << 20) | (static_cast<uint64_t>(y) << 10) | static_cast<uint64_t>(x);
+    auto it = tl_cells->find(key);
+    if (it != tl_cells->end()) return it->second;
+    Basis b;
+    const float inv = 1.0f / 512.0f;
+    const float in3[3] = {(x + 0.5f) * inv, (y + 0.5f) * inv, (z + 0.5f) * inv};
+    RGBEncode(in3, b.v);
+    tl_cells->emplace(key, b);
+    return b;
+}
+
+static void QueryRGBBasisCached(float3 rgb, float out[4]) {
+    const int R = 512;
+    float g[3] = {rgb.x * R - 0.5f, rgb.y * R - 0.5f, rgb.z * R - 0.5f};
+    int i0[3], i1[3];
+    float t[3];
+    for (int a = 0; a < 3; ++a) {
+        g[a] = std::fmin(std::fmax(g[a], 0.0f), R - 1.0f);
+        const float fi0 = std::floor(g[a]);
+        i0[a] = static_cast<int>(fi0);
+        i1[a] = i0[a] + 1;
+        if (i1[a] >= R) i1[a] = R - 1;
+        t[a] = g[a] - fi0;
+        if (i0[a] == i1[a]) t[a] = 0.0f;
+    }
+    const float wx0 = 1.0f - t[0], wx1 = t[0], wy0 = 1.0f - t[1], wy1 = t[1], wz0 = 1.0f - t[2], wz1 = t[2];
+    const float w[8] = {wx0 * wy0 * wz0, wx1 * wy0 * wz0, wx0 * wy1 * wz0, wx1 * wy1 * wz0,
+                        wx0 * wy0 * wz1, wx1 * wy0 * wz1, wx0 * wy1 * wz1, wx1 * wy1 * wz1};
+    const int cx[8] = {i0[0], i1[0], i0[0], i1[0], i0[0], i1[0], i0[0], i1[0]};
+    const int cy[8] = {i0[1], i0[1], i1[1], i1[1], i0[1], i0[1], i1[1], i1[1]};
+    const int cz[8] = {i0[2], i0[2], i0[2], i0[2], i1[2], i1[2], i1[2], i1[2]};
+    for (int k = 0; k < 4; ++k) out[k] = 0.0f;
+    for (int c = 0; c < 8; ++c) {
+        const Basis b = Cell(cx[c], cy[c], cz[c]);
+        for (int k = 0; k < 4; ++k) out[k] = std::fma(w[c], b.v[k], out[k]);
+    }
+}
+
+static float InteractC(const float lb[4], float3 rgb) {  // lut::Interact
+    const float lum = max3(rgb);
+    float rb[4];
+    if (lum > 1.0f) {
+        QueryRGBBasisCached(rgb / lum, rb);
+        return SpectralInteract(rb, lb) * lum;
+    }
+    QueryRGBBasisCached(rgb, rb);
+    return SpectralInteract(rb, lb);
+}
+
+// ---- sampling helpers (vector.cuh, sample.cuh) ----
+static void BuildTangentBasis(float3 n, float3& t, float3& b) {
+    const float3 a = (std::fabs(n.z) < 0.999f) ? float3{0, 0, 1} : float3{1, 0, 0};
+    t = normalize(cross(a, n));
+    b = cross(n, t);
+}
+static void BuildTangentBasisRandom(float3 n, float3& t, float3& b, float e) {
+    const float3 a = (std::fabs(n.z) < 0.999f) ? float3{0, 0, 1} : float3{1, 0, 0};
+    const float3 t0 = normalize(cross(a, n));
+    const float3 b0 = cross(n, t0);
+    const float s = std::sin(e * PI_F * 2.0f), c = std::cos(e * PI_F * 2.0f);
+    t = c * t0 + s * b0;
+    b = -s * t0 + c * b0;
+}
+static float2 ConcentricSampleDisk(float ux, float uy) {
+    const float sx = 2.0f * ux - 1.0f, sy = 2.0f * uy - 1.0f;
+    if (sx == 0.0f && sy == 0.0f) return {0.0f, 0.0f};
+    float r, phi;
+    if (std::fabs(sx) > std::fabs(sy)) { r = sx; phi = (PI_F / 4.0f) * (sy / std::fmax(std::fabs(sx), 1e-20f)); }
+    else { r = sy; phi = (PI_F / 2.0f) - (PI_F / 4.0f) * (sx / std::fmax(std::fabs(sy), 1e-20f)); }
+    return {r * std::cos(phi), r * std::sin(phi)};
+}
+struct Sample { float3 dir; float pdf; };
+static Sample SampleCosine(float3 e, float3 n) {
+    float3 T, B;
+    BuildTangentBasisRandom(n, T, B, e.z);
+    const float2 d = ConcentricSampleDisk(e.x, e.y);
+    const float z = std::sqrt(std::fmax(0.0f, 1.0f - d.x * d.x - d.y * d.y));
+    const float3 dir = d.x * T + d.y * B + z * n;
+    return {normalize(dir), z / PI_F};
+}
+static float GGX_D(float alpha, float NoH) {
+    const float a2 = alpha * alpha;
+    const float d = NoH * NoH * (a2 - 1.0f) + 1.0f;
+    return a2 / std::fmax(PI_F * d * d, 1e-12f);
+}
+static Sample SampleGGXBoundedVNDF(float3 V, float3 N, float alpha, float3 rnd) {
+    float3 X, Y;
+    BuildTangentBasisRandom(N, X, Y, rnd.z);
+    const float XoV = dot(X, V), YoV = dot(Y, V), NoV = dot(N, V);
+    const float3 lv{XoV, YoV, NoV};
+    const float NoV2 = std::fmin(1.0f, NoV * NoV);
+    const float a2 = alpha * alpha;
+    const float3 vs = float3{XoV * alpha, YoV * alpha, NoV} / std::sqrt(a2 + (1.0f - a2) * NoV2);
+    const float phi = 2.0f * PI_F * rnd.x;
+    const float s = 1.0f + std::sqrt(1.0f - NoV2);
+    const float s2 = s * s;
+    const float k = (1.0f - a2) * s2 / (s2 + a2 * NoV2);
+    const float b = NoV > 0 ? k * vs.z : vs.z;
+    const float z = std::fma(1.0f - rnd.y, 1.0f + b, -b);
+    const float sinT = std::sqrt(saturate_(1.0f - z * z));
+    const float3 os{sinT * std::cos(phi), sinT * std::sin(phi), z};
+    const float3 ms = vs + os;
+    const float3 H = normalize(float3{ms.x * alpha, ms.y * alpha, ms.z});
+    const float3 lL = reflect(-lv, H);
+    const float3 dir = lL.x * X + lL.y * Y + lL.z * N;
+    const float len2 = (1.0f - NoV2) * a2;
+    const float t = std::sqrt(len2 + NoV2);
+    return {dir, GGX_D(alpha, H.z) / (2.0f * (k * NoV + t))};
+}
+static float GGXBoundedVNDF_PDF(float3 V, float3 N, float3 L, float alpha) {
+    const float NoV = dot(N, V), NoL = dot(N, L);
+    if (NoV <= 0.0f || NoL <= 0.0f) return 0.0f;
+    const float3 H = normalize(V + L);
+    const float NoV2 = std::fmin(1.0f, NoV * NoV);
+    const float a2 = alpha * alpha;
+    const float s = 1.0f + std::sqrt(1.0f - NoV2);
+    const float s2 = s * s;
+    const float k = (1.0f - a2) * s2 / (s2 + a2 * NoV2);
+    const float NoH = std::fmax(0.0f, dot(N, H));
+    const float len2 = (1.0f - NoV2) * a2;
+    const float t = std::sqrt(len2 + NoV2);
+    return std::fmax(GGX_D(alpha, NoH) / (2.0f * (k * NoV + t)), 1e-12f);
+}
+static float FresnelSchlick(float F0, float cosX) {
+    const float c = clampf(1.0f - cosX, 0.0f, 1.0f);
+    const float c5 = c * c * c * c * c;
+    return F0 + (1.0f - F0) * c5;
+}
+static float GGX_G_Correlated(float alpha, float NoL, float NoV) {
+    const float a2 = alpha * alpha;
+    const float gv = NoL * std::sqrt(a2 + (1.0f - a2) * NoV * NoV);
+    const float gl = NoV * std::sqrt(a2 + (1.0f - a2) * NoL * NoL);
+    const float denom = gv + gl;
+    return (denom > 0.0f) ? (2.0f * NoL * NoV) / denom : 0.0f;
+}
+static float F0_UEStyle(float s) { return 0.08f * clampf(s, 0.0f, 1.0f); }
+
+// ---- materials (material.cuh) ----
+static float EvaluateBSDFWithCos(const float lb[4], int model, float3 rayDir, float3 L, const Ctx& c) {
+    const float3 N = c.normal;
+    const float NoL = std::fmax(0.0f, dot(N, L));
+    if (model == AP_MAT_DIFFUSE) return InteractC(lb, c.gb.albedo) * (NoL / PI_F);
+    if (model == AP_MAT_GGX) {
+        const float3 V = normalize(-rayDir);
+        const float NoV = std::fmax(0.0f, dot(N, V));
+        if (NoV <= 0.0f || NoL <= 0.0f) return 0.0f;
+        const float alpha = c.gb.roughness * c.gb.roughness;
+        const float al = InteractC(lb, c.gb.albedo);
+        const float F0 = lerp(F0_UEStyle(c.gb.specular), al, c.gb.metallic);
+        const float kD = 1.0f - c.gb.metallic;
+        const float3 Hn = normalize(V + L);
+        const float NoH = std::fmax(0.0f, dot(N, Hn));
+        const float LoH = std::fmax(0.0f, dot(L, Hn));
+        const float Fr = FresnelSchlick(F0, LoH);
+        const float D = GGX_D(alpha, NoH);
+        const float G = GGX_G_Correlated(alpha, NoL, NoV);
+        const float spec = (NoL > 0.0f) ? (Fr * D * G) / std::fmax(4.0f * NoV, 1e-7f) : 0.0f;
+        const float diff = kD * (1.0f - Fr) * al * (NoL / PI_F);
+        return spec + diff;
+    }
+    return 0.0f;
+}
+
+static float BSDF_PDF(const float lb[4], int model, const GB& gb, float3 N, float3 rayDir, float3 L) {
+    const float3 V = -rayDir;
+    const float NoL = std::fmax(0.0f, dot(N, L));
+    if (model == AP_MAT_DIFFUSE) return NoL > 0.0f ? (NoL * (1.0f / PI_F)) : 0.0f;
+    if (model == AP_MAT_GGX) {
+        const float NoV = std::fmax(0.0f, dot(N, V));
+        if (NoV <= 0.0f || NoL <= 0.0f) return 0.0f;
+        const float alpha = gb.roughness * gb.roughness;
+        const float F0 = lerp(F0_UEStyle(gb.specular), InteractC(lb, gb.albedo), gb.metallic);
+        const float pSpec = clampf(F0, 0.1f, 0.9f);
+        const float pdfMix = pSpec * GGXBoundedVNDF_PDF(V, N, L, alpha) + (1.0f - pSpec) * (NoL * (1.0f / PI_F));
+        return std::fmax(0.0f, pdfMix);
+    }
+    return 0.0f;
+}
+
+struct SampleCtx {
+    float3 next{0, 0, 0};
+    float pdf = 1.0f;
+    float3 emissive{0, 0, 0};
+    float brdf = 0.0f;
+    bool terminate = false, pureLight = false, delta = false;
+    bool inside = false;
+};
+
+static SampleCtx SampleBSDF(const float lb[4], Rng& rng, int model, float3 rayDir, const Ctx& c, bool inside) {
+    SampleCtx r;
+    const float3 N = c.normal;
+    const float3 V = -rayDir;
+    if (model == AP_MAT_DIFFUSE) {
+        const float e0 = rng.rand1(), e1 = rng.rand1(), e2 = rng.rand1();
+        const Sample s = SampleCosine({e0, e1, e2}, N);
+        r.next = s.dir; r.pdf = s.pdf; r.emissive = c.gb.emissive; r.inside = inside;
+        return r;
+    }
+    if (model == AP_MAT_LIGHT) {
+        r.pdf = 1.0f; r.emissive = c.gb.emissive; r.terminate = true; r.pureLight = true;
+        return r;
+    }
+    // GGX
+    r.emissive = c.gb.emissive;
+    const float NoV = std::fmax(0.0f, dot(N, V));
+    if (NoV <= 0.0f) { r.terminate = true; return r; }
+    const float alpha = c.gb.roughness * c.gb.roughness;
+    const float F0 = lerp(F0_UEStyle(c.gb.specular), InteractC(lb, c.gb.albedo), c.gb.metallic);
+    const float pSpec = clampf(F0, 0.1f, 0.9f);
+    const float u = rng.rand1();
+    if (u < pSpec) {
+        const float e0 = rng.rand1(), e1 = rng.rand1(), e2 = rng.rand1();
+        const Sample s = SampleGGXBoundedVNDF(V, N, alpha, {e0, e1, e2});
+        const float NoL = std::fmax(0.0f, dot(N, s.dir));
+        if (NoL <= 0.0f) { r.terminate = true; return r; }
+        r.next = s.dir;
+        r.pdf = pSpec * s.pdf + (1.0f - pSpec) * (NoL / PI_F);
+    } else {
+        const float e0 = rng.rand1(), e1 = rng.rand1(), e2 = rng.rand1();
+        const Sample s = SampleCosine({e0, e1, e2}, N);
+        r.next = s.dir;
+        r.pdf = pSpec * GGXBoundedVNDF_PDF(V, N, s.dir, alpha) + (1.0f - pSpec) * s.pdf;
+    }
+    r.inside = inside;
+    return r;
+}
+
+// ---- intersection context (intersection.cuh, aabb.cuh, sphere.cuh) ----
+static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t) {
+    Ctx c;
+    const ap_scene_object& ob = S.objs[idx];
+    c.hit = true;
+    c.objectIndex = idx;
+    c.lightIndex = S.light_index[idx];
+    c.distance = t;
+    const ap_gbuffer& g = ob.material.texture.base;
+    c.gb = {F3(g.albedo), F3(g.emissive), g.specular, g.metallic, g.roughness};
+    if (ob.type == AP_OBJ_TRIANGLE) {
+        float tt, u, v;
+        IntersectTriangleT(o, d, ob, tt, u, v);
+        const float w = 1.0f - u - v;
+        const float3x3 o2w = M3(ob.object_to_world);
+        const float3 n0 = o2w * F3(ob.additional.v0.normal);
+        const float3 n1 = o2w * F3(ob.additional.v1.normal);
+        const float3 n2 = o2w * F3(ob.additional.v2.normal);
+        const float3 n = normalize(n0 * w + n1 * u + n2 * v);
+        c.pos = o + d * t;
+        c.normal = dot(d, n) > 0.0f ? -n : n;
+        if (S.tri_mode == AP_TRIANGLE_MATERIAL_REFERENCE) c.gb = {{1, 1, 1}, {0, 0, 0}, 0.0f, 0.0f, 0.0f};
+        return c;
+    }
+    const float3x3 w2o = M3(ob.world_to_object), o2w = M3(ob.object_to_world);
+    const float3 lo = w2o * (o - F3(ob.center));
+    const float3 ld = normalize(w2o * d);
+    const float3 ext = F3(ob.extent);
+    if (ob.type == AP_OBJ_CUBE) {
+        const float3 div = invSafe(ld);
+        const float3 t1 = (-ext - lo) * div, t2 = (ext - lo) * div;
+        const float3 mn = fmin3(t1, t2), mx = fmax3(t1, t2);
+        const float tE = std::fmax(std::fmax(mn.x, mn.y), mn.z);
+        const float tX = std::fmin(std::fmin(mx.x, mx.y), mx.z);
+        const bool inside = tE < 0.0f;
+        const float tHit = inside ? tX : tE;
+        const float3 lp = lo + ld * tHit;
+        const float rx = std::fabs(lp.x) / ext.x, ry = std::fabs(lp.y) / ext.y, rz = std::fabs(lp.z) / ext.z;
+        float3 ln;
+        if (rx >= ry && rx >= rz) ln = {(lp.x > 0.0f) ? 1.0f : -1.0f, 0.0f, 0.0f};
+        else if (ry >= rz) ln = {0.0f, (lp.y > 0.0f) ? 1.0f : -1.0f, 0.0f};
+        else ln = {0.0f, 0.0f, (lp.z > 0.0f) ? 1.0f : -1.0f};
+        if (inside) ln = -ln;
+        c.pos = o2w * lp + F3(ob.center);
+        c.normal = normalize(o2w * ln);
+        return c;
+    }
+    const float r = max3(ext);
+    const float b = dot(lo, ld);
+    const float cc = dot(lo, lo) - r * r;
+    const float s = std::sqrt(b * b - 1.0f * cc);
+    const bool inside = (-b - s) <= 0.0f;
+    const float3 lp = lo + ld * t;
+    const float3 ln = normalize(lp) * (inside ? -1.0f : 1.0f);
+    c.pos = o2w * lp + F3(ob.center);
+    c.normal = normalize(o2w * ln);
+    return c;
+}
+
+extern void TraceOne(const or_bvh* h, float3 o, float3 d, float tMin, float tMax, bool anyHit, bool cull,
+                     float margin, int* id, float* t);
+
+// ---- NEE (nee.cuh) ----
+struct LightSample { float3 point, normal; float pFacet, pA; int object; bool valid; };
+
+static void OBBAxes(const ap_scene_object& o, float3& ux, float3& uy, float3& uz) {
+    const ap_float3x3& M = o.object_to_world;
+    ux = normalize(float3{M.x.x, M.y.x, M.z.x});
+    uy = normalize(float3{M.x.y, M.y.y, M.z.y});
+    uz = normalize(float3{M.x.z, M.y.z, M.z.z});
+}
+
+static float AreaToSolidAnglePDF(float3 sp, float3 lp, float3 ln, float pa) {
+    const float3 sl = lp - sp;
+    const float d2 = std::fmax(dot(sl, sl), 1e-20f);
+    const float3 dir = sl * (1.0f / std::sqrt(d2));
+    const float cosY = std::fmax(0.0f, dot(ln, -dir));
+    if (cosY <= 0.0f) return 0.0f;
+    return pa * (d2 / cosY);
+}
+
+static void Facets(const ap_scene_object& b, float3 n[6], float3 c[6], float3 tg[6], float3 bt[6], float s1[6], float s2[6]) {
+    float3 uX, uY, uZ;
+    OBBAxes(b, uX, uY, uZ);
+    const float3 C = F3(b.center), E = F3(b.extent);
+    const float3 nn[6] = {uX, -uX, uY, -uY, uZ, -uZ};
+    const float3 cc[6] = {C + uX * E.x, C - uX * E.x, C + uY * E.y, C - uY * E.y, C + uZ * E.z, C - uZ * E.z};
+    const float3 tt[6] = {uY, uY, uX, uX, uX, uX};
+    const float3 bb[6] = {uZ, uZ, uZ, uZ, uY, uY};
+    const float a1[6] = {E.y, E.y, E.x, E.x, E.x, E.x};
+    const float a2[6] = {E.z, E.z, E.z, E.z, E.y, E.y};
+    for (int f = 0; f < 6; ++f) { n[f] = nn[f]; c[f] = cc[f]; tg[f] = tt[f]; bt[f] = bb[f]; s1[f] = a1[f]; s2[f] = a2[f]; }
+}
+
+static float FacingWeight(float3 fn, float3 fc, float3 p, float area) {
+    return area * std::fmax(0.0f, dot(fn, normalize(p - fc)));
+}
+
+static float NEEPDFOmegaFromHit(const ap_scene_object& L, const ap_light_info& li, float3 sp, float3 lp, float3 ln) {
+    if (L.type == AP_OBJ_CUBE) {
+        float3 n[6], c[6], tg[6], bt[6];
+        float s1[6], s2[6], area[6];
+        Facets(L, n, c, tg, bt, s1, s2);
+        for (int f = 0; f < 6; ++f) area[f] = 4.0f * s1[f] * s2[f];
+        int hitF = -1;
+        for (int f = 0; f < 6; ++f)
+            if (dot(n[f], ln) > 0.999f) { hitF = f; break; }
+        if (hitF < 0) {
+            float best = 1e30f;
+            for (int f = 0; f < 6; ++f) {
+                const float d = std::fabs(dot(n[f], lp - c[f]));
+                if (d < best) { best = d; hitF = f; }
+            }
+        }
+        float sum = 0.0f, wh = 0.0f, ah = 0.0f;
+        for (int f = 0; f < 6; ++f) {
+            const float w = FacingWeight(n[f], c[f], sp, area[f]);
+            sum += w;
+            if (f == hitF) { wh = w; ah = area[f]; }
+        }
+        if (sum <= 0.0f || wh <= 0.0f) return 0.0f;
+        return AreaToSolidAnglePDF(sp, lp, ln, li.select_prob * (1.0f / ah) * (wh / sum));
+    }
+    if (L.type == AP_OBJ_SPHERE) {
+        const float3 sc = F3(L.center);
+        const float r = max3(F3(L.extent));
+        const float dist = length(sp - sc);
+        if (dist <= r) return 0.0f;
+        const float cosC = clampf(r / dist, 0.0f, 1.0f);
+        const float cap = 2.0f * PI_F * r * r * (1.0f - cosC);
+        if (cap <= 1e-20f) return 0.0f;
+        return AreaToSolidAnglePDF(sp, lp, ln, li.select_prob / cap);
+    }
+    return 0.0f;
+}
+
+static LightSample NEESampleLight(const Scene& S, float3 p, Rng& rng) {
+    LightSample ls{};
+    ls.valid = false;
+    const int n = static_cast<int>(S.lights.size());
+    if (n <= 0) return ls;
+    int pick = n - 1;
+    float pprob = 0.0f;
+    {
+        const float r = rng.rand1();
+        float acc = 0.0f;
+        for (int k = 0; k < n; ++k) {
+            acc += S.lights[k].select_prob;
+            if (r < acc) { pick = k; pprob = S.lights[k].select_prob; break; }
+        }
+    }
+    const ap_scene_object& L = S.objs[S.lights[pick].object_index];
+    if (L.type == AP_OBJ_CUBE) {
+        float3 fn[6], fc[6], tg[6], bt[6];
+        float s1[6], s2[6], area[6];
+        Facets(L, fn, fc, tg, bt, s1, s2);
+        for (int f = 0; f < 6; ++f) area[f] = 4.0f * s1[f] * s2[f];
+        int vi[6]; float vw[6]; int vc = 0; float vs = 0.0f;
+        for (int f = 0; f < 6; ++f) {
+            const float w = FacingWeight(fn[f], fc[f], p, area[f]);
+            if (w > 0.0f) { vi[vc] = f; vw[vc] = w; vs += w; ++vc; }
+        }
+        if (vc == 0) return ls;
+        int slot = 0, face = vi[0];
+        const float r = rng.rand1() * vs;
+        float acc = 0.0f;
+        for (int k = 0; k < vc; ++k) {
+            acc += vw[k];
+            if (r <= acc) { slot = k; face = vi[k]; break; }
+        }
+        const float u1 = 2.0f * rng.rand1() - 1.0f;
+        const float u2 = 2.0f * rng.rand1() - 1.0f;
+        ls.point = fc[face] + tg[face] * (u1 * s1[face]) + bt[face] * (u2 * s2[face]);
+        ls.normal = fn[face];
+        ls.pFacet = 1.0f * (vw[slot] / vs);
+        ls.pA = 1.0f / area[face];
+        ls.valid = true;
+    } else if (L.type == AP_OBJ_SPHERE) {
+        const float3 sc = F3(L.center);
+        const float r = max3(F3(L.extent));
+        const float3 pc = p - sc;
+        const float dist = length(pc);
+        if (dist <= r) return ls;
+        const float3 axis = pc / dist;
+        const float cosC = clampf(r / dist, 0.0f, 1.0f);
+        const float cap = 2.0f * PI_F * r * r * (1.0f - cosC);
+        const float u = rng.rand1();
+        const float vphi = rng.rand1();
+        const float cosT = lerp(cosC, 1.0f, u);
+        const float sinT = std::sqrt(std::fmax(0.0f, 1.0f - cosT * cosT));
+        const float phi = 2.0f * PI_F * vphi;
+        float3 t, b;
+        BuildTangentBasis(axis, t, b);
+        const float3 nn = t * (std::cos(phi) * sinT) + b * (std::sin(phi) * sinT) + axis * cosT;
+        ls.point = sc + nn * r;
+        ls.normal = nn;
+        ls.pFacet = 1.0f;
+        ls.pA = 1.0f / cap;
+        ls.valid = true;
+    } else {
+        return ls;
+    }
+    ls.object = S.lights[pick].object_index;
+    ls.pFacet = ls.pFacet * pprob;
+    return ls;
+}
+
+static float PowerHeuristic(float pA, float pB) {
+    const float a = pA * pA, b = pB * pB;
+    return a / std::fmax(1e-24f, a + b);
+}
+
+static float3 SkyLighting(float3 d) {
+    const float3 white{1, 1, 1}, blue{0.05f, 0.45f, 1.0f};
+    const float3 down = float3{0.2f, 0.6f, 1.0f} * 0.125f;
+    return 0.5f * lerp(lerp(blue, white, std::pow(std::fmax(d.y, 0.0f), 0.75f)), down,
+                       std::fmin(std::fmax(-d.y * 0.5f + 0.5f, 0.0f), 1.0f));
+}
+
+struct RenderOpts {
+    int max_depth, rr_min_depth, nee_max_depth, use_nee, nee_always, sky;
+};
+
+// CalculateRadiance (render.cuh:460-999) for one camera ray; returns the
+// clamped XYZ sample and the primary hit id.
+static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir, bool inside0,
+                                int* primary) {
+    float lam, lpdf;
+    {
+        const float r0 = rng.rand1(), r1 = rng.rand1(), r2 = rng.rand1();
+        SampleCIE1931(r0, r1, r2, &lam, &lpdf);
+    }
+    float lb[4];
+    QueryLambda(lam, lb);
+    const float d65 = D65Norm(lam);
+    const float3 xyz = LambdaToXYZ(lam);
+    int ndd = 0;
+    dir = normalize(dir);
+    float radiance = 0.0f, mask = 1.0f;
+    float3 ro = origin, rd = dir;
+    bool inside = inside0;
+    bool prevNEE = false;
+    float lastPdf = 0.0f;
+    float3 lastPos{0, 0, 0};
+    for (int depth = 0; depth < P.max_depth; ++depth) {
+        // Trace (render.cuh:75-163).
+        const float eps = INTERSECT_EPS(ndd);
+        int id;
+        float t;
+        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t);
+        if (depth == 0 && primary) *primary = id;
+        if (id < 0) {
+            (void)rng.rand1();  // offset draw
+            if (!inside) radiance = radiance + mask * (P.sky ? InteractC(lb, SkyLighting(rd)) : 0.0f) * d65;
+            break;
+        }
+        Ctx c = HitContext(S, id, ro, rd, t);
+        const int model = S.objs[id].material.shading_model;
+        SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
+        sc.brdf = EvaluateBSDFWithCos(lb, model, rd, sc.next, c);
+        const float offEps = eps * std::fmax(1.0f, length(c.pos));
+        c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
+        inside = sc.inside;
+        if (c.lightIndex >= 0) {
+            float misW = 1.0f;
+            if (P.use_nee && depth > 0 && prevNEE) {
+                const float pB = std::fmax(lastPdf, 0.0f);
+                const float pL = NEEPDFOmegaFromHit(S.objs[id], S.lights[c.lightIndex], lastPos, c.pos, c.normal);
+                if (pL >= 0.0f && pB > 0.0f) misW = PowerHeuristic(pB, pL);
+            }
+            radiance = radiance + mask * InteractC(lb, sc.emissive) * d65 * misW;
+        }
+        if (sc.terminate) break;
+        bool neeValid = false;
+        const float rate = P.nee_always ? 1.0f : std::sqrt(saturate_(mask));
+        if (P.use_nee && ndd < P.nee_max_depth && !sc.delta && !inside && rng.rand1() < rate) {
+            const LightSample ls = NEESampleLight(S, c.pos, rng);
+            if (ls.valid) {
+                const float pAT = std::fmax(1e-20f, ls.pA * ls.pFacet);
+                const float3 lu = ls.point - c.pos;
+                const float d2 = std::fmax(dot(lu, lu), static_cast<float>(1.0e-12));
+                const float3 L = lu * (1.0f / std::sqrt(d2));
+                // SegmentTrace -> AnyHitSegmentBVH (intersection.cuh:372-397).
+                bool occluded = false;
+                {
+                    float3 sd = ls.point - c.pos;
+                    const float dist = length(sd);
+                    if (dist > 0.0f) {
+                        sd = sd / dist;
+                        const float seps = INTERSECT_EPS(depth) * std::fmax(1.0f, std::fmax(length(c.pos), length(ls.point)));
+                        const float tMax = std::fmax(0.0f, dist - seps);
+                        if (tMax > 0.0f) {
+                            int sid;
+                            float st;
+                            TraceOne(S.bvh, c.pos, sd, 0.0f, tMax, true, S.margin > 0.0f, S.margin, &sid, &st);
+                            occluded = sid >= 0;
+                        }
+                    }
+                }
+                if (occluded) {
+                    neeValid = true;
+                } else {
+                    const float cosX = std::fmax(0.0f, dot(c.normal, L));
+                    const float cosY = std::fmax(0.0f, dot(ls.normal, -L));
+                    if (!(cosX <= 0.0f || cosY <= 0.0f)) {
+                        const float bc = EvaluateBSDFWithCos(lb, model, rd, L, c);
+                        const float G = cosY / d2;
+                        const ap_gbuffer& lg = S.objs[ls.object].material.texture.base;
+                        const float resp = InteractC(lb, F3(lg.emissive));
+                        const float contribution = bc * resp * (G / pAT);
+                        neeValid = true;
+                        const float pLo = AreaToSolidAnglePDF(c.pos, ls.point, ls.normal, ls.pFacet * ls.pA);
+                        const float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
+                        radiance = radiance + mask * (contribution * PowerHeuristic(pLo, pBs)) * d65;
+                    }
+                }
+            }
+        }
+        prevNEE = neeValid;
+        lastPdf = sc.pdf;
+        lastPos = c.pos;
+        const float contrib = sc.brdf / std::fmax(sc.pdf, 1e-12f);
+        float p = 1.0f;
+        if (ndd >= P.rr_min_depth) {
+            const float basic = clampf(mask * 0.5f + mask * 0.5f, 0.05f, 0.95f);
+            p = lerp(basic, 1.0f, inside ? 0.99f : 0.0f);
+            if (rng.rand1() > p) break;
+        }
+        mask = mask * contrib;
+        mask = mask / p;
+        ro = c.pos;
+        rd = sc.next;
+        if (!sc.delta) ndd++;
+    }
+    float r = radiance;
+    if (std::isnan(r) || std::isinf(r)) r = 0.0f;
+    const float3 pre = (xyz * r) / lpdf;
+    return {clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
+}
+
+static float GetArea(const ap_scene_object& o) {
+    if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SDF) {
+        const float ex = std::fabs(o.extent.x), ey = std::fabs(o.extent.y), ez = std::fabs(o.extent.z);
+        return 8.0f * (ex * ey + ex * ez + ey * ez);
+    }
+    if (o.type == AP_OBJ_SPHERE) { const float r = std::fabs(o.extent.x); return 4.0f * PI_F * r * r; }
+    if (o.type == AP_OBJ_TRIANGLE) {
+        const float3 e1 = F3(o.additional.v1.position) - F3(o.additional.v0.position);
+        const float3 e2 = F3(o.additional.v2.position) - F3(o.additional.v0.position);
+        return 0.5f * length(cross(e1, e2));
+    }
+    return 0.0f;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// Offline render (frames frame0 .. frame0+spp-1) of the scene into
+// hdr (W*H*3 running mean, updated in place), sums (W*H*3 doubles, added)
+// and primary ids (W*H, last frame). cull_margin > 0 uses the core's CULLED
+// traversal order, which returns the same hits.
+int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+              const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
+              int threads) {
+    Scene S{objs, n, bvh, {}, std::vector<int>(n, -1), prm->triangle_material, cull_margin};
+    float total = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        if (objs[i].material.shading_model == AP_MAT_LIGHT) {
+            S.light_index[i] = static_cast<int>(S.lights.size());
+            const float3 em = F3(objs[i].material.texture.base.emissive);
+            ap_light_info li{static_cast<int32_t>(i),
+                             std::fmax(GetArea(objs[i]) * dot(em, float3{0.2126f, 0.7152f, 0.0722f}), 1.0e-20f), 0.0f};
+            total += li.importance_weight;
+            S.lights.push_back(li);
+        }
+    }
+    for (auto& li : S.lights) li.select_prob = li.importance_weight / total;
+    const RenderOpts P{prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee, prm->nee_always, prm->sky};
+    const float3 camO{cam->origin[0], cam->origin[1], cam->origin[2]};
+    const float3 camD = normalize(float3{cam->direction[0], cam->direction[1], cam->direction[2]});
+    const float FOV = cam->fov_deg * 3.1415926f / 180.0f;
+    const float aspect = static_cast<float>(cam->width) / std::fmax(1.0f, static_cast<float>(cam->height));
+    const float tanY = std::tan(0.5f * FOV), tanX = tanY * aspect;
+    const float3 right = normalize(cross(camD, float3{0.0f, 1.0f, 0.0f}));
+    const float3 up = normalize(cross(camD, right));
+    bool inside0 = false;
+    for (size_t i = 0; i < n && !inside0; ++i) {
+        const ap_scene_object& ob = objs[i];
+        if (ob.type != AP_OBJ_CUBE && ob.type != AP_OBJ_SPHERE) continue;
+        const float3 lo = M3(ob.world_to_object) * (camO - F3(ob.center));
+        const float3 e = F3(ob.extent);
+        if (ob.type == AP_OBJ_CUBE)
+            inside0 = -e.x < lo.x && -e.y < lo.y && -e.z < lo.z && e.x > lo.x && e.y > lo.y && e.z > lo.z;
+        else
+            inside0 = dot(lo, lo) < max3(e) * max3(e);
+    }
+    const int W = cam->width, H = cam->height;
+    const int64_t npix = static_cast<int64_t>(W) * H;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+    for (int64_t p = 0; p < npix; ++p) {
+        const int i = static_cast<int>(p % W), j = static_cast<int>(p / W);
+        float3 acc{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
+        for (int s = 0; s < prm->spp; ++s) {
+            const int frame = prm->frame_index + s;
+            Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
+            const float u = (i + rng.rand1()) / static_cast<float>(W);
+            const float v = (j + rng.rand1()) / static_cast<float>(H);
+            const float nx = u * 2.f - 1.f;
+            const float ny = static_cast<float>((1.0 - v) * 2.f - 1.f);
+            const float3 rd = normalize(camD + right * (nx * tanX) + up * (ny * tanY));
+            int prim = -1;
+            float3 res = CalculateRadiance(S, P, rng, camO, rd, inside0, &prim);
+            res = fmax3(float3{0, 0, 0}, res);
+            acc = lerp(acc, res, 1.0f / static_cast<float>(1 + frame));
+            if (sums) {
+                sums[3 * p] += static_cast<double>(res.x);
+                sums[3 * p + 1] += static_cast<double>(res.y);
+                sums[3 * p + 2] += static_cast<double>(res.z);
+            }
+            if (primary && s == prm->spp - 1) primary[p] = prim;
+        }
+        hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -44,6 +44,8 @@ def lib():
         L.or_d65_norm.argtypes = [f32]
         L.or_lambda_to_xyz.argtypes = [f32, vp]
         L.or_xyz_to_srgb.argtypes = [vp, vp]
+        L.or_render.restype = i32
+        L.or_render.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, i32]
         data = os.path.join(REPO, "aperturerendererv3-optix-geometry_amd", "data")
         rc = L.or_spectrum_load(os.path.join(data, "spectrum_mlp.bin").encode(),
                                 os.path.join(data, "cie_tables.bin").encode())
@@ -97,6 +99,18 @@ class OracleBVH:
         lib().or_trace(self.h, _p(rays), n, 1 if any_hit else 0, 1 if cull else 0, float(cull_margin or 0.0),
                        _p(ids), _p(ts), _p(nodes), _p(prims), threads or os.cpu_count())
         return (ids, ts, nodes, prims) if stats else (ids, ts)
+
+    def render(self, camera, params, cull_margin=None, threads=None, hdr=None):
+        """Reference integrator (or_render.cpp): (hdr running mean, FP64 sums, primary ids)."""
+        import ctypes as C
+
+        w, h = camera.width, camera.height
+        hdr = np.zeros((h, w, 3), np.float32) if hdr is None else np.ascontiguousarray(hdr, np.float32)
+        sums = np.zeros((h, w, 3), np.float64)
+        prim = np.full((h, w), -1, np.int32)
+        lib().or_render(self.h, _p(self.objs), len(self.objs), C.addressof(camera), C.addressof(params),
+                        float(cull_margin or 0.0), _p(hdr), _p(sums), _p(prim), threads or os.cpu_count())
+        return hdr, sums, prim
 
     def __del__(self):
         try:

@@ -39,6 +39,26 @@ def test_layout_sizes():
     assert ap.SCENE_OBJECT_DTYPE.fields["v0_position"][1] == 288
 
 
+def test_scene_object_offsets_match_the_c_layout():
+    """Offsets of ap_layout.h (static_asserted there) seen through the dtype."""
+    f = ap.SCENE_OBJECT_DTYPE.fields
+    want = {"center": 0, "extent": 12, "shading_model": 32, "albedo": 64, "emissive": 76, "specular": 88,
+            "metallic": 92, "roughness": 96, "ior": 100, "sigma_a": 104, "normal": 108, "abbe_number": 120,
+            "tilt_number": 124, "textures": 128, "world_to_object": 160, "object_to_world": 196, "type": 232,
+            "sdf_type": 240, "sdf_info0": 256, "cached_volume_index": 272, "v0_position": 288, "v0_normal": 300,
+            "v0_uv": 312, "v1_position": 320, "v2_position": 352, "proxy_min": 384, "proxy_max": 396,
+            "proxy_center": 408, "object_index": 420, "light_index": 424, "visible": 428}
+    for k, off in want.items():
+        assert f[k][1] == off, k
+    # a scene written by the C generator reads back through the dtype
+    box = ap.synth_box_scene(1, 0, 15.0)
+    light = box[box["shading_model"] == ap.MAT_LIGHT]
+    assert len(light) == 1 and np.allclose(light["emissive"], 15.0) and np.allclose(light["albedo"], 0.0)
+    walls = box[box["type"] == ap.OBJ_TRIANGLE]
+    assert ((walls["albedo"] >= 0.1) & (walls["albedo"] <= 0.9)).all()
+    assert np.allclose(walls["ior"], 1.5) and np.allclose(walls["abbe_number"], 20.0)
+
+
 def test_errors_are_codes_not_aborts():
     lib = ap.lib()
     assert lib.ap_scene_create(None, 0, None, None) == -1
