@@ -26,6 +26,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libaperture_hip.so")
 
 AP_OK = 0
+ERR_INVALID_ARG, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_UNSUPPORTED, ERR_NO_DEVICE = -1, -2, -3, -4, -5
 
 # ---------------------------------------------------------------------------
 # Layouts (include/aperture/ap_layout.h)

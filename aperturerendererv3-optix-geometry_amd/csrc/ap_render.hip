@@ -336,7 +336,7 @@ __device__ LightSample sample_light(const LightRec* __restrict__ lights, int n, 
         const float phi = 2.0f * kPiF * vphi;
         f3 t, b;
         tangent_basis(axis, t, b);
-        const f3 nn = t * (cosf(phi) * sinT) + b * (sinf(phi) * sinT) + axis * cosT;
+        const f3 nn = t * (apfm::cos(phi) * sinT) + b * (apfm::sin(phi) * sinT) + axis * cosT;
         ls.point = sc + nn * r;
         ls.normal = nn;
         ls.p_facet = 1.0f;

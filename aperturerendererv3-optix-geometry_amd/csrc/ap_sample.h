@@ -8,6 +8,7 @@
 // reproduced.
 #pragma once
 
+#include "ap_fmath.h"
 #include "ap_vec.h"
 
 namespace ap {
@@ -28,7 +29,7 @@ AP_HD void tangent_basis_random(f3 n, f3& t, f3& b, float e) {
     const f3 t0 = normalize(cross(a, n));
     const f3 b0 = cross(n, t0);
     const float ang = e * kPiF * 2.0f;
-    const float s = sinf(ang), c = cosf(ang);
+    const float s = apfm::sin(ang), c = apfm::cos(ang);
     t = c * t0 + s * b0;
     b = -s * t0 + c * b0;
 }
@@ -45,7 +46,7 @@ AP_HD f2 concentric_disk(float ux, float uy) {
         r = sy;
         phi = (kPiF / 2.0f) - (kPiF / 4.0f) * (sx / fmaxf(fabsf(sy), 1e-20f));
     }
-    return {r * cosf(phi), r * sinf(phi)};
+    return {r * apfm::cos(phi), r * apfm::sin(phi)};
 }
 
 AP_HD DirSample cosine_hemisphere(f3 e, f3 n) {

@@ -14,6 +14,7 @@
 
 #include "ap_sample.h"
 #include "ap_spectrum.h"
+#include "ap_fmath.h"
 #include "ap_vec.h"
 
 namespace ap {
@@ -90,7 +91,7 @@ AP_HD DirSample sample_ggx_vndf(f3 V, f3 N, float alpha, f3 rnd) {
     const float b = NoV > 0 ? k * vs.z : vs.z;
     const float z = fmaf(1.0f - rnd.y, 1.0f + b, -b);
     const float sinT = sqrtf(saturatef(1.0f - z * z));
-    const f3 os{sinT * cosf(phi), sinT * sinf(phi), z};
+    const f3 os{sinT * apfm::cos(phi), sinT * apfm::sin(phi), z};
     const f3 ms = vs + os;
     const f3 H = normalize(f3{ms.x * alpha, ms.y * alpha, ms.z});
     const f3 lL = reflect(-lv, H);
@@ -165,7 +166,7 @@ AP_HD f3 sky_lighting(f3 d) {
     const f3 white{1.0f, 1.0f, 1.0f};
     const f3 blue{0.05f, 0.45f, 1.0f};
     const f3 down = f3{0.2f, 0.6f, 1.0f} * 0.125f;
-    return 0.5f * lerp3(lerp3(blue, white, powf(fmaxf(d.y, 0.0f), 0.75f)), down,
+    return 0.5f * lerp3(lerp3(blue, white, apfm::pow075(fmaxf(d.y, 0.0f))), down,
                         fminf(fmaxf(-d.y * 0.5f + 0.5f, 0.0f), 1.0f));
 }
 

@@ -13,6 +13,7 @@
 // Transcendentals are the HIP device library's accurate versions.
 #pragma once
 
+#include "ap_fmath.h"
 #include "ap_vec.h"
 
 namespace ap {
@@ -48,7 +49,7 @@ AP_HD float ndtrif(float u) {
     float x;
     if (uu < pLow || uu > pHigh) {
         const bool low = uu < pLow;
-        const float q = sqrtf(-2.0f * (low ? logf(uu) : log1pf(-uu)));
+        const float q = sqrtf(-2.0f * (low ? apfm::log(uu) : apfm::log1p(-uu)));
         float num = fmaf(fmaf(fmaf(fmaf(fmaf(c0, q, c1), q, c2), q, c3), q, c4), q, c5);
         float den = fmaf(fmaf(fmaf(fmaf(d0, q, d1), q, d2), q, d3), q, 1.0f);
         x = low ? num / den : -num / den;
@@ -60,9 +61,9 @@ AP_HD float ndtrif(float u) {
         const float den = fmaf(fmaf(fmaf(fmaf(fmaf(b0, r, b1), r, b2), r, b3), r, b4), r, 1.0f);
         x = num / den;
     }
-    const float expo = expf(fmaf(-0.5f, x * x, 0.0f));
+    const float expo = apfm::exp(fmaf(-0.5f, x * x, 0.0f));
     const float pdf = expo * 0.3989422804014327f;
-    const float cdf = 0.5f * erfcf(-x * 0.7071067811865476f);
+    const float cdf = 0.5f * apfm::erfc(-x * 0.7071067811865476f);
     return x - (cdf - uu) / pdf;
 }
 
@@ -70,15 +71,15 @@ AP_HD float clamp_lambda(float h) { return h < 360.0f ? 360.0f : (h > 780.0f ? 7
 
 AP_HD float lambda_pdf_x(float l) {
     const float k1 = (l - 595.8f) / 33.33f, k2 = (l - 446.8f) / 19.44f;
-    return (1.065f * expf(-0.5f * k1 * k1) + 0.366f * expf(-0.5f * k2 * k2)) / 106.81109281f;
+    return (1.065f * apfm::exp(-0.5f * k1 * k1) + 0.366f * apfm::exp(-0.5f * k2 * k2)) / 106.81109281f;
 }
 AP_HD float lambda_pdf_y(float l) {
-    const float k = (logf(l) - 6.3213077f) / 0.075f;
-    return 1.014f * expf(-0.5f * k * k) / 106.34513640f;
+    const float k = (apfm::log(l) - 6.3213077f) / 0.075f;
+    return 1.014f * apfm::exp(-0.5f * k * k) / 106.34513640f;
 }
 AP_HD float lambda_pdf_z(float l) {
-    const float k = (logf(l) - 6.1088030f) / 0.051f;
-    return 1.839f * expf(-0.5f * k * k) / 105.88243587f;
+    const float k = (apfm::log(l) - 6.1088030f) / 0.051f;
+    return 1.839f * apfm::exp(-0.5f * k * k) / 105.88243587f;
 }
 
 // SampleCIE1931: 0.1 uniform + X (two Gaussians) + Y, Z (log-normal).
@@ -98,10 +99,10 @@ AP_HD void sample_lambda(float r0, float r1, float r2, float& lambda, float& pdf
         }
     } else if (r0 < uR + xR + yR) {
         const float t = 0.0000000020798312f + r1 * (0.9999953206338883f - 0.0000000020798312f);
-        l = clamp_lambda(expf(6.3269327170812479f + 0.0750000000000000f * ndtrif(t)));
+        l = clamp_lambda(apfm::exp(6.3269327170812479f + 0.0750000000000000f * ndtrif(t)));
     } else {
         const float t = 0.0000049890553249f + r1 * (1.0f - 0.0000049890553249f);
-        l = clamp_lambda(expf(6.1114040395252154f + 0.0510000000000000f * ndtrif(t)));
+        l = clamp_lambda(apfm::exp(6.1114040395252154f + 0.0510000000000000f * ndtrif(t)));
     }
     const float p = uR * 1.0f / (780.0f - 360.0f) + xR * lambda_pdf_x(l) + yR * lambda_pdf_y(l) + zR * lambda_pdf_z(l);
     lambda = l;
@@ -175,15 +176,15 @@ AP_HD f4 rgb_encode(const float* w, float r, float g, float b) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const float ph = in3[c] * inv * kPiF * static_cast<float>(k + 1);
-            s[k + 8 * c] = sinf(ph);
-            s[k + 4 + 8 * c] = cosf(ph);
+            s[k + 8 * c] = apfm::sin(ph);
+            s[k + 4 + 8 * c] = apfm::cos(ph);
         }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const float ph = scale * kPiF * static_cast<float>(k + 1);
-        s[k + 24] = sinf(ph);
-        s[k + 28] = cosf(ph);
+        s[k + 24] = apfm::sin(ph);
+        s[k + 28] = apfm::cos(ph);
     }
     float h[32];
     mlp_linear<32, 32, true, true>(s, h, w + MlpLayout::rgb_in_w, w + MlpLayout::rgb_in_b);
@@ -201,8 +202,8 @@ AP_HD f4 lambda_encode(const float* w, float lambda) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const float ph = x * kPiF * static_cast<float>(k + 1);
-        h[k] = sinf(ph);
-        h[k + 16] = cosf(ph);
+        h[k] = apfm::sin(ph);
+        h[k + 16] = apfm::cos(ph);
     }
 #pragma unroll
     for (int k = 0; k < 7; ++k)

@@ -40,6 +40,8 @@ void or_lambda_encode(const float* lam, size_t n, float* out);
 float or_spectral_interact(const float* rgbK, const float* lamK);
 void or_query_rgb_basis(const float* rgb, size_t n, float* out);
 void or_query_lambda(const float* lam, size_t n, float* out);
+void or_interact(const float* lam, const float* rgb, size_t n, float* out);
+void or_fmath(int fn, const float* x, size_t n, float* out);
 float or_ndtrif(float u);
 void or_sample_cie1931(const float* u3, size_t n, float* lambda, float* pdf);
 float or_pdf_cie1931(float lambda);

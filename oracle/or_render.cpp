@@ -28,6 +28,7 @@
 
 #include "aperture/ap_render.h"
 #include "or_api.h"
+#include "ap_fmath.h"
 #include "or_math.h"
 #include "or_spectrum.h"
 
@@ -168,7 +169,7 @@ static void BuildTangentBasisRandom(float3 n, float3& t, float3& b, float e) {
     const float3 a = (std::fabs(n.z) < 0.999f) ? float3{0, 0, 1} : float3{1, 0, 0};
     const float3 t0 = normalize(cross(a, n));
     const float3 b0 = cross(n, t0);
-    const float s = std::sin(e * PI_F * 2.0f), c = std::cos(e * PI_F * 2.0f);
+    const float s = apfm::sin(e * PI_F * 2.0f), c = apfm::cos(e * PI_F * 2.0f);
     t = c * t0 + s * b0;
     b = -s * t0 + c * b0;
 }
@@ -178,7 +179,7 @@ static float2 ConcentricSampleDisk(float ux, float uy) {
     float r, phi;
     if (std::fabs(sx) > std::fabs(sy)) { r = sx; phi = (PI_F / 4.0f) * (sy / std::fmax(std::fabs(sx), 1e-20f)); }
     else { r = sy; phi = (PI_F / 2.0f) - (PI_F / 4.0f) * (sx / std::fmax(std::fabs(sy), 1e-20f)); }
-    return {r * std::cos(phi), r * std::sin(phi)};
+    return {r * apfm::cos(phi), r * apfm::sin(phi)};
 }
 struct Sample { float3 dir; float pdf; };
 static Sample SampleCosine(float3 e, float3 n) {
@@ -209,7 +210,7 @@ static Sample SampleGGXBoundedVNDF(float3 V, float3 N, float alpha, float3 rnd) 
     const float b = NoV > 0 ? k * vs.z : vs.z;
     const float z = std::fma(1.0f - rnd.y, 1.0f + b, -b);
     const float sinT = std::sqrt(saturate_(1.0f - z * z));
-    const float3 os{sinT * std::cos(phi), sinT * std::sin(phi), z};
+    const float3 os{sinT * apfm::cos(phi), sinT * apfm::sin(phi), z};
     const float3 ms = vs + os;
     const float3 H = normalize(float3{ms.x * alpha, ms.y * alpha, ms.z});
     const float3 lL = reflect(-lv, H);
@@ -529,7 +530,7 @@ static LightSample NEESampleLight(const Scene& S, float3 p, Rng& rng) {
         const float phi = 2.0f * PI_F * vphi;
         float3 t, b;
         BuildTangentBasis(axis, t, b);
-        const float3 nn = t * (std::cos(phi) * sinT) + b * (std::sin(phi) * sinT) + axis * cosT;
+        const float3 nn = t * (apfm::cos(phi) * sinT) + b * (apfm::sin(phi) * sinT) + axis * cosT;
         ls.point = sc + nn * r;
         ls.normal = nn;
         ls.pFacet = 1.0f;
@@ -551,7 +552,7 @@ static float PowerHeuristic(float pA, float pB) {
 static float3 SkyLighting(float3 d) {
     const float3 white{1, 1, 1}, blue{0.05f, 0.45f, 1.0f};
     const float3 down = float3{0.2f, 0.6f, 1.0f} * 0.125f;
-    return 0.5f * lerp(lerp(blue, white, std::pow(std::fmax(d.y, 0.0f), 0.75f)), down,
+    return 0.5f * lerp(lerp(blue, white, apfm::pow075(std::fmax(d.y, 0.0f))), down,
                        std::fmin(std::fmax(-d.y * 0.5f + 0.5f, 0.0f), 1.0f));
 }
 

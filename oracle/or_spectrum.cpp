@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "or_api.h"
+#include "ap_fmath.h"
 #include "or_math.h"
 #include "or_spectrum.h"
 
@@ -60,13 +61,13 @@ void RGBEncode(const float in3[3], float outK[4]) {
     for (int c = 0; c < 3; ++c)
         for (int k = 0; k < 4; ++k) {
             const float phase = in3[c] * scaleEpsInv * PI_F * static_cast<float>(k + 1);
-            s[k + 8 * c] = std::sin(phase);
-            s[k + 4 + 8 * c] = std::cos(phase);
+            s[k + 8 * c] = apfm::sin(phase);
+            s[k + 4 + 8 * c] = apfm::cos(phase);
         }
     for (int k = 0; k < 4; ++k) {
         const float phase = scale * PI_F * static_cast<float>(k + 1);
-        s[k + 24] = std::sin(phase);
-        s[k + 28] = std::cos(phase);
+        s[k + 24] = apfm::sin(phase);
+        s[k + 28] = apfm::cos(phase);
     }
     float h[32];
     Linear<32, 32, true, true>(s, h, T.rgb_in_w, T.rgb_in_b);
@@ -81,8 +82,8 @@ void LambdaEncode(float lambda, float outK[4]) {
     float h[32];
     for (int k = 0; k < 16; ++k) {
         const float phase = x * PI_F * static_cast<float>(k + 1);
-        h[k] = std::sin(phase);
-        h[k + 16] = std::cos(phase);
+        h[k] = apfm::sin(phase);
+        h[k + 16] = apfm::cos(phase);
     }
     for (int b = 0; b < 7; ++b) Residual(h, T.lam_blk_w[b], T.lam_blk_b[b]);
     Linear<32, 4, false, false>(h, outK, T.lam_out_w, nullptr);
@@ -178,7 +179,7 @@ float ndtrif(float u) {
     float x;
     if (uu < pLow || uu > pHigh) {
         const bool low = uu < pLow;
-        const float q = std::sqrt(-2.0f * (low ? std::log(uu) : std::log1p(-uu)));
+        const float q = std::sqrt(-2.0f * (low ? apfm::log(uu) : apfm::log1p(-uu)));
         float num = std::fma(c[0], q, c[1]);
         num = std::fma(num, q, c[2]);
         num = std::fma(num, q, c[3]);
@@ -206,9 +207,9 @@ float ndtrif(float u) {
         x = num / den;
     }
     const float INV_SQRT_2PI = 0.3989422804014327f, INV_SQRT2 = 0.7071067811865476f;
-    const float expo = std::exp(std::fma(-0.5f, x * x, 0.0f));
+    const float expo = apfm::exp(std::fma(-0.5f, x * x, 0.0f));
     const float pdf = expo * INV_SQRT_2PI;
-    const float cdf = 0.5f * std::erfc(-x * INV_SQRT2);
+    const float cdf = 0.5f * apfm::erfc(-x * INV_SQRT2);
     x -= (cdf - uu) / pdf;
     return x;
 }
@@ -227,25 +228,25 @@ float SampleX(float uComp, float uInv) {
 
 float SampleY(float u) {
     const float t = 0.0000000020798312f + u * (0.9999953206338883f - 0.0000000020798312f);
-    return clampL(std::exp(6.3269327170812479f + 0.0750000000000000f * ndtrif(t)));
+    return clampL(apfm::exp(6.3269327170812479f + 0.0750000000000000f * ndtrif(t)));
 }
 
 float SampleZ(float u) {
     const float t = 0.0000049890553249f + u * (1.0f - 0.0000049890553249f);
-    return clampL(std::exp(6.1114040395252154f + 0.0510000000000000f * ndtrif(t)));
+    return clampL(apfm::exp(6.1114040395252154f + 0.0510000000000000f * ndtrif(t)));
 }
 
 float PdfX(float l) {
     const float k1 = (l - 595.8f) / 33.33f, k2 = (l - 446.8f) / 19.44f;
-    return (1.065f * std::exp(-0.5f * k1 * k1) + 0.366f * std::exp(-0.5f * k2 * k2)) / 106.81109281f;
+    return (1.065f * apfm::exp(-0.5f * k1 * k1) + 0.366f * apfm::exp(-0.5f * k2 * k2)) / 106.81109281f;
 }
 float PdfY(float l) {
-    const float k = (std::log(l) - 6.3213077f) / 0.075f;
-    return 1.014f * std::exp(-0.5f * k * k) / 106.34513640f;
+    const float k = (apfm::log(l) - 6.3213077f) / 0.075f;
+    return 1.014f * apfm::exp(-0.5f * k * k) / 106.34513640f;
 }
 float PdfZ(float l) {
-    const float k = (std::log(l) - 6.1088030f) / 0.051f;
-    return 1.839f * std::exp(-0.5f * k * k) / 105.88243587f;
+    const float k = (apfm::log(l) - 6.1088030f) / 0.051f;
+    return 1.839f * apfm::exp(-0.5f * k * k) / 105.88243587f;
 }
 
 void SampleCIE1931(float r0, float r1, float r2, float* lambda, float* pdf) {
@@ -327,6 +328,31 @@ void or_query_rgb_basis(const float* rgb, size_t n, float* out) {
 }
 void or_query_lambda(const float* lam, size_t n, float* out) {
     for (size_t i = 0; i < n; ++i) QueryLambda(lam[i], out + 4 * i);
+}
+void or_interact(const float* lam, const float* rgb, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        float lb[4];
+        QueryLambda(lam[i], lb);
+        out[i] = Interact(lb, float3{rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]});
+    }
+}
+// The portable transcendentals shared with the core (ap_fmath.h), for their
+// accuracy tests: fn 0 sin, 1 cos, 2 exp, 3 log, 4 log1p, 5 erfc, 6 x^0.75.
+void or_fmath(int fn, const float* x, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const float v = x[i];
+        float r = 0.0f;
+        switch (fn) {
+            case 0: r = apfm::sin(v); break;
+            case 1: r = apfm::cos(v); break;
+            case 2: r = apfm::exp(v); break;
+            case 3: r = apfm::log(v); break;
+            case 4: r = apfm::log1p(v); break;
+            case 5: r = apfm::erfc(v); break;
+            default: r = apfm::pow075(v); break;
+        }
+        out[i] = r;
+    }
 }
 float or_ndtrif(float u) { return ndtrif(u); }
 void or_sample_cie1931(const float* u3, size_t n, float* lambda, float* pdf) {

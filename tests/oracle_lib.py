@@ -35,6 +35,8 @@ def lib():
             getattr(L, name).argtypes = [vp, sz, vp]
         L.or_spectral_interact.restype = f32
         L.or_spectral_interact.argtypes = [vp, vp]
+        L.or_interact.argtypes = [vp, vp, sz, vp]
+        L.or_fmath.argtypes = [i32, vp, sz, vp]
         L.or_ndtrif.restype = f32
         L.or_ndtrif.argtypes = [f32]
         L.or_sample_cie1931.argtypes = [vp, sz, vp, vp]
@@ -168,6 +170,26 @@ def query_lambda(lam):
     lam = _f32(lam, (-1,))
     out = np.zeros((len(lam), 4), np.float32)
     lib().or_query_lambda(_p(lam), len(lam), _p(out))
+    return out
+
+
+def interact(lam, rgb):
+    """lut::Interact(QueryLambda(lambda), rgb) per element."""
+    lam = _f32(lam, (-1,))
+    rgb = _f32(rgb, (-1, 3))
+    out = np.zeros(len(lam), np.float32)
+    lib().or_interact(_p(lam), _p(rgb), len(lam), _p(out))
+    return out
+
+
+FMATH = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "log1p": 4, "erfc": 5, "pow075": 6}
+
+
+def fmath(name, x):
+    """The portable FP32 transcendentals shared by the core and the oracle."""
+    x = _f32(x, (-1,))
+    out = np.zeros(len(x), np.float32)
+    lib().or_fmath(FMATH[name], _p(x), len(x), _p(out))
     return out
 
 
