@@ -137,10 +137,12 @@ _lib = None
 # Every symbol include/aperture/ap_core.h and ap_synth.h declare.
 EXPORTED_SYMBOLS = (
     "ap_scene_create", "ap_build_bvh", "ap_scene_destroy", "ap_scene_get_info", "ap_scene_set_traversal_mode",
+    "ap_scene_set_launch", "ap_scene_get_launch",
     "ap_scene_export_bvh",
     "ap_trace_closest", "ap_trace_any", "ap_trace_closest_stats", "ap_last_error_string",
     "ap_synth_triangle_soup", "ap_synth_camera_rays", "ap_synth_secondary_rays",
     "ap_device_count", "ap_device_set", "ap_device_name", "ap_device_malloc", "ap_device_free", "ap_memcpy",
+    "ap_device_copy",
     "ap_memset_async", "ap_stream_create", "ap_stream_destroy", "ap_stream_synchronize", "ap_device_synchronize",
     "ap_event_create", "ap_event_destroy", "ap_event_record", "ap_event_elapsed_ms",
     "ap_spectrum_setup", "ap_spectrum_encode_rgb", "ap_spectrum_encode_lambda", "ap_spectrum_query_rgb",
@@ -162,6 +164,8 @@ def lib():
         L.ap_build_bvh.argtypes = [vp, sz, ctypes.POINTER(BuildParams), vp, vp, ctypes.POINTER(ctypes.c_size_t)]
         L.ap_scene_get_info.argtypes = [vp, ctypes.POINTER(BVHInfo)]
         L.ap_scene_set_traversal_mode.argtypes = [vp, i32]
+        L.ap_scene_set_launch.argtypes = [vp, i32, i32]
+        L.ap_scene_get_launch.argtypes = [vp, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
             getattr(L, name).argtypes = [vp, vp, sz, vp, vp]
@@ -180,6 +184,7 @@ def lib():
         L.ap_device_free.argtypes = [vp]
         L.ap_memcpy.argtypes = [vp, vp, sz, i32]
         L.ap_memset_async.argtypes = [vp, i32, sz, vp]
+        L.ap_device_copy.argtypes = [vp, vp, sz, vp]
         L.ap_stream_create.argtypes = [pvp]
         L.ap_stream_destroy.argtypes = [vp]
         L.ap_stream_synchronize.argtypes = [vp]
@@ -288,6 +293,11 @@ class DeviceBuffer:
 
     def zero(self, stream=None):
         check(lib().ap_memset_async(self.ptr, 0, self.nbytes, _stream(stream)))
+
+    def copy_from(self, src, nbytes=None, stream=None):
+        """Stream-ordered device-to-device copy by the library's streaming kernel."""
+        n = self.nbytes if nbytes is None else nbytes
+        check(lib().ap_device_copy(self.ptr, _ptr(src), n, _stream(stream)))
 
     def free(self):
         if self.ptr:
@@ -457,6 +467,15 @@ class SceneRenderer:
 
     def set_traversal_mode(self, mode):
         check(lib().ap_scene_set_traversal_mode(self.handle, mode))
+
+    def set_launch(self, wave_rays=0, refill=0):
+        """Traversal launch shape (rays per wave, idle lanes per refill); 0 keeps a value."""
+        check(lib().ap_scene_set_launch(self.handle, wave_rays, refill))
+
+    def get_launch(self):
+        w, r = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().ap_scene_get_launch(self.handle, ctypes.byref(w), ctypes.byref(r)))
+        return w.value, r.value
 
     def export_bvh(self):
         inf = self.info()

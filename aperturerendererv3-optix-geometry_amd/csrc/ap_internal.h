@@ -76,6 +76,8 @@ struct DeviceScene {
     int32_t stack_cap;
     float line_margin;   // world-space growth of boxes for the CULL line test
     int32_t mode;        // AP_TRAVERSE_REFERENCE or AP_TRAVERSE_CULLED
+    int32_t refill;      // idle lanes that trigger a refill of a wave (1..64)
+    int32_t wave_rays;   // rays owned by one wave (a positive multiple of 64)
 };
 
 // Reference-layout BVH build result (bvh.cuh:54-58, 32-bit node variant).

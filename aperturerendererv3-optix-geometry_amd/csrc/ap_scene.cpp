@@ -165,6 +165,8 @@ static int build_device(ap_scene* s) {
         d.line_margin = mag * 0x1p-12f;
     }
     d.mode = AP_TRAVERSE_CULLED;
+    d.refill = 16;
+    d.wave_rays = 512;
     // The stack never holds more than (depth of an expanded node) + 2 entries.
     d.stack_cap = std::min(128, s->max_depth + 2);
     return AP_OK;
@@ -279,6 +281,23 @@ int ap_scene_get_info(const ap_scene* s, ap_bvh_info* out) {
     out->traversal_mode = s->dev.mode;
     out->line_margin = s->dev.line_margin;
     out->pad = 0;
+    return AP_OK;
+}
+
+int ap_scene_set_launch(ap_scene* scene, int32_t wave_rays, int32_t refill) {
+    if (!scene) return fail(AP_ERR_INVALID_ARG, "ap_scene_set_launch: NULL scene");
+    if (wave_rays < 0 || wave_rays > 65536 || wave_rays % 64 != 0)
+        return fail(AP_ERR_INVALID_ARG, "ap_scene_set_launch: wave_rays must be a multiple of 64 in [64, 65536]");
+    if (refill < 0 || refill > 64) return fail(AP_ERR_INVALID_ARG, "ap_scene_set_launch: refill must be in [1, 64]");
+    if (wave_rays > 0) scene->dev.wave_rays = wave_rays;
+    if (refill > 0) scene->dev.refill = refill;
+    return AP_OK;
+}
+
+int ap_scene_get_launch(const ap_scene* scene, int32_t* wave_rays, int32_t* refill) {
+    if (!scene) return fail(AP_ERR_INVALID_ARG, "ap_scene_get_launch: NULL scene");
+    if (wave_rays) *wave_rays = scene->dev.wave_rays;
+    if (refill) *refill = scene->dev.refill;
     return AP_OK;
 }
 

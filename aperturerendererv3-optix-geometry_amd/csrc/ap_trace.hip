@@ -1,6 +1,6 @@
 // BVH closest-hit and any-hit traversal for gfx950 (CDNA4).
 //
-// One ray per lane, 64-lane workgroups. The traversal reproduces the visit
+// Waves of 64 lanes, one ray per lane at a time. The traversal reproduces the visit
 // order of the reference TraverseBVH (core/render/intersection.cuh:235-370):
 // pop, re-test the popped node against the current closest distance, test a
 // leaf's primitives in slot order, or test both children and push the far
@@ -13,11 +13,14 @@
 //    the re-test reduces to `tEnter <= closest`, with no second node fetch.
 //  * A 64-B NodeRec holds both children's boxes; leaves are runs of 48-B
 //    PrimRecs in slot order, ended by PRIM_LAST.
-// The per-lane stack lives in LDS as [entry][lane] columns (conflict-free for
-// any per-lane depth). Its capacity is picked on the host from the BVH depth;
-// pushes beyond it are dropped exactly as the reference drops pushes beyond
-// its 128 entries (intersection.cuh:258-267), which with capacity >=
-// depth + 2 never happens.
+// The top RING entries of the per-lane stack live in LDS as [entry][lane]
+// columns (conflict-free), older ones in the lane's private memory; 4 KB of
+// LDS per wave leaves occupancy to the register budget. The capacity is
+// picked on the host from the BVH depth; pushes beyond it are dropped exactly
+// as the reference drops pushes beyond its 128 entries
+// (intersection.cuh:258-267), which with capacity >= depth + 2 never happens.
+#include <algorithm>
+
 #include "ap_internal.h"
 
 namespace ap {
@@ -69,54 +72,121 @@ __device__ bool sphere_t(f3 o, f3 d, const AnalyticRec& a, float& t) {
     return true;
 }
 
+constexpr uint32_t kXcds = 8;
+constexpr int RING = 8;  // LDS stack entries per lane (a power of two below every CAP)  // gfx950: workgroups are dealt round-robin to 8 XCDs
+
+__device__ __forceinline__ uint64_t lanes_below(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Chunked traversal. Wave w owns the contiguous ray range
+// [w*chunk, (w+1)*chunk) of `wave_rays` rays (waves on one XCD get adjacent
+// ranges, so coherent rays share that XCD's L2). Each lane traces one ray at
+// a time; once at least `refill` lanes are idle the wave hands them the next
+// rays of its range in one ballot, so a wave no longer waits for its slowest
+// ray, and the launch tail is bounded by one chunk. A step is one pop of the
+// reference loop: re-test, then a leaf's primitives or an interior node's
+// two children.
 template <bool ANY, bool STATS, bool CULL, int CAP>
 __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray* __restrict__ rays,
-                                                     size_t n, const uint32_t* __restrict__ dyn_n,
+                                                     uint32_t n_cap, const uint32_t* __restrict__ dyn_n,
                                                      ap_hit* __restrict__ hits,
                                                      ap_trace_stats* __restrict__ stats) {
-    __shared__ int32_t st_link[CAP * WAVE];
-    __shared__ float st_te[CAP * WAVE];
+    // Stack entries [lo, sp) live in an LDS ring of RING entries per lane
+    // (position i % RING); entries [0, lo) that a deeper push evicted live in
+    // the lane's private array (scratch memory, rarely touched).
+    __shared__ int32_t st_link[RING * WAVE];
+    __shared__ float st_te[RING * WAVE];
+    // volatile keeps the compiler from promoting these to registers.
+    volatile int32_t sp_link[CAP - RING];
+    volatile float sp_te[CAP - RING];
+    int lo = 0;
     const int lane = threadIdx.x;
-    const size_t i = static_cast<size_t>(blockIdx.x) * WAVE + lane;
-    // A device-side count (queues of the integrator) bounds a grid sized for capacity n.
-    if (dyn_n) n = *dyn_n;
-    if (i >= n) return;
+    // A device-side count (queues of the integrator) bounds a grid sized for capacity n_cap.
+    const uint32_t n = dyn_n ? min(*dyn_n, n_cap) : n_cap;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t wave_rank = (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
+    const uint32_t chunk = static_cast<uint32_t>(s.wave_rays);
+    if (static_cast<uint64_t>(wave_rank) * chunk >= n) return;
+    uint32_t next = wave_rank * chunk;   // wave-uniform
+    const uint32_t end = min(n, next + chunk);
+    const int refill = min(max(s.refill, 1), WAVE);  // at most WAVE, or an all-idle wave would never refill
 
-    const float4* rp = reinterpret_cast<const float4*>(rays + i);
-    const float4 r0 = rp[0];
-    const float4 r1 = rp[1];
-    const f3 o{r0.x, r0.y, r0.z};
-    const f3 d{r1.x, r1.y, r1.z};
-    const float tmin = r0.w;
-    float dist = r1.w;
+    bool live = false;
+    uint32_t ri = 0;
+    f3 o{0, 0, 0}, d{0, 0, 0}, inv{0, 0, 0}, w{0, 0, 0};
+    float tmin = 0.0f, dist = 0.0f;
     int32_t hit = -1;
+    int sp = 0;
     uint32_t nnode = 0, nprim = 0;
 
-    if (!s.root.valid) {
-        dist = __int_as_float(0x7f800000);
-    } else {
-        const f3 inv = inv_safe(d);
-        const f3 w = absf3(inv) * s.line_margin;
-        int sp = 0;
-        {
-            float te;
-            if (slab_enter<CULL>(o, inv, w, ld3(s.root.bmin), ld3(s.root.bmax), tmin, dist, te)) {
-                st_link[lane] = s.root.link;
-                st_te[lane] = te;
-                sp = 1;
-            } else if (STATS) {
-                nnode = 1;  // the reference pops and rejects the root
+    for (;;) {
+        const uint64_t idle = __ballot(!live);
+        if (next < end && __popcll(idle) >= refill) {
+            if (!live) {
+                const uint32_t j = next + static_cast<uint32_t>(__popcll(idle & lanes_below(lane)));
+                if (j < end) {
+                    ri = j;
+                    const float4* rp = reinterpret_cast<const float4*>(rays + j);
+                    const float4 r0 = rp[0];
+                    const float4 r1 = rp[1];
+                    o = f3{r0.x, r0.y, r0.z};
+                    d = f3{r1.x, r1.y, r1.z};
+                    tmin = r0.w;
+                    dist = r1.w;
+                    hit = -1;
+                    nnode = 0;
+                    nprim = 0;
+                    sp = 0;
+                    lo = 0;
+                    bool finished = true;
+                    if (!s.root.valid) {
+                        dist = __int_as_float(0x7f800000);
+                    } else {
+                        inv = inv_safe(d);
+                        w = absf3(inv) * s.line_margin;
+                        float te;
+                        if (slab_enter<CULL>(o, inv, w, ld3(s.root.bmin), ld3(s.root.bmax), tmin, dist, te)) {
+                            st_link[lane] = s.root.link;
+                            st_te[lane] = te;
+                            sp = 1;
+                            finished = false;
+                        } else if (STATS) {
+                            nnode = 1;  // the reference pops and rejects the root
+                        }
+                    }
+                    if (finished) {
+                        *reinterpret_cast<int2*>(hits + ri) = int2{hit, __float_as_int(dist)};
+                        if (STATS) stats[ri] = ap_trace_stats{nnode, nprim};
+                    } else {
+                        live = true;
+                    }
+                }
             }
+            next = min(end, next + static_cast<uint32_t>(__popcll(idle)));
         }
-        while (sp > 0) {
-            --sp;
-            const int32_t link = st_link[sp * WAVE + lane];
-            const float te = st_te[sp * WAVE + lane];
-            if (STATS) ++nnode;
-            if (!(te <= dist)) continue;
+        if (__ballot(live) == 0) {
+            if (next >= end) break;
+            continue;
+        }
+        if (!live) continue;
+
+        --sp;
+        int32_t link;
+        float te;
+        if (sp < lo) {
+            link = sp_link[sp];
+            te = sp_te[sp];
+            lo = sp;
+        } else {
+            link = st_link[(sp & (RING - 1)) * WAVE + lane];
+            te = st_te[(sp & (RING - 1)) * WAVE + lane];
+        }
+        if (STATS) ++nnode;
+        bool any_done = false;
+        if (te <= dist) {
             if (link < 0) {
                 int32_t slot = ~link;
-                bool done = false;
                 for (;;) {
                     const float4* pr = reinterpret_cast<const float4*>(s.prims + slot);
                     const float4 A = pr[0];
@@ -141,59 +211,66 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                     if (ok && !(t < tmin || t >= dist)) {
                         dist = t;
                         hit = __float_as_int(A.w);
-                        if (ANY) { done = true; break; }
+                        if (ANY) { any_done = true; break; }
                     }
                     if (flags & PRIM_LAST) break;
                     ++slot;
                 }
-                if (ANY && done) break;
             } else {
                 const float4* nr = reinterpret_cast<const float4*>(s.nodes + link);
                 const float4 a = nr[0];
-                const float4 b = nr[1];
+                const float4 bb = nr[1];
                 const float4 c = nr[2];
                 const float4 e = nr[3];
                 if (STATS) nnode += 2;
                 float tl = 0.0f, tr = 0.0f;
-                const bool hl = slab_enter<CULL>(o, inv, w, {a.x, a.y, a.z}, {b.x, b.y, b.z}, tmin, dist, tl);
+                const bool hl = slab_enter<CULL>(o, inv, w, {a.x, a.y, a.z}, {bb.x, bb.y, bb.z}, tmin, dist, tl);
                 const bool hr = slab_enter<CULL>(o, inv, w, {c.x, c.y, c.z}, {e.x, e.y, e.z}, tmin, dist, tr);
                 const int32_t ll = __float_as_int(a.w);
-                const int32_t rl = __float_as_int(b.w);
+                const int32_t rl = __float_as_int(bb.w);
+                auto push = [&](int32_t l, float t) {
+                    if (sp >= CAP) return;
+                    if (sp - lo == RING) {
+                        const int k = (lo & (RING - 1)) * WAVE + lane;
+                        sp_link[lo] = st_link[k];
+                        sp_te[lo] = st_te[k];
+                        ++lo;
+                    }
+                    const int k = (sp & (RING - 1)) * WAVE + lane;
+                    st_link[k] = l;
+                    st_te[k] = t;
+                    ++sp;
+                };
                 if (hl && hr) {
                     // Far child first, near child on top; ties keep left on top.
                     const bool left_far = tl > tr;
-                    const int32_t l0 = left_far ? ll : rl, l1 = left_far ? rl : ll;
-                    const float t0 = left_far ? tl : tr, t1 = left_far ? tr : tl;
-                    if (sp < CAP) { st_link[sp * WAVE + lane] = l0; st_te[sp * WAVE + lane] = t0; ++sp; }
-                    if (sp < CAP) { st_link[sp * WAVE + lane] = l1; st_te[sp * WAVE + lane] = t1; ++sp; }
+                    push(left_far ? ll : rl, left_far ? tl : tr);
+                    push(left_far ? rl : ll, left_far ? tr : tl);
                 } else if (hl || hr) {
-                    if (sp < CAP) {
-                        st_link[sp * WAVE + lane] = hl ? ll : rl;
-                        st_te[sp * WAVE + lane] = hl ? tl : tr;
-                        ++sp;
-                    }
+                    push(hl ? ll : rl, hl ? tl : tr);
                 }
             }
         }
+        if (sp == 0 || any_done) {
+            *reinterpret_cast<int2*>(hits + ri) = int2{hit, __float_as_int(dist)};
+            if (STATS) stats[ri] = ap_trace_stats{nnode, nprim};
+            live = false;
+        }
     }
-    int2 out;
-    out.x = hit;
-    out.y = __float_as_int(dist);
-    *reinterpret_cast<int2*>(hits + i) = out;
-    if (STATS) stats[i] = ap_trace_stats{nnode, nprim};
 }
 
 template <bool ANY, bool STATS, bool CULL>
-hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, size_t n, const uint32_t* dyn_n, ap_hit* hits,
+hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, uint32_t n, const uint32_t* dyn_n, ap_hit* hits,
                       ap_trace_stats* stats, hipStream_t stream) {
     const dim3 block(WAVE);
-    const dim3 grid(static_cast<unsigned>((n + WAVE - 1) / WAVE));
     const int cap = s.stack_cap;
-#define AP_CAP_CASE(C)                                                                   \
-    if (cap <= C) {                                                                      \
-        hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C>), grid, block, 0, stream, s, rays, n, \
-                           dyn_n, hits, stats);                                                 \
-        return hipGetLastError();                                                        \
+#define AP_CAP_CASE(C)                                                                                  \
+    if (cap <= C) {                                                                                     \
+        const uint32_t chunk = static_cast<uint32_t>(s.wave_rays);                                      \
+        const uint32_t waves = (n + chunk - 1) / chunk;                                                 \
+        hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C>), dim3(waves), block, 0, stream, s, rays, n, \
+                           dyn_n, hits, stats);                                                         \
+        return hipGetLastError();                                                                       \
     }
     AP_CAP_CASE(16)
     AP_CAP_CASE(24)
@@ -208,9 +285,22 @@ hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, size_t n, const 
 
 }  // namespace
 
-hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
+hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n_total, ap_hit* hits,
                         ap_trace_stats* stats, bool any_hit, hipStream_t stream, const uint32_t* dyn_n) {
-    if (n == 0) return hipSuccess;
+    if (n_total == 0) return hipSuccess;
+    // Ray indices are 32-bit inside the kernel; larger host batches go in slices.
+    constexpr size_t kSlice = size_t{1} << 31;
+    if (n_total > kSlice) {
+        if (dyn_n) return hipErrorInvalidValue;
+        for (size_t off = 0; off < n_total; off += kSlice) {
+            const size_t m = std::min(kSlice, n_total - off);
+            const hipError_t e = launch_trace(s, rays + off, m, hits + off, stats ? stats + off : nullptr, any_hit,
+                                              stream, nullptr);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    const uint32_t n = static_cast<uint32_t>(n_total);
     const bool cull = s.mode == AP_TRAVERSE_CULLED;
 #define AP_DISPATCH(A, S, C) \
     if (any_hit == A && (stats != nullptr) == S && cull == C) return launch_cap<A, S, C>(s, rays, n, dyn_n, hits, stats, stream);

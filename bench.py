@@ -47,30 +47,44 @@ def read_traffic():
 
 
 def cpu_baseline(objs, rays_a, rays_b, seconds, threads):
+    """The oracle's restatement of the reference TraverseBVH (its own slab
+    test, OpenMP over rays) on a random sample of both ray sets, sized by a
+    calibration pass to take about `seconds`."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as ol
 
     bvh = ol.OracleBVH(objs)
     g = np.random.default_rng(7)
-    pool = np.concatenate([rays_a[g.choice(len(rays_a), 4096, replace=False)],
-                           rays_b[g.choice(len(rays_b), 4096, replace=False)]])
+    half = 1 << 16
+    pool = np.concatenate([rays_a[g.choice(len(rays_a), min(half, len(rays_a)), replace=False)],
+                           rays_b[g.choice(len(rays_b), min(half, len(rays_b)), replace=False)]])
     g.shuffle(pool)
-    probe = pool[: max(threads, 16)]
-    t0 = time.perf_counter()
-    bvh.trace(probe, threads=threads)
-    dt = time.perf_counter() - t0
-    n = int(min(len(pool), max(len(probe), len(probe) * seconds / max(dt, 1e-6))))
+    # Calibrate on growing batches (the first also warms the thread pool).
+    n, dt = 4 * threads, 0.0
+    while n < len(pool):
+        t0 = time.perf_counter()
+        bvh.trace(pool[:n], threads=threads)
+        dt = time.perf_counter() - t0
+        if dt > 1.0:
+            break
+        n = min(len(pool), n * 4)
+    rate = n / max(dt, 1e-9)
+    n = int(min(len(pool), max(n, rate * seconds)))
     sample = pool[:n]
     t0 = time.perf_counter()
     bvh.trace(sample, threads=threads)
     dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bvh.trace(sample, threads=threads, cull_margin=np.float32(2.0 ** -12))
+    dt_cull = time.perf_counter() - t0
     return {
         "value": n / dt / 1e6,
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} rays drawn evenly from the primary and secondary sets, oracle restatement of the "
-                  f"reference TraverseBVH (its own slab test, no line culling), {dt:.1f} s",
+        "sample": f"{n} rays drawn at random from the primary and secondary sets (equal shares), oracle "
+                  f"restatement of the reference TraverseBVH (its own slab test), OpenMP, {dt:.1f} s; the same "
+                  f"sample with the core's line culling ran at {n / dt_cull / 1e6:.4f} Mrays/s",
     }
 
 
@@ -83,6 +97,10 @@ def main():
     ap_.add_argument("--res", type=int, default=4096)
     ap_.add_argument("--cpu-seconds", type=float, default=20.0)
     ap_.add_argument("--no-cpu", action="store_true")
+    ap_.add_argument("--sweep", action="store_true",
+                     help="time each traversal launch shape (rays per wave x refill) and print the table")
+    ap_.add_argument("--copy-probe", action="store_true",
+                     help="also run the library's streaming copy kernel over 1 GiB (HBM counter scale reference)")
     args = ap_.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the traversal core has no multi-GPU path in this build "
@@ -145,6 +163,34 @@ def main():
     for _ in range(args.warmup):
         step()
     stream.synchronize()
+
+    if args.sweep:
+        ev0, ev1 = ap.Event(), ap.Event()
+        for wr in (64, 128, 256, 512, 1024, 2048, 4096):
+            for rf in (1, 4, 8, 16, 32, 64):
+                scene.set_launch(wr, rf)
+                res = []
+                for d_r, d_h, n in ((d_prim, d_phit, n_prim), (d_sec, d_shit, n_sec)):
+                    scene.trace_closest(d_r, d_h, n=n, stream=stream)
+                    ev0.record(stream)
+                    for _ in range(3):
+                        scene.trace_closest(d_r, d_h, n=n, stream=stream)
+                    ev1.record(stream)
+                    stream.synchronize()
+                    res.append(n * 3 / ev0.elapsed_ms(ev1) / 1e3)
+                log(f"sweep wave_rays {wr:5d} refill {rf:2d}: primary {res[0]:7.1f} secondary {res[1]:7.1f} Mrays/s")
+        scene.set_launch(512, 16)
+
+    if args.copy_probe:
+        nbytes = 1 << 30
+        src, dst = ap.DeviceBuffer(nbytes), ap.DeviceBuffer(nbytes)
+        src.zero(stream)
+        for _ in range(2):
+            dst.copy_from(src, nbytes, stream)
+        stream.synchronize()
+        log(f"copy probe: 2 x {nbytes} bytes through ap_device_copy")
+        src.free()
+        dst.free()
 
     evs = [[ap.Event() for _ in range(3)] for _ in range(args.steps)]
     start, end = ap.Event(), ap.Event()

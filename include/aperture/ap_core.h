@@ -109,6 +109,15 @@ int ap_scene_get_info(const ap_scene* scene, ap_bvh_info* out_info);
  * reference counterpart (the reference has one compiled-in test). */
 int ap_scene_set_traversal_mode(ap_scene* scene, int mode);
 
+/* Launch shape of the traversal kernels: rays owned by one 64-lane wave
+ * (a positive multiple of 64, at most 65536) and the number of idle lanes
+ * (1..64) at which a wave refills them with its next rays. A value of 0
+ * keeps the current setting. Results are identical for every shape; only
+ * speed changes. No reference counterpart (the reference launches one
+ * thread per pixel). Defaults: 512 rays, refill at 16 idle lanes. */
+int ap_scene_set_launch(ap_scene* scene, int32_t wave_rays, int32_t refill);
+int ap_scene_get_launch(const ap_scene* scene, int32_t* wave_rays, int32_t* refill);
+
 /* Copies out the reference-layout BVH (BVHHost::nodes_, objectIndices_,
  * bvh.cuh:54-58) and light table (AdditionalLightInfo[], scene.cu:1056-1080)
  * to host arrays sized from ap_scene_get_info. Any pointer may be NULL. */

@@ -29,6 +29,11 @@ int ap_device_malloc(size_t bytes, void** out_ptr);
 int ap_device_free(void* ptr);
 /* Synchronous copy of `bytes` (kind = AP_COPY_*). */
 int ap_memcpy(void* dst, const void* src, size_t bytes, int kind);
+/* Stream-ordered device-to-device copy by a plain streaming kernel (16 B per
+ * lane; pointers 16-byte aligned). Reads each source byte and writes each
+ * destination byte once, so it is also the known-size reference for scaling
+ * HBM traffic counters. */
+int ap_device_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* Stream-ordered fill of `bytes` with the byte `value`. */
 int ap_memset_async(void* dst, int value, size_t bytes, void* stream);
 int ap_stream_create(void** out_stream);
