@@ -137,7 +137,7 @@ _lib = None
 # Every symbol include/aperture/ap_core.h and ap_synth.h declare.
 EXPORTED_SYMBOLS = (
     "ap_scene_create", "ap_build_bvh", "ap_scene_destroy", "ap_scene_get_info", "ap_scene_set_traversal_mode",
-    "ap_scene_set_launch", "ap_scene_get_launch",
+    "ap_scene_set_launch", "ap_scene_get_launch", "ap_scene_autotune",
     "ap_scene_export_bvh",
     "ap_trace_closest", "ap_trace_any", "ap_trace_closest_stats", "ap_last_error_string",
     "ap_synth_triangle_soup", "ap_synth_camera_rays", "ap_synth_secondary_rays",
@@ -166,6 +166,7 @@ def lib():
         L.ap_scene_set_traversal_mode.argtypes = [vp, i32]
         L.ap_scene_set_launch.argtypes = [vp, i32, i32]
         L.ap_scene_get_launch.argtypes = [vp, vp, vp]
+        L.ap_scene_autotune.argtypes = [vp, vp, sz, vp, i32, vp, vp, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
             getattr(L, name).argtypes = [vp, vp, sz, vp, vp]
@@ -476,6 +477,15 @@ class SceneRenderer:
         w, r = ctypes.c_int32(), ctypes.c_int32()
         check(lib().ap_scene_get_launch(self.handle, ctypes.byref(w), ctypes.byref(r)))
         return w.value, r.value
+
+    def autotune(self, rays, hits, n=None, any_hit=False, stream=None):
+        """Times the candidate launch shapes on a device ray batch, keeps the
+        fastest and returns (wave_rays, refill, ms)."""
+        n = n if n is not None else _numel_rays(rays)
+        w, r, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+        check(lib().ap_scene_autotune(self.handle, _ptr(rays), n, _ptr(hits), 1 if any_hit else 0, _stream(stream),
+                                      ctypes.byref(w), ctypes.byref(r), ctypes.byref(ms)))
+        return w.value, r.value, ms.value
 
     def export_bvh(self):
         inf = self.info()

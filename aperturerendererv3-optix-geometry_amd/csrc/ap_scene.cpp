@@ -301,6 +301,60 @@ int ap_scene_get_launch(const ap_scene* scene, int32_t* wave_rays, int32_t* refi
     return AP_OK;
 }
 
+int ap_scene_autotune(ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hits, int any_hit, void* stream,
+                      int32_t* best_wave_rays, int32_t* best_refill, float* best_ms) {
+    if (!scene) return fail(AP_ERR_INVALID_ARG, "ap_scene_autotune: NULL scene");
+    if (n == 0 || !rays || !hits) return fail(AP_ERR_INVALID_ARG, "ap_scene_autotune: empty or NULL ray batch");
+    static const int32_t kWaveRays[] = {128, 256, 512, 1024};
+    static const int32_t kRefill[] = {4, 8, 16};
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(he, "ap_scene_autotune: hipEventCreate");
+    }
+    const DeviceScene saved = scene->dev;
+    float best = -1.0f;
+    int32_t bw = saved.wave_rays, br = saved.refill;
+    for (int32_t wr : kWaveRays) {
+        for (int32_t rf : kRefill) {
+            scene->dev.wave_rays = wr;
+            scene->dev.refill = rf;
+            // One untimed launch, then the fastest of two timed launches.
+            he = launch_trace(scene->dev, rays, n, hits, nullptr, any_hit != 0, st);
+            float shape_ms = -1.0f;
+            for (int rep = 0; rep < 2 && he == hipSuccess; ++rep) {
+                he = hipEventRecord(e0, st);
+                if (he == hipSuccess) he = launch_trace(scene->dev, rays, n, hits, nullptr, any_hit != 0, st);
+                if (he == hipSuccess) he = hipEventRecord(e1, st);
+                if (he == hipSuccess) he = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+                if (he == hipSuccess && (shape_ms < 0.0f || ms < shape_ms)) shape_ms = ms;
+            }
+            if (he != hipSuccess) break;
+            if (best < 0.0f || shape_ms < best) {
+                best = shape_ms;
+                bw = wr;
+                br = rf;
+            }
+        }
+        if (he != hipSuccess) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    scene->dev = saved;
+    if (he != hipSuccess) return hip_fail(he, "ap_scene_autotune");
+    scene->dev.wave_rays = bw;
+    scene->dev.refill = br;
+    if (best_wave_rays) *best_wave_rays = bw;
+    if (best_refill) *best_refill = br;
+    if (best_ms) *best_ms = best;
+    return AP_OK;
+}
+
 int ap_scene_set_traversal_mode(ap_scene* s, int mode) {
     if (!s) return fail(AP_ERR_INVALID_ARG, "ap_scene_set_traversal_mode: NULL scene");
     if (mode != AP_TRAVERSE_REFERENCE && mode != AP_TRAVERSE_CULLED)

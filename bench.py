@@ -97,6 +97,7 @@ def main():
     ap_.add_argument("--res", type=int, default=4096)
     ap_.add_argument("--cpu-seconds", type=float, default=20.0)
     ap_.add_argument("--no-cpu", action="store_true")
+    ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap_.add_argument("--sweep", action="store_true",
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
     ap_.add_argument("--copy-probe", action="store_true",
@@ -150,12 +151,24 @@ def main():
                       "tests_per_ray": prims / n}
     log(f"visit counts: {algo}")
 
+    # Launch shape per ray set, picked by the library's autotuner (untimed).
+    shapes = {}
+    if args.no_tune:
+        shapes = {"primary": scene.get_launch(), "secondary": scene.get_launch()}
+    else:
+        for name, d_r, d_h, n in (("primary", d_prim, d_phit, n_prim), ("secondary", d_sec, d_shit, n_sec)):
+            wr, rf, ms = scene.autotune(d_r, d_h, n=n, stream=stream)
+            shapes[name] = (wr, rf)
+            log(f"autotune {name}: {wr} rays per wave, refill at {rf} idle lanes ({ms:.3f} ms)")
+
     def step(evs=None):
         if evs is not None:
             evs[0].record(stream)
+        scene.set_launch(*shapes["primary"])
         scene.trace_closest(d_prim, d_phit, n=n_prim, stream=stream)
         if evs is not None:
             evs[1].record(stream)
+        scene.set_launch(*shapes["secondary"])
         scene.trace_closest(d_sec, d_shit, n=n_sec, stream=stream)
         if evs is not None:
             evs[2].record(stream)
@@ -235,6 +248,7 @@ def main():
             "primary_mrays_s": round(n_prim / t_prim / 1e3, 1),
             "secondary_mrays_s": round(n_sec / t_sec / 1e3, 1),
             "traversal_mode": "culled",
+            "launch": {k: {"wave_rays": v[0], "refill": v[1]} for k, v in shapes.items()},
         },
         "roofline": {
             "bound": "hbm",

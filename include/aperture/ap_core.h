@@ -118,6 +118,16 @@ int ap_scene_set_traversal_mode(ap_scene* scene, int mode);
 int ap_scene_set_launch(ap_scene* scene, int32_t wave_rays, int32_t refill);
 int ap_scene_get_launch(const ap_scene* scene, int32_t* wave_rays, int32_t* refill);
 
+/* Autotuner of the launch shape: traces the caller's device ray batch
+ * (closest hit, or any hit when any_hit != 0) into `hits` under each
+ * candidate shape (rays per wave 128/256/512/1024 x refill 4/8/16; one
+ * untimed launch, then the faster of two launches timed with HIP events on
+ * `stream`), keeps the fastest on the scene and reports it. Synchronises
+ * `stream`. The hits written are those of an ordinary trace. No reference
+ * counterpart. */
+int ap_scene_autotune(ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hits, int any_hit, void* stream,
+                      int32_t* best_wave_rays, int32_t* best_refill, float* best_ms);
+
 /* Copies out the reference-layout BVH (BVHHost::nodes_, objectIndices_,
  * bvh.cuh:54-58) and light table (AdditionalLightInfo[], scene.cu:1056-1080)
  * to host arrays sized from ap_scene_get_info. Any pointer may be NULL. */

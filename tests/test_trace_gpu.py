@@ -128,3 +128,34 @@ def test_benchmark_size_sampled(gpu):
         assert np.array_equal(anyh["id"] >= 0, h["id"] >= 0), name
         assert (anyh["t"][h["id"] >= 0] >= h["t"][h["id"] >= 0]).all(), name
     scene.close()
+
+
+def test_launch_shapes_and_autotuner_keep_results(gpu):
+    """Every launch shape (rays per wave, refill threshold) returns the same
+    hits and visit counts; the autotuner picks one of its candidates and its
+    hits are an ordinary trace's."""
+    import gpu_util
+
+    objs = ap.synth_triangle_soup(30000, seed=4)
+    rays = np.concatenate([ap.synth_camera_rays(64, 64, [0.5, 0.5, -1.5], [0, 0, 1], 40.0),
+                           scenes.random_rays(5000, seed=8)])
+    scene = ap.SceneRenderer(objs)
+    print(f"[launch] {len(objs)} triangles, {len(rays)} rays", flush=True)
+    ref = gpu_util.trace(scene, rays, stats=True)
+    for wr, rf in ((64, 1), (64, 64), (128, 4), (1024, 16), (4096, 64), (65536, 1)):
+        scene.set_launch(wr, rf)
+        assert scene.get_launch() == (wr, rf)
+        got = gpu_util.trace(scene, rays, stats=True)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (wr, rf)
+        anyh = gpu_util.trace(scene, rays, any_hit=True)
+        assert np.array_equal(anyh["id"] >= 0, ref[0]["id"] >= 0)
+    d_rays = ap.DeviceBuffer.from_numpy(np.ascontiguousarray(rays, dtype=ap.RAY_DTYPE))
+    d_hits = ap.DeviceBuffer(len(rays) * 8)
+    wr, rf, ms = scene.autotune(d_rays, d_hits, n=len(rays))
+    assert wr in (128, 256, 512, 1024) and rf in (4, 8, 16) and ms > 0.0
+    assert scene.get_launch() == (wr, rf)
+    assert np.array_equal(d_hits.to_numpy(ap.HIT_DTYPE, len(rays)), ref[0])
+    for bad in ((63, 0), (-64, 0), (0, 65), (131072, 0)):
+        with pytest.raises(ap.ApertureError):
+            scene.set_launch(*bad)
+    scene.close()
