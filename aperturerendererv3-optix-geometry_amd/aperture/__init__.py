@@ -120,7 +120,7 @@ class RenderParams(ctypes.Structure):
 
 class Framebuffers(ctypes.Structure):
     _fields_ = [("hdr", ctypes.c_void_p), ("odd", ctypes.c_void_p), ("even", ctypes.c_void_p),
-                ("sum_xyz", ctypes.c_void_p), ("primary_id", ctypes.c_void_p)]
+                ("sum_xyz", ctypes.c_void_p), ("primary_id", ctypes.c_void_p), ("ray_count", ctypes.c_void_p)]
 
 
 TRIANGLE_MATERIAL_OBJECT, TRIANGLE_MATERIAL_REFERENCE = 0, 1
@@ -148,6 +148,7 @@ EXPORTED_SYMBOLS = (
     "ap_spectrum_setup", "ap_spectrum_encode_rgb", "ap_spectrum_encode_lambda", "ap_spectrum_query_rgb",
     "ap_spectrum_query_lambda", "ap_spectrum_interact", "ap_spectrum_sample_lambda",
     "ap_render_params_default", "ap_render", "ap_render_offline", "ap_xyz_to_srgb", "ap_synth_box_scene",
+    "ap_synth_heightfield_scene",
 )
 
 
@@ -206,6 +207,8 @@ def lib():
         L.ap_render_offline.argtypes = [vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), vp]
         L.ap_xyz_to_srgb.argtypes = [vp, sz, vp, vp]
         L.ap_synth_box_scene.argtypes = [ctypes.c_uint64, i32, ctypes.c_float, vp, sz, ctypes.POINTER(sz)]
+        L.ap_synth_heightfield_scene.argtypes = [i32, i32, ctypes.c_uint64, ctypes.c_float, vp, sz,
+                                                 ctypes.POINTER(sz)]
         for name in EXPORTED_SYMBOLS:
             if name in ("ap_render_params_default",):
                 getattr(L, name).restype = None
@@ -375,6 +378,15 @@ def synth_box_scene(seed=1, sphere_subdiv=3, light=15.0):
     return objs
 
 
+def synth_heightfield_scene(grid=1024, n_spheres=256, seed=4, light=20.0):
+    """Heightfield + spheres + 4 lights (ap_synth_heightfield_scene)."""
+    n = ctypes.c_size_t(0)
+    check(lib().ap_synth_heightfield_scene(grid, n_spheres, seed, light, None, 0, ctypes.byref(n)))
+    objs = np.zeros(n.value, dtype=SCENE_OBJECT_DTYPE)
+    check(lib().ap_synth_heightfield_scene(grid, n_spheres, seed, light, _ptr(objs), n.value, ctypes.byref(n)))
+    return objs
+
+
 def spectrum_setup():
     check(lib().ap_spectrum_setup())
 
@@ -518,9 +530,10 @@ class SceneRenderer:
                                             offset, _ptr(out), ctypes.byref(n_out)))
         return out[: n_out.value]
 
-    def render(self, camera, params, hdr, odd=None, even=None, sum_xyz=None, primary_id=None, stream=None):
+    def render(self, camera, params, hdr, odd=None, even=None, sum_xyz=None, primary_id=None, stream=None,
+               ray_count=None):
         """SceneRenderer::Render into caller-owned device framebuffers (DeviceBuffer or addresses)."""
-        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id))
+        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id), _ptr(ray_count))
         check(lib().ap_render(self.handle, ctypes.byref(camera), ctypes.byref(params), ctypes.byref(fb),
                               _stream(stream)))
 

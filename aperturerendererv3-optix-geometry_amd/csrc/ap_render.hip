@@ -357,9 +357,11 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     const float* __restrict__ eps, PathState* __restrict__ paths, const ap_ray* __restrict__ rays_in,
     const int32_t* __restrict__ path_in, const ap_hit* __restrict__ hits, ap_ray* __restrict__ rays_out,
     int32_t* __restrict__ path_out, ap_ray* __restrict__ shadow_rays, ShadowPayload* __restrict__ shadow_pay,
-    uint32_t* __restrict__ counters, int32_t* __restrict__ primary_id, int32_t primary_frame) {
+    uint32_t* __restrict__ counters, int32_t* __restrict__ primary_id, int32_t primary_frame,
+    unsigned long long* __restrict__ ray_count) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
+    if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
     if (i >= n) return;
     const int32_t pid = path_in[i];
     PathState S = paths[pid];
@@ -547,8 +549,10 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
 __global__ __launch_bounds__(kBlock) void resolve_kernel(const uint32_t* __restrict__ counters,
                                                          const ShadowPayload* __restrict__ pay,
                                                          const ap_hit* __restrict__ shits,
-                                                         PathState* __restrict__ paths) {
+                                                         PathState* __restrict__ paths,
+                                                         unsigned long long* __restrict__ ray_count) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (ray_count && i == 0) atomicAdd(&ray_count[1], static_cast<unsigned long long>(counters[2]));
     if (i >= counters[2]) return;
     const ShadowPayload p = pay[i];
     const bool occluded = shits[i].id >= 0;
@@ -868,11 +872,11 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
             hipLaunchKernelGGL(shade_kernel, grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats, r->lights,
                                s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q], r->path_of[in_q], r->hits,
                                r->rays[out_q], r->path_of[out_q], r->shadow_rays, r->shadow_payload, r->counters,
-                               fb->primary_id, last_frame);
+                               fb->primary_id, last_frame, reinterpret_cast<unsigned long long*>(fb->ray_count));
             e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
             hipLaunchKernelGGL(resolve_kernel, grid_of(total), dim3(kBlock), 0, stream, r->counters, r->shadow_payload,
-                               r->shadow_hits, r->paths);
+                               r->shadow_hits, r->paths, reinterpret_cast<unsigned long long*>(fb->ray_count));
             in_q = out_q;
             // Deep limits: stop once every path has ended.
             if (prm->max_depth > 32 && (depth % 16) == 15) {

@@ -123,6 +123,40 @@ static void triangle_object(ap_scene_object& o, f3 p0, f3 p1, f3 p2, f3 n0, f3 n
     update_proxy(o);
 }
 
+// Value noise on the integer lattice: a SplitMix64 hash of (ix, iz, octave,
+// seed) mapped to [-1, 1], blended with the quintic fade 6t^5 - 15t^4 + 10t^3.
+static float lattice(int64_t ix, int64_t iz, int octave, uint64_t seed) {
+    uint64_t h = splitmix64(seed ^ splitmix64(static_cast<uint64_t>(ix) * 0x9E3779B97F4A7C15ULL ^
+                                              (static_cast<uint64_t>(iz) << 21) ^ (static_cast<uint64_t>(octave) << 58)));
+    return static_cast<float>(h >> 40) * 0x1p-23f - 1.0f;
+}
+
+static float fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+
+static float value_noise(float x, float z, int octave, uint64_t seed) {
+    const float fx = floorf(x), fz = floorf(z);
+    const int64_t ix = static_cast<int64_t>(fx), iz = static_cast<int64_t>(fz);
+    const float tx = fade(x - fx), tz = fade(z - fz);
+    const float a = lattice(ix, iz, octave, seed), b = lattice(ix + 1, iz, octave, seed);
+    const float c = lattice(ix, iz + 1, octave, seed), d = lattice(ix + 1, iz + 1, octave, seed);
+    const float ab = a + (b - a) * tx, cd = c + (d - c) * tx;
+    return ab + (cd - ab) * tz;
+}
+
+// Fractal Brownian motion: octaves of value noise at base frequency 2 over
+// [-1, 1]^2, doubling frequency and halving amplitude, scaled so the octave
+// amplitudes sum to `amplitude`.
+static float fbm(float x, float z, int octaves, float amplitude, uint64_t seed) {
+    float sum = 0.0f, amp = 1.0f, norm = 0.0f, freq = 2.0f;
+    for (int o = 0; o < octaves; ++o) {
+        sum += amp * value_noise(x * freq, z * freq, o, seed);
+        norm += amp;
+        amp *= 0.5f;
+        freq *= 2.0f;
+    }
+    return amplitude * sum / norm;
+}
+
 }  // namespace ap
 
 using namespace ap;
@@ -213,6 +247,102 @@ int ap_synth_box_scene(uint64_t seed, int sphere_subdiv, float light, ap_scene_o
             const f3 n0 = sv[st[t]], n1 = sv[st[t + 1]], n2 = sv[st[t + 2]];
             triangle_object(out[k++], c + n0 * 0.3f, c + n1 * 0.3f, c + n2 * 0.3f, n0, n1, n2, a);
         }
+    }
+    return AP_OK;
+}
+
+int ap_synth_heightfield_scene(int grid, int n_spheres, uint64_t seed, float light, ap_scene_object* out,
+                               size_t capacity, size_t* n_out) {
+    if (!n_out || grid < 1 || grid > 8192 || n_spheres < 0 || n_spheres > (1 << 20))
+        return fail(AP_ERR_INVALID_ARG, "ap_synth_heightfield_scene: bad arguments");
+    std::vector<f3> sv;
+    std::vector<int> st;
+    icosphere(2, sv, st);
+    const size_t per_sphere = st.size() / 3;
+    const size_t n_field = 2 * static_cast<size_t>(grid) * grid;
+    const size_t n = n_field + static_cast<size_t>(n_spheres) * per_sphere + 4;
+    *n_out = n;
+    if (!out) return AP_OK;
+    if (capacity < n) return fail(AP_ERR_INVALID_ARG, "ap_synth_heightfield_scene: capacity too small");
+    constexpr int kOctaves = 6;
+    constexpr float kAmplitude = 0.2f;
+    const uint64_t noise_seed = seed;
+    const int nv = grid + 1;
+    const float cell = 2.0f / static_cast<float>(grid);
+    std::vector<f3> pos(static_cast<size_t>(nv) * nv), nrm(pos.size());
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < nv; ++i) {
+            const float x = -1.0f + cell * static_cast<float>(i), z = -1.0f + cell * static_cast<float>(j);
+            pos[static_cast<size_t>(j) * nv + i] = f3{x, fbm(x, z, kOctaves, kAmplitude, noise_seed), z};
+        }
+    // Vertex normals from central differences of the height samples.
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < nv; ++i) {
+            const int i0 = i > 0 ? i - 1 : i, i1 = i < grid ? i + 1 : i;
+            const int j0 = j > 0 ? j - 1 : j, j1 = j < grid ? j + 1 : j;
+            const float dx = pos[static_cast<size_t>(j) * nv + i1].y - pos[static_cast<size_t>(j) * nv + i0].y;
+            const float dz = pos[static_cast<size_t>(j1) * nv + i].y - pos[static_cast<size_t>(j0) * nv + i].y;
+            nrm[static_cast<size_t>(j) * nv + i] =
+                normalize(f3{-dx / (cell * static_cast<float>(i1 - i0)), 1.0f, -dz / (cell * static_cast<float>(j1 - j0))});
+        }
+    // Materials: per mesh, Lambert or GGX with equal odds, roughness U[0.05, 1],
+    // albedo U[0.05, 0.95] per channel, from PCG32(seed + 1).
+    Pcg32 mrng = pcg32_seed(seed + 1, 0);
+    struct Mat { int model; float rough; f3 albedo; };
+    auto draw_mat = [&]() {
+        Mat m;
+        m.model = pcg32_float(mrng) < 0.5f ? AP_MAT_DIFFUSE : AP_MAT_GGX;
+        m.rough = 0.05f + 0.95f * pcg32_float(mrng);
+        const float r = 0.05f + 0.9f * pcg32_float(mrng);
+        const float g = 0.05f + 0.9f * pcg32_float(mrng);
+        const float b = 0.05f + 0.9f * pcg32_float(mrng);
+        m.albedo = f3{r, g, b};
+        return m;
+    };
+    auto apply = [](ap_scene_object& o, const Mat& m) {
+        o.material.shading_model = m.model;
+        o.material.texture.base.roughness = m.rough;
+        o.material.texture.base.specular = 0.5f;
+        o.material.texture.base.metallic = 0.0f;
+    };
+    const Mat field = draw_mat();
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < grid; ++j)
+        for (int i = 0; i < grid; ++i) {
+            const size_t a = static_cast<size_t>(j) * nv + i, b = a + 1, c = a + nv, d = c + 1;
+            const size_t k = 2 * (static_cast<size_t>(j) * grid + i);
+            triangle_object(out[k], pos[a], pos[c], pos[d], nrm[a], nrm[c], nrm[d], field.albedo);
+            triangle_object(out[k + 1], pos[a], pos[d], pos[b], nrm[a], nrm[d], nrm[b], field.albedo);
+            apply(out[k], field);
+            apply(out[k + 1], field);
+        }
+    // Spheres resting on the surface: centre (x, h(x, z) + r, z), radius
+    // log-uniform in [0.01, 0.1], from PCG32(seed + 2).
+    Pcg32 srng = pcg32_seed(seed + 2, 0);
+    size_t k = n_field;
+    for (int sidx = 0; sidx < n_spheres; ++sidx) {
+        const float r = 0.01f * expf(logf(10.0f) * pcg32_float(srng));
+        const float x = -0.9f + 1.8f * pcg32_float(srng), z = -0.9f + 1.8f * pcg32_float(srng);
+        const f3 c{x, fbm(x, z, kOctaves, kAmplitude, noise_seed) + r, z};
+        const Mat m = draw_mat();
+        for (size_t t = 0; t < st.size(); t += 3) {
+            const f3 n0 = sv[st[t]], n1 = sv[st[t + 1]], n2 = sv[st[t + 2]];
+            triangle_object(out[k], c + n0 * r, c + n1 * r, c + n2 * r, n0, n1, n2, m.albedo);
+            apply(out[k], m);
+            ++k;
+        }
+    }
+    // Four thin-cube area lights (0.3 x 0.3) over the field.
+    const f3 lights[4] = {{-0.5f, 1.2f, -0.5f}, {0.5f, 1.3f, -0.4f}, {-0.4f, 1.25f, 0.5f}, {0.45f, 1.15f, 0.45f}};
+    for (const f3& c : lights) {
+        ap_scene_object& L = out[k++];
+        init_object(L, c, f3{0.15f, 0.005f, 0.15f}, AP_OBJ_CUBE);
+        L.material.shading_model = AP_MAT_LIGHT;
+        L.material.texture.base.albedo = {0.0f, 0.0f, 0.0f};
+        L.material.texture.base.emissive = {light, light, light};
+        update_proxy(L);
     }
     return AP_OK;
 }

@@ -88,6 +88,65 @@ def cpu_baseline(objs, rays_a, rays_b, seconds, threads):
     }
 
 
+def render_bench(ap, spp, cpu_seconds, threads):
+    """Samples/s of the wavefront path tracer at 1920x1080 on the heightfield
+    workload (1024^2-cell fBm heightfield, 2,097,152 triangles, plus 256
+    320-triangle icospheres and 4 thin-cube lights; Lambert/GGX 50/50), max
+    depth 16, Russian roulette from depth 4, sky on, guiding off. Frames are
+    timed with HIP events on the render stream after one untimed frame."""
+    t0 = time.perf_counter()
+    objs = ap.synth_heightfield_scene(1024, 256, 4, 20.0)
+    scene = ap.SceneRenderer(objs)
+    log(f"render scene: {len(objs)} objects ({time.perf_counter() - t0:.1f} s)")
+    W, H = 1920, 1080
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, W, H)
+    stream = ap.Stream()
+    hdr = ap.DeviceBuffer(W * H * 12)
+    hdr.zero(stream)
+    counts = ap.DeviceBuffer(16)
+    params = dict(max_depth=16, rr_min_depth=4)
+    scene.render(cam, ap.RenderParams.defaults(spp=1, **params), hdr, stream=stream)
+    stream.synchronize()
+    counts.zero(stream)
+    e0, e1 = ap.Event(), ap.Event()
+    e0.record(stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, **params), hdr, stream=stream, ray_count=counts)
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1)
+    rays = counts.to_numpy(np.uint64, 2)
+    out = {
+        "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 lights, 1920x1080, "
+                    "max depth 16, RR from depth 4, one wavelength per path, sky on, guiding off",
+        "spp": spp,
+        "ms_per_spp": round(ms / spp, 3),
+        "samples_per_s": round(W * H * spp / (ms / 1e3), 1),
+        "mrays_per_s": round(float(rays[0] + rays[1]) / (ms / 1e3) / 1e6, 1),
+        "rays_per_sample": round(float(rays[0] + rays[1]) / (W * H * spp), 3),
+    }
+    log(f"render: {out}")
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib as ol
+
+        bvh = ol.OracleBVH(objs)
+        margin = np.float32(scene.info().line_margin)
+        crop = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0 * 48 / 1080, 48, 48)
+        t0 = time.perf_counter()
+        bvh.render(crop, ap.RenderParams.defaults(spp=1, **params), cull_margin=margin, threads=threads)
+        dt = time.perf_counter() - t0
+        spp_cpu = max(1, int(cpu_seconds / max(dt, 1e-3)))
+        t0 = time.perf_counter()
+        bvh.render(crop, ap.RenderParams.defaults(spp=spp_cpu, frame_index=1, **params), cull_margin=margin,
+                   threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_samples_per_s"] = round(48 * 48 * spp_cpu / dt, 1)
+        out["cpu_sample"] = (f"oracle restatement of the reference integrator with the core's culled traversal, "
+                             f"{threads} threads, 48x48 centre crop x {spp_cpu} spp, {dt:.1f} s")
+    scene.close()
+    return out
+
+
 def main():
     ap_ = argparse.ArgumentParser()
     ap_.add_argument("--gpus", type=int, default=1)
@@ -97,6 +156,7 @@ def main():
     ap_.add_argument("--res", type=int, default=4096)
     ap_.add_argument("--cpu-seconds", type=float, default=20.0)
     ap_.add_argument("--no-cpu", action="store_true")
+    ap_.add_argument("--render-spp", type=int, default=8, help="spp of the 1080p render measurement (0 skips it)")
     ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap_.add_argument("--sweep", action="store_true",
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
@@ -264,11 +324,14 @@ def main():
         },
         "cpu_baseline": None,
     }
+    threads = min(os.cpu_count() or 1, 16)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
         line["cpu_baseline"] = cpu_baseline(objs, prim, sec, args.cpu_seconds, threads)
-    print(json.dumps(line), flush=True)
     scene.close()
+    del objs, prim, sec
+    if args.render_spp > 0:
+        line["render"] = render_bench(ap, args.render_spp, 0.0 if args.no_cpu else 10.0, threads)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

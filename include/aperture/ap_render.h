@@ -56,6 +56,7 @@ typedef struct ap_framebuffers {
     float* even;         /* W*H*3 running mean of even frames */
     double* sum_xyz;     /* W*H*3 FP64 per-pixel sums of the clamped XYZ samples (added) */
     int32_t* primary_id; /* W*H closest-hit objectIndex_ of the camera ray of the last sample, -1 on miss */
+    uint64_t* ray_count; /* 2 counters, added to: rays traced by closest-hit [0] and shadow [1] launches */
 } ap_framebuffers;
 
 /* Reference defaults (config.cuh), spp = 1, frame 0, seed 0. */

@@ -49,6 +49,18 @@ int ap_synth_secondary_rays(const ap_scene* scene, const ap_ray* primaries, cons
 int ap_synth_box_scene(uint64_t seed, int sphere_subdiv, float light, ap_scene_object* out, size_t capacity,
                        size_t* n_out);
 
+/* Heightfield scene: a grid x grid cell heightfield over [-1,1]^2 in x/z
+ * (2 grid^2 triangles, vertex normals from central differences), height a
+ * 6-octave value-noise fBm of amplitude 0.2 (lattice values from SplitMix64
+ * of the seed); n_spheres icospheres of subdivision 2 (320 triangles each),
+ * radius log-uniform in [0.01, 0.1], resting on the surface; four 0.3 x 0.3
+ * thin-cube lights of radiance `light` 1.15-1.3 above it. Materials per mesh:
+ * Lambert or GGX with equal odds, roughness U[0.05, 1], albedo U[0.05, 0.95]
+ * per channel. Writes up to `capacity` objects and the count to *n_out
+ * (2 grid^2 + 320 n_spheres + 4); out = NULL only reports the count. */
+int ap_synth_heightfield_scene(int grid, int n_spheres, uint64_t seed, float light, ap_scene_object* out,
+                               size_t capacity, size_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,6 +123,15 @@ def test_deep_paths_and_ggx(gpu):
     compare_with_oracle(objs, cam, box_params(spp=16, depth=48), "GGX mix, depth 48")
 
 
+def test_heightfield_scene_matches_oracle(gpu):
+    """The heightfield workload at a small grid: fBm field, icospheres,
+    Lambert/GGX materials, four lights, depth 16 with roulette from depth 4."""
+    objs = ap.synth_heightfield_scene(64, 16, 4, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 64, 36)
+    compare_with_oracle(objs, cam, ap.RenderParams.defaults(spp=8, max_depth=16, rr_min_depth=4),
+                        "heightfield 64^2 + 16 spheres")
+
+
 def test_sums_odd_even_and_progressive(gpu):
     objs = ap.synth_box_scene(1, 1, 15.0)
     cam = ap.Camera.make(BOX_CAM["origin"], BOX_CAM["direction"], BOX_CAM["fov_deg"], 40, 24)
