@@ -22,10 +22,12 @@
 //             computer scientist should know about floating-point arithmetic").
 //   erfc      the Chebyshev-fitted erfc of Numerical Recipes in C, 2nd ed.,
 //             sec. 6.2 (fractional error below 1.2e-7).
+//   atan2     Cephes atanf reduction and polynomial, quadrant rules of
+//             its atan2f.
 //   x^0.75    sqrt(x) * sqrt(sqrt(x)).
 // Accuracy against float64 (tests/test_fmath_cpu.py): sin and cos within
 // 1.6 ulp on [-500, 500], exp and log within 1.1 ulp, log1p and x^0.75
-// within 2.5 ulp, erfc within 3e-6 relative on [-4, 6].
+// within 2.5 ulp, erfc within 3e-6 relative on [-4, 6], atan2 within 3 ulp.
 #pragma once
 
 #include <cmath>
@@ -191,6 +193,44 @@ APFM_HD float erfc(float x) {
     p = fmaf(t, p, -1.26551223f);
     const float ans = t * exp(fmaf(-z, z, p));
     return x >= 0.0f ? ans : 2.0f - ans;
+}
+
+// atan on [0, inf) by the Cephes atanf reduction (tan(3pi/8), tan(pi/8)
+// thresholds) and its polynomial; odd extension.
+APFM_HD float atan(float x) {
+    if (x != x) return x;
+    const bool neg = x < 0.0f;
+    float a = neg ? -x : x;
+    float y0 = 0.0f;
+    if (a > 2.414213562373095f) {
+        y0 = 1.5707963267948966f;
+        a = -1.0f / a;
+    } else if (a > 0.4142135623730950f) {
+        y0 = 0.7853981633974483f;
+        a = (a - 1.0f) / (a + 1.0f);
+    }
+    const float z = a * a;
+    float p = fmaf(8.05374449538e-2f, z, -1.38776856032e-1f);
+    p = fmaf(p, z, 1.99777106478e-1f);
+    p = fmaf(p, z, -3.33329491539e-1f);
+    const float r = y0 + fmaf(p * z, a, a);
+    return neg ? -r : r;
+}
+
+// atan2 with the quadrant rules of the Cephes atan2f: x = 0 gives +-pi/2
+// (0 when y = 0 too), y = 0 gives 0 or pi.
+APFM_HD float atan2(float y, float x) {
+    if (x != x || y != y) return x + y;
+    const float pi = 3.14159265358979323846f;
+    if (x == 0.0f) {
+        if (y < 0.0f) return -0.5f * pi;
+        if (y == 0.0f) return 0.0f;
+        return 0.5f * pi;
+    }
+    if (y == 0.0f) return x < 0.0f ? pi : 0.0f;
+    float w = 0.0f;
+    if (x < 0.0f) w = y < 0.0f ? -pi : pi;
+    return w + atan(y / x);
 }
 
 // x^0.75 for x >= 0.

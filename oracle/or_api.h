@@ -29,6 +29,22 @@ float or_object_t(const ap_scene_object* objs, int idx, const ap_ray* r);
 
 /* Integrator restatement (or_render.cpp): frames frame_index .. +spp-1 into
  * hdr (running mean, in place), sums (added) and primary ids (last frame). */
+typedef struct or_guide or_guide;
+void or_pg_uv_to_sphere(const float* uv, size_t n, float* out);
+void or_pg_sphere_to_uv(const float* d, size_t n, float* out);
+void or_pg_voxel(const float* pos, const float* dir, size_t n, int32_t* out);
+void or_pg_mipmaps(ap_octahedron* p);
+void or_pg_pdf(const ap_octahedron* p, const float* d, size_t n, const float* q, const float* normal, float* out);
+void or_pg_sample(const ap_octahedron* p, uint64_t seed, size_t n, const float* q, const float* normal, float* dirs,
+                  float* pdfs);
+void or_pg_update(or_guide* g, const ap_path_guiding_sample* slots, size_t n);
+or_guide* or_guide_create(void);
+void or_guide_free(or_guide* g);
+size_t or_guide_probe_count(const or_guide* g);
+void or_guide_export(const or_guide* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds);
+int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                     const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
+                     int threads, or_guide* guide);
 int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
               const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
               int threads);

@@ -31,6 +31,7 @@
 #include "ap_fmath.h"
 #include "or_math.h"
 #include "or_spectrum.h"
+#include "or_guiding.h"
 
 namespace orc {
 
@@ -560,10 +561,66 @@ struct RenderOpts {
     int max_depth, rr_min_depth, nee_max_depth, use_nee, nee_always, sky;
 };
 
+// Per-path radiance collection for guiding (CalculateRadiance's
+// radianceSample* arrays, render.cuh:478-520, and CollectRadiance, :353-432).
+struct Collect {
+    float3 dir[PG_COLLECT], loc[PG_COLLECT], vdir[PG_COLLECT];
+    float sample[PG_COLLECT], mask[PG_COLLECT];
+    bool valid[PG_COLLECT];
+    void Clear() {
+        for (int i = 0; i < PG_COLLECT; ++i) {
+            dir[i] = loc[i] = vdir[i] = float3{0, 0, 0};
+            sample[i] = mask[i] = 0.0f;
+            valid[i] = false;
+        }
+    }
+    void Endpoint(float le, int ndd) {
+        if (le == 0.0f) return;
+        const int hi = std::min(ndd, PG_COLLECT);
+        for (int i = 0; i < hi; ++i)
+            if (mask[i] != 0.0f) { sample[i] = sample[i] + mask[i] * le; valid[i] = true; }
+    }
+    void Advance(float3 next, float contrib, float rrProb, int ndd, bool allowStart, float3 rayDir, float3 pos,
+                 float3 nrm) {
+        if (allowStart && ndd < PG_COLLECT && mask[ndd] == 0.0f) {
+            dir[ndd] = next;
+            loc[ndd] = pos;
+            vdir[ndd] = Reflect(rayDir, nrm);
+            mask[ndd] = 1.0f;
+        }
+        const float invQ = static_cast<float>(1.0 / static_cast<double>(rrProb));
+        const int hi = std::min(ndd + (allowStart ? 1 : 0), PG_COLLECT);
+        for (int i = 0; i < hi; ++i)
+            if (mask[i] != 0.0f) mask[i] = mask[i] * (contrib * invQ);
+    }
+    // Path end (render.cuh:910-957): valid vertices to PathGuidingSamples.
+    void Emit(float3 xyz, float lpdf, ap_path_guiding_sample out[PG_COLLECT]) const {
+        std::memset(out, 0, sizeof(ap_path_guiding_sample) * PG_COLLECT);
+        int w = 0;
+        for (int i = 0; i < PG_COLLECT; ++i) {
+            if (w >= PG_COLLECT) break;
+            if (!valid[i]) continue;
+            const float3 rx = xyz * sample[i] / lpdf;
+            const float strength = std::fmin(max3(rx), PG_MAX_BRIGHT);
+            if (strength < PG_MIN_BRIGHT) continue;
+            ap_path_guiding_sample& o = out[w++];
+            o.direction = ap_float3{dir[i].x, dir[i].y, dir[i].z};
+            const float3 r = fmax3(rx, float3{0.0f, 0.0f, 0.0f});
+            o.radiance = ap_float3{r.x, r.y, r.z};
+            const int4_ v = GetVoxelGrid(loc[i], vdir[i]);
+            o.voxel_grid = ap_int4{v.x, v.y, v.z, v.w};
+            o.valid = 1;
+        }
+    }
+};
+
 // CalculateRadiance (render.cuh:460-999) for one camera ray; returns the
-// clamped XYZ sample and the primary hit id.
+// clamped XYZ sample and the primary hit id. With a guiding state G the
+// probes of G guide the BSDF sampling and NEE's MIS pdf, and the path's
+// guiding samples are written to `slots` (PG_COLLECT entries) if non-null.
 static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir, bool inside0,
-                                int* primary) {
+                                int* primary, const GuideState* G = nullptr, int frame = 0,
+                                ap_path_guiding_sample* slots = nullptr) {
     float lam, lpdf;
     {
         const float r0 = rng.rand1(), r1 = rng.rand1(), r2 = rng.rand1();
@@ -573,9 +630,13 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
     QueryLambda(lam, lb);
     const float d65 = D65Norm(lam);
     const float3 xyz = LambdaToXYZ(lam);
+    const bool guiding = G != nullptr;
+    Collect col;
+    if (guiding) col.Clear();
     int ndd = 0;
     dir = normalize(dir);
     float radiance = 0.0f, mask = 1.0f;
+    float3 query{1.0f, 1.0f, 1.0f};  // maskForProbeQuery
     float3 ro = origin, rd = dir;
     bool inside = inside0;
     bool prevNEE = false;
@@ -588,9 +649,16 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         float t;
         TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t);
         if (depth == 0 && primary) *primary = id;
+        const float alpha = GuidingAlpha(frame);
         if (id < 0) {
             (void)rng.rand1();  // offset draw
-            if (!inside) radiance = radiance + mask * (P.sky ? InteractC(lb, SkyLighting(rd)) : 0.0f) * d65;
+            float endpoint = 0.0f;
+            if (!inside) {
+                const float le = P.sky ? InteractC(lb, SkyLighting(rd)) : 0.0f;
+                radiance = radiance + mask * le * d65;
+                if (guiding) endpoint = le * d65;
+            }
+            if (guiding) col.Endpoint(endpoint, ndd);
             break;
         }
         Ctx c = HitContext(S, id, ro, rd, t);
@@ -600,6 +668,33 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         const float offEps = eps * std::fmax(1.0f, length(c.pos));
         c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
         inside = sc.inside;
+        // Directional guiding of the sampled direction (render.cuh:546-639).
+        const ap_octahedron* probe = nullptr;
+        if (guiding && !sc.delta && !sc.pureLight && !inside && !G->probes.empty()) {
+            probe = G->Query(c.pos, Reflect(rd, c.normal));
+            if (probe) {
+                const float pB = sc.pdf;
+                const float pP = OctahedronHemispherePDF(*probe, sc.next, c.normal, query);
+                if (pB > 1.0e-12f && pP > 1.0e-12f) {
+                    if (rng.rand1() < alpha) {
+                        const GuideSample gs = SampleOctahedronHemisphere(*probe, rng, c.normal, query);
+                        if (dot(gs.dir, c.normal) > 1.0e-12f) {
+                            const float pBS = BSDF_PDF(lb, model, c.gb, c.normal, rd, gs.dir);
+                            sc.next = gs.dir;
+                            sc.brdf = EvaluateBSDFWithCos(lb, model, rd, gs.dir, c);
+                            sc.pdf = (1.0f - alpha) * std::fmax(pBS, 1.0e-12f) + alpha * gs.pdf;
+                        } else {
+                            probe = nullptr;
+                        }
+                    } else {
+                        sc.pdf = (1.0f - alpha) * pB + alpha * pP;
+                    }
+                } else {
+                    probe = nullptr;
+                }
+            }
+        }
+        float endpoint = 0.0f;
         if (c.lightIndex >= 0) {
             float misW = 1.0f;
             if (P.use_nee && depth > 0 && prevNEE) {
@@ -608,7 +703,11 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                 if (pL >= 0.0f && pB > 0.0f) misW = PowerHeuristic(pB, pL);
             }
             radiance = radiance + mask * InteractC(lb, sc.emissive) * d65 * misW;
+            if (guiding) endpoint = InteractC(lb, sc.emissive * misW) * d65;
+        } else if (guiding && !(sc.emissive.x == 0.0f && sc.emissive.y == 0.0f && sc.emissive.z == 0.0f)) {
+            endpoint = InteractC(lb, sc.emissive) * d65;
         }
+        if (guiding) col.Endpoint(endpoint, ndd);
         if (sc.terminate) break;
         bool neeValid = false;
         const float rate = P.nee_always ? 1.0f : std::sqrt(saturate_(mask));
@@ -643,13 +742,17 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                     const float cosY = std::fmax(0.0f, dot(ls.normal, -L));
                     if (!(cosX <= 0.0f || cosY <= 0.0f)) {
                         const float bc = EvaluateBSDFWithCos(lb, model, rd, L, c);
-                        const float G = cosY / d2;
+                        const float G2 = cosY / d2;
                         const ap_gbuffer& lg = S.objs[ls.object].material.texture.base;
                         const float resp = InteractC(lb, F3(lg.emissive));
-                        const float contribution = bc * resp * (G / pAT);
+                        const float contribution = bc * resp * (G2 / pAT);
                         neeValid = true;
                         const float pLo = AreaToSolidAnglePDF(c.pos, ls.point, ls.normal, ls.pFacet * ls.pA);
-                        const float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
+                        float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
+                        if (probe) {
+                            const float pPr = OctahedronHemispherePDF(*probe, L, c.normal, query);
+                            pBs = (1.0f - alpha) * pBs + alpha * pPr;
+                        }
                         radiance = radiance + mask * (contribution * PowerHeuristic(pLo, pBs)) * d65;
                     }
                 }
@@ -665,12 +768,22 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             p = lerp(basic, 1.0f, inside ? 0.99f : 0.0f);
             if (rng.rand1() > p) break;
         }
+        if (guiding) {
+            const bool allowStart = !sc.delta && !sc.pureLight && !inside;
+            col.Advance(sc.next, contrib, p, ndd, allowStart, rd, c.pos, c.normal);
+        }
         mask = mask * contrib;
         mask = mask / p;
+        if (guiding) {
+            query = fmax3(xyz * mask, float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
+            // sum3 + PROBES_QUERY_EPS is a double sum (the constant is a double literal).
+            query = query / static_cast<float>(static_cast<double>(query.x + query.y + query.z) + 1.0e-6);
+        }
         ro = c.pos;
         rd = sc.next;
         if (!sc.delta) ndd++;
     }
+    if (guiding && slots) col.Emit(xyz, lpdf, slots);
     float r = radiance;
     if (std::isnan(r) || std::isinf(r)) r = 0.0f;
     const float3 pre = (xyz * r) / lpdf;
@@ -695,15 +808,19 @@ static float GetArea(const ap_scene_object& o) {
 
 using namespace orc;
 
+struct or_guide {
+    GuideState state;
+};
+
 extern "C" {
 
 // Offline render (frames frame0 .. frame0+spp-1) of the scene into
 // hdr (W*H*3 running mean, updated in place), sums (W*H*3 doubles, added)
 // and primary ids (W*H, last frame). cull_margin > 0 uses the core's CULLED
 // traversal order, which returns the same hits.
-int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
-              const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
-              int threads) {
+int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                     const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
+                     int threads, or_guide* guide) {
     Scene S{objs, n, bvh, {}, std::vector<int>(n, -1), prm->triangle_material, cull_margin};
     float total = 0.0f;
     for (size_t i = 0; i < n; ++i) {
@@ -738,6 +855,41 @@ int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap
     }
     const int W = cam->width, H = cam->height;
     const int64_t npix = static_cast<int64_t>(W) * H;
+    if (prm->use_guiding) {
+        // Frames in order; each renders with the probes built from the frames
+        // before it, then folds its own samples in (scene.cu:308-386).
+        if (!guide) return -1;
+        GuideState& G = guide->state;
+        std::vector<ap_path_guiding_sample> slots(static_cast<size_t>(npix) * PG_COLLECT);
+        for (int s = 0; s < prm->spp; ++s) {
+            const int frame = prm->frame_index + s;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+            for (int64_t p = 0; p < npix; ++p) {
+                const int i = static_cast<int>(p % W), j = static_cast<int>(p / W);
+                Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
+                const float u = (i + rng.rand1()) / static_cast<float>(W);
+                const float v = (j + rng.rand1()) / static_cast<float>(H);
+                const float nx = u * 2.f - 1.f;
+                const float ny = static_cast<float>((1.0 - v) * 2.f - 1.f);
+                const float3 rd = normalize(camD + right * (nx * tanX) + up * (ny * tanY));
+                int prim = -1;
+                float3 res = CalculateRadiance(S, P, rng, camO, rd, inside0, &prim, &G, frame,
+                                               &slots[static_cast<size_t>(p) * PG_COLLECT]);
+                res = fmax3(float3{0, 0, 0}, res);
+                const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
+                const float3 acc = lerp(prev, res, 1.0f / static_cast<float>(1 + frame));
+                hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
+                if (sums) {
+                    sums[3 * p] += static_cast<double>(res.x);
+                    sums[3 * p + 1] += static_cast<double>(res.y);
+                    sums[3 * p + 2] += static_cast<double>(res.z);
+                }
+                if (primary && s == prm->spp - 1) primary[p] = prim;
+            }
+            G.Update(slots);
+        }
+        return 0;
+    }
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
     for (int64_t p = 0; p < npix; ++p) {
         const int i = static_cast<int>(p % W), j = static_cast<int>(p / W);
@@ -764,6 +916,71 @@ int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap
         hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
     }
     return 0;
+}
+
+int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+              const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
+              int threads) {
+    if (prm->use_guiding) return -1;
+    return or_render_guided(bvh, objs, n, cam, prm, cull_margin, hdr, sums, primary, threads, nullptr);
+}
+
+// ---- guiding primitives, for their CPU tests ----
+void or_pg_uv_to_sphere(const float* uv, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const float3 d = UVtoSphere(uv[2 * i], uv[2 * i + 1]);
+        out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+    }
+}
+void or_pg_sphere_to_uv(const float* d, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) SphereToUV(float3{d[3 * i], d[3 * i + 1], d[3 * i + 2]}, out[2 * i], out[2 * i + 1]);
+}
+void or_pg_voxel(const float* pos, const float* dir, size_t n, int32_t* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const int4_ v = GetVoxelGrid(float3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                                     float3{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]});
+        out[4 * i] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = v.w;
+    }
+}
+void or_pg_mipmaps(ap_octahedron* p) { GenerateMipmaps(*p); }
+// pdf of directions d under probe p and query q; with a normal, the hemisphere pdf.
+void or_pg_pdf(const ap_octahedron* p, const float* d, size_t n, const float* q, const float* normal, float* out) {
+    const float3 qq{q[0], q[1], q[2]};
+    for (size_t i = 0; i < n; ++i) {
+        const float3 dd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+        out[i] = normal ? OctahedronHemispherePDF(*p, dd, float3{normal[0], normal[1], normal[2]}, qq)
+                        : OctahedronPDF(*p, dd, qq);
+    }
+}
+// n samples from PCG32(seed, 0); with a normal, the hemisphere sampler.
+void or_pg_sample(const ap_octahedron* p, uint64_t seed, size_t n, const float* q, const float* normal, float* dirs,
+                  float* pdfs) {
+    Rng rng = Rng::seeded(seed, 0);
+    const float3 qq{q[0], q[1], q[2]};
+    for (size_t i = 0; i < n; ++i) {
+        const GuideSample s = normal ? SampleOctahedronHemisphere(*p, rng, float3{normal[0], normal[1], normal[2]}, qq)
+                                     : SampleOctahedron(*p, rng, qq);
+        dirs[3 * i] = s.dir.x; dirs[3 * i + 1] = s.dir.y; dirs[3 * i + 2] = s.dir.z;
+        pdfs[i] = s.pdf;
+    }
+}
+void or_pg_update(or_guide* g, const ap_path_guiding_sample* slots, size_t n) {
+    g->state.Update(std::vector<ap_path_guiding_sample>(slots, slots + n));
+}
+
+or_guide* or_guide_create(void) { return new or_guide(); }
+void or_guide_free(or_guide* g) { delete g; }
+size_t or_guide_probe_count(const or_guide* g) { return g ? g->state.probes.size() : 0; }
+
+// Copies out the probes and index volume; bounds = start xyz, end xyz, size xyz.
+void or_guide_export(const or_guide* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds) {
+    const GuideState& G = g->state;
+    if (probes && !G.probes.empty()) std::memcpy(probes, G.probes.data(), G.probes.size() * sizeof(ap_octahedron));
+    if (volume && !G.volume.empty()) std::memcpy(volume, G.volume.data(), G.volume.size() * sizeof(int32_t));
+    if (bounds) {
+        const int32_t b[9] = {G.start.x, G.start.y, G.start.z, G.end.x, G.end.y, G.end.z, G.size.x, G.size.y, G.size.z};
+        std::memcpy(bounds, b, sizeof(b));
+    }
 }
 
 }  // extern "C"

@@ -337,7 +337,8 @@ void or_interact(const float* lam, const float* rgb, size_t n, float* out) {
     }
 }
 // The portable transcendentals shared with the core (ap_fmath.h), for their
-// accuracy tests: fn 0 sin, 1 cos, 2 exp, 3 log, 4 log1p, 5 erfc, 6 x^0.75.
+// accuracy tests: fn 0 sin, 1 cos, 2 exp, 3 log, 4 log1p, 5 erfc, 6 x^0.75,
+// 7 atan, 8 atan2(x[i], x[n + i]) (then x holds 2n values).
 void or_fmath(int fn, const float* x, size_t n, float* out) {
     for (size_t i = 0; i < n; ++i) {
         const float v = x[i];
@@ -349,7 +350,9 @@ void or_fmath(int fn, const float* x, size_t n, float* out) {
             case 3: r = apfm::log(v); break;
             case 4: r = apfm::log1p(v); break;
             case 5: r = apfm::erfc(v); break;
-            default: r = apfm::pow075(v); break;
+            case 6: r = apfm::pow075(v); break;
+            case 7: r = apfm::atan(v); break;
+            default: r = apfm::atan2(v, x[n + i]); break;
         }
         out[i] = r;
     }

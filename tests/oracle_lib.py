@@ -48,6 +48,20 @@ def lib():
         L.or_xyz_to_srgb.argtypes = [vp, vp]
         L.or_render.restype = i32
         L.or_render.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, i32]
+        L.or_render_guided.restype = i32
+        L.or_render_guided.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, i32, vp]
+        L.or_guide_create.restype = vp
+        L.or_guide_free.argtypes = [vp]
+        L.or_guide_probe_count.restype = sz
+        L.or_guide_probe_count.argtypes = [vp]
+        L.or_guide_export.argtypes = [vp, vp, vp, vp]
+        L.or_pg_uv_to_sphere.argtypes = [vp, sz, vp]
+        L.or_pg_sphere_to_uv.argtypes = [vp, sz, vp]
+        L.or_pg_voxel.argtypes = [vp, vp, sz, vp]
+        L.or_pg_mipmaps.argtypes = [vp]
+        L.or_pg_pdf.argtypes = [vp, vp, sz, vp, vp, vp]
+        L.or_pg_sample.argtypes = [vp, ctypes.c_uint64, sz, vp, vp, vp, vp]
+        L.or_pg_update.argtypes = [vp, vp, sz]
         data = os.path.join(REPO, "aperturerendererv3-optix-geometry_amd", "data")
         rc = L.or_spectrum_load(os.path.join(data, "spectrum_mlp.bin").encode(),
                                 os.path.join(data, "cie_tables.bin").encode())
@@ -102,21 +116,110 @@ class OracleBVH:
                        _p(ids), _p(ts), _p(nodes), _p(prims), threads or os.cpu_count())
         return (ids, ts, nodes, prims) if stats else (ids, ts)
 
-    def render(self, camera, params, cull_margin=None, threads=None, hdr=None):
-        """Reference integrator (or_render.cpp): (hdr running mean, FP64 sums, primary ids)."""
+    def render(self, camera, params, cull_margin=None, threads=None, hdr=None, guide=None, sums=None):
+        """Reference integrator (or_render.cpp): (hdr running mean, FP64 sums, primary ids).
+        With params.use_guiding, `guide` (a Guide) carries the probes across calls."""
         import ctypes as C
 
         w, h = camera.width, camera.height
-        hdr = np.zeros((h, w, 3), np.float32) if hdr is None else np.ascontiguousarray(hdr, np.float32)
-        sums = np.zeros((h, w, 3), np.float64)
+        hdr = np.zeros((h, w, 3), np.float32) if hdr is None else np.ascontiguousarray(hdr, np.float32).copy()
+        sums = np.zeros((h, w, 3), np.float64) if sums is None else np.ascontiguousarray(sums, np.float64).copy()
         prim = np.full((h, w), -1, np.int32)
-        lib().or_render(self.h, _p(self.objs), len(self.objs), C.addressof(camera), C.addressof(params),
-                        float(cull_margin or 0.0), _p(hdr), _p(sums), _p(prim), threads or os.cpu_count())
+        rc = lib().or_render_guided(self.h, _p(self.objs), len(self.objs), C.addressof(camera), C.addressof(params),
+                                    float(cull_margin or 0.0), _p(hdr), _p(sums), _p(prim),
+                                    threads or os.cpu_count(), guide.h if guide is not None else None)
+        if rc != 0:
+            raise RuntimeError("or_render_guided failed (guiding needs a Guide)")
         return hdr, sums, prim
 
     def __del__(self):
         try:
             lib().or_bvh_free(self.h)
+        except Exception:
+            pass
+
+
+# ---- guiding primitives ----
+PROBE_DTYPE = np.dtype([("data", "<f4", (86, 4)), ("count", "<u8"), ("pad", "u1", (8,))])
+GUIDE_SAMPLE_DTYPE = np.dtype([("radiance", "<f4", (3,)), ("direction", "<f4", (3,)), ("pad0", "<f4", (2,)),
+                               ("voxel", "<i4", (4,)), ("valid", "u1"), ("pad1", "u1", (15,))])
+assert PROBE_DTYPE.itemsize == 1392 and GUIDE_SAMPLE_DTYPE.itemsize == 64
+
+
+def pg_uv_to_sphere(uv):
+    uv = _f32(uv, (-1, 2))
+    out = np.zeros((len(uv), 3), np.float32)
+    lib().or_pg_uv_to_sphere(_p(uv), len(uv), _p(out))
+    return out
+
+
+def pg_sphere_to_uv(d):
+    d = _f32(d, (-1, 3))
+    out = np.zeros((len(d), 2), np.float32)
+    lib().or_pg_sphere_to_uv(_p(d), len(d), _p(out))
+    return out
+
+
+def pg_voxel(pos, dirs):
+    pos, dirs = _f32(pos, (-1, 3)), _f32(dirs, (-1, 3))
+    out = np.zeros((len(pos), 4), np.int32)
+    lib().or_pg_voxel(_p(pos), _p(dirs), len(pos), _p(out))
+    return out
+
+
+def pg_probe(leaf_xyz):
+    """A probe with the given (64, 3) leaf texels and its mipmaps generated."""
+    p = np.zeros(1, PROBE_DTYPE)
+    p["data"][0, :64, :3] = leaf_xyz
+    lib().or_pg_mipmaps(_p(p))
+    return p
+
+
+def pg_pdf(probe, dirs, q, normal=None):
+    dirs = _f32(dirs, (-1, 3))
+    out = np.zeros(len(dirs), np.float32)
+    lib().or_pg_pdf(_p(probe), _p(dirs), len(dirs), _p(_f32(q)), None if normal is None else _p(_f32(normal)), _p(out))
+    return out
+
+
+def pg_sample(probe, n, q, seed=1, normal=None):
+    dirs = np.zeros((n, 3), np.float32)
+    pdfs = np.zeros(n, np.float32)
+    lib().or_pg_sample(_p(probe), seed, n, _p(_f32(q)), None if normal is None else _p(_f32(normal)), _p(dirs),
+                       _p(pdfs))
+    return dirs, pdfs
+
+
+class Guide:
+    """Guiding state of the oracle (probes + index volume) across render calls."""
+
+    def __init__(self):
+        self.h = lib().or_guide_create()
+
+    def probe_count(self):
+        return int(lib().or_guide_probe_count(self.h))
+
+    def update(self, slots):
+        slots = np.ascontiguousarray(slots, GUIDE_SAMPLE_DTYPE)
+        lib().or_pg_update(self.h, _p(slots), len(slots))
+
+    def export(self):
+        """(probes as an (N, 86, 4) float32 array + (N,) uint64 counts, volume int32, bounds int32[9])."""
+        n = self.probe_count()
+        raw = np.zeros(n * 1392, np.uint8)
+        bounds = np.zeros(9, np.int32)
+        lib().or_guide_export(self.h, None, None, _p(bounds))
+        nvol = int(bounds[6]) * int(bounds[7]) * int(bounds[8]) * 20
+        vol = np.zeros(max(nvol, 1), np.int32)
+        lib().or_guide_export(self.h, _p(raw), _p(vol), _p(bounds))
+        rec = raw.reshape(n, 1392)
+        data = rec[:, :1376].copy().view(np.float32).reshape(n, 86, 4)
+        counts = rec[:, 1376:1384].copy().view(np.uint64).reshape(n)
+        return data, counts, vol[:nvol], bounds
+
+    def __del__(self):
+        try:
+            lib().or_guide_free(self.h)
         except Exception:
             pass
 
@@ -182,7 +285,16 @@ def interact(lam, rgb):
     return out
 
 
-FMATH = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "log1p": 4, "erfc": 5, "pow075": 6}
+FMATH = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "log1p": 4, "erfc": 5, "pow075": 6, "atan": 7}
+
+
+def fmath_atan2(y, x):
+    y = _f32(y, (-1,))
+    x = _f32(x, (-1,))
+    both = np.concatenate([y, x])
+    out = np.zeros(len(y), np.float32)
+    lib().or_fmath(8, _p(both), len(y), _p(out))
+    return out
 
 
 def fmath(name, x):

@@ -24,6 +24,7 @@ CASES = {
             np.log, 1.1),
     "log1p": (np.concatenate([-RNG.uniform(0, 0.9999999, 200000), RNG.uniform(0, 10, 20000)]), np.log1p, 2.5),
     "pow075": (RNG.uniform(0, 1, 200000), lambda x: x ** 0.75, 2.5),
+    "atan": (np.concatenate([np.linspace(-50, 50, 200001), RNG.standard_cauchy(100000)]), np.arctan, 3.0),
 }
 
 
@@ -63,3 +64,13 @@ def test_sin_cos_quadrant_identities():
     assert np.abs(s * s + c * c - 1.0).max() < 4e-7
     assert np.array_equal(ol.fmath("sin", -xs), -s)  # odd
     assert np.array_equal(ol.fmath("cos", -xs), c)   # even
+
+
+def test_atan2_quadrants_and_accuracy():
+    y = RNG.uniform(-3, 3, 200000).astype(np.float32)
+    x = RNG.uniform(-3, 3, 200000).astype(np.float32)
+    got = ol.fmath_atan2(y, x)
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    assert ulp_err(got, ref).max() <= 3.0
+    sp = ol.fmath_atan2([0.0, 1.0, -1.0, 0.0, 0.0, 1.0, -1.0], [0.0, 0.0, 0.0, 1.0, -1.0, -1.0, -1.0])
+    assert np.allclose(sp, [0.0, np.pi / 2, -np.pi / 2, 0.0, np.pi, np.arctan2(1, -1), np.arctan2(-1, -1)], atol=1e-7)
