@@ -148,7 +148,7 @@ EXPORTED_SYMBOLS = (
     "ap_spectrum_setup", "ap_spectrum_encode_rgb", "ap_spectrum_encode_lambda", "ap_spectrum_query_rgb",
     "ap_spectrum_query_lambda", "ap_spectrum_interact", "ap_spectrum_sample_lambda",
     "ap_render_params_default", "ap_render", "ap_render_offline", "ap_xyz_to_srgb", "ap_synth_box_scene",
-    "ap_synth_heightfield_scene",
+    "ap_synth_heightfield_scene", "ap_guiding_reset", "ap_guiding_export",
 )
 
 
@@ -168,6 +168,8 @@ def lib():
         L.ap_scene_set_launch.argtypes = [vp, i32, i32]
         L.ap_scene_get_launch.argtypes = [vp, vp, vp]
         L.ap_scene_autotune.argtypes = [vp, vp, sz, vp, i32, vp, vp, vp, vp]
+        L.ap_guiding_reset.argtypes = [vp]
+        L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
             getattr(L, name).argtypes = [vp, vp, sz, vp, vp]
@@ -498,6 +500,24 @@ class SceneRenderer:
         check(lib().ap_scene_autotune(self.handle, _ptr(rays), n, _ptr(hits), 1 if any_hit else 0, _stream(stream),
                                       ctypes.byref(w), ctypes.byref(r), ctypes.byref(ms)))
         return w.value, r.value, ms.value
+
+    def guiding_reset(self):
+        check(lib().ap_guiding_reset(self.handle))
+
+    def guiding_export(self):
+        """(probe texels (N, 86, 4) float32, counts (N,) uint64, index volume int32, bounds int32[9])."""
+        bounds = np.zeros(9, np.int32)
+        count = ctypes.c_int32()
+        check(lib().ap_guiding_export(self.handle, None, 0, None, 0, _ptr(bounds), ctypes.byref(count)))
+        n = count.value
+        nvol = int(bounds[6]) * int(bounds[7]) * int(bounds[8]) * 20
+        raw = np.zeros(max(n, 1) * 1392, np.uint8)
+        vol = np.zeros(max(nvol, 1), np.int32)
+        check(lib().ap_guiding_export(self.handle, _ptr(raw), n, _ptr(vol), nvol, _ptr(bounds), ctypes.byref(count)))
+        rec = raw[: n * 1392].reshape(n, 1392)
+        data = rec[:, :1376].copy().view(np.float32).reshape(n, 86, 4)
+        counts = rec[:, 1376:1384].copy().view(np.uint64).reshape(n)
+        return data, counts, vol[:nvol], bounds
 
     def export_bvh(self):
         inf = self.info()

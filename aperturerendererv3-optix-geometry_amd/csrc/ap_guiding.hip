@@ -1,0 +1,546 @@
+// Per-frame build of the guiding probes (SceneRenderer::UpdatePathGuidingSample,
+// core/scene.cu:583-744, over core/guiding/guiding.cuh:70-517).
+//
+// Input: the frame's guiding samples, PG_COLLECT slots per path in pixel
+// order (written by the accumulate kernel). Steps, all on the render stream:
+//   1. key every valid slot by its voxel (x, y, z, face) packed so that the
+//      unsigned order is the reference's Int4Less order; radix-sort
+//      (key, slot) pairs; mark run heads and scan them: the unique voxels in
+//      sorted order (thrust sort + unique in the reference);
+//   2. grow the index volume to the union of its bounds and the voxels'
+//      (CopyOldIntoNew), then number the voxels without a probe in sorted
+//      order and append zeroed probes for them (FilterUnallocatedGrids +
+//      ScatterAssignIndices);
+//   3. key every valid slot by (probe, leaf texel), stable radix sort, and
+//      let the head of each run sum radiance * 4pi over the run in slot
+//      order (the reference adds with float atomics in arbitrary order, so
+//      its sums are not reproducible; this order is, and the oracle uses it);
+//   4. one workgroup per probe with samples merges the sums into the
+//      running mean in FP64 and rebuilds the mips (MergeProbeKernel).
+// The host reads back three small counts per frame (unique voxels with their
+// bounds, new probes), as the reference does.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "aperture/ap_guiding.h"
+#include "ap_guide.h"
+#include "ap_internal.h"
+
+namespace ap {
+
+// Voxel coordinates must lie in [-2^18, 2^18): 19 bits each, face in 5 bits.
+constexpr int kCoordBits = 19;
+constexpr int32_t kCoordBias = 1 << (kCoordBits - 1);
+constexpr uint64_t kNoKey = ~0ull;
+
+struct GuideDevice {
+    ap_octahedron* probes = nullptr;
+    ap_octahedron* temp = nullptr;
+    int32_t count = 0, cap = 0;
+    int32_t* volume = nullptr;
+    int32_t start[3] = {0, 0, 0}, end[3] = {0, 0, 0}, size[3] = {0, 0, 0};
+    GuideRec* recs = nullptr;
+    size_t rec_cap = 0;
+    ap_path_guiding_sample* slots = nullptr;
+    size_t slot_cap = 0;
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* flags = nullptr;
+    uint32_t* ranks = nullptr;
+    uint64_t* uniq = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    size_t sort_cap = 0;
+    int32_t* info = nullptr;      // device: [0] unique count, [1..3] min xyz, [4..6] max xyz, [7] new probes, [8] error
+    int32_t* info_host = nullptr; // pinned
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+inline dim3 grid_for(size_t n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
+
+__device__ __forceinline__ bool in_range(int v) { return v >= -kCoordBias && v < kCoordBias; }
+
+__device__ __forceinline__ uint64_t pack_voxel(const ap_int4& v) {
+    const uint64_t x = static_cast<uint64_t>(v.x + kCoordBias), y = static_cast<uint64_t>(v.y + kCoordBias),
+                   z = static_cast<uint64_t>(v.z + kCoordBias);
+    return (x << (2 * kCoordBits + 5)) | (y << (kCoordBits + 5)) | (z << 5) | static_cast<uint64_t>(v.w);
+}
+
+__device__ __forceinline__ ap_int4 unpack_voxel(uint64_t k) {
+    const int32_t mask = (1 << kCoordBits) - 1;
+    ap_int4 v;
+    v.x = static_cast<int32_t>((k >> (2 * kCoordBits + 5)) & mask) - kCoordBias;
+    v.y = static_cast<int32_t>((k >> (kCoordBits + 5)) & mask) - kCoordBias;
+    v.z = static_cast<int32_t>((k >> 5) & mask) - kCoordBias;
+    v.w = static_cast<int32_t>(k & 31u);
+    return v;
+}
+
+struct VolumeView {
+    int32_t* volume;
+    int32_t start[3], size[3];
+    __device__ int64_t slot(const ap_int4& v) const {
+        const int lx = v.x - start[0], ly = v.y - start[1], lz = v.z - start[2];
+        if (v.w < 0 || v.w >= PG_FACES) return -1;
+        if (lx < 0 || ly < 0 || lz < 0 || lx >= size[0] || ly >= size[1] || lz >= size[2]) return -1;
+        const size_t lin = static_cast<size_t>(lx) +
+                           static_cast<size_t>(size[0]) * (static_cast<size_t>(ly) + static_cast<size_t>(size[1]) * lz);
+        return static_cast<int64_t>(lin * PG_FACES + static_cast<size_t>(v.w));
+    }
+};
+
+__global__ void voxel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots, uint32_t n,
+                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, int32_t* __restrict__ info) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const ap_path_guiding_sample s = slots[i];
+    uint64_t k = kNoKey;
+    if (s.valid) {
+        const ap_int4 v = s.voxel_grid;
+        if (in_range(v.x) && in_range(v.y) && in_range(v.z) && v.w >= 0 && v.w < PG_FACES) {
+            k = pack_voxel(v);
+        } else {
+            atomicOr(&info[8], 1);
+        }
+    }
+    keys[i] = k;
+    vals[i] = i;
+}
+
+__global__ void run_heads_kernel(const uint64_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flags) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    flags[i] = (k != kNoKey && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
+}
+
+__global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
+                                     const uint32_t* __restrict__ ranks, uint32_t n, uint64_t* __restrict__ uniq,
+                                     int32_t* __restrict__ info) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (i == n - 1) info[0] = static_cast<int32_t>(ranks[i] + flags[i]);
+    if (!flags[i]) return;
+    uniq[ranks[i]] = keys[i];
+    const ap_int4 v = unpack_voxel(keys[i]);
+    atomicMin(&info[1], v.x);
+    atomicMin(&info[2], v.y);
+    atomicMin(&info[3], v.z);
+    atomicMax(&info[4], v.x);
+    atomicMax(&info[5], v.y);
+    atomicMax(&info[6], v.z);
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, size_t n, int32_t v) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// CopyOldIntoNew (guiding.cuh:171-244), one thread per old voxel.
+__global__ void copy_volume_kernel(const int32_t* __restrict__ old_vol, VolumeView oldv, VolumeView newv) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const size_t nvox = static_cast<size_t>(oldv.size[0]) * oldv.size[1] * oldv.size[2];
+    if (i >= nvox) return;
+    const int x = static_cast<int>(i % oldv.size[0]);
+    const int y = static_cast<int>((i / oldv.size[0]) % oldv.size[1]);
+    const int z = static_cast<int>(i / (static_cast<size_t>(oldv.size[0]) * oldv.size[1]));
+    const int tx = x + oldv.start[0] - newv.start[0], ty = y + oldv.start[1] - newv.start[1],
+              tz = z + oldv.start[2] - newv.start[2];
+    const size_t dst = static_cast<size_t>(tx) +
+                       static_cast<size_t>(newv.size[0]) * (static_cast<size_t>(ty) + static_cast<size_t>(newv.size[1]) * tz);
+    for (int f = 0; f < PG_FACES; ++f) newv.volume[dst * PG_FACES + f] = old_vol[i * PG_FACES + f];
+}
+
+__global__ void unindexed_kernel(const uint64_t* __restrict__ uniq, uint32_t n, VolumeView vv,
+                                 uint32_t* __restrict__ flags) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = vv.slot(unpack_voxel(uniq[i]));
+    flags[i] = (s >= 0 && vv.volume[s] < 0) ? 1u : 0u;
+}
+
+__global__ void assign_kernel(const uint64_t* __restrict__ uniq, const uint32_t* __restrict__ flags,
+                              const uint32_t* __restrict__ ranks, uint32_t n, VolumeView vv, int32_t base,
+                              int32_t* __restrict__ info) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (i == n - 1) info[7] = static_cast<int32_t>(ranks[i] + flags[i]);
+    if (!flags[i]) return;
+    const int64_t s = vv.slot(unpack_voxel(uniq[i]));
+    vv.volume[s] = base + static_cast<int32_t>(ranks[i]);
+}
+
+__global__ void zero_probes_kernel(ap_octahedron* __restrict__ p, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const size_t words = n * (sizeof(ap_octahedron) / 16);
+    if (i >= words) return;
+    reinterpret_cast<uint4*>(p)[i] = uint4{0u, 0u, 0u, 0u};
+}
+
+__global__ void texel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots, uint32_t n, VolumeView vv,
+                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const ap_path_guiding_sample s = slots[i];
+    uint64_t k = kNoKey;
+    if (s.valid) {
+        const ap_int4 v = s.voxel_grid;
+        if (in_range(v.x) && in_range(v.y) && in_range(v.z)) {
+            const int64_t sl = vv.slot(v);
+            const int32_t probe = sl >= 0 ? vv.volume[sl] : -1;
+            if (probe >= 0) {
+                const f3 d = normalize(f3{s.direction.x, s.direction.y, s.direction.z});
+                k = (static_cast<uint64_t>(probe) << 6) | static_cast<uint64_t>(pg_leaf_texel(normalize(d)));
+            }
+        }
+    }
+    keys[i] = k;
+    vals[i] = i;
+}
+
+// Head of each (probe, texel) run sums its samples in slot order.
+__global__ void texel_sums_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t n,
+                                  const ap_path_guiding_sample* __restrict__ slots, ap_octahedron* __restrict__ temp) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if (k == kNoKey || (i > 0 && keys[i - 1] == k)) return;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    unsigned long long cnt = 0;
+    for (uint32_t j = i; j < n && keys[j] == k; ++j) {
+        const ap_float3 r = slots[vals[j]].radiance;
+        sx += r.x * PG_JACOBIAN;
+        sy += r.y * PG_JACOBIAN;
+        sz += r.z * PG_JACOBIAN;
+        ++cnt;
+    }
+    ap_octahedron& t = temp[k >> 6];
+    ap_float4& leaf = t.data[k & 63u];
+    leaf.x = sx;
+    leaf.y = sy;
+    leaf.z = sz;
+    atomicAdd(&t.count, cnt);
+}
+
+// MergeProbeKernel (guiding.cuh:436-506): one 64-lane workgroup per probe.
+__global__ __launch_bounds__(64) void merge_kernel(ap_octahedron* __restrict__ probes, ap_octahedron* __restrict__ temp) {
+    ap_octahedron& od = probes[blockIdx.x];
+    ap_octahedron& nw = temp[blockIdx.x];
+    const unsigned long long cnew = nw.count;
+    if (cnew == 0ull) return;
+    const unsigned long long cold = od.count;
+    const double inv_den = 1.0 / static_cast<double>(cold + cnew);
+    const double alpha = static_cast<double>(cold) * inv_den;
+    const int k = threadIdx.x;
+    {
+        ap_float4 o = od.data[k];
+        const ap_float4 n = nw.data[k];
+        o.x = static_cast<float>(static_cast<double>(o.x) * alpha + static_cast<double>(n.x) * inv_den);
+        o.y = static_cast<float>(static_cast<double>(o.y) * alpha + static_cast<double>(n.y) * inv_den);
+        o.z = static_cast<float>(static_cast<double>(o.z) * alpha + static_cast<double>(n.z) * inv_den);
+        od.data[k] = o;
+        nw.data[k] = ap_float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    __syncthreads();
+    if (k == 0) {
+        const unsigned long long sum = cold + cnew;
+        od.count = sum < PG_MAX_COUNT ? sum : PG_MAX_COUNT;
+        nw.count = 0ull;
+        pg_generate_mipmaps(od);
+    }
+}
+
+template <class T>
+int grow(T** p, size_t* cap, size_t need, const char* what) {
+    if (*cap >= need) return AP_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), need * sizeof(T));
+    if (e != hipSuccess) return hip_fail(e, what);
+    *cap = need;
+    return AP_OK;
+}
+
+int ensure_sort(GuideDevice* g, size_t n) {
+    if (g->sort_cap >= n) return AP_OK;
+    for (int b = 0; b < 2; ++b) {
+        if (g->keys[b]) (void)hipFree(g->keys[b]);
+        if (g->vals[b]) (void)hipFree(g->vals[b]);
+        g->keys[b] = nullptr;
+        g->vals[b] = nullptr;
+    }
+    for (void* p : {static_cast<void*>(g->flags), static_cast<void*>(g->ranks), static_cast<void*>(g->uniq), g->tmp})
+        if (p) (void)hipFree(p);
+    g->flags = g->ranks = nullptr;
+    g->uniq = nullptr;
+    g->tmp = nullptr;
+    g->sort_cap = 0;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        e = hipMalloc(reinterpret_cast<void**>(&g->keys[b]), n * sizeof(uint64_t));
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->vals[b]), n * sizeof(uint32_t));
+    }
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->flags), n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->ranks), n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->uniq), n * sizeof(uint64_t));
+    size_t sort_bytes = 0, scan_bytes = 0;
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs(nullptr, sort_bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1],
+                                      static_cast<unsigned>(n), 0, 64, nullptr);
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(nullptr, scan_bytes, g->flags, g->ranks, 0u, n, rocprim::plus<uint32_t>(), nullptr);
+    if (e == hipSuccess) {
+        g->tmp_bytes = std::max(sort_bytes, scan_bytes);
+        e = hipMalloc(&g->tmp, g->tmp_bytes);
+    }
+    if (e != hipSuccess) return hip_fail(e, "guiding: sort buffers");
+    g->sort_cap = n;
+    return AP_OK;
+}
+
+int grow_probes(GuideDevice* g, int32_t need, hipStream_t st) {
+    if (need <= g->cap) return AP_OK;
+    const int32_t cap = std::max(need, std::max(1024, 2 * g->cap));
+    ap_octahedron *np = nullptr, *nt = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&np), static_cast<size_t>(cap) * sizeof(ap_octahedron));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&nt), static_cast<size_t>(cap) * sizeof(ap_octahedron));
+    if (e != hipSuccess) {
+        if (np) (void)hipFree(np);
+        return hip_fail(e, "guiding: probes");
+    }
+    hipLaunchKernelGGL(zero_probes_kernel, grid_for(static_cast<size_t>(cap) * (sizeof(ap_octahedron) / 16)),
+                       dim3(kBlock), 0, st, nt, static_cast<size_t>(cap));
+    if (g->count > 0) {
+        e = hipMemcpyAsync(np, g->probes, static_cast<size_t>(g->count) * sizeof(ap_octahedron),
+                           hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "guiding: probe copy");
+    }
+    if (g->probes) (void)hipFree(g->probes);
+    if (g->temp) (void)hipFree(g->temp);
+    g->probes = np;
+    g->temp = nt;
+    g->cap = cap;
+    return AP_OK;
+}
+
+VolumeView view_of(const GuideDevice* g) {
+    VolumeView v;
+    v.volume = g->volume;
+    for (int a = 0; a < 3; ++a) {
+        v.start[a] = g->start[a];
+        v.size[a] = g->size[a];
+    }
+    return v;
+}
+
+}  // namespace
+
+void guide_free(GuideDevice* g) {
+    if (!g) return;
+    void* ptrs[] = {g->probes, g->temp, g->volume, g->recs, g->slots, g->keys[0], g->keys[1], g->vals[0],
+                    g->vals[1], g->flags, g->ranks, g->uniq, g->tmp, g->info};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (g->info_host) (void)hipHostFree(g->info_host);
+    delete g;
+}
+
+int guide_prepare(GuideDevice** pg, size_t n_paths) {
+    if (!*pg) *pg = new GuideDevice();
+    GuideDevice* g = *pg;
+    int rc;
+    if ((rc = grow(&g->recs, &g->rec_cap, n_paths, "guiding: path records")) != AP_OK) return rc;
+    if ((rc = grow(&g->slots, &g->slot_cap, n_paths * PG_COLLECT, "guiding: sample slots")) != AP_OK) return rc;
+    if (!g->info) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&g->info), 16 * sizeof(int32_t));
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&g->info_host), 16 * sizeof(int32_t));
+        if (e != hipSuccess) return hip_fail(e, "guiding: info");
+    }
+    return AP_OK;
+}
+
+GuideView guide_view(const GuideDevice* g) {
+    GuideView v{};
+    if (!g) return v;
+    v.probes = g->probes;
+    v.probe_count = g->count;
+    v.volume = g->volume;
+    for (int a = 0; a < 3; ++a) {
+        v.start[a] = g->start[a];
+        v.size[a] = g->size[a];
+    }
+    v.recs = g->recs;
+    return v;
+}
+
+ap_path_guiding_sample* guide_slots(GuideDevice* g) { return g ? g->slots : nullptr; }
+
+int guide_reset(GuideDevice* g) {
+    if (!g) return AP_OK;
+    if (g->volume) (void)hipFree(g->volume);
+    g->volume = nullptr;
+    for (int a = 0; a < 3; ++a) g->start[a] = g->end[a] = g->size[a] = 0;
+    g->count = 0;
+    return AP_OK;
+}
+
+int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
+    const size_t n_slots = n_paths * PG_COLLECT;
+    if (n_slots == 0) return AP_OK;
+    if (n_slots > 0xffffffffull) return fail(AP_ERR_UNSUPPORTED, "guiding: too many sample slots in one frame");
+    const uint32_t n = static_cast<uint32_t>(n_slots);
+    int rc = ensure_sort(g, n_slots);
+    if (rc != AP_OK) return rc;
+    const int32_t init[16] = {0, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN, 0, 0};
+    hipError_t e = hipMemcpyAsync(g->info, init, sizeof(init), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: info reset");
+    // 1. unique voxels in sorted order.
+    hipLaunchKernelGGL(voxel_keys_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, g->keys[0], g->vals[0],
+                       g->info);
+    size_t bytes = g->tmp_bytes;
+    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], n, 0, 64, st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+    hipLaunchKernelGGL(run_heads_kernel, grid_for(n), dim3(kBlock), 0, st, g->keys[1], n, g->flags);
+    bytes = g->tmp_bytes;
+    e = rocprim::exclusive_scan(g->tmp, bytes, g->flags, g->ranks, 0u, n, rocprim::plus<uint32_t>(), st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: voxel scan");
+    hipLaunchKernelGGL(unique_voxels_kernel, grid_for(n), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, n,
+                       g->uniq, g->info);
+    e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
+    if (g->info_host[8]) return fail(AP_ERR_UNSUPPORTED, "guiding: a voxel coordinate is outside [-2^18, 2^18)");
+    const uint32_t n_uniq = static_cast<uint32_t>(g->info_host[0]);
+    if (n_uniq == 0) return AP_OK;
+    // 2. grow the index volume to the union of the bounds.
+    int32_t ds[3], de[3];
+    for (int a = 0; a < 3; ++a) {
+        ds[a] = std::min(g->start[a], g->info_host[1 + a]);
+        de[a] = std::max(g->end[a], g->info_host[4 + a]);
+    }
+    const bool changed = ds[0] != g->start[0] || ds[1] != g->start[1] || ds[2] != g->start[2] ||
+                         de[0] != g->end[0] || de[1] != g->end[1] || de[2] != g->end[2];
+    if (changed || !g->volume) {
+        const int32_t nd[3] = {de[0] - ds[0] + 1, de[1] - ds[1] + 1, de[2] - ds[2] + 1};
+        const size_t nvox = static_cast<size_t>(nd[0]) * nd[1] * nd[2];
+        const size_t elems = nvox * PG_FACES;
+        if (elems > (size_t{1} << 33)) return fail(AP_ERR_OUT_OF_MEMORY, "guiding: index volume too large");
+        int32_t* nv = nullptr;
+        e = hipMalloc(reinterpret_cast<void**>(&nv), elems * sizeof(int32_t));
+        if (e != hipSuccess) return hip_fail(e, "guiding: index volume");
+        hipLaunchKernelGGL(fill_i32_kernel, grid_for(elems), dim3(kBlock), 0, st, nv, elems, -1);
+        if (g->volume) {
+            VolumeView oldv = view_of(g), newv;
+            newv.volume = nv;
+            for (int a = 0; a < 3; ++a) {
+                newv.start[a] = ds[a];
+                newv.size[a] = nd[a];
+            }
+            const size_t old_vox = static_cast<size_t>(g->size[0]) * g->size[1] * g->size[2];
+            hipLaunchKernelGGL(copy_volume_kernel, grid_for(old_vox), dim3(kBlock), 0, st, g->volume, oldv, newv);
+            e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                (void)hipFree(nv);
+                return hip_fail(e, "guiding: volume copy");
+            }
+            (void)hipFree(g->volume);
+        }
+        g->volume = nv;
+        for (int a = 0; a < 3; ++a) {
+            g->start[a] = ds[a];
+            g->end[a] = de[a];
+            g->size[a] = nd[a];
+        }
+    }
+    // Probes for the voxels without one, numbered in sorted order.
+    hipLaunchKernelGGL(unindexed_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, n_uniq, view_of(g), g->flags);
+    bytes = g->tmp_bytes;
+    e = rocprim::exclusive_scan(g->tmp, bytes, g->flags, g->ranks, 0u, n_uniq, rocprim::plus<uint32_t>(), st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: probe scan");
+    hipLaunchKernelGGL(assign_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, g->flags, g->ranks, n_uniq,
+                       view_of(g), g->count, g->info);
+    e = hipMemcpyAsync(g->info_host + 7, g->info + 7, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: read new probes");
+    const int32_t n_new = g->info_host[7];
+    if (n_new > 0) {
+        if ((rc = grow_probes(g, g->count + n_new, st)) != AP_OK) return rc;
+        hipLaunchKernelGGL(zero_probes_kernel, grid_for(static_cast<size_t>(n_new) * (sizeof(ap_octahedron) / 16)),
+                           dim3(kBlock), 0, st, g->probes + g->count, static_cast<size_t>(n_new));
+        g->count += n_new;
+    }
+    // 3. per (probe, texel) sums in slot order.
+    hipLaunchKernelGGL(texel_keys_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, view_of(g), g->keys[0],
+                       g->vals[0]);
+    int probe_bits = 1;
+    while ((1ll << probe_bits) <= g->count) ++probe_bits;
+    const int end_bit = std::min(64, 6 + probe_bits + 1);
+    bytes = g->tmp_bytes;
+    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], n, 0, end_bit, st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: texel sort");
+    hipLaunchKernelGGL(texel_sums_kernel, grid_for(n), dim3(kBlock), 0, st, g->keys[1], g->vals[1], n, g->slots,
+                       g->temp);
+    // 4. FP64 merge and mips.
+    if (g->count > 0)
+        hipLaunchKernelGGL(merge_kernel, dim3(static_cast<unsigned>(g->count)), dim3(64), 0, st, g->probes, g->temp);
+    e = hipGetLastError();
+    return e == hipSuccess ? AP_OK : hip_fail(e, "guiding: update");
+}
+
+int guide_export(const GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count) {
+    if (count) *count = g ? g->count : 0;
+    if (bounds) {
+        for (int a = 0; a < 3; ++a) {
+            bounds[a] = g ? g->start[a] : 0;
+            bounds[3 + a] = g ? g->end[a] : 0;
+            bounds[6 + a] = g ? g->size[a] : 0;
+        }
+    }
+    if (!g) return AP_OK;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && probes && g->count > 0)
+        e = hipMemcpy(probes, g->probes, static_cast<size_t>(g->count) * sizeof(ap_octahedron), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && volume && g->volume)
+        e = hipMemcpy(volume, g->volume,
+                      static_cast<size_t>(g->size[0]) * g->size[1] * g->size[2] * PG_FACES * sizeof(int32_t),
+                      hipMemcpyDeviceToHost);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "guiding: export");
+}
+
+}  // namespace ap
+
+using namespace ap;
+
+extern "C" {
+
+int ap_guiding_reset(ap_scene* scene) {
+    if (!scene) return fail(AP_ERR_INVALID_ARG, "ap_guiding_reset: NULL scene");
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "ap_guiding_reset");
+    return guide_reset(scene->guide);
+}
+
+int ap_guiding_export(const ap_scene* scene, ap_octahedron* probes, size_t probe_capacity, int32_t* volume,
+                      size_t volume_capacity, int32_t* bounds, int32_t* probe_count) {
+    if (!scene) return fail(AP_ERR_INVALID_ARG, "ap_guiding_export: NULL scene");
+    int32_t b[9], count = 0;
+    int rc = guide_export(scene->guide, nullptr, nullptr, b, &count);
+    if (rc != AP_OK) return rc;
+    const size_t vol = static_cast<size_t>(b[6]) * b[7] * b[8] * PG_FACES;
+    if (probes && probe_capacity < static_cast<size_t>(count))
+        return fail(AP_ERR_INVALID_ARG, "ap_guiding_export: probe array too small");
+    if (volume && volume_capacity < vol) return fail(AP_ERR_INVALID_ARG, "ap_guiding_export: volume array too small");
+    rc = guide_export(scene->guide, probes, volume, nullptr, nullptr);
+    if (rc != AP_OK) return rc;
+    if (bounds) std::memcpy(bounds, b, sizeof(b));
+    if (probe_count) *probe_count = count;
+    return AP_OK;
+}
+
+}  // extern "C"

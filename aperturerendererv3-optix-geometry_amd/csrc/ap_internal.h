@@ -93,6 +93,8 @@ int bvh_max_depth(const HostBVH& bvh);
 
 namespace ap {
 struct RenderState;  // integrator buffers of a scene (ap_render.hip)
+struct GuideDevice;  // guiding probes and build buffers of a scene (ap_guiding.hip)
+void guide_free(GuideDevice* g);
 void render_state_free(RenderState* r);
 }  // namespace ap
 
@@ -109,6 +111,7 @@ struct ap_scene {
     void* d_analytic = nullptr;
     int64_t device_bytes = 0;
     ap::RenderState* render = nullptr;
+    ap::GuideDevice* guide = nullptr;
 };
 
 namespace ap {

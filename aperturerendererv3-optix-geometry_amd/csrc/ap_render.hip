@@ -31,6 +31,7 @@
 #include "ap_pcg.h"
 #include "ap_shade.h"
 #include "ap_spectrum.h"
+#include "ap_guide.h"
 
 namespace ap {
 
@@ -126,6 +127,59 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + static_cast<uint32_t>(__popcll(below));
 }
 
+// CollectRadiance (render.cuh:353-432) on a path's guiding record.
+__device__ __forceinline__ void pg_endpoint(GuideRec& r, float le, int ndd) {
+    if (le == 0.0f) return;
+    const int hi = ndd < PG_COLLECT ? ndd : PG_COLLECT;
+    for (int i = 0; i < hi; ++i)
+        if (r.mask[i] != 0.0f) {
+            r.sample[i] = r.sample[i] + r.mask[i] * le;
+            r.valid |= 1u << i;
+        }
+}
+
+__device__ __forceinline__ void pg_advance(GuideRec& r, f3 next, float contrib, float rr_prob, int ndd, bool allow,
+                                           f3 ray_dir, f3 pos, f3 nrm) {
+    if (allow && ndd < PG_COLLECT && r.mask[ndd] == 0.0f) {
+        r.dir[ndd][0] = next.x; r.dir[ndd][1] = next.y; r.dir[ndd][2] = next.z;
+        r.loc[ndd][0] = pos.x; r.loc[ndd][1] = pos.y; r.loc[ndd][2] = pos.z;
+        const f3 v = pg_reflect(ray_dir, nrm);
+        r.vdir[ndd][0] = v.x; r.vdir[ndd][1] = v.y; r.vdir[ndd][2] = v.z;
+        r.mask[ndd] = 1.0f;
+    }
+    const float inv_q = static_cast<float>(1.0 / static_cast<double>(rr_prob));
+    const int up = ndd + (allow ? 1 : 0);
+    const int hi = up < PG_COLLECT ? up : PG_COLLECT;
+    for (int i = 0; i < hi; ++i)
+        if (r.mask[i] != 0.0f) r.mask[i] = r.mask[i] * (contrib * inv_q);
+}
+
+// Path end (render.cuh:910-957): valid collection slots to guiding samples.
+__device__ __forceinline__ void pg_emit(const GuideRec& r, const float xyz[3], float lpdf,
+                                        ap_path_guiding_sample* __restrict__ out) {
+    int w = 0;
+    ap_path_guiding_sample rec[PG_COLLECT];
+    for (int k = 0; k < PG_COLLECT; ++k) {
+        __builtin_memset(&rec[k], 0, sizeof(ap_path_guiding_sample));
+    }
+    for (int i = 0; i < PG_COLLECT; ++i) {
+        if (w >= PG_COLLECT) break;
+        if (!((r.valid >> i) & 1u)) continue;
+        const f3 rx = f3{xyz[0], xyz[1], xyz[2]} * r.sample[i] / lpdf;
+        const float strength = fminf(max3(rx), PG_MAX_BRIGHT);
+        if (strength < PG_MIN_BRIGHT) continue;
+        ap_path_guiding_sample& o = rec[w++];
+        o.direction = ap_float3{r.dir[i][0], r.dir[i][1], r.dir[i][2]};
+        const f3 c = fmax3(rx, f3{0.0f, 0.0f, 0.0f});
+        o.radiance = ap_float3{c.x, c.y, c.z};
+        const PgVoxel v = pg_voxel(f3{r.loc[i][0], r.loc[i][1], r.loc[i][2]},
+                                   f3{r.vdir[i][0], r.vdir[i][1], r.vdir[i][2]});
+        o.voxel_grid = ap_int4{v.x, v.y, v.z, v.f};
+        o.valid = 1;
+    }
+    for (int k = 0; k < PG_COLLECT; ++k) out[k] = rec[k];
+}
+
 struct CamParams {
     f3 origin, dir, right, up;
     float tan_x, tan_y;
@@ -151,7 +205,8 @@ __global__ void begin_bounce_kernel(uint32_t* c, int out) {
 __global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FrameParams fp, const f4* __restrict__ lam_lut,
                                                           const float* __restrict__ tables,
                                                           const float* __restrict__ eps, PathState* __restrict__ paths,
-                                                          ap_ray* __restrict__ rays, int32_t* __restrict__ path_of) {
+                                                          ap_ray* __restrict__ rays, int32_t* __restrict__ path_of,
+                                                          GuideRec* __restrict__ recs) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t total = static_cast<uint32_t>(fp.n_pixels) * static_cast<uint32_t>(fp.frames);
     if (i >= total) return;
@@ -191,6 +246,11 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FramePa
     s.rng_state = rng.state;
     s.rng_inc = rng.inc;
     paths[i] = s;
+    if (recs) {
+        GuideRec z;
+        __builtin_memset(&z, 0, sizeof(z));
+        recs[i] = z;
+    }
     rays[i] = ap_ray{cam.origin.x, cam.origin.y, cam.origin.z, eps[0], d.x, d.y, d.z, __int_as_float(0x7f800000)};
     path_of[i] = static_cast<int32_t>(i);
 }
@@ -351,6 +411,7 @@ __device__ LightSample sample_light(const LightRec* __restrict__ lights, int n, 
     return ls;
 }
 
+template <bool GUIDE>
 __global__ __launch_bounds__(kBlock) void shade_kernel(
     ShadeParams sp, const GeoRec* __restrict__ geo, const MatRec* __restrict__ mats,
     const LightRec* __restrict__ lights, const AnalyticRec* __restrict__ analytic, const f4* __restrict__ rgb_lut,
@@ -358,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     const int32_t* __restrict__ path_in, const ap_hit* __restrict__ hits, ap_ray* __restrict__ rays_out,
     int32_t* __restrict__ path_out, ap_ray* __restrict__ shadow_rays, ShadowPayload* __restrict__ shadow_pay,
     uint32_t* __restrict__ counters, int32_t* __restrict__ primary_id, int32_t primary_frame,
-    unsigned long long* __restrict__ ray_count) {
+    unsigned long long* __restrict__ ray_count, GuideView gv) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
     if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
@@ -380,10 +441,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     if (H.id < 0) {
         // Miss: SkyLighting and the offset draw of Trace, then the path ends.
         (void)rand1(rng);
+        float le_end = 0.0f;
         if (!inside) {
             const float le = sp.sky ? interact(rgb_lut, lb, sky_lighting(d)) : 0.0f;
             S.radiance = S.radiance + S.mask * le * S.d65;
+            le_end = le * S.d65;
         }
+        if (GUIDE) pg_endpoint(gv.recs[pid], le_end, S.ndd);
         S.flags &= ~PATH_PREV_NEE;
     } else {
         const GeoRec g = geo[H.id];
@@ -432,13 +496,46 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                 }
             }
         }
-        const float bsdf_cos = eval_bsdf_cos(M, lb, d, next_dir, N);
+        float bsdf_cos = eval_bsdf_cos(M, lb, d, next_dir, N);
         // Trace's offset of the shading point (render.cuh:156-161).
         {
             const float off = eps[S.ndd] * fmaxf(1.0f, length(hp));
             hp = hp + N * signf(dot(N, next_dir)) * off * (rand1(rng) + 0.5f);
         }
+        // Directional guiding of the sampled direction (render.cuh:546-639).
+        const ap_octahedron* probe = nullptr;
+        float g_alpha = 0.0f;
+        f3 g_query{1.0f, 1.0f, 1.0f};
+        if (GUIDE) {
+            g_alpha = pg_alpha(S.frame);
+            g_query = pg_query(S.mask, S.xyz, sp.depth == 0);
+            if (!pure_light && !inside && gv.probe_count > 0) {
+                probe = pg_query_probe(gv, hp, pg_reflect(d, N));
+                if (probe) {
+                    const float p_b = pdf;
+                    const float p_p = pg_hemisphere_pdf(*probe, next_dir, N, g_query);
+                    if (p_b > 1.0e-12f && p_p > 1.0e-12f) {
+                        if (rand1(rng) < g_alpha) {
+                            const PgSample gs = pg_sample_hemisphere(*probe, rng, N, g_query);
+                            if (dot(gs.dir, N) > 1.0e-12f) {
+                                const float p_bs = bsdf_pdf(M, lb, N, d, gs.dir);
+                                next_dir = gs.dir;
+                                bsdf_cos = eval_bsdf_cos(M, lb, d, gs.dir, N);
+                                pdf = (1.0f - g_alpha) * fmaxf(p_bs, 1.0e-12f) + g_alpha * gs.pdf;
+                            } else {
+                                probe = nullptr;
+                            }
+                        } else {
+                            pdf = (1.0f - g_alpha) * p_b + g_alpha * p_p;
+                        }
+                    } else {
+                        probe = nullptr;
+                    }
+                }
+            }
+        }
         // Emission of a light hit, MIS-weighted against the previous NEE (render.cuh:666-710).
+        float le_end = 0.0f;
         if (hi.light_index >= 0) {
             float mis = 1.0f;
             if (sp.use_nee && sp.depth > 0 && (S.flags & PATH_PREV_NEE)) {
@@ -449,8 +546,14 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             }
             const float le = interact_basis(M.emissive_basis, M.emissive_scale, lb);
             S.radiance = S.radiance + S.mask * le * S.d65 * mis;
+            if (GUIDE) {
+                const f3 e{M.emissive_rgb[0], M.emissive_rgb[1], M.emissive_rgb[2]};
+                le_end = interact(rgb_lut, lb, e * mis) * S.d65;
+            }
+        } else if (GUIDE && !(M.emissive_rgb[0] == 0.0f && M.emissive_rgb[1] == 0.0f && M.emissive_rgb[2] == 0.0f)) {
+            le_end = interact_basis(M.emissive_basis, M.emissive_scale, lb) * S.d65;
         }
-        (void)pure_light;
+        if (GUIDE) pg_endpoint(gv.recs[pid], le_end, S.ndd);
         if (terminate) {
             S.flags &= ~PATH_PREV_NEE;
         } else {
@@ -476,7 +579,11 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                         const float resp = interact_basis(LM.emissive_basis, LM.emissive_scale, lb);
                         const float contribution = bc * resp * (G / pat);
                         const float pLo = area_to_solid_angle_pdf(hp, ls.point, ls.normal, ls.p_facet * ls.p_area);
-                        const float pBs = fmaxf(bsdf_pdf(M, lb, N, d, L), 0.0f);
+                        float pBs = fmaxf(bsdf_pdf(M, lb, N, d, L), 0.0f);
+                        if (GUIDE && probe) {
+                            const float p_pr = pg_hemisphere_pdf(*probe, L, N, g_query);
+                            pBs = (1.0f - g_alpha) * pBs + g_alpha * p_pr;
+                        }
                         const float rad = contribution * power_heuristic(pLo, pBs);
                         add = S.mask * rad * S.d65;
                     }
@@ -520,6 +627,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                 if (rand1(rng) > p_rr) survived = false;
             }
             if (survived) {
+                if (GUIDE) pg_advance(gv.recs[pid], next_dir, contrib, p_rr, S.ndd, !inside, d, hp, N);
                 S.mask = S.mask * contrib;
                 S.mask = S.mask / p_rr;
                 S.ndd += 1;  // no delta surfaces in this core
@@ -571,7 +679,9 @@ __device__ __forceinline__ float fix_nan_inf(float x) {
 
 __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, const PathState* __restrict__ paths,
                                                             float* __restrict__ hdr, float* __restrict__ odd,
-                                                            float* __restrict__ even, double* __restrict__ sums) {
+                                                            float* __restrict__ even, double* __restrict__ sums,
+                                                            const GuideRec* __restrict__ recs,
+                                                            ap_path_guiding_sample* __restrict__ slots) {
     const int32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= fp.n_pixels) return;
     f3 acc{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
@@ -581,6 +691,7 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int32_t f = 0; f < fp.frames; ++f) {
         const PathState& S = paths[static_cast<size_t>(f) * fp.n_pixels + p];
+        if (slots && f == 0) pg_emit(recs[p], S.xyz, S.lambda_pdf, slots + static_cast<size_t>(p) * PG_COLLECT);
         const float rad = fix_nan_inf(S.radiance);
         const f3 xyz{S.xyz[0], S.xyz[1], S.xyz[2]};
         const f3 pre = (xyz * rad) / S.lambda_pdf;
@@ -628,6 +739,10 @@ __global__ void build_mats_kernel(const MatIn* __restrict__ in, int n, const f4*
     r.shading_model = m.model;
     r.light_index = m.light_index;
     r.pad = 0;
+    r.emissive_rgb[0] = m.emissive.x;
+    r.emissive_rgb[1] = m.emissive.y;
+    r.emissive_rgb[2] = m.emissive.z;
+    r.pad2 = 0;
     out[i] = r;
 }
 
@@ -815,7 +930,6 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
         return fail(AP_ERR_INVALID_ARG, "ap_render: bad spp, frame_index or max_depth");
     if (prm->triangle_material != AP_TRIANGLE_MATERIAL_OBJECT && prm->triangle_material != AP_TRIANGLE_MATERIAL_REFERENCE)
         return fail(AP_ERR_INVALID_ARG, "ap_render: bad triangle_material");
-    if (prm->use_guiding) return fail(AP_ERR_UNSUPPORTED, "ap_render: path guiding is not available in this build");
     if (prm->spp == 0 || prm->max_depth == 0) return AP_OK;
     const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
     const float* tables = nullptr;
@@ -834,9 +948,14 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
         if ((rc = build_shading(s, r, prm->triangle_material, rgb_lut)) != AP_OK) return rc;
     }
     const int32_t n_pix = cam->width * cam->height;
-    // Frames per batch: about 4M paths in flight.
-    const int32_t per_batch = std::max(1, std::min(prm->spp, static_cast<int32_t>((4u << 20) / static_cast<uint32_t>(n_pix))));
+    // Frames per batch: about 4M paths in flight. With guiding, one frame at
+    // a time: each frame samples the probes built from the frames before it
+    // (SceneRenderer::Render then UpdatePathGuidingSample, scene.cu:308-386).
+    const bool guiding = prm->use_guiding != 0;
+    const int32_t per_batch =
+        guiding ? 1 : std::max(1, std::min(prm->spp, static_cast<int32_t>((4u << 20) / static_cast<uint32_t>(n_pix))));
     if ((rc = ensure_paths(r, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
+    if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(n_pix))) != AP_OK) return rc;
 
     // Camera basis of RenderCamera (scene.cu:220-228), direction normalised
     // as Camera::SetDirection does (camera.hpp:38-41).
@@ -859,8 +978,9 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
         const uint32_t total = static_cast<uint32_t>(nf) * static_cast<uint32_t>(n_pix);
         FrameParams fp{prm->frame_index + f0, nf, n_pix, inside, prm->seed};
         hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, r->counters, total, 0u, 0u);
+        const GuideView gv = guiding ? guide_view(s->guide) : GuideView{};
         hipLaunchKernelGGL(generate_kernel, grid_of(total), dim3(kBlock), 0, stream, cp, fp, lam_lut, tables, r->eps,
-                           r->paths, r->rays[0], r->path_of[0]);
+                           r->paths, r->rays[0], r->path_of[0], gv.recs);
         int in_q = 0;
         for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
             const int out_q = 1 - in_q;
@@ -869,10 +989,19 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
             if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
             ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                            prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
-            hipLaunchKernelGGL(shade_kernel, grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats, r->lights,
-                               s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q], r->path_of[in_q], r->hits,
-                               r->rays[out_q], r->path_of[out_q], r->shadow_rays, r->shadow_payload, r->counters,
-                               fb->primary_id, last_frame, reinterpret_cast<unsigned long long*>(fb->ray_count));
+            if (guiding) {
+                hipLaunchKernelGGL(shade_kernel<true>, grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats,
+                                   r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
+                                   r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
+                                   r->shadow_payload, r->counters, fb->primary_id, last_frame,
+                                   reinterpret_cast<unsigned long long*>(fb->ray_count), gv);
+            } else {
+                hipLaunchKernelGGL(shade_kernel<false>, grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats,
+                                   r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
+                                   r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
+                                   r->shadow_payload, r->counters, fb->primary_id, last_frame,
+                                   reinterpret_cast<unsigned long long*>(fb->ray_count), gv);
+            }
             e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
             hipLaunchKernelGGL(resolve_kernel, grid_of(total), dim3(kBlock), 0, stream, r->counters, r->shadow_payload,
@@ -888,9 +1017,10 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
             }
         }
         hipLaunchKernelGGL(accumulate_kernel, grid_of(n_pix), dim3(kBlock), 0, stream, fp, r->paths, fb->hdr, fb->odd,
-                           fb->even, fb->sum_xyz);
+                           fb->even, fb->sum_xyz, gv.recs, guiding ? guide_slots(s->guide) : nullptr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
+        if (guiding && (rc = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK) return rc;
     }
     return AP_OK;
 }

@@ -56,6 +56,7 @@ int upload(const std::vector<T>& host, void** dev, int64_t& bytes) {
 
 void free_device(ap_scene* s) {
     if (s->render) { render_state_free(s->render); s->render = nullptr; }
+    if (s->guide) { guide_free(s->guide); s->guide = nullptr; }
     if (s->d_nodes) (void)hipFree(s->d_nodes);
     if (s->d_prims) (void)hipFree(s->d_prims);
     if (s->d_analytic) (void)hipFree(s->d_analytic);

@@ -167,9 +167,6 @@ def test_render_rejects_unsupported_and_bad_arguments(gpu):
     cam = ap.Camera.make(BOX_CAM["origin"], BOX_CAM["direction"], BOX_CAM["fov_deg"], 8, 8)
     hdr = ap.DeviceBuffer(8 * 8 * 12)
     with pytest.raises(ap.ApertureError) as e:
-        scene.render(cam, ap.RenderParams.defaults(use_guiding=1), hdr)
-    assert e.value.code == ap.ERR_UNSUPPORTED
-    with pytest.raises(ap.ApertureError) as e:
         scene.render(cam, ap.RenderParams.defaults(triangle_material=7), hdr)
     assert e.value.code == ap.ERR_INVALID_ARG
     bad = ap.Camera.make(BOX_CAM["origin"], BOX_CAM["direction"], BOX_CAM["fov_deg"], 0, 8)
