@@ -92,8 +92,10 @@ def render_bench(ap, spp, cpu_seconds, threads):
     """Samples/s of the wavefront path tracer at 1920x1080 on the heightfield
     workload (1024^2-cell fBm heightfield, 2,097,152 triangles, plus 256
     320-triangle icospheres and 4 thin-cube lights; Lambert/GGX 50/50), max
-    depth 16, Russian roulette from depth 4, sky on, guiding off. Frames are
-    timed with HIP events on the render stream after one untimed frame."""
+    depth 16, Russian roulette from depth 4, sky on. Measured twice: plain,
+    and with directional guiding (probes rebuilt after every frame, the
+    rebuild inside the timed region). Frames are timed with HIP events on the
+    render stream after one untimed frame."""
     t0 = time.perf_counter()
     objs = ap.synth_heightfield_scene(1024, 256, 4, 20.0)
     scene = ap.SceneRenderer(objs)
@@ -107,22 +109,40 @@ def render_bench(ap, spp, cpu_seconds, threads):
     params = dict(max_depth=16, rr_min_depth=4)
     scene.render(cam, ap.RenderParams.defaults(spp=1, **params), hdr, stream=stream)
     stream.synchronize()
-    counts.zero(stream)
-    e0, e1 = ap.Event(), ap.Event()
-    e0.record(stream)
-    scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, **params), hdr, stream=stream, ray_count=counts)
-    e1.record(stream)
+    def timed(guiding):
+        counts.zero(stream)
+        e0, e1 = ap.Event(), ap.Event()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, use_guiding=guiding, **params), hdr,
+                     stream=stream, ray_count=counts)
+        e1.record(stream)
+        stream.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3
+        ms = e0.elapsed_ms(e1)
+        rays = counts.to_numpy(np.uint64, 2)
+        return ms, wall, rays
+
+    ms, _, rays = timed(0)
+    scene.render(cam, ap.RenderParams.defaults(spp=1, use_guiding=1, **params), hdr, stream=stream)
     stream.synchronize()
-    ms = e0.elapsed_ms(e1)
-    rays = counts.to_numpy(np.uint64, 2)
+    gms, gwall, grays = timed(1)
+    probes = len(scene.guiding_export()[1])
     out = {
         "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 lights, 1920x1080, "
-                    "max depth 16, RR from depth 4, one wavelength per path, sky on, guiding off",
+                    "max depth 16, RR from depth 4, one wavelength per path, sky on",
         "spp": spp,
         "ms_per_spp": round(ms / spp, 3),
         "samples_per_s": round(W * H * spp / (ms / 1e3), 1),
         "mrays_per_s": round(float(rays[0] + rays[1]) / (ms / 1e3) / 1e6, 1),
         "rays_per_sample": round(float(rays[0] + rays[1]) / (W * H * spp), 3),
+        "guided": {
+            "ms_per_spp": round(gms / spp, 3),
+            "wall_ms_per_spp": round(gwall / spp, 3),
+            "samples_per_s": round(W * H * spp / (gms / 1e3), 1),
+            "mrays_per_s": round(float(grays[0] + grays[1]) / (gms / 1e3) / 1e6, 1),
+            "probes": probes,
+        },
     }
     log(f"render: {out}")
     if cpu_seconds > 0:
