@@ -29,6 +29,14 @@ float or_object_t(const ap_scene_object* objs, int idx, const ap_ray* r);
 
 /* Integrator restatement (or_render.cpp): frames frame_index .. +spp-1 into
  * hdr (running mean, in place), sums (added) and primary ids (last frame). */
+void or_nrc_encode(const float* q, size_t n, const float* bmin, const float* bmax, float* out);
+void or_nrc_forward(const float* params, const float* feat, size_t n, int round_bf16, float* out);
+double or_nrc_grad(const float* params, const float* feat, const float* target, size_t n, int round_bf16,
+                   double* grad);
+double or_nrc_loss_fixed(const float* params, const float* feat, const float* target, const double* den, size_t n,
+                         int round_bf16);
+void or_nrc_adam(float* params, float* m, float* v, const float* grad, int t, float lr, float beta1, float beta2,
+                 float eps);
 typedef struct or_guide or_guide;
 void or_pg_uv_to_sphere(const float* uv, size_t n, float* out);
 void or_pg_sphere_to_uv(const float* d, size_t n, float* out);

@@ -62,6 +62,13 @@ def lib():
         L.or_pg_pdf.argtypes = [vp, vp, sz, vp, vp, vp]
         L.or_pg_sample.argtypes = [vp, ctypes.c_uint64, sz, vp, vp, vp, vp]
         L.or_pg_update.argtypes = [vp, vp, sz]
+        L.or_nrc_encode.argtypes = [vp, sz, vp, vp, vp]
+        L.or_nrc_forward.argtypes = [vp, vp, sz, i32, vp]
+        L.or_nrc_grad.restype = ctypes.c_double
+        L.or_nrc_grad.argtypes = [vp, vp, vp, sz, i32, vp]
+        L.or_nrc_loss_fixed.restype = ctypes.c_double
+        L.or_nrc_loss_fixed.argtypes = [vp, vp, vp, vp, sz, i32]
+        L.or_nrc_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32]
         data = os.path.join(REPO, "aperturerendererv3-optix-geometry_amd", "data")
         rc = L.or_spectrum_load(os.path.join(data, "spectrum_mlp.bin").encode(),
                                 os.path.join(data, "cie_tables.bin").encode())
@@ -188,6 +195,45 @@ def pg_sample(probe, n, q, seed=1, normal=None):
     lib().or_pg_sample(_p(probe), seed, n, _p(_f32(q)), None if normal is None else _p(_f32(normal)), _p(dirs),
                        _p(pdfs))
     return dirs, pdfs
+
+
+# ---- radiance-cache network ----
+NRC_PARAMS = 21504
+
+
+def nrc_encode(queries, bmin, bmax):
+    q = _f32(queries, (-1, 16))
+    out = np.zeros((len(q), 64), np.float32)
+    lib().or_nrc_encode(_p(q), len(q), _p(_f32(bmin)), _p(_f32(bmax)), _p(out))
+    return out
+
+
+def nrc_forward(params, feat, round_bf16=True):
+    feat = _f32(feat, (-1, 64))
+    out = np.zeros((len(feat), 3), np.float32)
+    lib().or_nrc_forward(_p(_f32(params)), _p(feat), len(feat), 1 if round_bf16 else 0, _p(out))
+    return out
+
+
+def nrc_grad(params, feat, target, round_bf16=True):
+    feat = _f32(feat, (-1, 64))
+    grad = np.zeros(NRC_PARAMS, np.float64)
+    loss = lib().or_nrc_grad(_p(_f32(params)), _p(feat), _p(_f32(target, (-1, 3))), len(feat),
+                             1 if round_bf16 else 0, _p(grad))
+    return loss, grad
+
+
+def nrc_loss_fixed(params, feat, target, den, round_bf16=False):
+    feat = _f32(feat, (-1, 64))
+    den = np.ascontiguousarray(den, np.float64).reshape(-1)
+    return lib().or_nrc_loss_fixed(_p(_f32(params)), _p(feat), _p(_f32(target, (-1, 3))), _p(den), len(feat),
+                                   1 if round_bf16 else 0)
+
+
+def nrc_adam(params, m, v, grad, t, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8):
+    params, m, v = (np.ascontiguousarray(a, np.float32).copy() for a in (params, m, v))
+    lib().or_nrc_adam(_p(params), _p(m), _p(v), _p(_f32(grad)), t, lr, beta1, beta2, eps)
+    return params, m, v
 
 
 class Guide:

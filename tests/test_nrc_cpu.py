@@ -1,0 +1,127 @@
+"""CPU tests pinning the oracle's radiance-cache network (oracle/or_nrc.cpp):
+
+  * in FP64 mode the analytic gradient of the relative L2 loss equals central
+    finite differences for parameters of every layer (pins forward and
+    backward algebra, the ReLU masks and the loss normalisation);
+  * the encoding equals a numpy restatement (frequencies 2^k pi, k = 0..5);
+  * Adam equals the textbook update computed in numpy;
+  * BF16 mode stays within BF16 precision of the FP64 network;
+  * a few hundred Adam steps fit a smooth target function (training works).
+"""
+import numpy as np
+
+import oracle_lib as ol
+
+G = np.random.default_rng(4)
+
+
+def init_params(seed=0, scale=1.0):
+    g = np.random.default_rng(seed)
+    p = np.zeros(ol.NRC_PARAMS, np.float32)
+    for l in range(5):
+        p[l * 4096:(l + 1) * 4096] = g.normal(0, scale * np.sqrt(2.0 / 64), 4096)
+    out = p[5 * 4096:].reshape(16, 64)
+    out[:3] = g.normal(0, scale * np.sqrt(1.0 / 64), (3, 64))
+    return p
+
+
+def random_queries(n, seed=1):
+    g = np.random.default_rng(seed)
+    q = np.zeros((n, 16), np.float32)
+    q[:, :3] = g.uniform(-1, 1, (n, 3))
+    d = g.normal(size=(n, 3))
+    q[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nn = g.normal(size=(n, 3))
+    q[:, 6:9] = nn / np.linalg.norm(nn, axis=1, keepdims=True)
+    q[:, 9] = g.uniform(0.05, 1, n)
+    q[:, 10:13] = g.uniform(0.05, 0.95, (n, 3))
+    return q
+
+
+BMIN, BMAX = np.array([-1.0, -1.0, -1.0], np.float32), np.array([1.0, 1.0, 1.0], np.float32)
+
+
+def test_encoding_matches_numpy():
+    q = random_queries(500)
+    f = ol.nrc_encode(q, BMIN, BMAX)
+    p = (q[:, :3] - BMIN) / (BMAX - BMIN)
+    for a in range(3):
+        for k in range(6):
+            # The FP32 argument p * (2^k pi), then sin/cos within 2 ulp.
+            arg = (p[:, a] * (np.float32(2 ** k) * np.float32(np.pi))).astype(np.float64)
+            assert np.allclose(f[:, a * 12 + 2 * k], np.sin(arg), atol=3e-7)
+            assert np.allclose(f[:, a * 12 + 2 * k + 1], np.cos(arg), atol=3e-7)
+    assert np.array_equal(f[:, 36:39], q[:, 3:6]) and np.array_equal(f[:, 39:42], q[:, 6:9])
+    assert np.array_equal(f[:, 42], q[:, 9]) and np.array_equal(f[:, 43:46], q[:, 10:13])
+    assert (f[:, 46] == 1.0).all() and (f[:, 47:] == 0.0).all()
+
+
+def test_fp64_gradient_matches_finite_differences():
+    p = init_params(3)
+    feat = ol.nrc_encode(random_queries(8, seed=5), BMIN, BMAX)
+    target = G.uniform(0, 2, (8, 3)).astype(np.float32)
+    _, grad = ol.nrc_grad(p, feat, target, round_bf16=False)
+    # The relative loss stops the gradient through its denominator y^2 + 0.01:
+    # differentiate the loss with the denominators held at the base point.
+    y = ol.nrc_forward(p, feat, round_bf16=False).astype(np.float64)
+    den = y * y + 0.01
+    # Two parameters with non-zero gradient from every layer.
+    picks = []
+    for lo, hi in [(l * 4096, (l + 1) * 4096) for l in range(5)] + [(5 * 4096, 5 * 4096 + 3 * 64)]:
+        nz = lo + np.flatnonzero(grad[lo:hi])
+        assert len(nz) > 0
+        picks += list(nz[:: max(1, len(nz) // 2)][:2])
+    for k in picks:
+        h = 1e-5 * max(1.0, abs(float(p[k])))  # small enough not to cross a ReLU kink
+        pp, pm = p.astype(np.float64).copy(), p.astype(np.float64).copy()
+        # Parameters are float32 in the interface: step by exactly representable amounts.
+        hp = np.float32(p[k] + h) - p[k]
+        pp = p.copy(); pp[k] = p[k] + hp
+        pm = p.copy(); pm[k] = p[k] - hp
+        lp = ol.nrc_loss_fixed(pp, feat, target, den)
+        lm = ol.nrc_loss_fixed(pm, feat, target, den)
+        fd = (lp - lm) / (2 * float(hp))
+        assert abs(fd - grad[k]) <= 1e-5 * abs(grad[k]) + 1e-9, (k, fd, grad[k])
+    # The padded output rows never receive gradient.
+    assert np.all(grad[5 * 4096 + 3 * 64:] == 0.0)
+
+
+def test_adam_matches_numpy():
+    p = init_params(7)
+    m = G.normal(0, 0.01, p.size).astype(np.float32)
+    v = np.abs(G.normal(0, 0.01, p.size)).astype(np.float32)
+    g = G.normal(0, 0.1, p.size).astype(np.float32)
+    t, lr, b1, b2, eps = 3, np.float32(1e-3), np.float32(0.9), np.float32(0.99), np.float32(1e-8)
+    p2, m2, v2 = ol.nrc_adam(p, m, v, g, t, lr, b1, b2, eps)
+    mm = b1 * m + (np.float32(1) - b1) * g
+    vv = b2 * v + (np.float32(1) - b2) * (g * g)
+    c1 = np.float32(1) - np.float32(b1 ** t)
+    c2 = np.float32(1) - np.float32(b2 ** t)
+    pp = p - lr * (mm / c1) / (np.sqrt(vv / c2) + eps)
+    assert np.array_equal(m2, mm) and np.array_equal(v2, vv)
+    assert np.allclose(p2, pp, rtol=1e-6, atol=1e-9)
+
+
+def test_bf16_mode_tracks_fp64():
+    p = init_params(9)
+    feat = ol.nrc_encode(random_queries(256, seed=9), BMIN, BMAX)
+    y64 = ol.nrc_forward(p, feat, round_bf16=False)
+    y16 = ol.nrc_forward(p, feat, round_bf16=True)
+    assert np.abs(y16 - y64).max() < 0.05 * np.abs(y64).max()
+
+
+def test_training_fits_a_smooth_target():
+    q = random_queries(2048, seed=11)
+    feat = ol.nrc_encode(q, BMIN, BMAX)
+    target = np.stack([0.5 + 0.4 * q[:, 0], 0.3 + 0.2 * q[:, 3] ** 2, 0.2 + 0.1 * q[:, 7]], 1).astype(np.float32)
+    p = init_params(13, scale=0.5)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    first = None
+    for t in range(1, 201):
+        idx = G.choice(len(q), 256, replace=False)
+        loss, g = ol.nrc_grad(p, feat[idx], target[idx], round_bf16=True)
+        first = loss if first is None else first
+        p, m, v = ol.nrc_adam(p, m, v, g.astype(np.float32), t, lr=np.float32(3e-3))
+    final, _ = ol.nrc_grad(p, feat, target, round_bf16=True)
+    assert final < 0.1 * first, (first, final)
