@@ -149,6 +149,8 @@ EXPORTED_SYMBOLS = (
     "ap_spectrum_query_lambda", "ap_spectrum_interact", "ap_spectrum_sample_lambda",
     "ap_render_params_default", "ap_render", "ap_render_offline", "ap_xyz_to_srgb", "ap_synth_box_scene",
     "ap_synth_heightfield_scene", "ap_guiding_reset", "ap_guiding_export",
+    "ap_nrc_create", "ap_nrc_destroy", "ap_nrc_set_bounds", "ap_nrc_get_params", "ap_nrc_set_params",
+    "ap_nrc_infer", "ap_nrc_train_step", "ap_nrc_gradient", "ap_nrc_last_gradient",
 )
 
 
@@ -169,6 +171,15 @@ def lib():
         L.ap_scene_get_launch.argtypes = [vp, vp, vp]
         L.ap_scene_autotune.argtypes = [vp, vp, sz, vp, i32, vp, vp, vp, vp]
         L.ap_guiding_reset.argtypes = [vp]
+        L.ap_nrc_create.argtypes = [ctypes.c_uint64, vp]
+        L.ap_nrc_destroy.argtypes = [vp]
+        L.ap_nrc_set_bounds.argtypes = [vp, vp, vp]
+        L.ap_nrc_get_params.argtypes = [vp, vp]
+        L.ap_nrc_set_params.argtypes = [vp, vp]
+        L.ap_nrc_infer.argtypes = [vp, vp, sz, vp, vp]
+        L.ap_nrc_train_step.argtypes = [vp, vp, sz, ctypes.c_float, vp]
+        L.ap_nrc_gradient.argtypes = [vp, vp, sz, vp, vp, vp]
+        L.ap_nrc_last_gradient.argtypes = [vp, vp]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
@@ -445,6 +456,70 @@ def xyz_to_srgb(xyz):
     check(lib().ap_xyz_to_srgb(d_in.ptr, n, d_out.ptr, None))
     synchronize()
     return d_out.to_numpy(np.float32, a.size).reshape(a.shape)
+
+
+NRC_PARAMS = 21504
+
+
+class RadianceCache:
+    """The learned radiance cache (include/aperture/ap_nrc.h)."""
+
+    def __init__(self, seed=1):
+        self._h = ctypes.c_void_p()
+        check(lib().ap_nrc_create(seed, ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def set_bounds(self, bmin, bmax):
+        lo = np.ascontiguousarray(bmin, np.float32)
+        hi = np.ascontiguousarray(bmax, np.float32)
+        check(lib().ap_nrc_set_bounds(self.handle, _ptr(lo), _ptr(hi)))
+
+    def params(self):
+        p = np.zeros(NRC_PARAMS, np.float32)
+        check(lib().ap_nrc_get_params(self.handle, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        host = np.ascontiguousarray(p, np.float32)
+        check(lib().ap_nrc_set_params(self.handle, _ptr(host)))
+
+    def infer(self, queries):
+        """Host (n, 16) query records -> host (n, 3) RGB."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
+        d_q, d_o = DeviceBuffer.from_numpy(q), DeviceBuffer(max(len(q) * 12, 4))
+        check(lib().ap_nrc_infer(self.handle, d_q.ptr, len(q), d_o.ptr, None))
+        synchronize()
+        return d_o.to_numpy(np.float32, len(q) * 3).reshape(-1, 3)
+
+    def gradient(self, samples):
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 20)
+        d_s = DeviceBuffer.from_numpy(s)
+        g = np.zeros(NRC_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        check(lib().ap_nrc_gradient(self.handle, d_s.ptr, len(s), _ptr(g), ctypes.byref(loss), None))
+        return float(loss.value), g
+
+    def train_step(self, samples_device, n, lr=1e-3, stream=None):
+        check(lib().ap_nrc_train_step(self.handle, _ptr(samples_device), n, lr, _stream(stream)))
+
+    def last_gradient(self):
+        g = np.zeros(NRC_PARAMS, np.float32)
+        check(lib().ap_nrc_last_gradient(self.handle, _ptr(g)))
+        return g
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().ap_nrc_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def build_bvh(scene_objects, leaf_size=8, bin_count=12):
