@@ -118,6 +118,25 @@ class RenderParams(ctypes.Structure):
         return p
 
 
+class NrcRenderParams(ctypes.Structure):
+    """ap_nrc_render_params (include/aperture/ap_nrc.h)."""
+    _fields_ = [("spread", ctypes.c_float), ("train_paths", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
+                ("batch", ctypes.c_uint32), ("learning_rate", ctypes.c_float), ("unbiased_every", ctypes.c_uint32)]
+
+    @classmethod
+    def defaults(cls, **kw):
+        p = cls()
+        lib().ap_nrc_render_params_default(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class NrcFrameStats(ctypes.Structure):
+    _fields_ = [("queries", ctypes.c_uint32), ("train_paths", ctypes.c_uint32), ("samples", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
 class Framebuffers(ctypes.Structure):
     _fields_ = [("hdr", ctypes.c_void_p), ("odd", ctypes.c_void_p), ("even", ctypes.c_void_p),
                 ("sum_xyz", ctypes.c_void_p), ("primary_id", ctypes.c_void_p), ("ray_count", ctypes.c_void_p)]
@@ -150,7 +169,8 @@ EXPORTED_SYMBOLS = (
     "ap_render_params_default", "ap_render", "ap_render_offline", "ap_xyz_to_srgb", "ap_synth_box_scene",
     "ap_synth_heightfield_scene", "ap_guiding_reset", "ap_guiding_export",
     "ap_nrc_create", "ap_nrc_destroy", "ap_nrc_set_bounds", "ap_nrc_get_params", "ap_nrc_set_params",
-    "ap_nrc_infer", "ap_nrc_train_step", "ap_nrc_gradient", "ap_nrc_last_gradient",
+    "ap_nrc_infer", "ap_nrc_train_step", "ap_nrc_gradient", "ap_nrc_last_gradient", "ap_nrc_get_bounds",
+    "ap_nrc_render_params_default", "ap_render_nrc", "ap_nrc_last_frame",
 )
 
 
@@ -180,6 +200,9 @@ def lib():
         L.ap_nrc_train_step.argtypes = [vp, vp, sz, ctypes.c_float, vp]
         L.ap_nrc_gradient.argtypes = [vp, vp, sz, vp, vp, vp]
         L.ap_nrc_last_gradient.argtypes = [vp, vp]
+        L.ap_nrc_get_bounds.argtypes = [vp, vp, vp]
+        L.ap_render_nrc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.ap_nrc_last_frame.argtypes = [vp, vp, vp, sz]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
@@ -245,7 +268,7 @@ def _ptr(x):
     if isinstance(x, int):
         return x
     if isinstance(x, np.ndarray):
-        return x.ctypes.data
+        return x.ctypes.data_as(ctypes.c_void_p)  # keeps a temporary array alive through the call
     if isinstance(x, DeviceBuffer):
         return x.ptr
     return x.data_ptr()
@@ -477,6 +500,11 @@ class RadianceCache:
         hi = np.ascontiguousarray(bmax, np.float32)
         check(lib().ap_nrc_set_bounds(self.handle, _ptr(lo), _ptr(hi)))
 
+    def bounds(self):
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        check(lib().ap_nrc_get_bounds(self.handle, _ptr(lo), _ptr(hi)))
+        return lo, hi
+
     def params(self):
         p = np.zeros(NRC_PARAMS, np.float32)
         check(lib().ap_nrc_get_params(self.handle, _ptr(p)))
@@ -631,6 +659,22 @@ class SceneRenderer:
         fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id), _ptr(ray_count))
         check(lib().ap_render(self.handle, ctypes.byref(camera), ctypes.byref(params), ctypes.byref(fb),
                               _stream(stream)))
+
+    def render_nrc(self, camera, params, cache, nrc_params, hdr, odd=None, even=None, sum_xyz=None,
+                   primary_id=None, stream=None, ray_count=None):
+        """render() with the learned radiance cache `cache` (a RadianceCache)."""
+        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id), _ptr(ray_count))
+        check(lib().ap_render_nrc(self.handle, ctypes.byref(camera), ctypes.byref(params), cache.handle,
+                                  ctypes.byref(nrc_params), ctypes.byref(fb), _stream(stream)))
+
+    def nrc_last_frame(self, samples=True):
+        """Counters of the last render_nrc frame and its training samples, (n, 20) float32."""
+        st = NrcFrameStats()
+        check(lib().ap_nrc_last_frame(self.handle, ctypes.byref(st), None, 0))
+        out = np.zeros((st.samples, 20), np.float32)
+        if samples and st.samples:
+            check(lib().ap_nrc_last_frame(self.handle, ctypes.byref(st), _ptr(out), st.samples))
+        return st, out
 
     def render_offline(self, camera, params):
         """SceneRenderer::Render(size, origin, dir, spp): host (H, W, 3) linear XYZ."""

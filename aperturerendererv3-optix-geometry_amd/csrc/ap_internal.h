@@ -91,6 +91,8 @@ int bvh_max_depth(const HostBVH& bvh);
 
 }  // namespace ap
 
+struct ap_nrc;  // learned radiance cache handle (ap_nrc.hip)
+
 namespace ap {
 struct RenderState;  // integrator buffers of a scene (ap_render.hip)
 struct GuideDevice;  // guiding probes and build buffers of a scene (ap_guiding.hip)
@@ -129,4 +131,17 @@ int spectrum_get(const f4** rgb_lut, const f4** lambda_lut, const float** tables
 hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
                         ap_trace_stats* stats, bool any_hit, hipStream_t stream,
                         const uint32_t* dyn_n = nullptr);
+
+// Radiance cache entry points for the integrator (ap_nrc.hip).
+struct NrcState;
+NrcState* nrc_state(ap_nrc* h);
+void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]);
+// Inference over min(*dyn_n, n_cap) query records (16 floats each); output
+// row r is record rows[r].
+int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const uint32_t* dyn_n, uint32_t n_cap,
+                  float* out, hipStream_t st);
+// `steps` Adam steps on batches of `batch` rows drawn from the first *pool_n
+// samples (20 floats each) of `pool`; nothing changes when *pool_n is 0.
+int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_n, uint32_t batch, uint32_t steps,
+                   float lr, uint64_t seed, hipStream_t st);
 }  // namespace ap

@@ -109,16 +109,21 @@ __device__ void load_weights(const uint16_t* __restrict__ w, uint16_t* sw) {
     }
 }
 
-// Forward pass over rows [0, n) of records (stride floats each). With TRAIN,
-// stores the input features and hidden activations as [feature][row] BF16
-// (acts + (layer * 64 + feature) * n_cap + row).
+// Forward pass over rows [0, n) of records (stride floats each); with dyn_n
+// over rows [0, min(*dyn_n, n)); with `rows`, row r reads record rows[r].
+// With TRAIN, stores the input features and
+// hidden activations as [feature][row] BF16 (acts + (layer * 64 + feature) *
+// n_cap + row).
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict__ w, Bounds bnd,
                                                       const float* __restrict__ rec, int stride, uint32_t n,
-                                                      float* __restrict__ out, uint16_t* __restrict__ acts,
-                                                      uint32_t n_cap) {
+                                                      const uint32_t* __restrict__ dyn_n,
+                                                      const int32_t* __restrict__ rows, float* __restrict__ out,
+                                                      uint16_t* __restrict__ acts, uint32_t n_cap) {
     __shared__ __attribute__((aligned(16))) uint16_t sw[(NL * NF + NOP) * LS];
     __shared__ __attribute__((aligned(16))) uint16_t sa[4][16 * LS];
+    if (dyn_n) n = min(*dyn_n, n);
+    if (blockIdx.x * 64u >= n) return;
     load_weights(w, sw);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -129,7 +134,8 @@ __global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict
         {
             float f[NF];
             if (r < n) {
-                encode(rec + static_cast<size_t>(r) * stride, bnd, f);
+                const size_t src = rows ? static_cast<size_t>(rows[r]) : static_cast<size_t>(r);
+                encode(rec + src * stride, bnd, f);
             } else {
                 for (int k = 0; k < NF; ++k) f[k] = 0.0f;
             }
@@ -177,14 +183,18 @@ __global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict
 }
 
 // Relative L2 loss: dL/dy = 2 (y - t) / (y^2 + 0.01) / (3n), stored BF16 as
-// [o][row] (o < 16, rows 3..15 zero); per-block loss partials.
+// [o][row] (o < 16, rows 3..15 zero); per-block loss partials. A gate that
+// reads 0 zeroes every delta (an empty sample pool).
 __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, const float* __restrict__ rec,
-                                                   uint32_t n, uint32_t n_cap, uint16_t* __restrict__ dout,
-                                                   float* __restrict__ loss_part) {
+                                                   uint32_t n, uint32_t n_cap, const uint32_t* __restrict__ gate,
+                                                   uint16_t* __restrict__ dout, float* __restrict__ loss_part) {
     __shared__ float red[256];
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    const bool open = gate == nullptr || *gate != 0u;
     float l = 0.0f;
-    if (r < n) {
+    if (r < n && !open) {
+        for (int o = 0; o < NOP; ++o) dout[static_cast<size_t>(o) * n_cap + r] = 0;
+    } else if (r < n) {
         const float norm = 1.0f / (3.0f * static_cast<float>(n));
         for (int o = 0; o < NOP; ++o) {
             uint16_t h = 0;
@@ -313,9 +323,11 @@ __global__ void reduce_grad_kernel(const float* __restrict__ part, int chunks, f
 // Adam (the oracle's or_nrc_adam), then the BF16 copies W[o][i] and W^T[i][o].
 __global__ void adam_kernel(float* __restrict__ params, float* __restrict__ m, float* __restrict__ v,
                             const float* __restrict__ grad, float lr, float beta1, float beta2, float eps, float c1,
-                            float c2, uint16_t* __restrict__ w, uint16_t* __restrict__ wt) {
+                            float c2, uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
+                            const uint32_t* __restrict__ gate) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= NPAR) return;
+    if (gate && *gate == 0u) return;
     const float g = grad[p];
     const float mm = beta1 * m[p] + (1.0f - beta1) * g;
     const float vv = beta2 * v[p] + (1.0f - beta2) * (g * g);
@@ -332,6 +344,26 @@ __global__ void adam_kernel(float* __restrict__ params, float* __restrict__ m, f
     else wt[NL * NF * NF + i * WT_OUT_STRIDE + o] = h;
 }
 
+// Row j of step `step`: sample (off + (step * batch + j) * P) mod N of the
+// pool, P = 2^31 - 1 a prime above any pool size, so that the rows of the
+// steps of a frame walk the pool without repetition while they last.
+__global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ pool,
+                                                     const uint32_t* __restrict__ pool_n, uint32_t batch,
+                                                     uint32_t step, uint64_t off, float* __restrict__ out) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= batch) return;
+    const uint32_t n = *pool_n;
+    float4* dst = reinterpret_cast<float4*>(out + static_cast<size_t>(j) * SREC);
+    if (n == 0u) {
+        for (int k = 0; k < SREC / 4; ++k) dst[k] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+        return;
+    }
+    const uint64_t idx =
+        (off % n + (static_cast<uint64_t>(step) * batch + j) % n * 2147483647ull % n) % n;
+    const float4* src = reinterpret_cast<const float4*>(pool + idx * SREC);
+    for (int k = 0; k < SREC / 4; ++k) dst[k] = src[k];
+}
+
 }  // namespace
 
 struct NrcState {
@@ -343,6 +375,8 @@ struct NrcState {
     uint16_t *acts = nullptr, *deltas = nullptr, *dout = nullptr;
     float *y = nullptr, *part = nullptr, *loss_part = nullptr;
     float last_loss = 0.0f;
+    float* batch = nullptr;   // gathered training rows of nrc_train_pool
+    uint32_t batch_cap = 0;
 };
 
 namespace {
@@ -391,18 +425,18 @@ inline unsigned blocks_for(uint32_t n) {
 int nrc_infer(NrcState* s, const float* queries, size_t n, float* out, hipStream_t st) {
     if (n == 0) return AP_OK;
     hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(static_cast<uint32_t>(n))), dim3(256), 0, st, s->w,
-                       s->bounds, queries, QREC, static_cast<uint32_t>(n), out, nullptr, 0u);
+                       s->bounds, queries, QREC, static_cast<uint32_t>(n), nullptr, nullptr, out, nullptr, 0u);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: infer");
 }
 
 // Forward, loss and backward of one batch; the gradient lands in s->grad.
-int nrc_gradient(NrcState* s, const float* samples, uint32_t n, hipStream_t st) {
+int nrc_gradient(NrcState* s, const float* samples, uint32_t n, hipStream_t st, const uint32_t* gate = nullptr) {
     int rc = ensure_train(s, n);
     if (rc != AP_OK) return rc;
     hipLaunchKernelGGL(forward_kernel<true>, dim3(blocks_for(n)), dim3(256), 0, st, s->w, s->bounds, samples, SREC, n,
-                       s->y, s->acts, n);
-    hipLaunchKernelGGL(loss_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->y, samples, n, n, s->dout,
+                       nullptr, nullptr, s->y, s->acts, n);
+    hipLaunchKernelGGL(loss_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->y, samples, n, n, gate, s->dout,
                        s->loss_part);
     hipLaunchKernelGGL(backward_kernel, dim3(blocks_for(n)), dim3(256), 0, st, s->wt, n, n, s->acts, s->dout,
                        s->deltas);
@@ -414,21 +448,61 @@ int nrc_gradient(NrcState* s, const float* samples, uint32_t n, hipStream_t st) 
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: gradient");
 }
 
-int nrc_adam(NrcState* s, float lr, hipStream_t st) {
+int nrc_adam(NrcState* s, float lr, hipStream_t st, const uint32_t* gate = nullptr) {
     s->step += 1;
     const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
     const float c1 = 1.0f - std::pow(b1, static_cast<float>(s->step));
     const float c2 = 1.0f - std::pow(b2, static_cast<float>(s->step));
     hipLaunchKernelGGL(adam_kernel, dim3((NPAR + 255) / 256), dim3(256), 0, st, s->params, s->m, s->v, s->grad, lr,
-                       b1, b2, eps, c1, c2, s->w, s->wt);
+                       b1, b2, eps, c1, c2, s->w, s->wt, gate);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: adam");
+}
+
+int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const uint32_t* dyn_n, uint32_t n_cap,
+                  float* out, hipStream_t st) {
+    if (n_cap == 0) return AP_OK;
+    hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(n_cap)), dim3(256), 0, st, s->w, s->bounds, queries,
+                       QREC, n_cap, dyn_n, rows, out, nullptr, 0u);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: infer");
+}
+
+int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_n, uint32_t batch, uint32_t steps, float lr,
+                   uint64_t seed, hipStream_t st) {
+    if (steps == 0) return AP_OK;
+    if (batch == 0 || batch % kChunkRows != 0 || batch > (1u << 24))
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc: batch must be a positive multiple of 256 up to 2^24");
+    if (s->batch_cap < batch) {
+        if (s->batch) (void)hipFree(s->batch);
+        s->batch = nullptr;
+        s->batch_cap = 0;
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->batch), static_cast<size_t>(batch) * SREC * 4);
+        if (e != hipSuccess) return hip_fail(e, "ap_nrc: batch buffer");
+        s->batch_cap = batch;
+    }
+    const uint64_t off = splitmix64(seed);
+    for (uint32_t k = 0; k < steps; ++k) {
+        hipLaunchKernelGGL(gather_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, pool, pool_n, batch, k, off,
+                           s->batch);
+        int rc = nrc_gradient(s, s->batch, batch, st, pool_n);
+        if (rc == AP_OK) rc = nrc_adam(s, lr, st, pool_n);
+        if (rc != AP_OK) return rc;
+    }
+    return AP_OK;
+}
+
+void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        s->bounds.lo[a] = lo[a];
+        s->bounds.hi[a] = hi[a];
+    }
 }
 
 void nrc_free(NrcState* s) {
     if (!s) return;
     void* ptrs[] = {s->params, s->m, s->v, s->grad, s->w, s->wt, s->acts, s->deltas, s->dout, s->y, s->part,
-                    s->loss_part};
+                    s->loss_part, s->batch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -441,6 +515,8 @@ using namespace ap;
 struct ap_nrc {
     NrcState* s;
 };
+
+ap::NrcState* ap::nrc_state(ap_nrc* h) { return h ? h->s : nullptr; }
 
 extern "C" {
 
@@ -551,6 +627,15 @@ int ap_nrc_train_step(ap_nrc* nrc, const float* samples, size_t n, float lr, voi
     int rc = nrc_gradient(nrc->s, samples, static_cast<uint32_t>(n), st);
     if (rc != AP_OK) return rc;
     return nrc_adam(nrc->s, lr, st);
+}
+
+int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]) {
+    if (!nrc || !bmin || !bmax) return fail(AP_ERR_INVALID_ARG, "ap_nrc_get_bounds: NULL argument");
+    for (int a = 0; a < 3; ++a) {
+        bmin[a] = nrc->s->bounds.lo[a];
+        bmax[a] = nrc->s->bounds.hi[a];
+    }
+    return AP_OK;
 }
 
 int ap_nrc_last_gradient(const ap_nrc* nrc, float* host_grad) {

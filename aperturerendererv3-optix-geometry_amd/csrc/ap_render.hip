@@ -26,6 +26,7 @@
 #include <tuple>
 #include <vector>
 
+#include "aperture/ap_nrc.h"
 #include "aperture/ap_render.h"
 #include "ap_internal.h"
 #include "ap_pcg.h"
@@ -72,6 +73,28 @@ struct alignas(16) ShadowPayload {
     uint32_t pad;
 };
 
+// Radiance-cache state of a path: a_0 and the running sqrt-spread sum of the
+// spread heuristic, NRC_* flags with the recorded training-vertex count in
+// bits 8.., and the training slot (-1 for a rendering-only path).
+struct alignas(16) NrcPath {
+    float a0, s;
+    uint32_t flags;
+    int32_t slot;
+};
+enum : uint32_t { NRC_TRAIN = 1u, NRC_UNBIASED = 2u, NRC_SUFFIX = 4u };
+constexpr int kNrcV = AP_NRC_TRAIN_VERTICES;
+constexpr int kNrcRec = 16;    // floats per query / training-vertex record
+constexpr int kNrcSample = 20; // floats per training sample
+
+struct NrcView {
+    NrcPath* paths;
+    float* queries;      // per path: the query record of its cache vertex
+    int32_t* qpath;      // compacted paths that ended into the cache
+    float* verts;        // per training slot, kNrcV records: query, [13] L_k, [14] M_k
+    uint32_t* counters;  // [0] cache queries, [1] training samples
+    float spread;
+};
+
 struct RenderState {
     int device = -1;
     int tri_mode = -1;
@@ -94,6 +117,20 @@ struct RenderState {
     float* eps = nullptr;             // INTERSECT_EPS(d) for d = 0 .. kEpsEntries-1
     float* target_tmp = nullptr;      // offline render target
     size_t target_cap = 0;
+    // radiance-cache buffers (ap_render_nrc)
+    size_t nrc_cap = 0;               // paths
+    NrcPath* nrc_paths = nullptr;
+    float* nrc_queries = nullptr;
+    int32_t* nrc_qpath = nullptr;
+    float* nrc_qrgb = nullptr;
+    size_t nrc_train_cap = 0;         // training slots
+    float* nrc_verts = nullptr;
+    uint32_t* nrc_vmask = nullptr;
+    uint32_t* nrc_offsets = nullptr;
+    float* nrc_pool = nullptr;
+    uint32_t* nrc_counters = nullptr;
+    uint32_t nrc_last_train = 0;      // training paths of the last frame
+    bool nrc_rendered = false;
 };
 
 constexpr int kEpsEntries = 1024;
@@ -102,7 +139,8 @@ void render_state_free(RenderState* r) {
     if (!r) return;
     void* ptrs[] = {r->geo, r->mats, r->lights, r->paths, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
                     r->hits, r->shadow_rays, r->shadow_payload, r->shadow_hits, r->counters, r->eps,
-                    r->target_tmp};
+                    r->target_tmp, r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_qrgb, r->nrc_verts,
+                    r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete r;
@@ -411,7 +449,15 @@ __device__ LightSample sample_light(const LightRec* __restrict__ lights, int n, 
     return ls;
 }
 
-template <bool GUIDE>
+// Writes the query record of a cache vertex (the layout of ap_nrc.h).
+__device__ __forceinline__ void nrc_record(float* __restrict__ q, f3 pos, f3 d, f3 n, const MatRec& M) {
+    float4* q4 = reinterpret_cast<float4*>(q);
+    q4[0] = float4{pos.x, pos.y, pos.z, -d.x};
+    q4[1] = float4{-d.y, -d.z, n.x, n.y};
+    q4[2] = float4{n.z, M.roughness, M.albedo_rgb[0], M.albedo_rgb[1]};
+}
+
+template <bool GUIDE, bool NRC>
 __global__ __launch_bounds__(kBlock) void shade_kernel(
     ShadeParams sp, const GeoRec* __restrict__ geo, const MatRec* __restrict__ mats,
     const LightRec* __restrict__ lights, const AnalyticRec* __restrict__ analytic, const f4* __restrict__ rgb_lut,
@@ -419,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     const int32_t* __restrict__ path_in, const ap_hit* __restrict__ hits, ap_ray* __restrict__ rays_out,
     int32_t* __restrict__ path_out, ap_ray* __restrict__ shadow_rays, ShadowPayload* __restrict__ shadow_pay,
     uint32_t* __restrict__ counters, int32_t* __restrict__ primary_id, int32_t primary_frame,
-    unsigned long long* __restrict__ ray_count, GuideView gv) {
+    unsigned long long* __restrict__ ray_count, GuideView gv, NrcView nv) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
     if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
@@ -434,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     const bool inside = (S.flags & PATH_INSIDE) != 0u;
     if (sp.depth == 0 && primary_id && S.frame == primary_frame) primary_id[S.pixel] = H.id;
 
-    bool emit_next = false, emit_shadow = false;
+    bool emit_next = false, emit_shadow = false, emit_query = false;
     ap_ray next{}, shadow{};
     ShadowPayload pay{pid, 0.0f, 0u, 0u};
 
@@ -554,6 +600,45 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             le_end = interact_basis(M.emissive_basis, M.emissive_scale, lb) * S.d65;
         }
         if (GUIDE) pg_endpoint(gv.recs[pid], le_end, S.ndd);
+        if (NRC) {
+            // Spread heuristic and cache termination (ap_nrc.h).
+            NrcPath P = nv.paths[pid];
+            const float cos_i = fabsf(dot(d, N));
+            const float dist2 = H.t * H.t;
+            if (sp.depth == 0) {
+                P.a0 = dist2 / (12.566370614359172f * cos_i);
+                P.s = 0.0f;
+            } else {
+                P.s = P.s + sqrtf(dist2 / (fmaxf(S.last_pdf, 1e-12f) * cos_i));
+            }
+            const bool scatter = !pure_light && !(M.shading_model == SM_GGX && !(fmaxf(0.0f, dot(N, -d)) > 0.0f));
+            bool cache_end = false;
+            if (scatter && !inside && P.s * P.s > nv.spread * P.a0) {
+                if (!(P.flags & NRC_TRAIN)) {
+                    cache_end = true;
+                } else if (!(P.flags & NRC_UNBIASED)) {
+                    if (P.flags & NRC_SUFFIX) {
+                        cache_end = true;
+                    } else {
+                        P.flags |= NRC_SUFFIX;
+                        P.s = 0.0f;
+                    }
+                }
+            }
+            if (cache_end) {
+                float* q = nv.queries + static_cast<size_t>(pid) * kNrcRec;
+                nrc_record(q, hi.pos, d, N, M);
+                reinterpret_cast<float4*>(q)[3] = float4{M.albedo_rgb[2], 0.0f, 0.0f, 0.0f};
+                emit_query = true;
+                terminate = true;
+            } else if ((P.flags & NRC_TRAIN) && scatter && (P.flags >> 8) < static_cast<uint32_t>(kNrcV)) {
+                float* v = nv.verts + (static_cast<size_t>(P.slot) * kNrcV + (P.flags >> 8)) * kNrcRec;
+                nrc_record(v, hi.pos, d, N, M);
+                reinterpret_cast<float4*>(v)[3] = float4{M.albedo_rgb[2], S.radiance, S.mask, 0.0f};
+                P.flags += 1u << 8;
+            }
+            nv.paths[pid] = P;
+        }
         if (terminate) {
             S.flags &= ~PATH_PREV_NEE;
         } else {
@@ -652,6 +737,121 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         shadow_rays[si] = shadow;
         shadow_pay[si] = pay;
     }
+    if (NRC) {
+        const uint32_t qi = wave_append(&nv.counters[0], emit_query);
+        if (emit_query) nv.qpath[qi] = pid;
+    }
+}
+
+// Training paths of a frame (the pixels offset + k * stride, k < n_train)
+// and the reset of the cache counters.
+__global__ __launch_bounds__(kBlock) void nrc_init_kernel(int32_t n_pix, uint32_t stride, uint32_t offset,
+                                                          uint32_t n_train, uint32_t unbiased_every,
+                                                          NrcPath* __restrict__ paths, uint32_t* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) { counters[0] = 0; counters[1] = 0; }
+    if (i >= static_cast<uint32_t>(n_pix)) return;
+    NrcPath P{0.0f, 0.0f, 0u, -1};
+    if (i >= offset && (i - offset) % stride == 0u) {
+        const uint32_t t = (i - offset) / stride;
+        if (t < n_train) {
+            P.flags = NRC_TRAIN | ((unbiased_every != 0u && t % unbiased_every == 0u) ? NRC_UNBIASED : 0u);
+            P.slot = static_cast<int32_t>(t);
+        }
+    }
+    paths[i] = P;
+}
+
+// Radiance of the cache vertices: throughput x the prediction (clamped at 0)
+// at the path's wavelength.
+__global__ __launch_bounds__(kBlock) void nrc_apply_kernel(const uint32_t* __restrict__ counters,
+                                                           const int32_t* __restrict__ qpath,
+                                                           const float* __restrict__ rgb, const f4* __restrict__ rgb_lut,
+                                                           PathState* __restrict__ paths) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= counters[0]) return;
+    PathState& S = paths[qpath[q]];
+    const f3 c = fmax3(f3{rgb[3 * q], rgb[3 * q + 1], rgb[3 * q + 2]}, f3{0.0f, 0.0f, 0.0f});
+    const float le = interact(rgb_lut, S.lambda_basis, c);
+    S.radiance = S.radiance + S.mask * le * S.d65;
+}
+
+__device__ __forceinline__ bool finite3(f3 v) {
+    return !(isnan(v.x) || isnan(v.y) || isnan(v.z) || isinf(v.x) || isinf(v.y) || isinf(v.z));
+}
+
+// Targets of the training vertices, in place of L_k / M_k; vmask marks the finite ones.
+__global__ __launch_bounds__(kBlock) void nrc_targets_kernel(uint32_t n_train, uint32_t stride, uint32_t offset,
+                                                             const NrcPath* __restrict__ npaths,
+                                                             const PathState* __restrict__ paths,
+                                                             float* __restrict__ verts, uint32_t* __restrict__ vmask) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_train) return;
+    const uint32_t pid = offset + t * stride;
+    const uint32_t nvx = min(npaths[pid].flags >> 8, static_cast<uint32_t>(kNrcV));
+    const PathState& S = paths[pid];
+    const float r_end = S.radiance;
+    const f3 xyz{S.xyz[0], S.xyz[1], S.xyz[2]};
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < nvx; ++k) {
+        float* v = verts + (static_cast<size_t>(t) * kNrcV + k) * kNrcRec;
+        const float L = (r_end - v[13]) / v[14];
+        const f3 rgb = xyz_to_srgb((xyz * L) / S.lambda_pdf);
+        if (finite3(rgb)) {
+            v[13] = rgb.x; v[14] = rgb.y; v[15] = rgb.z;
+            m |= 1u << k;
+        }
+    }
+    vmask[t] = m;
+}
+
+// Exclusive prefix sum of the per-slot sample counts in one block (slot
+// order, so the sample pool is the same on every run); the total goes to
+// counters[1].
+__global__ __launch_bounds__(1024) void nrc_scan_kernel(uint32_t n_train, const uint32_t* __restrict__ vmask,
+                                                        uint32_t* __restrict__ offsets, uint32_t* __restrict__ counters) {
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (n_train + 1023u) / 1024u;
+    const uint32_t lo = threadIdx.x * per, hi = min(lo + per, n_train);
+    uint32_t sum = 0;
+    for (uint32_t t = lo; t < hi; ++t) sum += __popc(vmask[t]);
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint32_t add = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t t = lo; t < hi; ++t) {
+        offsets[t] = run;
+        run += __popc(vmask[t]);
+    }
+    if (threadIdx.x == 1023u) counters[1] = part[1023];
+}
+
+// Training samples in slot, vertex order: query (13 floats), 3 zeros, target
+// RGB, pixel * 8 + vertex as uint32 bits.
+__global__ __launch_bounds__(kBlock) void nrc_compact_kernel(uint32_t n_train, uint32_t stride, uint32_t offset,
+                                                             const uint32_t* __restrict__ vmask,
+                                                             const uint32_t* __restrict__ offsets,
+                                                             const float* __restrict__ verts, float* __restrict__ pool) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_train) return;
+    const uint32_t m = vmask[t];
+    const uint32_t pix = offset + t * stride;
+    uint32_t j = offsets[t];
+    for (int k = 0; k < kNrcV; ++k) {
+        if (!((m >> k) & 1u)) continue;
+        const float* v = verts + (static_cast<size_t>(t) * kNrcV + k) * kNrcRec;
+        float* o = pool + static_cast<size_t>(j) * kNrcSample;
+        for (int e = 0; e < 13; ++e) o[e] = v[e];
+        o[13] = 0.0f; o[14] = 0.0f; o[15] = 0.0f;
+        o[16] = v[13]; o[17] = v[14]; o[18] = v[15];
+        reinterpret_cast<uint32_t*>(o)[19] = pix * 8u + static_cast<uint32_t>(k);
+        ++j;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void resolve_kernel(const uint32_t* __restrict__ counters,
@@ -743,6 +943,10 @@ __global__ void build_mats_kernel(const MatIn* __restrict__ in, int n, const f4*
     r.emissive_rgb[1] = m.emissive.y;
     r.emissive_rgb[2] = m.emissive.z;
     r.pad2 = 0;
+    r.albedo_rgb[0] = m.albedo.x;
+    r.albedo_rgb[1] = m.albedo.y;
+    r.albedo_rgb[2] = m.albedo.z;
+    r.pad3 = 0;
     out[i] = r;
 }
 
@@ -920,10 +1124,49 @@ bool camera_inside(const ap_scene* s, f3 o) {
     return false;
 }
 
+int ensure_nrc(RenderState* r, size_t cap, size_t train_cap) {
+    if (!r->nrc_counters) {
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->nrc_counters), 4 * sizeof(uint32_t));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(cache counters)");
+    }
+    int rc;
+    if (r->nrc_cap < cap) {
+        void* old[] = {r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_qrgb};
+        for (void* p : old)
+            if (p) (void)hipFree(p);
+        r->nrc_paths = nullptr; r->nrc_queries = nullptr; r->nrc_qpath = nullptr; r->nrc_qrgb = nullptr;
+        r->nrc_cap = 0;
+        if ((rc = dev_alloc(&r->nrc_paths, cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_queries, cap * kNrcRec)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_qpath, cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_qrgb, cap * 3)) != AP_OK) return rc;
+        r->nrc_cap = cap;
+    }
+    train_cap = std::max<size_t>(train_cap, 1);
+    if (r->nrc_train_cap < train_cap) {
+        void* old[] = {r->nrc_verts, r->nrc_vmask, r->nrc_offsets, r->nrc_pool};
+        for (void* p : old)
+            if (p) (void)hipFree(p);
+        r->nrc_verts = nullptr; r->nrc_vmask = nullptr; r->nrc_offsets = nullptr; r->nrc_pool = nullptr;
+        r->nrc_train_cap = 0;
+        if ((rc = dev_alloc(&r->nrc_verts, train_cap * kNrcV * kNrcRec)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_vmask, train_cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_offsets, train_cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_pool, train_cap * kNrcV * kNrcSample)) != AP_OK) return rc;
+        r->nrc_train_cap = train_cap;
+    }
+    return AP_OK;
+}
+
+// Frame hash of the training-path offset and the training draws.
+uint64_t nrc_frame_hash(uint64_t seed, int32_t frame) {
+    return splitmix64(splitmix64(static_cast<uint64_t>(frame)) ^ seed ^ 0x6e7263ull);
+}
+
 }  // namespace
 
 int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm, const ap_framebuffers* fb,
-                  hipStream_t stream) {
+                  hipStream_t stream, ap_nrc* cache = nullptr, const ap_nrc_render_params* np = nullptr) {
     if (!s || !cam || !prm || !fb || !fb->hdr) return fail(AP_ERR_INVALID_ARG, "ap_render: NULL argument");
     if (cam->width <= 0 || cam->height <= 0) return fail(AP_ERR_INVALID_ARG, "ap_render: bad resolution");
     if (prm->spp < 0 || prm->frame_index < 0 || prm->max_depth < 0 || prm->max_depth > kEpsEntries - 2)
@@ -951,11 +1194,34 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     // Frames per batch: about 4M paths in flight. With guiding, one frame at
     // a time: each frame samples the probes built from the frames before it
     // (SceneRenderer::Render then UpdatePathGuidingSample, scene.cu:308-386).
+    // With the radiance cache, one frame at a time too: each frame queries
+    // the cache the frames before it trained.
     const bool guiding = prm->use_guiding != 0;
-    const int32_t per_batch =
-        guiding ? 1 : std::max(1, std::min(prm->spp, static_cast<int32_t>((4u << 20) / static_cast<uint32_t>(n_pix))));
+    const bool nrc = cache != nullptr;
+    const int32_t per_batch = (guiding || nrc)
+        ? 1 : std::max(1, std::min(prm->spp, static_cast<int32_t>((4u << 20) / static_cast<uint32_t>(n_pix))));
     if ((rc = ensure_paths(r, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
     if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(n_pix))) != AP_OK) return rc;
+    NrcState* ns = nrc ? nrc_state(cache) : nullptr;
+    uint32_t n_stride = 1, n_train_max = 0;
+    if (nrc) {
+        if (np->train_paths > 0) {
+            n_stride = std::max<uint32_t>(1u, static_cast<uint32_t>(n_pix) / np->train_paths);
+            n_train_max = (static_cast<uint32_t>(n_pix) + n_stride - 1) / n_stride;
+        }
+        if ((rc = ensure_nrc(r, static_cast<size_t>(n_pix), n_train_max)) != AP_OK) return rc;
+        // Cache bounds: the scene's root box grown on each side by 1e-3 of its largest extent.
+        if (s->dev.root.valid) {
+            float lo[3], hi[3], ext = 0.0f;
+            for (int a = 0; a < 3; ++a) ext = std::max(ext, s->dev.root.bmax[a] - s->dev.root.bmin[a]);
+            const float m = 1.0e-3f * std::max(ext, 1.0e-3f);
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = s->dev.root.bmin[a] - m;
+                hi[a] = s->dev.root.bmax[a] + m;
+            }
+            nrc_set_bounds(ns, lo, hi);
+        }
+    }
 
     // Camera basis of RenderCamera (scene.cu:220-228), direction normalised
     // as Camera::SetDirection does (camera.hpp:38-41).
@@ -981,6 +1247,18 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
         const GuideView gv = guiding ? guide_view(s->guide) : GuideView{};
         hipLaunchKernelGGL(generate_kernel, grid_of(total), dim3(kBlock), 0, stream, cp, fp, lam_lut, tables, r->eps,
                            r->paths, r->rays[0], r->path_of[0], gv.recs);
+        NrcView nv{};
+        uint32_t n_offset = 0, n_train = 0;
+        if (nrc) {
+            const uint64_t h = nrc_frame_hash(prm->seed, fp.frame0);
+            if (n_train_max > 0) {
+                n_offset = static_cast<uint32_t>(h % n_stride);
+                n_train = std::min(n_train_max, (static_cast<uint32_t>(n_pix) - n_offset + n_stride - 1) / n_stride);
+            }
+            hipLaunchKernelGGL(nrc_init_kernel, grid_of(static_cast<size_t>(n_pix)), dim3(kBlock), 0, stream, n_pix,
+                               n_stride, n_offset, n_train, np->unbiased_every, r->nrc_paths, r->nrc_counters);
+            nv = NrcView{r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_verts, r->nrc_counters, np->spread};
+        }
         int in_q = 0;
         for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
             const int out_q = 1 - in_q;
@@ -989,18 +1267,22 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
             if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
             ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                            prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
+            auto* rc_ptr = reinterpret_cast<unsigned long long*>(fb->ray_count);
             if (guiding) {
-                hipLaunchKernelGGL(shade_kernel<true>, grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats,
-                                   r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
+                hipLaunchKernelGGL((shade_kernel<true, false>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
+                                   r->mats, r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
                                    r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
-                                   r->shadow_payload, r->counters, fb->primary_id, last_frame,
-                                   reinterpret_cast<unsigned long long*>(fb->ray_count), gv);
+                                   r->shadow_payload, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
+            } else if (nrc) {
+                hipLaunchKernelGGL((shade_kernel<false, true>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
+                                   r->mats, r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
+                                   r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
+                                   r->shadow_payload, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
             } else {
-                hipLaunchKernelGGL(shade_kernel<false>, grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats,
-                                   r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
+                hipLaunchKernelGGL((shade_kernel<false, false>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
+                                   r->mats, r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
                                    r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
-                                   r->shadow_payload, r->counters, fb->primary_id, last_frame,
-                                   reinterpret_cast<unsigned long long*>(fb->ray_count), gv);
+                                   r->shadow_payload, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
             }
             e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
@@ -1016,11 +1298,34 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
                 if (live == 0) break;
             }
         }
+        if (nrc) {
+            // Cache vertices, then the training samples of the frame.
+            if ((rc = nrc_infer_dyn(ns, r->nrc_queries, r->nrc_qpath, &r->nrc_counters[0], total, r->nrc_qrgb,
+                                    stream)) != AP_OK)
+                return rc;
+            hipLaunchKernelGGL(nrc_apply_kernel, grid_of(total), dim3(kBlock), 0, stream, r->nrc_counters,
+                               r->nrc_qpath, r->nrc_qrgb, rgb_lut, r->paths);
+            if (n_train > 0) {
+                hipLaunchKernelGGL(nrc_targets_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
+                                   n_offset, r->nrc_paths, r->paths, r->nrc_verts, r->nrc_vmask);
+                hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, n_train, r->nrc_vmask,
+                                   r->nrc_offsets, r->nrc_counters);
+                hipLaunchKernelGGL(nrc_compact_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
+                                   n_offset, r->nrc_vmask, r->nrc_offsets, r->nrc_verts, r->nrc_pool);
+            }
+            r->nrc_last_train = n_train;
+            r->nrc_rendered = true;
+        }
         hipLaunchKernelGGL(accumulate_kernel, grid_of(n_pix), dim3(kBlock), 0, stream, fp, r->paths, fb->hdr, fb->odd,
                            fb->even, fb->sum_xyz, gv.recs, guiding ? guide_slots(s->guide) : nullptr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
         if (guiding && (rc = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK) return rc;
+        if (nrc && n_train > 0 && np->train_steps > 0) {
+            rc = nrc_train_pool(ns, r->nrc_pool, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
+                                nrc_frame_hash(prm->seed, fp.frame0) ^ 0x747261696eull, stream);
+            if (rc != AP_OK) return rc;
+        }
     }
     return AP_OK;
 }
@@ -1074,6 +1379,50 @@ int ap_render_offline(ap_scene* scene, const ap_camera* camera, const ap_render_
     }
     (void)hipFree(target);
     return rc;
+}
+
+void ap_nrc_render_params_default(ap_nrc_render_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->spread = 0.01f;
+    p->train_paths = 32768;
+    p->train_steps = 4;
+    p->batch = 16384;
+    p->learning_rate = 0.005f;
+    p->unbiased_every = 16;
+}
+
+int ap_render_nrc(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, ap_nrc* cache,
+                  const ap_nrc_render_params* nrc_params, const ap_framebuffers* fb, void* stream) {
+    if (!cache || !nrc_params) return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: NULL cache or parameters");
+    if (params && params->use_guiding)
+        return fail(AP_ERR_UNSUPPORTED, "ap_render_nrc: guiding is not supported with the radiance cache");
+    if (!(nrc_params->spread > 0.0f) || !(nrc_params->learning_rate >= 0.0f))
+        return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: spread must be > 0 and learning_rate >= 0");
+    if (nrc_params->train_steps > 0 &&
+        (nrc_params->batch == 0 || nrc_params->batch % 256 != 0 || nrc_params->batch > (1u << 24)))
+        return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: batch must be a positive multiple of 256 up to 2^24");
+    return render_frames(scene, camera, params, fb, static_cast<hipStream_t>(stream), cache, nrc_params);
+}
+
+int ap_nrc_last_frame(ap_scene* scene, ap_nrc_frame_stats* stats, float* host_samples, size_t capacity) {
+    if (!scene || !stats) return fail(AP_ERR_INVALID_ARG, "ap_nrc_last_frame: NULL argument");
+    std::memset(stats, 0, sizeof(*stats));
+    RenderState* r = scene->render;
+    if (!r || !r->nrc_rendered) return fail(AP_ERR_INVALID_ARG, "ap_nrc_last_frame: no ap_render_nrc frame yet");
+    uint32_t c[2] = {0, 0};
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(c, r->nrc_counters, sizeof(c), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "ap_nrc_last_frame");
+    stats->queries = c[0];
+    stats->train_paths = r->nrc_last_train;
+    stats->samples = r->nrc_last_train > 0 ? c[1] : 0u;
+    if (host_samples && stats->samples > 0 && capacity > 0) {
+        const size_t n = std::min<size_t>(capacity, stats->samples);
+        e = hipMemcpy(host_samples, r->nrc_pool, n * kNrcSample * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "ap_nrc_last_frame: samples");
+    }
+    return AP_OK;
 }
 
 int ap_xyz_to_srgb(const float* xyz, size_t n, float* rgb, void* stream) {

@@ -36,6 +36,8 @@ struct alignas(16) MatRec {
     int32_t pad;
     float emissive_rgb[3];   // the GBuffer emissive (guiding endpoints query it with a MIS weight)
     int32_t pad2;
+    float albedo_rgb[3];     // the GBuffer albedo (radiance-cache query records)
+    int32_t pad3;
 };
 
 AP_HD float interact_basis(f4 rgb_basis, float scale, f4 lambda_basis) {

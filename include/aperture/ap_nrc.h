@@ -21,6 +21,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "ap_render.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -47,6 +49,65 @@ int ap_nrc_train_step(ap_nrc* nrc, const float* samples, size_t n, float lr, voi
 int ap_nrc_gradient(ap_nrc* nrc, const float* samples, size_t n, float* host_grad, float* loss, void* stream);
 /* Gradient of the last training step or gradient call. */
 int ap_nrc_last_gradient(const ap_nrc* nrc, float* host_grad);
+/* Bounds the position encoding uses (set by ap_nrc_set_bounds or ap_render_nrc). */
+int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]);
+
+/* Rendering with the cache.
+ *
+ * A camera path ends into the cache at the first surface vertex x_n where
+ * the path spread a(x_1..x_n) = (sum_{i=2..n} sqrt(|x_{i-1} - x_i|^2 /
+ * (p(w_{i-1}) |cos theta_i|)))^2 exceeds spread * a_0, a_0 = |x_0 - x_1|^2 /
+ * (4 pi |cos theta_1|): its radiance there is the cache's RGB prediction
+ * (clamped at 0) turned into the path's wavelength by the RGB-to-spectrum
+ * basis, times the path throughput. No NEE is done at that vertex; the
+ * vertex's own emission is added as usual. Only vertices that scatter
+ * (not lights, not GGX seen from behind) and paths that did not start
+ * inside an object end into the cache.
+ *
+ * Training paths: the pixels p = offset + k * stride of a frame (stride =
+ * max(1, pixels / train_paths), offset from SplitMix64 of the frame). They
+ * go on past the first crossing (the spread restarts there) and end into
+ * the cache at the second; every unbiased_every-th never does. Each of their
+ * scattering vertices (up to AP_NRC_TRAIN_VERTICES, the cache vertex
+ * excluded) gives a training sample: its query record and the target
+ * XYZ2SRGBLinearD65(xyz(lambda) * (L_end - L_k) / M_k / p(lambda)), L_k
+ * the path radiance after the vertex's emission, M_k the throughput on
+ * arrival. After each frame, train_steps Adam steps run on batches of
+ * `batch` samples drawn without repetition (while they last) from the frame's
+ * samples. */
+#define AP_NRC_TRAIN_VERTICES 8
+
+typedef struct ap_nrc_render_params {
+    float spread;            /* c of the spread heuristic (0.01) */
+    uint32_t train_paths;    /* training paths per frame (32768); 0: the cache stays frozen */
+    uint32_t train_steps;    /* Adam steps per frame (4) */
+    uint32_t batch;          /* samples per step, a positive multiple of 256 (16384) */
+    float learning_rate;     /* Adam step size (0.005) */
+    uint32_t unbiased_every; /* every k-th training path never ends into the cache (16); 0: none */
+} ap_nrc_render_params;
+
+typedef struct ap_nrc_frame_stats {
+    uint32_t queries;        /* paths that ended into the cache */
+    uint32_t train_paths;    /* training paths */
+    uint32_t samples;        /* training samples */
+    uint32_t pad;
+} ap_nrc_frame_stats;
+
+void ap_nrc_render_params_default(ap_nrc_render_params* p);
+
+/* ap_render (include/aperture/ap_render.h) with the cache: frames one at a
+ * time, each rendered with the cache as the frames before it trained it,
+ * then trained on its own samples. The scene's root box, grown by 1e-3 of
+ * its extent on each side, becomes the cache bounds. Guiding is not
+ * supported with the cache (AP_ERR_UNSUPPORTED). Asynchronous on `stream`. */
+int ap_render_nrc(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, ap_nrc* cache,
+                  const ap_nrc_render_params* nrc_params, const ap_framebuffers* fb, void* stream);
+
+/* The last ap_render_nrc frame of the scene: counters and, if host_samples
+ * is not NULL, min(samples, capacity) training samples (20 floats; the last
+ * float holds pixel * 8 + vertex as uint32 bits) in pixel, vertex order.
+ * Synchronous. */
+int ap_nrc_last_frame(ap_scene* scene, ap_nrc_frame_stats* stats, float* host_samples, size_t capacity);
 
 #ifdef __cplusplus
 }

@@ -16,10 +16,17 @@
 // restriction: PCG32 replaces curand (stream per pixel, start state hashed
 // from seed, frame and pixel with SplitMix64); rsqrtf is 1/sqrtf; triangles may carry their object's
 // material (param) instead of IntersectTriangleObject's default GBuffer;
-// glass, textures and path guiding are not restated; a path that misses while
-// "inside a medium" ends (the reference continues from an undefined point).
+// glass and textures are not restated; a path that misses while "inside a
+// medium" ends (the reference continues from an undefined point).
+// Directional path guiding (or_guiding.h) follows render.cuh:546-639 and
+// scene.cu:308-386.
+// The learned radiance cache has no reference counterpart: CalculateRadianceNRC
+// restates the core's cache termination (spread heuristic, training paths,
+// training targets; include/aperture/ap_nrc.h) with the network evaluated by
+// or_nrc.cpp's FP64 forward with BF16 rounding points.
 // Parity pins: white-furnace and analytic-irradiance scenes
-// (tests/test_render_cpu.py), NEE pdf pairing and power-heuristic identities.
+// (tests/test_render_cpu.py), NEE pdf pairing and power-heuristic identities;
+// guiding and cache pins in tests/test_guiding_cpu.py and tests/test_nrc_cpu.py.
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -790,6 +797,218 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
     return {clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
 }
 
+// Radiance-cache options of one frame (include/aperture/ap_nrc.h).
+struct NrcOpts {
+    float spread;
+    const float* params;
+    float bmin[3], bmax[3];
+};
+constexpr int NRC_V = 8;
+
+// Per-path training output: up to NRC_V samples of 20 floats, validity.
+struct NrcTrain {
+    float rec[NRC_V][20];
+    bool valid[NRC_V];
+};
+
+// CalculateRadiance without guiding, with the core's cache termination: a
+// path ends into the cache at the first scattering vertex where the squared
+// spread sum exceeds spread * a0; a training path restarts the sum at its
+// first crossing and ends at the second (an unbiased one never ends there).
+// Training paths record each scattering vertex before the cache vertex.
+static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir,
+                                   bool inside0, const NrcOpts& N, bool train, bool unbiased, int pixel,
+                                   NrcTrain* out, bool* queried) {
+    float lam, lpdf;
+    {
+        const float r0 = rng.rand1(), r1 = rng.rand1(), r2 = rng.rand1();
+        SampleCIE1931(r0, r1, r2, &lam, &lpdf);
+    }
+    float lb[4];
+    QueryLambda(lam, lb);
+    const float d65 = D65Norm(lam);
+    const float3 xyz = LambdaToXYZ(lam);
+    int ndd = 0;
+    dir = normalize(dir);
+    float radiance = 0.0f, mask = 1.0f;
+    float3 ro = origin, rd = dir;
+    bool inside = inside0;
+    bool prevNEE = false;
+    float lastPdf = 0.0f;
+    float3 lastPos{0, 0, 0};
+    float a0 = 0.0f, spreadSum = 0.0f;
+    bool suffix = false;
+    int nv = 0;
+    float vrec[NRC_V][16];
+    bool cacheEnd = false;
+    float query[16];
+    if (out) for (int k = 0; k < NRC_V; ++k) out->valid[k] = false;
+    if (queried) *queried = false;
+    auto record = [&](float* q, float3 pos, const Ctx& c) {
+        q[0] = pos.x; q[1] = pos.y; q[2] = pos.z;
+        q[3] = -rd.x; q[4] = -rd.y; q[5] = -rd.z;
+        q[6] = c.normal.x; q[7] = c.normal.y; q[8] = c.normal.z;
+        q[9] = c.gb.roughness;
+        q[10] = c.gb.albedo.x; q[11] = c.gb.albedo.y; q[12] = c.gb.albedo.z;
+        q[13] = q[14] = q[15] = 0.0f;
+    };
+    for (int depth = 0; depth < P.max_depth; ++depth) {
+        const float eps = INTERSECT_EPS(ndd);
+        int id;
+        float t;
+        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t);
+        if (id < 0) {
+            (void)rng.rand1();
+            if (!inside) {
+                const float le = P.sky ? InteractC(lb, SkyLighting(rd)) : 0.0f;
+                radiance = radiance + mask * le * d65;
+            }
+            break;
+        }
+        Ctx c = HitContext(S, id, ro, rd, t);
+        const float3 pos0 = c.pos;
+        const int model = S.objs[id].material.shading_model;
+        SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
+        sc.brdf = EvaluateBSDFWithCos(lb, model, rd, sc.next, c);
+        const float offEps = eps * std::fmax(1.0f, length(c.pos));
+        c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
+        inside = sc.inside;
+        if (c.lightIndex >= 0) {
+            float misW = 1.0f;
+            if (P.use_nee && depth > 0 && prevNEE) {
+                const float pB = std::fmax(lastPdf, 0.0f);
+                const float pL = NEEPDFOmegaFromHit(S.objs[id], S.lights[c.lightIndex], lastPos, c.pos, c.normal);
+                if (pL >= 0.0f && pB > 0.0f) misW = PowerHeuristic(pB, pL);
+            }
+            radiance = radiance + mask * InteractC(lb, sc.emissive) * d65 * misW;
+        }
+        // Spread heuristic (the core's shade kernel, NRC branch).
+        {
+            const float cosI = std::fabs(dot(rd, c.normal));
+            const float dist2 = t * t;
+            if (depth == 0) {
+                a0 = dist2 / (12.566370614359172f * cosI);
+                spreadSum = 0.0f;
+            } else {
+                spreadSum = spreadSum + std::sqrt(dist2 / (std::fmax(lastPdf, 1e-12f) * cosI));
+            }
+            const bool scatter = !sc.pureLight &&
+                                 !(model == AP_MAT_GGX && !(std::fmax(0.0f, dot(c.normal, -rd)) > 0.0f));
+            if (scatter && !inside && spreadSum * spreadSum > N.spread * a0) {
+                if (!train) {
+                    cacheEnd = true;
+                } else if (!unbiased) {
+                    if (suffix) {
+                        cacheEnd = true;
+                    } else {
+                        suffix = true;
+                        spreadSum = 0.0f;
+                    }
+                }
+            }
+            if (cacheEnd) {
+                record(query, pos0, c);
+                break;
+            }
+            if (train && scatter && nv < NRC_V) {
+                record(vrec[nv], pos0, c);
+                vrec[nv][13] = radiance;
+                vrec[nv][14] = mask;
+                ++nv;
+            }
+        }
+        if (sc.terminate) break;
+        bool neeValid = false;
+        const float rate = P.nee_always ? 1.0f : std::sqrt(saturate_(mask));
+        if (P.use_nee && ndd < P.nee_max_depth && !sc.delta && !inside && rng.rand1() < rate) {
+            const LightSample ls = NEESampleLight(S, c.pos, rng);
+            if (ls.valid) {
+                const float pAT = std::fmax(1e-20f, ls.pA * ls.pFacet);
+                const float3 lu = ls.point - c.pos;
+                const float d2 = std::fmax(dot(lu, lu), static_cast<float>(1.0e-12));
+                const float3 L = lu * (1.0f / std::sqrt(d2));
+                bool occluded = false;
+                {
+                    float3 sd = ls.point - c.pos;
+                    const float dist = length(sd);
+                    if (dist > 0.0f) {
+                        sd = sd / dist;
+                        const float seps = INTERSECT_EPS(depth) * std::fmax(1.0f, std::fmax(length(c.pos), length(ls.point)));
+                        const float tMax = std::fmax(0.0f, dist - seps);
+                        if (tMax > 0.0f) {
+                            int sid;
+                            float st;
+                            TraceOne(S.bvh, c.pos, sd, 0.0f, tMax, true, S.margin > 0.0f, S.margin, &sid, &st);
+                            occluded = sid >= 0;
+                        }
+                    }
+                }
+                if (occluded) {
+                    neeValid = true;
+                } else {
+                    const float cosX = std::fmax(0.0f, dot(c.normal, L));
+                    const float cosY = std::fmax(0.0f, dot(ls.normal, -L));
+                    if (!(cosX <= 0.0f || cosY <= 0.0f)) {
+                        const float bc = EvaluateBSDFWithCos(lb, model, rd, L, c);
+                        const float G2 = cosY / d2;
+                        const ap_gbuffer& lg = S.objs[ls.object].material.texture.base;
+                        const float resp = InteractC(lb, F3(lg.emissive));
+                        const float contribution = bc * resp * (G2 / pAT);
+                        neeValid = true;
+                        const float pLo = AreaToSolidAnglePDF(c.pos, ls.point, ls.normal, ls.pFacet * ls.pA);
+                        const float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
+                        radiance = radiance + mask * (contribution * PowerHeuristic(pLo, pBs)) * d65;
+                    }
+                }
+            }
+        }
+        prevNEE = neeValid;
+        lastPdf = sc.pdf;
+        lastPos = c.pos;
+        const float contrib = sc.brdf / std::fmax(sc.pdf, 1e-12f);
+        float p = 1.0f;
+        if (ndd >= P.rr_min_depth) {
+            const float basic = clampf(mask * 0.5f + mask * 0.5f, 0.05f, 0.95f);
+            p = lerp(basic, 1.0f, inside ? 0.99f : 0.0f);
+            if (rng.rand1() > p) break;
+        }
+        mask = mask * contrib;
+        mask = mask / p;
+        ro = c.pos;
+        rd = sc.next;
+        if (!sc.delta) ndd++;
+    }
+    if (cacheEnd) {
+        // The cache's prediction, clamped at 0, at the path's wavelength.
+        float feat[64], y[3];
+        or_nrc_encode(query, 1, N.bmin, N.bmax, feat);
+        or_nrc_forward(N.params, feat, 1, 1, y);
+        const float3 cv = fmax3(float3{y[0], y[1], y[2]}, float3{0.0f, 0.0f, 0.0f});
+        radiance = radiance + mask * InteractC(lb, cv) * d65;
+        if (queried) *queried = true;
+    }
+    if (out) {
+        for (int k = 0; k < nv; ++k) {
+            const float L = (radiance - vrec[k][13]) / vrec[k][14];
+            const float3 rgb = XYZ2SRGB((xyz * L) / lpdf);
+            if (std::isnan(rgb.x) || std::isnan(rgb.y) || std::isnan(rgb.z) || std::isinf(rgb.x) ||
+                std::isinf(rgb.y) || std::isinf(rgb.z))
+                continue;
+            float* o = out->rec[k];
+            for (int e = 0; e < 13; ++e) o[e] = vrec[k][e];
+            o[13] = o[14] = o[15] = 0.0f;
+            o[16] = rgb.x; o[17] = rgb.y; o[18] = rgb.z;
+            const uint32_t key = static_cast<uint32_t>(pixel) * 8u + static_cast<uint32_t>(k);
+            std::memcpy(&o[19], &key, 4);
+            out->valid[k] = true;
+        }
+    }
+    float r = radiance;
+    if (std::isnan(r) || std::isinf(r)) r = 0.0f;
+    const float3 pre = (xyz * r) / lpdf;
+    return {clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
+}
+
 static float GetArea(const ap_scene_object& o) {
     if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SDF) {
         const float ex = std::fabs(o.extent.x), ey = std::fabs(o.extent.y), ez = std::fabs(o.extent.z);
@@ -802,6 +1021,66 @@ static float GetArea(const ap_scene_object& o) {
         return 0.5f * length(cross(e1, e2));
     }
     return 0.0f;
+}
+
+// Scene, options and camera of a render call (RenderCamera, scene.cu:201-230;
+// light table, scene.cu:1056-1080).
+struct Setup {
+    Scene S;
+    RenderOpts P;
+    float3 camO, camD, right, up;
+    float tanX, tanY;
+    bool inside0;
+    int W, H;
+};
+
+static Setup MakeSetup(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                       const ap_render_params* prm, float cull_margin) {
+    Setup U{Scene{objs, n, bvh, {}, std::vector<int>(n, -1), prm->triangle_material, cull_margin},
+            RenderOpts{prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee, prm->nee_always, prm->sky},
+            {}, {}, {}, {}, 0.0f, 0.0f, false, cam->width, cam->height};
+    Scene& S = U.S;
+    float total = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        if (objs[i].material.shading_model == AP_MAT_LIGHT) {
+            S.light_index[i] = static_cast<int>(S.lights.size());
+            const float3 em = F3(objs[i].material.texture.base.emissive);
+            ap_light_info li{static_cast<int32_t>(i),
+                             std::fmax(GetArea(objs[i]) * dot(em, float3{0.2126f, 0.7152f, 0.0722f}), 1.0e-20f), 0.0f};
+            total += li.importance_weight;
+            S.lights.push_back(li);
+        }
+    }
+    for (auto& li : S.lights) li.select_prob = li.importance_weight / total;
+    U.camO = float3{cam->origin[0], cam->origin[1], cam->origin[2]};
+    U.camD = normalize(float3{cam->direction[0], cam->direction[1], cam->direction[2]});
+    const float FOV = cam->fov_deg * 3.1415926f / 180.0f;
+    const float aspect = static_cast<float>(cam->width) / std::fmax(1.0f, static_cast<float>(cam->height));
+    U.tanY = std::tan(0.5f * FOV);
+    U.tanX = U.tanY * aspect;
+    U.right = normalize(cross(U.camD, float3{0.0f, 1.0f, 0.0f}));
+    U.up = normalize(cross(U.camD, U.right));
+    for (size_t i = 0; i < n && !U.inside0; ++i) {
+        const ap_scene_object& ob = objs[i];
+        if (ob.type != AP_OBJ_CUBE && ob.type != AP_OBJ_SPHERE) continue;
+        const float3 lo = M3(ob.world_to_object) * (U.camO - F3(ob.center));
+        const float3 e = F3(ob.extent);
+        if (ob.type == AP_OBJ_CUBE)
+            U.inside0 = -e.x < lo.x && -e.y < lo.y && -e.z < lo.z && e.x > lo.x && e.y > lo.y && e.z > lo.z;
+        else
+            U.inside0 = dot(lo, lo) < max3(e) * max3(e);
+    }
+    return U;
+}
+
+// Camera ray of pixel p (two jitter draws from rng).
+static float3 CameraRay(const Setup& U, Rng& rng, int64_t p) {
+    const int i = static_cast<int>(p % U.W), j = static_cast<int>(p / U.W);
+    const float u = (i + rng.rand1()) / static_cast<float>(U.W);
+    const float v = (j + rng.rand1()) / static_cast<float>(U.H);
+    const float nx = u * 2.f - 1.f;
+    const float ny = static_cast<float>((1.0 - v) * 2.f - 1.f);
+    return normalize(U.camD + U.right * (nx * U.tanX) + U.up * (ny * U.tanY));
 }
 
 }  // namespace orc
@@ -821,40 +1100,11 @@ extern "C" {
 int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
                      const ap_render_params* prm, float cull_margin, float* hdr, double* sums, int32_t* primary,
                      int threads, or_guide* guide) {
-    Scene S{objs, n, bvh, {}, std::vector<int>(n, -1), prm->triangle_material, cull_margin};
-    float total = 0.0f;
-    for (size_t i = 0; i < n; ++i) {
-        if (objs[i].material.shading_model == AP_MAT_LIGHT) {
-            S.light_index[i] = static_cast<int>(S.lights.size());
-            const float3 em = F3(objs[i].material.texture.base.emissive);
-            ap_light_info li{static_cast<int32_t>(i),
-                             std::fmax(GetArea(objs[i]) * dot(em, float3{0.2126f, 0.7152f, 0.0722f}), 1.0e-20f), 0.0f};
-            total += li.importance_weight;
-            S.lights.push_back(li);
-        }
-    }
-    for (auto& li : S.lights) li.select_prob = li.importance_weight / total;
-    const RenderOpts P{prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee, prm->nee_always, prm->sky};
-    const float3 camO{cam->origin[0], cam->origin[1], cam->origin[2]};
-    const float3 camD = normalize(float3{cam->direction[0], cam->direction[1], cam->direction[2]});
-    const float FOV = cam->fov_deg * 3.1415926f / 180.0f;
-    const float aspect = static_cast<float>(cam->width) / std::fmax(1.0f, static_cast<float>(cam->height));
-    const float tanY = std::tan(0.5f * FOV), tanX = tanY * aspect;
-    const float3 right = normalize(cross(camD, float3{0.0f, 1.0f, 0.0f}));
-    const float3 up = normalize(cross(camD, right));
-    bool inside0 = false;
-    for (size_t i = 0; i < n && !inside0; ++i) {
-        const ap_scene_object& ob = objs[i];
-        if (ob.type != AP_OBJ_CUBE && ob.type != AP_OBJ_SPHERE) continue;
-        const float3 lo = M3(ob.world_to_object) * (camO - F3(ob.center));
-        const float3 e = F3(ob.extent);
-        if (ob.type == AP_OBJ_CUBE)
-            inside0 = -e.x < lo.x && -e.y < lo.y && -e.z < lo.z && e.x > lo.x && e.y > lo.y && e.z > lo.z;
-        else
-            inside0 = dot(lo, lo) < max3(e) * max3(e);
-    }
-    const int W = cam->width, H = cam->height;
-    const int64_t npix = static_cast<int64_t>(W) * H;
+    const Setup U = MakeSetup(bvh, objs, n, cam, prm, cull_margin);
+    const Scene& S = U.S;
+    const RenderOpts& P = U.P;
+    const bool inside0 = U.inside0;
+    const int64_t npix = static_cast<int64_t>(U.W) * U.H;
     if (prm->use_guiding) {
         // Frames in order; each renders with the probes built from the frames
         // before it, then folds its own samples in (scene.cu:308-386).
@@ -865,15 +1115,10 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
             const int frame = prm->frame_index + s;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
             for (int64_t p = 0; p < npix; ++p) {
-                const int i = static_cast<int>(p % W), j = static_cast<int>(p / W);
                 Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
-                const float u = (i + rng.rand1()) / static_cast<float>(W);
-                const float v = (j + rng.rand1()) / static_cast<float>(H);
-                const float nx = u * 2.f - 1.f;
-                const float ny = static_cast<float>((1.0 - v) * 2.f - 1.f);
-                const float3 rd = normalize(camD + right * (nx * tanX) + up * (ny * tanY));
+                const float3 rd = CameraRay(U, rng, p);
                 int prim = -1;
-                float3 res = CalculateRadiance(S, P, rng, camO, rd, inside0, &prim, &G, frame,
+                float3 res = CalculateRadiance(S, P, rng, U.camO, rd, inside0, &prim, &G, frame,
                                                &slots[static_cast<size_t>(p) * PG_COLLECT]);
                 res = fmax3(float3{0, 0, 0}, res);
                 const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
@@ -892,18 +1137,13 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
     }
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
     for (int64_t p = 0; p < npix; ++p) {
-        const int i = static_cast<int>(p % W), j = static_cast<int>(p / W);
         float3 acc{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
         for (int s = 0; s < prm->spp; ++s) {
             const int frame = prm->frame_index + s;
             Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
-            const float u = (i + rng.rand1()) / static_cast<float>(W);
-            const float v = (j + rng.rand1()) / static_cast<float>(H);
-            const float nx = u * 2.f - 1.f;
-            const float ny = static_cast<float>((1.0 - v) * 2.f - 1.f);
-            const float3 rd = normalize(camD + right * (nx * tanX) + up * (ny * tanY));
+            const float3 rd = CameraRay(U, rng, p);
             int prim = -1;
-            float3 res = CalculateRadiance(S, P, rng, camO, rd, inside0, &prim);
+            float3 res = CalculateRadiance(S, P, rng, U.camO, rd, inside0, &prim);
             res = fmax3(float3{0, 0, 0}, res);
             acc = lerp(acc, res, 1.0f / static_cast<float>(1 + frame));
             if (sums) {
@@ -923,6 +1163,74 @@ int or_render(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap
               int threads) {
     if (prm->use_guiding) return -1;
     return or_render_guided(bvh, objs, n, cam, prm, cull_margin, hdr, sums, primary, threads, nullptr);
+}
+
+// Frames of the core's ap_render_nrc with the cache frozen at `params`
+// (bounds bmin/bmax): hdr and sums as or_render; for the last frame, the
+// training samples at samples[(pixel * 8 + vertex) * 20] with valid flags,
+// and the number of paths that ended into the cache.
+int or_render_nrc(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                  const ap_render_params* prm, float cull_margin, const float* params, const float* bmin,
+                  const float* bmax, float spread, uint32_t train_paths, uint32_t unbiased_every, float* hdr,
+                  double* sums, float* samples, uint8_t* valid, uint32_t* n_queries, int threads) {
+    const Setup U = MakeSetup(bvh, objs, n, cam, prm, cull_margin);
+    const int64_t npix = static_cast<int64_t>(U.W) * U.H;
+    NrcOpts N{spread, params, {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]}};
+    uint32_t stride = 1, n_train_max = 0;
+    if (train_paths > 0) {
+        stride = std::max<uint32_t>(1u, static_cast<uint32_t>(npix) / train_paths);
+        n_train_max = (static_cast<uint32_t>(npix) + stride - 1) / stride;
+    }
+    uint32_t queries = 0;
+    for (int s = 0; s < prm->spp; ++s) {
+        const int frame = prm->frame_index + s;
+        const bool last = s == prm->spp - 1;
+        const uint64_t h = SplitMix64(SplitMix64(static_cast<uint64_t>(frame)) ^ prm->seed ^ 0x6e7263ull);
+        uint32_t offset = 0, n_train = 0;
+        if (n_train_max > 0) {
+            offset = static_cast<uint32_t>(h % stride);
+            n_train = std::min(n_train_max, (static_cast<uint32_t>(npix) - offset + stride - 1) / stride);
+        }
+        uint32_t frame_queries = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1) reduction(+ : frame_queries)
+        for (int64_t p = 0; p < npix; ++p) {
+            const uint32_t up = static_cast<uint32_t>(p);
+            bool train = false, unbiased = false;
+            if (up >= offset && (up - offset) % stride == 0u) {
+                const uint32_t t = (up - offset) / stride;
+                if (t < n_train) {
+                    train = true;
+                    unbiased = unbiased_every != 0u && t % unbiased_every == 0u;
+                }
+            }
+            Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
+            const float3 rd = CameraRay(U, rng, p);
+            NrcTrain tr;
+            bool q = false;
+            float3 res = CalculateRadianceNRC(U.S, U.P, rng, U.camO, rd, U.inside0, N, train, unbiased,
+                                              static_cast<int>(p), &tr, &q);
+            frame_queries += q ? 1u : 0u;
+            res = fmax3(float3{0, 0, 0}, res);
+            const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
+            const float3 acc = lerp(prev, res, 1.0f / static_cast<float>(1 + frame));
+            hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
+            if (sums) {
+                sums[3 * p] += static_cast<double>(res.x);
+                sums[3 * p + 1] += static_cast<double>(res.y);
+                sums[3 * p + 2] += static_cast<double>(res.z);
+            }
+            if (last && samples && valid) {
+                for (int k = 0; k < NRC_V; ++k) {
+                    const size_t slot = static_cast<size_t>(p) * NRC_V + k;
+                    valid[slot] = train && tr.valid[k] ? 1 : 0;
+                    if (valid[slot]) std::memcpy(samples + slot * 20, tr.rec[k], 20 * sizeof(float));
+                }
+            }
+        }
+        if (last) queries = frame_queries;
+    }
+    if (n_queries) *n_queries = queries;
+    return 0;
 }
 
 // ---- guiding primitives, for their CPU tests ----

@@ -48,6 +48,9 @@ def lib():
         L.or_xyz_to_srgb.argtypes = [vp, vp]
         L.or_render.restype = i32
         L.or_render.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, i32]
+        L.or_render_nrc.restype = i32
+        L.or_render_nrc.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, ctypes.c_float, ctypes.c_uint32,
+                                    ctypes.c_uint32, vp, vp, vp, vp, vp, i32]
         L.or_render_guided.restype = i32
         L.or_render_guided.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, i32, vp]
         L.or_guide_create.restype = vp
@@ -79,7 +82,8 @@ def lib():
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data
+    """Pointer argument that keeps the array alive for the duration of the call."""
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
 def _objs(objs):
@@ -138,6 +142,26 @@ class OracleBVH:
         if rc != 0:
             raise RuntimeError("or_render_guided failed (guiding needs a Guide)")
         return hdr, sums, prim
+
+    def render_nrc(self, camera, params, nrc_params, bmin, bmax, spread=0.01, train_paths=0, unbiased_every=0,
+                   cull_margin=None, threads=None, hdr=None, sums=None):
+        """The core's cache rendering with the cache frozen at nrc_params:
+        (hdr, sums, samples of the last frame keyed pixel*8+vertex, cache queries)."""
+        import ctypes as C
+
+        w, h = camera.width, camera.height
+        hdr = np.zeros((h, w, 3), np.float32) if hdr is None else np.ascontiguousarray(hdr, np.float32).copy()
+        sums = np.zeros((h, w, 3), np.float64) if sums is None else np.ascontiguousarray(sums, np.float64).copy()
+        samples = np.zeros((w * h * 8, 20), np.float32)
+        valid = np.zeros(w * h * 8, np.uint8)
+        nq = C.c_uint32(0)
+        rc = lib().or_render_nrc(self.h, _p(self.objs), len(self.objs), C.addressof(camera), C.addressof(params),
+                                 float(cull_margin or 0.0), _p(_f32(nrc_params)), _p(_f32(bmin)), _p(_f32(bmax)),
+                                 spread, train_paths, unbiased_every, _p(hdr), _p(sums), _p(samples), _p(valid),
+                                 C.byref(nq), threads or os.cpu_count())
+        if rc != 0:
+            raise RuntimeError("or_render_nrc failed")
+        return hdr, sums, samples[valid != 0], nq.value
 
     def __del__(self):
         try:

@@ -1,0 +1,117 @@
+"""GPU checks of rendering with the learned radiance cache (ap_render_nrc)
+against the oracle's restatement (CalculateRadianceNRC, oracle/or_render.cpp)
+with the cache frozen (train_steps = 0).
+
+Bars:
+  * paths, termination decisions and query records follow the same FP32
+    operations on both sides: the number of paths ending into the cache, the
+    training-sample keys (pixel, vertex) and the 13 query floats of every
+    training sample are bit-identical, and so is every pixel whose path never
+    reached the cache;
+  * the cache's prediction is the MFMA network (FP32 accumulation) on the GPU
+    and the FP64 network with the same BF16 rounding points in the oracle, so
+    pixels and targets that depend on it agree within 1e-4 relative for
+    99.5% and 2e-2 for all (a BF16 rounding boundary can fall between the
+    two accumulations);
+  * a cache that is never reached leaves ap_render's image bit for bit;
+  * training is deterministic: two runs from the same weights give the same
+    image and the same weights, and the weights move.
+"""
+import numpy as np
+import pytest
+
+import aperture as ap
+import oracle_lib as ol
+import test_nrc_cpu as ncpu
+import test_render_gpu as rg
+
+pytestmark = pytest.mark.gpu
+
+
+def render_nrc(scene, cam, prm, cache, nprm):
+    n = cam.width * cam.height
+    d_hdr = ap.DeviceBuffer(n * 12)
+    d_hdr.zero()
+    d_sums = ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64))
+    scene.render_nrc(cam, prm, cache, nprm, d_hdr, sum_xyz=d_sums)
+    ap.synchronize()
+    shp = (cam.height, cam.width, 3)
+    return d_hdr.to_numpy(np.float32, n * 3).reshape(shp), d_sums.to_numpy(np.float64, n * 3).reshape(shp)
+
+
+def frozen_parity(objs, cam, prm, label, train_paths=256, unbiased_every=4, spread=0.01, weights_seed=41):
+    print(f"[nrc-render] {label}: {len(objs)} objects, {cam.width}x{cam.height}, spp {prm.spp}", flush=True)
+    scene = ap.SceneRenderer(objs)
+    margin = np.float32(scene.info().line_margin)
+    cache = ap.RadianceCache(seed=1)
+    w = ncpu.init_params(weights_seed, scale=0.7)
+    cache.set_params(w)
+    nprm = ap.NrcRenderParams.defaults(train_paths=train_paths, train_steps=0, unbiased_every=unbiased_every,
+                                       spread=spread)
+    g_hdr, g_sums = render_nrc(scene, cam, prm, cache, nprm)
+    st, g_s = scene.nrc_last_frame()
+    lo, hi = cache.bounds()
+    o_hdr, o_sums, o_s, o_nq = ol.OracleBVH(objs).render_nrc(cam, prm, w, lo, hi, spread=spread,
+                                                             train_paths=train_paths, unbiased_every=unbiased_every,
+                                                             cull_margin=margin)
+    rel = np.abs(g_sums - o_sums) / (np.abs(o_sums) + 1e-6)
+    ident = float(np.all(g_sums == o_sums, axis=-1).mean())
+    print(f"[nrc-render] {label}: queries {st.queries} vs {o_nq}, samples {st.samples} vs {len(o_s)}, "
+          f"pixels bit-identical {ident:.4f}, rel err p99.5 {np.quantile(rel, 0.995):.3g} max {rel.max():.3g}",
+          flush=True)
+    assert st.queries == o_nq and st.queries > 0
+    assert st.samples == len(o_s) and st.samples > 0
+    assert np.array_equal(g_s[:, 19].view(np.uint32), o_s[:, 19].view(np.uint32))
+    assert np.array_equal(g_s[:, :16], o_s[:, :16])
+    assert np.quantile(rel, 0.995) < 1e-4 and rel.max() < 2e-2
+    t = np.abs(g_s[:, 16:19] - o_s[:, 16:19]) / (np.abs(o_s[:, 16:19]) + 1e-4)
+    assert np.quantile(t, 0.995) < 1e-4 and t.max() < 2e-2
+    return scene, cache
+
+
+def test_frozen_cache_box_matches_oracle(gpu):
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 48, 48)
+    prm = ap.RenderParams.defaults(spp=2, max_depth=12, seed=7)
+    frozen_parity(objs, cam, prm, "box")
+
+
+def test_frozen_cache_heightfield_matches_oracle(gpu):
+    objs = ap.synth_heightfield_scene(24, 3, 5, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 48, 32)
+    prm = ap.RenderParams.defaults(spp=1, max_depth=16, rr_min_depth=4, seed=11)
+    frozen_parity(objs, cam, prm, "heightfield", train_paths=512, unbiased_every=16)
+
+
+def test_unreached_cache_is_ap_render(gpu):
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 32, 32)
+    prm = ap.RenderParams.defaults(spp=3, max_depth=8, seed=2)
+    scene = ap.SceneRenderer(objs)
+    ref = rg.render_gpu(scene, cam, prm)
+    cache = ap.RadianceCache(seed=1)
+    hdr, sums = render_nrc(scene, cam, prm, cache, ap.NrcRenderParams.defaults(spread=1e30, train_paths=64,
+                                                                                 train_steps=0))
+    assert scene.nrc_last_frame(samples=False)[0].queries == 0
+    assert np.array_equal(hdr, ref["hdr"]) and np.array_equal(sums, ref["sums"])
+
+
+def test_training_is_deterministic_and_moves_the_weights(gpu):
+    objs = ap.synth_heightfield_scene(32, 3, 5, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 64, 64)
+    prm = ap.RenderParams.defaults(spp=6, max_depth=16, rr_min_depth=4, seed=3)
+    nprm = ap.NrcRenderParams.defaults(train_paths=1024, batch=1024, train_steps=2)
+    out = []
+    for _ in range(2):
+        scene = ap.SceneRenderer(objs)
+        cache = ap.RadianceCache(seed=9)
+        w0 = cache.params()
+        hdr, _ = render_nrc(scene, cam, prm, cache, nprm)
+        st, s = scene.nrc_last_frame()
+        out.append((hdr, cache.params(), w0, st.samples))
+    (h1, w1, w0, n1), (h2, w2, _, n2) = out
+    print(f"[nrc-render] training: samples/frame {n1}, weight change {np.abs(w1 - w0).max():.3g}", flush=True)
+    assert n1 == n2 and n1 > 0
+    assert np.array_equal(h1, h2) and np.array_equal(w1, w2)
+    assert np.abs(w1 - w0).max() > 0.0
+    assert np.isfinite(h1).all() and np.isfinite(w1).all()
