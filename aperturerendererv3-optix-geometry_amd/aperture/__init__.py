@@ -170,7 +170,7 @@ EXPORTED_SYMBOLS = (
     "ap_synth_heightfield_scene", "ap_guiding_reset", "ap_guiding_export",
     "ap_nrc_create", "ap_nrc_destroy", "ap_nrc_set_bounds", "ap_nrc_get_params", "ap_nrc_set_params",
     "ap_nrc_infer", "ap_nrc_train_step", "ap_nrc_gradient", "ap_nrc_last_gradient", "ap_nrc_get_bounds",
-    "ap_nrc_render_params_default", "ap_render_nrc", "ap_nrc_last_frame",
+    "ap_nrc_set_ema", "ap_nrc_render_params_default", "ap_render_nrc", "ap_nrc_last_frame",
 )
 
 
@@ -201,6 +201,7 @@ def lib():
         L.ap_nrc_gradient.argtypes = [vp, vp, sz, vp, vp, vp]
         L.ap_nrc_last_gradient.argtypes = [vp, vp]
         L.ap_nrc_get_bounds.argtypes = [vp, vp, vp]
+        L.ap_nrc_set_ema.argtypes = [vp, ctypes.c_float]
         L.ap_render_nrc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.ap_nrc_last_frame.argtypes = [vp, vp, vp, sz]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
@@ -499,6 +500,9 @@ class RadianceCache:
         lo = np.ascontiguousarray(bmin, np.float32)
         hi = np.ascontiguousarray(bmax, np.float32)
         check(lib().ap_nrc_set_bounds(self.handle, _ptr(lo), _ptr(hi)))
+
+    def set_ema(self, decay):
+        check(lib().ap_nrc_set_ema(self.handle, decay))
 
     def bounds(self):
         lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)

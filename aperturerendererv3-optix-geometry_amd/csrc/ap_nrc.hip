@@ -320,11 +320,15 @@ __global__ void reduce_grad_kernel(const float* __restrict__ part, int chunks, f
     grad[p] = s;
 }
 
-// Adam (the oracle's or_nrc_adam), then the BF16 copies W[o][i] and W^T[i][o].
+// Adam (the oracle's or_nrc_adam), then the BF16 copies W[o][i] and W^T[i][o]
+// of the training weights, and the exponential moving average of the
+// parameters (decay ema_decay, debiased by ema_c = 1 - decay^t) whose BF16
+// copy answers inference.
 __global__ void adam_kernel(float* __restrict__ params, float* __restrict__ m, float* __restrict__ v,
                             const float* __restrict__ grad, float lr, float beta1, float beta2, float eps, float c1,
                             float c2, uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
-                            const uint32_t* __restrict__ gate) {
+                            const uint32_t* __restrict__ gate, float ema_decay, float ema_c,
+                            float* __restrict__ ema, uint16_t* __restrict__ w_inf) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= NPAR) return;
     if (gate && *gate == 0u) return;
@@ -342,6 +346,9 @@ __global__ void adam_kernel(float* __restrict__ params, float* __restrict__ m, f
     const int l = p / (NF * NF), rem = p % (NF * NF), o = rem / NF, i = rem % NF;
     if (l < NL) wt[l * NF * NF + i * NF + o] = h;
     else wt[NL * NF * NF + i * WT_OUT_STRIDE + o] = h;
+    const float e = ema_decay * ema[p] + (1.0f - ema_decay) * x;
+    ema[p] = e;
+    w_inf[p] = ema_decay > 0.0f ? f2bf(e / ema_c) : h;
 }
 
 // Row j of step `step`: sample (off + (step * batch + j) * P) mod N of the
@@ -369,6 +376,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ p
 struct NrcState {
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     uint16_t *w = nullptr, *wt = nullptr;
+    float* ema = nullptr;          // moving average of the parameters (zero-started, debiased)
+    uint16_t* w_inf = nullptr;     // BF16 weights inference reads: the debiased average, or w
+    float ema_decay = 0.0f;
     Bounds bounds{{-1.0f, -1.0f, -1.0f}, {1.0f, 1.0f, 1.0f}};
     int step = 0;
     uint32_t cap = 0;
@@ -391,6 +401,8 @@ int upload_params(NrcState* s, const std::vector<float>& p) {
     }
     hipError_t e = hipMemcpy(s->params, p.data(), NPAR * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->w, w.data(), NPAR * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->w_inf, w.data(), NPAR * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->ema, 0, NPAR * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(s->wt, wt.data(), NPAR_T * sizeof(uint16_t), hipMemcpyHostToDevice);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: upload parameters");
 }
@@ -424,8 +436,8 @@ inline unsigned blocks_for(uint32_t n) {
 
 int nrc_infer(NrcState* s, const float* queries, size_t n, float* out, hipStream_t st) {
     if (n == 0) return AP_OK;
-    hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(static_cast<uint32_t>(n))), dim3(256), 0, st, s->w,
-                       s->bounds, queries, QREC, static_cast<uint32_t>(n), nullptr, nullptr, out, nullptr, 0u);
+    hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(static_cast<uint32_t>(n))), dim3(256), 0, st,
+                       s->w_inf, s->bounds, queries, QREC, static_cast<uint32_t>(n), nullptr, nullptr, out, nullptr, 0u);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: infer");
 }
@@ -453,8 +465,9 @@ int nrc_adam(NrcState* s, float lr, hipStream_t st, const uint32_t* gate = nullp
     const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
     const float c1 = 1.0f - std::pow(b1, static_cast<float>(s->step));
     const float c2 = 1.0f - std::pow(b2, static_cast<float>(s->step));
+    const float ec = 1.0f - std::pow(s->ema_decay, static_cast<float>(s->step));
     hipLaunchKernelGGL(adam_kernel, dim3((NPAR + 255) / 256), dim3(256), 0, st, s->params, s->m, s->v, s->grad, lr,
-                       b1, b2, eps, c1, c2, s->w, s->wt, gate);
+                       b1, b2, eps, c1, c2, s->w, s->wt, gate, s->ema_decay, ec, s->ema, s->w_inf);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: adam");
 }
@@ -462,8 +475,8 @@ int nrc_adam(NrcState* s, float lr, hipStream_t st, const uint32_t* gate = nullp
 int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const uint32_t* dyn_n, uint32_t n_cap,
                   float* out, hipStream_t st) {
     if (n_cap == 0) return AP_OK;
-    hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(n_cap)), dim3(256), 0, st, s->w, s->bounds, queries,
-                       QREC, n_cap, dyn_n, rows, out, nullptr, 0u);
+    hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(n_cap)), dim3(256), 0, st, s->w_inf, s->bounds,
+                       queries, QREC, n_cap, dyn_n, rows, out, nullptr, 0u);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: infer");
 }
@@ -502,7 +515,7 @@ void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]) {
 void nrc_free(NrcState* s) {
     if (!s) return;
     void* ptrs[] = {s->params, s->m, s->v, s->grad, s->w, s->wt, s->acts, s->deltas, s->dout, s->y, s->part,
-                    s->loss_part, s->batch};
+                    s->loss_part, s->batch, s->ema, s->w_inf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -525,9 +538,10 @@ int ap_nrc_create(uint64_t seed, ap_nrc** out) {
     *out = nullptr;
     NrcState* s = new NrcState();
     hipError_t e = hipSuccess;
-    for (float** p : {&s->params, &s->m, &s->v, &s->grad})
+    for (float** p : {&s->params, &s->m, &s->v, &s->grad, &s->ema})
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), NPAR * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->w), NPAR * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->w_inf), NPAR * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->wt), NPAR_T * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMemset(s->m, 0, NPAR * sizeof(float));
     if (e == hipSuccess) e = hipMemset(s->v, 0, NPAR * sizeof(float));
@@ -627,6 +641,13 @@ int ap_nrc_train_step(ap_nrc* nrc, const float* samples, size_t n, float lr, voi
     int rc = nrc_gradient(nrc->s, samples, static_cast<uint32_t>(n), st);
     if (rc != AP_OK) return rc;
     return nrc_adam(nrc->s, lr, st);
+}
+
+int ap_nrc_set_ema(ap_nrc* nrc, float decay) {
+    if (!nrc) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_ema: NULL cache");
+    if (!(decay >= 0.0f && decay < 1.0f)) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_ema: decay must be in [0, 1)");
+    nrc->s->ema_decay = decay;
+    return AP_OK;
 }
 
 int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]) {

@@ -88,7 +88,84 @@ def cpu_baseline(objs, rays_a, rays_b, seconds, threads):
     }
 
 
-def render_bench(ap, spp, cpu_seconds, threads):
+SRGB = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
+                 [0.0556434, -0.2040259, 1.0572252]])
+
+
+def rel_mse(xyz, ref_xyz):
+    """mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2) on linear sRGB, FP64."""
+    a = np.maximum(xyz.reshape(-1, 3).astype(np.float64) @ SRGB.T, 0.0)
+    b = np.maximum(ref_xyz.reshape(-1, 3).astype(np.float64) @ SRGB.T, 0.0)
+    return float((((a - b) ** 2) / (b ** 2 + 1e-2)).mean())
+
+
+def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup):
+    """Equal-time relMSE of the radiance-cache renderer against the plain
+    wavefront tracer. The plain tracer renders plain_spp frames (timed with
+    HIP events); the cache renderer starts from freshly initialised weights
+    and renders one frame per call, training after each, for the same GPU
+    time: its first `warmup` frames go to a scratch image, the rest are
+    averaged. Both images are compared with a plain render of ref_spp frames
+    on another seed."""
+    W, H = cam.width, cam.height
+    n = W * H
+    ref = ap.DeviceBuffer(n * 12)
+    ref.zero(stream)
+    t0 = time.perf_counter()
+    scene.render(cam, ap.RenderParams.defaults(spp=ref_spp, seed=1000, **params), ref, stream=stream)
+    stream.synchronize()
+    log(f"nrc: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    ref_img = ref.to_numpy(np.float32, n * 3)
+    img = ap.DeviceBuffer(n * 12)
+    img.zero(stream)
+    e0, e1 = ap.Event(), ap.Event()
+    e0.record(stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=plain_spp, seed=5, **params), img, stream=stream)
+    e1.record(stream)
+    stream.synchronize()
+    budget = e0.elapsed_ms(e1)
+    plain = rel_mse(img.to_numpy(np.float32, n * 3), ref_img)
+    cache = ap.RadianceCache(seed=1)
+    nprm = ap.NrcRenderParams.defaults()
+    scratch = ap.DeviceBuffer(n * 12)
+    scratch.zero(stream)
+    img.zero(stream)
+    used, frames = 0.0, 0
+    while used < budget:
+        warm = frames < warmup
+        e0.record(stream)
+        f = frames if warm else frames - warmup
+        scene.render_nrc(cam, ap.RenderParams.defaults(spp=1, frame_index=f, seed=5 + (77 if warm else 0),
+                                                       **params), cache, nprm, scratch if warm else img,
+                         stream=stream)
+        e1.record(stream)
+        stream.synchronize()
+        used += e0.elapsed_ms(e1)
+        frames += 1
+    st, _ = scene.nrc_last_frame(samples=False)
+    nrc = rel_mse(img.to_numpy(np.float32, n * 3), ref_img) if frames > warmup else float("nan")
+    cache.close()
+    out = {
+        "equal_time_ms": round(budget, 1),
+        "plain_spp": plain_spp,
+        "plain_relmse": round(plain, 5),
+        "nrc_frames": frames - warmup,
+        "nrc_warmup_frames": warmup,
+        "nrc_ms": round(used, 1),
+        "nrc_relmse": round(nrc, 5),
+        "relmse_ratio_plain_over_nrc": round(plain / nrc, 3) if nrc > 0 else None,
+        "reference_spp": ref_spp,
+        "reference_self_noise_estimate": round(plain * plain_spp / ref_spp, 6),
+        "cache_queries_per_frame": st.queries,
+        "training_paths_per_frame": st.train_paths,
+        "training_samples_per_frame": st.samples,
+        "cache": "64-64x5-3 ReLU, BF16 MFMA / FP32 accumulate, relative L2, Adam lr 0.005, 4 steps x 16384 per frame",
+    }
+    log(f"nrc: {out}")
+    return out
+
+
+def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=1024, nrc_warmup=4):
     """Samples/s of the wavefront path tracer at 1920x1080 on the heightfield
     workload (1024^2-cell fBm heightfield, 2,097,152 triangles, plus 256
     320-triangle icospheres and 4 thin-cube lights; Lambert/GGX 50/50), max
@@ -145,6 +222,8 @@ def render_bench(ap, spp, cpu_seconds, threads):
         },
     }
     log(f"render: {out}")
+    if nrc_spp > 0:
+        out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, nrc_ref_spp, nrc_warmup)
     if cpu_seconds > 0:
         sys.path.insert(0, os.path.join(REPO, "tests"))
         import oracle_lib as ol
@@ -177,6 +256,10 @@ def main():
     ap_.add_argument("--cpu-seconds", type=float, default=20.0)
     ap_.add_argument("--no-cpu", action="store_true")
     ap_.add_argument("--render-spp", type=int, default=8, help="spp of the 1080p render measurement (0 skips it)")
+    ap_.add_argument("--nrc-spp", type=int, default=64,
+                     help="plain spp that sets the equal-time budget of the radiance-cache comparison (0 skips it)")
+    ap_.add_argument("--nrc-ref-spp", type=int, default=1024, help="spp of the plain reference image")
+    ap_.add_argument("--nrc-warmup", type=int, default=4, help="cache frames rendered before the averaged ones")
     ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap_.add_argument("--sweep", action="store_true",
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
@@ -350,7 +433,9 @@ def main():
     scene.close()
     del objs, prim, sec
     if args.render_spp > 0:
-        line["render"] = render_bench(ap, args.render_spp, 0.0 if args.no_cpu else 10.0, threads)
+        line["render"] = render_bench(ap, args.render_spp, 0.0 if args.no_cpu else 10.0, threads,
+                                      nrc_spp=args.nrc_spp, nrc_ref_spp=args.nrc_ref_spp,
+                                      nrc_warmup=args.nrc_warmup)
     print(json.dumps(line), flush=True)
 
 

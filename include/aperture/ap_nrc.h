@@ -49,6 +49,12 @@ int ap_nrc_train_step(ap_nrc* nrc, const float* samples, size_t n, float lr, voi
 int ap_nrc_gradient(ap_nrc* nrc, const float* samples, size_t n, float* host_grad, float* loss, void* stream);
 /* Gradient of the last training step or gradient call. */
 int ap_nrc_last_gradient(const ap_nrc* nrc, float* host_grad);
+/* Inference weights: with decay > 0, the exponential moving average of the
+ * parameters over the Adam steps since the last ap_nrc_set_params/create,
+ * ema_t = decay ema_{t-1} + (1 - decay) w_t from ema_0 = 0, divided by
+ * 1 - decay^t; with 0 (the default) the current parameters. Training always
+ * uses the current parameters. */
+int ap_nrc_set_ema(ap_nrc* nrc, float decay);
 /* Bounds the position encoding uses (set by ap_nrc_set_bounds or ap_render_nrc). */
 int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]);
 

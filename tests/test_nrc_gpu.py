@@ -91,3 +91,31 @@ def test_training_fits_a_smooth_target(cache):
     final, _ = cache.gradient(s)
     print(f"[nrc] training: loss {first:.4g} -> {final:.4g}", flush=True)
     assert final < 0.05 * first
+
+
+def test_inference_uses_the_debiased_moving_average(cache):
+    """With decay d, inference answers with ema_t / (1 - d^t), ema_t = d ema_{t-1}
+    + (1 - d) w_t from 0 over the Adam steps since set_params (FP32 on both
+    sides), compared with the oracle forward at the stated tolerance."""
+    d = np.float32(0.9)
+    cache.set_ema(float(d))
+    p = cpu.init_params(33)
+    cache.set_params(p)
+    q = cpu.random_queries(2048, seed=34)
+    target = np.random.default_rng(35).uniform(0, 2, (2048, 3)).astype(np.float32)
+    dev = ap.DeviceBuffer.from_numpy(samples_for(q, target))
+    ema = np.zeros_like(p)
+    for t in range(1, 6):
+        cache.train_step(dev, 2048, lr=1e-3)
+        ap.synchronize()
+        w = cache.params()
+        ema = (d * ema + (np.float32(1.0) - d) * w).astype(np.float32)
+    avg = (ema / np.float32(1.0 - float(d) ** 5)).astype(np.float32)
+    got = cache.infer(q)
+    cache.set_ema(0.0)
+    want = ol.nrc_forward(avg, ol.nrc_encode(q, cpu.BMIN, cpu.BMAX), round_bf16=True)
+    now = ol.nrc_forward(cache.params(), ol.nrc_encode(q, cpu.BMIN, cpu.BMAX), round_bf16=True)
+    rel = np.abs(got - want) / (np.abs(want) + 1e-3)
+    print(f"[nrc] ema: median rel err {np.median(rel):.3g}, p99 {np.quantile(rel, 0.99):.3g}; "
+          f"current weights differ by {np.median(np.abs(now - want) / (np.abs(want) + 1e-3)):.3g}", flush=True)
+    assert np.median(rel) < 1e-3 and np.quantile(rel, 0.99) < 2e-2
