@@ -54,24 +54,38 @@ struct MatIn {
     int32_t model, light_index;
 };
 
-struct alignas(16) PathState {
+// A path carries NL wavelengths: NL = 1 is the reference's one wavelength
+// per path; NL = 4 is hero-wavelength sampling (the hero lam[0] drawn by
+// SampleCIE1931, lam[j] = lam[0] + 105 j nm wrapped into [360, 780]). The
+// hero's pdfs drive every directional decision; radiance and throughput are
+// kept per wavelength, and the pixel estimate is
+// sum_j xyz(lam[j]) L_j / sum_k p(lam[k]). Spectral bases, D65 weights and
+// CMF values are recomputed from lam[] where they are needed.
+template <int NL>
+struct alignas(16) PathT {
     uint64_t rng_state, rng_inc;
-    f4 lambda_basis;
-    float xyz[3]; float lambda_pdf;
-    float radiance, mask, d65, last_pdf;
+    float lam[NL];
+    float radiance[NL];
+    float mask[NL];
+    float psum;        // sum of the wavelength pdfs (the wavelength pdf when NL = 1)
+    float last_pdf;
     float last_pos[3]; int32_t ndd;    // nonDeltaDepth
-    uint32_t flags; int32_t pixel; int32_t frame; float lambda;
+    uint32_t flags; int32_t pixel; int32_t frame;
 };
-static_assert(sizeof(PathState) == 96, "PathState is 96 B");
+static_assert(sizeof(PathT<1>) == 64, "PathT<1> is 64 B");
+static_assert(sizeof(PathT<4>) == 112, "PathT<4> is 112 B");
+constexpr size_t kPathBytes = sizeof(PathT<4>);
 
 enum : uint32_t { PATH_PREV_NEE = 1u, PATH_INSIDE = 2u };
 
-struct alignas(16) ShadowPayload {
+template <int NL>
+struct alignas(16) ShadowPay {
     int32_t path;
-    float add;        // mask * NEE radiance * D65 norm (0 when the cosines fail)
     uint32_t cos_ok;
-    uint32_t pad;
+    float add[NL];    // mask * NEE radiance * D65 norm per wavelength (0 when the cosines fail)
 };
+constexpr size_t kShadowPayBytes = sizeof(ShadowPay<4>);
+constexpr float kHeroStep = 105.0f;    // (780 - 360) / 4
 
 // Radiance-cache state of a path: a_0 and the running sqrt-spread sum of the
 // spread heuristic, NRC_* flags with the recorded training-vertex count in
@@ -83,14 +97,15 @@ struct alignas(16) NrcPath {
 };
 enum : uint32_t { NRC_TRAIN = 1u, NRC_UNBIASED = 2u, NRC_SUFFIX = 4u };
 constexpr int kNrcV = AP_NRC_TRAIN_VERTICES;
-constexpr int kNrcRec = 16;    // floats per query / training-vertex record
+constexpr int kNrcRec = 16;    // floats per query record
+constexpr int kNrcVert = 24;   // floats per training-vertex record: query, [16..19] L_k, [20..23] M_k
 constexpr int kNrcSample = 20; // floats per training sample
 
 struct NrcView {
     NrcPath* paths;
     float* queries;      // per path: the query record of its cache vertex
     int32_t* qpath;      // compacted paths that ended into the cache
-    float* verts;        // per training slot, kNrcV records: query, [13] L_k, [14] M_k
+    float* verts;        // per training slot, kNrcV records of kNrcVert floats
     uint32_t* counters;  // [0] cache queries, [1] training samples
     float spread;
 };
@@ -106,12 +121,12 @@ struct RenderState {
     int32_t n_mats = 0;
     // path buffers
     size_t capacity = 0;
-    PathState* paths = nullptr;
+    void* paths = nullptr;            // PathT<NL>, kPathBytes per entry
     ap_ray* rays[2] = {nullptr, nullptr};
     int32_t* path_of[2] = {nullptr, nullptr};
     ap_hit* hits = nullptr;
     ap_ray* shadow_rays = nullptr;
-    ShadowPayload* shadow_payload = nullptr;
+    void* shadow_payload = nullptr;   // ShadowPay<NL>, kShadowPayBytes per entry
     ap_hit* shadow_hits = nullptr;
     uint32_t* counters = nullptr;     // [0],[1] path queues, [2] shadow queue
     float* eps = nullptr;             // INTERSECT_EPS(d) for d = 0 .. kEpsEntries-1
@@ -240,11 +255,58 @@ __global__ void begin_bounce_kernel(uint32_t* c, int out) {
     if (threadIdx.x == 0) { c[out] = 0; c[2] = 0; }
 }
 
-__global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FrameParams fp, const f4* __restrict__ lam_lut,
-                                                          const float* __restrict__ tables,
-                                                          const float* __restrict__ eps, PathState* __restrict__ paths,
-                                                          ap_ray* __restrict__ rays, int32_t* __restrict__ path_of,
-                                                          GuideRec* __restrict__ recs) {
+// The wavelengths of a path: the hero lam[0] and its rotations.
+template <int NL>
+__device__ __forceinline__ void hero_wavelengths(float hero, float hero_pdf, float lam[NL], float& psum) {
+    lam[0] = hero;
+    psum = hero_pdf;
+    for (int j = 1; j < NL; ++j) {
+        float l = hero + kHeroStep * static_cast<float>(j);
+        if (l > 780.0f) l -= 420.0f;
+        lam[j] = l;
+        psum = psum + lambda_pdf_mix(l);
+    }
+}
+
+// Per-shade spectral context of a path's wavelengths.
+template <int NL>
+struct Spec {
+    f4 lb[NL];
+    float d65[NL];
+};
+
+template <int NL>
+__device__ __forceinline__ Spec<NL> spec_of(const float lam[NL], const f4* __restrict__ lam_lut,
+                                            const float* __restrict__ tables) {
+    Spec<NL> sp;
+    for (int j = 0; j < NL; ++j) {
+        sp.lb[j] = query_lambda(lam_lut, lam[j]);
+        sp.d65[j] = d65_norm(tables, lam[j]);
+    }
+    return sp;
+}
+
+// sum_j xyz(lam[j]) L_j / psum (the pixel estimate before clamping).
+template <int NL>
+__device__ __forceinline__ f3 path_xyz(const float lam[NL], const float* L, float psum,
+                                       const float* __restrict__ tables) {
+    f3 acc = lambda_to_xyz(tables, lam[0]) * L[0];
+    for (int j = 1; j < NL; ++j) acc = acc + lambda_to_xyz(tables, lam[j]) * L[j];
+    return acc / psum;
+}
+
+template <int NL>
+__device__ __forceinline__ float mean_mask(const float* m) {
+    if (NL == 1) return m[0];
+    float s = m[0];
+    for (int j = 1; j < NL; ++j) s = s + m[j];
+    return s * (1.0f / static_cast<float>(NL));
+}
+
+template <int NL>
+__global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FrameParams fp, const float* __restrict__ eps,
+                                                          PathT<NL>* __restrict__ paths, ap_ray* __restrict__ rays,
+                                                          int32_t* __restrict__ path_of, GuideRec* __restrict__ recs) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t total = static_cast<uint32_t>(fp.n_pixels) * static_cast<uint32_t>(fp.frames);
     if (i >= total) return;
@@ -266,21 +328,18 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FramePa
         sample_lambda(r0, r1, r2, lam, lpdf);
     }
     d = normalize(d);
-    PathState s;
-    s.lambda_basis = query_lambda(lam_lut, lam);
-    const f3 xyz = lambda_to_xyz(tables, lam);
-    s.xyz[0] = xyz.x; s.xyz[1] = xyz.y; s.xyz[2] = xyz.z;
-    s.lambda_pdf = lpdf;
-    s.d65 = d65_norm(tables, lam);
-    s.radiance = 0.0f;
-    s.mask = 1.0f;
+    PathT<NL> s;
+    hero_wavelengths<NL>(lam, lpdf, s.lam, s.psum);
+    for (int j = 0; j < NL; ++j) {
+        s.radiance[j] = 0.0f;
+        s.mask[j] = 1.0f;
+    }
     s.last_pdf = 0.0f;
     s.last_pos[0] = s.last_pos[1] = s.last_pos[2] = 0.0f;
     s.ndd = 0;
     s.flags = fp.inside ? PATH_INSIDE : 0u;
     s.pixel = p;
     s.frame = frame;
-    s.lambda = lam;
     s.rng_state = rng.state;
     s.rng_inc = rng.inc;
     paths[i] = s;
@@ -457,41 +516,49 @@ __device__ __forceinline__ void nrc_record(float* __restrict__ q, f3 pos, f3 d, 
     q4[2] = float4{n.z, M.roughness, M.albedo_rgb[0], M.albedo_rgb[1]};
 }
 
-template <bool GUIDE, bool NRC>
+template <int NL, bool GUIDE, bool NRC>
 __global__ __launch_bounds__(kBlock) void shade_kernel(
     ShadeParams sp, const GeoRec* __restrict__ geo, const MatRec* __restrict__ mats,
     const LightRec* __restrict__ lights, const AnalyticRec* __restrict__ analytic, const f4* __restrict__ rgb_lut,
-    const float* __restrict__ eps, PathState* __restrict__ paths, const ap_ray* __restrict__ rays_in,
-    const int32_t* __restrict__ path_in, const ap_hit* __restrict__ hits, ap_ray* __restrict__ rays_out,
-    int32_t* __restrict__ path_out, ap_ray* __restrict__ shadow_rays, ShadowPayload* __restrict__ shadow_pay,
-    uint32_t* __restrict__ counters, int32_t* __restrict__ primary_id, int32_t primary_frame,
-    unsigned long long* __restrict__ ray_count, GuideView gv, NrcView nv) {
+    const f4* __restrict__ lam_lut, const float* __restrict__ tables, const float* __restrict__ eps,
+    PathT<NL>* __restrict__ paths, const ap_ray* __restrict__ rays_in, const int32_t* __restrict__ path_in,
+    const ap_hit* __restrict__ hits, ap_ray* __restrict__ rays_out, int32_t* __restrict__ path_out,
+    ap_ray* __restrict__ shadow_rays, ShadowPay<NL>* __restrict__ shadow_pay, uint32_t* __restrict__ counters,
+    int32_t* __restrict__ primary_id, int32_t primary_frame, unsigned long long* __restrict__ ray_count, GuideView gv,
+    NrcView nv) {
+    static_assert(!GUIDE || NL == 1, "guiding runs with one wavelength per path");
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
     if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
     if (i >= n) return;
     const int32_t pid = path_in[i];
-    PathState S = paths[pid];
+    PathT<NL> S = paths[pid];
     Pcg32 rng{S.rng_state, S.rng_inc};
     const ap_ray R = rays_in[i];
     const ap_hit H = hits[i];
     const f3 o{R.ox, R.oy, R.oz}, d{R.dx, R.dy, R.dz};
-    const f4 lb = S.lambda_basis;
+    const Spec<NL> sc = spec_of<NL>(S.lam, lam_lut, tables);
+    const f4 lb = sc.lb[0];    // the hero's basis: every sampling decision uses it
     const bool inside = (S.flags & PATH_INSIDE) != 0u;
     if (sp.depth == 0 && primary_id && S.frame == primary_frame) primary_id[S.pixel] = H.id;
 
     bool emit_next = false, emit_shadow = false, emit_query = false;
     ap_ray next{}, shadow{};
-    ShadowPayload pay{pid, 0.0f, 0u, 0u};
+    ShadowPay<NL> pay;
+    pay.path = pid;
+    pay.cos_ok = 0u;
+    for (int j = 0; j < NL; ++j) pay.add[j] = 0.0f;
 
     if (H.id < 0) {
         // Miss: SkyLighting and the offset draw of Trace, then the path ends.
         (void)rand1(rng);
         float le_end = 0.0f;
         if (!inside) {
-            const float le = sp.sky ? interact(rgb_lut, lb, sky_lighting(d)) : 0.0f;
-            S.radiance = S.radiance + S.mask * le * S.d65;
-            le_end = le * S.d65;
+            for (int j = 0; j < NL; ++j) {
+                const float le = sp.sky ? interact(rgb_lut, sc.lb[j], sky_lighting(d)) : 0.0f;
+                S.radiance[j] = S.radiance[j] + S.mask[j] * le * sc.d65[j];
+                if (j == 0) le_end = le * sc.d65[0];
+            }
         }
         if (GUIDE) pg_endpoint(gv.recs[pid], le_end, S.ndd);
         S.flags &= ~PATH_PREV_NEE;
@@ -542,7 +609,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                 }
             }
         }
-        float bsdf_cos = eval_bsdf_cos(M, lb, d, next_dir, N);
+        float bsdf_cos[NL];
+        for (int j = 0; j < NL; ++j) bsdf_cos[j] = eval_bsdf_cos(M, sc.lb[j], d, next_dir, N);
         // Trace's offset of the shading point (render.cuh:156-161).
         {
             const float off = eps[S.ndd] * fmaxf(1.0f, length(hp));
@@ -553,8 +621,10 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         float g_alpha = 0.0f;
         f3 g_query{1.0f, 1.0f, 1.0f};
         if (GUIDE) {
+            const f3 xyz0 = lambda_to_xyz(tables, S.lam[0]);
+            const float xyz_a[3] = {xyz0.x, xyz0.y, xyz0.z};
             g_alpha = pg_alpha(S.frame);
-            g_query = pg_query(S.mask, S.xyz, sp.depth == 0);
+            g_query = pg_query(S.mask[0], xyz_a, sp.depth == 0);
             if (!pure_light && !inside && gv.probe_count > 0) {
                 probe = pg_query_probe(gv, hp, pg_reflect(d, N));
                 if (probe) {
@@ -566,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                             if (dot(gs.dir, N) > 1.0e-12f) {
                                 const float p_bs = bsdf_pdf(M, lb, N, d, gs.dir);
                                 next_dir = gs.dir;
-                                bsdf_cos = eval_bsdf_cos(M, lb, d, gs.dir, N);
+                                bsdf_cos[0] = eval_bsdf_cos(M, lb, d, gs.dir, N);
                                 pdf = (1.0f - g_alpha) * fmaxf(p_bs, 1.0e-12f) + g_alpha * gs.pdf;
                             } else {
                                 probe = nullptr;
@@ -590,14 +660,16 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                                                   hp, N);
                 if (pL >= 0.0f && pB > 0.0f) mis = power_heuristic(pB, pL);
             }
-            const float le = interact_basis(M.emissive_basis, M.emissive_scale, lb);
-            S.radiance = S.radiance + S.mask * le * S.d65 * mis;
+            for (int j = 0; j < NL; ++j) {
+                const float le = interact_basis(M.emissive_basis, M.emissive_scale, sc.lb[j]);
+                S.radiance[j] = S.radiance[j] + S.mask[j] * le * sc.d65[j] * mis;
+            }
             if (GUIDE) {
                 const f3 e{M.emissive_rgb[0], M.emissive_rgb[1], M.emissive_rgb[2]};
-                le_end = interact(rgb_lut, lb, e * mis) * S.d65;
+                le_end = interact(rgb_lut, lb, e * mis) * sc.d65[0];
             }
         } else if (GUIDE && !(M.emissive_rgb[0] == 0.0f && M.emissive_rgb[1] == 0.0f && M.emissive_rgb[2] == 0.0f)) {
-            le_end = interact_basis(M.emissive_basis, M.emissive_scale, lb) * S.d65;
+            le_end = interact_basis(M.emissive_basis, M.emissive_scale, lb) * sc.d65[0];
         }
         if (GUIDE) pg_endpoint(gv.recs[pid], le_end, S.ndd);
         if (NRC) {
@@ -632,9 +704,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                 emit_query = true;
                 terminate = true;
             } else if ((P.flags & NRC_TRAIN) && scatter && (P.flags >> 8) < static_cast<uint32_t>(kNrcV)) {
-                float* v = nv.verts + (static_cast<size_t>(P.slot) * kNrcV + (P.flags >> 8)) * kNrcRec;
+                float* v = nv.verts + (static_cast<size_t>(P.slot) * kNrcV + (P.flags >> 8)) * kNrcVert;
                 nrc_record(v, hi.pos, d, N, M);
-                reinterpret_cast<float4*>(v)[3] = float4{M.albedo_rgb[2], S.radiance, S.mask, 0.0f};
+                reinterpret_cast<float4*>(v)[3] = float4{M.albedo_rgb[2], 0.0f, 0.0f, 0.0f};
+                for (int j = 0; j < 4; ++j) {
+                    v[16 + j] = j < NL ? S.radiance[j < NL ? j : 0] : 0.0f;
+                    v[20 + j] = j < NL ? S.mask[j < NL ? j : 0] : 0.0f;
+                }
                 P.flags += 1u << 8;
             }
             nv.paths[pid] = P;
@@ -645,7 +721,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             // NEE (render.cuh:766-827, 206-343).
             bool nee_valid = false;
             bool nee_deferred = false;
-            const float rate = sp.nee_always ? 1.0f : sqrtf(saturatef(S.mask));
+            const float mm = mean_mask<NL>(S.mask);
+            const float rate = sp.nee_always ? 1.0f : sqrtf(saturatef(mm));
             if (sp.use_nee && S.ndd < sp.nee_max_depth && !inside && rand1(rng) < rate) {
                 const LightSample ls = sample_light(lights, sp.n_lights, hp, rng);
                 if (ls.valid) {
@@ -656,21 +733,25 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                     const float cosX = fmaxf(0.0f, dot(N, L));
                     const float cosY = fmaxf(0.0f, dot(ls.normal, -L));
                     const bool cos_ok = !(cosX <= 0.0f || cosY <= 0.0f);
-                    float add = 0.0f;
+                    float add[NL];
+                    for (int j = 0; j < NL; ++j) add[j] = 0.0f;
                     if (cos_ok) {
-                        const float bc = eval_bsdf_cos(M, lb, d, L, N);
                         const float G = cosY / d2;
                         const MatRec& LM = mats[ls.mat];
-                        const float resp = interact_basis(LM.emissive_basis, LM.emissive_scale, lb);
-                        const float contribution = bc * resp * (G / pat);
                         const float pLo = area_to_solid_angle_pdf(hp, ls.point, ls.normal, ls.p_facet * ls.p_area);
                         float pBs = fmaxf(bsdf_pdf(M, lb, N, d, L), 0.0f);
                         if (GUIDE && probe) {
                             const float p_pr = pg_hemisphere_pdf(*probe, L, N, g_query);
                             pBs = (1.0f - g_alpha) * pBs + g_alpha * p_pr;
                         }
-                        const float rad = contribution * power_heuristic(pLo, pBs);
-                        add = S.mask * rad * S.d65;
+                        const float w = power_heuristic(pLo, pBs);
+                        for (int j = 0; j < NL; ++j) {
+                            const float bc = eval_bsdf_cos(M, sc.lb[j], d, L, N);
+                            const float resp = interact_basis(LM.emissive_basis, LM.emissive_scale, sc.lb[j]);
+                            const float contribution = bc * resp * (G / pat);
+                            const float rad = contribution * w;
+                            add[j] = S.mask[j] * rad * sc.d65[j];
+                        }
                     }
                     // AnyHitSegmentBVH (intersection.cuh:372-397).
                     const f3 seg = ls.point - hp;
@@ -688,11 +769,12 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                     if (traced) {
                         emit_shadow = true;
                         nee_deferred = true;
-                        pay.add = add;
+                        for (int j = 0; j < NL; ++j) pay.add[j] = add[j];
                         pay.cos_ok = cos_ok ? 1u : 0u;
                     } else {
                         nee_valid = cos_ok;
-                        if (cos_ok) S.radiance = S.radiance + add;
+                        if (cos_ok)
+                            for (int j = 0; j < NL; ++j) S.radiance[j] = S.radiance[j] + add[j];
                     }
                 }
             }
@@ -702,19 +784,22 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             }
             S.last_pdf = pdf;
             S.last_pos[0] = hp.x; S.last_pos[1] = hp.y; S.last_pos[2] = hp.z;
-            const float contrib = bsdf_cos / fmaxf(pdf, 1e-12f);
+            float contrib[NL];
+            for (int j = 0; j < NL; ++j) contrib[j] = bsdf_cos[j] / fmaxf(pdf, 1e-12f);
             // Russian roulette (render.cuh:838-859).
             float p_rr = 1.0f;
             bool survived = true;
             if (S.ndd >= sp.rr_min_depth) {
-                const float basic = clampf(S.mask * 0.5f + S.mask * 0.5f, 0.05f, 0.95f);
+                const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
                 p_rr = lerpf(basic, 1.0f, inside ? 0.99f : 0.0f);
                 if (rand1(rng) > p_rr) survived = false;
             }
             if (survived) {
-                if (GUIDE) pg_advance(gv.recs[pid], next_dir, contrib, p_rr, S.ndd, !inside, d, hp, N);
-                S.mask = S.mask * contrib;
-                S.mask = S.mask / p_rr;
+                if (GUIDE) pg_advance(gv.recs[pid], next_dir, contrib[0], p_rr, S.ndd, !inside, d, hp, N);
+                for (int j = 0; j < NL; ++j) {
+                    S.mask[j] = S.mask[j] * contrib[j];
+                    S.mask[j] = S.mask[j] / p_rr;
+                }
                 S.ndd += 1;  // no delta surfaces in this core
                 if (sp.depth + 1 < sp.max_depth) {
                     emit_next = true;
@@ -764,39 +849,51 @@ __global__ __launch_bounds__(kBlock) void nrc_init_kernel(int32_t n_pix, uint32_
 
 // Radiance of the cache vertices: throughput x the prediction (clamped at 0)
 // at the path's wavelength.
+template <int NL>
 __global__ __launch_bounds__(kBlock) void nrc_apply_kernel(const uint32_t* __restrict__ counters,
                                                            const int32_t* __restrict__ qpath,
                                                            const float* __restrict__ rgb, const f4* __restrict__ rgb_lut,
-                                                           PathState* __restrict__ paths) {
+                                                           const f4* __restrict__ lam_lut,
+                                                           const float* __restrict__ tables,
+                                                           PathT<NL>* __restrict__ paths) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= counters[0]) return;
-    PathState& S = paths[qpath[q]];
+    PathT<NL>& S = paths[qpath[q]];
     const f3 c = fmax3(f3{rgb[3 * q], rgb[3 * q + 1], rgb[3 * q + 2]}, f3{0.0f, 0.0f, 0.0f});
-    const float le = interact(rgb_lut, S.lambda_basis, c);
-    S.radiance = S.radiance + S.mask * le * S.d65;
+    for (int j = 0; j < NL; ++j) {
+        const float le = interact(rgb_lut, query_lambda(lam_lut, S.lam[j]), c);
+        S.radiance[j] = S.radiance[j] + S.mask[j] * le * d65_norm(tables, S.lam[j]);
+    }
 }
 
 __device__ __forceinline__ bool finite3(f3 v) {
     return !(isnan(v.x) || isnan(v.y) || isnan(v.z) || isinf(v.x) || isinf(v.y) || isinf(v.z));
 }
 
-// Targets of the training vertices, in place of L_k / M_k; vmask marks the finite ones.
+// Targets of the training vertices, XYZ2SRGB(sum_j xyz_j (L_end,j - L_k,j) /
+// M_k,j / psum), written over [13..15]; vmask marks the finite ones. With
+// several wavelengths a wavelength whose throughput at the vertex is zero
+// adds nothing.
+template <int NL>
 __global__ __launch_bounds__(kBlock) void nrc_targets_kernel(uint32_t n_train, uint32_t stride, uint32_t offset,
                                                              const NrcPath* __restrict__ npaths,
-                                                             const PathState* __restrict__ paths,
+                                                             const PathT<NL>* __restrict__ paths,
+                                                             const float* __restrict__ tables,
                                                              float* __restrict__ verts, uint32_t* __restrict__ vmask) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n_train) return;
     const uint32_t pid = offset + t * stride;
     const uint32_t nvx = min(npaths[pid].flags >> 8, static_cast<uint32_t>(kNrcV));
-    const PathState& S = paths[pid];
-    const float r_end = S.radiance;
-    const f3 xyz{S.xyz[0], S.xyz[1], S.xyz[2]};
+    const PathT<NL>& S = paths[pid];
     uint32_t m = 0;
     for (uint32_t k = 0; k < nvx; ++k) {
-        float* v = verts + (static_cast<size_t>(t) * kNrcV + k) * kNrcRec;
-        const float L = (r_end - v[13]) / v[14];
-        const f3 rgb = xyz_to_srgb((xyz * L) / S.lambda_pdf);
+        float* v = verts + (static_cast<size_t>(t) * kNrcV + k) * kNrcVert;
+        float L[NL];
+        for (int j = 0; j < NL; ++j) {
+            const float mk = v[20 + j];
+            L[j] = (NL > 1 && !(mk > 0.0f)) ? 0.0f : (S.radiance[j] - v[16 + j]) / mk;
+        }
+        const f3 rgb = xyz_to_srgb(path_xyz<NL>(S.lam, L, S.psum, tables));
         if (finite3(rgb)) {
             v[13] = rgb.x; v[14] = rgb.y; v[15] = rgb.z;
             m |= 1u << k;
@@ -844,7 +941,7 @@ __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(uint32_t n_train, u
     uint32_t j = offsets[t];
     for (int k = 0; k < kNrcV; ++k) {
         if (!((m >> k) & 1u)) continue;
-        const float* v = verts + (static_cast<size_t>(t) * kNrcV + k) * kNrcRec;
+        const float* v = verts + (static_cast<size_t>(t) * kNrcV + k) * kNrcVert;
         float* o = pool + static_cast<size_t>(j) * kNrcSample;
         for (int e = 0; e < 13; ++e) o[e] = v[e];
         o[13] = 0.0f; o[14] = 0.0f; o[15] = 0.0f;
@@ -854,19 +951,21 @@ __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(uint32_t n_train, u
     }
 }
 
+template <int NL>
 __global__ __launch_bounds__(kBlock) void resolve_kernel(const uint32_t* __restrict__ counters,
-                                                         const ShadowPayload* __restrict__ pay,
+                                                         const ShadowPay<NL>* __restrict__ pay,
                                                          const ap_hit* __restrict__ shits,
-                                                         PathState* __restrict__ paths,
+                                                         PathT<NL>* __restrict__ paths,
                                                          unsigned long long* __restrict__ ray_count) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (ray_count && i == 0) atomicAdd(&ray_count[1], static_cast<unsigned long long>(counters[2]));
     if (i >= counters[2]) return;
-    const ShadowPayload p = pay[i];
+    const ShadowPay<NL> p = pay[i];
     const bool occluded = shits[i].id >= 0;
-    PathState& S = paths[p.path];
+    PathT<NL>& S = paths[p.path];
     const bool cos_ok = p.cos_ok != 0u;
-    if (!occluded && cos_ok) S.radiance = S.radiance + p.add;
+    if (!occluded && cos_ok)
+        for (int j = 0; j < NL; ++j) S.radiance[j] = S.radiance[j] + p.add[j];
     if (occluded || cos_ok) S.flags |= PATH_PREV_NEE;
     else S.flags &= ~PATH_PREV_NEE;
 }
@@ -877,7 +976,9 @@ __device__ __forceinline__ float fix_nan_inf(float x) {
     return x;
 }
 
-__global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, const PathState* __restrict__ paths,
+template <int NL>
+__global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, const PathT<NL>* __restrict__ paths,
+                                                            const float* __restrict__ tables,
                                                             float* __restrict__ hdr, float* __restrict__ odd,
                                                             float* __restrict__ even, double* __restrict__ sums,
                                                             const GuideRec* __restrict__ recs,
@@ -890,11 +991,15 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
     if (even) ae = f3{even[3 * p], even[3 * p + 1], even[3 * p + 2]};
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int32_t f = 0; f < fp.frames; ++f) {
-        const PathState& S = paths[static_cast<size_t>(f) * fp.n_pixels + p];
-        if (slots && f == 0) pg_emit(recs[p], S.xyz, S.lambda_pdf, slots + static_cast<size_t>(p) * PG_COLLECT);
-        const float rad = fix_nan_inf(S.radiance);
-        const f3 xyz{S.xyz[0], S.xyz[1], S.xyz[2]};
-        const f3 pre = (xyz * rad) / S.lambda_pdf;
+        const PathT<NL>& S = paths[static_cast<size_t>(f) * fp.n_pixels + p];
+        if (slots && f == 0) {
+            const f3 x0 = lambda_to_xyz(tables, S.lam[0]);
+            const float xyz_a[3] = {x0.x, x0.y, x0.z};
+            pg_emit(recs[p], xyz_a, S.psum, slots + static_cast<size_t>(p) * PG_COLLECT);
+        }
+        float rad[NL];
+        for (int j = 0; j < NL; ++j) rad[j] = fix_nan_inf(S.radiance[j]);
+        const f3 pre = path_xyz<NL>(S.lam, rad, S.psum, tables);
         f3 res{clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
         res = fmax3(f3{0.0f, 0.0f, 0.0f}, res);
         const int32_t frame = S.frame;
@@ -1094,14 +1199,14 @@ int ensure_paths(RenderState* r, size_t cap) {
         if (p) (void)hipFree(p);
     r->capacity = 0;
     int rc;
-    if ((rc = dev_alloc(&r->paths, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(reinterpret_cast<uint8_t**>(&r->paths), cap * kPathBytes)) != AP_OK) return rc;
     for (int q = 0; q < 2; ++q) {
         if ((rc = dev_alloc(&r->rays[q], cap)) != AP_OK) return rc;
         if ((rc = dev_alloc(&r->path_of[q], cap)) != AP_OK) return rc;
     }
     if ((rc = dev_alloc(&r->hits, cap)) != AP_OK) return rc;
     if ((rc = dev_alloc(&r->shadow_rays, cap)) != AP_OK) return rc;
-    if ((rc = dev_alloc(&r->shadow_payload, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(reinterpret_cast<uint8_t**>(&r->shadow_payload), cap * kShadowPayBytes)) != AP_OK) return rc;
     if ((rc = dev_alloc(&r->shadow_hits, cap)) != AP_OK) return rc;
     r->capacity = cap;
     return AP_OK;
@@ -1149,7 +1254,7 @@ int ensure_nrc(RenderState* r, size_t cap, size_t train_cap) {
             if (p) (void)hipFree(p);
         r->nrc_verts = nullptr; r->nrc_vmask = nullptr; r->nrc_offsets = nullptr; r->nrc_pool = nullptr;
         r->nrc_train_cap = 0;
-        if ((rc = dev_alloc(&r->nrc_verts, train_cap * kNrcV * kNrcRec)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_verts, train_cap * kNrcV * kNrcVert)) != AP_OK) return rc;
         if ((rc = dev_alloc(&r->nrc_vmask, train_cap)) != AP_OK) return rc;
         if ((rc = dev_alloc(&r->nrc_offsets, train_cap)) != AP_OK) return rc;
         if ((rc = dev_alloc(&r->nrc_pool, train_cap * kNrcV * kNrcSample)) != AP_OK) return rc;
@@ -1165,6 +1270,137 @@ uint64_t nrc_frame_hash(uint64_t seed, int32_t frame) {
 
 }  // namespace
 
+// Context of one render call, shared by the wavelength variants.
+struct RunCtx {
+    ap_scene* s;
+    RenderState* r;
+    const ap_render_params* prm;
+    const ap_framebuffers* fb;
+    hipStream_t stream;
+    const ap_nrc_render_params* np;
+    NrcState* ns;
+    const f4* rgb_lut;
+    const f4* lam_lut;
+    const float* tables;
+    int32_t n_pix, per_batch;
+    bool guiding, nrc;
+    uint32_t n_stride, n_train_max;
+    CamParams cp;
+    int32_t inside, last_frame;
+};
+
+template <int NL>
+int run_frames(const RunCtx& c) {
+    ap_scene* s = c.s;
+    RenderState* r = c.r;
+    const ap_render_params* prm = c.prm;
+    const ap_framebuffers* fb = c.fb;
+    const hipStream_t stream = c.stream;
+    const ap_nrc_render_params* np = c.np;
+    NrcState* ns = c.ns;
+    const f4 *rgb_lut = c.rgb_lut, *lam_lut = c.lam_lut;
+    const float* tables = c.tables;
+    const int32_t n_pix = c.n_pix, per_batch = c.per_batch;
+    const bool guiding = c.guiding, nrc = c.nrc;
+    const uint32_t n_stride = c.n_stride, n_train_max = c.n_train_max;
+    const CamParams cp = c.cp;
+    const int32_t inside = c.inside, last_frame = c.last_frame;
+    auto* paths = static_cast<PathT<NL>*>(r->paths);
+    auto* pays = static_cast<ShadowPay<NL>*>(r->shadow_payload);
+    int rc = AP_OK;
+    for (int32_t f0 = 0; f0 < prm->spp; f0 += per_batch) {
+        const int32_t nf = std::min(per_batch, prm->spp - f0);
+        const uint32_t total = static_cast<uint32_t>(nf) * static_cast<uint32_t>(n_pix);
+        FrameParams fp{prm->frame_index + f0, nf, n_pix, inside, prm->seed};
+        hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, r->counters, total, 0u, 0u);
+        const GuideView gv = guiding ? guide_view(s->guide) : GuideView{};
+        hipLaunchKernelGGL(generate_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, cp, fp, r->eps, paths,
+                           r->rays[0], r->path_of[0], gv.recs);
+        NrcView nv{};
+        uint32_t n_offset = 0, n_train = 0;
+        if (nrc) {
+            const uint64_t h = nrc_frame_hash(prm->seed, fp.frame0);
+            if (n_train_max > 0) {
+                n_offset = static_cast<uint32_t>(h % n_stride);
+                n_train = std::min(n_train_max, (static_cast<uint32_t>(n_pix) - n_offset + n_stride - 1) / n_stride);
+            }
+            hipLaunchKernelGGL(nrc_init_kernel, grid_of(static_cast<size_t>(n_pix)), dim3(kBlock), 0, stream, n_pix,
+                               n_stride, n_offset, n_train, np->unbiased_every, r->nrc_paths, r->nrc_counters);
+            nv = NrcView{r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_verts, r->nrc_counters, np->spread};
+        }
+        int in_q = 0;
+        for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
+            const int out_q = 1 - in_q;
+            hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, out_q);
+            hipError_t e = launch_trace(s->dev, r->rays[in_q], total, r->hits, nullptr, false, stream, &r->counters[in_q]);
+            if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
+            ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
+                           prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
+            auto* rc_ptr = reinterpret_cast<unsigned long long*>(fb->ray_count);
+            if (guiding) {
+                if constexpr (NL == 1)  // guiding is rejected with several wavelengths before this point
+                    hipLaunchKernelGGL((shade_kernel<1, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
+                                       r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
+                                       paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
+                                       r->path_of[out_q], r->shadow_rays, pays, r->counters, fb->primary_id,
+                                       last_frame, rc_ptr, gv, nv);
+            } else if (nrc) {
+                hipLaunchKernelGGL((shade_kernel<NL, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
+                                   r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, paths,
+                                   r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
+                                   r->shadow_rays, pays, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
+            } else {
+                hipLaunchKernelGGL((shade_kernel<NL, false, false>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
+                                   r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, paths,
+                                   r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
+                                   r->shadow_rays, pays, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
+            }
+            e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
+            if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
+            hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->counters, pays,
+                               r->shadow_hits, paths, reinterpret_cast<unsigned long long*>(fb->ray_count));
+            in_q = out_q;
+            // Deep limits: stop once every path has ended.
+            if (prm->max_depth > 32 && (depth % 16) == 15) {
+                uint32_t live = 0;
+                e = hipMemcpyAsync(&live, &r->counters[in_q], sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) return hip_fail(e, "ap_render: live count");
+                if (live == 0) break;
+            }
+        }
+        if (nrc) {
+            // Cache vertices, then the training samples of the frame.
+            if ((rc = nrc_infer_dyn(ns, r->nrc_queries, r->nrc_qpath, &r->nrc_counters[0], total, r->nrc_qrgb,
+                                    stream)) != AP_OK)
+                return rc;
+            hipLaunchKernelGGL(nrc_apply_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->nrc_counters,
+                               r->nrc_qpath, r->nrc_qrgb, rgb_lut, lam_lut, tables, paths);
+            if (n_train > 0) {
+                hipLaunchKernelGGL(nrc_targets_kernel<NL>, grid_of(n_train), dim3(kBlock), 0, stream, n_train,
+                                   n_stride, n_offset, r->nrc_paths, paths, tables, r->nrc_verts, r->nrc_vmask);
+                hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, n_train, r->nrc_vmask,
+                                   r->nrc_offsets, r->nrc_counters);
+                hipLaunchKernelGGL(nrc_compact_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
+                                   n_offset, r->nrc_vmask, r->nrc_offsets, r->nrc_verts, r->nrc_pool);
+            }
+            r->nrc_last_train = n_train;
+            r->nrc_rendered = true;
+        }
+        hipLaunchKernelGGL(accumulate_kernel<NL>, grid_of(n_pix), dim3(kBlock), 0, stream, fp, paths, tables, fb->hdr,
+                           fb->odd, fb->even, fb->sum_xyz, gv.recs, guiding ? guide_slots(s->guide) : nullptr);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
+        if (guiding && (rc = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK) return rc;
+        if (nrc && n_train > 0 && np->train_steps > 0) {
+            rc = nrc_train_pool(ns, r->nrc_pool, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
+                                nrc_frame_hash(prm->seed, fp.frame0) ^ 0x747261696eull, stream);
+            if (rc != AP_OK) return rc;
+        }
+    }
+    return AP_OK;
+}
+
 int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm, const ap_framebuffers* fb,
                   hipStream_t stream, ap_nrc* cache = nullptr, const ap_nrc_render_params* np = nullptr) {
     if (!s || !cam || !prm || !fb || !fb->hdr) return fail(AP_ERR_INVALID_ARG, "ap_render: NULL argument");
@@ -1173,6 +1409,10 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
         return fail(AP_ERR_INVALID_ARG, "ap_render: bad spp, frame_index or max_depth");
     if (prm->triangle_material != AP_TRIANGLE_MATERIAL_OBJECT && prm->triangle_material != AP_TRIANGLE_MATERIAL_REFERENCE)
         return fail(AP_ERR_INVALID_ARG, "ap_render: bad triangle_material");
+    if (prm->wavelengths != 0 && prm->wavelengths != 1 && prm->wavelengths != 4)
+        return fail(AP_ERR_INVALID_ARG, "ap_render: wavelengths must be 1 or 4 (0 reads as 1)");
+    if (prm->wavelengths == 4 && prm->use_guiding)
+        return fail(AP_ERR_UNSUPPORTED, "ap_render: guiding runs with one wavelength per path");
     if (prm->spp == 0 || prm->max_depth == 0) return AP_OK;
     const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
     const float* tables = nullptr;
@@ -1239,95 +1479,9 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     const int32_t inside = camera_inside(s, cp.origin) ? 1 : 0;
     const int32_t last_frame = prm->frame_index + prm->spp - 1;
 
-    for (int32_t f0 = 0; f0 < prm->spp; f0 += per_batch) {
-        const int32_t nf = std::min(per_batch, prm->spp - f0);
-        const uint32_t total = static_cast<uint32_t>(nf) * static_cast<uint32_t>(n_pix);
-        FrameParams fp{prm->frame_index + f0, nf, n_pix, inside, prm->seed};
-        hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, r->counters, total, 0u, 0u);
-        const GuideView gv = guiding ? guide_view(s->guide) : GuideView{};
-        hipLaunchKernelGGL(generate_kernel, grid_of(total), dim3(kBlock), 0, stream, cp, fp, lam_lut, tables, r->eps,
-                           r->paths, r->rays[0], r->path_of[0], gv.recs);
-        NrcView nv{};
-        uint32_t n_offset = 0, n_train = 0;
-        if (nrc) {
-            const uint64_t h = nrc_frame_hash(prm->seed, fp.frame0);
-            if (n_train_max > 0) {
-                n_offset = static_cast<uint32_t>(h % n_stride);
-                n_train = std::min(n_train_max, (static_cast<uint32_t>(n_pix) - n_offset + n_stride - 1) / n_stride);
-            }
-            hipLaunchKernelGGL(nrc_init_kernel, grid_of(static_cast<size_t>(n_pix)), dim3(kBlock), 0, stream, n_pix,
-                               n_stride, n_offset, n_train, np->unbiased_every, r->nrc_paths, r->nrc_counters);
-            nv = NrcView{r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_verts, r->nrc_counters, np->spread};
-        }
-        int in_q = 0;
-        for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
-            const int out_q = 1 - in_q;
-            hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, out_q);
-            hipError_t e = launch_trace(s->dev, r->rays[in_q], total, r->hits, nullptr, false, stream, &r->counters[in_q]);
-            if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
-            ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
-                           prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
-            auto* rc_ptr = reinterpret_cast<unsigned long long*>(fb->ray_count);
-            if (guiding) {
-                hipLaunchKernelGGL((shade_kernel<true, false>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
-                                   r->mats, r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
-                                   r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
-                                   r->shadow_payload, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
-            } else if (nrc) {
-                hipLaunchKernelGGL((shade_kernel<false, true>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
-                                   r->mats, r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
-                                   r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
-                                   r->shadow_payload, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
-            } else {
-                hipLaunchKernelGGL((shade_kernel<false, false>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
-                                   r->mats, r->lights, s->dev.analytic, rgb_lut, r->eps, r->paths, r->rays[in_q],
-                                   r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q], r->shadow_rays,
-                                   r->shadow_payload, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
-            }
-            e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
-            if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
-            hipLaunchKernelGGL(resolve_kernel, grid_of(total), dim3(kBlock), 0, stream, r->counters, r->shadow_payload,
-                               r->shadow_hits, r->paths, reinterpret_cast<unsigned long long*>(fb->ray_count));
-            in_q = out_q;
-            // Deep limits: stop once every path has ended.
-            if (prm->max_depth > 32 && (depth % 16) == 15) {
-                uint32_t live = 0;
-                e = hipMemcpyAsync(&live, &r->counters[in_q], sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-                if (e == hipSuccess) e = hipStreamSynchronize(stream);
-                if (e != hipSuccess) return hip_fail(e, "ap_render: live count");
-                if (live == 0) break;
-            }
-        }
-        if (nrc) {
-            // Cache vertices, then the training samples of the frame.
-            if ((rc = nrc_infer_dyn(ns, r->nrc_queries, r->nrc_qpath, &r->nrc_counters[0], total, r->nrc_qrgb,
-                                    stream)) != AP_OK)
-                return rc;
-            hipLaunchKernelGGL(nrc_apply_kernel, grid_of(total), dim3(kBlock), 0, stream, r->nrc_counters,
-                               r->nrc_qpath, r->nrc_qrgb, rgb_lut, r->paths);
-            if (n_train > 0) {
-                hipLaunchKernelGGL(nrc_targets_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
-                                   n_offset, r->nrc_paths, r->paths, r->nrc_verts, r->nrc_vmask);
-                hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, n_train, r->nrc_vmask,
-                                   r->nrc_offsets, r->nrc_counters);
-                hipLaunchKernelGGL(nrc_compact_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
-                                   n_offset, r->nrc_vmask, r->nrc_offsets, r->nrc_verts, r->nrc_pool);
-            }
-            r->nrc_last_train = n_train;
-            r->nrc_rendered = true;
-        }
-        hipLaunchKernelGGL(accumulate_kernel, grid_of(n_pix), dim3(kBlock), 0, stream, fp, r->paths, fb->hdr, fb->odd,
-                           fb->even, fb->sum_xyz, gv.recs, guiding ? guide_slots(s->guide) : nullptr);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
-        if (guiding && (rc = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK) return rc;
-        if (nrc && n_train > 0 && np->train_steps > 0) {
-            rc = nrc_train_pool(ns, r->nrc_pool, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
-                                nrc_frame_hash(prm->seed, fp.frame0) ^ 0x747261696eull, stream);
-            if (rc != AP_OK) return rc;
-        }
-    }
-    return AP_OK;
+    RunCtx c{s, r, prm, fb, stream, np, ns, rgb_lut, lam_lut, tables, n_pix, per_batch, guiding, nrc, n_stride,
+             n_train_max, cp, inside, last_frame};
+    return prm->wavelengths == 4 ? run_frames<4>(c) : run_frames<1>(c);
 }
 
 }  // namespace ap
@@ -1350,6 +1504,7 @@ void ap_render_params_default(ap_render_params* p) {
     p->triangle_material = AP_TRIANGLE_MATERIAL_OBJECT;
     p->use_guiding = 0;
     p->seed = 0;
+    p->wavelengths = 1;
 }
 
 int ap_render(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, const ap_framebuffers* fb,

@@ -82,6 +82,14 @@ AP_HD float lambda_pdf_z(float l) {
     return 1.839f * apfm::exp(-0.5f * k * k) / 105.88243587f;
 }
 
+// PdfLambdaToCIE1931 of the mixture below, floored at 1e-16 as the sampler floors it.
+AP_HD float lambda_pdf_mix(float l) {
+    constexpr float total = 0.1f + 1.0f + 1.0f + 1.0f;
+    constexpr float uR = 0.1f / total, xR = 1.0f / total, yR = 1.0f / total, zR = 1.0f / total;
+    const float p = uR * 1.0f / (780.0f - 360.0f) + xR * lambda_pdf_x(l) + yR * lambda_pdf_y(l) + zR * lambda_pdf_z(l);
+    return fmaxf(p, 1e-16f);
+}
+
 // SampleCIE1931: 0.1 uniform + X (two Gaussians) + Y, Z (log-normal).
 AP_HD void sample_lambda(float r0, float r1, float r2, float& lambda, float& pdf) {
     constexpr float total = 0.1f + 1.0f + 1.0f + 1.0f;
@@ -104,9 +112,8 @@ AP_HD void sample_lambda(float r0, float r1, float r2, float& lambda, float& pdf
         const float t = 0.0000049890553249f + r1 * (1.0f - 0.0000049890553249f);
         l = clamp_lambda(apfm::exp(6.1114040395252154f + 0.0510000000000000f * ndtrif(t)));
     }
-    const float p = uR * 1.0f / (780.0f - 360.0f) + xR * lambda_pdf_x(l) + yR * lambda_pdf_y(l) + zR * lambda_pdf_z(l);
     lambda = l;
-    pdf = fmaxf(p, 1e-16f);
+    pdf = lambda_pdf_mix(l);
 }
 
 // tables: D65[421] then CMF[421*3].

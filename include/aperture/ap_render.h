@@ -46,6 +46,11 @@ typedef struct ap_render_params {
     int32_t triangle_material; /* AP_TRIANGLE_MATERIAL_* (default OBJECT) */
     int32_t use_guiding;       /* SceneSetting::bUsePathGuiding_ (probes, when built) */
     uint64_t seed;             /* PCG32 seed: stream per (pixel, frame) */
+    int32_t wavelengths;       /* 1 (or 0): one wavelength per path (the reference's CalculateRadiance);
+                                * 4: hero-wavelength sampling: the hero drawn by SampleCIE1931 and three
+                                * rotations by 105 nm over [360, 780], one path for all four, the pixel
+                                * estimate sum_j xyz(l_j) L(l_j) / sum_k p(l_k). Not with guiding. */
+    int32_t pad0;
 } ap_render_params;
 
 /* Caller-owned device framebuffers (GUI.hpp:232-274 owns them in the
@@ -59,7 +64,7 @@ typedef struct ap_framebuffers {
     uint64_t* ray_count; /* 2 counters, added to: rays traced by closest-hit [0] and shadow [1] launches */
 } ap_framebuffers;
 
-/* Reference defaults (config.cuh), spp = 1, frame 0, seed 0. */
+/* Reference defaults (config.cuh), spp = 1, frame 0, seed 0, one wavelength per path. */
 void ap_render_params_default(ap_render_params* params);
 
 /* Replaces SceneRenderer::Render(hdr, ldr, size, origin, dir, frameIndex, ...)'s

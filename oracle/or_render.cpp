@@ -621,28 +621,75 @@ struct Collect {
     }
 };
 
-// CalculateRadiance (render.cuh:460-999) for one camera ray; returns the
-// clamped XYZ sample and the primary hit id. With a guiding state G the
-// probes of G guide the BSDF sampling and NEE's MIS pdf, and the path's
-// guiding samples are written to `slots` (PG_COLLECT entries) if non-null.
-static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir, bool inside0,
-                                int* primary, const GuideState* G = nullptr, int frame = 0,
-                                ap_path_guiding_sample* slots = nullptr) {
+// The wavelengths of a path (the core's hero-wavelength mode when n = 4):
+// the hero and its rotations by 105 nm over [360, 780], with their bases,
+// D65 weights, CMF values and the sum of their pdfs.
+constexpr int MAXL = 4;
+struct Waves {
+    int n;
+    float lam[MAXL], lb[MAXL][4], d65[MAXL];
+    float3 xyz[MAXL];
+    float psum;
+};
+
+static Waves MakeWaves(Rng& rng, int n) {
+    Waves w;
+    w.n = n;
     float lam, lpdf;
     {
         const float r0 = rng.rand1(), r1 = rng.rand1(), r2 = rng.rand1();
         SampleCIE1931(r0, r1, r2, &lam, &lpdf);
     }
-    float lb[4];
-    QueryLambda(lam, lb);
-    const float d65 = D65Norm(lam);
-    const float3 xyz = LambdaToXYZ(lam);
+    w.lam[0] = lam;
+    w.psum = lpdf;
+    for (int j = 1; j < n; ++j) {
+        float l = lam + 105.0f * static_cast<float>(j);
+        if (l > 780.0f) l -= 420.0f;
+        w.lam[j] = l;
+        w.psum = w.psum + PdfCIE1931Mix(l);
+    }
+    for (int j = 0; j < n; ++j) {
+        QueryLambda(w.lam[j], w.lb[j]);
+        w.d65[j] = D65Norm(w.lam[j]);
+        w.xyz[j] = LambdaToXYZ(w.lam[j]);
+    }
+    return w;
+}
+
+// sum_j xyz_j L_j / psum.
+static float3 WavesXYZ(const Waves& w, const float* L) {
+    float3 acc = w.xyz[0] * L[0];
+    for (int j = 1; j < w.n; ++j) acc = acc + w.xyz[j] * L[j];
+    return acc / w.psum;
+}
+
+static float MeanMask(const float* m, int n) {
+    if (n == 1) return m[0];
+    float s = m[0];
+    for (int j = 1; j < n; ++j) s = s + m[j];
+    return s * (1.0f / static_cast<float>(n));
+}
+
+// CalculateRadiance (render.cuh:460-999) for one camera ray; returns the
+// clamped XYZ sample and the primary hit id. With a guiding state G the
+// probes of G guide the BSDF sampling and NEE's MIS pdf, and the path's
+// guiding samples are written to `slots` (PG_COLLECT entries) if non-null.
+// nl = 4 is the core's hero-wavelength mode (one wavelength without guiding
+// is the reference's own estimator): the hero's basis drives every sampling
+// decision, radiance and throughput are kept per wavelength, Russian
+// roulette and the NEE rate read the mean throughput.
+static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir, bool inside0,
+                                int* primary, const GuideState* G = nullptr, int frame = 0,
+                                ap_path_guiding_sample* slots = nullptr, int nl = 1) {
+    const Waves wv = MakeWaves(rng, nl);
+    const float* lb = wv.lb[0];
     const bool guiding = G != nullptr;
     Collect col;
     if (guiding) col.Clear();
     int ndd = 0;
     dir = normalize(dir);
-    float radiance = 0.0f, mask = 1.0f;
+    float radiance[MAXL], mask[MAXL];
+    for (int j = 0; j < nl; ++j) { radiance[j] = 0.0f; mask[j] = 1.0f; }
     float3 query{1.0f, 1.0f, 1.0f};  // maskForProbeQuery
     float3 ro = origin, rd = dir;
     bool inside = inside0;
@@ -661,9 +708,11 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             (void)rng.rand1();  // offset draw
             float endpoint = 0.0f;
             if (!inside) {
-                const float le = P.sky ? InteractC(lb, SkyLighting(rd)) : 0.0f;
-                radiance = radiance + mask * le * d65;
-                if (guiding) endpoint = le * d65;
+                for (int j = 0; j < nl; ++j) {
+                    const float le = P.sky ? InteractC(wv.lb[j], SkyLighting(rd)) : 0.0f;
+                    radiance[j] = radiance[j] + mask[j] * le * wv.d65[j];
+                    if (guiding) endpoint = le * wv.d65[0];
+                }
             }
             if (guiding) col.Endpoint(endpoint, ndd);
             break;
@@ -671,7 +720,8 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         Ctx c = HitContext(S, id, ro, rd, t);
         const int model = S.objs[id].material.shading_model;
         SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
-        sc.brdf = EvaluateBSDFWithCos(lb, model, rd, sc.next, c);
+        float brdf[MAXL];
+        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], model, rd, sc.next, c);
         const float offEps = eps * std::fmax(1.0f, length(c.pos));
         c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
         inside = sc.inside;
@@ -688,7 +738,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                         if (dot(gs.dir, c.normal) > 1.0e-12f) {
                             const float pBS = BSDF_PDF(lb, model, c.gb, c.normal, rd, gs.dir);
                             sc.next = gs.dir;
-                            sc.brdf = EvaluateBSDFWithCos(lb, model, rd, gs.dir, c);
+                            brdf[0] = EvaluateBSDFWithCos(lb, model, rd, gs.dir, c);
                             sc.pdf = (1.0f - alpha) * std::fmax(pBS, 1.0e-12f) + alpha * gs.pdf;
                         } else {
                             probe = nullptr;
@@ -709,15 +759,17 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                 const float pL = NEEPDFOmegaFromHit(S.objs[id], S.lights[c.lightIndex], lastPos, c.pos, c.normal);
                 if (pL >= 0.0f && pB > 0.0f) misW = PowerHeuristic(pB, pL);
             }
-            radiance = radiance + mask * InteractC(lb, sc.emissive) * d65 * misW;
-            if (guiding) endpoint = InteractC(lb, sc.emissive * misW) * d65;
+            for (int j = 0; j < nl; ++j)
+                radiance[j] = radiance[j] + mask[j] * InteractC(wv.lb[j], sc.emissive) * wv.d65[j] * misW;
+            if (guiding) endpoint = InteractC(lb, sc.emissive * misW) * wv.d65[0];
         } else if (guiding && !(sc.emissive.x == 0.0f && sc.emissive.y == 0.0f && sc.emissive.z == 0.0f)) {
-            endpoint = InteractC(lb, sc.emissive) * d65;
+            endpoint = InteractC(lb, sc.emissive) * wv.d65[0];
         }
         if (guiding) col.Endpoint(endpoint, ndd);
         if (sc.terminate) break;
         bool neeValid = false;
-        const float rate = P.nee_always ? 1.0f : std::sqrt(saturate_(mask));
+        const float mm = MeanMask(mask, nl);
+        const float rate = P.nee_always ? 1.0f : std::sqrt(saturate_(mm));
         if (P.use_nee && ndd < P.nee_max_depth && !sc.delta && !inside && rng.rand1() < rate) {
             const LightSample ls = NEESampleLight(S, c.pos, rng);
             if (ls.valid) {
@@ -748,11 +800,8 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                     const float cosX = std::fmax(0.0f, dot(c.normal, L));
                     const float cosY = std::fmax(0.0f, dot(ls.normal, -L));
                     if (!(cosX <= 0.0f || cosY <= 0.0f)) {
-                        const float bc = EvaluateBSDFWithCos(lb, model, rd, L, c);
                         const float G2 = cosY / d2;
                         const ap_gbuffer& lg = S.objs[ls.object].material.texture.base;
-                        const float resp = InteractC(lb, F3(lg.emissive));
-                        const float contribution = bc * resp * (G2 / pAT);
                         neeValid = true;
                         const float pLo = AreaToSolidAnglePDF(c.pos, ls.point, ls.normal, ls.pFacet * ls.pA);
                         float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
@@ -760,7 +809,13 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                             const float pPr = OctahedronHemispherePDF(*probe, L, c.normal, query);
                             pBs = (1.0f - alpha) * pBs + alpha * pPr;
                         }
-                        radiance = radiance + mask * (contribution * PowerHeuristic(pLo, pBs)) * d65;
+                        const float w = PowerHeuristic(pLo, pBs);
+                        for (int j = 0; j < nl; ++j) {
+                            const float bc = EvaluateBSDFWithCos(wv.lb[j], model, rd, L, c);
+                            const float resp = InteractC(wv.lb[j], F3(lg.emissive));
+                            const float contribution = bc * resp * (G2 / pAT);
+                            radiance[j] = radiance[j] + mask[j] * (contribution * w) * wv.d65[j];
+                        }
                     }
                 }
             }
@@ -768,21 +823,24 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         prevNEE = neeValid;
         lastPdf = sc.pdf;
         lastPos = c.pos;
-        const float contrib = sc.brdf / std::fmax(sc.pdf, 1e-12f);
+        float contrib[MAXL];
+        for (int j = 0; j < nl; ++j) contrib[j] = brdf[j] / std::fmax(sc.pdf, 1e-12f);
         float p = 1.0f;
         if (ndd >= P.rr_min_depth) {
-            const float basic = clampf(mask * 0.5f + mask * 0.5f, 0.05f, 0.95f);
+            const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
             p = lerp(basic, 1.0f, inside ? 0.99f : 0.0f);
             if (rng.rand1() > p) break;
         }
         if (guiding) {
             const bool allowStart = !sc.delta && !sc.pureLight && !inside;
-            col.Advance(sc.next, contrib, p, ndd, allowStart, rd, c.pos, c.normal);
+            col.Advance(sc.next, contrib[0], p, ndd, allowStart, rd, c.pos, c.normal);
         }
-        mask = mask * contrib;
-        mask = mask / p;
+        for (int j = 0; j < nl; ++j) {
+            mask[j] = mask[j] * contrib[j];
+            mask[j] = mask[j] / p;
+        }
         if (guiding) {
-            query = fmax3(xyz * mask, float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
+            query = fmax3(wv.xyz[0] * mask[0], float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
             // sum3 + PROBES_QUERY_EPS is a double sum (the constant is a double literal).
             query = query / static_cast<float>(static_cast<double>(query.x + query.y + query.z) + 1.0e-6);
         }
@@ -790,10 +848,13 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         rd = sc.next;
         if (!sc.delta) ndd++;
     }
-    if (guiding && slots) col.Emit(xyz, lpdf, slots);
-    float r = radiance;
-    if (std::isnan(r) || std::isinf(r)) r = 0.0f;
-    const float3 pre = (xyz * r) / lpdf;
+    if (guiding && slots) col.Emit(wv.xyz[0], wv.psum, slots);
+    float r[MAXL] = {};
+    for (int j = 0; j < nl; ++j) {
+        r[j] = radiance[j];
+        if (std::isnan(r[j]) || std::isinf(r[j])) r[j] = 0.0f;
+    }
+    const float3 pre = WavesXYZ(wv, r);
     return {clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
 }
 
@@ -818,19 +879,13 @@ struct NrcTrain {
 // Training paths record each scattering vertex before the cache vertex.
 static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir,
                                    bool inside0, const NrcOpts& N, bool train, bool unbiased, int pixel,
-                                   NrcTrain* out, bool* queried) {
-    float lam, lpdf;
-    {
-        const float r0 = rng.rand1(), r1 = rng.rand1(), r2 = rng.rand1();
-        SampleCIE1931(r0, r1, r2, &lam, &lpdf);
-    }
-    float lb[4];
-    QueryLambda(lam, lb);
-    const float d65 = D65Norm(lam);
-    const float3 xyz = LambdaToXYZ(lam);
+                                   NrcTrain* out, bool* queried, int nl = 1) {
+    const Waves wv = MakeWaves(rng, nl);
+    const float* lb = wv.lb[0];
     int ndd = 0;
     dir = normalize(dir);
-    float radiance = 0.0f, mask = 1.0f;
+    float radiance[MAXL], mask[MAXL];
+    for (int j = 0; j < nl; ++j) { radiance[j] = 0.0f; mask[j] = 1.0f; }
     float3 ro = origin, rd = dir;
     bool inside = inside0;
     bool prevNEE = false;
@@ -839,7 +894,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
     float a0 = 0.0f, spreadSum = 0.0f;
     bool suffix = false;
     int nv = 0;
-    float vrec[NRC_V][16];
+    float vrec[NRC_V][16], vR[NRC_V][MAXL], vM[NRC_V][MAXL];
     bool cacheEnd = false;
     float query[16];
     if (out) for (int k = 0; k < NRC_V; ++k) out->valid[k] = false;
@@ -860,8 +915,10 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         if (id < 0) {
             (void)rng.rand1();
             if (!inside) {
-                const float le = P.sky ? InteractC(lb, SkyLighting(rd)) : 0.0f;
-                radiance = radiance + mask * le * d65;
+                for (int j = 0; j < nl; ++j) {
+                    const float le = P.sky ? InteractC(wv.lb[j], SkyLighting(rd)) : 0.0f;
+                    radiance[j] = radiance[j] + mask[j] * le * wv.d65[j];
+                }
             }
             break;
         }
@@ -869,7 +926,8 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         const float3 pos0 = c.pos;
         const int model = S.objs[id].material.shading_model;
         SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
-        sc.brdf = EvaluateBSDFWithCos(lb, model, rd, sc.next, c);
+        float brdf[MAXL];
+        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], model, rd, sc.next, c);
         const float offEps = eps * std::fmax(1.0f, length(c.pos));
         c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
         inside = sc.inside;
@@ -880,7 +938,8 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
                 const float pL = NEEPDFOmegaFromHit(S.objs[id], S.lights[c.lightIndex], lastPos, c.pos, c.normal);
                 if (pL >= 0.0f && pB > 0.0f) misW = PowerHeuristic(pB, pL);
             }
-            radiance = radiance + mask * InteractC(lb, sc.emissive) * d65 * misW;
+            for (int j = 0; j < nl; ++j)
+                radiance[j] = radiance[j] + mask[j] * InteractC(wv.lb[j], sc.emissive) * wv.d65[j] * misW;
         }
         // Spread heuristic (the core's shade kernel, NRC branch).
         {
@@ -912,14 +971,14 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             }
             if (train && scatter && nv < NRC_V) {
                 record(vrec[nv], pos0, c);
-                vrec[nv][13] = radiance;
-                vrec[nv][14] = mask;
+                for (int j = 0; j < nl; ++j) { vR[nv][j] = radiance[j]; vM[nv][j] = mask[j]; }
                 ++nv;
             }
         }
         if (sc.terminate) break;
         bool neeValid = false;
-        const float rate = P.nee_always ? 1.0f : std::sqrt(saturate_(mask));
+        const float mm = MeanMask(mask, nl);
+        const float rate = P.nee_always ? 1.0f : std::sqrt(saturate_(mm));
         if (P.use_nee && ndd < P.nee_max_depth && !sc.delta && !inside && rng.rand1() < rate) {
             const LightSample ls = NEESampleLight(S, c.pos, rng);
             if (ls.valid) {
@@ -949,15 +1008,18 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
                     const float cosX = std::fmax(0.0f, dot(c.normal, L));
                     const float cosY = std::fmax(0.0f, dot(ls.normal, -L));
                     if (!(cosX <= 0.0f || cosY <= 0.0f)) {
-                        const float bc = EvaluateBSDFWithCos(lb, model, rd, L, c);
                         const float G2 = cosY / d2;
                         const ap_gbuffer& lg = S.objs[ls.object].material.texture.base;
-                        const float resp = InteractC(lb, F3(lg.emissive));
-                        const float contribution = bc * resp * (G2 / pAT);
                         neeValid = true;
                         const float pLo = AreaToSolidAnglePDF(c.pos, ls.point, ls.normal, ls.pFacet * ls.pA);
                         const float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
-                        radiance = radiance + mask * (contribution * PowerHeuristic(pLo, pBs)) * d65;
+                        const float w = PowerHeuristic(pLo, pBs);
+                        for (int j = 0; j < nl; ++j) {
+                            const float bc = EvaluateBSDFWithCos(wv.lb[j], model, rd, L, c);
+                            const float resp = InteractC(wv.lb[j], F3(lg.emissive));
+                            const float contribution = bc * resp * (G2 / pAT);
+                            radiance[j] = radiance[j] + mask[j] * (contribution * w) * wv.d65[j];
+                        }
                     }
                 }
             }
@@ -965,32 +1027,37 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         prevNEE = neeValid;
         lastPdf = sc.pdf;
         lastPos = c.pos;
-        const float contrib = sc.brdf / std::fmax(sc.pdf, 1e-12f);
+        float contrib[MAXL];
+        for (int j = 0; j < nl; ++j) contrib[j] = brdf[j] / std::fmax(sc.pdf, 1e-12f);
         float p = 1.0f;
         if (ndd >= P.rr_min_depth) {
-            const float basic = clampf(mask * 0.5f + mask * 0.5f, 0.05f, 0.95f);
+            const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
             p = lerp(basic, 1.0f, inside ? 0.99f : 0.0f);
             if (rng.rand1() > p) break;
         }
-        mask = mask * contrib;
-        mask = mask / p;
+        for (int j = 0; j < nl; ++j) {
+            mask[j] = mask[j] * contrib[j];
+            mask[j] = mask[j] / p;
+        }
         ro = c.pos;
         rd = sc.next;
         if (!sc.delta) ndd++;
     }
     if (cacheEnd) {
-        // The cache's prediction, clamped at 0, at the path's wavelength.
+        // The cache's prediction, clamped at 0, at the path's wavelengths.
         float feat[64], y[3];
         or_nrc_encode(query, 1, N.bmin, N.bmax, feat);
         or_nrc_forward(N.params, feat, 1, 1, y);
         const float3 cv = fmax3(float3{y[0], y[1], y[2]}, float3{0.0f, 0.0f, 0.0f});
-        radiance = radiance + mask * InteractC(lb, cv) * d65;
+        for (int j = 0; j < nl; ++j) radiance[j] = radiance[j] + mask[j] * InteractC(wv.lb[j], cv) * wv.d65[j];
         if (queried) *queried = true;
     }
     if (out) {
         for (int k = 0; k < nv; ++k) {
-            const float L = (radiance - vrec[k][13]) / vrec[k][14];
-            const float3 rgb = XYZ2SRGB((xyz * L) / lpdf);
+            float L[MAXL] = {};
+            for (int j = 0; j < nl; ++j)
+                L[j] = (nl > 1 && !(vM[k][j] > 0.0f)) ? 0.0f : (radiance[j] - vR[k][j]) / vM[k][j];
+            const float3 rgb = XYZ2SRGB(WavesXYZ(wv, L));
             if (std::isnan(rgb.x) || std::isnan(rgb.y) || std::isnan(rgb.z) || std::isinf(rgb.x) ||
                 std::isinf(rgb.y) || std::isinf(rgb.z))
                 continue;
@@ -1003,9 +1070,12 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             out->valid[k] = true;
         }
     }
-    float r = radiance;
-    if (std::isnan(r) || std::isinf(r)) r = 0.0f;
-    const float3 pre = (xyz * r) / lpdf;
+    float r[MAXL] = {};
+    for (int j = 0; j < nl; ++j) {
+        r[j] = radiance[j];
+        if (std::isnan(r[j]) || std::isinf(r[j])) r[j] = 0.0f;
+    }
+    const float3 pre = WavesXYZ(wv, r);
     return {clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
 }
 
@@ -1105,6 +1175,8 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
     const RenderOpts& P = U.P;
     const bool inside0 = U.inside0;
     const int64_t npix = static_cast<int64_t>(U.W) * U.H;
+    const int nl = prm->wavelengths == 4 ? 4 : 1;
+    if (nl != 1 && prm->use_guiding) return -1;
     if (prm->use_guiding) {
         // Frames in order; each renders with the probes built from the frames
         // before it, then folds its own samples in (scene.cu:308-386).
@@ -1119,7 +1191,7 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
                 const float3 rd = CameraRay(U, rng, p);
                 int prim = -1;
                 float3 res = CalculateRadiance(S, P, rng, U.camO, rd, inside0, &prim, &G, frame,
-                                               &slots[static_cast<size_t>(p) * PG_COLLECT]);
+                                               &slots[static_cast<size_t>(p) * PG_COLLECT], 1);
                 res = fmax3(float3{0, 0, 0}, res);
                 const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
                 const float3 acc = lerp(prev, res, 1.0f / static_cast<float>(1 + frame));
@@ -1143,7 +1215,7 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
             Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
             const float3 rd = CameraRay(U, rng, p);
             int prim = -1;
-            float3 res = CalculateRadiance(S, P, rng, U.camO, rd, inside0, &prim);
+            float3 res = CalculateRadiance(S, P, rng, U.camO, rd, inside0, &prim, nullptr, 0, nullptr, nl);
             res = fmax3(float3{0, 0, 0}, res);
             acc = lerp(acc, res, 1.0f / static_cast<float>(1 + frame));
             if (sums) {
@@ -1175,6 +1247,7 @@ int or_render_nrc(const or_bvh* bvh, const ap_scene_object* objs, size_t n, cons
                   double* sums, float* samples, uint8_t* valid, uint32_t* n_queries, int threads) {
     const Setup U = MakeSetup(bvh, objs, n, cam, prm, cull_margin);
     const int64_t npix = static_cast<int64_t>(U.W) * U.H;
+    const int nl = prm->wavelengths == 4 ? 4 : 1;
     NrcOpts N{spread, params, {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]}};
     uint32_t stride = 1, n_train_max = 0;
     if (train_paths > 0) {
@@ -1208,7 +1281,7 @@ int or_render_nrc(const or_bvh* bvh, const ap_scene_object* objs, size_t n, cons
             NrcTrain tr;
             bool q = false;
             float3 res = CalculateRadianceNRC(U.S, U.P, rng, U.camO, rd, U.inside0, N, train, unbiased,
-                                              static_cast<int>(p), &tr, &q);
+                                              static_cast<int>(p), &tr, &q, nl);
             frame_queries += q ? 1u : 0u;
             res = fmax3(float3{0, 0, 0}, res);
             const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};

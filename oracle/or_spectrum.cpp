@@ -249,17 +249,24 @@ float PdfZ(float l) {
     return 1.839f * apfm::exp(-0.5f * k * k) / 105.88243587f;
 }
 
-void SampleCIE1931(float r0, float r1, float r2, float* lambda, float* pdf) {
+// The mixture pdf SampleCIE1931 returns, floored at 1e-16.
+float PdfCIE1931Mix(float l) {
     const float total = 0.1f + 1.0f + 1.0f + 1.0f;
     const float uR = 0.1f / total, xR = 1.0f / total, yR = 1.0f / total, zR = 1.0f / total;
+    const float p = uR * 1.0f / (780.0f - 360.0f) + xR * PdfX(l) + yR * PdfY(l) + zR * PdfZ(l);
+    return std::fmax(p, 1e-16f);
+}
+
+void SampleCIE1931(float r0, float r1, float r2, float* lambda, float* pdf) {
+    const float total = 0.1f + 1.0f + 1.0f + 1.0f;
+    const float uR = 0.1f / total, xR = 1.0f / total, yR = 1.0f / total;
     float l;
     if (r0 < uR) l = r1 * (780.0f - 360.0f) + 360.0f;
     else if (r0 < uR + xR) l = SampleX(r1, r2);
     else if (r0 < uR + xR + yR) l = SampleY(r1);
     else l = SampleZ(r1);
-    const float p = uR * 1.0f / (780.0f - 360.0f) + xR * PdfX(l) + yR * PdfY(l) + zR * PdfZ(l);
     *lambda = l;
-    *pdf = std::fmax(p, 1e-16f);
+    *pdf = PdfCIE1931Mix(l);
 }
 
 float D65Norm(float lambda) {

@@ -32,6 +32,7 @@ void QueryLambda(float lambda, float out[4]);
 float Interact(const float lamBasis[4], float3 rgb);
 float ndtrif(float u);
 void SampleCIE1931(float r0, float r1, float r2, float* lambda, float* pdf);
+float PdfCIE1931Mix(float lambda);
 float D65Norm(float lambda);
 float3 LambdaToXYZ(float lambda);
 float3 XYZ2SRGB(float3 xyz);

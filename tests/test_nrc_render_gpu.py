@@ -115,3 +115,10 @@ def test_training_is_deterministic_and_moves_the_weights(gpu):
     assert np.array_equal(h1, h2) and np.array_equal(w1, w2)
     assert np.abs(w1 - w0).max() > 0.0
     assert np.isfinite(h1).all() and np.isfinite(w1).all()
+
+
+def test_frozen_cache_four_wavelengths_matches_oracle(gpu):
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 40, 40)
+    prm = ap.RenderParams.defaults(spp=2, max_depth=12, seed=8, wavelengths=4)
+    frozen_parity(objs, cam, prm, "box, four wavelengths per path")

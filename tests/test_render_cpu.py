@@ -8,7 +8,10 @@ closed forms of the reference's estimator (core/render/render.cuh:460-999):
     the point-to-rectangle form factor; the NEE-only and BSDF-only limits of
     the same estimator converge to the same value;
   * the FP64 sums agree with the FP32 running mean; seeds change the noise
-    only.
+    only;
+  * hero-wavelength sampling (four wavelengths per path, the core's
+    extension) keeps the closed forms and cuts the colour noise of the
+    light seen directly by a large factor.
 """
 import math
 
@@ -54,10 +57,11 @@ def test_direct_view_of_light():
     assert abs(xyz[0] / xyz[1] - D65_XZ[0]) < 0.02 and abs(xyz[2] / xyz[1] - D65_XZ[1]) < 0.02
 
 
-def render_floor(use_nee, nee_always, spp, seed=0):
+def render_floor(use_nee, nee_always, spp, seed=0, wavelengths=1):
     objs = floor_and_light(0.5, 10.0, a=0.25, h=1.0)
     cam = ap.Camera.make([0.0, 0.5, 0.0], [0.0, -1.0, 0.001], 1.0, 4, 4)
-    prm = ap.RenderParams.defaults(spp=spp, max_depth=2, sky=0, use_nee=use_nee, nee_always=nee_always, seed=seed)
+    prm = ap.RenderParams.defaults(spp=spp, max_depth=2, sky=0, use_nee=use_nee, nee_always=nee_always, seed=seed,
+                                   wavelengths=wavelengths)
     hdr, sums, prim = ol.OracleBVH(objs).render(cam, prm, cull_margin=2 ** -12)
     assert (prim >= 0).all() and (prim < len(objs) - 1).all()  # every camera ray lands on the floor
     return hdr, sums
@@ -92,3 +96,30 @@ def test_seed_changes_noise_not_mean():
     # must sit within a few standard errors of zero.
     d = (m[0] - m[1]).reshape(-1, 3).sum(1)
     assert abs(d.mean()) < 4.0 * d.std() / math.sqrt(len(d)), (d.mean(), d.std())
+
+
+def test_hero_wavelengths_keep_the_closed_form():
+    expected = 0.5 * 10.0 * 4 * corner_form_factor(0.25, 0.25)
+    hdr, sums = render_floor(1, 1, 2048, seed=3, wavelengths=4)
+    y = hdr.reshape(-1, 3)[:, 1].mean()
+    assert abs(y / expected - 1.0) < 0.03, (y, expected)
+    bsdf, _ = render_floor(0, 0, 8192, seed=4, wavelengths=4)
+    b = bsdf.reshape(-1, 3)[:, 1].mean()
+    assert abs(b / expected - 1.0) < 0.08, (b, expected)
+
+
+def test_hero_wavelengths_cut_colour_noise():
+    objs = floor_and_light(0.5, 15.0, a=2.0, h=1.0)
+    cam = ap.Camera.make([0.0, 0.2, 0.0], [0.0, 1.0, 0.001], 10.0, 16, 16)
+    bvh = ol.OracleBVH(objs)
+    out = {}
+    for nl in (1, 4):
+        prm = ap.RenderParams.defaults(spp=4, max_depth=1, sky=0, wavelengths=nl, seed=nl)
+        hdr, _, _ = bvh.render(cam, prm, cull_margin=2 ** -12)
+        px = hdr.reshape(-1, 3).astype(np.float64)
+        out[nl] = (px.mean(0), px.var(0))
+        assert abs(px[:, 1].mean() / 15.0 - 1.0) < 0.03, (nl, px.mean(0))
+    # Every pixel sees the same emitter: the spread across pixels is the
+    # spectral-sampling noise alone.
+    ratio = out[1][1] / out[4][1]
+    assert (ratio > 8.0).all(), ratio

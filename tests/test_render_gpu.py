@@ -13,7 +13,9 @@ Bars:
   * the FP64 per-pixel sums equal spp x the running mean to FP32 rounding,
     and odd/even halves average to the full mean;
   * progressive calls (frame_index advancing) equal one call of the summed spp;
-  * the gray floor under a square light reproduces the closed-form irradiance.
+  * the gray floor under a square light reproduces the closed-form irradiance;
+  * hero-wavelength sampling (four wavelengths per path) is bit-identical to
+    the oracle's restatement of it as well.
 """
 import numpy as np
 import pytest
@@ -178,5 +180,21 @@ def test_render_rejects_unsupported_and_bad_arguments(gpu):
     with pytest.raises(ap.ApertureError) as e:
         gscene.render(cam, ap.RenderParams.defaults(), hdr)
     assert e.value.code == ap.ERR_UNSUPPORTED
+    with pytest.raises(ap.ApertureError) as e:
+        scene.render(cam, ap.RenderParams.defaults(wavelengths=3), hdr)
+    assert e.value.code == ap.ERR_INVALID_ARG
+    with pytest.raises(ap.ApertureError) as e:
+        scene.render(cam, ap.RenderParams.defaults(wavelengths=4, use_guiding=1), hdr)
+    assert e.value.code == ap.ERR_UNSUPPORTED
     gscene.close()
     scene.close()
+
+
+def test_hero_wavelengths_match_oracle(gpu):
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make(BOX_CAM["origin"], BOX_CAM["direction"], BOX_CAM["fov_deg"], 48, 48)
+    compare_with_oracle(objs, cam, box_params(spp=8, wavelengths=4), "box scene, four wavelengths per path")
+    hf = ap.synth_heightfield_scene(64, 16, 4, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 48, 27)
+    compare_with_oracle(hf, cam, ap.RenderParams.defaults(spp=4, max_depth=16, rr_min_depth=4, wavelengths=4),
+                        "heightfield 64^2, four wavelengths per path")
