@@ -122,7 +122,8 @@ class RenderParams(ctypes.Structure):
 class NrcRenderParams(ctypes.Structure):
     """ap_nrc_render_params (include/aperture/ap_nrc.h)."""
     _fields_ = [("spread", ctypes.c_float), ("train_paths", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
-                ("batch", ctypes.c_uint32), ("learning_rate", ctypes.c_float), ("unbiased_every", ctypes.c_uint32)]
+                ("batch", ctypes.c_uint32), ("learning_rate", ctypes.c_float), ("unbiased_every", ctypes.c_uint32),
+                ("train_passes", ctypes.c_uint32)]
 
     @classmethod
     def defaults(cls, **kw):

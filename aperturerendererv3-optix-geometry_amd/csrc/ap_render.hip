@@ -119,6 +119,7 @@ struct RenderState {
     LightRec* lights = nullptr;
     int32_t n_lights = 0;
     int32_t n_mats = 0;
+    std::vector<int32_t> analytic_objs;   // cube and sphere objects (the camera-inside test)
     // path buffers
     size_t capacity = 0;
     void* paths = nullptr;            // PathT<NL>, kPathBytes per entry
@@ -239,12 +240,17 @@ struct CamParams {
     int32_t width, height;
 };
 
+// Paths of a batch: frames x n_pixels, path i of a frame on pixel
+// pix_offset + i * pix_stride (the identity for a rendered frame; the
+// training pixels for a training-only pass of the radiance cache).
 struct FrameParams {
     int32_t frame0;
     int32_t frames;
     int32_t n_pixels;
     int32_t inside;
     uint64_t seed;
+    uint32_t pix_offset;
+    uint32_t pix_stride;
 };
 
 __global__ void set_counters_kernel(uint32_t* c, uint32_t q0, uint32_t q1, uint32_t s) {
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FramePa
     const uint32_t total = static_cast<uint32_t>(fp.n_pixels) * static_cast<uint32_t>(fp.frames);
     if (i >= total) return;
     const int32_t f = static_cast<int32_t>(i / fp.n_pixels);
-    const int32_t p = static_cast<int32_t>(i % fp.n_pixels);
+    const int32_t p = static_cast<int32_t>(fp.pix_offset + (i % fp.n_pixels) * fp.pix_stride);
     const int32_t px = p % cam.width, py = p / cam.width;
     const int32_t frame = fp.frame0 + f;
     Pcg32 rng = pcg32_path(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), fp.seed);
@@ -830,15 +836,16 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
 
 // Training paths of a frame (the pixels offset + k * stride, k < n_train)
 // and the reset of the cache counters.
+// With all_train (a training-only pass), path i is training slot i.
 __global__ __launch_bounds__(kBlock) void nrc_init_kernel(int32_t n_pix, uint32_t stride, uint32_t offset,
-                                                          uint32_t n_train, uint32_t unbiased_every,
+                                                          uint32_t n_train, uint32_t unbiased_every, uint32_t all_train,
                                                           NrcPath* __restrict__ paths, uint32_t* __restrict__ counters) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i == 0) { counters[0] = 0; counters[1] = 0; }
     if (i >= static_cast<uint32_t>(n_pix)) return;
     NrcPath P{0.0f, 0.0f, 0u, -1};
-    if (i >= offset && (i - offset) % stride == 0u) {
-        const uint32_t t = (i - offset) / stride;
+    if (all_train || (i >= offset && (i - offset) % stride == 0u)) {
+        const uint32_t t = all_train ? i : (i - offset) / stride;
         if (t < n_train) {
             P.flags = NRC_TRAIN | ((unbiased_every != 0u && t % unbiased_every == 0u) ? NRC_UNBIASED : 0u);
             P.slot = static_cast<int32_t>(t);
@@ -860,8 +867,12 @@ __global__ __launch_bounds__(kBlock) void nrc_apply_kernel(const uint32_t* __res
     if (q >= counters[0]) return;
     PathT<NL>& S = paths[qpath[q]];
     const f3 c = fmax3(f3{rgb[3 * q], rgb[3 * q + 1], rgb[3 * q + 2]}, f3{0.0f, 0.0f, 0.0f});
+    // interact() with the RGB basis looked up once for all wavelengths.
+    const float lum = max3(c);
+    const f4 cb = query_rgb_basis(rgb_lut, lum > 1.0f ? c / lum : c);
     for (int j = 0; j < NL; ++j) {
-        const float le = interact(rgb_lut, query_lambda(lam_lut, S.lam[j]), c);
+        const f4 lb = query_lambda(lam_lut, S.lam[j]);
+        const float le = lum > 1.0f ? spectral_interact(cb, lb) * lum : spectral_interact(cb, lb);
         S.radiance[j] = S.radiance[j] + S.mask[j] * le * d65_norm(tables, S.lam[j]);
     }
 }
@@ -876,13 +887,13 @@ __device__ __forceinline__ bool finite3(f3 v) {
 // adds nothing.
 template <int NL>
 __global__ __launch_bounds__(kBlock) void nrc_targets_kernel(uint32_t n_train, uint32_t stride, uint32_t offset,
-                                                             const NrcPath* __restrict__ npaths,
+                                                             uint32_t pid_direct, const NrcPath* __restrict__ npaths,
                                                              const PathT<NL>* __restrict__ paths,
                                                              const float* __restrict__ tables,
                                                              float* __restrict__ verts, uint32_t* __restrict__ vmask) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n_train) return;
-    const uint32_t pid = offset + t * stride;
+    const uint32_t pid = pid_direct ? t : offset + t * stride;
     const uint32_t nvx = min(npaths[pid].flags >> 8, static_cast<uint32_t>(kNrcV));
     const PathT<NL>& S = paths[pid];
     uint32_t m = 0;
@@ -1187,6 +1198,10 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
     }
     r->n_lights = static_cast<int32_t>(lights.size());
     r->n_mats = static_cast<int32_t>(mats.size());
+    r->analytic_objs.clear();
+    for (size_t i = 0; i < n; ++i)
+        if (s->objects[i].type == AP_OBJ_CUBE || s->objects[i].type == AP_OBJ_SPHERE)
+            r->analytic_objs.push_back(static_cast<int32_t>(i));
     r->tri_mode = tri_mode;
     return AP_OK;
 }
@@ -1214,9 +1229,9 @@ int ensure_paths(RenderState* r, size_t cap) {
 
 // OverlapBase at the camera origin over every object (intersection.cuh:399-414,
 // aabb.cuh:91-106, sphere.cuh:179-192): whether the camera starts inside one.
-bool camera_inside(const ap_scene* s, f3 o) {
-    for (const ap_scene_object& ob : s->objects) {
-        if (ob.type != AP_OBJ_CUBE && ob.type != AP_OBJ_SPHERE) continue;
+bool camera_inside(const ap_scene* s, const std::vector<int32_t>& analytic, f3 o) {
+    for (const int32_t k : analytic) {
+        const ap_scene_object& ob = s->objects[k];
         const f3 lo = mul(M3(ob.world_to_object), o - F(ob.center));
         const f3 e = F(ob.extent);
         if (ob.type == AP_OBJ_CUBE) {
@@ -1308,26 +1323,25 @@ int run_frames(const RunCtx& c) {
     auto* paths = static_cast<PathT<NL>*>(r->paths);
     auto* pays = static_cast<ShadowPay<NL>*>(r->shadow_payload);
     int rc = AP_OK;
-    for (int32_t f0 = 0; f0 < prm->spp; f0 += per_batch) {
-        const int32_t nf = std::min(per_batch, prm->spp - f0);
-        const uint32_t total = static_cast<uint32_t>(nf) * static_cast<uint32_t>(n_pix);
-        FrameParams fp{prm->frame_index + f0, nf, n_pix, inside, prm->seed};
+    // One batch of paths: a rendered batch of frames, or (train_only) the
+    // training paths of one pass of the radiance cache, which renders
+    // nothing into the framebuffers.
+    auto run_batch = [&](const FrameParams& fp, bool train_only, uint32_t n_offset, uint32_t n_train,
+                         uint64_t train_seed) -> int {
+        const uint32_t total = static_cast<uint32_t>(fp.frames) * static_cast<uint32_t>(fp.n_pixels);
         hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, r->counters, total, 0u, 0u);
-        const GuideView gv = guiding ? guide_view(s->guide) : GuideView{};
+        const GuideView gv = (guiding && !train_only) ? guide_view(s->guide) : GuideView{};
         hipLaunchKernelGGL(generate_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, cp, fp, r->eps, paths,
                            r->rays[0], r->path_of[0], gv.recs);
         NrcView nv{};
-        uint32_t n_offset = 0, n_train = 0;
         if (nrc) {
-            const uint64_t h = nrc_frame_hash(prm->seed, fp.frame0);
-            if (n_train_max > 0) {
-                n_offset = static_cast<uint32_t>(h % n_stride);
-                n_train = std::min(n_train_max, (static_cast<uint32_t>(n_pix) - n_offset + n_stride - 1) / n_stride);
-            }
-            hipLaunchKernelGGL(nrc_init_kernel, grid_of(static_cast<size_t>(n_pix)), dim3(kBlock), 0, stream, n_pix,
-                               n_stride, n_offset, n_train, np->unbiased_every, r->nrc_paths, r->nrc_counters);
+            hipLaunchKernelGGL(nrc_init_kernel, grid_of(static_cast<size_t>(total)), dim3(kBlock), 0, stream,
+                               static_cast<int32_t>(total), n_stride, n_offset, n_train, np->unbiased_every,
+                               train_only ? 1u : 0u, r->nrc_paths, r->nrc_counters);
             nv = NrcView{r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_verts, r->nrc_counters, np->spread};
         }
+        auto* rc_ptr = train_only ? nullptr : reinterpret_cast<unsigned long long*>(fb->ray_count);
+        int32_t* prim = train_only ? nullptr : fb->primary_id;
         int in_q = 0;
         for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
             const int out_q = 1 - in_q;
@@ -1336,29 +1350,28 @@ int run_frames(const RunCtx& c) {
             if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
             ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                            prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
-            auto* rc_ptr = reinterpret_cast<unsigned long long*>(fb->ray_count);
             if (guiding) {
                 if constexpr (NL == 1)  // guiding is rejected with several wavelengths before this point
                     hipLaunchKernelGGL((shade_kernel<1, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
                                        r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
                                        paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
-                                       r->path_of[out_q], r->shadow_rays, pays, r->counters, fb->primary_id,
-                                       last_frame, rc_ptr, gv, nv);
+                                       r->path_of[out_q], r->shadow_rays, pays, r->counters, prim, last_frame,
+                                       rc_ptr, gv, nv);
             } else if (nrc) {
                 hipLaunchKernelGGL((shade_kernel<NL, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
                                    r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, paths,
                                    r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
-                                   r->shadow_rays, pays, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
+                                   r->shadow_rays, pays, r->counters, prim, last_frame, rc_ptr, gv, nv);
             } else {
                 hipLaunchKernelGGL((shade_kernel<NL, false, false>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
                                    r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, paths,
                                    r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
-                                   r->shadow_rays, pays, r->counters, fb->primary_id, last_frame, rc_ptr, gv, nv);
+                                   r->shadow_rays, pays, r->counters, prim, last_frame, rc_ptr, gv, nv);
             }
             e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
             hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->counters, pays,
-                               r->shadow_hits, paths, reinterpret_cast<unsigned long long*>(fb->ray_count));
+                               r->shadow_hits, paths, rc_ptr);
             in_q = out_q;
             // Deep limits: stop once every path has ended.
             if (prm->max_depth > 32 && (depth % 16) == 15) {
@@ -1369,16 +1382,18 @@ int run_frames(const RunCtx& c) {
                 if (live == 0) break;
             }
         }
+        int rc2 = AP_OK;
         if (nrc) {
-            // Cache vertices, then the training samples of the frame.
-            if ((rc = nrc_infer_dyn(ns, r->nrc_queries, r->nrc_qpath, &r->nrc_counters[0], total, r->nrc_qrgb,
-                                    stream)) != AP_OK)
-                return rc;
+            // Cache vertices, then the training samples of the batch.
+            if ((rc2 = nrc_infer_dyn(ns, r->nrc_queries, r->nrc_qpath, &r->nrc_counters[0], total, r->nrc_qrgb,
+                                     stream)) != AP_OK)
+                return rc2;
             hipLaunchKernelGGL(nrc_apply_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->nrc_counters,
                                r->nrc_qpath, r->nrc_qrgb, rgb_lut, lam_lut, tables, paths);
             if (n_train > 0) {
                 hipLaunchKernelGGL(nrc_targets_kernel<NL>, grid_of(n_train), dim3(kBlock), 0, stream, n_train,
-                                   n_stride, n_offset, r->nrc_paths, paths, tables, r->nrc_verts, r->nrc_vmask);
+                                   n_stride, n_offset, train_only ? 1u : 0u, r->nrc_paths, paths, tables,
+                                   r->nrc_verts, r->nrc_vmask);
                 hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, n_train, r->nrc_vmask,
                                    r->nrc_offsets, r->nrc_counters);
                 hipLaunchKernelGGL(nrc_compact_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
@@ -1387,15 +1402,51 @@ int run_frames(const RunCtx& c) {
             r->nrc_last_train = n_train;
             r->nrc_rendered = true;
         }
-        hipLaunchKernelGGL(accumulate_kernel<NL>, grid_of(n_pix), dim3(kBlock), 0, stream, fp, paths, tables, fb->hdr,
-                           fb->odd, fb->even, fb->sum_xyz, gv.recs, guiding ? guide_slots(s->guide) : nullptr);
+        if (!train_only)
+            hipLaunchKernelGGL(accumulate_kernel<NL>, grid_of(fp.n_pixels), dim3(kBlock), 0, stream, fp, paths, tables,
+                               fb->hdr, fb->odd, fb->even, fb->sum_xyz, gv.recs,
+                               guiding ? guide_slots(s->guide) : nullptr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
-        if (guiding && (rc = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK) return rc;
+        if (guiding && !train_only && (rc2 = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK)
+            return rc2;
         if (nrc && n_train > 0 && np->train_steps > 0) {
-            rc = nrc_train_pool(ns, r->nrc_pool, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
-                                nrc_frame_hash(prm->seed, fp.frame0) ^ 0x747261696eull, stream);
-            if (rc != AP_OK) return rc;
+            rc2 = nrc_train_pool(ns, r->nrc_pool, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
+                                 train_seed, stream);
+            if (rc2 != AP_OK) return rc2;
+        }
+        return AP_OK;
+    };
+    // Training pixels of a frame (or pass) from its hash: offset, count.
+    auto train_pixels = [&](uint64_t h, uint32_t& n_offset, uint32_t& n_train) {
+        n_offset = 0;
+        n_train = 0;
+        if (n_train_max > 0) {
+            n_offset = static_cast<uint32_t>(h % n_stride);
+            n_train = std::min(n_train_max, (static_cast<uint32_t>(n_pix) - n_offset + n_stride - 1) / n_stride);
+        }
+    };
+    for (int32_t f0 = 0; f0 < prm->spp; f0 += per_batch) {
+        const int32_t nf = std::min(per_batch, prm->spp - f0);
+        const FrameParams fp{prm->frame_index + f0, nf, n_pix, inside, prm->seed, 0u, 1u};
+        uint32_t n_offset = 0, n_train = 0;
+        uint64_t h = 0;
+        if (nrc) {
+            h = nrc_frame_hash(prm->seed, fp.frame0);
+            train_pixels(h, n_offset, n_train);
+        }
+        if ((rc = run_batch(fp, false, n_offset, n_train, h ^ 0x747261696eull)) != AP_OK) return rc;
+        // Training-only passes of the cache: the training paths alone, on
+        // their own pixels and random streams.
+        const uint32_t passes = (nrc && n_train_max > 0) ? np->train_passes : 0u;
+        for (uint32_t k = 0; k < passes; ++k) {
+            const uint64_t hk = splitmix64(h ^ (0x70617373ull + k));
+            uint32_t o2 = 0, t2 = 0;
+            train_pixels(hk, o2, t2);
+            if (t2 == 0) continue;
+            const FrameParams tp{fp.frame0, 1, static_cast<int32_t>(t2), inside, splitmix64(prm->seed ^ hk), o2,
+                                 n_stride};
+            if ((rc = run_batch(tp, true, o2, t2, hk ^ 0x747261696eull)) != AP_OK) return rc;
         }
     }
     return AP_OK;
@@ -1476,7 +1527,7 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     cp.up = normalize(cross(cp.dir, cp.right));
     cp.width = cam->width;
     cp.height = cam->height;
-    const int32_t inside = camera_inside(s, cp.origin) ? 1 : 0;
+    const int32_t inside = camera_inside(s, r->analytic_objs, cp.origin) ? 1 : 0;
     const int32_t last_frame = prm->frame_index + prm->spp - 1;
 
     RunCtx c{s, r, prm, fb, stream, np, ns, rgb_lut, lam_lut, tables, n_pix, per_batch, guiding, nrc, n_stride,
@@ -1545,6 +1596,7 @@ void ap_nrc_render_params_default(ap_nrc_render_params* p) {
     p->batch = 16384;
     p->learning_rate = 0.005f;
     p->unbiased_every = 16;
+    p->train_passes = 0;
 }
 
 int ap_render_nrc(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, ap_nrc* cache,

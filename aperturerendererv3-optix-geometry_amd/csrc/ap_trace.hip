@@ -80,8 +80,11 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) {
 }
 
 // Chunked traversal. Wave w owns the contiguous ray range
-// [w*chunk, (w+1)*chunk) of `wave_rays` rays (waves on one XCD get adjacent
-// ranges, so coherent rays share that XCD's L2). Each lane traces one ray at
+// [w*chunk, (w+1)*chunk) (waves on one XCD get adjacent ranges, so coherent
+// rays share that XCD's L2). chunk is `wave_rays`, or less when the actual
+// ray count (a queue's device-side count, or a small batch) would leave most
+// launched waves idle: then the rays are spread over every launched wave in
+// multiples of 64. Each lane traces one ray at
 // a time; once at least `refill` lanes are idle the wave hands them the next
 // rays of its range in one ballot, so a wave no longer waits for its slowest
 // ray, and the launch tail is bounded by one chunk. A step is one pop of the
@@ -106,7 +109,11 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
     const uint32_t n = dyn_n ? min(*dyn_n, n_cap) : n_cap;
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t wave_rank = (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
-    const uint32_t chunk = static_cast<uint32_t>(s.wave_rays);
+    uint32_t chunk = static_cast<uint32_t>(s.wave_rays);
+    {
+        const uint32_t share = ((n + nb - 1) / nb + WAVE - 1) / WAVE * WAVE;  // wave-uniform
+        chunk = min(chunk, max(share, static_cast<uint32_t>(WAVE)));
+    }
     if (static_cast<uint64_t>(wave_rank) * chunk >= n) return;
     uint32_t next = wave_rank * chunk;   // wave-uniform
     const uint32_t end = min(n, next + chunk);
@@ -264,10 +271,13 @@ hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, uint32_t n, cons
                       ap_trace_stats* stats, hipStream_t stream) {
     const dim3 block(WAVE);
     const int cap = s.stack_cap;
+    // Enough waves for the chip even when a small batch asks for few chunks:
+    // up to kMinWaves waves of at least 64 rays each.
+    constexpr uint32_t kMinWaves = 4096;
 #define AP_CAP_CASE(C)                                                                                  \
     if (cap <= C) {                                                                                     \
         const uint32_t chunk = static_cast<uint32_t>(s.wave_rays);                                      \
-        const uint32_t waves = (n + chunk - 1) / chunk;                                                 \
+        const uint32_t waves = std::max((n + chunk - 1) / chunk, std::min((n + WAVE - 1) / WAVE, kMinWaves)); \
         hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C>), dim3(waves), block, 0, stream, s, rays, n, \
                            dyn_n, hits, stats);                                                         \
         return hipGetLastError();                                                                       \

@@ -99,11 +99,13 @@ def rel_mse(xyz, ref_xyz):
     return float((((a - b) ** 2) / (b ** 2 + 1e-2)).mean())
 
 
-def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup):
+def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_passes):
     """Equal-time relMSE of the radiance-cache renderer against the plain
-    wavefront tracer. The plain tracer renders plain_spp frames (timed with
-    HIP events); the cache renderer starts from freshly initialised weights
-    and renders one frame per call, training after each, for the same GPU
+    wavefront tracer, both with four hero wavelengths per path. The plain
+    tracer renders plain_spp frames (timed with HIP events); the cache
+    renderer starts from freshly initialised weights and renders frame after
+    frame, training after each (and on train_passes training-only passes),
+    for as many frames as its calibrated per-frame time fits in the same GPU
     time: its first `warmup` frames go to a scratch image, the rest are
     averaged. Both images are compared with a plain render of ref_spp frames
     on another seed."""
@@ -112,7 +114,7 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup):
     ref = ap.DeviceBuffer(n * 12)
     ref.zero(stream)
     t0 = time.perf_counter()
-    scene.render(cam, ap.RenderParams.defaults(spp=ref_spp, seed=1000, **params), ref, stream=stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=ref_spp, seed=1000, wavelengths=4, **params), ref, stream=stream)
     stream.synchronize()
     log(f"nrc: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
     ref_img = ref.to_numpy(np.float32, n * 3)
@@ -120,28 +122,46 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup):
     img.zero(stream)
     e0, e1 = ap.Event(), ap.Event()
     e0.record(stream)
-    scene.render(cam, ap.RenderParams.defaults(spp=plain_spp, seed=5, **params), img, stream=stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=plain_spp, seed=5, wavelengths=4, **params), img, stream=stream)
     e1.record(stream)
     stream.synchronize()
     budget = e0.elapsed_ms(e1)
     plain = rel_mse(img.to_numpy(np.float32, n * 3), ref_img)
     cache = ap.RadianceCache(seed=1)
-    nprm = ap.NrcRenderParams.defaults()
+    nprm = ap.NrcRenderParams.defaults(train_paths=65536, train_passes=train_passes)
     scratch = ap.DeviceBuffer(n * 12)
     scratch.zero(stream)
+    w0 = cache.params()
+
+    def nrc_frames(frames, buf, seed):
+        if frames > 0:
+            scene.render_nrc(cam, ap.RenderParams.defaults(spp=frames, seed=seed, wavelengths=4, **params), cache,
+                             nprm, buf, stream=stream)
+
+    # Calibration (untimed; it also allocates the path, query and training
+    # buffers): GPU time per frame of the cache renderer, frames submitted
+    # back to back as in the timed run.
+    nrc_frames(2, scratch, 3)
+    stream.synchronize()
+    e0.record(stream)
+    nrc_frames(8, scratch, 4)
+    e1.record(stream)
+    stream.synchronize()
+    per_frame = e0.elapsed_ms(e1) / 8
+    total = max(warmup + 1, int(budget / per_frame))
+    # Timed run from the initial weights with a fresh optimiser: `warmup`
+    # frames into a scratch image, the rest averaged into the compared image.
+    cache.set_params(w0)
+    scratch.zero(stream)
     img.zero(stream)
-    used, frames = 0.0, 0
-    while used < budget:
-        warm = frames < warmup
-        e0.record(stream)
-        f = frames if warm else frames - warmup
-        scene.render_nrc(cam, ap.RenderParams.defaults(spp=1, frame_index=f, seed=5 + (77 if warm else 0),
-                                                       **params), cache, nprm, scratch if warm else img,
-                         stream=stream)
-        e1.record(stream)
-        stream.synchronize()
-        used += e0.elapsed_ms(e1)
-        frames += 1
+    stream.synchronize()
+    e0.record(stream)
+    nrc_frames(warmup, scratch, 82)
+    nrc_frames(total - warmup, img, 5)
+    e1.record(stream)
+    stream.synchronize()
+    used = e0.elapsed_ms(e1)
+    frames = total
     st, _ = scene.nrc_last_frame(samples=False)
     nrc = rel_mse(img.to_numpy(np.float32, n * 3), ref_img) if frames > warmup else float("nan")
     cache.close()
@@ -151,7 +171,9 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup):
         "plain_relmse": round(plain, 5),
         "nrc_frames": frames - warmup,
         "nrc_warmup_frames": warmup,
+        "nrc_training_passes_per_frame": train_passes,
         "nrc_ms": round(used, 1),
+        "nrc_ms_per_frame_calibrated": round(per_frame, 3),
         "nrc_relmse": round(nrc, 5),
         "relmse_ratio_plain_over_nrc": round(plain / nrc, 3) if nrc > 0 else None,
         "reference_spp": ref_spp,
@@ -159,20 +181,24 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup):
         "cache_queries_per_frame": st.queries,
         "training_paths_per_frame": st.train_paths,
         "training_samples_per_frame": st.samples,
-        "cache": "64-64x5-3 ReLU, BF16 MFMA / FP32 accumulate, relative L2, Adam lr 0.005, 4 steps x 16384 per frame",
+        "cache": "64-64x5-3 ReLU, BF16 MFMA / FP32 accumulate, relative L2, Adam lr 0.005, 4 steps x 16384 per "
+                 "frame from 65,536 training paths",
+        "spectral": "four hero wavelengths per path in the reference, plain and cache images",
     }
     log(f"nrc: {out}")
     return out
 
 
-def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=1024, nrc_warmup=4):
+def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc_warmup=4, nrc_passes=0):
     """Samples/s of the wavefront path tracer at 1920x1080 on the heightfield
     workload (1024^2-cell fBm heightfield, 2,097,152 triangles, plus 256
-    320-triangle icospheres and 4 thin-cube lights; Lambert/GGX 50/50), max
-    depth 16, Russian roulette from depth 4, sky on. Measured twice: plain,
-    and with directional guiding (probes rebuilt after every frame, the
-    rebuild inside the timed region). Frames are timed with HIP events on the
-    render stream after one untimed frame."""
+    320-triangle icospheres; Lambert/GGX 50/50; lit by 4 thin-cube lights, no
+    sky), max depth 16, Russian roulette from depth 4. Measured with one
+    wavelength per path (the reference's estimator), with four hero
+    wavelengths per path, and with directional guiding (one wavelength;
+    probes rebuilt after every frame, the rebuild inside the timed region).
+    Frames are timed with HIP events on the render stream after one untimed
+    frame."""
     t0 = time.perf_counter()
     objs = ap.synth_heightfield_scene(1024, 256, 4, 20.0)
     scene = ap.SceneRenderer(objs)
@@ -183,47 +209,47 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=1024, nrc
     hdr = ap.DeviceBuffer(W * H * 12)
     hdr.zero(stream)
     counts = ap.DeviceBuffer(16)
-    params = dict(max_depth=16, rr_min_depth=4)
+    params = dict(max_depth=16, rr_min_depth=4, sky=0)
     scene.render(cam, ap.RenderParams.defaults(spp=1, **params), hdr, stream=stream)
     stream.synchronize()
-    def timed(guiding):
+
+    def timed(**kw):
         counts.zero(stream)
         e0, e1 = ap.Event(), ap.Event()
         t0 = time.perf_counter()
         e0.record(stream)
-        scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, use_guiding=guiding, **params), hdr,
-                     stream=stream, ray_count=counts)
+        scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, **params, **kw), hdr, stream=stream,
+                     ray_count=counts)
         e1.record(stream)
         stream.synchronize()
         wall = (time.perf_counter() - t0) * 1e3
         ms = e0.elapsed_ms(e1)
         rays = counts.to_numpy(np.uint64, 2)
-        return ms, wall, rays
+        return {
+            "ms_per_spp": round(ms / spp, 3),
+            "wall_ms_per_spp": round(wall / spp, 3),
+            "samples_per_s": round(W * H * spp / (ms / 1e3), 1),
+            "mrays_per_s": round(float(rays[0] + rays[1]) / (ms / 1e3) / 1e6, 1),
+            "rays_per_sample": round(float(rays[0] + rays[1]) / (W * H * spp), 3),
+        }
 
-    ms, _, rays = timed(0)
+    one = timed(wavelengths=1)
+    four = timed(wavelengths=4)
     scene.render(cam, ap.RenderParams.defaults(spp=1, use_guiding=1, **params), hdr, stream=stream)
     stream.synchronize()
-    gms, gwall, grays = timed(1)
-    probes = len(scene.guiding_export()[1])
+    guided = timed(use_guiding=1)
+    guided["probes"] = len(scene.guiding_export()[1])
     out = {
-        "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 lights, 1920x1080, "
-                    "max depth 16, RR from depth 4, one wavelength per path, sky on",
+        "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 thin-cube lights, "
+                    "no sky, 1920x1080, max depth 16, RR from depth 4",
         "spp": spp,
-        "ms_per_spp": round(ms / spp, 3),
-        "samples_per_s": round(W * H * spp / (ms / 1e3), 1),
-        "mrays_per_s": round(float(rays[0] + rays[1]) / (ms / 1e3) / 1e6, 1),
-        "rays_per_sample": round(float(rays[0] + rays[1]) / (W * H * spp), 3),
-        "guided": {
-            "ms_per_spp": round(gms / spp, 3),
-            "wall_ms_per_spp": round(gwall / spp, 3),
-            "samples_per_s": round(W * H * spp / (gms / 1e3), 1),
-            "mrays_per_s": round(float(grays[0] + grays[1]) / (gms / 1e3) / 1e6, 1),
-            "probes": probes,
-        },
+        "one_wavelength": one,
+        "four_wavelengths": four,
+        "guided_one_wavelength": guided,
     }
     log(f"render: {out}")
     if nrc_spp > 0:
-        out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, nrc_ref_spp, nrc_warmup)
+        out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes)
     if cpu_seconds > 0:
         sys.path.insert(0, os.path.join(REPO, "tests"))
         import oracle_lib as ol
@@ -258,8 +284,9 @@ def main():
     ap_.add_argument("--render-spp", type=int, default=8, help="spp of the 1080p render measurement (0 skips it)")
     ap_.add_argument("--nrc-spp", type=int, default=64,
                      help="plain spp that sets the equal-time budget of the radiance-cache comparison (0 skips it)")
-    ap_.add_argument("--nrc-ref-spp", type=int, default=1024, help="spp of the plain reference image")
-    ap_.add_argument("--nrc-warmup", type=int, default=4, help="cache frames rendered before the averaged ones")
+    ap_.add_argument("--nrc-ref-spp", type=int, default=2048, help="spp of the plain reference image")
+    ap_.add_argument("--nrc-warmup", type=int, default=0, help="cache frames rendered before the averaged ones")
+    ap_.add_argument("--nrc-passes", type=int, default=0, help="training-only passes of the cache after each frame")
     ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap_.add_argument("--sweep", action="store_true",
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
@@ -435,7 +462,7 @@ def main():
     if args.render_spp > 0:
         line["render"] = render_bench(ap, args.render_spp, 0.0 if args.no_cpu else 10.0, threads,
                                       nrc_spp=args.nrc_spp, nrc_ref_spp=args.nrc_ref_spp,
-                                      nrc_warmup=args.nrc_warmup)
+                                      nrc_warmup=args.nrc_warmup, nrc_passes=args.nrc_passes)
     print(json.dumps(line), flush=True)
 
 

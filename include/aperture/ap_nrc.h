@@ -90,6 +90,9 @@ typedef struct ap_nrc_render_params {
     uint32_t batch;          /* samples per step, a positive multiple of 256 (16384) */
     float learning_rate;     /* Adam step size (0.005) */
     uint32_t unbiased_every; /* every k-th training path never ends into the cache (16); 0: none */
+    uint32_t train_passes;   /* training-only passes after each frame (0): the training paths of
+                              * another pixel offset and random stream, traced and trained on,
+                              * nothing added to the framebuffers */
 } ap_nrc_render_params;
 
 typedef struct ap_nrc_frame_stats {
@@ -109,7 +112,8 @@ void ap_nrc_render_params_default(ap_nrc_render_params* p);
 int ap_render_nrc(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, ap_nrc* cache,
                   const ap_nrc_render_params* nrc_params, const ap_framebuffers* fb, void* stream);
 
-/* The last ap_render_nrc frame of the scene: counters and, if host_samples
+/* The last batch of paths of ap_render_nrc (its last frame, or the last
+ * training-only pass after it): counters and, if host_samples
  * is not NULL, min(samples, capacity) training samples (20 floats; the last
  * float holds pixel * 8 + vertex as uint32 bits) in pixel, vertex order.
  * Synchronous. */

@@ -122,3 +122,26 @@ def test_frozen_cache_four_wavelengths_matches_oracle(gpu):
     cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 40, 40)
     prm = ap.RenderParams.defaults(spp=2, max_depth=12, seed=8, wavelengths=4)
     frozen_parity(objs, cam, prm, "box, four wavelengths per path")
+
+
+def test_training_passes_touch_only_the_cache(gpu):
+    """Training-only passes trace and train but add nothing to the image:
+    with the cache frozen they leave the image bit for bit, and with
+    training on they move the weights further than the frames alone."""
+    objs = ap.synth_heightfield_scene(32, 3, 5, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 64, 48)
+    prm = ap.RenderParams.defaults(spp=3, max_depth=16, rr_min_depth=4, seed=4, wavelengths=4)
+    images, moves = [], []
+    for passes, steps in ((0, 0), (3, 0), (0, 2), (3, 2)):
+        scene = ap.SceneRenderer(objs)
+        cache = ap.RadianceCache(seed=2)
+        w0 = cache.params()
+        hdr, _ = render_nrc(scene, cam, prm, cache, ap.NrcRenderParams.defaults(
+            train_paths=512, batch=512, train_steps=steps, train_passes=passes))
+        images.append(hdr)
+        moves.append(float(np.abs(cache.params() - w0).sum()))
+        scene.close()
+        cache.close()
+    print(f"[nrc-render] training passes: weight change {moves}", flush=True)
+    assert np.array_equal(images[0], images[1])
+    assert moves[0] == 0.0 and moves[1] == 0.0 and moves[3] > moves[2] > 0.0
