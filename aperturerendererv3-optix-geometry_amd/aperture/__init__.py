@@ -172,7 +172,9 @@ EXPORTED_SYMBOLS = (
     "ap_synth_heightfield_scene", "ap_guiding_reset", "ap_guiding_export",
     "ap_nrc_create", "ap_nrc_destroy", "ap_nrc_set_bounds", "ap_nrc_get_params", "ap_nrc_set_params",
     "ap_nrc_infer", "ap_nrc_train_step", "ap_nrc_gradient", "ap_nrc_last_gradient", "ap_nrc_get_bounds",
-    "ap_nrc_set_ema", "ap_nrc_render_params_default", "ap_render_nrc", "ap_nrc_last_frame",
+    "ap_nrc_set_ema", "ap_nrc_render_params_default",
+    "ap_batch_create", "ap_batch_destroy", "ap_batch_render", "ap_batch_set_launch", "ap_batch_get_launch",
+    "ap_batch_set_graph", "ap_batch_autotune", "ap_render_nrc", "ap_nrc_last_frame",
 )
 
 
@@ -204,6 +206,13 @@ def lib():
         L.ap_nrc_last_gradient.argtypes = [vp, vp]
         L.ap_nrc_get_bounds.argtypes = [vp, vp, vp]
         L.ap_nrc_set_ema.argtypes = [vp, ctypes.c_float]
+        L.ap_batch_create.argtypes = [vp, vp, sz, vp]
+        L.ap_batch_destroy.argtypes = [vp]
+        L.ap_batch_render.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.ap_batch_set_launch.argtypes = [vp, i32, i32]
+        L.ap_batch_get_launch.argtypes = [vp, vp, vp]
+        L.ap_batch_set_graph.argtypes = [vp, i32]
+        L.ap_batch_autotune.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ap_render_nrc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.ap_nrc_last_frame.argtypes = [vp, vp, vp, sz]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
@@ -547,6 +556,60 @@ class RadianceCache:
     def close(self):
         if self._h is not None and self._h.value:
             lib().ap_nrc_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched multi-scene rendering (include/aperture/ap_batch.h) over
+    SceneRenderer objects, which must outlive the batch."""
+
+    def __init__(self, scenes, cameras):
+        self._scenes = list(scenes)
+        self._cams = (Camera * len(cameras))(*cameras)
+        arr = (ctypes.c_void_p * len(self._scenes))(*[sc.handle for sc in self._scenes])
+        self._h = ctypes.c_void_p()
+        check(lib().ap_batch_create(arr, self._cams, len(self._scenes), ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def render(self, params, hdrs, sums=None, stream=None):
+        """Adds params.spp frames of every scene; hdrs[k] / sums[k] are device
+        buffers (DeviceBuffer or address, None to skip). Returns the rays traced."""
+        n = len(self._scenes)
+        h = (ctypes.c_void_p * n)(*[_ptr(x) if x is not None else None for x in hdrs])
+        sm = None if sums is None else (ctypes.c_void_p * n)(*[_ptr(x) if x is not None else None for x in sums])
+        rays = (ctypes.c_uint64 * 2)()
+        check(lib().ap_batch_render(self.handle, ctypes.byref(params), h, sm, rays, _stream(stream)))
+        return int(rays[0]), int(rays[1])
+
+    def set_launch(self, wave_rays, refill):
+        check(lib().ap_batch_set_launch(self.handle, wave_rays, refill))
+
+    def get_launch(self):
+        w, r = ctypes.c_int(), ctypes.c_int()
+        check(lib().ap_batch_get_launch(self.handle, ctypes.byref(w), ctypes.byref(r)))
+        return w.value, r.value
+
+    def set_graph(self, enable):
+        check(lib().ap_batch_set_graph(self.handle, 1 if enable else 0))
+
+    def autotune(self, params, frames=2, stream=None):
+        w, r, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        check(lib().ap_batch_autotune(self.handle, ctypes.byref(params), frames, ctypes.byref(w), ctypes.byref(r),
+                                      ctypes.byref(ms), _stream(stream)))
+        return w.value, r.value, ms.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().ap_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -131,6 +131,12 @@ int spectrum_get(const f4** rgb_lut, const f4** lambda_lut, const float** tables
 hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
                         ap_trace_stats* stats, bool any_hit, hipStream_t stream,
                         const uint32_t* dyn_n = nullptr);
+// Rays of several scenes in one queue: ray j traverses scenes[ray_scene[j]]
+// (a device array); `shape` gives the launch shape, stack capacity (the
+// largest of the scenes') and traversal mode.
+hipError_t launch_trace_multi(const DeviceScene& shape, const DeviceScene* scenes, const uint16_t* ray_scene,
+                              const ap_ray* rays, size_t n_total, ap_hit* hits, bool any_hit, hipStream_t stream,
+                              const uint32_t* dyn_n);
 
 // Radiance cache entry points for the integrator (ap_nrc.hip).
 struct NrcState;

@@ -26,6 +26,7 @@
 #include <tuple>
 #include <vector>
 
+#include "aperture/ap_batch.h"
 #include "aperture/ap_nrc.h"
 #include "aperture/ap_render.h"
 #include "ap_internal.h"
@@ -108,6 +109,34 @@ struct NrcView {
     float* verts;        // per training slot, kNrcV records of kNrcVert floats
     uint32_t* counters;  // [0] cache queries, [1] training samples
     float spread;
+};
+
+// One scene of a batch (ap_batch_render): its traversal descriptor, shading
+// records, camera, the first path of its pixels within a batch frame, and
+// the caller's framebuffers of the current call.
+struct alignas(16) BatchScene {
+    const GeoRec* geo;
+    const MatRec* mats;
+    const LightRec* lights;
+    const AnalyticRec* analytic;
+    float* hdr;
+    double* sums;
+    int32_t n_lights;
+    int32_t inside;
+    uint32_t pix_begin;
+    uint32_t n_pix;
+    float cam[14];     // origin, dir, right, up (f3 each), tan_x, tan_y
+    int32_t width, height;
+};
+
+// What the batch kernels see of a batch.
+struct BatchView {
+    const BatchScene* scenes;
+    const uint32_t* pix_begin;   // n_scenes + 1 prefix of pixel counts
+    int32_t n_scenes;
+    uint16_t* ray_scene[2];      // scene of each queued extension ray
+    uint16_t* shadow_scene;      // scene of each queued shadow ray
+    int32_t* frame;              // frame index of the next batch frame (advanced by the graph)
 };
 
 struct RenderState {
@@ -522,7 +551,7 @@ __device__ __forceinline__ void nrc_record(float* __restrict__ q, f3 pos, f3 d, 
     q4[2] = float4{n.z, M.roughness, M.albedo_rgb[0], M.albedo_rgb[1]};
 }
 
-template <int NL, bool GUIDE, bool NRC>
+template <int NL, bool GUIDE, bool NRC, bool MULTI = false>
 __global__ __launch_bounds__(kBlock) void shade_kernel(
     ShadeParams sp, const GeoRec* __restrict__ geo, const MatRec* __restrict__ mats,
     const LightRec* __restrict__ lights, const AnalyticRec* __restrict__ analytic, const f4* __restrict__ rgb_lut,
@@ -531,8 +560,9 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     const ap_hit* __restrict__ hits, ap_ray* __restrict__ rays_out, int32_t* __restrict__ path_out,
     ap_ray* __restrict__ shadow_rays, ShadowPay<NL>* __restrict__ shadow_pay, uint32_t* __restrict__ counters,
     int32_t* __restrict__ primary_id, int32_t primary_frame, unsigned long long* __restrict__ ray_count, GuideView gv,
-    NrcView nv) {
+    NrcView nv, BatchView bv = BatchView{}) {
     static_assert(!GUIDE || NL == 1, "guiding runs with one wavelength per path");
+    static_assert(!MULTI || (!GUIDE && !NRC), "a batch renders without guiding or the cache");
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
     if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
@@ -545,6 +575,18 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     const f3 o{R.ox, R.oy, R.oz}, d{R.dx, R.dy, R.dz};
     const Spec<NL> sc = spec_of<NL>(S.lam, lam_lut, tables);
     const f4 lb = sc.lb[0];    // the hero's basis: every sampling decision uses it
+    // The path's scene: the batch's record of it, or the kernel arguments.
+    uint16_t scene_id = 0;
+    int32_t n_lights = sp.n_lights;
+    if constexpr (MULTI) {
+        scene_id = static_cast<uint16_t>(S.flags >> 16);
+        const BatchScene& B = bv.scenes[scene_id];
+        geo = B.geo;
+        mats = B.mats;
+        lights = B.lights;
+        analytic = B.analytic;
+        n_lights = B.n_lights;
+    }
     const bool inside = (S.flags & PATH_INSIDE) != 0u;
     if (sp.depth == 0 && primary_id && S.frame == primary_frame) primary_id[S.pixel] = H.id;
 
@@ -730,7 +772,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             const float mm = mean_mask<NL>(S.mask);
             const float rate = sp.nee_always ? 1.0f : sqrtf(saturatef(mm));
             if (sp.use_nee && S.ndd < sp.nee_max_depth && !inside && rand1(rng) < rate) {
-                const LightSample ls = sample_light(lights, sp.n_lights, hp, rng);
+                const LightSample ls = sample_light(lights, n_lights, hp, rng);
                 if (ls.valid) {
                     const float pat = fmaxf(1e-20f, ls.p_area * ls.p_facet);
                     const f3 lu = ls.point - hp;
@@ -822,11 +864,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     if (emit_next) {
         rays_out[qi] = next;
         path_out[qi] = pid;
+        if constexpr (MULTI) bv.ray_scene[sp.out_q][qi] = scene_id;
     }
     const uint32_t si = wave_append(&counters[2], emit_shadow);
     if (emit_shadow) {
         shadow_rays[si] = shadow;
         shadow_pay[si] = pay;
+        if constexpr (MULTI) bv.shadow_scene[si] = scene_id;
     }
     if (NRC) {
         const uint32_t qi = wave_append(&nv.counters[0], emit_query);
@@ -1035,6 +1079,101 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
         sums[3 * p + 1] += s1;
         sums[3 * p + 2] += s2;
     }
+}
+
+// Scene of batch path i: the last scene whose first pixel is <= i.
+__device__ __forceinline__ int32_t batch_scene_of(const uint32_t* __restrict__ pix_begin, int32_t n, uint32_t i) {
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (pix_begin[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// generate_kernel over the pixels of every scene of a batch: path i is pixel
+// i - pix_begin of scene s, with that scene's camera; the frame index comes
+// from the batch's device counter (a captured graph replays the launch).
+template <int NL>
+__global__ __launch_bounds__(kBlock) void generate_batch_kernel(BatchView bv, uint32_t total, uint64_t seed,
+                                                                const float* __restrict__ eps,
+                                                                PathT<NL>* __restrict__ paths, ap_ray* __restrict__ rays,
+                                                                int32_t* __restrict__ path_of) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
+    const BatchScene& B = bv.scenes[sc];
+    const int32_t p = static_cast<int32_t>(i - B.pix_begin);
+    const int32_t px = p % B.width, py = p / B.width;
+    const int32_t frame = *bv.frame;
+    const f3 origin{B.cam[0], B.cam[1], B.cam[2]}, cdir{B.cam[3], B.cam[4], B.cam[5]};
+    const f3 right{B.cam[6], B.cam[7], B.cam[8]}, up{B.cam[9], B.cam[10], B.cam[11]};
+    Pcg32 rng = pcg32_path(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), seed);
+    // RenderCamera (scene.cu:214-230), as generate_kernel.
+    const float u = (static_cast<float>(px) + rand1(rng)) / static_cast<float>(B.width);
+    const float v = (static_cast<float>(py) + rand1(rng)) / static_cast<float>(B.height);
+    const float nx = u * 2.0f - 1.0f;
+    const float ny = static_cast<float>((1.0 - static_cast<double>(v)) * 2.0f - 1.0f);
+    f3 d = normalize(cdir + right * (nx * B.cam[12]) + up * (ny * B.cam[13]));
+    float lam, lpdf;
+    {
+        const float r0 = rand1(rng), r1 = rand1(rng), r2 = rand1(rng);
+        sample_lambda(r0, r1, r2, lam, lpdf);
+    }
+    d = normalize(d);
+    PathT<NL> st;
+    hero_wavelengths<NL>(lam, lpdf, st.lam, st.psum);
+    for (int j = 0; j < NL; ++j) {
+        st.radiance[j] = 0.0f;
+        st.mask[j] = 1.0f;
+    }
+    st.last_pdf = 0.0f;
+    st.last_pos[0] = st.last_pos[1] = st.last_pos[2] = 0.0f;
+    st.ndd = 0;
+    st.flags = (B.inside ? PATH_INSIDE : 0u) | (static_cast<uint32_t>(sc) << 16);
+    st.pixel = p;
+    st.frame = frame;
+    st.rng_state = rng.state;
+    st.rng_inc = rng.inc;
+    paths[i] = st;
+    rays[i] = ap_ray{origin.x, origin.y, origin.z, eps[0], d.x, d.y, d.z, __int_as_float(0x7f800000)};
+    path_of[i] = static_cast<int32_t>(i);
+    bv.ray_scene[0][i] = static_cast<uint16_t>(sc);
+}
+
+// accumulate_kernel for one frame of a batch, into each scene's framebuffers
+// (skipped for a scene whose hdr is NULL).
+template <int NL>
+__global__ __launch_bounds__(kBlock) void accumulate_batch_kernel(BatchView bv, uint32_t total,
+                                                                  const PathT<NL>* __restrict__ paths,
+                                                                  const float* __restrict__ tables) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
+    const BatchScene& B = bv.scenes[sc];
+    if (!B.hdr) return;
+    const uint32_t p = i - B.pix_begin;
+    const PathT<NL>& S = paths[i];
+    float rad[NL];
+    for (int j = 0; j < NL; ++j) rad[j] = fix_nan_inf(S.radiance[j]);
+    const f3 pre = path_xyz<NL>(S.lam, rad, S.psum, tables);
+    f3 res{clampf(pre.x, 0.0f, 1000.0f), clampf(pre.y, 0.0f, 1000.0f), clampf(pre.z, 0.0f, 1000.0f)};
+    res = fmax3(f3{0.0f, 0.0f, 0.0f}, res);
+    const float w = 1.0f / static_cast<float>(1 + S.frame);
+    float* h = B.hdr + 3 * static_cast<size_t>(p);
+    const f3 acc = lerp3(f3{h[0], h[1], h[2]}, res, w);
+    h[0] = acc.x; h[1] = acc.y; h[2] = acc.z;
+    if (B.sums) {
+        double* q = B.sums + 3 * static_cast<size_t>(p);
+        q[0] += static_cast<double>(res.x);
+        q[1] += static_cast<double>(res.y);
+        q[2] += static_cast<double>(res.z);
+    }
+}
+
+__global__ void advance_frame_kernel(int32_t* frame) {
+    if (threadIdx.x == 0) *frame += 1;
 }
 
 __global__ void build_mats_kernel(const MatIn* __restrict__ in, int n, const f4* __restrict__ lut,
@@ -1535,9 +1674,209 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     return prm->wavelengths == 4 ? run_frames<4>(c) : run_frames<1>(c);
 }
 
+// ---------------------------------------------------------------------------
+// Batched multi-scene rendering (include/aperture/ap_batch.h).
+// ---------------------------------------------------------------------------
+struct BatchState {
+    std::vector<ap_scene*> scenes;
+    std::vector<ap_camera> cams;
+    uint32_t total = 0;                 // paths of one batch frame (pixels of all scenes)
+    std::vector<uint32_t> pix_begin;    // n + 1
+    std::vector<BatchScene> h_scenes;
+    std::vector<DeviceScene> h_devs;
+    BatchScene* d_scenes = nullptr;
+    DeviceScene* d_devs = nullptr;
+    uint32_t* d_pix_begin = nullptr;
+    DeviceScene shape{};                // launch shape, largest stack, traversal mode
+    uint8_t* paths = nullptr;
+    ap_ray* rays[2] = {nullptr, nullptr};
+    int32_t* path_of[2] = {nullptr, nullptr};
+    ap_hit* hits = nullptr;
+    ap_ray* shadow_rays = nullptr;
+    uint8_t* shadow_payload = nullptr;
+    ap_hit* shadow_hits = nullptr;
+    uint16_t* ray_scene[2] = {nullptr, nullptr};
+    uint16_t* shadow_scene = nullptr;
+    uint32_t* counters = nullptr;
+    int32_t* frame = nullptr;
+    unsigned long long* rays_traced = nullptr;
+    float* eps = nullptr;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool key_valid = false;
+    ap_render_params key{};
+    int32_t key_wave_rays = 0, key_refill = 0;
+    hipStream_t key_stream = nullptr;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key_valid = false;
+    }
+    ~BatchState() {
+        drop_graph();
+        void* ptrs[] = {d_scenes, d_devs, d_pix_begin, paths, rays[0], rays[1], path_of[0], path_of[1], hits,
+                        shadow_rays, shadow_payload, shadow_hits, ray_scene[0], ray_scene[1], shadow_scene, counters,
+                        frame, rays_traced, eps};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+    BatchView view() const {
+        return BatchView{d_scenes, d_pix_begin, static_cast<int32_t>(scenes.size()), {ray_scene[0], ray_scene[1]},
+                         shadow_scene, frame};
+    }
+};
+
+namespace {
+
+// Shading records of every scene for the call's triangle-material mode and
+// the batch records with this call's framebuffers, uploaded.
+int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const* sums, const f4* rgb_lut) {
+    int rc;
+    for (size_t k = 0; k < b->scenes.size(); ++k) {
+        ap_scene* s = b->scenes[k];
+        if (!s->render) s->render = new RenderState();
+        RenderState* r = s->render;
+        if (r->tri_mode != tri_mode && (rc = build_shading(s, r, tri_mode, rgb_lut)) != AP_OK) return rc;
+        BatchScene& B = b->h_scenes[k];
+        B.geo = r->geo;
+        B.mats = r->mats;
+        B.lights = r->lights;
+        B.analytic = s->dev.analytic;
+        B.hdr = hdr ? hdr[k] : nullptr;
+        B.sums = sums ? sums[k] : nullptr;
+        B.n_lights = r->n_lights;
+        const f3 o{b->cams[k].origin[0], b->cams[k].origin[1], b->cams[k].origin[2]};
+        B.inside = camera_inside(s, r->analytic_objs, o) ? 1 : 0;
+        b->h_devs[k] = s->dev;
+    }
+    hipError_t e = hipMemcpy(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_devs, b->h_devs.data(), b->h_devs.size() * sizeof(DeviceScene), hipMemcpyHostToDevice);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: upload scene records");
+}
+
+// One batch frame: every scene's paths through every bounce, then the
+// running means and the frame counter. Nothing here synchronises, so the
+// sequence can be captured into a graph.
+template <int NL>
+int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, const f4* rgb_lut,
+                const f4* lam_lut, const float* tables) {
+    const BatchView bv = b->view();
+    auto* paths = reinterpret_cast<PathT<NL>*>(b->paths);
+    auto* pays = reinterpret_cast<ShadowPay<NL>*>(b->shadow_payload);
+    const uint32_t total = b->total;
+    hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
+    hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
+                       b->eps, paths, b->rays[0], b->path_of[0]);
+    int in_q = 0;
+    for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
+        const int out_q = 1 - in_q;
+        hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, out_q);
+        hipError_t e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[in_q], b->rays[in_q], total, b->hits,
+                                          false, stream, &b->counters[in_q]);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
+        const ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
+                             prm->nee_always, prm->sky, in_q, out_q, 0};
+        hipLaunchKernelGGL((shade_kernel<NL, false, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
+                           nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
+                           b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
+                           b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, NrcView{}, bv);
+        e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
+                               stream, &b->counters[2]);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
+        hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
+                           b->shadow_hits, paths, b->rays_traced);
+        in_q = out_q;
+    }
+    hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
+                       tables);
+    hipLaunchKernelGGL(advance_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: launch");
+}
+
+bool same_key(const BatchState* b, const ap_render_params* p, hipStream_t st) {
+    const ap_render_params& k = b->key;
+    return b->key_valid && b->key_stream == st && k.seed == p->seed && k.max_depth == p->max_depth &&
+           k.rr_min_depth == p->rr_min_depth && k.nee_max_depth == p->nee_max_depth && k.use_nee == p->use_nee &&
+           k.nee_always == p->nee_always && k.sky == p->sky && k.wavelengths == p->wavelengths &&
+           b->key_wave_rays == b->shape.wave_rays && b->key_refill == b->shape.refill;
+}
+
+// spp batch frames from frame index frame0 (graph or direct launches).
+int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_t frames, hipStream_t stream) {
+    const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
+    const float* tables = nullptr;
+    int rc = spectrum_get(&rgb_lut, &lam_lut, &tables);
+    if (rc != AP_OK) return rc;
+    hipError_t e = hipMemcpy(b->frame, &frame0, sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "ap_batch: frame counter");
+    const bool four = prm->wavelengths == 4;
+    auto frame = [&]() {
+        return four ? batch_frame<4>(b, prm, stream, rgb_lut, lam_lut, tables)
+                    : batch_frame<1>(b, prm, stream, rgb_lut, lam_lut, tables);
+    };
+    if (!b->use_graph || stream == nullptr) {
+        for (int32_t f = 0; f < frames; ++f)
+            if ((rc = frame()) != AP_OK) return rc;
+        return AP_OK;
+    }
+    if (!same_key(b, prm, stream)) {
+        b->drop_graph();
+        e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: begin capture");
+        rc = frame();
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(stream, &g);
+        if (rc != AP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: end capture");
+        b->graph = g;
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            b->drop_graph();
+            return hip_fail(e, "ap_batch: instantiate graph");
+        }
+        b->key = *prm;
+        b->key_wave_rays = b->shape.wave_rays;
+        b->key_refill = b->shape.refill;
+        b->key_stream = stream;
+        b->key_valid = true;
+    }
+    for (int32_t f = 0; f < frames; ++f) {
+        e = hipGraphLaunch(b->exec, stream);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: graph launch");
+    }
+    return AP_OK;
+}
+
+int batch_check_params(const ap_render_params* prm) {
+    if (prm->spp < 0 || prm->frame_index < 0 || prm->max_depth < 0 || prm->max_depth > kEpsEntries - 2)
+        return fail(AP_ERR_INVALID_ARG, "ap_batch: bad spp, frame_index or max_depth");
+    if (prm->triangle_material != AP_TRIANGLE_MATERIAL_OBJECT && prm->triangle_material != AP_TRIANGLE_MATERIAL_REFERENCE)
+        return fail(AP_ERR_INVALID_ARG, "ap_batch: bad triangle_material");
+    if (prm->wavelengths != 0 && prm->wavelengths != 1 && prm->wavelengths != 4)
+        return fail(AP_ERR_INVALID_ARG, "ap_batch: wavelengths must be 1 or 4 (0 reads as 1)");
+    if (prm->use_guiding) return fail(AP_ERR_UNSUPPORTED, "ap_batch: guiding is not supported in a batch");
+    return AP_OK;
+}
+
+}  // namespace
+
 }  // namespace ap
 
 using namespace ap;
+
+struct ap_batch {
+    BatchState s;
+};
 
 extern "C" {
 
@@ -1629,6 +1968,209 @@ int ap_nrc_last_frame(ap_scene* scene, ap_nrc_frame_stats* stats, float* host_sa
         e = hipMemcpy(host_samples, r->nrc_pool, n * kNrcSample * sizeof(float), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(e, "ap_nrc_last_frame: samples");
     }
+    return AP_OK;
+}
+
+int ap_batch_create(ap_scene* const* scenes, const ap_camera* cameras, size_t n, ap_batch** out) {
+    if (!out) return fail(AP_ERR_INVALID_ARG, "ap_batch_create: NULL output");
+    *out = nullptr;
+    if (!scenes || !cameras || n == 0 || n > 65535)
+        return fail(AP_ERR_INVALID_ARG, "ap_batch_create: need 1..65535 scenes and cameras");
+    auto* b = new ap_batch();
+    BatchState& B = b->s;
+    B.pix_begin.assign(1, 0u);
+    uint64_t total = 0;
+    int stack_cap = 1;
+    bool all_culled = true;
+    for (size_t k = 0; k < n; ++k) {
+        const ap_camera& c = cameras[k];
+        if (!scenes[k] || c.width <= 0 || c.height <= 0) {
+            delete b;
+            return fail(AP_ERR_INVALID_ARG, "ap_batch_create: NULL scene or bad resolution (scene " + std::to_string(k) + ")");
+        }
+        B.scenes.push_back(scenes[k]);
+        B.cams.push_back(c);
+        total += static_cast<uint64_t>(c.width) * static_cast<uint64_t>(c.height);
+        B.pix_begin.push_back(static_cast<uint32_t>(std::min<uint64_t>(total, 0xffffffffull)));
+        stack_cap = std::max(stack_cap, scenes[k]->dev.stack_cap);
+        all_culled = all_culled && scenes[k]->dev.mode == AP_TRAVERSE_CULLED;
+    }
+    if (total > (uint64_t{1} << 31)) {
+        delete b;
+        return fail(AP_ERR_INVALID_ARG, "ap_batch_create: more than 2^31 pixels per batch frame");
+    }
+    B.total = static_cast<uint32_t>(total);
+    B.h_scenes.resize(n);
+    B.h_devs.resize(n);
+    for (size_t k = 0; k < n; ++k) {
+        // Camera basis of RenderCamera (scene.cu:220-228), as render_frames builds it.
+        const ap_camera& cam = B.cams[k];
+        BatchScene& S = B.h_scenes[k];
+        std::memset(&S, 0, sizeof(S));
+        const f3 dir = normalize(f3{cam.direction[0], cam.direction[1], cam.direction[2]});
+        const float fov = cam.fov_deg * 3.1415926f / 180.0f;
+        const float aspect = static_cast<float>(cam.width) / fmaxf(1.0f, static_cast<float>(cam.height));
+        const float tan_y = tanf(0.5f * fov);
+        const f3 right = normalize(cross(dir, f3{0.0f, 1.0f, 0.0f}));
+        const f3 up = normalize(cross(dir, right));
+        const float c[14] = {cam.origin[0], cam.origin[1], cam.origin[2], dir.x, dir.y, dir.z, right.x, right.y,
+                             right.z, up.x, up.y, up.z, tan_y * aspect, tan_y};
+        std::memcpy(S.cam, c, sizeof(c));
+        S.width = cam.width;
+        S.height = cam.height;
+        S.pix_begin = B.pix_begin[k];
+        S.n_pix = B.pix_begin[k + 1] - B.pix_begin[k];
+    }
+    B.shape = scenes[0]->dev;
+    B.shape.wave_rays = 512;
+    B.shape.refill = 16;
+    B.shape.stack_cap = stack_cap;
+    B.shape.mode = all_culled ? AP_TRAVERSE_CULLED : AP_TRAVERSE_REFERENCE;
+    const size_t cap = B.total;
+    int rc = AP_OK;
+    auto alloc = [&](auto** p, size_t bytes) {
+        if (rc != AP_OK) return;
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) rc = hip_fail(e, "ap_batch_create: hipMalloc");
+    };
+    alloc(&B.d_scenes, n * sizeof(BatchScene));
+    alloc(&B.d_devs, n * sizeof(DeviceScene));
+    alloc(&B.d_pix_begin, (n + 1) * sizeof(uint32_t));
+    alloc(&B.paths, cap * kPathBytes);
+    for (int q = 0; q < 2; ++q) {
+        alloc(&B.rays[q], cap * sizeof(ap_ray));
+        alloc(&B.path_of[q], cap * sizeof(int32_t));
+        alloc(&B.ray_scene[q], cap * sizeof(uint16_t));
+    }
+    alloc(&B.hits, cap * sizeof(ap_hit));
+    alloc(&B.shadow_rays, cap * sizeof(ap_ray));
+    alloc(&B.shadow_payload, cap * kShadowPayBytes);
+    alloc(&B.shadow_hits, cap * sizeof(ap_hit));
+    alloc(&B.shadow_scene, cap * sizeof(uint16_t));
+    alloc(&B.counters, 4 * sizeof(uint32_t));
+    alloc(&B.frame, sizeof(int32_t));
+    alloc(&B.rays_traced, 2 * sizeof(unsigned long long));
+    alloc(&B.eps, kEpsEntries * sizeof(float));
+    if (rc == AP_OK) {
+        std::vector<float> e(kEpsEntries);
+        for (int d = 0; d < kEpsEntries; ++d) e[d] = 1.0e-3f * std::exp(fminf(static_cast<float>(d) * 0.5f, 4.0f));
+        hipError_t he = hipMemcpy(B.eps, e.data(), e.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy(B.d_pix_begin, B.pix_begin.data(), B.pix_begin.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice);
+        if (he != hipSuccess) rc = hip_fail(he, "ap_batch_create: upload");
+    }
+    if (rc != AP_OK) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return AP_OK;
+}
+
+int ap_batch_destroy(ap_batch* batch) {
+    delete batch;
+    return AP_OK;
+}
+
+int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* const* hdr, double* const* sums,
+                    uint64_t ray_count[2], void* stream) {
+    if (!batch || !params) return fail(AP_ERR_INVALID_ARG, "ap_batch_render: NULL argument");
+    int rc = batch_check_params(params);
+    if (rc != AP_OK) return rc;
+    if (params->spp == 0 || params->max_depth == 0) return AP_OK;
+    BatchState& B = batch->s;
+    const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
+    const float* tables = nullptr;
+    if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
+    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut)) != AP_OK) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);
+    if (e != hipSuccess) return hip_fail(e, "ap_batch_render: reset counters");
+    if ((rc = batch_run(&B, params, params->frame_index, params->spp, st)) != AP_OK) return rc;
+    if (ray_count) {
+        unsigned long long c[2] = {0, 0};
+        e = hipMemcpyAsync(c, B.rays_traced, sizeof(c), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch_render: ray counts");
+        ray_count[0] = c[0];
+        ray_count[1] = c[1];
+    }
+    return AP_OK;
+}
+
+int ap_batch_set_launch(ap_batch* batch, int wave_rays, int refill) {
+    if (!batch) return fail(AP_ERR_INVALID_ARG, "ap_batch_set_launch: NULL batch");
+    if (wave_rays < 64 || wave_rays % 64 != 0 || wave_rays > (1 << 16) || refill < 1 || refill > 64)
+        return fail(AP_ERR_INVALID_ARG, "ap_batch_set_launch: wave_rays must be a multiple of 64 up to 65536, refill 1..64");
+    batch->s.shape.wave_rays = wave_rays;
+    batch->s.shape.refill = refill;
+    return AP_OK;
+}
+
+int ap_batch_get_launch(const ap_batch* batch, int* wave_rays, int* refill) {
+    if (!batch || !wave_rays || !refill) return fail(AP_ERR_INVALID_ARG, "ap_batch_get_launch: NULL argument");
+    *wave_rays = batch->s.shape.wave_rays;
+    *refill = batch->s.shape.refill;
+    return AP_OK;
+}
+
+int ap_batch_set_graph(ap_batch* batch, int enable) {
+    if (!batch) return fail(AP_ERR_INVALID_ARG, "ap_batch_set_graph: NULL batch");
+    batch->s.use_graph = enable != 0;
+    if (!batch->s.use_graph) batch->s.drop_graph();
+    return AP_OK;
+}
+
+int ap_batch_autotune(ap_batch* batch, const ap_render_params* params, int frames, int* best_wave_rays,
+                      int* best_refill, float* best_ms_per_frame, void* stream) {
+    if (!batch || !params || frames <= 0) return fail(AP_ERR_INVALID_ARG, "ap_batch_autotune: bad argument");
+    int rc = batch_check_params(params);
+    if (rc != AP_OK) return rc;
+    BatchState& B = batch->s;
+    const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
+    const float* tables = nullptr;
+    if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
+    if ((rc = batch_prepare(&B, params->triangle_material, nullptr, nullptr, rgb_lut)) != AP_OK) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) return hip_fail(e, "ap_batch_autotune: events");
+    const int cand_w[3] = {128, 256, 512};
+    const int cand_r[2] = {8, 16};
+    float best = -1.0f;
+    int bw = B.shape.wave_rays, br = B.shape.refill;
+    for (int wi = 0; wi < 3 && rc == AP_OK; ++wi)
+        for (int ri = 0; ri < 2 && rc == AP_OK; ++ri) {
+            B.shape.wave_rays = cand_w[wi];
+            B.shape.refill = cand_r[ri];
+            if ((rc = batch_run(&B, params, 0, 1, st)) != AP_OK) break;
+            e = hipEventRecord(e0, st);
+            if (e == hipSuccess && (rc = batch_run(&B, params, 1, frames, st)) != AP_OK) break;
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "ap_batch_autotune: timing");
+                break;
+            }
+            ms /= static_cast<float>(frames);
+            if (best < 0.0f || ms < best) {
+                best = ms;
+                bw = cand_w[wi];
+                br = cand_r[ri];
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    B.shape.wave_rays = bw;
+    B.shape.refill = br;
+    if (rc != AP_OK) return rc;
+    if (best_wave_rays) *best_wave_rays = bw;
+    if (best_refill) *best_refill = br;
+    if (best_ms_per_frame) *best_ms_per_frame = best;
     return AP_OK;
 }
 

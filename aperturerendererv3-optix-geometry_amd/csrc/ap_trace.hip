@@ -90,11 +90,16 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) {
 // ray, and the launch tail is bounded by one chunk. A step is one pop of the
 // reference loop: re-test, then a leaf's primitives or an interior node's
 // two children.
-template <bool ANY, bool STATS, bool CULL, int CAP>
+// MULTI: rays of several scenes in one queue; ray j belongs to scene
+// ray_scene[j] and traverses scenes[ray_scene[j]] (`s` then only carries the
+// launch shape). Otherwise every ray traverses `s`.
+template <bool ANY, bool STATS, bool CULL, int CAP, bool MULTI = false>
 __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray* __restrict__ rays,
                                                      uint32_t n_cap, const uint32_t* __restrict__ dyn_n,
                                                      ap_hit* __restrict__ hits,
-                                                     ap_trace_stats* __restrict__ stats) {
+                                                     ap_trace_stats* __restrict__ stats,
+                                                     const DeviceScene* __restrict__ scenes = nullptr,
+                                                     const uint16_t* __restrict__ ray_scene = nullptr) {
     // Stack entries [lo, sp) live in an LDS ring of RING entries per lane
     // (position i % RING); entries [0, lo) that a deeper push evicted live in
     // the lane's private array (scratch memory, rarely touched).
@@ -126,6 +131,9 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
     int32_t hit = -1;
     int sp = 0;
     uint32_t nnode = 0, nprim = 0;
+    const NodeRec* nodes = s.nodes;
+    const PrimRec* prims = s.prims;
+    const AnalyticRec* analytic = s.analytic;
 
     for (;;) {
         const uint64_t idle = __ballot(!live);
@@ -147,14 +155,24 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                     sp = 0;
                     lo = 0;
                     bool finished = true;
-                    if (!s.root.valid) {
+                    RootInfo root = s.root;
+                    float margin = s.line_margin;
+                    if constexpr (MULTI) {
+                        const DeviceScene& D = scenes[ray_scene[j]];
+                        nodes = D.nodes;
+                        prims = D.prims;
+                        analytic = D.analytic;
+                        root = D.root;
+                        margin = D.line_margin;
+                    }
+                    if (!root.valid) {
                         dist = __int_as_float(0x7f800000);
                     } else {
                         inv = inv_safe(d);
-                        w = absf3(inv) * s.line_margin;
+                        w = absf3(inv) * margin;
                         float te;
-                        if (slab_enter<CULL>(o, inv, w, ld3(s.root.bmin), ld3(s.root.bmax), tmin, dist, te)) {
-                            st_link[lane] = s.root.link;
+                        if (slab_enter<CULL>(o, inv, w, ld3(root.bmin), ld3(root.bmax), tmin, dist, te)) {
+                            st_link[lane] = root.link;
                             st_te[lane] = te;
                             sp = 1;
                             finished = false;
@@ -195,7 +213,7 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
             if (link < 0) {
                 int32_t slot = ~link;
                 for (;;) {
-                    const float4* pr = reinterpret_cast<const float4*>(s.prims + slot);
+                    const float4* pr = reinterpret_cast<const float4*>(prims + slot);
                     const float4 A = pr[0];
                     const float4 B = pr[1];
                     const float4 C = pr[2];
@@ -207,9 +225,9 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                         float u, v;
                         ok = tri_mt(o, d, {A.x, A.y, A.z}, {B.x, B.y, B.z}, {C.x, C.y, C.z}, t, u, v);
                     } else if (kind == PRIM_KIND_CUBE) {
-                        ok = cube_t(o, d, s.analytic[__float_as_int(C.w)], t);
+                        ok = cube_t(o, d, analytic[__float_as_int(C.w)], t);
                     } else if (kind == PRIM_KIND_SPHERE) {
-                        ok = sphere_t(o, d, s.analytic[__float_as_int(C.w)], t);
+                        ok = sphere_t(o, d, analytic[__float_as_int(C.w)], t);
                     } else {
                         ok = false;
                     }
@@ -224,7 +242,7 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                     ++slot;
                 }
             } else {
-                const float4* nr = reinterpret_cast<const float4*>(s.nodes + link);
+                const float4* nr = reinterpret_cast<const float4*>(nodes + link);
                 const float4 a = nr[0];
                 const float4 bb = nr[1];
                 const float4 c = nr[2];
@@ -266,9 +284,10 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
     }
 }
 
-template <bool ANY, bool STATS, bool CULL>
+template <bool ANY, bool STATS, bool CULL, bool MULTI = false>
 hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, uint32_t n, const uint32_t* dyn_n, ap_hit* hits,
-                      ap_trace_stats* stats, hipStream_t stream) {
+                      ap_trace_stats* stats, hipStream_t stream, const DeviceScene* scenes = nullptr,
+                      const uint16_t* ray_scene = nullptr) {
     const dim3 block(WAVE);
     const int cap = s.stack_cap;
     // Enough waves for the chip even when a small batch asks for few chunks:
@@ -278,8 +297,8 @@ hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, uint32_t n, cons
     if (cap <= C) {                                                                                     \
         const uint32_t chunk = static_cast<uint32_t>(s.wave_rays);                                      \
         const uint32_t waves = std::max((n + chunk - 1) / chunk, std::min((n + WAVE - 1) / WAVE, kMinWaves)); \
-        hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C>), dim3(waves), block, 0, stream, s, rays, n, \
-                           dyn_n, hits, stats);                                                         \
+        hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C, MULTI>), dim3(waves), block, 0, stream, s, rays, \
+                           n, dyn_n, hits, stats, scenes, ray_scene);                                   \
         return hipGetLastError();                                                                       \
     }
     AP_CAP_CASE(16)
@@ -324,6 +343,19 @@ hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n_total
     AP_DISPATCH(true, true, false)
 #undef AP_DISPATCH
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_trace_multi(const DeviceScene& shape, const DeviceScene* scenes, const uint16_t* ray_scene,
+                              const ap_ray* rays, size_t n_total, ap_hit* hits, bool any_hit, hipStream_t stream,
+                              const uint32_t* dyn_n) {
+    if (n_total == 0) return hipSuccess;
+    if (n_total > (size_t{1} << 31)) return hipErrorInvalidValue;
+    const uint32_t n = static_cast<uint32_t>(n_total);
+    if (shape.mode == AP_TRAVERSE_CULLED)
+        return any_hit ? launch_cap<true, false, true, true>(shape, rays, n, dyn_n, hits, nullptr, stream, scenes, ray_scene)
+                       : launch_cap<false, false, true, true>(shape, rays, n, dyn_n, hits, nullptr, stream, scenes, ray_scene);
+    return any_hit ? launch_cap<true, false, false, true>(shape, rays, n, dyn_n, hits, nullptr, stream, scenes, ray_scene)
+                   : launch_cap<false, false, false, true>(shape, rays, n, dyn_n, hits, nullptr, stream, scenes, ray_scene);
 }
 
 }  // namespace ap

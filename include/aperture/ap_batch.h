@@ -1,0 +1,61 @@
+/*
+ * Batched multi-scene rendering (libaperture_hip.so). No reference
+ * counterpart: the reference renders one SceneRenderer per call
+ * (core/scene.cu:308-346). A batch renders the frames of many scenes of
+ * uneven size together: every scene's paths share one set of queues, each
+ * bounce is one traversal launch over the rays of all scenes, and one
+ * frame of the whole batch is a HIP graph captured once and replayed.
+ * The traversal launch shape is the batch's own, set by
+ * ap_batch_set_launch or picked by ap_batch_autotune.
+ *
+ * Each scene's image equals, bit for bit, what ap_render gives for that
+ * scene alone with the same parameters (same per-pixel random streams,
+ * same traversal and shading), with or without the graph.
+ */
+#ifndef APERTURE_AP_BATCH_H
+#define APERTURE_AP_BATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ap_render.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ap_batch ap_batch;
+
+/* A batch of n scenes (at most 65535), scene k seen by cameras[k]. The
+ * scenes stay owned by the caller and must outlive the batch. */
+int ap_batch_create(ap_scene* const* scenes, const ap_camera* cameras, size_t n, ap_batch** out);
+int ap_batch_destroy(ap_batch* batch);
+
+/* Adds params->spp frames (frame_index, frame_index + 1, ...) of every
+ * scene: hdr[k] (W_k*H_k*3 linear XYZ running mean, required unless NULL to
+ * skip the scene's image) and, if sums is not NULL and sums[k] is not NULL,
+ * FP64 per-pixel sums. The shading, traversal and spectral parameters are
+ * ap_render's; guiding is not supported (AP_ERR_UNSUPPORTED), odd/even and
+ * primary-id buffers are not produced. ray_count (host, may be NULL) gets
+ * the closest-hit and shadow rays traced. Asynchronous on `stream` except
+ * for the first call with new parameters, which captures the graph. */
+int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* const* hdr, double* const* sums,
+                    uint64_t ray_count[2], void* stream);
+
+/* Traversal launch shape of the batch (ap_scene_set_launch's meaning). */
+int ap_batch_set_launch(ap_batch* batch, int wave_rays, int refill);
+int ap_batch_get_launch(const ap_batch* batch, int* wave_rays, int* refill);
+/* 1 (default): frames run as a captured HIP graph; 0: the same kernels are
+ * launched one by one. */
+int ap_batch_set_graph(ap_batch* batch, int enable);
+
+/* Renders one untimed and `frames` timed batch frames (into no image) for
+ * each launch shape of {128, 256, 512} rays per wave x {8, 16} refill, and
+ * keeps the fastest; reports it and its time per frame. */
+int ap_batch_autotune(ap_batch* batch, const ap_render_params* params, int frames, int* best_wave_rays,
+                      int* best_refill, float* best_ms_per_frame, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APERTURE_AP_BATCH_H */

@@ -1,0 +1,94 @@
+"""GPU checks of batched multi-scene rendering (ap_batch_*).
+
+A batch renders every scene's pixels through shared queues and one
+traversal launch per bounce over the rays of all scenes, with a frame
+captured as a HIP graph and replayed. Per-pixel random streams, traversal and
+shading are those of ap_render, so each scene's running mean and FP64 sums
+must equal, bit for bit, what ap_render gives for that scene alone - with the
+graph replayed over several frames (each replay checked through the final
+image and the sums of every frame), without the graph, with four wavelengths
+per path, and after the autotuner has picked a launch shape.
+"""
+import numpy as np
+import pytest
+
+import aperture as ap
+import test_render_gpu as rg
+
+pytestmark = pytest.mark.gpu
+
+
+def scenes_and_cameras():
+    objs = [ap.synth_box_scene(1, 2, 15.0), ap.synth_heightfield_scene(48, 6, 4, 20.0),
+            ap.synth_box_scene(4, 1, 10.0), ap.synth_heightfield_scene(20, 2, 9, 20.0)]
+    cams = [ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 40, 32),
+            ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 64, 36),
+            ap.Camera.make([0.2, 0.1, 3.0], [0, 0, -1], 50.0, 16, 16),
+            ap.Camera.make([0.0, 0.6, 1.2], [0.0, -0.8, -1.4], 60.0, 33, 17)]
+    return objs, cams
+
+
+def batch_render(batch, cams, prm, stream):
+    hdrs, sums = [], []
+    for c in cams:
+        n = c.width * c.height
+        h = ap.DeviceBuffer(n * 12)
+        h.zero()
+        hdrs.append(h)
+        sums.append(ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64)))
+    rays = batch.render(prm, hdrs, sums, stream=stream)
+    stream.synchronize()
+    out = []
+    for c, h, s in zip(cams, hdrs, sums):
+        n = c.width * c.height
+        out.append((h.to_numpy(np.float32, n * 3).reshape(c.height, c.width, 3),
+                    s.to_numpy(np.float64, n * 3).reshape(c.height, c.width, 3)))
+    return out, rays
+
+
+def check_against_single(scenes, cams, prm, got, label):
+    for k, (sc, cam) in enumerate(zip(scenes, cams)):
+        ref = rg.render_gpu(sc, cam, prm)
+        ident = float(np.all(got[k][0] == ref["hdr"], axis=-1).mean())
+        print(f"[batch] {label}: scene {k} {cam.width}x{cam.height}: pixels bit-identical {ident:.4f}", flush=True)
+        assert np.array_equal(got[k][0], ref["hdr"]), (label, k)
+        assert np.array_equal(got[k][1], ref["sums"]), (label, k)
+
+
+def test_batch_equals_single_scene_renders(gpu):
+    objs, cams = scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    prm = ap.RenderParams.defaults(spp=3, max_depth=12, rr_min_depth=4, seed=6)
+    got, rays = batch_render(batch, cams, prm, stream)
+    print(f"[batch] graph: rays {rays}", flush=True)
+    assert rays[0] > 0 and rays[1] > 0
+    check_against_single(scenes, cams, prm, got, "graph, 3 frames")
+    batch.set_graph(False)
+    got, _ = batch_render(batch, cams, prm, stream)
+    check_against_single(scenes, cams, prm, got, "direct launches")
+    batch.set_graph(True)
+    prm4 = ap.RenderParams.defaults(spp=2, frame_index=5, max_depth=12, rr_min_depth=4, seed=7, wavelengths=4)
+    got, _ = batch_render(batch, cams, prm4, stream)
+    check_against_single(scenes, cams, prm4, got, "four wavelengths, from frame 5")
+    batch.close()
+
+
+def test_autotuner_keeps_images(gpu):
+    objs, cams = scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    prm = ap.RenderParams.defaults(spp=2, max_depth=12, rr_min_depth=4, seed=8)
+    w, r, ms = batch.autotune(prm, frames=2, stream=stream)
+    print(f"[batch] autotune: {w} rays per wave, refill {r}, {ms:.3f} ms per frame", flush=True)
+    assert (w, r) == batch.get_launch() and w in (128, 256, 512) and r in (8, 16) and ms > 0.0
+    got, _ = batch_render(batch, cams, prm, stream)
+    check_against_single(scenes, cams, prm, got, "autotuned")
+    with pytest.raises(ap.ApertureError):
+        batch.set_launch(100, 8)
+    with pytest.raises(ap.ApertureError) as e:
+        batch.render(ap.RenderParams.defaults(use_guiding=1), [None] * len(cams), stream=stream)
+    assert e.value.code == ap.ERR_UNSUPPORTED
+    batch.close()
