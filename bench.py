@@ -272,6 +272,79 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     return out
 
 
+def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
+    """Batched heterogeneous rendering: n_scenes heightfield + icosphere
+    scenes (triangles log-uniform in [1e4, 5e5], 4% of them in spheres;
+    resolutions uniform in {128^2, 256^2, 512^2, 1024^2}; 4 thin-cube lights,
+    no sky) rendered together by one batch: shared queues, one traversal
+    launch per bounce over the rays of all scenes, each frame a replayed HIP
+    graph, launch shape autotuned. Timed against the same frames rendered
+    scene by scene with ap_render, and against the batch without the graph."""
+    g = np.random.default_rng(seed)
+    objs, cams = [], []
+    t0 = time.perf_counter()
+    for k in range(n_scenes):
+        tris = float(np.exp(g.uniform(np.log(1e4), np.log(5e5))))
+        spheres = max(1, int(tris * 0.04 / 320))
+        grid = max(8, int(round(np.sqrt(max(tris - 320 * spheres, 128) / 2))))
+        res = int(g.choice([128, 256, 512, 1024]))
+        objs.append(ap.synth_heightfield_scene(grid, spheres, 100 + k, 20.0))
+        cams.append(ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, res, res))
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    log(f"batch: {n_scenes} scenes, {sum(len(o) for o in objs)} objects, "
+        f"{sum(c.width * c.height for c in cams)} pixels per frame ({time.perf_counter() - t0:.1f} s)")
+    stream = ap.Stream()
+    params = dict(max_depth=depth, rr_min_depth=4, sky=0, wavelengths=4)
+    hdrs = []
+    for c in cams:
+        h = ap.DeviceBuffer(c.width * c.height * 12)
+        h.zero(stream)
+        hdrs.append(h)
+    batch = ap.Batch(scenes, cams)
+    w, r, tune_ms = batch.autotune(ap.RenderParams.defaults(spp=1, **params), frames=2, stream=stream)
+    log(f"batch: autotune -> {w} rays per wave, refill {r} ({tune_ms:.2f} ms per frame)")
+    pix = sum(c.width * c.height for c in cams)
+    e0, e1 = ap.Event(), ap.Event()
+
+    def timed(fn):
+        fn(ap.RenderParams.defaults(spp=1, frame_index=0, **params))  # untimed (graph capture, allocation)
+        stream.synchronize()
+        e0.record(stream)
+        out = fn(ap.RenderParams.defaults(spp=spp, frame_index=1, **params))
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1), out
+
+    ms_graph, rays = timed(lambda p: batch.render(p, hdrs, stream=stream))
+    batch.set_graph(False)
+    ms_direct, _ = timed(lambda p: batch.render(p, hdrs, stream=stream))
+
+    def one_by_one(p):
+        for sc, c, h in zip(scenes, cams, hdrs):
+            sc.render(c, p, h, stream=stream)
+
+    ms_seq, _ = timed(one_by_one)
+    batch.close()
+    for sc in scenes:
+        sc.close()
+    out = {
+        "workload": f"{n_scenes} heightfield+icosphere scenes, triangles log-uniform in [1e4, 5e5], resolutions "
+                    f"uniform in {{128^2, 256^2, 512^2, 1024^2}}, {spp} spp, max depth {depth}, RR from depth 4, "
+                    "four hero wavelengths, 4 lights, no sky",
+        "scenes": n_scenes,
+        "triangles": int(sum(len(o) for o in objs)),
+        "pixels_per_frame": int(pix),
+        "launch": {"wave_rays": w, "refill": r},
+        "graph": {"ms": round(ms_graph, 2), "samples_per_s": round(pix * spp / (ms_graph / 1e3), 1),
+                  "mrays_per_s": round((rays[0] + rays[1]) / (ms_graph / 1e3) / 1e6, 1)},
+        "direct_launches": {"ms": round(ms_direct, 2), "samples_per_s": round(pix * spp / (ms_direct / 1e3), 1)},
+        "scene_by_scene": {"ms": round(ms_seq, 2), "samples_per_s": round(pix * spp / (ms_seq / 1e3), 1)},
+        "speedup_over_scene_by_scene": round(ms_seq / ms_graph, 3),
+    }
+    log(f"batch: {out}")
+    return out
+
+
 def main():
     ap_ = argparse.ArgumentParser()
     ap_.add_argument("--gpus", type=int, default=1)
@@ -287,6 +360,7 @@ def main():
     ap_.add_argument("--nrc-ref-spp", type=int, default=2048, help="spp of the plain reference image")
     ap_.add_argument("--nrc-warmup", type=int, default=0, help="cache frames rendered before the averaged ones")
     ap_.add_argument("--nrc-passes", type=int, default=0, help="training-only passes of the cache after each frame")
+    ap_.add_argument("--batch-spp", type=int, default=16, help="spp of the 48-scene batch measurement (0 skips it)")
     ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap_.add_argument("--sweep", action="store_true",
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
@@ -463,6 +537,8 @@ def main():
         line["render"] = render_bench(ap, args.render_spp, 0.0 if args.no_cpu else 10.0, threads,
                                       nrc_spp=args.nrc_spp, nrc_ref_spp=args.nrc_ref_spp,
                                       nrc_warmup=args.nrc_warmup, nrc_passes=args.nrc_passes)
+    if args.batch_spp > 0:
+        line["batch"] = batch_bench(ap, args.batch_spp, 12)
     print(json.dumps(line), flush=True)
 
 
