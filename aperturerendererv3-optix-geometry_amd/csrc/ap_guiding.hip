@@ -49,11 +49,13 @@ struct GuideDevice {
     uint32_t* vals[2] = {nullptr, nullptr};
     uint32_t* flags = nullptr;
     uint32_t* ranks = nullptr;
+    uint32_t* list = nullptr;     // valid slots of the frame, in slot order
     uint64_t* uniq = nullptr;
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
     size_t sort_cap = 0;
-    int32_t* info = nullptr;      // device: [0] unique count, [1..3] min xyz, [4..6] max xyz, [7] new probes, [8] error
+    int32_t* info = nullptr;      // device: [0] unique count, [1..3] min xyz, [4..6] max xyz, [7] new probes, [8] error,
+                                  // [9] valid slots
     int32_t* info_host = nullptr; // pinned
 };
 
@@ -93,22 +95,37 @@ struct VolumeView {
     }
 };
 
-__global__ void voxel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots, uint32_t n,
-                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, int32_t* __restrict__ info) {
+// Valid slots (FilterValidSamples, guiding.cuh:70-99); a valid slot whose
+// voxel is outside the packable range is reported in info[8].
+__global__ void valid_slots_kernel(const ap_path_guiding_sample* __restrict__ slots, uint32_t n,
+                                   uint32_t* __restrict__ flags, int32_t* __restrict__ info) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const ap_path_guiding_sample s = slots[i];
-    uint64_t k = kNoKey;
+    uint32_t f = 0;
     if (s.valid) {
         const ap_int4 v = s.voxel_grid;
-        if (in_range(v.x) && in_range(v.y) && in_range(v.z) && v.w >= 0 && v.w < PG_FACES) {
-            k = pack_voxel(v);
-        } else {
-            atomicOr(&info[8], 1);
-        }
+        if (in_range(v.x) && in_range(v.y) && in_range(v.z) && v.w >= 0 && v.w < PG_FACES) f = 1;
+        else atomicOr(&info[8], 1);
     }
-    keys[i] = k;
-    vals[i] = i;
+    flags[i] = f;
+}
+__global__ void compact_slots_kernel(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ ranks,
+                                     uint32_t n, uint32_t* __restrict__ list, int32_t* __restrict__ info) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (i == n - 1) info[9] = static_cast<int32_t>(ranks[i] + flags[i]);
+    if (flags[i]) list[ranks[i]] = i;
+}
+// (voxel, face) key of each valid slot, the slot index as value.
+__global__ void voxel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots,
+                                  const uint32_t* __restrict__ list, uint32_t m, uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t j = list[i];
+    keys[i] = pack_voxel(slots[j].voxel_grid);
+    vals[i] = j;
 }
 
 __global__ void run_heads_kernel(const uint64_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flags) {
@@ -181,11 +198,12 @@ __global__ void zero_probes_kernel(ap_octahedron* __restrict__ p, size_t n) {
     reinterpret_cast<uint4*>(p)[i] = uint4{0u, 0u, 0u, 0u};
 }
 
-__global__ void texel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots, uint32_t n, VolumeView vv,
-                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void texel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots, const uint32_t* __restrict__ list,
+                                  uint32_t n, VolumeView vv, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const ap_path_guiding_sample s = slots[i];
+    const uint32_t slot = list[i];
+    const ap_path_guiding_sample s = slots[slot];
     uint64_t k = kNoKey;
     if (s.valid) {
         const ap_int4 v = s.voxel_grid;
@@ -199,7 +217,7 @@ __global__ void texel_keys_kernel(const ap_path_guiding_sample* __restrict__ slo
         }
     }
     keys[i] = k;
-    vals[i] = i;
+    vals[i] = slot;
 }
 
 // Head of each (probe, texel) run sums its samples in slot order.
@@ -274,9 +292,10 @@ int ensure_sort(GuideDevice* g, size_t n) {
         g->keys[b] = nullptr;
         g->vals[b] = nullptr;
     }
-    for (void* p : {static_cast<void*>(g->flags), static_cast<void*>(g->ranks), static_cast<void*>(g->uniq), g->tmp})
+    for (void* p : {static_cast<void*>(g->flags), static_cast<void*>(g->ranks), static_cast<void*>(g->list),
+                    static_cast<void*>(g->uniq), g->tmp})
         if (p) (void)hipFree(p);
-    g->flags = g->ranks = nullptr;
+    g->flags = g->ranks = g->list = nullptr;
     g->uniq = nullptr;
     g->tmp = nullptr;
     g->sort_cap = 0;
@@ -287,6 +306,7 @@ int ensure_sort(GuideDevice* g, size_t n) {
     }
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->flags), n * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->ranks), n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->list), n * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->uniq), n * sizeof(uint64_t));
     size_t sort_bytes = 0, scan_bytes = 0;
     if (e == hipSuccess)
@@ -400,22 +420,33 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
     const int32_t init[16] = {0, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN, 0, 0};
     hipError_t e = hipMemcpyAsync(g->info, init, sizeof(init), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: info reset");
-    // 1. unique voxels in sorted order.
-    hipLaunchKernelGGL(voxel_keys_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, g->keys[0], g->vals[0],
-                       g->info);
+    // 0. the valid slots, in slot order (most of a frame's slots are empty).
+    hipLaunchKernelGGL(valid_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, g->flags, g->info);
     size_t bytes = g->tmp_bytes;
-    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], n, 0, 64, st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
-    hipLaunchKernelGGL(run_heads_kernel, grid_for(n), dim3(kBlock), 0, st, g->keys[1], n, g->flags);
-    bytes = g->tmp_bytes;
     e = rocprim::exclusive_scan(g->tmp, bytes, g->flags, g->ranks, 0u, n, rocprim::plus<uint32_t>(), st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: slot scan");
+    hipLaunchKernelGGL(compact_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->flags, g->ranks, n, g->list, g->info);
+    e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: read valid count");
+    if (g->info_host[8]) return fail(AP_ERR_UNSUPPORTED, "guiding: a voxel coordinate is outside [-2^18, 2^18)");
+    const uint32_t m = static_cast<uint32_t>(g->info_host[9]);
+    if (m == 0) return AP_OK;
+    // 1. unique voxels in sorted order.
+    hipLaunchKernelGGL(voxel_keys_kernel, grid_for(m), dim3(kBlock), 0, st, g->slots, g->list, m, g->keys[0],
+                       g->vals[0]);
+    bytes = g->tmp_bytes;
+    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, 64, st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+    hipLaunchKernelGGL(run_heads_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], m, g->flags);
+    bytes = g->tmp_bytes;
+    e = rocprim::exclusive_scan(g->tmp, bytes, g->flags, g->ranks, 0u, m, rocprim::plus<uint32_t>(), st);
     if (e != hipSuccess) return hip_fail(e, "guiding: voxel scan");
-    hipLaunchKernelGGL(unique_voxels_kernel, grid_for(n), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, n,
+    hipLaunchKernelGGL(unique_voxels_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, m,
                        g->uniq, g->info);
     e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
-    if (g->info_host[8]) return fail(AP_ERR_UNSUPPORTED, "guiding: a voxel coordinate is outside [-2^18, 2^18)");
     const uint32_t n_uniq = static_cast<uint32_t>(g->info_host[0]);
     if (n_uniq == 0) return AP_OK;
     // 2. grow the index volume to the union of the bounds.
@@ -476,15 +507,15 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
         g->count += n_new;
     }
     // 3. per (probe, texel) sums in slot order.
-    hipLaunchKernelGGL(texel_keys_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, view_of(g), g->keys[0],
-                       g->vals[0]);
+    hipLaunchKernelGGL(texel_keys_kernel, grid_for(m), dim3(kBlock), 0, st, g->slots, g->list, m, view_of(g),
+                       g->keys[0], g->vals[0]);
     int probe_bits = 1;
     while ((1ll << probe_bits) <= g->count) ++probe_bits;
     const int end_bit = std::min(64, 6 + probe_bits + 1);
     bytes = g->tmp_bytes;
-    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], n, 0, end_bit, st);
+    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, end_bit, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: texel sort");
-    hipLaunchKernelGGL(texel_sums_kernel, grid_for(n), dim3(kBlock), 0, st, g->keys[1], g->vals[1], n, g->slots,
+    hipLaunchKernelGGL(texel_sums_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], g->vals[1], m, g->slots,
                        g->temp);
     // 4. FP64 merge and mips.
     if (g->count > 0)

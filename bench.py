@@ -250,6 +250,19 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     log(f"render: {out}")
     if nrc_spp > 0:
         out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes)
+        # The same comparison in a closed interior: the C1 box (coloured walls,
+        # two spheres, one ceiling light) at 1920x1080, where long path tails
+        # carry most of the light.
+        box = ap.SceneRenderer(ap.synth_box_scene(1, 3, 15.0))
+        box_cam = ap.Camera.make([0.0, 0.0, 3.4], [0.0, 0.0, -1.0], 40.0, W, H)
+        box_params = dict(max_depth=8, rr_min_depth=3, sky=1)
+        box.render(box_cam, ap.RenderParams.defaults(spp=1, wavelengths=4, **box_params), hdr, stream=stream)
+        stream.synchronize()
+        res = nrc_bench(ap, box, box_cam, stream, box_params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes)
+        res["workload"] = ("closed box of the C1 workload (5 walls, two 1280-triangle spheres, 0.5x0.5 ceiling "
+                           "light) at 1920x1080, max depth 8, RR from depth 3")
+        out["nrc_vs_plain_closed_box"] = res
+        box.close()
     if cpu_seconds > 0:
         sys.path.insert(0, os.path.join(REPO, "tests"))
         import oracle_lib as ol
