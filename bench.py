@@ -99,7 +99,8 @@ def rel_mse(xyz, ref_xyz):
     return float((((a - b) ** 2) / (b ** 2 + 1e-2)).mean())
 
 
-def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_passes):
+def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_passes, boost_frames=0,
+              boost_steps=16):
     """Equal-time relMSE of the radiance-cache renderer against the plain
     wavefront tracer, both with four hero wavelengths per path. The plain
     tracer renders plain_spp frames (timed with HIP events); the cache
@@ -107,8 +108,10 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     frame, training after each (and on train_passes training-only passes),
     for as many frames as its calibrated per-frame time fits in the same GPU
     time: its first `warmup` frames go to a scratch image, the rest are
-    averaged. Both images are compared with a plain render of ref_spp frames
-    on another seed."""
+    averaged; the first `boost_frames` averaged frames train with
+    `boost_steps` Adam steps instead of 4, their extra cost calibrated and
+    charged to the budget. Both images are compared with a plain render of
+    ref_spp frames on another seed."""
     W, H = cam.width, cam.height
     n = W * H
     ref = ap.DeviceBuffer(n * 12)
@@ -133,10 +136,13 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     scratch.zero(stream)
     w0 = cache.params()
 
-    def nrc_frames(frames, buf, seed):
+    boost = ap.NrcRenderParams.from_buffer_copy(nprm)
+    boost.train_steps = boost_steps
+
+    def nrc_frames(frames, buf, seed, first=0, p=nprm):
         if frames > 0:
-            scene.render_nrc(cam, ap.RenderParams.defaults(spp=frames, seed=seed, wavelengths=4, **params), cache,
-                             nprm, buf, stream=stream)
+            scene.render_nrc(cam, ap.RenderParams.defaults(spp=frames, frame_index=first, seed=seed, wavelengths=4,
+                                                           **params), cache, p, buf, stream=stream)
 
     # Calibration (untimed; it also allocates the path, query and training
     # buffers): GPU time per frame of the cache renderer, frames submitted
@@ -148,7 +154,15 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     e1.record(stream)
     stream.synchronize()
     per_frame = e0.elapsed_ms(e1) / 8
-    total = max(warmup + 1, int(budget / per_frame))
+    e0.record(stream)
+    nrc_frames(4, scratch, 4, 0, boost)
+    e1.record(stream)
+    stream.synchronize()
+    per_boost = e0.elapsed_ms(e1) / 4
+    # Frames that fit the budget: warmup and boosted frames at their own cost.
+    nb_fit = boost_frames
+    rest = budget - warmup * per_frame - nb_fit * per_boost
+    total = max(warmup + 1, warmup + nb_fit + int(max(rest, 0.0) / per_frame))
     # Timed run from the initial weights with a fresh optimiser: `warmup`
     # frames into a scratch image, the rest averaged into the compared image.
     cache.set_params(w0)
@@ -157,7 +171,10 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     stream.synchronize()
     e0.record(stream)
     nrc_frames(warmup, scratch, 82)
-    nrc_frames(total - warmup, img, 5)
+    # The first boost_frames averaged frames train with boost_steps steps each.
+    nb = min(boost_frames, total - warmup)
+    nrc_frames(nb, img, 5, 0, boost)
+    nrc_frames(total - warmup - nb, img, 5, nb)
     e1.record(stream)
     stream.synchronize()
     used = e0.elapsed_ms(e1)
@@ -172,8 +189,10 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
         "nrc_frames": frames - warmup,
         "nrc_warmup_frames": warmup,
         "nrc_training_passes_per_frame": train_passes,
+        "nrc_boost": {"frames": boost_frames, "steps": boost_steps},
         "nrc_ms": round(used, 1),
         "nrc_ms_per_frame_calibrated": round(per_frame, 3),
+        "nrc_ms_per_boosted_frame_calibrated": round(per_boost, 3),
         "nrc_relmse": round(nrc, 5),
         "relmse_ratio_plain_over_nrc": round(plain / nrc, 3) if nrc > 0 else None,
         "reference_spp": ref_spp,
@@ -182,14 +201,15 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
         "training_paths_per_frame": st.train_paths,
         "training_samples_per_frame": st.samples,
         "cache": "64-64x5-3 ReLU, BF16 MFMA / FP32 accumulate, relative L2, Adam lr 0.005, 4 steps x 16384 per "
-                 "frame from 65,536 training paths",
+                 "frame from 65,536 training paths (the first boost frames take more steps, at their own cost)",
         "spectral": "four hero wavelengths per path in the reference, plain and cache images",
     }
     log(f"nrc: {out}")
     return out
 
 
-def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc_warmup=4, nrc_passes=0):
+def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc_warmup=0, nrc_passes=0,
+                 nrc_boost=(8, 16)):
     """Samples/s of the wavefront path tracer at 1920x1080 on the heightfield
     workload (1024^2-cell fBm heightfield, 2,097,152 triangles, plus 256
     320-triangle icospheres; Lambert/GGX 50/50; lit by 4 thin-cube lights, no
@@ -249,7 +269,8 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     }
     log(f"render: {out}")
     if nrc_spp > 0:
-        out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes)
+        out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes,
+                                        *nrc_boost)
         # The same comparison in a closed interior: the C1 box (coloured walls,
         # two spheres, one ceiling light) at 1920x1080, where long path tails
         # carry most of the light.
@@ -258,7 +279,8 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
         box_params = dict(max_depth=8, rr_min_depth=3, sky=1)
         box.render(box_cam, ap.RenderParams.defaults(spp=1, wavelengths=4, **box_params), hdr, stream=stream)
         stream.synchronize()
-        res = nrc_bench(ap, box, box_cam, stream, box_params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes)
+        res = nrc_bench(ap, box, box_cam, stream, box_params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes,
+                        *nrc_boost)
         res["workload"] = ("closed box of the C1 workload (5 walls, two 1280-triangle spheres, 0.5x0.5 ceiling "
                            "light) at 1920x1080, max depth 8, RR from depth 3")
         out["nrc_vs_plain_closed_box"] = res
