@@ -259,6 +259,16 @@ AP_HD f3 pg_query(float mask, const float xyz[3], bool first) {
     return q / den;
 }
 
+// maskForProbeQuery with several wavelengths per path: the throughput's XYZ,
+// sum_j xyz(l_j) mask_j, floored and normalised as above (one wavelength:
+// exactly pg_query).
+AP_HD f3 pg_query_xyz(f3 txyz, bool first) {
+    if (first) return {1.0f, 1.0f, 1.0f};
+    f3 q{fmaxf(txyz.x, PG_QUERY_EPS), fmaxf(txyz.y, PG_QUERY_EPS), fmaxf(txyz.z, PG_QUERY_EPS)};
+    const float den = static_cast<float>(static_cast<double>(q.x + q.y + q.z) + 1.0e-6);
+    return q / den;
+}
+
 AP_HD void pg_generate_mipmaps(ap_octahedron& p) {
     for (int i = 0; i < PG_RES2; ++i) {
         ap_float4& t = p.data[i];
@@ -289,13 +299,14 @@ AP_HD void pg_generate_mipmaps(ap_octahedron& p) {
     p.data[PG_TOTAL] = ap_float4{0.0f, 0.0f, 0.0f, 0.0f};
 }
 
-// Per-path radiance collection (CalculateRadiance's radianceSample* arrays).
+// Per-path radiance collection (CalculateRadiance's radianceSample* arrays),
+// with room for four wavelengths per path (index 0 alone with one).
 struct alignas(16) GuideRec {
     float dir[PG_COLLECT][3];
     float loc[PG_COLLECT][3];
     float vdir[PG_COLLECT][3];
-    float sample[PG_COLLECT];
-    float mask[PG_COLLECT];
+    float sample[PG_COLLECT][4];
+    float mask[PG_COLLECT][4];
     uint32_t valid;   // bit i: slot i collected radiance
     uint32_t pad[3];
 };

@@ -210,36 +210,56 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + static_cast<uint32_t>(__popcll(below));
 }
 
-// CollectRadiance (render.cuh:353-432) on a path's guiding record.
-__device__ __forceinline__ void pg_endpoint(GuideRec& r, float le, int ndd) {
-    if (le == 0.0f) return;
+template <int NL>
+__device__ __forceinline__ f3 path_xyz(const float lam[NL], const float* L, float psum,
+                                       const float* __restrict__ tables);
+
+// CollectRadiance (render.cuh:353-432) on a path's guiding record. A slot
+// is open while any of its wavelengths' throughputs is non-zero (with one
+// wavelength, the reference's mask[i] != 0).
+template <int NL>
+__device__ __forceinline__ bool pg_open(const GuideRec& r, int i) {
+    bool open = false;
+    for (int j = 0; j < NL; ++j) open = open || r.mask[i][j] != 0.0f;
+    return open;
+}
+
+template <int NL>
+__device__ __forceinline__ void pg_endpoint(GuideRec& r, const float* le, int ndd) {
+    bool any = false;
+    for (int j = 0; j < NL; ++j) any = any || le[j] != 0.0f;
+    if (!any) return;
     const int hi = ndd < PG_COLLECT ? ndd : PG_COLLECT;
     for (int i = 0; i < hi; ++i)
-        if (r.mask[i] != 0.0f) {
-            r.sample[i] = r.sample[i] + r.mask[i] * le;
+        if (pg_open<NL>(r, i)) {
+            for (int j = 0; j < NL; ++j) r.sample[i][j] = r.sample[i][j] + r.mask[i][j] * le[j];
             r.valid |= 1u << i;
         }
 }
 
-__device__ __forceinline__ void pg_advance(GuideRec& r, f3 next, float contrib, float rr_prob, int ndd, bool allow,
-                                           f3 ray_dir, f3 pos, f3 nrm) {
-    if (allow && ndd < PG_COLLECT && r.mask[ndd] == 0.0f) {
+template <int NL>
+__device__ __forceinline__ void pg_advance(GuideRec& r, f3 next, const float* contrib, float rr_prob, int ndd,
+                                           bool allow, f3 ray_dir, f3 pos, f3 nrm) {
+    if (allow && ndd < PG_COLLECT && !pg_open<NL>(r, ndd)) {
         r.dir[ndd][0] = next.x; r.dir[ndd][1] = next.y; r.dir[ndd][2] = next.z;
         r.loc[ndd][0] = pos.x; r.loc[ndd][1] = pos.y; r.loc[ndd][2] = pos.z;
         const f3 v = pg_reflect(ray_dir, nrm);
         r.vdir[ndd][0] = v.x; r.vdir[ndd][1] = v.y; r.vdir[ndd][2] = v.z;
-        r.mask[ndd] = 1.0f;
+        for (int j = 0; j < NL; ++j) r.mask[ndd][j] = 1.0f;
     }
     const float inv_q = static_cast<float>(1.0 / static_cast<double>(rr_prob));
     const int up = ndd + (allow ? 1 : 0);
     const int hi = up < PG_COLLECT ? up : PG_COLLECT;
     for (int i = 0; i < hi; ++i)
-        if (r.mask[i] != 0.0f) r.mask[i] = r.mask[i] * (contrib * inv_q);
+        if (pg_open<NL>(r, i))
+            for (int j = 0; j < NL; ++j) r.mask[i][j] = r.mask[i][j] * (contrib[j] * inv_q);
 }
 
-// Path end (render.cuh:910-957): valid collection slots to guiding samples.
-__device__ __forceinline__ void pg_emit(const GuideRec& r, const float xyz[3], float lpdf,
-                                        ap_path_guiding_sample* __restrict__ out) {
+// Path end (render.cuh:910-957): valid collection slots to guiding samples,
+// radiance sum_j xyz(l_j) sample_j / psum.
+template <int NL>
+__device__ __forceinline__ void pg_emit(const GuideRec& r, const float* lam, float psum,
+                                        const float* __restrict__ tables, ap_path_guiding_sample* __restrict__ out) {
     int w = 0;
     ap_path_guiding_sample rec[PG_COLLECT];
     for (int k = 0; k < PG_COLLECT; ++k) {
@@ -248,7 +268,9 @@ __device__ __forceinline__ void pg_emit(const GuideRec& r, const float xyz[3], f
     for (int i = 0; i < PG_COLLECT; ++i) {
         if (w >= PG_COLLECT) break;
         if (!((r.valid >> i) & 1u)) continue;
-        const f3 rx = f3{xyz[0], xyz[1], xyz[2]} * r.sample[i] / lpdf;
+        float smp[NL];
+        for (int j = 0; j < NL; ++j) smp[j] = r.sample[i][j];
+        const f3 rx = path_xyz<NL>(lam, smp, psum, tables);
         const float strength = fminf(max3(rx), PG_MAX_BRIGHT);
         if (strength < PG_MIN_BRIGHT) continue;
         ap_path_guiding_sample& o = rec[w++];
@@ -561,7 +583,6 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     ap_ray* __restrict__ shadow_rays, ShadowPay<NL>* __restrict__ shadow_pay, uint32_t* __restrict__ counters,
     int32_t* __restrict__ primary_id, int32_t primary_frame, unsigned long long* __restrict__ ray_count, GuideView gv,
     NrcView nv, BatchView bv = BatchView{}) {
-    static_assert(!GUIDE || NL == 1, "guiding runs with one wavelength per path");
     static_assert(!MULTI || (!GUIDE && !NRC), "a batch renders without guiding or the cache");
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
@@ -600,15 +621,16 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     if (H.id < 0) {
         // Miss: SkyLighting and the offset draw of Trace, then the path ends.
         (void)rand1(rng);
-        float le_end = 0.0f;
+        float le_end[NL];
+        for (int j = 0; j < NL; ++j) le_end[j] = 0.0f;
         if (!inside) {
             for (int j = 0; j < NL; ++j) {
                 const float le = sp.sky ? interact(rgb_lut, sc.lb[j], sky_lighting(d)) : 0.0f;
                 S.radiance[j] = S.radiance[j] + S.mask[j] * le * sc.d65[j];
-                if (j == 0) le_end = le * sc.d65[0];
+                le_end[j] = le * sc.d65[j];
             }
         }
-        if (GUIDE) pg_endpoint(gv.recs[pid], le_end, S.ndd);
+        if (GUIDE) pg_endpoint<NL>(gv.recs[pid], le_end, S.ndd);
         S.flags &= ~PATH_PREV_NEE;
     } else {
         const GeoRec g = geo[H.id];
@@ -669,10 +691,16 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         float g_alpha = 0.0f;
         f3 g_query{1.0f, 1.0f, 1.0f};
         if (GUIDE) {
-            const f3 xyz0 = lambda_to_xyz(tables, S.lam[0]);
-            const float xyz_a[3] = {xyz0.x, xyz0.y, xyz0.z};
             g_alpha = pg_alpha(S.frame);
-            g_query = pg_query(S.mask[0], xyz_a, sp.depth == 0);
+            if (NL == 1) {
+                const f3 xyz0 = lambda_to_xyz(tables, S.lam[0]);
+                const float xyz_a[3] = {xyz0.x, xyz0.y, xyz0.z};
+                g_query = pg_query(S.mask[0], xyz_a, sp.depth == 0);
+            } else {
+                f3 t = lambda_to_xyz(tables, S.lam[0]) * S.mask[0];
+                for (int j = 1; j < NL; ++j) t = t + lambda_to_xyz(tables, S.lam[j]) * S.mask[j];
+                g_query = pg_query_xyz(t, sp.depth == 0);
+            }
             if (!pure_light && !inside && gv.probe_count > 0) {
                 probe = pg_query_probe(gv, hp, pg_reflect(d, N));
                 if (probe) {
@@ -684,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                             if (dot(gs.dir, N) > 1.0e-12f) {
                                 const float p_bs = bsdf_pdf(M, lb, N, d, gs.dir);
                                 next_dir = gs.dir;
-                                bsdf_cos[0] = eval_bsdf_cos(M, lb, d, gs.dir, N);
+                                for (int j = 0; j < NL; ++j) bsdf_cos[j] = eval_bsdf_cos(M, sc.lb[j], d, gs.dir, N);
                                 pdf = (1.0f - g_alpha) * fmaxf(p_bs, 1.0e-12f) + g_alpha * gs.pdf;
                             } else {
                                 probe = nullptr;
@@ -699,7 +727,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             }
         }
         // Emission of a light hit, MIS-weighted against the previous NEE (render.cuh:666-710).
-        float le_end = 0.0f;
+        float le_end[NL];
+        for (int j = 0; j < NL; ++j) le_end[j] = 0.0f;
         if (hi.light_index >= 0) {
             float mis = 1.0f;
             if (sp.use_nee && sp.depth > 0 && (S.flags & PATH_PREV_NEE)) {
@@ -714,12 +743,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             }
             if (GUIDE) {
                 const f3 e{M.emissive_rgb[0], M.emissive_rgb[1], M.emissive_rgb[2]};
-                le_end = interact(rgb_lut, lb, e * mis) * sc.d65[0];
+                for (int j = 0; j < NL; ++j) le_end[j] = interact(rgb_lut, sc.lb[j], e * mis) * sc.d65[j];
             }
         } else if (GUIDE && !(M.emissive_rgb[0] == 0.0f && M.emissive_rgb[1] == 0.0f && M.emissive_rgb[2] == 0.0f)) {
-            le_end = interact_basis(M.emissive_basis, M.emissive_scale, lb) * sc.d65[0];
+            for (int j = 0; j < NL; ++j)
+                le_end[j] = interact_basis(M.emissive_basis, M.emissive_scale, sc.lb[j]) * sc.d65[j];
         }
-        if (GUIDE) pg_endpoint(gv.recs[pid], le_end, S.ndd);
+        if (GUIDE) pg_endpoint<NL>(gv.recs[pid], le_end, S.ndd);
         if (NRC) {
             // Spread heuristic and cache termination (ap_nrc.h).
             NrcPath P = nv.paths[pid];
@@ -843,7 +873,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                 if (rand1(rng) > p_rr) survived = false;
             }
             if (survived) {
-                if (GUIDE) pg_advance(gv.recs[pid], next_dir, contrib[0], p_rr, S.ndd, !inside, d, hp, N);
+                if (GUIDE) pg_advance<NL>(gv.recs[pid], next_dir, contrib, p_rr, S.ndd, !inside, d, hp, N);
                 for (int j = 0; j < NL; ++j) {
                     S.mask[j] = S.mask[j] * contrib[j];
                     S.mask[j] = S.mask[j] / p_rr;
@@ -1047,11 +1077,7 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int32_t f = 0; f < fp.frames; ++f) {
         const PathT<NL>& S = paths[static_cast<size_t>(f) * fp.n_pixels + p];
-        if (slots && f == 0) {
-            const f3 x0 = lambda_to_xyz(tables, S.lam[0]);
-            const float xyz_a[3] = {x0.x, x0.y, x0.z};
-            pg_emit(recs[p], xyz_a, S.psum, slots + static_cast<size_t>(p) * PG_COLLECT);
-        }
+        if (slots && f == 0) pg_emit<NL>(recs[p], S.lam, S.psum, tables, slots + static_cast<size_t>(p) * PG_COLLECT);
         float rad[NL];
         for (int j = 0; j < NL; ++j) rad[j] = fix_nan_inf(S.radiance[j]);
         const f3 pre = path_xyz<NL>(S.lam, rad, S.psum, tables);
@@ -1490,12 +1516,11 @@ int run_frames(const RunCtx& c) {
             ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                            prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
             if (guiding) {
-                if constexpr (NL == 1)  // guiding is rejected with several wavelengths before this point
-                    hipLaunchKernelGGL((shade_kernel<1, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
-                                       r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
-                                       paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
-                                       r->path_of[out_q], r->shadow_rays, pays, r->counters, prim, last_frame,
-                                       rc_ptr, gv, nv);
+                hipLaunchKernelGGL((shade_kernel<NL, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
+                                   r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
+                                   paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
+                                   r->path_of[out_q], r->shadow_rays, pays, r->counters, prim, last_frame,
+                                   rc_ptr, gv, nv);
             } else if (nrc) {
                 hipLaunchKernelGGL((shade_kernel<NL, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
                                    r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, paths,
@@ -1601,8 +1626,6 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
         return fail(AP_ERR_INVALID_ARG, "ap_render: bad triangle_material");
     if (prm->wavelengths != 0 && prm->wavelengths != 1 && prm->wavelengths != 4)
         return fail(AP_ERR_INVALID_ARG, "ap_render: wavelengths must be 1 or 4 (0 reads as 1)");
-    if (prm->wavelengths == 4 && prm->use_guiding)
-        return fail(AP_ERR_UNSUPPORTED, "ap_render: guiding runs with one wavelength per path");
     if (prm->spp == 0 || prm->max_depth == 0) return AP_OK;
     const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
     const float* tables = nullptr;

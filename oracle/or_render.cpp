@@ -568,59 +568,6 @@ struct RenderOpts {
     int max_depth, rr_min_depth, nee_max_depth, use_nee, nee_always, sky;
 };
 
-// Per-path radiance collection for guiding (CalculateRadiance's
-// radianceSample* arrays, render.cuh:478-520, and CollectRadiance, :353-432).
-struct Collect {
-    float3 dir[PG_COLLECT], loc[PG_COLLECT], vdir[PG_COLLECT];
-    float sample[PG_COLLECT], mask[PG_COLLECT];
-    bool valid[PG_COLLECT];
-    void Clear() {
-        for (int i = 0; i < PG_COLLECT; ++i) {
-            dir[i] = loc[i] = vdir[i] = float3{0, 0, 0};
-            sample[i] = mask[i] = 0.0f;
-            valid[i] = false;
-        }
-    }
-    void Endpoint(float le, int ndd) {
-        if (le == 0.0f) return;
-        const int hi = std::min(ndd, PG_COLLECT);
-        for (int i = 0; i < hi; ++i)
-            if (mask[i] != 0.0f) { sample[i] = sample[i] + mask[i] * le; valid[i] = true; }
-    }
-    void Advance(float3 next, float contrib, float rrProb, int ndd, bool allowStart, float3 rayDir, float3 pos,
-                 float3 nrm) {
-        if (allowStart && ndd < PG_COLLECT && mask[ndd] == 0.0f) {
-            dir[ndd] = next;
-            loc[ndd] = pos;
-            vdir[ndd] = Reflect(rayDir, nrm);
-            mask[ndd] = 1.0f;
-        }
-        const float invQ = static_cast<float>(1.0 / static_cast<double>(rrProb));
-        const int hi = std::min(ndd + (allowStart ? 1 : 0), PG_COLLECT);
-        for (int i = 0; i < hi; ++i)
-            if (mask[i] != 0.0f) mask[i] = mask[i] * (contrib * invQ);
-    }
-    // Path end (render.cuh:910-957): valid vertices to PathGuidingSamples.
-    void Emit(float3 xyz, float lpdf, ap_path_guiding_sample out[PG_COLLECT]) const {
-        std::memset(out, 0, sizeof(ap_path_guiding_sample) * PG_COLLECT);
-        int w = 0;
-        for (int i = 0; i < PG_COLLECT; ++i) {
-            if (w >= PG_COLLECT) break;
-            if (!valid[i]) continue;
-            const float3 rx = xyz * sample[i] / lpdf;
-            const float strength = std::fmin(max3(rx), PG_MAX_BRIGHT);
-            if (strength < PG_MIN_BRIGHT) continue;
-            ap_path_guiding_sample& o = out[w++];
-            o.direction = ap_float3{dir[i].x, dir[i].y, dir[i].z};
-            const float3 r = fmax3(rx, float3{0.0f, 0.0f, 0.0f});
-            o.radiance = ap_float3{r.x, r.y, r.z};
-            const int4_ v = GetVoxelGrid(loc[i], vdir[i]);
-            o.voxel_grid = ap_int4{v.x, v.y, v.z, v.w};
-            o.valid = 1;
-        }
-    }
-};
-
 // The wavelengths of a path (the core's hero-wavelength mode when n = 4):
 // the hero and its rotations by 105 nm over [360, 780], with their bases,
 // D65 weights, CMF values and the sum of their pdfs.
@@ -670,6 +617,75 @@ static float MeanMask(const float* m, int n) {
     return s * (1.0f / static_cast<float>(n));
 }
 
+// Per-path radiance collection for guiding (CalculateRadiance's
+// radianceSample* arrays, render.cuh:478-520, and CollectRadiance, :353-432),
+// per wavelength; a slot is open while any wavelength's throughput is
+// non-zero (with one wavelength, the reference's mask[i] != 0).
+struct Collect {
+    float3 dir[PG_COLLECT], loc[PG_COLLECT], vdir[PG_COLLECT];
+    float sample[PG_COLLECT][MAXL], mask[PG_COLLECT][MAXL];
+    bool valid[PG_COLLECT];
+    int n = 1;
+    void Clear(int nl) {
+        n = nl;
+        for (int i = 0; i < PG_COLLECT; ++i) {
+            dir[i] = loc[i] = vdir[i] = float3{0, 0, 0};
+            for (int j = 0; j < MAXL; ++j) sample[i][j] = mask[i][j] = 0.0f;
+            valid[i] = false;
+        }
+    }
+    bool Open(int i) const {
+        bool o = false;
+        for (int j = 0; j < n; ++j) o = o || mask[i][j] != 0.0f;
+        return o;
+    }
+    void Endpoint(const float* le, int ndd) {
+        bool any = false;
+        for (int j = 0; j < n; ++j) any = any || le[j] != 0.0f;
+        if (!any) return;
+        const int hi = std::min(ndd, PG_COLLECT);
+        for (int i = 0; i < hi; ++i)
+            if (Open(i)) {
+                for (int j = 0; j < n; ++j) sample[i][j] = sample[i][j] + mask[i][j] * le[j];
+                valid[i] = true;
+            }
+    }
+    void Advance(float3 next, const float* contrib, float rrProb, int ndd, bool allowStart, float3 rayDir,
+                 float3 pos, float3 nrm) {
+        if (allowStart && ndd < PG_COLLECT && !Open(ndd)) {
+            dir[ndd] = next;
+            loc[ndd] = pos;
+            vdir[ndd] = Reflect(rayDir, nrm);
+            for (int j = 0; j < n; ++j) mask[ndd][j] = 1.0f;
+        }
+        const float invQ = static_cast<float>(1.0 / static_cast<double>(rrProb));
+        const int hi = std::min(ndd + (allowStart ? 1 : 0), PG_COLLECT);
+        for (int i = 0; i < hi; ++i)
+            if (Open(i))
+                for (int j = 0; j < n; ++j) mask[i][j] = mask[i][j] * (contrib[j] * invQ);
+    }
+    // Path end (render.cuh:910-957): valid vertices to PathGuidingSamples,
+    // radiance sum_j xyz_j sample_j / psum.
+    void Emit(const Waves& wv, ap_path_guiding_sample out[PG_COLLECT]) const {
+        std::memset(out, 0, sizeof(ap_path_guiding_sample) * PG_COLLECT);
+        int w = 0;
+        for (int i = 0; i < PG_COLLECT; ++i) {
+            if (w >= PG_COLLECT) break;
+            if (!valid[i]) continue;
+            const float3 rx = WavesXYZ(wv, sample[i]);
+            const float strength = std::fmin(max3(rx), PG_MAX_BRIGHT);
+            if (strength < PG_MIN_BRIGHT) continue;
+            ap_path_guiding_sample& o = out[w++];
+            o.direction = ap_float3{dir[i].x, dir[i].y, dir[i].z};
+            const float3 r = fmax3(rx, float3{0.0f, 0.0f, 0.0f});
+            o.radiance = ap_float3{r.x, r.y, r.z};
+            const int4_ v = GetVoxelGrid(loc[i], vdir[i]);
+            o.voxel_grid = ap_int4{v.x, v.y, v.z, v.w};
+            o.valid = 1;
+        }
+    }
+};
+
 // CalculateRadiance (render.cuh:460-999) for one camera ray; returns the
 // clamped XYZ sample and the primary hit id. With a guiding state G the
 // probes of G guide the BSDF sampling and NEE's MIS pdf, and the path's
@@ -685,7 +701,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
     const float* lb = wv.lb[0];
     const bool guiding = G != nullptr;
     Collect col;
-    if (guiding) col.Clear();
+    if (guiding) col.Clear(nl);
     int ndd = 0;
     dir = normalize(dir);
     float radiance[MAXL], mask[MAXL];
@@ -706,12 +722,12 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         const float alpha = GuidingAlpha(frame);
         if (id < 0) {
             (void)rng.rand1();  // offset draw
-            float endpoint = 0.0f;
+            float endpoint[MAXL] = {};
             if (!inside) {
                 for (int j = 0; j < nl; ++j) {
                     const float le = P.sky ? InteractC(wv.lb[j], SkyLighting(rd)) : 0.0f;
                     radiance[j] = radiance[j] + mask[j] * le * wv.d65[j];
-                    if (guiding) endpoint = le * wv.d65[0];
+                    endpoint[j] = le * wv.d65[j];
                 }
             }
             if (guiding) col.Endpoint(endpoint, ndd);
@@ -738,7 +754,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                         if (dot(gs.dir, c.normal) > 1.0e-12f) {
                             const float pBS = BSDF_PDF(lb, model, c.gb, c.normal, rd, gs.dir);
                             sc.next = gs.dir;
-                            brdf[0] = EvaluateBSDFWithCos(lb, model, rd, gs.dir, c);
+                            for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], model, rd, gs.dir, c);
                             sc.pdf = (1.0f - alpha) * std::fmax(pBS, 1.0e-12f) + alpha * gs.pdf;
                         } else {
                             probe = nullptr;
@@ -751,7 +767,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                 }
             }
         }
-        float endpoint = 0.0f;
+        float endpoint[MAXL] = {};
         if (c.lightIndex >= 0) {
             float misW = 1.0f;
             if (P.use_nee && depth > 0 && prevNEE) {
@@ -761,9 +777,10 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             }
             for (int j = 0; j < nl; ++j)
                 radiance[j] = radiance[j] + mask[j] * InteractC(wv.lb[j], sc.emissive) * wv.d65[j] * misW;
-            if (guiding) endpoint = InteractC(lb, sc.emissive * misW) * wv.d65[0];
+            if (guiding)
+                for (int j = 0; j < nl; ++j) endpoint[j] = InteractC(wv.lb[j], sc.emissive * misW) * wv.d65[j];
         } else if (guiding && !(sc.emissive.x == 0.0f && sc.emissive.y == 0.0f && sc.emissive.z == 0.0f)) {
-            endpoint = InteractC(lb, sc.emissive) * wv.d65[0];
+            for (int j = 0; j < nl; ++j) endpoint[j] = InteractC(wv.lb[j], sc.emissive) * wv.d65[j];
         }
         if (guiding) col.Endpoint(endpoint, ndd);
         if (sc.terminate) break;
@@ -833,14 +850,17 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         }
         if (guiding) {
             const bool allowStart = !sc.delta && !sc.pureLight && !inside;
-            col.Advance(sc.next, contrib[0], p, ndd, allowStart, rd, c.pos, c.normal);
+            col.Advance(sc.next, contrib, p, ndd, allowStart, rd, c.pos, c.normal);
         }
         for (int j = 0; j < nl; ++j) {
             mask[j] = mask[j] * contrib[j];
             mask[j] = mask[j] / p;
         }
         if (guiding) {
-            query = fmax3(wv.xyz[0] * mask[0], float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
+            // maskForProbeQuery: the throughput's XYZ (one wavelength: xyz * mask).
+            float3 t = wv.xyz[0] * mask[0];
+            for (int j = 1; j < nl; ++j) t = t + wv.xyz[j] * mask[j];
+            query = fmax3(t, float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
             // sum3 + PROBES_QUERY_EPS is a double sum (the constant is a double literal).
             query = query / static_cast<float>(static_cast<double>(query.x + query.y + query.z) + 1.0e-6);
         }
@@ -848,7 +868,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         rd = sc.next;
         if (!sc.delta) ndd++;
     }
-    if (guiding && slots) col.Emit(wv.xyz[0], wv.psum, slots);
+    if (guiding && slots) col.Emit(wv, slots);
     float r[MAXL] = {};
     for (int j = 0; j < nl; ++j) {
         r[j] = radiance[j];
@@ -1176,7 +1196,6 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
     const bool inside0 = U.inside0;
     const int64_t npix = static_cast<int64_t>(U.W) * U.H;
     const int nl = prm->wavelengths == 4 ? 4 : 1;
-    if (nl != 1 && prm->use_guiding) return -1;
     if (prm->use_guiding) {
         // Frames in order; each renders with the probes built from the frames
         // before it, then folds its own samples in (scene.cu:308-386).
@@ -1191,7 +1210,7 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
                 const float3 rd = CameraRay(U, rng, p);
                 int prim = -1;
                 float3 res = CalculateRadiance(S, P, rng, U.camO, rd, inside0, &prim, &G, frame,
-                                               &slots[static_cast<size_t>(p) * PG_COLLECT], 1);
+                                               &slots[static_cast<size_t>(p) * PG_COLLECT], nl);
                 res = fmax3(float3{0, 0, 0}, res);
                 const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
                 const float3 acc = lerp(prev, res, 1.0f / static_cast<float>(1 + frame));

@@ -14,7 +14,8 @@ mathematics fixes, independent of the restated code:
     and FP64 count-weighted means a numpy restatement of the merge expects
     (scene.cu:583-744, guiding.cuh:349-506);
   * guiding only changes the sampling pdf, so the gray floor under a square
-    light still converges to its closed form.
+    light still converges to its closed form (one and four wavelengths per
+    path).
 """
 import math
 
@@ -175,14 +176,17 @@ def test_build_step_against_numpy_merge():
 
 
 def test_guided_floor_matches_closed_form():
-    """Guiding (alpha = 0.5 from frame 64 on) must leave the estimate unbiased."""
+    """Guiding (alpha = 0.5 from frame 64 on) must leave the estimate unbiased,
+    with one wavelength per path and with four."""
     objs = rc.floor_and_light(0.5, 10.0, a=0.25, h=1.0)
     expected = 0.5 * 10.0 * 4 * rc.corner_form_factor(0.25, 0.25)
     cam = ap.Camera.make([0.0, 0.5, 0.0], [0.0, -1.0, 0.001], 1.0, 4, 4)
     bvh = ol.OracleBVH(objs)
-    g = ol.Guide()
-    prm = ap.RenderParams.defaults(spp=2048, frame_index=64, max_depth=2, sky=0, use_nee=0, use_guiding=1)
-    hdr, sums, prim = bvh.render(cam, prm, cull_margin=2 ** -12, guide=g)
-    assert g.probe_count() > 0
-    y = float((sums[..., 1] / 2048).mean())
-    assert abs(y / expected - 1.0) < 0.05, (y, expected)
+    for nl, spp in ((1, 2048), (4, 1024)):
+        g = ol.Guide()
+        prm = ap.RenderParams.defaults(spp=spp, frame_index=64, max_depth=2, sky=0, use_nee=0, use_guiding=1,
+                                       wavelengths=nl)
+        hdr, sums, prim = bvh.render(cam, prm, cull_margin=2 ** -12, guide=g)
+        assert g.probe_count() > 0
+        y = float((sums[..., 1] / spp).mean())
+        assert abs(y / expected - 1.0) < 0.05, (nl, y, expected)

@@ -67,3 +67,12 @@ def test_guided_heightfield_ggx(gpu):
     objs = ap.synth_heightfield_scene(64, 16, 4, 20.0)
     cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 64, 36)
     guided_parity(objs, cam, dict(spp=3, max_depth=16, rr_min_depth=4), "heightfield 64^2 + 16 spheres")
+
+
+def test_guided_four_wavelengths(gpu):
+    """Guiding with four hero wavelengths per path: per-wavelength collection,
+    the throughput's XYZ as the probe query, sum_j xyz_j L_j / psum as the
+    sample radiance, all bit-identical to the oracle."""
+    objs = ap.synth_heightfield_scene(48, 8, 5, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 48, 27)
+    guided_parity(objs, cam, dict(spp=3, max_depth=16, rr_min_depth=4, wavelengths=4), "heightfield, four wavelengths")

@@ -183,9 +183,6 @@ def test_render_rejects_unsupported_and_bad_arguments(gpu):
     with pytest.raises(ap.ApertureError) as e:
         scene.render(cam, ap.RenderParams.defaults(wavelengths=3), hdr)
     assert e.value.code == ap.ERR_INVALID_ARG
-    with pytest.raises(ap.ApertureError) as e:
-        scene.render(cam, ap.RenderParams.defaults(wavelengths=4, use_guiding=1), hdr)
-    assert e.value.code == ap.ERR_UNSUPPORTED
     gscene.close()
     scene.close()
 
