@@ -215,8 +215,9 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     320-triangle icospheres; Lambert/GGX 50/50; lit by 4 thin-cube lights, no
     sky), max depth 16, Russian roulette from depth 4. Measured with one
     wavelength per path (the reference's estimator), with four hero
-    wavelengths per path, and with directional guiding (one wavelength;
-    probes rebuilt after every frame, the rebuild inside the timed region).
+    wavelengths per path, and with directional guiding (one and four
+    wavelengths; probes rebuilt after every frame, the rebuild inside the
+    timed region).
     Frames are timed with HIP events on the render stream after one untimed
     frame."""
     t0 = time.perf_counter()
@@ -259,6 +260,11 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     stream.synchronize()
     guided = timed(use_guiding=1)
     guided["probes"] = len(scene.guiding_export()[1])
+    scene.guiding_reset()
+    scene.render(cam, ap.RenderParams.defaults(spp=1, use_guiding=1, wavelengths=4, **params), hdr, stream=stream)
+    stream.synchronize()
+    guided4 = timed(use_guiding=1, wavelengths=4)
+    guided4["probes"] = len(scene.guiding_export()[1])
     out = {
         "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 thin-cube lights, "
                     "no sky, 1920x1080, max depth 16, RR from depth 4",
@@ -266,6 +272,7 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
         "one_wavelength": one,
         "four_wavelengths": four,
         "guided_one_wavelength": guided,
+        "guided_four_wavelengths": guided4,
     }
     log(f"render: {out}")
     if nrc_spp > 0:
