@@ -37,6 +37,13 @@ double or_nrc_loss_fixed(const float* params, const float* feat, const float* ta
                          int round_bf16);
 void or_nrc_adam(float* params, float* m, float* v, const float* grad, int t, float lr, float beta1, float beta2,
                  float eps);
+size_t or_nrc_params_w(int width);
+void or_nrc_forward_w(const float* params, int width, const float* feat, size_t n, int mode, float* out);
+double or_nrc_grad_w(const float* params, int width, const float* feat, const float* target, size_t n, int mode,
+                     double* grad);
+double or_nrc_loss_fixed_w(const float* params, int width, const float* feat, const float* target, const double* den,
+                           size_t n, int mode);
+void or_f16_round(const float* x, size_t n, float* out);
 typedef struct or_guide or_guide;
 void or_pg_uv_to_sphere(const float* uv, size_t n, float* out);
 void or_pg_sphere_to_uv(const float* d, size_t n, float* out);

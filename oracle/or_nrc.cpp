@@ -10,14 +10,18 @@
 //   axis (36), outgoing direction (3), normal (3), roughness (1), albedo (3),
 //   a constant 1, zero padding; 5 hidden layers of 64; 3 outputs (RGB).
 // Parameters (21504 floats): 5 hidden matrices W[o][i] 64x64, then the output
-// matrix 16x64 whose rows 3..15 stay zero.
+// matrix 16x64 whose rows 3..15 stay zero. The width-w variant (the *_w
+// functions; w = 32 or 64) has W0 w x 64, W1..W4 w x w and the output 16 x w:
+// w*64 + 4*w*w + 16*w parameters.
 //
 // Arithmetic is FP64. With round_bf16 the values the core stores in BF16 are
 // rounded to BF16 (round to nearest even) at the same points: weights,
-// encoded inputs, hidden activations, output deltas and hidden deltas. The
-// accumulation order differs from the core's MFMA (FP32), so the core is
-// compared with a stated tolerance. Without round_bf16 the functions are the
-// exact FP64 network, pinned by finite differences in tests/test_nrc_cpu.py.
+// encoded inputs, hidden activations, output deltas and hidden deltas; the
+// *_w functions take a precision (0 none, 1 BF16, 2 FP16 with round to
+// nearest even, subnormals and overflow to infinity). The accumulation order
+// differs from the core's MFMA (FP32), so the core is compared with a stated
+// tolerance. Without rounding the functions are the exact FP64 network,
+// pinned by finite differences in tests/test_nrc_cpu.py.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -30,6 +34,61 @@ namespace {
 
 constexpr int IN = 64, H = 64, NH = 5, OUTP = 16, NOUT = 3;
 constexpr int PARAMS = NH * H * IN + OUTP * H;  // 21504
+
+// FP32 -> IEEE binary16 -> FP32, round to nearest even.
+double f16(double x) {
+    float f = static_cast<float>(x);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const int32_t exp = static_cast<int32_t>((u >> 23) & 0xffu);
+    uint32_t mant = u & 0x7fffffu;
+    uint32_t h;
+    if (exp == 0xff) {
+        h = sign | 0x7c00u | (mant ? 0x200u : 0u);
+    } else {
+        const int32_t e = exp - 127 + 15;
+        if (e >= 31) {
+            h = sign | 0x7c00u;
+        } else if (e <= 0) {
+            if (e < -10) {
+                h = sign;
+            } else {
+                mant |= 0x800000u;
+                const int shift = 14 - e;
+                uint32_t v = mant >> shift;
+                const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+                if (rem > half || (rem == half && (v & 1u))) ++v;
+                h = sign | v;
+            }
+        } else {
+            uint32_t v = (static_cast<uint32_t>(e) << 10) | (mant >> 13);
+            const uint32_t rem = mant & 0x1fffu;
+            if (rem > 0x1000u || (rem == 0x1000u && (v & 1u))) ++v;
+            h = sign | v;  // a carry into the exponent rounds up correctly, to infinity at the top
+        }
+    }
+    // binary16 -> float
+    const uint32_t hs = (h & 0x8000u) << 16, he = (h >> 10) & 0x1fu, hm = h & 0x3ffu;
+    uint32_t out;
+    if (he == 0) {
+        if (hm == 0) {
+            out = hs;
+        } else {
+            int k = 0;
+            uint32_t m = hm;
+            while (!(m & 0x400u)) { m <<= 1; ++k; }
+            out = hs | (static_cast<uint32_t>(127 - 15 - k + 1) << 23) | ((m & 0x3ffu) << 13);
+        }
+    } else if (he == 31) {
+        out = hs | 0x7f800000u | (hm << 13);
+    } else {
+        out = hs | ((he - 15 + 127) << 23) | (hm << 13);
+    }
+    float r;
+    std::memcpy(&r, &out, 4);
+    return r;
+}
 
 double bf16(double x) {
     float f = static_cast<float>(x);
@@ -73,9 +132,132 @@ void forward(const Net& net, const float* feat, bool rnd, double acts[NH + 1][H]
     }
 }
 
+// Width-w network with a rounding mode (0 none, 1 BF16, 2 FP16).
+double rnd(double x, int mode) { return mode == 1 ? bf16(x) : (mode == 2 ? f16(x) : x); }
+
+struct NetW {
+    int w;
+    std::vector<double> m[NH + 1];  // m[l][o * in + i]
+    int in(int l) const { return l == 0 ? IN : w; }
+    int out(int l) const { return l < NH ? w : OUTP; }
+    NetW(const float* params, int width, int mode) : w(width) {
+        size_t off = 0;
+        for (int l = 0; l <= NH; ++l) {
+            m[l].resize(static_cast<size_t>(out(l)) * in(l));
+            for (size_t k = 0; k < m[l].size(); ++k) m[l][k] = rnd(params[off + k], mode);
+            off += m[l].size();
+        }
+    }
+};
+
+void forward_w(const NetW& net, const float* feat, int mode, std::vector<std::vector<double>>& acts, double y[NOUT]) {
+    acts.assign(NH + 1, std::vector<double>(std::max(IN, net.w), 0.0));
+    for (int i = 0; i < IN; ++i) acts[0][i] = rnd(feat[i], mode);
+    for (int l = 0; l < NH; ++l)
+        for (int o = 0; o < net.w; ++o) {
+            double s = 0.0;
+            for (int i = 0; i < net.in(l); ++i) s += net.m[l][static_cast<size_t>(o) * net.in(l) + i] * acts[l][i];
+            s = s > 0.0 ? s : 0.0;
+            acts[l + 1][o] = rnd(s, mode);
+        }
+    for (int o = 0; o < NOUT; ++o) {
+        double s = 0.0;
+        for (int i = 0; i < net.w; ++i) s += net.m[NH][static_cast<size_t>(o) * net.w + i] * acts[NH][i];
+        y[o] = mode ? static_cast<float>(s) : s;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t or_nrc_params_w(int width) {
+    return static_cast<size_t>(width) * IN + 4u * width * width + static_cast<size_t>(OUTP) * width;
+}
+
+void or_nrc_forward_w(const float* params, int width, const float* feat, size_t n, int mode, float* out) {
+    const NetW net(params, width, mode);
+    std::vector<std::vector<double>> acts;
+    double y[NOUT];
+    for (size_t r = 0; r < n; ++r) {
+        forward_w(net, feat + 64 * r, mode, acts, y);
+        for (int o = 0; o < NOUT; ++o) out[NOUT * r + o] = static_cast<float>(y[o]);
+    }
+}
+
+// Gradient of the mean relative L2 loss of the width-w network (as or_nrc_grad).
+double or_nrc_grad_w(const float* params, int width, const float* feat, const float* target, size_t n, int mode,
+                     double* grad) {
+    const NetW net(params, width, mode);
+    const size_t np = or_nrc_params_w(width);
+    for (size_t k = 0; k < np; ++k) grad[k] = 0.0;
+    size_t off[NH + 1];
+    {
+        size_t o = 0;
+        for (int l = 0; l <= NH; ++l) { off[l] = o; o += static_cast<size_t>(net.out(l)) * net.in(l); }
+    }
+    double loss = 0.0;
+    const double norm = 1.0 / (static_cast<double>(n) * NOUT);
+    std::vector<std::vector<double>> acts;
+    double y[NOUT];
+    const int w = net.w;
+    for (size_t r = 0; r < n; ++r) {
+        forward_w(net, feat + 64 * r, mode, acts, y);
+        double dy[OUTP] = {0.0};
+        for (int o = 0; o < NOUT; ++o) {
+            const double t = target[NOUT * r + o];
+            const double den = y[o] * y[o] + 0.01;
+            const double e = y[o] - t;
+            loss += e * e / den * norm;
+            dy[o] = rnd(2.0 * e / den * norm, mode);
+        }
+        for (int o = 0; o < NOUT; ++o)
+            for (int i = 0; i < w; ++i) grad[off[NH] + static_cast<size_t>(o) * w + i] += dy[o] * acts[NH][i];
+        std::vector<double> delta(w), prev(w);
+        for (int i = 0; i < w; ++i) {
+            double s = 0.0;
+            for (int o = 0; o < NOUT; ++o) s += net.m[NH][static_cast<size_t>(o) * w + i] * dy[o];
+            s = acts[NH][i] > 0.0 ? s : 0.0;
+            delta[i] = rnd(s, mode);
+        }
+        for (int l = NH - 1; l >= 0; --l) {
+            const int nin = net.in(l);
+            for (int o = 0; o < w; ++o)
+                for (int i = 0; i < nin; ++i) grad[off[l] + static_cast<size_t>(o) * nin + i] += delta[o] * acts[l][i];
+            if (l == 0) break;
+            for (int i = 0; i < w; ++i) {
+                double s = 0.0;
+                for (int o = 0; o < w; ++o) s += net.m[l][static_cast<size_t>(o) * w + i] * delta[o];
+                s = acts[l][i] > 0.0 ? s : 0.0;
+                prev[i] = rnd(s, mode);
+            }
+            delta = prev;
+        }
+    }
+    return loss;
+}
+
+// Loss of the width-w network with the denominators held at `den` (FP64).
+double or_nrc_loss_fixed_w(const float* params, int width, const float* feat, const float* target, const double* den,
+                           size_t n, int mode) {
+    const NetW net(params, width, mode);
+    std::vector<std::vector<double>> acts;
+    double y[NOUT], loss = 0.0;
+    const double norm = 1.0 / (static_cast<double>(n) * NOUT);
+    for (size_t r = 0; r < n; ++r) {
+        forward_w(net, feat + 64 * r, mode, acts, y);
+        for (int o = 0; o < NOUT; ++o) {
+            const double e = y[o] - target[NOUT * r + o];
+            loss += e * e / den[NOUT * r + o] * norm;
+        }
+    }
+    return loss;
+}
+
+// FP16 round trip of FP32 values (round to nearest even), for its CPU pins.
+void or_f16_round(const float* x, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) out[i] = static_cast<float>(f16(x[i]));
+}
 
 // Encoding of query records (16 floats: pos 3, dir 3, normal 3, roughness,
 // albedo 3, pad 3) into 64 features, with the core's FP32 operations.

@@ -72,6 +72,14 @@ def lib():
         L.or_nrc_loss_fixed.restype = ctypes.c_double
         L.or_nrc_loss_fixed.argtypes = [vp, vp, vp, vp, sz, i32]
         L.or_nrc_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32]
+        L.or_nrc_params_w.restype = sz
+        L.or_nrc_params_w.argtypes = [i32]
+        L.or_nrc_forward_w.argtypes = [vp, i32, vp, sz, i32, vp]
+        L.or_nrc_grad_w.restype = ctypes.c_double
+        L.or_nrc_grad_w.argtypes = [vp, i32, vp, vp, sz, i32, vp]
+        L.or_f16_round.argtypes = [vp, sz, vp]
+        L.or_nrc_loss_fixed_w.restype = ctypes.c_double
+        L.or_nrc_loss_fixed_w.argtypes = [vp, i32, vp, vp, vp, sz, i32]
         data = os.path.join(REPO, "aperturerendererv3-optix-geometry_amd", "data")
         rc = L.or_spectrum_load(os.path.join(data, "spectrum_mlp.bin").encode(),
                                 os.path.join(data, "cie_tables.bin").encode())
@@ -258,6 +266,42 @@ def nrc_adam(params, m, v, grad, t, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8):
     params, m, v = (np.ascontiguousarray(a, np.float32).copy() for a in (params, m, v))
     lib().or_nrc_adam(_p(params), _p(m), _p(v), _p(_f32(grad)), t, lr, beta1, beta2, eps)
     return params, m, v
+
+
+NRC_NONE, NRC_BF16, NRC_FP16 = 0, 1, 2
+
+
+def nrc_params_w(width):
+    return int(lib().or_nrc_params_w(width))
+
+
+def nrc_forward_w(params, width, feat, mode):
+    feat = _f32(feat, (-1, 64))
+    out = np.zeros((len(feat), 3), np.float32)
+    lib().or_nrc_forward_w(_p(_f32(params)), width, _p(feat), len(feat), mode, _p(out))
+    return out
+
+
+def nrc_grad_w(params, width, feat, target, mode):
+    feat = _f32(feat, (-1, 64))
+    grad = np.zeros(nrc_params_w(width), np.float64)
+    loss = lib().or_nrc_grad_w(_p(_f32(params)), width, _p(feat), _p(_f32(target, (-1, 3))), len(feat), mode,
+                               _p(grad))
+    return loss, grad
+
+
+def nrc_loss_fixed_w(params, width, feat, target, den, mode=0):
+    feat = _f32(feat, (-1, 64))
+    den = np.ascontiguousarray(den, np.float64).reshape(-1)
+    return lib().or_nrc_loss_fixed_w(_p(_f32(params)), width, _p(feat), _p(_f32(target, (-1, 3))), _p(den),
+                                     len(feat), mode)
+
+
+def f16_round(x):
+    x = _f32(x).reshape(-1)
+    out = np.zeros_like(x)
+    lib().or_f16_round(_p(x), len(x), _p(out))
+    return out
 
 
 class Guide:

@@ -125,3 +125,59 @@ def test_training_fits_a_smooth_target():
         p, m, v = ol.nrc_adam(p, m, v, g.astype(np.float32), t, lr=np.float32(3e-3))
     final, _ = ol.nrc_grad(p, feat, target, round_bf16=True)
     assert final < 0.1 * first, (first, final)
+
+
+def init_params_w(width, seed=0):
+    g = np.random.default_rng(seed)
+    n = ol.nrc_params_w(width)
+    p = np.zeros(n, np.float32)
+    p[:width * 64] = g.normal(0, np.sqrt(2.0 / 64), width * 64)
+    p[width * 64:width * 64 + 4 * width * width] = g.normal(0, np.sqrt(2.0 / width), 4 * width * width)
+    out = p[width * 64 + 4 * width * width:].reshape(16, width)
+    out[:3] = g.normal(0, np.sqrt(1.0 / width), (3, width))
+    return p
+
+
+def test_f16_rounding_matches_numpy():
+    g = np.random.default_rng(12)
+    x = np.concatenate([g.normal(0, 1, 20000), g.normal(0, 1e-5, 5000), g.normal(0, 1e4, 5000),
+                        [65504.0, 65519.0, 65520.0, 1e6, -1e6, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26, 0.0, -0.0],
+                        np.float32(1.0) + np.arange(-4, 5) * 2.0 ** -11]).astype(np.float32)
+    got = ol.f16_round(x)
+    with np.errstate(over="ignore"):  # values past 65520 round to infinity on purpose
+        want = x.astype(np.float16).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_general_width64_bf16_equals_the_specialised_network():
+    p = init_params(41)
+    feat = ol.nrc_encode(random_queries(64, seed=42), BMIN, BMAX)
+    t = G.uniform(0, 2, (64, 3)).astype(np.float32)
+    assert ol.nrc_params_w(64) == ol.NRC_PARAMS and ol.nrc_params_w(32) == 32 * 64 + 4 * 32 * 32 + 16 * 32
+    assert np.array_equal(ol.nrc_forward_w(p, 64, feat, ol.NRC_BF16), ol.nrc_forward(p, feat, round_bf16=True))
+    l1, g1 = ol.nrc_grad_w(p, 64, feat, t, ol.NRC_BF16)
+    l2, g2 = ol.nrc_grad(p, feat, t, round_bf16=True)
+    assert l1 == l2 and np.array_equal(g1, g2)
+
+
+def test_width32_gradient_matches_finite_differences():
+    p = init_params_w(32, seed=43)
+    feat = ol.nrc_encode(random_queries(16, seed=44), BMIN, BMAX)
+    t = G.uniform(0.1, 1.5, (16, 3)).astype(np.float32)
+    loss, grad = ol.nrc_grad_w(p, 32, feat, t, ol.NRC_NONE)
+    y = ol.nrc_forward_w(p, 32, feat, ol.NRC_NONE).astype(np.float64)
+    # The denominator y^2 + 0.01 is a stop-gradient: hold it at the base point.
+    den = y ** 2 + 0.01
+    assert abs(ol.nrc_loss_fixed_w(p, 32, feat, t, den) / loss - 1.0) < 1e-6
+    rng = np.random.default_rng(45)
+    picks = np.concatenate([rng.integers(0, 2048, 6), rng.integers(2048, 2048 + 4096, 6),
+                            rng.integers(2048 + 4096, 2048 + 4096 + 96, 4)])
+    for k in picks:
+        q1, q2 = p.copy(), p.copy()
+        q1[k] += np.float32(1e-5)
+        q2[k] -= np.float32(1e-5)
+        fd = (ol.nrc_loss_fixed_w(q1, 32, feat, t, den) - ol.nrc_loss_fixed_w(q2, 32, feat, t, den)) / \
+            (float(q1[k]) - float(q2[k]))
+        assert abs(fd - grad[k]) < 1e-4 * max(1e-3, abs(grad[k])) + 1e-6, (k, fd, grad[k])
+    assert np.all(grad[2048 + 4096 + 3 * 32:] == 0.0)
+    assert ol.nrc_params_w(32) == 6656
