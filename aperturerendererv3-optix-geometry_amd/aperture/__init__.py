@@ -172,7 +172,8 @@ EXPORTED_SYMBOLS = (
     "ap_synth_heightfield_scene", "ap_guiding_reset", "ap_guiding_export",
     "ap_nrc_create", "ap_nrc_destroy", "ap_nrc_set_bounds", "ap_nrc_get_params", "ap_nrc_set_params",
     "ap_nrc_infer", "ap_nrc_train_step", "ap_nrc_gradient", "ap_nrc_last_gradient", "ap_nrc_get_bounds",
-    "ap_nrc_set_ema", "ap_nrc_render_params_default",
+    "ap_nrc_set_ema", "ap_nrc_render_params_default", "ap_nrc_create_grouped", "ap_nrc_info",
+    "ap_nrc_set_group_bounds", "ap_nrc_infer_grouped", "ap_nrc_gradient_grouped", "ap_nrc_train_step_grouped",
     "ap_batch_create", "ap_batch_destroy", "ap_batch_render", "ap_batch_set_launch", "ap_batch_get_launch",
     "ap_batch_set_graph", "ap_batch_autotune", "ap_render_nrc", "ap_nrc_last_frame",
 )
@@ -206,6 +207,12 @@ def lib():
         L.ap_nrc_last_gradient.argtypes = [vp, vp]
         L.ap_nrc_get_bounds.argtypes = [vp, vp, vp]
         L.ap_nrc_set_ema.argtypes = [vp, ctypes.c_float]
+        L.ap_nrc_create_grouped.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64, vp]
+        L.ap_nrc_info.argtypes = [vp, vp, vp, vp, vp]
+        L.ap_nrc_set_group_bounds.argtypes = [vp, ctypes.c_uint32, vp, vp]
+        L.ap_nrc_infer_grouped.argtypes = [vp, vp, vp, vp, vp]
+        L.ap_nrc_gradient_grouped.argtypes = [vp, vp, sz, vp, vp, vp]
+        L.ap_nrc_train_step_grouped.argtypes = [vp, vp, sz, ctypes.c_float, vp]
         L.ap_batch_create.argtypes = [vp, vp, sz, vp]
         L.ap_batch_destroy.argtypes = [vp]
         L.ap_batch_render.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -497,20 +504,44 @@ NRC_PARAMS = 21504
 
 
 class RadianceCache:
-    """The learned radiance cache (include/aperture/ap_nrc.h)."""
+    """The learned radiance cache (include/aperture/ap_nrc.h): one network,
+    or `groups` independent networks of hidden width 32 or 64 in "bf16" or
+    "fp16" served by grouped launches."""
 
-    def __init__(self, seed=1):
+    PRECISIONS = {"bf16": 0, "fp16": 1}
+
+    def __init__(self, seed=1, groups=1, width=64, precision="bf16"):
         self._h = ctypes.c_void_p()
-        check(lib().ap_nrc_create(seed, ctypes.byref(self._h)))
+        if groups == 1 and width == 64 and precision == "bf16":
+            check(lib().ap_nrc_create(seed, ctypes.byref(self._h)))
+        else:
+            check(lib().ap_nrc_create_grouped(groups, width, self.PRECISIONS[precision], seed,
+                                              ctypes.byref(self._h)))
+        g, w, pr, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32(), ctypes.c_size_t()
+        check(lib().ap_nrc_info(self.handle, ctypes.byref(g), ctypes.byref(w), ctypes.byref(pr), ctypes.byref(n)))
+        self.groups, self.width, self.n_params = g.value, w.value, n.value
+        self.precision = "fp16" if pr.value == 1 else "bf16"
 
     @property
     def handle(self):
         return self._h.value
 
-    def set_bounds(self, bmin, bmax):
+    def loss_scale(self, batch):
+        """Power of two an FP16 network's loss gradient is scaled by (1 for BF16)."""
+        if self.precision != "fp16":
+            return 1.0
+        k = 0
+        while (1 << k) < 3 * batch:
+            k += 1
+        return 2.0 ** (k - 4)
+
+    def set_bounds(self, bmin, bmax, group=None):
         lo = np.ascontiguousarray(bmin, np.float32)
         hi = np.ascontiguousarray(bmax, np.float32)
-        check(lib().ap_nrc_set_bounds(self.handle, _ptr(lo), _ptr(hi)))
+        if group is None:
+            check(lib().ap_nrc_set_bounds(self.handle, _ptr(lo), _ptr(hi)))
+        else:
+            check(lib().ap_nrc_set_group_bounds(self.handle, group, _ptr(lo), _ptr(hi)))
 
     def set_ema(self, decay):
         check(lib().ap_nrc_set_ema(self.handle, decay))
@@ -520,13 +551,18 @@ class RadianceCache:
         check(lib().ap_nrc_get_bounds(self.handle, _ptr(lo), _ptr(hi)))
         return lo, hi
 
+    def _shape(self, a):
+        return a if self.groups == 1 else a.reshape(self.groups, self.n_params)
+
     def params(self):
-        p = np.zeros(NRC_PARAMS, np.float32)
+        p = np.zeros(self.groups * self.n_params, np.float32)
         check(lib().ap_nrc_get_params(self.handle, _ptr(p)))
-        return p
+        return self._shape(p)
 
     def set_params(self, p):
-        host = np.ascontiguousarray(p, np.float32)
+        host = np.ascontiguousarray(p, np.float32).reshape(-1)
+        if host.size != self.groups * self.n_params:
+            raise ValueError(f"expected {self.groups * self.n_params} parameters, got {host.size}")
         check(lib().ap_nrc_set_params(self.handle, _ptr(host)))
 
     def infer(self, queries):
@@ -537,21 +573,41 @@ class RadianceCache:
         synchronize()
         return d_o.to_numpy(np.float32, len(q) * 3).reshape(-1, 3)
 
+    def infer_grouped(self, queries, seg):
+        """Host (n, 16) records, rows [seg[g], seg[g+1]) for network g -> host (n, 3)."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
+        sg = np.ascontiguousarray(seg, np.uint32)
+        d_q, d_o = DeviceBuffer.from_numpy(q if len(q) else np.zeros((1, 16), np.float32)), \
+            DeviceBuffer(max(len(q) * 12, 4))
+        check(lib().ap_nrc_infer_grouped(self.handle, d_q.ptr, _ptr(sg), d_o.ptr, None))
+        synchronize()
+        return d_o.to_numpy(np.float32, len(q) * 3).reshape(-1, 3)
+
     def gradient(self, samples):
-        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 20)
-        d_s = DeviceBuffer.from_numpy(s)
-        g = np.zeros(NRC_PARAMS, np.float32)
-        loss = ctypes.c_float()
-        check(lib().ap_nrc_gradient(self.handle, d_s.ptr, len(s), _ptr(g), ctypes.byref(loss), None))
-        return float(loss.value), g
+        """Loss and gradient over host samples: (n, 20) for one network,
+        (groups, batch, 20) for a grouped cache (losses per network)."""
+        s = np.ascontiguousarray(samples, np.float32)
+        d_s = DeviceBuffer.from_numpy(s.reshape(-1, 20))
+        g = np.zeros(self.groups * self.n_params, np.float32)
+        loss = np.zeros(self.groups, np.float32)
+        if self.groups == 1:
+            check(lib().ap_nrc_gradient(self.handle, d_s.ptr, s.size // 20, _ptr(g), _ptr(loss), None))
+            return float(loss[0]), g
+        batch = s.size // 20 // self.groups
+        check(lib().ap_nrc_gradient_grouped(self.handle, d_s.ptr, batch, _ptr(g), _ptr(loss), None))
+        return loss, self._shape(g)
 
     def train_step(self, samples_device, n, lr=1e-3, stream=None):
-        check(lib().ap_nrc_train_step(self.handle, _ptr(samples_device), n, lr, _stream(stream)))
+        """One Adam step; n is the batch per network."""
+        if self.groups == 1:
+            check(lib().ap_nrc_train_step(self.handle, _ptr(samples_device), n, lr, _stream(stream)))
+        else:
+            check(lib().ap_nrc_train_step_grouped(self.handle, _ptr(samples_device), n, lr, _stream(stream)))
 
     def last_gradient(self):
-        g = np.zeros(NRC_PARAMS, np.float32)
+        g = np.zeros(self.groups * self.n_params, np.float32)
         check(lib().ap_nrc_last_gradient(self.handle, _ptr(g)))
-        return g
+        return self._shape(g)
 
     def close(self):
         if self._h is not None and self._h.value:

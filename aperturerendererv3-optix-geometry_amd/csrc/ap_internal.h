@@ -141,6 +141,8 @@ hipError_t launch_trace_multi(const DeviceScene& shape, const DeviceScene* scene
 // Radiance cache entry points for the integrator (ap_nrc.hip).
 struct NrcState;
 NrcState* nrc_state(ap_nrc* h);
+// Networks in the cache (1 for a plain cache).
+uint32_t nrc_groups(const NrcState* s);
 void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]);
 // Inference over min(*dyn_n, n_cap) query records (16 floats each); output
 // row r is record rows[r].

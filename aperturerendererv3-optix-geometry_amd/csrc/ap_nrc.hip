@@ -1,7 +1,12 @@
-// Learned radiance cache: a 64 -> 64 x 5 -> 3 ReLU network without biases
-// (neural radiance caching, Mueller et al. 2021), BF16 weights and
-// activations, FP32 accumulation on gfx950 MFMA (v_mfma_f32_16x16x32_bf16),
-// trained online with a relative L2 loss and Adam (include/aperture/ap_nrc.h).
+// Learned radiance cache: 64 input features -> W x 5 -> 3 ReLU networks
+// without biases (neural radiance caching, Mueller et al. 2021) of hidden
+// width W = 64 or 32, BF16 or FP16 weights and activations, FP32
+// accumulation on gfx950 MFMA (v_mfma_f32_16x16x32_bf16 / _f16), trained
+// online with a relative L2 loss and Adam (include/aperture/ap_nrc.h). A
+// cache holds G independent networks of one shape (one per scene of a
+// batch); every kernel serves all of them in one launch as a grouped GEMM:
+// a 64-row tile belongs to one group and the block loads that group's
+// weights.
 //
 // Every layer is computed transposed, y^T = W x^T, so one MFMA tile is 16
 // outputs x 16 batch rows x 32 inputs:
@@ -10,15 +15,19 @@
 //   C (16 x 16): lane l holds y[row = l%16][o = 16t + 4(l/16) .. +3]
 // Both operands are one 16-byte load (weights and the wave's activation tile
 // live in LDS, rows padded to 72 elements against bank conflicts); C goes
-// back to LDS as the next layer's B after ReLU and BF16 rounding.
+// back to LDS as the next layer's B after ReLU and rounding.
 //
 // Training keeps the activations and deltas of a batch in HBM as
-// [feature][row] BF16 so that the weight gradient dW = delta^T a is again
-// one 16-byte load per operand with the batch as the MFMA K dimension:
+// [feature][row] 16-bit values so that the weight gradient dW = delta^T a is
+// again one 16-byte load per operand with the batch as the MFMA K dimension:
 //   A: lane l holds delta[o = 16t + l%16][rows r0 + 8(l/16) .. +7]
 //   B: lane l holds a[i = 16u + l%16][rows r0 + 8(l/16) .. +7]
-// Per-chunk partial gradients are summed in chunk order (deterministic), and
-// Adam updates the FP32 master weights and rewrites the BF16 copies.
+// The rows of a training batch are laid out [group][batch]. Per-chunk
+// partial gradients are summed in chunk order (deterministic), and Adam
+// updates the FP32 master weights and rewrites the 16-bit copies. FP16
+// networks scale the loss gradient by a power of two (exactly undone on the
+// FP32 gradient) so that it stays clear of FP16's subnormal range, and
+// saturate conversions at +-65504.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -32,22 +41,59 @@ namespace ap {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int NF = 64;          // features per layer
-constexpr int NL = 5;           // hidden layers
-constexpr int NOP = 16;         // padded output rows
+constexpr int NF = 64;                         // input features
+constexpr int NL = 5;                          // hidden layers
+constexpr int NOP = 16;                        // padded output rows
 constexpr int NOUT = 3;
-constexpr int NPAR = NL * NF * NF + NOP * NF;  // 21504
-constexpr int OUT_OFF = NL * NF * NF;          // 20480
 constexpr int WT_OUT_STRIDE = 32;              // transposed output weights: [i][o < 32]
-constexpr int NPAR_T = NL * NF * NF + NF * WT_OUT_STRIDE;
 constexpr int LS = 72;                         // padded LDS row (elements)
 constexpr int QREC = 16;                       // floats per query record
 constexpr int SREC = 20;                       // floats per training sample (query + target rgb + pad)
 constexpr int kChunkRows = 256;                // rows per weight-gradient partial
+constexpr uint32_t kMaxGroups = 4096;
 constexpr float kPiF = 3.14159265358979323846f;
+
+// Parameter layout of a width-W network: W x 64 input matrix, four W x W
+// hidden matrices, the 16 x W output matrix (rows 3..15 zero), each W[o][i].
+template <int W>
+struct Geo {
+    static constexpr int NPAR = W * NF + 4 * W * W + NOP * W;
+    static constexpr int NPAR_T = W * NF + 4 * W * W + WT_OUT_STRIDE * W;
+    static constexpr int ROWS = NL * W + NOP;  // LDS rows of the forward weights
+    static constexpr int LT = W + 8;           // LDS row of a transposed hidden matrix
+    static constexpr int NACT = NF + NL * W;   // stored activation features per row
+    __host__ __device__ static constexpr int off(int l) { return l == 0 ? 0 : W * NF + (l - 1) * W * W; }
+    __host__ __device__ static constexpr int act(int l) { return l == 0 ? 0 : NF + (l - 1) * W; }
+    // Layer, output and input of parameter p.
+    __host__ __device__ static void decode(int p, int& l, int& o, int& i) {
+        if (p < W * NF) {
+            l = 0;
+            o = p / NF;
+            i = p % NF;
+            return;
+        }
+        p -= W * NF;
+        if (p < 4 * W * W) {
+            l = 1 + p / (W * W);
+            p %= W * W;
+            o = p / W;
+            i = p % W;
+            return;
+        }
+        p -= 4 * W * W;
+        l = NL;
+        o = p / W;
+        i = p % W;
+    }
+    // Index of parameter (l, o, i) in the transposed copy: W_l^T[i][o].
+    __host__ __device__ static int tindex(int l, int o, int i) {
+        return l < NL ? off(l) + i * W + o : off(NL) + i * WT_OUT_STRIDE + o;
+    }
+};
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -55,125 +101,225 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return static_cast<uint16_t>(u >> 16);
 }
-__host__ __device__ __forceinline__ uint16_t f2bf_h(float f) {
+uint16_t f2bf_h(float f) {
     uint32_t u;
-    __builtin_memcpy(&u, &f, 4);
+    std::memcpy(&u, &f, 4);
     if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return 0x7fc0;
     u += 0x7fffu + ((u >> 16) & 1u);
     return static_cast<uint16_t>(u >> 16);
 }
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
-
-__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
-    const u16x8 v = *reinterpret_cast<const u16x8*>(p);
-    return __builtin_bit_cast(bf16x8, v);
+// FP32 -> FP16, round to nearest even, saturated at +-65504 (host).
+uint16_t f2h_h(float f) {
+    if (f > 65504.0f) f = 65504.0f;
+    if (f < -65504.0f) f = -65504.0f;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint16_t sign = static_cast<uint16_t>((u >> 16) & 0x8000u);
+    u &= 0x7fffffffu;
+    if (u > 0x7f800000u) return static_cast<uint16_t>(sign | 0x7e00u);
+    if (u >= 0x38800000u) {  // at least 2^-14: normal
+        uint32_t r = u - (112u << 23);
+        r += 0xfffu + ((r >> 13) & 1u);
+        return static_cast<uint16_t>(sign | (r >> 13));
+    }
+    if (u <= 0x33000000u) return sign;  // at most 2^-25: rounds to zero
+    const uint32_t e = u >> 23, m = (u & 0x7fffffu) | 0x800000u;
+    const uint32_t sh = 126u - e;  // value in units of 2^-24 is m / 2^sh
+    uint32_t q = m >> sh;
+    const uint32_t rem = m & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return static_cast<uint16_t>(sign | q);
 }
 
-__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+// Rounding of an FP32 value to the network's 16-bit type.
+template <bool F16>
+__device__ __forceinline__ uint16_t cvt(float f) {
+    if constexpr (F16) {
+        const float c = f > 65504.0f ? 65504.0f : (f < -65504.0f ? -65504.0f : f);
+        return __builtin_bit_cast(uint16_t, static_cast<_Float16>(c));
+    } else {
+        return f2bf(f);
+    }
+}
+// A stored activation is above zero (BF16 and FP16 share the sign bit).
+__device__ __forceinline__ bool positive(uint16_t h) { return (h & 0x8000u) == 0u && (h & 0x7fffu) != 0u; }
+
+__device__ __forceinline__ u16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const u16x8*>(p); }
+
+template <bool F16>
+__device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 c) {
+    if constexpr (F16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
+                                                      0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                       c, 0, 0, 0);
 }
 
 struct Bounds {
     float lo[3], hi[3];
 };
 
-// Encoding of one query record into 64 features (the oracle's or_nrc_encode).
-__device__ void encode(const float* s, const Bounds& b, float f[NF]) {
-    for (int k = 0; k < NF; ++k) f[k] = 0.0f;
-    for (int a = 0; a < 3; ++a) {
-        const float p = (s[a] - b.lo[a]) / (b.hi[a] - b.lo[a]);
-        for (int k = 0; k < 6; ++k) {
-            const float arg = p * (static_cast<float>(1 << k) * kPiF);
-            float sn, cs;
-            apfm::sincos(arg, sn, cs);
-            f[a * 12 + 2 * k] = sn;
-            f[a * 12 + 2 * k + 1] = cs;
+// Rows of each network. groups == 0: one network over rows [0, n).
+// rows_per_group != 0: the training layout, group g owns rows
+// [g * rows_per_group, (g + 1) * rows_per_group). Otherwise region g is rows
+// [seg[g], seg[g + 1]) of which the first count[g] are live (all without
+// `count`), cut into 64-row tiles numbered from tile0[g].
+struct GroupMap {
+    uint32_t groups = 0;
+    uint32_t rows_per_group = 0;
+    const uint32_t* seg = nullptr;
+    const uint32_t* tile0 = nullptr;
+    const uint32_t* count = nullptr;
+    const Bounds* bounds = nullptr;
+};
+
+// Group and live rows [begin, end) of tile v; false for a tile with no live row.
+__device__ __forceinline__ bool tile_rows(const GroupMap& gm, uint32_t v, uint32_t n, uint32_t& g, uint32_t& begin,
+                                          uint32_t& end) {
+    if (gm.groups == 0 || gm.rows_per_group != 0) {
+        begin = v * 64u;
+        g = gm.rows_per_group ? begin / gm.rows_per_group : 0u;
+        end = gm.rows_per_group ? min(n, (g + 1u) * gm.rows_per_group) : n;
+        return begin < end;
+    }
+    uint32_t lo = 0, hi = gm.groups;  // largest g with tile0[g] <= v
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (gm.tile0[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    g = lo;
+    begin = gm.seg[g] + (v - gm.tile0[g]) * 64u;
+    uint32_t size = gm.seg[g + 1] - gm.seg[g];
+    if (gm.count) size = min(size, gm.count[g]);
+    end = gm.seg[g] + size;
+    return begin < end;
+}
+
+__device__ __forceinline__ uint32_t tile_count(const GroupMap& gm, uint32_t n) {
+    if (gm.groups == 0 || gm.rows_per_group != 0) return (n + 63u) / 64u;
+    return gm.tile0[gm.groups];
+}
+
+// Features [16q, 16q + 16) of the encoding of one query record (the
+// oracle's or_nrc_encode): sin/cos pairs of the normalised position at
+// frequencies 2^k pi for features 0..35, then direction, normal, roughness,
+// albedo and a constant 1, zeros from 47.
+__device__ __forceinline__ float plain_feature(const float* s, int k) {
+    if (k < 39) return s[3 + (k - 36)];
+    if (k < 42) return s[6 + (k - 39)];
+    if (k == 42) return s[9];
+    if (k < 46) return s[10 + (k - 43)];
+    return k == 46 ? 1.0f : 0.0f;
+}
+__device__ void encode_part(const float* s, const Bounds& b, int q, float f[16]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = 16 * q + 2 * j;
+        float v0, v1;
+        if (k < 36) {
+            const int a = k / 12, fr = (k % 12) / 2;
+            const float p = (s[a] - b.lo[a]) / (b.hi[a] - b.lo[a]);
+            const float arg = p * (static_cast<float>(1 << fr) * kPiF);
+            apfm::sincos(arg, v0, v1);
+        } else {
+            v0 = plain_feature(s, k);
+            v1 = plain_feature(s, k + 1);
         }
+        f[2 * j] = v0;
+        f[2 * j + 1] = v1;
     }
-    for (int k = 0; k < 3; ++k) {
-        f[36 + k] = s[3 + k];
-        f[39 + k] = s[6 + k];
-        f[43 + k] = s[10 + k];
-    }
-    f[42] = s[9];
-    f[46] = 1.0f;
 }
 
-// Loads the BF16 weights of all layers into LDS rows of LS elements.
+// Loads one network's 16-bit weights into LDS rows of LS elements (layer l,
+// output o at row l * W + o).
+template <int W>
 __device__ void load_weights(const uint16_t* __restrict__ w, uint16_t* sw) {
-    for (int e = threadIdx.x; e < NPAR; e += blockDim.x) {
-        const int l = e / (NF * NF);
-        const int rem = e - l * NF * NF;  // hidden: o*64+i; output (l == 5): o*64+i, o < 16
-        const int o = rem / NF, i = rem % NF;
-        sw[(l * NF + o) * LS + i] = w[e];
+    for (int e = threadIdx.x; e < Geo<W>::NPAR / 8; e += blockDim.x) {
+        int l, o, i;
+        Geo<W>::decode(e * 8, l, o, i);
+        *reinterpret_cast<u16x8*>(&sw[(l * W + o) * LS + i]) = ld8(&w[e * 8]);
     }
 }
 
-// Forward pass over rows [0, n) of records (stride floats each); with dyn_n
-// over rows [0, min(*dyn_n, n)); with `rows`, row r reads record rows[r].
-// With TRAIN, stores the input features and
-// hidden activations as [feature][row] BF16 (acts + (layer * 64 + feature) *
-// n_cap + row).
-template <bool TRAIN>
-__global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict__ w, Bounds bnd,
+// Forward pass over the rows of `gm` (see GroupMap; with groups == 0 over
+// rows [0, n), or [0, min(*dyn_n, n)) with dyn_n); with `rows`, row r reads
+// record rows[r]. With TRAIN, stores the input features and hidden
+// activations as [feature][row] (acts + (Geo::act(layer) + feature) * n_cap
+// + row).
+template <bool TRAIN, int W, bool F16>
+__global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict__ w, Bounds bnd, GroupMap gm,
                                                       const float* __restrict__ rec, int stride, uint32_t n,
                                                       const uint32_t* __restrict__ dyn_n,
                                                       const int32_t* __restrict__ rows, float* __restrict__ out,
                                                       uint16_t* __restrict__ acts, uint32_t n_cap) {
-    __shared__ __attribute__((aligned(16))) uint16_t sw[(NL * NF + NOP) * LS];
+    using G = Geo<W>;
+    __shared__ __attribute__((aligned(16))) uint16_t sw[G::ROWS * LS];
     __shared__ __attribute__((aligned(16))) uint16_t sa[4][16 * LS];
     if (dyn_n) n = min(*dyn_n, n);
-    if (blockIdx.x * 64u >= n) return;
-    load_weights(w, sw);
-    __syncthreads();
+    const uint32_t tiles = tile_count(gm, n);
+    if (blockIdx.x >= tiles) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 15, q = lane >> 4;
     uint16_t* act = sa[wave];
-    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
-        const uint32_t r = base + static_cast<uint32_t>(wave) * 16u + static_cast<uint32_t>(row);
+    uint32_t loaded = 0xffffffffu;
+    for (uint32_t v = blockIdx.x; v < tiles; v += gridDim.x) {
+        uint32_t g, begin, end;
+        if (!tile_rows(gm, v, n, g, begin, end)) continue;  // uniform over the block
+        if (g != loaded) {
+            __syncthreads();
+            load_weights<W>(w + static_cast<size_t>(g) * G::NPAR, sw);
+            __syncthreads();
+            loaded = g;
+        }
+        const Bounds b = gm.bounds ? gm.bounds[g] : bnd;
+        const uint32_t r = begin + static_cast<uint32_t>(wave) * 16u + static_cast<uint32_t>(row);
+        const bool live = r < end;
         {
-            float f[NF];
-            if (r < n) {
+            float f[16];
+            if (live) {
                 const size_t src = rows ? static_cast<size_t>(rows[r]) : static_cast<size_t>(r);
-                encode(rec + src * stride, bnd, f);
+                encode_part(rec + src * stride, b, q, f);
             } else {
-                for (int k = 0; k < NF; ++k) f[k] = 0.0f;
+                for (int k = 0; k < 16; ++k) f[k] = 0.0f;
             }
             for (int k = 0; k < 16; ++k) {
-                const uint16_t h = f2bf(f[16 * q + k]);
+                const uint16_t h = cvt<F16>(f[k]);
                 act[row * LS + 16 * q + k] = h;
-                if (TRAIN && r < n) acts[static_cast<size_t>(16 * q + k) * n_cap + r] = h;
+                if (TRAIN && live) acts[static_cast<size_t>(16 * q + k) * n_cap + r] = h;
             }
         }
         __syncthreads();
         for (int l = 0; l < NL; ++l) {
-            f32x4 acc[4];
-            for (int t = 0; t < 4; ++t) {
+            f32x4 acc[W / 16];
+            const int ks = (l == 0 ? NF : W) / 32;
+            for (int t = 0; t < W / 16; ++t) {
                 acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                for (int s = 0; s < 2; ++s) {
-                    const bf16x8 a = ld8(&sw[(l * NF + 16 * t + row) * LS + 32 * s + 8 * q]);
-                    const bf16x8 b = ld8(&act[row * LS + 32 * s + 8 * q]);
-                    acc[t] = mfma(a, b, acc[t]);
+                for (int s = 0; s < ks; ++s) {
+                    const u16x8 a = ld8(&sw[(l * W + 16 * t + row) * LS + 32 * s + 8 * q]);
+                    const u16x8 bb = ld8(&act[row * LS + 32 * s + 8 * q]);
+                    acc[t] = mfma<F16>(a, bb, acc[t]);
                 }
             }
             __syncthreads();
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < W / 16; ++t)
                 for (int k = 0; k < 4; ++k) {
                     const int o = 16 * t + 4 * q + k;
-                    const float v = acc[t][k] > 0.0f ? acc[t][k] : 0.0f;
-                    const uint16_t h = f2bf(v);
+                    const float x = acc[t][k] > 0.0f ? acc[t][k] : 0.0f;
+                    const uint16_t h = cvt<F16>(x);
                     act[row * LS + o] = h;
-                    if (TRAIN && r < n) acts[static_cast<size_t>((l + 1) * NF + o) * n_cap + r] = h;
+                    if (TRAIN && live) acts[static_cast<size_t>(G::act(l + 1) + o) * n_cap + r] = h;
                 }
             __syncthreads();
         }
         f32x4 y{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int s = 0; s < 2; ++s) {
-            const bf16x8 a = ld8(&sw[(NL * NF + row) * LS + 32 * s + 8 * q]);
-            const bf16x8 b = ld8(&act[row * LS + 32 * s + 8 * q]);
-            y = mfma(a, b, y);
+        for (int s = 0; s < W / 32; ++s) {
+            const u16x8 a = ld8(&sw[(NL * W + row) * LS + 32 * s + 8 * q]);
+            const u16x8 bb = ld8(&act[row * LS + 32 * s + 8 * q]);
+            y = mfma<F16>(a, bb, y);
         }
-        if (q == 0 && r < n) {
+        if (q == 0 && live) {
             out[static_cast<size_t>(r) * NOUT + 0] = y[0];
             out[static_cast<size_t>(r) * NOUT + 1] = y[1];
             out[static_cast<size_t>(r) * NOUT + 2] = y[2];
@@ -182,20 +328,26 @@ __global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict
     }
 }
 
-// Relative L2 loss: dL/dy = 2 (y - t) / (y^2 + 0.01) / (3n), stored BF16 as
-// [o][row] (o < 16, rows 3..15 zero); per-block loss partials. A gate that
-// reads 0 zeroes every delta (an empty sample pool).
+// Relative L2 loss of each group's mean: dL/dy = 2 (y - t) / (y^2 + 0.01) /
+// (3 rows_per_group), times `scale`, stored as [o][row] (o < 16, rows 3..15
+// zero); per-block loss partials (unscaled). A gate that reads 0 (gate[g]
+// with gate_per_group) zeroes every delta of its rows (an empty sample pool).
+template <bool F16>
 __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, const float* __restrict__ rec,
-                                                   uint32_t n, uint32_t n_cap, const uint32_t* __restrict__ gate,
-                                                   uint16_t* __restrict__ dout, float* __restrict__ loss_part) {
+                                                   uint32_t n, uint32_t rows_per_group, uint32_t n_cap,
+                                                   const uint32_t* __restrict__ gate, int gate_per_group,
+                                                   float scale, uint16_t* __restrict__ dout,
+                                                   float* __restrict__ loss_part) {
     __shared__ float red[256];
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    const bool open = gate == nullptr || *gate != 0u;
+    const uint32_t g = r / rows_per_group;
+    const bool open = gate == nullptr || gate[gate_per_group ? g : 0u] != 0u;
     float l = 0.0f;
     if (r < n && !open) {
         for (int o = 0; o < NOP; ++o) dout[static_cast<size_t>(o) * n_cap + r] = 0;
     } else if (r < n) {
-        const float norm = 1.0f / (3.0f * static_cast<float>(n));
+        const float norm = 1.0f / (3.0f * static_cast<float>(rows_per_group));
+        const float gnorm = norm * scale;
         for (int o = 0; o < NOP; ++o) {
             uint16_t h = 0;
             if (o < NOUT) {
@@ -204,7 +356,7 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, 
                 const float den = v * v + 0.01f;
                 const float e = v - t;
                 l += e * e / den * norm;
-                h = f2bf(2.0f * e / den * norm);
+                h = cvt<F16>(2.0f * e / den * gnorm);
             }
             dout[static_cast<size_t>(o) * n_cap + r] = h;
         }
@@ -219,27 +371,44 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, 
 }
 
 // Delta chain: delta_4 = (W_out^T dout) * [a_4 > 0], then
-// delta_{h-1} = (W_h^T delta_h) * [a_{h-1} > 0], stored [feature][row] BF16.
-__global__ __launch_bounds__(256) void backward_kernel(const uint16_t* __restrict__ wt, uint32_t n, uint32_t n_cap,
+// delta_{h-1} = (W_h^T delta_h) * [a_{h-1} > 0], stored [feature][row].
+// Rows [g * rows_per_group, ...) use network g.
+template <int W, bool F16>
+__global__ __launch_bounds__(256) void backward_kernel(const uint16_t* __restrict__ wt, uint32_t n,
+                                                       uint32_t rows_per_group, uint32_t n_cap,
                                                        const uint16_t* __restrict__ acts,
                                                        const uint16_t* __restrict__ dout,
                                                        uint16_t* __restrict__ deltas) {
-    __shared__ __attribute__((aligned(16))) uint16_t swt[NL * NF * LS + NF * 40];
+    using G = Geo<W>;
+    constexpr int LT = G::LT;
+    constexpr int OUT_ROW0 = 4 * W * LT;  // transposed output matrix: rows i, 40 elements
+    __shared__ __attribute__((aligned(16))) uint16_t swt[4 * W * LT + W * 40];
     __shared__ __attribute__((aligned(16))) uint16_t sd[4][16 * LS];
-    for (int e = threadIdx.x; e < NPAR_T; e += blockDim.x) {
-        if (e < NL * NF * NF) {
-            const int l = e / (NF * NF), rem = e % (NF * NF);
-            swt[(l * NF + rem / NF) * LS + rem % NF] = wt[e];
-        } else {
-            const int rem = e - NL * NF * NF;
-            swt[NL * NF * LS + (rem / WT_OUT_STRIDE) * 40 + rem % WT_OUT_STRIDE] = wt[e];
-        }
-    }
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 15, q = lane >> 4;
     uint16_t* del = sd[wave];
+    uint32_t loaded = 0xffffffffu;
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t g = base / rows_per_group;
+        if (g != loaded) {
+            __syncthreads();
+            const uint16_t* src = wt + static_cast<size_t>(g) * G::NPAR_T;
+            // Hidden layers 1..4 transposed ([i][o], W x W) and the output layer ([i][o < 32]).
+            for (int e = threadIdx.x; e < (4 * W * W + W * WT_OUT_STRIDE) / 8; e += blockDim.x) {
+                const int x = e * 8;
+                if (x < 4 * W * W) {
+                    const int h = x / (W * W), rem = x % (W * W);
+                    *reinterpret_cast<u16x8*>(&swt[(h * W + rem / W) * LT + rem % W]) =
+                        ld8(&src[G::off(1) + x]);
+                } else {
+                    const int rem = x - 4 * W * W;
+                    *reinterpret_cast<u16x8*>(&swt[OUT_ROW0 + (rem / WT_OUT_STRIDE) * 40 + rem % WT_OUT_STRIDE]) =
+                        ld8(&src[G::off(NL) + rem]);
+                }
+            }
+            __syncthreads();
+            loaded = g;
+        }
         const uint32_t r = base + static_cast<uint32_t>(wave) * 16u + static_cast<uint32_t>(row);
         const bool live = r < n;
         // Output deltas as [row][o < 32] (o >= 16 zero).
@@ -248,34 +417,34 @@ __global__ __launch_bounds__(256) void backward_kernel(const uint16_t* __restric
             del[row * LS + o] = (o < NOP && live) ? dout[static_cast<size_t>(o) * n_cap + r] : uint16_t{0};
         }
         __syncthreads();
-        f32x4 acc[4];
-        for (int t = 0; t < 4; ++t) {
-            const bf16x8 a = ld8(&swt[NL * NF * LS + (16 * t + row) * 40 + 8 * q]);
-            const bf16x8 b = ld8(&del[row * LS + 8 * q]);
-            acc[t] = mfma(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        f32x4 acc[W / 16];
+        for (int t = 0; t < W / 16; ++t) {
+            const u16x8 a = ld8(&swt[OUT_ROW0 + (16 * t + row) * 40 + 8 * q]);
+            const u16x8 b = ld8(&del[row * LS + 8 * q]);
+            acc[t] = mfma<F16>(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
         }
         __syncthreads();
         for (int h = NL - 1; h >= 0; --h) {
             // acc holds W_{h+1}^T delta_{h+1} (W_out for h = 4) at i = 16t + 4q + k.
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < W / 16; ++t)
                 for (int k = 0; k < 4; ++k) {
                     const int i = 16 * t + 4 * q + k;
                     uint16_t hv = 0;
                     if (live) {
-                        const uint16_t a = acts[static_cast<size_t>((h + 1) * NF + i) * n_cap + r];
-                        hv = bf2f(a) > 0.0f ? f2bf(acc[t][k]) : uint16_t{0};
-                        deltas[static_cast<size_t>(h * NF + i) * n_cap + r] = hv;
+                        const uint16_t a = acts[static_cast<size_t>(G::act(h + 1) + i) * n_cap + r];
+                        hv = positive(a) ? cvt<F16>(acc[t][k]) : uint16_t{0};
+                        deltas[static_cast<size_t>(h * W + i) * n_cap + r] = hv;
                     }
                     del[row * LS + i] = hv;
                 }
             __syncthreads();
             if (h == 0) break;
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < W / 16; ++t) {
                 acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                for (int s = 0; s < 2; ++s) {
-                    const bf16x8 a = ld8(&swt[(h * NF + 16 * t + row) * LS + 32 * s + 8 * q]);
-                    const bf16x8 b = ld8(&del[row * LS + 32 * s + 8 * q]);
-                    acc[t] = mfma(a, b, acc[t]);
+                for (int s = 0; s < W / 32; ++s) {
+                    const u16x8 a = ld8(&swt[((h - 1) * W + 16 * t + row) * LT + 32 * s + 8 * q]);
+                    const u16x8 b = ld8(&del[row * LS + 32 * s + 8 * q]);
+                    acc[t] = mfma<F16>(a, b, acc[t]);
                 }
             }
             __syncthreads();
@@ -283,72 +452,86 @@ __global__ __launch_bounds__(256) void backward_kernel(const uint16_t* __restric
     }
 }
 
-// Weight-gradient partials of one row chunk: block (chunk, layer).
+// Weight-gradient partials of one row chunk: block (chunk, layer); the
+// chunk's partial is in its group's parameter order.
+template <int W, bool F16>
 __global__ __launch_bounds__(256) void wgrad_kernel(uint32_t n_cap, const uint16_t* __restrict__ acts,
                                                     const uint16_t* __restrict__ deltas,
                                                     const uint16_t* __restrict__ dout, float* __restrict__ part) {
+    using G = Geo<W>;
     const int chunk = blockIdx.x, layer = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, q = lane >> 4;
     const uint32_t r0 = static_cast<uint32_t>(chunk) * kChunkRows;
     const bool out_layer = layer == NL;
-    const int tiles = out_layer ? 4 : 16;
-    const uint16_t* dl = out_layer ? dout : deltas + static_cast<size_t>(layer) * NF * n_cap;
-    const uint16_t* al = acts + static_cast<size_t>(layer) * NF * n_cap;  // input of this layer
-    float* dst = part + static_cast<size_t>(chunk) * NPAR + static_cast<size_t>(layer) * NF * NF;
+    const int ni = layer == 0 ? NF : W;
+    const int tiles = (out_layer ? 1 : W / 16) * (ni / 16);
+    const uint16_t* dl = out_layer ? dout : deltas + static_cast<size_t>(layer) * W * n_cap;
+    const uint16_t* al = acts + static_cast<size_t>(G::act(layer)) * n_cap;  // input of this layer
+    float* dst = part + static_cast<size_t>(chunk) * G::NPAR + G::off(layer);
     for (int tile = wave; tile < tiles; tile += 4) {
-        const int to = out_layer ? 0 : tile / 4, ti = out_layer ? tile : tile % 4;
+        const int to = tile / (ni / 16), ti = tile % (ni / 16);
         f32x4 acc{0.0f, 0.0f, 0.0f, 0.0f};
         for (uint32_t k = 0; k < kChunkRows; k += 32) {
             const uint32_t rr = r0 + k + 8u * static_cast<uint32_t>(q);
-            const bf16x8 a = ld8(&dl[static_cast<size_t>(16 * to + col) * n_cap + rr]);
-            const bf16x8 b = ld8(&al[static_cast<size_t>(16 * ti + col) * n_cap + rr]);
-            acc = mfma(a, b, acc);
+            const u16x8 a = ld8(&dl[static_cast<size_t>(16 * to + col) * n_cap + rr]);
+            const u16x8 b = ld8(&al[static_cast<size_t>(16 * ti + col) * n_cap + rr]);
+            acc = mfma<F16>(a, b, acc);
         }
         for (int k = 0; k < 4; ++k) {
             const int o = 16 * to + 4 * q + k, i = 16 * ti + col;
-            dst[o * NF + i] = acc[k];
+            dst[o * ni + i] = acc[k];
         }
     }
 }
 
-__global__ void reduce_grad_kernel(const float* __restrict__ part, int chunks, float* __restrict__ grad) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= NPAR) return;
+// grad[g][p] = inv_scale * (sum over the chunks of group g of part[chunk][p]).
+template <int W>
+__global__ void reduce_grad_kernel(const float* __restrict__ part, int chunks_per_group, uint32_t groups,
+                                   float inv_scale, float* __restrict__ grad) {
+    constexpr int NPAR = Geo<W>::NPAR;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= groups * NPAR) return;
+    const uint32_t g = p / NPAR, lp = p % NPAR;
     float s = 0.0f;
-    for (int c = 0; c < chunks; ++c) s += part[static_cast<size_t>(c) * NPAR + p];
-    grad[p] = s;
+    for (int c = 0; c < chunks_per_group; ++c)
+        s += part[(static_cast<size_t>(g) * chunks_per_group + c) * NPAR + lp];
+    grad[p] = s * inv_scale;
 }
 
-// Adam (the oracle's or_nrc_adam), then the BF16 copies W[o][i] and W^T[i][o]
-// of the training weights, and the exponential moving average of the
-// parameters (decay ema_decay, debiased by ema_c = 1 - decay^t) whose BF16
-// copy answers inference.
-__global__ void adam_kernel(float* __restrict__ params, float* __restrict__ m, float* __restrict__ v,
-                            const float* __restrict__ grad, float lr, float beta1, float beta2, float eps, float c1,
-                            float c2, uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
-                            const uint32_t* __restrict__ gate, float ema_decay, float ema_c,
+// Adam (the oracle's or_nrc_adam), then the 16-bit copies W[o][i] and
+// W^T[i][o] of the training weights, and the exponential moving average of
+// the parameters (decay ema_decay, debiased by ema_c = 1 - decay^t) whose
+// 16-bit copy answers inference. A gate that reads 0 (gate[g] with
+// gate_per_group) leaves its group untouched.
+template <int W, bool F16>
+__global__ void adam_kernel(uint32_t groups, float* __restrict__ params, float* __restrict__ m,
+                            float* __restrict__ v, const float* __restrict__ grad, float lr, float beta1, float beta2,
+                            float eps, float c1, float c2, uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
+                            const uint32_t* __restrict__ gate, int gate_per_group, float ema_decay, float ema_c,
                             float* __restrict__ ema, uint16_t* __restrict__ w_inf) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= NPAR) return;
-    if (gate && *gate == 0u) return;
-    const float g = grad[p];
-    const float mm = beta1 * m[p] + (1.0f - beta1) * g;
-    const float vv = beta2 * v[p] + (1.0f - beta2) * (g * g);
+    using G = Geo<W>;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= groups * G::NPAR) return;
+    const uint32_t g = p / G::NPAR;
+    if (gate && gate[gate_per_group ? g : 0u] == 0u) return;
+    const float gr = grad[p];
+    const float mm = beta1 * m[p] + (1.0f - beta1) * gr;
+    const float vv = beta2 * v[p] + (1.0f - beta2) * (gr * gr);
     m[p] = mm;
     v[p] = vv;
     const float mh = mm / c1;
     const float vh = vv / c2;
     const float x = params[p] - lr * mh / (sqrtf(vh) + eps);
     params[p] = x;
-    const uint16_t h = f2bf(x);
+    const uint16_t h = cvt<F16>(x);
     w[p] = h;
-    const int l = p / (NF * NF), rem = p % (NF * NF), o = rem / NF, i = rem % NF;
-    if (l < NL) wt[l * NF * NF + i * NF + o] = h;
-    else wt[NL * NF * NF + i * WT_OUT_STRIDE + o] = h;
+    int l, o, i;
+    G::decode(static_cast<int>(p - g * G::NPAR), l, o, i);
+    wt[static_cast<size_t>(g) * G::NPAR_T + G::tindex(l, o, i)] = h;
     const float e = ema_decay * ema[p] + (1.0f - ema_decay) * x;
     ema[p] = e;
-    w_inf[p] = ema_decay > 0.0f ? f2bf(e / ema_c) : h;
+    w_inf[p] = ema_decay > 0.0f ? cvt<F16>(e / ema_c) : h;
 }
 
 // Row j of step `step`: sample (off + (step * batch + j) * P) mod N of the
@@ -374,36 +557,85 @@ __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ p
 }  // namespace
 
 struct NrcState {
+    int width = 64;
+    bool f16 = false;
+    uint32_t groups = 1;
+    size_t npar = 0, npar_t = 0;   // per network
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     uint16_t *w = nullptr, *wt = nullptr;
     float* ema = nullptr;          // moving average of the parameters (zero-started, debiased)
-    uint16_t* w_inf = nullptr;     // BF16 weights inference reads: the debiased average, or w
+    uint16_t* w_inf = nullptr;     // 16-bit weights inference reads: the debiased average, or w
     float ema_decay = 0.0f;
-    Bounds bounds{{-1.0f, -1.0f, -1.0f}, {1.0f, 1.0f, 1.0f}};
+    Bounds bounds{{-1.0f, -1.0f, -1.0f}, {1.0f, 1.0f, 1.0f}};  // network 0's (all networks' with one)
+    std::vector<Bounds> host_bounds;                            // per network
+    Bounds* dev_bounds = nullptr;                               // per network, device copy
+    uint32_t* map = nullptr;       // seg[groups + 1] and tile0[groups + 1] of ap_nrc_infer_grouped
     int step = 0;
     uint32_t cap = 0;
     uint16_t *acts = nullptr, *deltas = nullptr, *dout = nullptr;
     float *y = nullptr, *part = nullptr, *loss_part = nullptr;
-    float last_loss = 0.0f;
     float* batch = nullptr;   // gathered training rows of nrc_train_pool
     uint32_t batch_cap = 0;
 };
 
 namespace {
 
+template <int W_, bool F16_>
+struct Net {
+    static constexpr int W = W_;
+    static constexpr bool F16 = F16_;
+};
+
+// Calls fn with the Net<W, F16> tag of the state's network shape.
+template <class Fn>
+int with_net(const NrcState* s, Fn&& fn) {
+    if (s->width == 32) return s->f16 ? fn(Net<32, true>{}) : fn(Net<32, false>{});
+    return s->f16 ? fn(Net<64, true>{}) : fn(Net<64, false>{});
+}
+
+size_t npar_of(int width) { return static_cast<size_t>(width) * NF + 4u * width * width + NOP * width; }
+size_t npar_t_of(int width) { return static_cast<size_t>(width) * NF + 4u * width * width + WT_OUT_STRIDE * width; }
+
+// Power of two the loss gradient of an FP16 network is scaled by for a
+// batch of `rows` per network: 2^(ceil(log2(3 rows)) - 4); 1 for BF16.
+float loss_scale(const NrcState* s, uint32_t rows) {
+    if (!s->f16) return 1.0f;
+    int k = 0;
+    while ((1ull << k) < 3ull * rows) ++k;
+    return std::ldexp(1.0f, k - 4);
+}
+
+// Layer, output and input of parameter p of a width-w network (host).
+void decode_h(int width, int p, int& l, int& o, int& i) {
+    if (width == 32) Geo<32>::decode(p, l, o, i);
+    else Geo<64>::decode(p, l, o, i);
+}
+int tindex_h(int width, int l, int o, int i) {
+    return width == 32 ? Geo<32>::tindex(l, o, i) : Geo<64>::tindex(l, o, i);
+}
+
+int upload_bounds(NrcState* s) {
+    const hipError_t e = hipMemcpy(s->dev_bounds, s->host_bounds.data(), s->groups * sizeof(Bounds),
+                                   hipMemcpyHostToDevice);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: bounds");
+}
+
 int upload_params(NrcState* s, const std::vector<float>& p) {
-    std::vector<uint16_t> w(NPAR), wt(NPAR_T, 0);
-    for (int k = 0; k < NPAR; ++k) {
-        w[k] = f2bf_h(p[k]);
-        const int l = k / (NF * NF), rem = k % (NF * NF), o = rem / NF, i = rem % NF;
-        if (l < NL) wt[l * NF * NF + i * NF + o] = w[k];
-        else wt[NL * NF * NF + i * WT_OUT_STRIDE + o] = w[k];
-    }
-    hipError_t e = hipMemcpy(s->params, p.data(), NPAR * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(s->w, w.data(), NPAR * sizeof(uint16_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(s->w_inf, w.data(), NPAR * sizeof(uint16_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(s->ema, 0, NPAR * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpy(s->wt, wt.data(), NPAR_T * sizeof(uint16_t), hipMemcpyHostToDevice);
+    const size_t np = s->npar, nt = s->npar_t, G = s->groups;
+    std::vector<uint16_t> w(G * np), wt(G * nt, 0);
+    for (size_t g = 0; g < G; ++g)
+        for (size_t k = 0; k < np; ++k) {
+            const uint16_t h = s->f16 ? f2h_h(p[g * np + k]) : f2bf_h(p[g * np + k]);
+            w[g * np + k] = h;
+            int l, o, i;
+            decode_h(s->width, static_cast<int>(k), l, o, i);
+            wt[g * nt + tindex_h(s->width, l, o, i)] = h;
+        }
+    hipError_t e = hipMemcpy(s->params, p.data(), G * np * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->w, w.data(), G * np * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->w_inf, w.data(), G * np * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->ema, 0, G * np * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(s->wt, wt.data(), G * nt * sizeof(uint16_t), hipMemcpyHostToDevice);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: upload parameters");
 }
 
@@ -415,12 +647,13 @@ int ensure_train(NrcState* s, uint32_t n) {
     s->acts = s->deltas = s->dout = nullptr;
     s->y = s->part = s->loss_part = nullptr;
     s->cap = 0;
-    const size_t N = n;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->acts), (NL + 1) * NF * N * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->deltas), NL * NF * N * sizeof(uint16_t));
+    const size_t N = n, Wd = static_cast<size_t>(s->width);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->acts), (NF + NL * Wd) * N * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->deltas), NL * Wd * N * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->dout), NOP * N * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->y), N * NOUT * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->part), (N / kChunkRows) * NPAR * sizeof(float));
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void**>(&s->part), (N / kChunkRows) * s->npar * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->loss_part), ((N + 255) / 256) * sizeof(float));
     if (e != hipSuccess) return hip_fail(e, "ap_nrc: training buffers");
     s->cap = n;
@@ -432,42 +665,72 @@ inline unsigned blocks_for(uint32_t n) {
     return b < 2048u ? (b > 0 ? b : 1u) : 2048u;
 }
 
-}  // namespace
-
-int nrc_infer(NrcState* s, const float* queries, size_t n, float* out, hipStream_t st) {
-    if (n == 0) return AP_OK;
-    hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(static_cast<uint32_t>(n))), dim3(256), 0, st,
-                       s->w_inf, s->bounds, queries, QREC, static_cast<uint32_t>(n), nullptr, nullptr, out, nullptr, 0u);
+// Inference over the rows of gm (groups == 0: rows [0, min(*dyn_n, n)) of
+// network 0 with its bounds `s->bounds`).
+int infer(NrcState* s, const float* queries, const GroupMap& gm, const int32_t* rows, const uint32_t* dyn_n,
+          uint32_t n, float* out, hipStream_t st) {
+    with_net(s, [&](auto net) {
+        using N = decltype(net);
+        hipLaunchKernelGGL((forward_kernel<false, N::W, N::F16>), dim3(blocks_for(n)), dim3(256), 0, st, s->w_inf,
+                           s->bounds, gm, queries, QREC, n, dyn_n, rows, out, nullptr, 0u);
+        return 0;
+    });
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: infer");
 }
 
-// Forward, loss and backward of one batch; the gradient lands in s->grad.
-int nrc_gradient(NrcState* s, const float* samples, uint32_t n, hipStream_t st, const uint32_t* gate = nullptr) {
+}  // namespace
+
+int nrc_infer(NrcState* s, const float* queries, size_t n, float* out, hipStream_t st) {
+    if (n == 0) return AP_OK;
+    return infer(s, queries, GroupMap{}, nullptr, nullptr, static_cast<uint32_t>(n), out, st);
+}
+
+// Forward, loss and backward of one batch of `rows` rows per network, laid
+// out [network][row]; the gradient lands in s->grad. gate: see loss_kernel.
+int nrc_gradient(NrcState* s, const float* samples, uint32_t rows, hipStream_t st, const uint32_t* gate = nullptr,
+                 int gate_per_group = 0) {
+    const uint32_t n = rows * s->groups;
     int rc = ensure_train(s, n);
     if (rc != AP_OK) return rc;
-    hipLaunchKernelGGL(forward_kernel<true>, dim3(blocks_for(n)), dim3(256), 0, st, s->w, s->bounds, samples, SREC, n,
-                       nullptr, nullptr, s->y, s->acts, n);
-    hipLaunchKernelGGL(loss_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->y, samples, n, n, gate, s->dout,
-                       s->loss_part);
-    hipLaunchKernelGGL(backward_kernel, dim3(blocks_for(n)), dim3(256), 0, st, s->wt, n, n, s->acts, s->dout,
-                       s->deltas);
-    hipLaunchKernelGGL(wgrad_kernel, dim3(n / kChunkRows, NL + 1), dim3(256), 0, st, n, s->acts, s->deltas, s->dout,
-                       s->part);
-    hipLaunchKernelGGL(reduce_grad_kernel, dim3((NPAR + 255) / 256), dim3(256), 0, st, s->part,
-                       static_cast<int>(n / kChunkRows), s->grad);
+    GroupMap gm;
+    gm.groups = s->groups;
+    gm.rows_per_group = rows;
+    gm.bounds = s->groups > 1 ? s->dev_bounds : nullptr;
+    const float scale = loss_scale(s, rows);
+    with_net(s, [&](auto net) {
+        using N = decltype(net);
+        constexpr int NPAR = Geo<N::W>::NPAR;
+        hipLaunchKernelGGL((forward_kernel<true, N::W, N::F16>), dim3(blocks_for(n)), dim3(256), 0, st, s->w,
+                           s->bounds, gm, samples, SREC, n, nullptr, nullptr, s->y, s->acts, n);
+        hipLaunchKernelGGL(loss_kernel<N::F16>, dim3((n + 255) / 256), dim3(256), 0, st, s->y, samples, n, rows, n,
+                           gate, gate_per_group, scale, s->dout, s->loss_part);
+        hipLaunchKernelGGL((backward_kernel<N::W, N::F16>), dim3(blocks_for(n)), dim3(256), 0, st, s->wt, n, rows, n,
+                           s->acts, s->dout, s->deltas);
+        hipLaunchKernelGGL((wgrad_kernel<N::W, N::F16>), dim3(n / kChunkRows, NL + 1), dim3(256), 0, st, n, s->acts,
+                           s->deltas, s->dout, s->part);
+        hipLaunchKernelGGL(reduce_grad_kernel<N::W>, dim3((s->groups * NPAR + 255) / 256), dim3(256), 0, st,
+                           s->part, static_cast<int>(rows / kChunkRows), s->groups, 1.0f / scale, s->grad);
+        return 0;
+    });
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: gradient");
 }
 
-int nrc_adam(NrcState* s, float lr, hipStream_t st, const uint32_t* gate = nullptr) {
+int nrc_adam(NrcState* s, float lr, hipStream_t st, const uint32_t* gate = nullptr, int gate_per_group = 0) {
     s->step += 1;
     const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
     const float c1 = 1.0f - std::pow(b1, static_cast<float>(s->step));
     const float c2 = 1.0f - std::pow(b2, static_cast<float>(s->step));
     const float ec = 1.0f - std::pow(s->ema_decay, static_cast<float>(s->step));
-    hipLaunchKernelGGL(adam_kernel, dim3((NPAR + 255) / 256), dim3(256), 0, st, s->params, s->m, s->v, s->grad, lr,
-                       b1, b2, eps, c1, c2, s->w, s->wt, gate, s->ema_decay, ec, s->ema, s->w_inf);
+    with_net(s, [&](auto net) {
+        using N = decltype(net);
+        constexpr int NPAR = Geo<N::W>::NPAR;
+        hipLaunchKernelGGL((adam_kernel<N::W, N::F16>), dim3((s->groups * NPAR + 255) / 256), dim3(256), 0, st,
+                           s->groups, s->params, s->m, s->v, s->grad, lr, b1, b2, eps, c1, c2, s->w, s->wt, gate,
+                           gate_per_group, s->ema_decay, ec, s->ema, s->w_inf);
+        return 0;
+    });
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: adam");
 }
@@ -475,10 +738,7 @@ int nrc_adam(NrcState* s, float lr, hipStream_t st, const uint32_t* gate = nullp
 int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const uint32_t* dyn_n, uint32_t n_cap,
                   float* out, hipStream_t st) {
     if (n_cap == 0) return AP_OK;
-    hipLaunchKernelGGL(forward_kernel<false>, dim3(blocks_for(n_cap)), dim3(256), 0, st, s->w_inf, s->bounds,
-                       queries, QREC, n_cap, dyn_n, rows, out, nullptr, 0u);
-    const hipError_t e = hipGetLastError();
-    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: infer");
+    return infer(s, queries, GroupMap{}, rows, dyn_n, n_cap, out, st);
 }
 
 int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_n, uint32_t batch, uint32_t steps, float lr,
@@ -515,7 +775,7 @@ void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]) {
 void nrc_free(NrcState* s) {
     if (!s) return;
     void* ptrs[] = {s->params, s->m, s->v, s->grad, s->w, s->wt, s->acts, s->deltas, s->dout, s->y, s->part,
-                    s->loss_part, s->batch, s->ema, s->w_inf};
+                    s->loss_part, s->batch, s->ema, s->w_inf, s->dev_bounds, s->map};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -530,38 +790,83 @@ struct ap_nrc {
 };
 
 ap::NrcState* ap::nrc_state(ap_nrc* h) { return h ? h->s : nullptr; }
+uint32_t ap::nrc_groups(const NrcState* s) { return s->groups; }
+
+namespace {
+
+int check_batch(const char* who, size_t n) {
+    if (n == 0 || n % kChunkRows != 0 || n > (1u << 24))
+        return fail(AP_ERR_INVALID_ARG, std::string(who) + ": batch must be a positive multiple of 256 up to 2^24");
+    return AP_OK;
+}
+
+// Copies the loss partials of the last gradient into per-network losses.
+hipError_t read_losses(const NrcState* s, uint32_t rows, float* loss) {
+    const uint32_t per = rows / 256, n = per * s->groups;
+    std::vector<float> lp(n);
+    const hipError_t e = hipMemcpy(lp.data(), s->loss_part, n * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (uint32_t g = 0; g < s->groups; ++g) {
+        float sum = 0.0f;
+        for (uint32_t k = 0; k < per; ++k) sum += lp[g * per + k];
+        loss[g] = sum;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
 
 extern "C" {
 
-int ap_nrc_create(uint64_t seed, ap_nrc** out) {
+int ap_nrc_create_grouped(uint32_t groups, uint32_t width, int32_t precision, uint64_t seed, ap_nrc** out) {
     if (!out) return fail(AP_ERR_INVALID_ARG, "ap_nrc_create: NULL output");
     *out = nullptr;
+    if (groups == 0 || groups > kMaxGroups)
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc_create_grouped: groups must be in [1, 4096]");
+    if (width != 32 && width != 64) return fail(AP_ERR_INVALID_ARG, "ap_nrc_create_grouped: width must be 32 or 64");
+    if (precision != AP_NRC_BF16 && precision != AP_NRC_FP16)
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc_create_grouped: precision must be AP_NRC_BF16 or AP_NRC_FP16");
     NrcState* s = new NrcState();
+    s->width = static_cast<int>(width);
+    s->f16 = precision == AP_NRC_FP16;
+    s->groups = groups;
+    s->npar = npar_of(s->width);
+    s->npar_t = npar_t_of(s->width);
+    s->host_bounds.assign(groups, s->bounds);
+    const size_t np = groups * s->npar;
     hipError_t e = hipSuccess;
     for (float** p : {&s->params, &s->m, &s->v, &s->grad, &s->ema})
-        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), NPAR * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->w), NPAR * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->w_inf), NPAR * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->wt), NPAR_T * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMemset(s->m, 0, NPAR * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(s->v, 0, NPAR * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(s->grad, 0, NPAR * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), np * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->w), np * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->w_inf), np * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->wt), groups * s->npar_t * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->dev_bounds), groups * sizeof(Bounds));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->map), 2 * (groups + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(s->m, 0, np * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(s->v, 0, np * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(s->grad, 0, np * sizeof(float));
     if (e != hipSuccess) {
         nrc_free(s);
         return hip_fail(e, "ap_nrc_create");
     }
-    // He-normal initial weights (hidden sd sqrt(2/64), output sd sqrt(1/64), padded output rows 0)
-    // from PCG32(seed) by Box-Muller.
-    std::vector<float> p(NPAR, 0.0f);
-    Pcg32 rng = pcg32_seed(seed, 7);
-    auto normal = [&]() {
-        const float u1 = 1.0f - pcg32_float(rng), u2 = pcg32_float(rng);
-        return std::sqrt(-2.0f * std::log(u1)) * std::cos(6.283185307f * u2);
-    };
-    for (int k = 0; k < OUT_OFF; ++k) p[k] = normal() * std::sqrt(2.0f / NF);
-    for (int o = 0; o < NOUT; ++o)
-        for (int i = 0; i < NF; ++i) p[OUT_OFF + o * NF + i] = normal() * std::sqrt(1.0f / NF);
-    const int rc = upload_params(s, p);
+    // He-normal initial weights (sd sqrt(2 / inputs) for the hidden layers,
+    // sqrt(1 / width) for the output rows 0..2, rows 3..15 zero) from
+    // PCG32(seed, 7 + network) by Box-Muller.
+    std::vector<float> p(np, 0.0f);
+    for (uint32_t g = 0; g < groups; ++g) {
+        Pcg32 rng = pcg32_seed(seed, 7 + g);
+        auto normal = [&]() {
+            const float u1 = 1.0f - pcg32_float(rng), u2 = pcg32_float(rng);
+            return std::sqrt(-2.0f * std::log(u1)) * std::cos(6.283185307f * u2);
+        };
+        float* q = p.data() + g * s->npar;
+        const size_t hidden = s->npar - NOP * width;
+        for (size_t k = 0; k < hidden; ++k) q[k] = normal() * std::sqrt(2.0f / (k < width * NF ? NF : width));
+        for (uint32_t o = 0; o < NOUT; ++o)
+            for (uint32_t i = 0; i < width; ++i) q[hidden + o * width + i] = normal() * std::sqrt(1.0f / width);
+    }
+    int rc = upload_params(s, p);
+    if (rc == AP_OK) rc = upload_bounds(s);
     if (rc != AP_OK) {
         nrc_free(s);
         return rc;
@@ -570,6 +875,8 @@ int ap_nrc_create(uint64_t seed, ap_nrc** out) {
     return AP_OK;
 }
 
+int ap_nrc_create(uint64_t seed, ap_nrc** out) { return ap_nrc_create_grouped(1, 64, AP_NRC_BF16, seed, out); }
+
 int ap_nrc_destroy(ap_nrc* nrc) {
     if (!nrc) return AP_OK;
     nrc_free(nrc->s);
@@ -577,70 +884,139 @@ int ap_nrc_destroy(ap_nrc* nrc) {
     return AP_OK;
 }
 
+int ap_nrc_info(const ap_nrc* nrc, uint32_t* groups, uint32_t* width, int32_t* precision,
+                size_t* params_per_network) {
+    if (!nrc) return fail(AP_ERR_INVALID_ARG, "ap_nrc_info: NULL cache");
+    if (groups) *groups = nrc->s->groups;
+    if (width) *width = static_cast<uint32_t>(nrc->s->width);
+    if (precision) *precision = nrc->s->f16 ? AP_NRC_FP16 : AP_NRC_BF16;
+    if (params_per_network) *params_per_network = nrc->s->npar;
+    return AP_OK;
+}
+
 int ap_nrc_set_bounds(ap_nrc* nrc, const float bmin[3], const float bmax[3]) {
     if (!nrc || !bmin || !bmax) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_bounds: NULL argument");
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 3; ++a)
         if (!(bmax[a] > bmin[a])) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_bounds: empty bounds");
-        nrc->s->bounds.lo[a] = bmin[a];
-        nrc->s->bounds.hi[a] = bmax[a];
+    nrc_set_bounds(nrc->s, bmin, bmax);
+    for (Bounds& b : nrc->s->host_bounds) b = nrc->s->bounds;
+    return upload_bounds(nrc->s);
+}
+
+int ap_nrc_set_group_bounds(ap_nrc* nrc, uint32_t group, const float bmin[3], const float bmax[3]) {
+    if (!nrc || !bmin || !bmax) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_group_bounds: NULL argument");
+    if (group >= nrc->s->groups) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_group_bounds: no such network");
+    Bounds b;
+    for (int a = 0; a < 3; ++a) {
+        if (!(bmax[a] > bmin[a])) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_group_bounds: empty bounds");
+        b.lo[a] = bmin[a];
+        b.hi[a] = bmax[a];
     }
-    return AP_OK;
+    nrc->s->host_bounds[group] = b;
+    if (group == 0) nrc->s->bounds = b;
+    return upload_bounds(nrc->s);
 }
 
 int ap_nrc_get_params(const ap_nrc* nrc, float* host_params) {
     if (!nrc || !host_params) return fail(AP_ERR_INVALID_ARG, "ap_nrc_get_params: NULL argument");
-    const hipError_t e = hipMemcpy(host_params, nrc->s->params, NPAR * sizeof(float), hipMemcpyDeviceToHost);
+    const hipError_t e = hipMemcpy(host_params, nrc->s->params, nrc->s->groups * nrc->s->npar * sizeof(float),
+                                   hipMemcpyDeviceToHost);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc_get_params");
 }
 
 int ap_nrc_set_params(ap_nrc* nrc, const float* host_params) {
     if (!nrc || !host_params) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_params: NULL argument");
-    std::vector<float> p(host_params, host_params + NPAR);
-    for (int o = NOUT; o < NOP; ++o)
-        for (int i = 0; i < NF; ++i) p[OUT_OFF + o * NF + i] = 0.0f;
-    nrc->s->step = 0;
-    hipError_t e = hipMemset(nrc->s->m, 0, NPAR * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(nrc->s->v, 0, NPAR * sizeof(float));
+    NrcState* s = nrc->s;
+    const size_t np = s->groups * s->npar, out0 = s->npar - NOP * s->width;
+    std::vector<float> p(host_params, host_params + np);
+    for (uint32_t g = 0; g < s->groups; ++g)
+        for (size_t k = out0 + NOUT * s->width; k < s->npar; ++k) p[g * s->npar + k] = 0.0f;
+    s->step = 0;
+    hipError_t e = hipMemset(s->m, 0, np * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(s->v, 0, np * sizeof(float));
     if (e != hipSuccess) return hip_fail(e, "ap_nrc_set_params");
-    return upload_params(nrc->s, p);
+    return upload_params(s, p);
 }
 
 int ap_nrc_infer(ap_nrc* nrc, const float* queries, size_t n, float* out_rgb, void* stream) {
     if (!nrc) return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer: NULL cache");
+    if (nrc->s->groups != 1) return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer: a grouped cache takes ap_nrc_infer_grouped");
     if (n == 0) return AP_OK;
     if (!queries || !out_rgb) return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer: NULL buffer");
     if (n > 0x7fffffffull) return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer: too many queries");
     return nrc_infer(nrc->s, queries, n, out_rgb, static_cast<hipStream_t>(stream));
 }
 
-int ap_nrc_gradient(ap_nrc* nrc, const float* samples, size_t n, float* host_grad, float* loss, void* stream) {
-    if (!nrc || !samples) return fail(AP_ERR_INVALID_ARG, "ap_nrc_gradient: NULL argument");
-    if (n == 0 || n % kChunkRows != 0 || n > (1u << 24))
-        return fail(AP_ERR_INVALID_ARG, "ap_nrc_gradient: batch must be a positive multiple of 256 up to 2^24");
+int ap_nrc_infer_grouped(ap_nrc* nrc, const float* queries, const uint32_t* host_seg, float* out_rgb, void* stream) {
+    if (!nrc || !host_seg) return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer_grouped: NULL argument");
+    NrcState* s = nrc->s;
+    const uint32_t G = s->groups;
+    std::vector<uint32_t> map(2 * (G + 1));
+    uint64_t tiles = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        if (host_seg[g + 1] < host_seg[g])
+            return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer_grouped: row offsets must not decrease");
+        map[g] = host_seg[g];
+        map[G + 1 + g] = static_cast<uint32_t>(tiles);
+        tiles += (host_seg[g + 1] - host_seg[g] + 63u) / 64u;
+    }
+    map[G] = host_seg[G];
+    map[2 * G + 1] = static_cast<uint32_t>(tiles);
+    if (host_seg[G] > 0x7fffffffu) return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer_grouped: too many queries");
+    if (tiles == 0) return AP_OK;
+    if (!queries || !out_rgb) return fail(AP_ERR_INVALID_ARG, "ap_nrc_infer_grouped: NULL buffer");
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    int rc = nrc_gradient(nrc->s, samples, static_cast<uint32_t>(n), st);
+    // The map is rewritten only once the stream's earlier work has read it.
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipMemcpy(s->map, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "ap_nrc_infer_grouped");
+    GroupMap gm;
+    gm.groups = G;
+    gm.seg = s->map;
+    gm.tile0 = s->map + G + 1;
+    gm.bounds = s->dev_bounds;
+    return infer(s, queries, gm, nullptr, nullptr, static_cast<uint32_t>(tiles * 64u), out_rgb, st);
+}
+
+int ap_nrc_gradient_grouped(ap_nrc* nrc, const float* samples, size_t batch, float* host_grad, float* host_loss,
+                            void* stream) {
+    if (!nrc || !samples) return fail(AP_ERR_INVALID_ARG, "ap_nrc_gradient: NULL argument");
+    int rc = check_batch("ap_nrc_gradient", batch);
+    if (rc != AP_OK) return rc;
+    if (batch * nrc->s->groups > (1u << 24))
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc_gradient: at most 2^24 rows over all networks");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = nrc_gradient(nrc->s, samples, static_cast<uint32_t>(batch), st);
     if (rc != AP_OK) return rc;
     hipError_t e = hipStreamSynchronize(st);
     if (e == hipSuccess && host_grad)
-        e = hipMemcpy(host_grad, nrc->s->grad, NPAR * sizeof(float), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && loss) {
-        std::vector<float> lp((n + 255) / 256);
-        e = hipMemcpy(lp.data(), nrc->s->loss_part, lp.size() * sizeof(float), hipMemcpyDeviceToHost);
-        float sum = 0.0f;
-        for (float x : lp) sum += x;
-        *loss = sum;
-    }
+        e = hipMemcpy(host_grad, nrc->s->grad, nrc->s->groups * nrc->s->npar * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && host_loss) e = read_losses(nrc->s, static_cast<uint32_t>(batch), host_loss);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc_gradient");
 }
 
-int ap_nrc_train_step(ap_nrc* nrc, const float* samples, size_t n, float lr, void* stream) {
+int ap_nrc_gradient(ap_nrc* nrc, const float* samples, size_t n, float* host_grad, float* loss, void* stream) {
+    if (nrc && nrc->s->groups != 1)
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc_gradient: a grouped cache takes ap_nrc_gradient_grouped");
+    return ap_nrc_gradient_grouped(nrc, samples, n, host_grad, loss, stream);
+}
+
+int ap_nrc_train_step_grouped(ap_nrc* nrc, const float* samples, size_t batch, float lr, void* stream) {
     if (!nrc || !samples) return fail(AP_ERR_INVALID_ARG, "ap_nrc_train_step: NULL argument");
-    if (n == 0 || n % kChunkRows != 0 || n > (1u << 24))
-        return fail(AP_ERR_INVALID_ARG, "ap_nrc_train_step: batch must be a positive multiple of 256 up to 2^24");
+    int rc = check_batch("ap_nrc_train_step", batch);
+    if (rc != AP_OK) return rc;
+    if (batch * nrc->s->groups > (1u << 24))
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc_train_step: at most 2^24 rows over all networks");
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    int rc = nrc_gradient(nrc->s, samples, static_cast<uint32_t>(n), st);
+    rc = nrc_gradient(nrc->s, samples, static_cast<uint32_t>(batch), st);
     if (rc != AP_OK) return rc;
     return nrc_adam(nrc->s, lr, st);
+}
+
+int ap_nrc_train_step(ap_nrc* nrc, const float* samples, size_t n, float lr, void* stream) {
+    if (nrc && nrc->s->groups != 1)
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc_train_step: a grouped cache takes ap_nrc_train_step_grouped");
+    return ap_nrc_train_step_grouped(nrc, samples, n, lr, stream);
 }
 
 int ap_nrc_set_ema(ap_nrc* nrc, float decay) {
@@ -661,7 +1037,8 @@ int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]) {
 
 int ap_nrc_last_gradient(const ap_nrc* nrc, float* host_grad) {
     if (!nrc || !host_grad) return fail(AP_ERR_INVALID_ARG, "ap_nrc_last_gradient: NULL argument");
-    const hipError_t e = hipMemcpy(host_grad, nrc->s->grad, NPAR * sizeof(float), hipMemcpyDeviceToHost);
+    const hipError_t e = hipMemcpy(host_grad, nrc->s->grad, nrc->s->groups * nrc->s->npar * sizeof(float),
+                                   hipMemcpyDeviceToHost);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc_last_gradient");
 }
 

@@ -1964,6 +1964,8 @@ void ap_nrc_render_params_default(ap_nrc_render_params* p) {
 int ap_render_nrc(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, ap_nrc* cache,
                   const ap_nrc_render_params* nrc_params, const ap_framebuffers* fb, void* stream) {
     if (!cache || !nrc_params) return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: NULL cache or parameters");
+    if (nrc_groups(nrc_state(cache)) != 1)
+        return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: the cache must hold one network");
     if (params && params->use_guiding)
         return fail(AP_ERR_UNSUPPORTED, "ap_render_nrc: guiding is not supported with the radiance cache");
     if (!(nrc_params->spread > 0.0f) || !(nrc_params->learning_rate >= 0.0f))

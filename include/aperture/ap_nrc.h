@@ -14,6 +14,18 @@
  *
  * Parameters (21504 floats): five hidden matrices W[o][i] of 64 x 64, then
  * the 16 x 64 output matrix whose rows 3..15 stay zero.
+ *
+ * Grouped caches (ap_nrc_create_grouped) hold G independent networks of
+ * hidden width W = 64 or 32 (64 -> W x 5 -> 3; per network W x 64, four
+ * W x W, then 16 x W: 64W + 4W^2 + 16W floats, 6656 for W = 32), BF16 or FP16
+ * weights and activations, each with its own bounds; every launch serves
+ * all of them (a grouped GEMM, one network per 64-row tile). FP16 networks
+ * round to nearest even and saturate at +-65504, and scale the loss
+ * gradient by S = 2^(ceil(log2(3 batch)) - 4) before its FP16 rounding,
+ * dividing the FP32 weight gradient by S again. Parameters, gradients and
+ * Adam state are G consecutive blocks of one network's layout. The plain
+ * entry points (ap_nrc_infer, _gradient, _train_step) take caches of one
+ * network of any width and precision.
  */
 #ifndef APERTURE_AP_NRC_H
 #define APERTURE_AP_NRC_H
@@ -28,6 +40,8 @@ extern "C" {
 #endif
 
 #define AP_NRC_PARAMS 21504
+#define AP_NRC_BF16 0
+#define AP_NRC_FP16 1
 
 typedef struct ap_nrc ap_nrc;
 
@@ -57,6 +71,27 @@ int ap_nrc_last_gradient(const ap_nrc* nrc, float* host_grad);
 int ap_nrc_set_ema(ap_nrc* nrc, float decay);
 /* Bounds the position encoding uses (set by ap_nrc_set_bounds or ap_render_nrc). */
 int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]);
+
+/* Grouped caches: `groups` (1..4096) networks of hidden width 32 or 64 in
+ * AP_NRC_BF16 or AP_NRC_FP16; network g starts from PCG32(seed, 7 + g)
+ * (ap_nrc_create is groups 1, width 64, BF16). */
+int ap_nrc_create_grouped(uint32_t groups, uint32_t width, int32_t precision, uint64_t seed, ap_nrc** out);
+int ap_nrc_info(const ap_nrc* nrc, uint32_t* groups, uint32_t* width, int32_t* precision,
+                size_t* params_per_network);
+/* Bounds of network `group` (ap_nrc_set_bounds sets every network's). */
+int ap_nrc_set_group_bounds(ap_nrc* nrc, uint32_t group, const float bmin[3], const float bmax[3]);
+/* Inference of every network at once: query rows [host_seg[g],
+ * host_seg[g + 1]) go to network g (G + 1 non-decreasing offsets), output
+ * row r is query row r. Waits for `stream` before it rewrites its row map,
+ * then launches asynchronously. */
+int ap_nrc_infer_grouped(ap_nrc* nrc, const float* queries, const uint32_t* host_seg, float* out_rgb, void* stream);
+/* Gradient (synchronous; host_grad G x params, host_loss G floats, each
+ * network's mean loss) and training step (asynchronous) of every network on
+ * G x batch device samples laid out [network][batch], batch a positive
+ * multiple of 256, G x batch at most 2^24. */
+int ap_nrc_gradient_grouped(ap_nrc* nrc, const float* samples, size_t batch, float* host_grad, float* host_loss,
+                            void* stream);
+int ap_nrc_train_step_grouped(ap_nrc* nrc, const float* samples, size_t batch, float lr, void* stream);
 
 /* Rendering with the cache.
  *

@@ -35,12 +35,15 @@ double or_nrc_grad(const float* params, const float* feat, const float* target, 
                    double* grad);
 double or_nrc_loss_fixed(const float* params, const float* feat, const float* target, const double* den, size_t n,
                          int round_bf16);
+// Adam over n parameters (any network width or number of networks).
+void or_nrc_adam_n(float* params, float* m, float* v, const float* grad, size_t n, int t, float lr, float beta1,
+                   float beta2, float eps);
 void or_nrc_adam(float* params, float* m, float* v, const float* grad, int t, float lr, float beta1, float beta2,
                  float eps);
 size_t or_nrc_params_w(int width);
 void or_nrc_forward_w(const float* params, int width, const float* feat, size_t n, int mode, float* out);
 double or_nrc_grad_w(const float* params, int width, const float* feat, const float* target, size_t n, int mode,
-                     double* grad);
+                     double scale, double* grad);
 double or_nrc_loss_fixed_w(const float* params, int width, const float* feat, const float* target, const double* den,
                            size_t n, int mode);
 void or_f16_round(const float* x, size_t n, float* out);

@@ -132,8 +132,14 @@ void forward(const Net& net, const float* feat, bool rnd, double acts[NH + 1][H]
     }
 }
 
+// FP16 rounding saturated at +-65504 (what the FP16 networks store).
+double f16sat(double x) {
+    const double r = f16(x);
+    return std::isinf(r) ? std::copysign(65504.0, r) : r;
+}
+
 // Width-w network with a rounding mode (0 none, 1 BF16, 2 FP16).
-double rnd(double x, int mode) { return mode == 1 ? bf16(x) : (mode == 2 ? f16(x) : x); }
+double rnd(double x, int mode) { return mode == 1 ? bf16(x) : (mode == 2 ? f16sat(x) : x); }
 
 struct NetW {
     int w;
@@ -185,9 +191,11 @@ void or_nrc_forward_w(const float* params, int width, const float* feat, size_t 
     }
 }
 
-// Gradient of the mean relative L2 loss of the width-w network (as or_nrc_grad).
+// Gradient of the mean relative L2 loss of the width-w network (as
+// or_nrc_grad), with the loss gradient scaled by `scale` before its rounding
+// and the weight gradient divided by it again.
 double or_nrc_grad_w(const float* params, int width, const float* feat, const float* target, size_t n, int mode,
-                     double* grad) {
+                     double scale, double* grad) {
     const NetW net(params, width, mode);
     const size_t np = or_nrc_params_w(width);
     for (size_t k = 0; k < np; ++k) grad[k] = 0.0;
@@ -209,7 +217,7 @@ double or_nrc_grad_w(const float* params, int width, const float* feat, const fl
             const double den = y[o] * y[o] + 0.01;
             const double e = y[o] - t;
             loss += e * e / den * norm;
-            dy[o] = rnd(2.0 * e / den * norm, mode);
+            dy[o] = rnd(2.0 * e / den * norm * scale, mode);
         }
         for (int o = 0; o < NOUT; ++o)
             for (int i = 0; i < w; ++i) grad[off[NH] + static_cast<size_t>(o) * w + i] += dy[o] * acts[NH][i];
@@ -234,6 +242,7 @@ double or_nrc_grad_w(const float* params, int width, const float* feat, const fl
             delta = prev;
         }
     }
+    for (size_t k = 0; k < np; ++k) grad[k] /= scale;
     return loss;
 }
 
@@ -363,11 +372,11 @@ double or_nrc_loss_fixed(const float* params, const float* feat, const float* ta
 }
 
 // One Adam step (t = step number from 1) on FP32 master parameters.
-void or_nrc_adam(float* params, float* m, float* v, const float* grad, int t, float lr, float beta1, float beta2,
-                 float eps) {
+void or_nrc_adam_n(float* params, float* m, float* v, const float* grad, size_t n, int t, float lr, float beta1,
+                   float beta2, float eps) {
     const float c1 = 1.0f - std::pow(beta1, static_cast<float>(t));
     const float c2 = 1.0f - std::pow(beta2, static_cast<float>(t));
-    for (int k = 0; k < PARAMS; ++k) {
+    for (size_t k = 0; k < n; ++k) {
         const float g = grad[k];
         m[k] = beta1 * m[k] + (1.0f - beta1) * g;
         v[k] = beta2 * v[k] + (1.0f - beta2) * (g * g);
@@ -375,6 +384,11 @@ void or_nrc_adam(float* params, float* m, float* v, const float* grad, int t, fl
         const float vh = v[k] / c2;
         params[k] = params[k] - lr * mh / (std::sqrt(vh) + eps);
     }
+}
+
+void or_nrc_adam(float* params, float* m, float* v, const float* grad, int t, float lr, float beta1, float beta2,
+                 float eps) {
+    or_nrc_adam_n(params, m, v, grad, PARAMS, t, lr, beta1, beta2, eps);
 }
 
 }  // extern "C"

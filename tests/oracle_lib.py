@@ -72,11 +72,12 @@ def lib():
         L.or_nrc_loss_fixed.restype = ctypes.c_double
         L.or_nrc_loss_fixed.argtypes = [vp, vp, vp, vp, sz, i32]
         L.or_nrc_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32]
+        L.or_nrc_adam_n.argtypes = [vp, vp, vp, vp, sz, i32, f32, f32, f32, f32]
         L.or_nrc_params_w.restype = sz
         L.or_nrc_params_w.argtypes = [i32]
         L.or_nrc_forward_w.argtypes = [vp, i32, vp, sz, i32, vp]
         L.or_nrc_grad_w.restype = ctypes.c_double
-        L.or_nrc_grad_w.argtypes = [vp, i32, vp, vp, sz, i32, vp]
+        L.or_nrc_grad_w.argtypes = [vp, i32, vp, vp, sz, i32, ctypes.c_double, vp]
         L.or_f16_round.argtypes = [vp, sz, vp]
         L.or_nrc_loss_fixed_w.restype = ctypes.c_double
         L.or_nrc_loss_fixed_w.argtypes = [vp, i32, vp, vp, vp, sz, i32]
@@ -264,7 +265,10 @@ def nrc_loss_fixed(params, feat, target, den, round_bf16=False):
 
 def nrc_adam(params, m, v, grad, t, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8):
     params, m, v = (np.ascontiguousarray(a, np.float32).copy() for a in (params, m, v))
-    lib().or_nrc_adam(_p(params), _p(m), _p(v), _p(_f32(grad)), t, lr, beta1, beta2, eps)
+    if params.size == NRC_PARAMS:
+        lib().or_nrc_adam(_p(params), _p(m), _p(v), _p(_f32(grad)), t, lr, beta1, beta2, eps)
+    else:
+        lib().or_nrc_adam_n(_p(params), _p(m), _p(v), _p(_f32(grad)), params.size, t, lr, beta1, beta2, eps)
     return params, m, v
 
 
@@ -282,11 +286,13 @@ def nrc_forward_w(params, width, feat, mode):
     return out
 
 
-def nrc_grad_w(params, width, feat, target, mode):
+def nrc_grad_w(params, width, feat, target, mode, scale=1.0):
+    """Loss and FP64 gradient; `scale` multiplies the loss gradient before
+    its rounding and divides the weight gradient after."""
     feat = _f32(feat, (-1, 64))
     grad = np.zeros(nrc_params_w(width), np.float64)
     loss = lib().or_nrc_grad_w(_p(_f32(params)), width, _p(feat), _p(_f32(target, (-1, 3))), len(feat), mode,
-                               _p(grad))
+                               float(scale), _p(grad))
     return loss, grad
 
 

@@ -181,3 +181,42 @@ def test_width32_gradient_matches_finite_differences():
         assert abs(fd - grad[k]) < 1e-4 * max(1e-3, abs(grad[k])) + 1e-6, (k, fd, grad[k])
     assert np.all(grad[2048 + 4096 + 3 * 32:] == 0.0)
     assert ol.nrc_params_w(32) == 6656
+
+
+def test_loss_scale_is_exact_and_keeps_fp16_gradients_out_of_underflow():
+    """The loss-gradient scale multiplies before rounding and divides after:
+    without rounding a power of two changes no bit. In FP16 a batch of 4096
+    puts the unscaled loss gradient (~4e-5) among FP16's subnormals; the
+    scale 2^(ceil(log2(3 * 4096)) - 4) = 1024 lifts it out, so that the
+    gradient agrees with one scaled 64 times further (no underflow at all)
+    far more closely than the unscaled one does, and stays within FP16's
+    forward rounding of the FP64 gradient."""
+    p = init_params_w(32, seed=46)
+    n = 4096
+    feat = ol.nrc_encode(random_queries(n, seed=47), BMIN, BMAX)
+    t = G.uniform(0.1, 1.5, (n, 3)).astype(np.float32)
+    l1, g1 = ol.nrc_grad_w(p, 32, feat, t, ol.NRC_NONE)
+    l2, g2 = ol.nrc_grad_w(p, 32, feat, t, ol.NRC_NONE, scale=1024.0)
+    assert l1 == l2 and np.array_equal(g1, g2)
+    _, g_raw = ol.nrc_grad_w(p, 32, feat, t, ol.NRC_FP16)
+    _, g_sc = ol.nrc_grad_w(p, 32, feat, t, ol.NRC_FP16, scale=1024.0)
+    _, g_hi = ol.nrc_grad_w(p, 32, feat, t, ol.NRC_FP16, scale=1024.0 * 64)
+
+    def rel(g, h):
+        return float(np.linalg.norm(g - h) / np.linalg.norm(h))
+
+    assert rel(g_sc, g_hi) < 1e-7 and rel(g_raw, g_hi) > 100.0 * rel(g_sc, g_hi), (rel(g_sc, g_hi), rel(g_raw, g_hi))
+    assert rel(g_sc, g1) < 5e-3
+
+
+def test_fp16_mode_saturates():
+    """Weights beyond FP16's range become +-65504, not infinities, so the
+    network stays finite (an infinity times a zero activation would be NaN)."""
+    p = init_params_w(32, seed=48)
+    p[5] = 1e6
+    feat = ol.nrc_encode(random_queries(64, seed=49), BMIN, BMAX)
+    y = ol.nrc_forward_w(p, 32, feat, ol.NRC_FP16)
+    assert np.isfinite(y).all()
+    q = p.copy()
+    q[5] = 65504.0
+    assert np.array_equal(y, ol.nrc_forward_w(q, 32, feat, ol.NRC_FP16))
