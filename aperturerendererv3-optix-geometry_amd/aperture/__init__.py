@@ -175,7 +175,8 @@ EXPORTED_SYMBOLS = (
     "ap_nrc_set_ema", "ap_nrc_render_params_default", "ap_nrc_create_grouped", "ap_nrc_info",
     "ap_nrc_set_group_bounds", "ap_nrc_infer_grouped", "ap_nrc_gradient_grouped", "ap_nrc_train_step_grouped",
     "ap_batch_create", "ap_batch_destroy", "ap_batch_render", "ap_batch_set_launch", "ap_batch_get_launch",
-    "ap_batch_set_graph", "ap_batch_autotune", "ap_render_nrc", "ap_nrc_last_frame",
+    "ap_batch_set_graph", "ap_batch_autotune", "ap_render_nrc", "ap_nrc_last_frame", "ap_batch_render_nrc",
+    "ap_batch_nrc_last_frame",
 )
 
 
@@ -222,6 +223,8 @@ def lib():
         L.ap_batch_autotune.argtypes = [vp, vp, i32, vp, vp, vp, vp]
         L.ap_render_nrc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.ap_nrc_last_frame.argtypes = [vp, vp, vp, sz]
+        L.ap_batch_render_nrc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.ap_batch_nrc_last_frame.argtypes = [vp, vp, sz]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
@@ -645,6 +648,24 @@ class Batch:
         rays = (ctypes.c_uint64 * 2)()
         check(lib().ap_batch_render(self.handle, ctypes.byref(params), h, sm, rays, _stream(stream)))
         return int(rays[0]), int(rays[1])
+
+    def render_nrc(self, params, cache, nrc_params, hdrs, sums=None, stream=None):
+        """render() with per-scene radiance caches: `cache` is a RadianceCache
+        with one network per scene. Returns the rays traced."""
+        n = len(self._scenes)
+        h = (ctypes.c_void_p * n)(*[_ptr(x) if x is not None else None for x in hdrs])
+        sm = None if sums is None else (ctypes.c_void_p * n)(*[_ptr(x) if x is not None else None for x in sums])
+        rays = (ctypes.c_uint64 * 2)()
+        check(lib().ap_batch_render_nrc(self.handle, ctypes.byref(params), cache.handle, ctypes.byref(nrc_params), h,
+                                        sm, rays, _stream(stream)))
+        return int(rays[0]), int(rays[1])
+
+    def nrc_last_frame(self):
+        """Per-scene NrcFrameStats of the last render_nrc frame."""
+        n = len(self._scenes)
+        st = (NrcFrameStats * n)()
+        check(lib().ap_batch_nrc_last_frame(self.handle, st, n))
+        return list(st)
 
     def set_launch(self, wave_rays, refill):
         check(lib().ap_batch_set_launch(self.handle, wave_rays, refill))

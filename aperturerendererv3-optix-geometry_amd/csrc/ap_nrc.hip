@@ -534,23 +534,29 @@ __global__ void adam_kernel(uint32_t groups, float* __restrict__ params, float* 
     w_inf[p] = ema_decay > 0.0f ? cvt<F16>(e / ema_c) : h;
 }
 
-// Row j of step `step`: sample (off + (step * batch + j) * P) mod N of the
-// pool, P = 2^31 - 1 a prime above any pool size, so that the rows of the
-// steps of a frame walk the pool without repetition while they last.
+// Row j of step `step` of network g: sample (off + (step * batch + j) * P)
+// mod N_g of the network's pool (pool_n[g] samples from pool_begin[g], or
+// from 0 without pool_begin), P = 2^31 - 1 a prime above any pool size, so
+// that the rows of the steps of a frame walk the pool without repetition
+// while they last. Rows are laid out [network][batch].
 __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ pool,
+                                                     const uint32_t* __restrict__ pool_begin,
                                                      const uint32_t* __restrict__ pool_n, uint32_t batch,
-                                                     uint32_t step, uint64_t off, float* __restrict__ out) {
-    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-    if (j >= batch) return;
-    const uint32_t n = *pool_n;
-    float4* dst = reinterpret_cast<float4*>(out + static_cast<size_t>(j) * SREC);
+                                                     uint32_t groups, uint32_t step, uint64_t off,
+                                                     float* __restrict__ out) {
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= batch * groups) return;
+    const uint32_t g = r / batch, j = r % batch;
+    const uint32_t n = pool_n[g];
+    float4* dst = reinterpret_cast<float4*>(out + static_cast<size_t>(r) * SREC);
     if (n == 0u) {
         for (int k = 0; k < SREC / 4; ++k) dst[k] = float4{0.0f, 0.0f, 0.0f, 0.0f};
         return;
     }
     const uint64_t idx =
         (off % n + (static_cast<uint64_t>(step) * batch + j) % n * 2147483647ull % n) % n;
-    const float4* src = reinterpret_cast<const float4*>(pool + idx * SREC);
+    const size_t base = pool_begin ? pool_begin[g] : 0u;
+    const float4* src = reinterpret_cast<const float4*>(pool + (base + idx) * SREC);
     for (int k = 0; k < SREC / 4; ++k) dst[k] = src[k];
 }
 
@@ -741,28 +747,53 @@ int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const 
     return infer(s, queries, GroupMap{}, rows, dyn_n, n_cap, out, st);
 }
 
-int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_n, uint32_t batch, uint32_t steps, float lr,
-                   uint64_t seed, hipStream_t st) {
+int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_begin, const uint32_t* pool_n,
+                   uint32_t batch, uint32_t steps, float lr, uint64_t seed, hipStream_t st) {
     if (steps == 0) return AP_OK;
-    if (batch == 0 || batch % kChunkRows != 0 || batch > (1u << 24))
-        return fail(AP_ERR_INVALID_ARG, "ap_nrc: batch must be a positive multiple of 256 up to 2^24");
-    if (s->batch_cap < batch) {
+    if (batch == 0 || batch % kChunkRows != 0 || batch * static_cast<uint64_t>(s->groups) > (1u << 24))
+        return fail(AP_ERR_INVALID_ARG, "ap_nrc: batch must be a positive multiple of 256, at most 2^24 rows in all");
+    const uint32_t rows = batch * s->groups;
+    if (s->batch_cap < rows) {
         if (s->batch) (void)hipFree(s->batch);
         s->batch = nullptr;
         s->batch_cap = 0;
-        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->batch), static_cast<size_t>(batch) * SREC * 4);
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->batch), static_cast<size_t>(rows) * SREC * 4);
         if (e != hipSuccess) return hip_fail(e, "ap_nrc: batch buffer");
-        s->batch_cap = batch;
+        s->batch_cap = rows;
     }
     const uint64_t off = splitmix64(seed);
+    const int per_group = s->groups > 1 ? 1 : 0;
     for (uint32_t k = 0; k < steps; ++k) {
-        hipLaunchKernelGGL(gather_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, pool, pool_n, batch, k, off,
-                           s->batch);
-        int rc = nrc_gradient(s, s->batch, batch, st, pool_n);
-        if (rc == AP_OK) rc = nrc_adam(s, lr, st, pool_n);
+        hipLaunchKernelGGL(gather_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, pool, pool_begin, pool_n,
+                           batch, s->groups, k, off, s->batch);
+        int rc = nrc_gradient(s, s->batch, batch, st, pool_n, per_group);
+        if (rc == AP_OK) rc = nrc_adam(s, lr, st, pool_n, per_group);
         if (rc != AP_OK) return rc;
     }
     return AP_OK;
+}
+
+int nrc_infer_groups(NrcState* s, const float* queries, const int32_t* rows, const uint32_t* seg,
+                     const uint32_t* tile0, uint32_t n_bound, float* out, hipStream_t st) {
+    if (n_bound == 0) return AP_OK;
+    GroupMap gm;
+    gm.groups = s->groups;
+    gm.seg = seg;
+    gm.tile0 = tile0;
+    gm.bounds = s->dev_bounds;
+    return infer(s, queries, gm, rows, nullptr, n_bound, out, st);
+}
+
+int nrc_set_group_bounds(NrcState* s, uint32_t g, const float lo[3], const float hi[3]) {
+    Bounds b;
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = lo[a];
+        b.hi[a] = hi[a];
+    }
+    if (g == 0) s->bounds = b;
+    if (std::memcmp(&s->host_bounds[g], &b, sizeof(Bounds)) == 0) return AP_OK;
+    s->host_bounds[g] = b;
+    return upload_bounds(s);
 }
 
 void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]) {

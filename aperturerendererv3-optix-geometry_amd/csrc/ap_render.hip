@@ -583,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     ap_ray* __restrict__ shadow_rays, ShadowPay<NL>* __restrict__ shadow_pay, uint32_t* __restrict__ counters,
     int32_t* __restrict__ primary_id, int32_t primary_frame, unsigned long long* __restrict__ ray_count, GuideView gv,
     NrcView nv, BatchView bv = BatchView{}) {
-    static_assert(!MULTI || (!GUIDE && !NRC), "a batch renders without guiding or the cache");
+    static_assert(!MULTI || !GUIDE, "a batch renders without guiding");
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
     if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
@@ -955,19 +955,14 @@ __device__ __forceinline__ bool finite3(f3 v) {
     return !(isnan(v.x) || isnan(v.y) || isnan(v.z) || isinf(v.x) || isinf(v.y) || isinf(v.z));
 }
 
-// Targets of the training vertices, XYZ2SRGB(sum_j xyz_j (L_end,j - L_k,j) /
-// M_k,j / psum), written over [13..15]; vmask marks the finite ones. With
-// several wavelengths a wavelength whose throughput at the vertex is zero
-// adds nothing.
+// Targets of the training vertices of slot t (path pid), XYZ2SRGB(sum_j
+// xyz_j (L_end,j - L_k,j) / M_k,j / psum), written over [13..15]; returns
+// the mask of the finite ones. With several wavelengths a wavelength whose
+// throughput at the vertex is zero adds nothing.
 template <int NL>
-__global__ __launch_bounds__(kBlock) void nrc_targets_kernel(uint32_t n_train, uint32_t stride, uint32_t offset,
-                                                             uint32_t pid_direct, const NrcPath* __restrict__ npaths,
-                                                             const PathT<NL>* __restrict__ paths,
-                                                             const float* __restrict__ tables,
-                                                             float* __restrict__ verts, uint32_t* __restrict__ vmask) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n_train) return;
-    const uint32_t pid = pid_direct ? t : offset + t * stride;
+__device__ __forceinline__ uint32_t nrc_slot_targets(uint32_t t, uint32_t pid, const NrcPath* __restrict__ npaths,
+                                                     const PathT<NL>* __restrict__ paths,
+                                                     const float* __restrict__ tables, float* __restrict__ verts) {
     const uint32_t nvx = min(npaths[pid].flags >> 8, static_cast<uint32_t>(kNrcV));
     const PathT<NL>& S = paths[pid];
     uint32_t m = 0;
@@ -984,7 +979,21 @@ __global__ __launch_bounds__(kBlock) void nrc_targets_kernel(uint32_t n_train, u
             m |= 1u << k;
         }
     }
-    vmask[t] = m;
+    return m;
+}
+
+// Targets of every training slot (slot t: pixel offset + t * stride, or
+// path t with pid_direct); vmask marks the finite ones.
+template <int NL>
+__global__ __launch_bounds__(kBlock) void nrc_targets_kernel(uint32_t n_train, uint32_t stride, uint32_t offset,
+                                                             uint32_t pid_direct, const NrcPath* __restrict__ npaths,
+                                                             const PathT<NL>* __restrict__ paths,
+                                                             const float* __restrict__ tables,
+                                                             float* __restrict__ verts, uint32_t* __restrict__ vmask) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_train) return;
+    const uint32_t pid = pid_direct ? t : offset + t * stride;
+    vmask[t] = nrc_slot_targets<NL>(t, pid, npaths, paths, tables, verts);
 }
 
 // Exclusive prefix sum of the per-slot sample counts in one block (slot
@@ -1015,14 +1024,16 @@ __global__ __launch_bounds__(1024) void nrc_scan_kernel(uint32_t n_train, const 
 
 // Training samples in slot, vertex order: query (13 floats), 3 zeros, target
 // RGB, pixel * 8 + vertex as uint32 bits.
+// With slot_pix, slot t's pixel is slot_pix[t] (a batch: the path index).
 __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(uint32_t n_train, uint32_t stride, uint32_t offset,
                                                              const uint32_t* __restrict__ vmask,
                                                              const uint32_t* __restrict__ offsets,
-                                                             const float* __restrict__ verts, float* __restrict__ pool) {
+                                                             const float* __restrict__ verts, float* __restrict__ pool,
+                                                             const uint32_t* __restrict__ slot_pix = nullptr) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n_train) return;
     const uint32_t m = vmask[t];
-    const uint32_t pix = offset + t * stride;
+    const uint32_t pix = slot_pix ? slot_pix[t] : offset + t * stride;
     uint32_t j = offsets[t];
     for (int k = 0; k < kNrcV; ++k) {
         if (!((m >> k) & 1u)) continue;
@@ -1200,6 +1211,160 @@ __global__ __launch_bounds__(kBlock) void accumulate_batch_kernel(BatchView bv, 
 
 __global__ void advance_frame_kernel(int32_t* frame) {
     if (threadIdx.x == 0) *frame += 1;
+}
+
+// Radiance-cache layout of one scene of a batch: its training-path stride,
+// the most training paths one frame can give it, and its first training slot
+// (slots of scene s: [slot_base, slot_base + n_train_max)).
+struct NrcBatchScene {
+    uint32_t stride, n_train_max, slot_base, pad;
+};
+
+// Training-path offset and count of a scene for the frame hash h
+// (ap_render_nrc's rule, per scene).
+__device__ __forceinline__ void nrc_scene_train(const NrcBatchScene& T, uint32_t n_pix, uint64_t h, uint32_t& offset,
+                                                uint32_t& n_train) {
+    offset = 0;
+    n_train = 0;
+    if (T.n_train_max == 0) return;
+    offset = static_cast<uint32_t>(h % T.stride);
+    n_train = min(T.n_train_max, (n_pix - offset + T.stride - 1) / T.stride);
+}
+
+// nrc_init_kernel over every scene of a batch; also clears the per-scene
+// query counters.
+__global__ __launch_bounds__(kBlock) void nrc_init_batch_kernel(BatchView bv, uint32_t total,
+                                                                const NrcBatchScene* __restrict__ T, uint64_t h,
+                                                                uint32_t unbiased_every, NrcPath* __restrict__ paths,
+                                                                uint32_t* __restrict__ counters,
+                                                                uint32_t* __restrict__ qcount,
+                                                                uint32_t* __restrict__ qcursor) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) { counters[0] = 0; counters[1] = 0; }
+    if (i < static_cast<uint32_t>(bv.n_scenes)) { qcount[i] = 0; qcursor[i] = 0; }
+    if (i >= total) return;
+    const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
+    const uint32_t p = i - bv.pix_begin[sc], n_pix = bv.pix_begin[sc + 1] - bv.pix_begin[sc];
+    const NrcBatchScene Ts = T[sc];
+    uint32_t offset, n_train;
+    nrc_scene_train(Ts, n_pix, h, offset, n_train);
+    NrcPath P{0.0f, 0.0f, 0u, -1};
+    if (n_train > 0 && p >= offset && (p - offset) % Ts.stride == 0u) {
+        const uint32_t t = (p - offset) / Ts.stride;
+        if (t < n_train) {
+            P.flags = NRC_TRAIN | ((unbiased_every != 0u && t % unbiased_every == 0u) ? NRC_UNBIASED : 0u);
+            P.slot = static_cast<int32_t>(Ts.slot_base + t);
+        }
+    }
+    paths[i] = P;
+}
+
+// Cache queries per scene (the scene of the querying path).
+template <int NL>
+__global__ __launch_bounds__(kBlock) void nrc_query_count_kernel(const uint32_t* __restrict__ counters,
+                                                                 const int32_t* __restrict__ qpath,
+                                                                 const PathT<NL>* __restrict__ paths,
+                                                                 uint32_t* __restrict__ qcount) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= counters[0]) return;
+    atomicAdd(&qcount[paths[qpath[q]].flags >> 16], 1u);
+}
+
+// Row segments of the scenes' queries: seg[s] = first row of scene s,
+// tile0[s] = its first 64-row tile (exclusive prefix sums, one block).
+__global__ __launch_bounds__(1024) void nrc_query_segments_kernel(uint32_t n, const uint32_t* __restrict__ qcount,
+                                                                  uint32_t* __restrict__ seg,
+                                                                  uint32_t* __restrict__ tile0) {
+    __shared__ uint32_t rows[1024], tiles[1024];
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t lo = threadIdx.x * per, hi = min(lo + per, n);
+    uint32_t sr = 0, st = 0;
+    for (uint32_t s = lo; s < hi; ++s) {
+        sr += qcount[s];
+        st += (qcount[s] + 63u) / 64u;
+    }
+    rows[threadIdx.x] = sr;
+    tiles[threadIdx.x] = st;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint32_t ar = threadIdx.x >= d ? rows[threadIdx.x - d] : 0u;
+        const uint32_t at = threadIdx.x >= d ? tiles[threadIdx.x - d] : 0u;
+        __syncthreads();
+        rows[threadIdx.x] += ar;
+        tiles[threadIdx.x] += at;
+        __syncthreads();
+    }
+    uint32_t r = rows[threadIdx.x] - sr, t = tiles[threadIdx.x] - st;
+    for (uint32_t s = lo; s < hi; ++s) {
+        seg[s] = r;
+        tile0[s] = t;
+        r += qcount[s];
+        t += (qcount[s] + 63u) / 64u;
+    }
+    if (threadIdx.x == 1023u) {
+        seg[n] = rows[1023];
+        tile0[n] = tiles[1023];
+    }
+}
+
+// Queries grouped by scene: scene s's query paths at rows [seg[s], seg[s+1]).
+template <int NL>
+__global__ __launch_bounds__(kBlock) void nrc_query_scatter_kernel(const uint32_t* __restrict__ counters,
+                                                                   const int32_t* __restrict__ qpath,
+                                                                   const PathT<NL>* __restrict__ paths,
+                                                                   const uint32_t* __restrict__ seg,
+                                                                   uint32_t* __restrict__ qcursor,
+                                                                   int32_t* __restrict__ sorted) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= counters[0]) return;
+    const int32_t pid = qpath[q];
+    const uint32_t sc = paths[pid].flags >> 16;
+    sorted[seg[sc] + atomicAdd(&qcursor[sc], 1u)] = pid;
+}
+
+// nrc_targets_kernel over the training slots of every scene of a batch;
+// slot_pix[t] = the slot's path.
+template <int NL>
+__global__ __launch_bounds__(kBlock) void nrc_targets_batch_kernel(uint32_t slots, BatchView bv,
+                                                                   const NrcBatchScene* __restrict__ T, uint64_t h,
+                                                                   const NrcPath* __restrict__ npaths,
+                                                                   const PathT<NL>* __restrict__ paths,
+                                                                   const float* __restrict__ tables,
+                                                                   float* __restrict__ verts,
+                                                                   uint32_t* __restrict__ vmask,
+                                                                   uint32_t* __restrict__ slot_pix) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= slots) return;
+    int32_t lo = 0, hi = bv.n_scenes - 1;  // the last scene whose slots start at or before t
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (T[mid].slot_base <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const NrcBatchScene Ts = T[lo];
+    uint32_t offset, n_train;
+    nrc_scene_train(Ts, bv.pix_begin[lo + 1] - bv.pix_begin[lo], h, offset, n_train);
+    const uint32_t k = t - Ts.slot_base;
+    if (k >= n_train) {
+        vmask[t] = 0u;
+        slot_pix[t] = 0u;
+        return;
+    }
+    const uint32_t pid = bv.pix_begin[lo] + offset + k * Ts.stride;
+    slot_pix[t] = pid;
+    vmask[t] = nrc_slot_targets<NL>(t, pid, npaths, paths, tables, verts);
+}
+
+// Each scene's training samples: pool rows [pool_begin[s], + pool_n[s]).
+__global__ void nrc_pool_ranges_kernel(int32_t n, uint32_t slots, const NrcBatchScene* __restrict__ T,
+                                       const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ counters,
+                                       uint32_t* __restrict__ pool_begin, uint32_t* __restrict__ pool_n) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t b = T[s].slot_base, e = b + T[s].n_train_max, all = counters[1];
+    const uint32_t pb = b < slots ? offsets[b] : all, pe = e < slots ? offsets[e] : all;
+    pool_begin[s] = pb;
+    pool_n[s] = pe - pb;
 }
 
 __global__ void build_mats_kernel(const MatIn* __restrict__ in, int n, const f4* __restrict__ lut,
@@ -1575,7 +1740,7 @@ int run_frames(const RunCtx& c) {
         if (guiding && !train_only && (rc2 = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK)
             return rc2;
         if (nrc && n_train > 0 && np->train_steps > 0) {
-            rc2 = nrc_train_pool(ns, r->nrc_pool, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
+            rc2 = nrc_train_pool(ns, r->nrc_pool, nullptr, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
                                  train_seed, stream);
             if (rc2 != AP_OK) return rc2;
         }
@@ -1731,6 +1896,31 @@ struct BatchState {
     ap_render_params key{};
     int32_t key_wave_rays = 0, key_refill = 0;
     hipStream_t key_stream = nullptr;
+    // Per-scene radiance caches (ap_batch_render_nrc).
+    uint32_t nrc_slots = 0;                 // training slots of all scenes
+    uint32_t nrc_table_paths = 0xffffffffu; // train_paths the table was built for
+    std::vector<NrcBatchScene> h_nrc_table;
+    NrcBatchScene* nrc_table = nullptr;
+    NrcPath* nrc_paths = nullptr;
+    float* nrc_queries = nullptr;
+    int32_t* nrc_qpath = nullptr;
+    int32_t* nrc_qsorted = nullptr;
+    float* nrc_qrgb = nullptr;
+    uint32_t* nrc_counters = nullptr;       // [0] queries, [1] training samples
+    uint32_t* nrc_qcount = nullptr;         // per scene: queries, then their segments
+    uint32_t* nrc_qcursor = nullptr;
+    uint32_t* nrc_seg = nullptr;
+    uint32_t* nrc_tile0 = nullptr;
+    uint32_t* nrc_pool_begin = nullptr;
+    uint32_t* nrc_pool_n = nullptr;
+    size_t nrc_slot_cap = 0;
+    float* nrc_verts = nullptr;
+    uint32_t* nrc_vmask = nullptr;
+    uint32_t* nrc_offsets = nullptr;
+    uint32_t* nrc_slot_pix = nullptr;
+    float* nrc_pool = nullptr;
+    uint64_t nrc_last_hash = 0;
+    bool nrc_rendered = false;
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1743,7 +1933,9 @@ struct BatchState {
         drop_graph();
         void* ptrs[] = {d_scenes, d_devs, d_pix_begin, paths, rays[0], rays[1], path_of[0], path_of[1], hits,
                         shadow_rays, shadow_payload, shadow_hits, ray_scene[0], ray_scene[1], shadow_scene, counters,
-                        frame, rays_traced, eps};
+                        frame, rays_traced, eps, nrc_table, nrc_paths, nrc_queries, nrc_qpath, nrc_qsorted, nrc_qrgb,
+                        nrc_counters, nrc_qcount, nrc_qcursor, nrc_seg, nrc_tile0, nrc_pool_begin, nrc_pool_n,
+                        nrc_verts, nrc_vmask, nrc_offsets, nrc_slot_pix, nrc_pool};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
     }
@@ -1877,6 +2069,152 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
         e = hipGraphLaunch(b->exec, stream);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: graph launch");
     }
+    return AP_OK;
+}
+
+// Buffers, per-scene training layout and cache bounds of rendering a batch
+// with per-scene caches.
+int batch_nrc_prepare(BatchState* b, const ap_nrc_render_params* np, NrcState* ns) {
+    const size_t n = b->scenes.size(), cap = b->total;
+    int rc = AP_OK;
+    if (!b->nrc_paths) {
+        if ((rc = dev_alloc(&b->nrc_paths, cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_queries, cap * kNrcRec)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_qpath, cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_qsorted, cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_qrgb, cap * 3)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_counters, 4)) != AP_OK) return rc;
+        for (uint32_t** p : {&b->nrc_qcount, &b->nrc_qcursor, &b->nrc_pool_begin, &b->nrc_pool_n})
+            if ((rc = dev_alloc(p, n)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_seg, n + 1)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_tile0, n + 1)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_table, n)) != AP_OK) return rc;
+    }
+    if (b->nrc_table_paths != np->train_paths) {
+        // Per scene, ap_render_nrc's stride and most training paths; slots scene after scene.
+        b->h_nrc_table.assign(n, NrcBatchScene{1u, 0u, 0u, 0u});
+        uint64_t slots = 0;
+        for (size_t k = 0; k < n; ++k) {
+            NrcBatchScene& T = b->h_nrc_table[k];
+            const uint32_t n_pix = b->pix_begin[k + 1] - b->pix_begin[k];
+            if (np->train_paths > 0) {
+                T.stride = std::max<uint32_t>(1u, n_pix / np->train_paths);
+                T.n_train_max = (n_pix + T.stride - 1) / T.stride;
+            }
+            T.slot_base = static_cast<uint32_t>(slots);
+            slots += T.n_train_max;
+        }
+        const hipError_t e = hipMemcpy(b->nrc_table, b->h_nrc_table.data(), n * sizeof(NrcBatchScene),
+                                       hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: training layout");
+        b->nrc_slots = static_cast<uint32_t>(slots);
+        b->nrc_table_paths = np->train_paths;
+    }
+    const size_t slot_cap = std::max<size_t>(b->nrc_slots, 1);
+    if (b->nrc_slot_cap < slot_cap) {
+        void* old[] = {b->nrc_verts, b->nrc_vmask, b->nrc_offsets, b->nrc_slot_pix, b->nrc_pool};
+        for (void* p : old)
+            if (p) (void)hipFree(p);
+        b->nrc_verts = nullptr; b->nrc_vmask = nullptr; b->nrc_offsets = nullptr; b->nrc_slot_pix = nullptr;
+        b->nrc_pool = nullptr;
+        b->nrc_slot_cap = 0;
+        if ((rc = dev_alloc(&b->nrc_verts, slot_cap * kNrcV * kNrcVert)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_vmask, slot_cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_offsets, slot_cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_slot_pix, slot_cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_pool, slot_cap * kNrcV * kNrcSample)) != AP_OK) return rc;
+        b->nrc_slot_cap = slot_cap;
+    }
+    // Network k's bounds: scene k's root box grown on each side by 1e-3 of its
+    // largest extent (ap_render_nrc's).
+    for (size_t k = 0; k < n; ++k) {
+        const DeviceScene& d = b->scenes[k]->dev;
+        if (!d.root.valid) continue;
+        float lo[3], hi[3], ext = 0.0f;
+        for (int a = 0; a < 3; ++a) ext = std::max(ext, d.root.bmax[a] - d.root.bmin[a]);
+        const float m = 1.0e-3f * std::max(ext, 1.0e-3f);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = d.root.bmin[a] - m;
+            hi[a] = d.root.bmax[a] + m;
+        }
+        if ((rc = nrc_set_group_bounds(ns, static_cast<uint32_t>(k), lo, hi)) != AP_OK) return rc;
+    }
+    return AP_OK;
+}
+
+// One batch frame with per-scene caches: batch_frame's bounces with paths
+// ending into their scene's network, the grouped inference of all networks,
+// the scenes' training samples, the running means, then the networks' Adam
+// steps on their own scenes' samples (one set of launches for all).
+template <int NL>
+int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_render_params* np, NrcState* ns,
+                    int32_t frame, hipStream_t stream, const f4* rgb_lut, const f4* lam_lut, const float* tables) {
+    const BatchView bv = b->view();
+    auto* paths = reinterpret_cast<PathT<NL>*>(b->paths);
+    auto* pays = reinterpret_cast<ShadowPay<NL>*>(b->shadow_payload);
+    const uint32_t total = b->total;
+    const int32_t n = static_cast<int32_t>(b->scenes.size());
+    const uint64_t h = nrc_frame_hash(prm->seed, frame);
+    b->nrc_last_hash = h;
+    hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
+    hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
+                       b->eps, paths, b->rays[0], b->path_of[0]);
+    hipLaunchKernelGGL(nrc_init_batch_kernel, grid_of(total), dim3(kBlock), 0, stream, bv, total, b->nrc_table, h,
+                       np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount, b->nrc_qcursor);
+    const NrcView nv{b->nrc_paths, b->nrc_queries, b->nrc_qpath, b->nrc_verts, b->nrc_counters, np->spread};
+    int in_q = 0;
+    for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
+        const int out_q = 1 - in_q;
+        hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, out_q);
+        hipError_t e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[in_q], b->rays[in_q], total, b->hits,
+                                          false, stream, &b->counters[in_q]);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
+        const ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
+                             prm->nee_always, prm->sky, in_q, out_q, 0};
+        hipLaunchKernelGGL((shade_kernel<NL, false, true, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
+                           nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
+                           b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
+                           b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, nv, bv);
+        e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
+                               stream, &b->counters[2]);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
+        hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
+                           b->shadow_hits, paths, b->rays_traced);
+        in_q = out_q;
+    }
+    // Cache vertices: queries grouped by scene, every network in one inference.
+    hipLaunchKernelGGL(nrc_query_count_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->nrc_counters,
+                       b->nrc_qpath, paths, b->nrc_qcount);
+    hipLaunchKernelGGL(nrc_query_segments_kernel, dim3(1), dim3(1024), 0, stream, static_cast<uint32_t>(n),
+                       b->nrc_qcount, b->nrc_seg, b->nrc_tile0);
+    hipLaunchKernelGGL(nrc_query_scatter_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->nrc_counters,
+                       b->nrc_qpath, paths, b->nrc_seg, b->nrc_qcursor, b->nrc_qsorted);
+    int rc = nrc_infer_groups(ns, b->nrc_queries, b->nrc_qsorted, b->nrc_seg, b->nrc_tile0,
+                              total + 64u * static_cast<uint32_t>(n), b->nrc_qrgb, stream);
+    if (rc != AP_OK) return rc;
+    hipLaunchKernelGGL(nrc_apply_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->nrc_counters,
+                       b->nrc_qsorted, b->nrc_qrgb, rgb_lut, lam_lut, tables, paths);
+    const uint32_t slots = b->nrc_slots;
+    if (slots > 0) {
+        hipLaunchKernelGGL(nrc_targets_batch_kernel<NL>, grid_of(slots), dim3(kBlock), 0, stream, slots, bv,
+                           b->nrc_table, h, b->nrc_paths, paths, tables, b->nrc_verts, b->nrc_vmask,
+                           b->nrc_slot_pix);
+        hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, slots, b->nrc_vmask, b->nrc_offsets,
+                           b->nrc_counters);
+        hipLaunchKernelGGL(nrc_pool_ranges_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, slots,
+                           b->nrc_table, b->nrc_offsets, b->nrc_counters, b->nrc_pool_begin, b->nrc_pool_n);
+        hipLaunchKernelGGL(nrc_compact_kernel, grid_of(slots), dim3(kBlock), 0, stream, slots, 1u, 0u, b->nrc_vmask,
+                           b->nrc_offsets, b->nrc_verts, b->nrc_pool, b->nrc_slot_pix);
+    }
+    hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
+                       tables);
+    hipLaunchKernelGGL(advance_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "ap_batch: launch");
+    b->nrc_rendered = true;
+    if (slots > 0 && np->train_steps > 0)
+        return nrc_train_pool(ns, b->nrc_pool, b->nrc_pool_begin, b->nrc_pool_n, np->batch, np->train_steps,
+                              np->learning_rate, h ^ 0x747261696eull, stream);
     return AP_OK;
 }
 
@@ -2120,6 +2458,79 @@ int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* cons
         if (e != hipSuccess) return hip_fail(e, "ap_batch_render: ray counts");
         ray_count[0] = c[0];
         ray_count[1] = c[1];
+    }
+    return AP_OK;
+}
+
+int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc* cache,
+                        const ap_nrc_render_params* nrc_params, float* const* hdr, double* const* sums,
+                        uint64_t ray_count[2], void* stream) {
+    if (!batch || !params || !cache || !nrc_params) return fail(AP_ERR_INVALID_ARG, "ap_batch_render_nrc: NULL argument");
+    int rc = batch_check_params(params);
+    if (rc != AP_OK) return rc;
+    BatchState& B = batch->s;
+    NrcState* ns = nrc_state(cache);
+    if (nrc_groups(ns) != B.scenes.size())
+        return fail(AP_ERR_INVALID_ARG, "ap_batch_render_nrc: the cache must hold one network per scene");
+    if (!(nrc_params->spread > 0.0f) || !(nrc_params->learning_rate >= 0.0f))
+        return fail(AP_ERR_INVALID_ARG, "ap_batch_render_nrc: spread must be > 0 and learning_rate >= 0");
+    if (nrc_params->train_passes != 0)
+        return fail(AP_ERR_UNSUPPORTED, "ap_batch_render_nrc: training-only passes are not supported in a batch");
+    if (nrc_params->train_steps > 0 &&
+        (nrc_params->batch == 0 || nrc_params->batch % 256 != 0 ||
+         static_cast<uint64_t>(nrc_params->batch) * B.scenes.size() > (1u << 24)))
+        return fail(AP_ERR_INVALID_ARG,
+                    "ap_batch_render_nrc: batch must be a positive multiple of 256, at most 2^24 over all scenes");
+    if (params->spp == 0 || params->max_depth == 0) return AP_OK;
+    const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
+    const float* tables = nullptr;
+    if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
+    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut)) != AP_OK) return rc;
+    if ((rc = batch_nrc_prepare(&B, nrc_params, ns)) != AP_OK) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);
+    if (e == hipSuccess) e = hipMemcpy(B.frame, &params->frame_index, sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: reset counters");
+    // Frame after frame, each rendered with the networks as the frames before
+    // it trained them; launched directly (the Adam step count changes per frame).
+    for (int32_t f = 0; f < params->spp; ++f) {
+        const int32_t frame = params->frame_index + f;
+        rc = params->wavelengths == 4
+            ? batch_frame_nrc<4>(&B, params, nrc_params, ns, frame, st, rgb_lut, lam_lut, tables)
+            : batch_frame_nrc<1>(&B, params, nrc_params, ns, frame, st, rgb_lut, lam_lut, tables);
+        if (rc != AP_OK) return rc;
+    }
+    if (ray_count) {
+        unsigned long long c[2] = {0, 0};
+        e = hipMemcpyAsync(c, B.rays_traced, sizeof(c), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: ray counts");
+        ray_count[0] = c[0];
+        ray_count[1] = c[1];
+    }
+    return AP_OK;
+}
+
+int ap_batch_nrc_last_frame(ap_batch* batch, ap_nrc_frame_stats* per_scene, size_t n) {
+    if (!batch || !per_scene) return fail(AP_ERR_INVALID_ARG, "ap_batch_nrc_last_frame: NULL argument");
+    BatchState& B = batch->s;
+    if (!B.nrc_rendered) return fail(AP_ERR_INVALID_ARG, "ap_batch_nrc_last_frame: no ap_batch_render_nrc frame yet");
+    if (n != B.scenes.size()) return fail(AP_ERR_INVALID_ARG, "ap_batch_nrc_last_frame: one entry per scene");
+    std::vector<uint32_t> q(n), pn(n, 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(q.data(), B.nrc_qcount, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && B.nrc_slots > 0)
+        e = hipMemcpy(pn.data(), B.nrc_pool_n, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "ap_batch_nrc_last_frame");
+    for (size_t k = 0; k < n; ++k) {
+        const NrcBatchScene& T = B.h_nrc_table[k];
+        const uint32_t n_pix = B.pix_begin[k + 1] - B.pix_begin[k];
+        uint32_t n_train = 0;
+        if (T.n_train_max > 0) {
+            const uint32_t offset = static_cast<uint32_t>(B.nrc_last_hash % T.stride);
+            n_train = std::min(T.n_train_max, (n_pix - offset + T.stride - 1) / T.stride);
+        }
+        per_scene[k] = ap_nrc_frame_stats{q[k], n_train, pn[k], 0u};
     }
     return AP_OK;
 }

@@ -18,6 +18,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "ap_nrc.h"
 #include "ap_render.h"
 
 #ifdef __cplusplus
@@ -41,6 +42,25 @@ int ap_batch_destroy(ap_batch* batch);
  * for the first call with new parameters, which captures the graph. */
 int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* const* hdr, double* const* sums,
                     uint64_t ray_count[2], void* stream);
+
+/* ap_batch_render with per-scene learned radiance caches (ap_nrc.h): `cache`
+ * holds one network per scene (ap_nrc_create_grouped with groups = n, any
+ * width and precision), network k serving scene k with scene k's root box,
+ * grown as ap_render_nrc grows it, as its bounds. Frame after frame, every
+ * scene's paths end into its own network as ap_render_nrc describes, all
+ * networks answer in one grouped inference, and each trains on its own
+ * scene's samples (training paths chosen per scene as ap_render_nrc does,
+ * batch rows per network, batch x n at most 2^24) in the same launches. With
+ * a width-64 BF16 cache, scene k's image and network k's weights equal, bit
+ * for bit, ap_render_nrc of scene k alone with a one-network cache that
+ * starts from network k. train_passes must be 0. Frames are launched one by
+ * one (no graph). */
+int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc* cache,
+                        const ap_nrc_render_params* nrc_params, float* const* hdr, double* const* sums,
+                        uint64_t ray_count[2], void* stream);
+/* Per-scene counters of the last frame of ap_batch_render_nrc (n = scenes):
+ * queries, training paths, training samples. Synchronous. */
+int ap_batch_nrc_last_frame(ap_batch* batch, ap_nrc_frame_stats* per_scene, size_t n);
 
 /* Traversal launch shape of the batch (ap_scene_set_launch's meaning). */
 int ap_batch_set_launch(ap_batch* batch, int wave_rays, int refill);

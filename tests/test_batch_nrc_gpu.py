@@ -1,0 +1,134 @@
+"""GPU checks of batched rendering with per-scene radiance caches
+(ap_batch_render_nrc): one network per scene in a grouped cache, every
+scene's cache queries answered by one grouped inference and every network
+trained in the same launches.
+
+Bars:
+  * with width-64 BF16 networks the batch is ap_render_nrc scene by scene:
+    each scene's image and FP64 sums and each network's weights after
+    training equal, bit for bit, what ap_render_nrc gives for the scene
+    alone with a one-network cache starting from the same weights (which
+    tests/test_nrc_render_gpu.py ties to the oracle), and the per-scene
+    counters match;
+  * width-32 FP16 networks: the same paths end into the caches (query and
+    sample counts equal the BF16 run's), training is deterministic and moves
+    every network;
+  * trained per-scene FP16 caches lower each scene's relMSE against a
+    4096-spp plain reference well below the plain batch's at equal spp.
+"""
+import numpy as np
+import pytest
+
+import aperture as ap
+import test_batch_gpu as bg
+import test_nrc_cpu as ncpu
+import test_nrc_render_gpu as nr
+
+pytestmark = pytest.mark.gpu
+
+
+def batch_render_nrc(batch, cams, prm, cache, nprm, stream):
+    hdrs, sums = [], []
+    for c in cams:
+        n = c.width * c.height
+        h = ap.DeviceBuffer(n * 12)
+        h.zero()
+        hdrs.append(h)
+        sums.append(ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64)))
+    batch.render_nrc(prm, cache, nprm, hdrs, sums, stream=stream)
+    stream.synchronize()
+    out = []
+    for c, h, s in zip(cams, hdrs, sums):
+        n = c.width * c.height
+        out.append((h.to_numpy(np.float32, n * 3).reshape(c.height, c.width, 3),
+                    s.to_numpy(np.float64, n * 3).reshape(c.height, c.width, 3)))
+    return out
+
+
+def weights(n):
+    return np.stack([ncpu.init_params(60 + k, scale=0.7) for k in range(n)])
+
+
+@pytest.mark.parametrize("steps", [0, 2])
+def test_batch_caches_equal_single_scene_caches(gpu, steps):
+    print(f"[batch-nrc] width 64 BF16, train_steps {steps}", flush=True)
+    objs, cams = bg.scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    prm = ap.RenderParams.defaults(spp=3, max_depth=10, rr_min_depth=3, seed=21)
+    nprm = ap.NrcRenderParams.defaults(train_paths=256, batch=512, train_steps=steps, unbiased_every=4)
+    w = weights(len(scenes))
+    cache = ap.RadianceCache(seed=1, groups=len(scenes), width=64, precision="bf16")
+    cache.set_params(w)
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    got = batch_render_nrc(batch, cams, prm, cache, nprm, stream)
+    stats = batch.nrc_last_frame()
+    after = cache.params()
+    for k, (sc, cam) in enumerate(zip(scenes, cams)):
+        single = ap.RadianceCache(seed=1)
+        single.set_params(w[k])
+        hdr, sums = nr.render_nrc(sc, cam, prm, single, nprm)
+        st, _ = sc.nrc_last_frame(samples=False)
+        ident = float(np.all(got[k][0] == hdr, axis=-1).mean())
+        print(f"[batch-nrc]   scene {k} {cam.width}x{cam.height}: queries {stats[k].queries} vs {st.queries}, "
+              f"samples {stats[k].samples} vs {st.samples}, pixels bit-identical {ident:.4f}", flush=True)
+        assert (stats[k].queries, stats[k].train_paths, stats[k].samples) == (st.queries, st.train_paths, st.samples)
+        assert np.array_equal(got[k][0], hdr) and np.array_equal(got[k][1], sums)
+        assert np.array_equal(after[k], single.params())
+        single.close()
+    assert sum(s.queries for s in stats) > 0 and sum(s.samples for s in stats) > 0
+    if steps:
+        assert all(np.abs(after[k] - w[k]).max() > 0.0 for k in range(len(scenes)) if stats[k].samples > 0)
+    batch.close()
+    cache.close()
+
+
+def test_fp16_width32_caches_train_deterministically(gpu):
+    objs, cams = bg.scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    prm = ap.RenderParams.defaults(spp=4, max_depth=10, rr_min_depth=3, seed=22, wavelengths=4)
+    nprm = ap.NrcRenderParams.defaults(train_paths=512, batch=512, train_steps=2)
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    runs = []
+    for width, precision in ((32, "fp16"), (32, "fp16"), (64, "bf16")):
+        cache = ap.RadianceCache(seed=3, groups=len(scenes), width=width, precision=precision)
+        w0 = cache.params()
+        out = batch_render_nrc(batch, cams, prm, cache, nprm, stream)
+        runs.append((out, cache.params(), w0, [(s.queries, s.samples) for s in batch.nrc_last_frame()]))
+        cache.close()
+    (o1, w1, w0, s1), (o2, w2, _, s2), (_, _, _, s3) = runs
+    print(f"[batch-nrc] fp16 width 32: per-scene (queries, samples) {s1}; "
+          f"weight change {[float(np.abs(w1[k] - w0[k]).max()) for k in range(len(scenes))]}", flush=True)
+    assert s1 == s2 == s3
+    for k in range(len(scenes)):
+        assert np.array_equal(o1[k][0], o2[k][0]) and np.isfinite(o1[k][0]).all()
+        if s1[k][1] > 0:
+            assert np.abs(w1[k] - w0[k]).max() > 0.0
+    assert np.array_equal(w1, w2)
+    batch.close()
+
+
+def test_per_scene_fp16_caches_gain_over_the_plain_batch(gpu):
+    objs = [ap.synth_box_scene(1, 2, 15.0), ap.synth_box_scene(4, 1, 10.0)]
+    cams = [ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 96, 96), ap.Camera.make([0.2, 0.1, 3.0], [0, 0, -1], 50.0,
+                                                                                   64, 64)]
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    P = dict(max_depth=8, rr_min_depth=3, wavelengths=4)
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    refs = [x[0] for x in bg.batch_render(batch, cams, ap.RenderParams.defaults(spp=4096, seed=1000, **P),
+                                           stream)[0]]
+    plain = [x[0] for x in bg.batch_render(batch, cams, ap.RenderParams.defaults(spp=32, seed=5, **P), stream)[0]]
+    cache = ap.RadianceCache(seed=1, groups=2, width=32, precision="fp16")
+    batch_render_nrc(batch, cams, ap.RenderParams.defaults(spp=128, seed=77, **P), cache,
+                     ap.NrcRenderParams.defaults(train_paths=2048, batch=4096), stream)
+    nrc = [x[0] for x in batch_render_nrc(batch, cams, ap.RenderParams.defaults(spp=32, seed=5, **P), cache,
+                                          ap.NrcRenderParams.defaults(train_paths=0, train_steps=0), stream)]
+    for k in range(2):
+        e_plain, e_nrc = nr.srgb_rel_mse(plain[k], refs[k]), nr.srgb_rel_mse(nrc[k], refs[k])
+        print(f"[batch-nrc] scene {k}: relMSE plain {e_plain:.4g}, per-scene FP16 cache {e_nrc:.4g} "
+              f"({e_plain / e_nrc:.2f}x lower)", flush=True)
+        assert e_nrc < 0.5 * e_plain
+    batch.close()
+    cache.close()
