@@ -154,11 +154,12 @@ int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const 
 // network whose pool_n is 0.
 int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_begin, const uint32_t* pool_n,
                    uint32_t batch, uint32_t steps, float lr, uint64_t seed, hipStream_t st);
-// Inference of every network of a grouped cache: rows [seg[g], seg[g+1])
-// (device) go to network g, cut into 64-row tiles numbered from tile0[g];
+// Inference of every network of a grouped cache: region g = rows [seg[g],
+// seg[g+1]) (device), of which the first count[g] are live (all without
+// count), goes to network g, cut into 64-row tiles numbered from tile0[g];
 // row r reads record rows[r]. n_bound: an upper bound on the tiles x 64.
 int nrc_infer_groups(NrcState* s, const float* queries, const int32_t* rows, const uint32_t* seg,
-                     const uint32_t* tile0, uint32_t n_bound, float* out, hipStream_t st);
+                     const uint32_t* tile0, const uint32_t* count, uint32_t n_bound, float* out, hipStream_t st);
 // Bounds of network g (uploaded when they change; synchronous then).
 int nrc_set_group_bounds(NrcState* s, uint32_t g, const float lo[3], const float hi[3]);
 }  // namespace ap

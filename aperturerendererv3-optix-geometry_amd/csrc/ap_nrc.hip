@@ -774,12 +774,13 @@ int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_begin, c
 }
 
 int nrc_infer_groups(NrcState* s, const float* queries, const int32_t* rows, const uint32_t* seg,
-                     const uint32_t* tile0, uint32_t n_bound, float* out, hipStream_t st) {
+                     const uint32_t* tile0, const uint32_t* count, uint32_t n_bound, float* out, hipStream_t st) {
     if (n_bound == 0) return AP_OK;
     GroupMap gm;
     gm.groups = s->groups;
     gm.seg = seg;
     gm.tile0 = tile0;
+    gm.count = count;
     gm.bounds = s->dev_bounds;
     return infer(s, queries, gm, rows, nullptr, n_bound, out, st);
 }

@@ -109,6 +109,7 @@ struct NrcView {
     float* verts;        // per training slot, kNrcV records of kNrcVert floats
     uint32_t* counters;  // [0] cache queries, [1] training samples
     float spread;
+    uint32_t* qcount;    // a batch: per-scene cache queries, scene s's at qpath[pix_begin[s] ...]
 };
 
 // One scene of a batch (ap_batch_render): its traversal descriptor, shading
@@ -208,6 +209,26 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     base = __shfl(base, leader);
     const uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64u - lane)));
     return base + static_cast<uint32_t>(__popcll(below));
+}
+
+// wave_append onto counters[key]: one atomic per distinct key of the wave.
+__device__ __forceinline__ uint32_t wave_append_keyed(uint32_t* counters, uint32_t key, bool pred) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
+    uint64_t pending = __ballot(pred);
+    uint32_t slot = 0;
+    while (pending != 0ull) {
+        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+        const uint32_t k = static_cast<uint32_t>(__shfl(static_cast<int>(key), leader));
+        const bool mine = pred && key == k;
+        const uint64_t m = __ballot(mine);
+        uint32_t base = 0;
+        if (static_cast<int>(lane) == leader) base = atomicAdd(&counters[k], static_cast<uint32_t>(__popcll(m)));
+        base = static_cast<uint32_t>(__shfl(static_cast<int>(base), leader));
+        if (mine) slot = base + static_cast<uint32_t>(__popcll(m & below));
+        pending &= ~m;
+    }
+    return slot;
 }
 
 template <int NL>
@@ -903,8 +924,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         if constexpr (MULTI) bv.shadow_scene[si] = scene_id;
     }
     if (NRC) {
-        const uint32_t qi = wave_append(&nv.counters[0], emit_query);
-        if (emit_query) nv.qpath[qi] = pid;
+        if constexpr (MULTI) {
+            const uint32_t qi = wave_append_keyed(nv.qcount, scene_id, emit_query);
+            if (emit_query) nv.qpath[bv.pix_begin[scene_id] + qi] = pid;
+        } else {
+            const uint32_t qi = wave_append(&nv.counters[0], emit_query);
+            if (emit_query) nv.qpath[qi] = pid;
+        }
     }
 }
 
@@ -928,8 +954,22 @@ __global__ __launch_bounds__(kBlock) void nrc_init_kernel(int32_t n_pix, uint32_
     paths[i] = P;
 }
 
-// Radiance of the cache vertices: throughput x the prediction (clamped at 0)
-// at the path's wavelength.
+// Radiance of a cache vertex: throughput x the prediction c (clamped at 0)
+// at the path's wavelengths.
+template <int NL>
+__device__ __forceinline__ void nrc_apply_one(PathT<NL>& S, f3 c, const f4* __restrict__ rgb_lut,
+                                              const f4* __restrict__ lam_lut, const float* __restrict__ tables) {
+    c = fmax3(c, f3{0.0f, 0.0f, 0.0f});
+    // interact() with the RGB basis looked up once for all wavelengths.
+    const float lum = max3(c);
+    const f4 cb = query_rgb_basis(rgb_lut, lum > 1.0f ? c / lum : c);
+    for (int j = 0; j < NL; ++j) {
+        const f4 lb = query_lambda(lam_lut, S.lam[j]);
+        const float le = lum > 1.0f ? spectral_interact(cb, lb) * lum : spectral_interact(cb, lb);
+        S.radiance[j] = S.radiance[j] + S.mask[j] * le * d65_norm(tables, S.lam[j]);
+    }
+}
+
 template <int NL>
 __global__ __launch_bounds__(kBlock) void nrc_apply_kernel(const uint32_t* __restrict__ counters,
                                                            const int32_t* __restrict__ qpath,
@@ -939,16 +979,7 @@ __global__ __launch_bounds__(kBlock) void nrc_apply_kernel(const uint32_t* __res
                                                            PathT<NL>* __restrict__ paths) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= counters[0]) return;
-    PathT<NL>& S = paths[qpath[q]];
-    const f3 c = fmax3(f3{rgb[3 * q], rgb[3 * q + 1], rgb[3 * q + 2]}, f3{0.0f, 0.0f, 0.0f});
-    // interact() with the RGB basis looked up once for all wavelengths.
-    const float lum = max3(c);
-    const f4 cb = query_rgb_basis(rgb_lut, lum > 1.0f ? c / lum : c);
-    for (int j = 0; j < NL; ++j) {
-        const f4 lb = query_lambda(lam_lut, S.lam[j]);
-        const float le = lum > 1.0f ? spectral_interact(cb, lb) * lum : spectral_interact(cb, lb);
-        S.radiance[j] = S.radiance[j] + S.mask[j] * le * d65_norm(tables, S.lam[j]);
-    }
+    nrc_apply_one<NL>(paths[qpath[q]], f3{rgb[3 * q], rgb[3 * q + 1], rgb[3 * q + 2]}, rgb_lut, lam_lut, tables);
 }
 
 __device__ __forceinline__ bool finite3(f3 v) {
@@ -1232,16 +1263,15 @@ __device__ __forceinline__ void nrc_scene_train(const NrcBatchScene& T, uint32_t
 }
 
 // nrc_init_kernel over every scene of a batch; also clears the per-scene
-// query counters.
+// query counts.
 __global__ __launch_bounds__(kBlock) void nrc_init_batch_kernel(BatchView bv, uint32_t total,
                                                                 const NrcBatchScene* __restrict__ T, uint64_t h,
                                                                 uint32_t unbiased_every, NrcPath* __restrict__ paths,
                                                                 uint32_t* __restrict__ counters,
-                                                                uint32_t* __restrict__ qcount,
-                                                                uint32_t* __restrict__ qcursor) {
+                                                                uint32_t* __restrict__ qcount) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i == 0) { counters[0] = 0; counters[1] = 0; }
-    if (i < static_cast<uint32_t>(bv.n_scenes)) { qcount[i] = 0; qcursor[i] = 0; }
+    if (i < static_cast<uint32_t>(bv.n_scenes)) qcount[i] = 0;
     if (i >= total) return;
     const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
     const uint32_t p = i - bv.pix_begin[sc], n_pix = bv.pix_begin[sc + 1] - bv.pix_begin[sc];
@@ -1259,67 +1289,22 @@ __global__ __launch_bounds__(kBlock) void nrc_init_batch_kernel(BatchView bv, ui
     paths[i] = P;
 }
 
-// Cache queries per scene (the scene of the querying path).
+// nrc_apply_kernel over a batch: scene s's queries are rows pix_begin[s] ..
+// pix_begin[s] + qcount[s] of qpath and rgb.
 template <int NL>
-__global__ __launch_bounds__(kBlock) void nrc_query_count_kernel(const uint32_t* __restrict__ counters,
+__global__ __launch_bounds__(kBlock) void nrc_apply_batch_kernel(BatchView bv, uint32_t total,
+                                                                 const uint32_t* __restrict__ qcount,
                                                                  const int32_t* __restrict__ qpath,
-                                                                 const PathT<NL>* __restrict__ paths,
-                                                                 uint32_t* __restrict__ qcount) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= counters[0]) return;
-    atomicAdd(&qcount[paths[qpath[q]].flags >> 16], 1u);
-}
-
-// Row segments of the scenes' queries: seg[s] = first row of scene s,
-// tile0[s] = its first 64-row tile (exclusive prefix sums, one block).
-__global__ __launch_bounds__(1024) void nrc_query_segments_kernel(uint32_t n, const uint32_t* __restrict__ qcount,
-                                                                  uint32_t* __restrict__ seg,
-                                                                  uint32_t* __restrict__ tile0) {
-    __shared__ uint32_t rows[1024], tiles[1024];
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t lo = threadIdx.x * per, hi = min(lo + per, n);
-    uint32_t sr = 0, st = 0;
-    for (uint32_t s = lo; s < hi; ++s) {
-        sr += qcount[s];
-        st += (qcount[s] + 63u) / 64u;
-    }
-    rows[threadIdx.x] = sr;
-    tiles[threadIdx.x] = st;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {
-        const uint32_t ar = threadIdx.x >= d ? rows[threadIdx.x - d] : 0u;
-        const uint32_t at = threadIdx.x >= d ? tiles[threadIdx.x - d] : 0u;
-        __syncthreads();
-        rows[threadIdx.x] += ar;
-        tiles[threadIdx.x] += at;
-        __syncthreads();
-    }
-    uint32_t r = rows[threadIdx.x] - sr, t = tiles[threadIdx.x] - st;
-    for (uint32_t s = lo; s < hi; ++s) {
-        seg[s] = r;
-        tile0[s] = t;
-        r += qcount[s];
-        t += (qcount[s] + 63u) / 64u;
-    }
-    if (threadIdx.x == 1023u) {
-        seg[n] = rows[1023];
-        tile0[n] = tiles[1023];
-    }
-}
-
-// Queries grouped by scene: scene s's query paths at rows [seg[s], seg[s+1]).
-template <int NL>
-__global__ __launch_bounds__(kBlock) void nrc_query_scatter_kernel(const uint32_t* __restrict__ counters,
-                                                                   const int32_t* __restrict__ qpath,
-                                                                   const PathT<NL>* __restrict__ paths,
-                                                                   const uint32_t* __restrict__ seg,
-                                                                   uint32_t* __restrict__ qcursor,
-                                                                   int32_t* __restrict__ sorted) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= counters[0]) return;
-    const int32_t pid = qpath[q];
-    const uint32_t sc = paths[pid].flags >> 16;
-    sorted[seg[sc] + atomicAdd(&qcursor[sc], 1u)] = pid;
+                                                                 const float* __restrict__ rgb,
+                                                                 const f4* __restrict__ rgb_lut,
+                                                                 const f4* __restrict__ lam_lut,
+                                                                 const float* __restrict__ tables,
+                                                                 PathT<NL>* __restrict__ paths) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
+    if (i - bv.pix_begin[sc] >= qcount[sc]) return;
+    nrc_apply_one<NL>(paths[qpath[i]], f3{rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]}, rgb_lut, lam_lut, tables);
 }
 
 // nrc_targets_kernel over the training slots of every scene of a batch;
@@ -1903,14 +1888,12 @@ struct BatchState {
     NrcBatchScene* nrc_table = nullptr;
     NrcPath* nrc_paths = nullptr;
     float* nrc_queries = nullptr;
-    int32_t* nrc_qpath = nullptr;
-    int32_t* nrc_qsorted = nullptr;
+    int32_t* nrc_qpath = nullptr;           // scene s's query paths from pix_begin[s]
     float* nrc_qrgb = nullptr;
-    uint32_t* nrc_counters = nullptr;       // [0] queries, [1] training samples
-    uint32_t* nrc_qcount = nullptr;         // per scene: queries, then their segments
-    uint32_t* nrc_qcursor = nullptr;
-    uint32_t* nrc_seg = nullptr;
-    uint32_t* nrc_tile0 = nullptr;
+    uint32_t* nrc_counters = nullptr;       // [1] training samples
+    uint32_t* nrc_qcount = nullptr;         // per scene: cache queries
+    uint32_t* nrc_tile0 = nullptr;          // per scene: first 64-row tile of its query region
+    uint32_t nrc_tiles = 0;
     uint32_t* nrc_pool_begin = nullptr;
     uint32_t* nrc_pool_n = nullptr;
     size_t nrc_slot_cap = 0;
@@ -1933,8 +1916,8 @@ struct BatchState {
         drop_graph();
         void* ptrs[] = {d_scenes, d_devs, d_pix_begin, paths, rays[0], rays[1], path_of[0], path_of[1], hits,
                         shadow_rays, shadow_payload, shadow_hits, ray_scene[0], ray_scene[1], shadow_scene, counters,
-                        frame, rays_traced, eps, nrc_table, nrc_paths, nrc_queries, nrc_qpath, nrc_qsorted, nrc_qrgb,
-                        nrc_counters, nrc_qcount, nrc_qcursor, nrc_seg, nrc_tile0, nrc_pool_begin, nrc_pool_n,
+                        frame, rays_traced, eps, nrc_table, nrc_paths, nrc_queries, nrc_qpath, nrc_qrgb,
+                        nrc_counters, nrc_qcount, nrc_tile0, nrc_pool_begin, nrc_pool_n,
                         nrc_verts, nrc_vmask, nrc_offsets, nrc_slot_pix, nrc_pool};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -2081,14 +2064,16 @@ int batch_nrc_prepare(BatchState* b, const ap_nrc_render_params* np, NrcState* n
         if ((rc = dev_alloc(&b->nrc_paths, cap)) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_queries, cap * kNrcRec)) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_qpath, cap)) != AP_OK) return rc;
-        if ((rc = dev_alloc(&b->nrc_qsorted, cap)) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_qrgb, cap * 3)) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_counters, 4)) != AP_OK) return rc;
-        for (uint32_t** p : {&b->nrc_qcount, &b->nrc_qcursor, &b->nrc_pool_begin, &b->nrc_pool_n})
+        for (uint32_t** p : {&b->nrc_qcount, &b->nrc_pool_begin, &b->nrc_pool_n})
             if ((rc = dev_alloc(p, n)) != AP_OK) return rc;
-        if ((rc = dev_alloc(&b->nrc_seg, n + 1)) != AP_OK) return rc;
-        if ((rc = dev_alloc(&b->nrc_tile0, n + 1)) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_table, n)) != AP_OK) return rc;
+        // Query regions: scene k's rows pix_begin[k] .. pix_begin[k + 1], cut into 64-row tiles.
+        std::vector<uint32_t> tile0(n + 1, 0);
+        for (size_t k = 0; k < n; ++k) tile0[k + 1] = tile0[k] + (b->pix_begin[k + 1] - b->pix_begin[k] + 63u) / 64u;
+        if ((rc = upload_vec(tile0, &b->nrc_tile0)) != AP_OK) return rc;
+        b->nrc_tiles = tile0[n];
     }
     if (b->nrc_table_paths != np->train_paths) {
         // Per scene, ap_render_nrc's stride and most training paths; slots scene after scene.
@@ -2160,8 +2145,9 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
     hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
                        b->eps, paths, b->rays[0], b->path_of[0]);
     hipLaunchKernelGGL(nrc_init_batch_kernel, grid_of(total), dim3(kBlock), 0, stream, bv, total, b->nrc_table, h,
-                       np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount, b->nrc_qcursor);
-    const NrcView nv{b->nrc_paths, b->nrc_queries, b->nrc_qpath, b->nrc_verts, b->nrc_counters, np->spread};
+                       np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount);
+    const NrcView nv{b->nrc_paths, b->nrc_queries, b->nrc_qpath, b->nrc_verts, b->nrc_counters, np->spread,
+                     b->nrc_qcount};
     int in_q = 0;
     for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
         const int out_q = 1 - in_q;
@@ -2182,18 +2168,12 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
                            b->shadow_hits, paths, b->rays_traced);
         in_q = out_q;
     }
-    // Cache vertices: queries grouped by scene, every network in one inference.
-    hipLaunchKernelGGL(nrc_query_count_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->nrc_counters,
-                       b->nrc_qpath, paths, b->nrc_qcount);
-    hipLaunchKernelGGL(nrc_query_segments_kernel, dim3(1), dim3(1024), 0, stream, static_cast<uint32_t>(n),
-                       b->nrc_qcount, b->nrc_seg, b->nrc_tile0);
-    hipLaunchKernelGGL(nrc_query_scatter_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->nrc_counters,
-                       b->nrc_qpath, paths, b->nrc_seg, b->nrc_qcursor, b->nrc_qsorted);
-    int rc = nrc_infer_groups(ns, b->nrc_queries, b->nrc_qsorted, b->nrc_seg, b->nrc_tile0,
-                              total + 64u * static_cast<uint32_t>(n), b->nrc_qrgb, stream);
+    // Cache vertices: every scene's queries to its own network, one inference for all.
+    int rc = nrc_infer_groups(ns, b->nrc_queries, b->nrc_qpath, b->d_pix_begin, b->nrc_tile0, b->nrc_qcount,
+                              b->nrc_tiles * 64u, b->nrc_qrgb, stream);
     if (rc != AP_OK) return rc;
-    hipLaunchKernelGGL(nrc_apply_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->nrc_counters,
-                       b->nrc_qsorted, b->nrc_qrgb, rgb_lut, lam_lut, tables, paths);
+    hipLaunchKernelGGL(nrc_apply_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total,
+                       b->nrc_qcount, b->nrc_qpath, b->nrc_qrgb, rgb_lut, lam_lut, tables, paths);
     const uint32_t slots = b->nrc_slots;
     if (slots > 0) {
         hipLaunchKernelGGL(nrc_targets_batch_kernel<NL>, grid_of(slots), dim3(kBlock), 0, stream, slots, bv,

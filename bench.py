@@ -387,6 +387,140 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
     return out
 
 
+def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_frames=8, boost_steps=16):
+    """Equal-time relMSE of batched rendering with per-scene learned radiance
+    caches (one width-32 FP16 network per scene in a grouped cache: the
+    scenes' cache queries answered by one grouped inference per frame, every
+    network trained on its own scene's samples in the same launches) against
+    the plain batch, both with four hero wavelengths per path. n_scenes
+    scenes, alternately a heightfield + icosphere scene (triangles
+    log-uniform in [1e4, 2e5], open sky side, no sky light) and a closed box
+    (coloured walls, one ceiling light), resolutions uniform in {256^2,
+    512^2}. The plain batch renders plain_spp frames (HIP events); the cache
+    batch starts from fresh networks and renders as many frames as its
+    calibrated time per frame fits in the same GPU time, its first
+    boost_frames frames training with boost_steps Adam steps (their cost
+    calibrated and charged). relMSE per scene against a plain batch of
+    ref_spp frames on another seed; reported as the mean over scenes."""
+    g = np.random.default_rng(seed)
+    objs, cams = [], []
+    for k in range(n_scenes):
+        res = int(g.choice([256, 512]))
+        if k % 2 == 0:
+            tris = float(np.exp(g.uniform(np.log(1e4), np.log(2e5))))
+            spheres = max(1, int(tris * 0.04 / 320))
+            grid = max(8, int(round(np.sqrt(max(tris - 320 * spheres, 128) / 2))))
+            objs.append(ap.synth_heightfield_scene(grid, spheres, 300 + k, 20.0))
+            cams.append(ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, res, res))
+        else:
+            objs.append(ap.synth_box_scene(1 + k, 2, 15.0))
+            cams.append(ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, res, res))
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    pix = sum(c.width * c.height for c in cams)
+    stream = ap.Stream()
+    params = dict(max_depth=depth, rr_min_depth=4, sky=0, wavelengths=4)
+    batch = ap.Batch(scenes, cams)
+    batch.autotune(ap.RenderParams.defaults(spp=1, **params), frames=2, stream=stream)
+
+    def buffers():
+        out = []
+        for c in cams:
+            h = ap.DeviceBuffer(c.width * c.height * 12)
+            h.zero(stream)
+            out.append(h)
+        return out
+
+    def images(bufs):
+        return [b.to_numpy(np.float32, c.width * c.height * 3) for b, c in zip(bufs, cams)]
+
+    t0 = time.perf_counter()
+    ref = buffers()
+    batch.render(ap.RenderParams.defaults(spp=ref_spp, seed=1000, **params), ref, stream=stream)
+    stream.synchronize()
+    ref_imgs = images(ref)
+    log(f"batch nrc: {n_scenes} scenes, {pix} pixels per frame, reference {ref_spp} spp in "
+        f"{time.perf_counter() - t0:.1f} s")
+    e0, e1 = ap.Event(), ap.Event()
+    img = buffers()
+    batch.render(ap.RenderParams.defaults(spp=1, seed=4, **params), img, stream=stream)  # untimed graph capture
+    img = buffers()
+    stream.synchronize()
+    e0.record(stream)
+    batch.render(ap.RenderParams.defaults(spp=plain_spp, seed=5, **params), img, stream=stream)
+    e1.record(stream)
+    stream.synchronize()
+    budget = e0.elapsed_ms(e1)
+    plain = [rel_mse(a, b) for a, b in zip(images(img), ref_imgs)]
+    cache = ap.RadianceCache(seed=1, groups=n_scenes, width=32, precision="fp16")
+    w0 = cache.params()
+    nprm = ap.NrcRenderParams.defaults(train_paths=8192, batch=4096)
+    boost = ap.NrcRenderParams.from_buffer_copy(nprm)
+    boost.train_steps = boost_steps
+
+    def nrc_frames(frames, bufs, seed, first=0, p=nprm):
+        if frames > 0:
+            batch.render_nrc(ap.RenderParams.defaults(spp=frames, frame_index=first, seed=seed, **params), cache, p,
+                             bufs, stream=stream)
+
+    scratch = buffers()
+    nrc_frames(2, scratch, 3)
+    stream.synchronize()
+    e0.record(stream)
+    nrc_frames(6, scratch, 4)
+    e1.record(stream)
+    stream.synchronize()
+    per_frame = e0.elapsed_ms(e1) / 6
+    e0.record(stream)
+    nrc_frames(3, scratch, 4, 0, boost)
+    e1.record(stream)
+    stream.synchronize()
+    per_boost = e0.elapsed_ms(e1) / 3
+    total = max(1, boost_frames + int(max(budget - boost_frames * per_boost, 0.0) / per_frame))
+    cache.set_params(w0)
+    img = buffers()
+    stream.synchronize()
+    e0.record(stream)
+    nb = min(boost_frames, total)
+    nrc_frames(nb, img, 5, 0, boost)
+    nrc_frames(total - nb, img, 5, nb)
+    e1.record(stream)
+    stream.synchronize()
+    used = e0.elapsed_ms(e1)
+    st = batch.nrc_last_frame()
+    nrc = [rel_mse(a, b) for a, b in zip(images(img), ref_imgs)]
+    cache.close()
+    batch.close()
+    for sc in scenes:
+        sc.close()
+    kinds = ["heightfield" if k % 2 == 0 else "box" for k in range(n_scenes)]
+    mean = lambda v, kind=None: float(np.mean([x for x, t in zip(v, kinds) if kind in (None, t)]))
+    out = {
+        "workload": f"{n_scenes} scenes (heightfield+icosphere and closed boxes, alternating), resolutions uniform in "
+                    f"{{256^2, 512^2}}, max depth {depth}, RR from depth 4, four hero wavelengths, no sky; one "
+                    "width-32 FP16 radiance-cache network per scene",
+        "pixels_per_frame": int(pix),
+        "equal_time_ms": round(budget, 1),
+        "plain_spp": plain_spp,
+        "plain_ms_per_frame": round(budget / plain_spp, 3),
+        "nrc_frames": total,
+        "nrc_ms": round(used, 1),
+        "nrc_ms_per_frame_calibrated": round(per_frame, 3),
+        "nrc_ms_per_boosted_frame_calibrated": round(per_boost, 3),
+        "nrc_boost": {"frames": boost_frames, "steps": boost_steps},
+        "queries_last_frame": int(sum(x.queries for x in st)),
+        "samples_last_frame": int(sum(x.samples for x in st)),
+        "relmse_plain_mean": round(mean(plain), 5),
+        "relmse_nrc_mean": round(mean(nrc), 5),
+        "relmse_ratio_plain_over_nrc": round(mean(plain) / mean(nrc), 3),
+        "ratio_heightfield_scenes": round(mean(plain, "heightfield") / mean(nrc, "heightfield"), 3),
+        "ratio_box_scenes": round(mean(plain, "box") / mean(nrc, "box"), 3),
+        "per_scene_ratio_min_max": [round(min(a / b for a, b in zip(plain, nrc)), 3),
+                                    round(max(a / b for a, b in zip(plain, nrc)), 3)],
+    }
+    log(f"batch nrc: {out}")
+    return out
+
+
 def main():
     ap_ = argparse.ArgumentParser()
     ap_.add_argument("--gpus", type=int, default=1)
@@ -403,6 +537,11 @@ def main():
     ap_.add_argument("--nrc-warmup", type=int, default=0, help="cache frames rendered before the averaged ones")
     ap_.add_argument("--nrc-passes", type=int, default=0, help="training-only passes of the cache after each frame")
     ap_.add_argument("--batch-spp", type=int, default=16, help="spp of the 48-scene batch measurement (0 skips it)")
+    ap_.add_argument("--batch-nrc-scenes", type=int, default=16,
+                     help="scenes of the per-scene radiance-cache batch comparison (0 skips it)")
+    ap_.add_argument("--batch-nrc-spp", type=int, default=64,
+                     help="plain spp that sets the equal-time budget of the per-scene cache batch comparison")
+    ap_.add_argument("--batch-nrc-ref-spp", type=int, default=1024, help="spp of its plain reference images")
     ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap_.add_argument("--sweep", action="store_true",
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
@@ -581,6 +720,8 @@ def main():
                                       nrc_warmup=args.nrc_warmup, nrc_passes=args.nrc_passes)
     if args.batch_spp > 0:
         line["batch"] = batch_bench(ap, args.batch_spp, 12)
+    if args.batch_nrc_scenes > 0:
+        line["batch_nrc"] = batch_nrc_bench(ap, args.batch_nrc_scenes, args.batch_nrc_spp, args.batch_nrc_ref_spp)
     print(json.dumps(line), flush=True)
 
 
