@@ -141,7 +141,21 @@ class NrcFrameStats(ctypes.Structure):
 
 class Framebuffers(ctypes.Structure):
     _fields_ = [("hdr", ctypes.c_void_p), ("odd", ctypes.c_void_p), ("even", ctypes.c_void_p),
-                ("sum_xyz", ctypes.c_void_p), ("primary_id", ctypes.c_void_p), ("ray_count", ctypes.c_void_p)]
+                ("sum_xyz", ctypes.c_void_p), ("primary_id", ctypes.c_void_p), ("ray_count", ctypes.c_void_p),
+                ("gbuffer", ctypes.c_void_p), ("statistics", ctypes.c_void_p)]
+
+
+# denoise::ScreenGBuffer / ScreenStatisticsBuffer (include/aperture/ap_layout.h).
+SCREEN_GBUFFER_DTYPE = np.dtype({"names": ["albedo", "normal", "metallic", "roughness", "depth", "valid_frame_index",
+                                           "frame_index", "shading_model"],
+                                 "formats": [("<f2", 3), ("<f2", 3), "<f2", "<f2", "<f4", "<u8", "<u8", ("<f2", 5)],
+                                 "offsets": [0, 6, 12, 14, 16, 24, 32, 48], "itemsize": 64})
+SCREEN_STATISTICS_DTYPE = np.dtype({"names": ["num", "mean", "m2_div_num", "m3_div_num"],
+                                    "formats": ["<u4", ("<f4", 3), ("<f4", 3), ("<f4", 3)],
+                                    "offsets": [0, 4, 16, 28], "itemsize": 48})
+TONE_LINEAR, TONE_GAMMA, TONE_ACES = 0, 1, 2
+(DISPLAY_IMAGE, DISPLAY_ALBEDO, DISPLAY_NORMAL, DISPLAY_DEPTH, DISPLAY_METALLIC, DISPLAY_ROUGHNESS,
+ DISPLAY_STATISTICS_NUM, DISPLAY_STATISTICS_MEAN, DISPLAY_STATISTICS_M2, DISPLAY_STATISTICS_M3) = range(10)
 
 
 TRIANGLE_MATERIAL_OBJECT, TRIANGLE_MATERIAL_REFERENCE = 0, 1
@@ -176,7 +190,7 @@ EXPORTED_SYMBOLS = (
     "ap_nrc_set_group_bounds", "ap_nrc_infer_grouped", "ap_nrc_gradient_grouped", "ap_nrc_train_step_grouped",
     "ap_batch_create", "ap_batch_destroy", "ap_batch_render", "ap_batch_set_launch", "ap_batch_get_launch",
     "ap_batch_set_graph", "ap_batch_autotune", "ap_render_nrc", "ap_nrc_last_frame", "ap_batch_render_nrc",
-    "ap_batch_nrc_last_frame",
+    "ap_batch_nrc_last_frame", "ap_tonemap",
 )
 
 
@@ -225,6 +239,7 @@ def lib():
         L.ap_nrc_last_frame.argtypes = [vp, vp, vp, sz]
         L.ap_batch_render_nrc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.ap_batch_nrc_last_frame.argtypes = [vp, vp, sz]
+        L.ap_tonemap.argtypes = [vp, vp, vp, vp, i32, i32, i32, i32, ctypes.c_float, vp]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
@@ -501,6 +516,15 @@ def xyz_to_srgb(xyz):
     check(lib().ap_xyz_to_srgb(d_in.ptr, n, d_out.ptr, None))
     synchronize()
     return d_out.to_numpy(np.float32, a.size).reshape(a.shape)
+
+
+def tonemap(hdr, width, height, tone=TONE_LINEAR, display=DISPLAY_IMAGE, exposure=1.0, gbuffer=None,
+            statistics=None, ldr=None, stream=None):
+    """ap_tonemap over device buffers; returns the device ldr buffer (uint32 ARGB)."""
+    out = ldr if ldr is not None else DeviceBuffer(width * height * 4)
+    check(lib().ap_tonemap(_ptr(hdr), _ptr(gbuffer), _ptr(statistics), _ptr(out), width, height, tone, display,
+                           exposure, _stream(stream)))
+    return out
 
 
 NRC_PARAMS = 21504
@@ -800,16 +824,18 @@ class SceneRenderer:
         return out[: n_out.value]
 
     def render(self, camera, params, hdr, odd=None, even=None, sum_xyz=None, primary_id=None, stream=None,
-               ray_count=None):
+               ray_count=None, gbuffer=None, statistics=None):
         """SceneRenderer::Render into caller-owned device framebuffers (DeviceBuffer or addresses)."""
-        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id), _ptr(ray_count))
+        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id), _ptr(ray_count),
+                          _ptr(gbuffer), _ptr(statistics))
         check(lib().ap_render(self.handle, ctypes.byref(camera), ctypes.byref(params), ctypes.byref(fb),
                               _stream(stream)))
 
     def render_nrc(self, camera, params, cache, nrc_params, hdr, odd=None, even=None, sum_xyz=None,
-                   primary_id=None, stream=None, ray_count=None):
+                   primary_id=None, stream=None, ray_count=None, gbuffer=None, statistics=None):
         """render() with the learned radiance cache `cache` (a RadianceCache)."""
-        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id), _ptr(ray_count))
+        fb = Framebuffers(_ptr(hdr), _ptr(odd), _ptr(even), _ptr(sum_xyz), _ptr(primary_id), _ptr(ray_count),
+                          _ptr(gbuffer), _ptr(statistics))
         check(lib().ap_render_nrc(self.handle, ctypes.byref(camera), ctypes.byref(params), cache.handle,
                                   ctypes.byref(nrc_params), ctypes.byref(fb), _stream(stream)))
 

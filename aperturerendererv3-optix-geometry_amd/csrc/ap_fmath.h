@@ -233,6 +233,15 @@ APFM_HD float atan2(float y, float x) {
     return w + atan(y / x);
 }
 
+// x^y for x >= 0 as exp(y log x) (a few ulp; 0^y = 0 for y > 0, NaN for x < 0).
+APFM_HD float pow(float x, float y) {
+    if (x != x || y != y) return quiet_nan();
+    if (x < 0.0f) return quiet_nan();
+    if (x == 1.0f || y == 0.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : inf();
+    return exp(y * log(x));
+}
+
 // x^0.75 for x >= 0.
 APFM_HD float pow075(float x) {
     const float s = sqrtf(x);

@@ -34,6 +34,7 @@
 #include "ap_shade.h"
 #include "ap_spectrum.h"
 #include "ap_guide.h"
+#include "ap_display.h"
 
 namespace ap {
 
@@ -177,6 +178,9 @@ struct RenderState {
     uint32_t* nrc_counters = nullptr;
     uint32_t nrc_last_train = 0;      // training paths of the last frame
     bool nrc_rendered = false;
+    // screen G-buffer samples of the paths in flight (framebuffers with a G-buffer)
+    GSample* gsamples = nullptr;
+    size_t gsample_cap = 0;
 };
 
 constexpr int kEpsEntries = 1024;
@@ -186,7 +190,7 @@ void render_state_free(RenderState* r) {
     void* ptrs[] = {r->geo, r->mats, r->lights, r->paths, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
                     r->hits, r->shadow_rays, r->shadow_payload, r->shadow_hits, r->counters, r->eps,
                     r->target_tmp, r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_qrgb, r->nrc_verts,
-                    r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters};
+                    r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters, r->gsamples};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete r;
@@ -441,6 +445,7 @@ struct ShadeParams {
     int32_t in_q;
     int32_t out_q;
     int32_t n_lights;
+    GSample* gsample = nullptr;   // depth 0: each path's first-hit G-buffer sample, or none
 };
 
 // Hit attributes as the reference intersectors report them.
@@ -641,6 +646,11 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
 
     if (H.id < 0) {
         // Miss: SkyLighting and the offset draw of Trace, then the path ends.
+        if (sp.depth == 0 && sp.gsample) {
+            GSample g{};
+            g.model = kShadingModels - 1;
+            sp.gsample[pid] = g;
+        }
         (void)rand1(rng);
         float le_end[NL];
         for (int j = 0; j < NL; ++j) le_end[j] = 0.0f;
@@ -659,6 +669,19 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         const MatRec M = mats[hi.mat];
         const f3 N = hi.normal;
         f3 hp = hi.pos;
+        if (sp.depth == 0 && sp.gsample) {
+            // What RecordScreenGBuffer reads at the first hit (render.cuh:640-643).
+            GSample g;
+            g.albedo[0] = M.albedo_rgb[0]; g.albedo[1] = M.albedo_rgb[1]; g.albedo[2] = M.albedo_rgb[2];
+            g.metallic = M.metallic;
+            g.normal[0] = N.x; g.normal[1] = N.y; g.normal[2] = N.z;
+            g.roughness = M.roughness;
+            g.depth = H.t;
+            g.model = M.shading_model;
+            g.hit = 1;
+            g.pad = 0;
+            sp.gsample[pid] = g;
+        }
         // SampleBSDF (material.cuh:177-330) for DIFFUSE / LIGHT / GGX.
         f3 next_dir{0.0f, 0.0f, 0.0f};
         float pdf = 1.0f;
@@ -1109,9 +1132,16 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
                                                             float* __restrict__ hdr, float* __restrict__ odd,
                                                             float* __restrict__ even, double* __restrict__ sums,
                                                             const GuideRec* __restrict__ recs,
-                                                            ap_path_guiding_sample* __restrict__ slots) {
+                                                            ap_path_guiding_sample* __restrict__ slots,
+                                                            const GSample* __restrict__ gsamples,
+                                                            ap_screen_gbuffer* __restrict__ gbuffer,
+                                                            ap_screen_statistics* __restrict__ stats) {
     const int32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= fp.n_pixels) return;
+    ap_screen_gbuffer gb;
+    ap_screen_statistics sb;
+    if (gbuffer) gb = gbuffer[p];
+    if (stats) sb = stats[p];
     f3 acc{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
     f3 ao{0, 0, 0}, ae{0, 0, 0};
     if (odd) ao = f3{odd[3 * p], odd[3 * p + 1], odd[3 * p + 2]};
@@ -1136,7 +1166,15 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
         s0 += static_cast<double>(res.x);
         s1 += static_cast<double>(res.y);
         s2 += static_cast<double>(res.z);
+        // Frame order per pixel, as the reference's frames would record them.
+        if (gbuffer) record_gbuffer(gb, gsamples[static_cast<size_t>(f) * fp.n_pixels + p]);
+        if (stats) {
+            const float x[3] = {res.x, res.y, res.z};
+            record_statistics(sb, x);
+        }
     }
+    if (gbuffer) gbuffer[p] = gb;
+    if (stats) stats[p] = sb;
     hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
     if (odd && even) {
         odd[3 * p] = ao.x; odd[3 * p + 1] = ao.y; odd[3 * p + 2] = ao.z;
@@ -1665,6 +1703,7 @@ int run_frames(const RunCtx& c) {
             if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
             ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                            prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
+            if (fb->gbuffer && !train_only) sp.gsample = r->gsamples;
             if (guiding) {
                 hipLaunchKernelGGL((shade_kernel<NL, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
                                    r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
@@ -1719,7 +1758,8 @@ int run_frames(const RunCtx& c) {
         if (!train_only)
             hipLaunchKernelGGL(accumulate_kernel<NL>, grid_of(fp.n_pixels), dim3(kBlock), 0, stream, fp, paths, tables,
                                fb->hdr, fb->odd, fb->even, fb->sum_xyz, gv.recs,
-                               guiding ? guide_slots(s->guide) : nullptr);
+                               guiding ? guide_slots(s->guide) : nullptr, r->gsamples, fb->gbuffer,
+                               fb->statistics);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
         if (guiding && !train_only && (rc2 = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK)
@@ -1805,6 +1845,13 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
         ? 1 : std::max(1, std::min(prm->spp, static_cast<int32_t>((4u << 20) / static_cast<uint32_t>(n_pix))));
     if ((rc = ensure_paths(r, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
     if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(n_pix))) != AP_OK) return rc;
+    if (fb->gbuffer && r->gsample_cap < static_cast<size_t>(per_batch) * n_pix) {
+        if (r->gsamples) (void)hipFree(r->gsamples);
+        r->gsamples = nullptr;
+        r->gsample_cap = 0;
+        if ((rc = dev_alloc(&r->gsamples, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
+        r->gsample_cap = static_cast<size_t>(per_batch) * n_pix;
+    }
     NrcState* ns = nrc ? nrc_state(cache) : nullptr;
     uint32_t n_stride = 1, n_train_max = 0;
     if (nrc) {

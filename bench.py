@@ -100,7 +100,7 @@ def rel_mse(xyz, ref_xyz):
 
 
 def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_passes, boost_frames=0,
-              boost_steps=16):
+              boost_steps=16, spread=0.01, width=64, precision="bf16"):
     """Equal-time relMSE of the radiance-cache renderer against the plain
     wavefront tracer, both with four hero wavelengths per path. The plain
     tracer renders plain_spp frames (timed with HIP events); the cache
@@ -130,8 +130,8 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     stream.synchronize()
     budget = e0.elapsed_ms(e1)
     plain = rel_mse(img.to_numpy(np.float32, n * 3), ref_img)
-    cache = ap.RadianceCache(seed=1)
-    nprm = ap.NrcRenderParams.defaults(train_paths=65536, train_passes=train_passes)
+    cache = ap.RadianceCache(seed=1, width=width, precision=precision)
+    nprm = ap.NrcRenderParams.defaults(train_paths=65536, train_passes=train_passes, spread=spread)
     scratch = ap.DeviceBuffer(n * 12)
     scratch.zero(stream)
     w0 = cache.params()
@@ -200,8 +200,9 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
         "cache_queries_per_frame": st.queries,
         "training_paths_per_frame": st.train_paths,
         "training_samples_per_frame": st.samples,
-        "cache": "64-64x5-3 ReLU, BF16 MFMA / FP32 accumulate, relative L2, Adam lr 0.005, 4 steps x 16384 per "
-                 "frame from 65,536 training paths (the first boost frames take more steps, at their own cost)",
+        "cache": f"64-{width}x5-3 ReLU, {precision.upper()} MFMA / FP32 accumulate, relative L2, Adam lr 0.005, "
+                 "4 steps x 16384 per frame from 65,536 training paths (the first boost frames take more steps, at "
+                 f"their own cost), spread threshold {spread}",
         "spectral": "four hero wavelengths per path in the reference, plain and cache images",
     }
     log(f"nrc: {out}")
@@ -265,6 +266,32 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     stream.synchronize()
     guided4 = timed(use_guiding=1, wavelengths=4)
     guided4["probes"] = len(scene.guiding_export()[1])
+    # Frames with the screen G-buffer and statistics recorded (the extra
+    # per-path first-hit record and the per-pixel updates in accumulate).
+    gbuf = ap.DeviceBuffer(W * H * ap.SCREEN_GBUFFER_DTYPE.itemsize)
+    stats = ap.DeviceBuffer(W * H * ap.SCREEN_STATISTICS_DTYPE.itemsize)
+    gbuf.zero(stream)
+    stats.zero(stream)
+    e0, e1 = ap.Event(), ap.Event()
+    e0.record(stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, wavelengths=4, **params), hdr, stream=stream,
+                 gbuffer=gbuf, statistics=stats)
+    e1.record(stream)
+    stream.synchronize()
+    with_records = {"ms_per_spp": round(e0.elapsed_ms(e1) / spp, 3)}
+    # ap_tonemap at 1920x1080 (ACES): 12 B read + 4 B written per pixel.
+    ldr = ap.DeviceBuffer(W * H * 4)
+    ap.tonemap(hdr, W, H, tone=ap.TONE_ACES, ldr=ldr, stream=stream)
+    reps = 50
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        ap.tonemap(hdr, W, H, tone=ap.TONE_ACES, ldr=ldr, stream=stream)
+    e1.record(stream)
+    stream.synchronize()
+    tm_ms = e0.elapsed_ms(e1) / reps
+    tonemap = {"ms": round(tm_ms, 4), "gb_per_s": round(W * H * 16 / (tm_ms / 1e3) / 1e9, 1),
+               "curve": "ACES", "algorithmic_bytes": W * H * 16}
     out = {
         "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 thin-cube lights, "
                     "no sky, 1920x1080, max depth 16, RR from depth 4",
@@ -273,6 +300,8 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
         "four_wavelengths": four,
         "guided_one_wavelength": guided,
         "guided_four_wavelengths": guided4,
+        "four_wavelengths_with_gbuffer_and_statistics": with_records,
+        "tonemap_1080p": tonemap,
     }
     log(f"render: {out}")
     if nrc_spp > 0:

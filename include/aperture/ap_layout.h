@@ -15,6 +15,8 @@
  *   SceneSetting       13 B   core/render/object.cuh:10-27
  *   PathGuidingSample  64 B   core/guiding/guiding.cuh:22-32
  *   Octahedron (probe) 1392 B core/guiding/octahedron.cuh:203-231
+ *   ScreenGBuffer      64 B   core/denoise/denoise.cuh:33-82
+ *   ScreenStatisticsBuffer 48 B core/denoise/denoise_statistic.cuh:11-24
  *
  * Texture handles (cudaTextureObject_t in the reference) are carried as
  * uint64_t; the core treats any non-zero handle as "not supported" and
@@ -147,6 +149,33 @@ typedef struct ap_octahedron {
     uint64_t count;            /* 1376 */
 } ap_octahedron;
 
+/* denoise::ScreenGBuffer, 64 B, align 16 (core/denoise/denoise.cuh:33-82):
+ * per-pixel running means over frames of the camera ray's first hit. The
+ * half fields are IEEE binary16 bit patterns. shading_model[k] is the
+ * fraction of frames whose first hit had shading model k (diffuse, light,
+ * glass, GGX), [4] the fraction that missed. */
+typedef struct ap_screen_gbuffer {
+    AP_ALIGN(16) uint16_t albedo[3];    /* 0 */
+    uint16_t normal[3];                 /* 6 */
+    uint16_t metallic;                  /* 12 */
+    uint16_t roughness;                 /* 14 */
+    float depth;                        /* 16 */
+    uint64_t valid_frame_index;         /* 24 frames that hit */
+    uint64_t frame_index;               /* 32 frames */
+    AP_ALIGN(16) uint16_t shading_model[5]; /* 48 */
+} ap_screen_gbuffer;
+
+/* denoise::ScreenStatisticsBuffer, 48 B, align 16
+ * (core/denoise/denoise_statistic.cuh:11-24): per-pixel count, mean and
+ * second and third central moments (divided by the count) of the Box-Cox
+ * transformed samples. */
+typedef struct ap_screen_statistics {
+    AP_ALIGN(16) uint32_t num;          /* 0 */
+    ap_float3 mean;                     /* 4 */
+    ap_float3 m2_div_num;               /* 16 */
+    ap_float3 m3_div_num;               /* 28 */
+} ap_screen_statistics;
+
 AP_STATIC_ASSERT(sizeof(ap_gbuffer) == 96, "GBuffer must be 96 B");
 AP_STATIC_ASSERT(offsetof(ap_gbuffer, normal) == 44, "GBuffer.normal_ at 44");
 AP_STATIC_ASSERT(offsetof(ap_gbuffer, albedo_tex) == 64, "GBuffer.albedoTex_ at 64");
@@ -169,6 +198,13 @@ AP_STATIC_ASSERT(sizeof(ap_light_info) == 12, "AdditionalLightInfo must be 12 B"
 AP_STATIC_ASSERT(sizeof(ap_scene_setting) == 13, "SceneSetting must be 13 B");
 AP_STATIC_ASSERT(sizeof(ap_path_guiding_sample) == 64, "PathGuidingSample must be 64 B");
 AP_STATIC_ASSERT(sizeof(ap_octahedron) == 1392, "Octahedron must be 1392 B");
+AP_STATIC_ASSERT(sizeof(ap_screen_gbuffer) == 64, "ScreenGBuffer must be 64 B");
+AP_STATIC_ASSERT(offsetof(ap_screen_gbuffer, depth) == 16, "ScreenGBuffer.depth_ at 16");
+AP_STATIC_ASSERT(offsetof(ap_screen_gbuffer, valid_frame_index) == 24, "ScreenGBuffer.validFrameIndex_ at 24");
+AP_STATIC_ASSERT(offsetof(ap_screen_gbuffer, frame_index) == 32, "ScreenGBuffer.frameIndex_ at 32");
+AP_STATIC_ASSERT(offsetof(ap_screen_gbuffer, shading_model) == 48, "ScreenGBuffer.shadingModel_ at 48");
+AP_STATIC_ASSERT(sizeof(ap_screen_statistics) == 48, "ScreenStatisticsBuffer must be 48 B");
+AP_STATIC_ASSERT(offsetof(ap_screen_statistics, m3_div_num) == 28, "ScreenStatisticsBuffer.M3DivNum_ at 28");
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,9 @@ typedef struct ap_framebuffers {
     double* sum_xyz;     /* W*H*3 FP64 per-pixel sums of the clamped XYZ samples (added) */
     int32_t* primary_id; /* W*H closest-hit objectIndex_ of the camera ray of the last sample, -1 on miss */
     uint64_t* ray_count; /* 2 counters, added to: rays traced by closest-hit [0] and shadow [1] launches */
+    ap_screen_gbuffer* gbuffer;       /* W*H, updated per frame as RecordScreenGBuffer (denoise.cuh:109-166); may be NULL */
+    ap_screen_statistics* statistics; /* W*H, updated per sample as RecordScreenStatisticBuffer
+                                       * (denoise_statistic.cuh:36-70); may be NULL */
 } ap_framebuffers;
 
 /* Reference defaults (config.cuh), spp = 1, frame 0, seed 0, one wavelength per path. */
@@ -82,6 +85,33 @@ int ap_render_offline(ap_scene* scene, const ap_camera* camera, const ap_render_
 
 /* Replaces XYZ2SRGBLinearD65 (core/tonemap.cuh:116-126) over n device XYZ triples. */
 int ap_xyz_to_srgb(const float* xyz, size_t n, float* rgb, void* stream);
+
+/* Tone curves of ap_tonemap (toneType of scene.cu:176-198). */
+#define AP_TONE_LINEAR 0
+#define AP_TONE_GAMMA 1  /* x^(1/2.2) (tonemap.cuh:6-9) */
+#define AP_TONE_ACES 2   /* the ACES fit of tonemap.cuh:67-113 */
+
+/* What ap_tonemap shows (SceneSetting's debug flags, scene.cu:405-447). */
+#define AP_DISPLAY_IMAGE 0           /* hdr: XYZ -> linear sRGB x exposure -> tone curve */
+#define AP_DISPLAY_ALBEDO 1          /* gbuffer albedo */
+#define AP_DISPLAY_NORMAL 2          /* gbuffer normal * 0.5 + 0.5 */
+#define AP_DISPLAY_DEPTH 3           /* viridis(1 / (depth + 1)) */
+#define AP_DISPLAY_METALLIC 4        /* viridis(metallic) */
+#define AP_DISPLAY_ROUGHNESS 5       /* viridis(roughness) */
+#define AP_DISPLAY_STATISTICS_NUM 6  /* red: log(num + 1) / 100 */
+#define AP_DISPLAY_STATISTICS_MEAN 7 /* |mean| / 4 */
+#define AP_DISPLAY_STATISTICS_M2 8   /* |M2 / num| / 4 */
+#define AP_DISPLAY_STATISTICS_M3 9   /* |M3 / num| / 4 */
+
+/* Replaces the Tonemap kernel and its dispatch (core/scene.cu:88-199,
+ * 400-447): width*height pixels into ldr as 0xff000000 | r << 16 | g << 8 |
+ * b, each channel 255 * saturate(v) truncated. The image mode reads hdr
+ * (linear XYZ running means); the G-buffer and statistics modes read
+ * gbuffer / statistics (the others may be NULL). exposure replaces the
+ * reference's device global (SetExposure, scene.cu:1167). Asynchronous. */
+int ap_tonemap(const float* hdr, const ap_screen_gbuffer* gbuffer, const ap_screen_statistics* statistics,
+               uint32_t* ldr, int32_t width, int32_t height, int32_t tone_type, int32_t display, float exposure,
+               void* stream);
 
 #ifdef __cplusplus
 }

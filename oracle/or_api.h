@@ -87,6 +87,31 @@ float or_d65_norm(float lambda);
 void or_lambda_to_xyz(float lambda, float* xyz);
 void or_xyz_to_srgb(const float* xyz, float* rgb);
 
+/* Display and statistics framebuffers (or_display.cpp). or_gsample is what
+ * RecordScreenGBuffer reads of a camera ray's first hit (the layout of the
+ * core's per-path record): albedo, metallic, normal, roughness, hit
+ * distance, shading model (4 for a miss), hit flag. */
+typedef struct or_gsample {
+    float albedo[3];
+    float metallic;
+    float normal[3];
+    float roughness;
+    float depth;
+    int32_t model;
+    int32_t hit;
+    int32_t pad;
+} or_gsample;
+void or_half_round(const float* x, size_t n, uint16_t* out);
+void or_record_gbuffer(ap_screen_gbuffer* g, const or_gsample* samples, size_t n);
+void or_record_statistics(ap_screen_statistics* h, const float* xyz, size_t n);
+void or_tonemap(const float* hdr, const ap_screen_gbuffer* gb, const ap_screen_statistics* st, uint32_t* ldr,
+                size_t n, int tone, int display, float exposure);
+/* or_render with the screen G-buffer and statistics recorded per frame in
+ * frame order (either may be NULL). */
+int or_render_screen(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                     const ap_render_params* prm, float cull_margin, float* hdr, double* sums,
+                     ap_screen_gbuffer* gbuffer, ap_screen_statistics* statistics, int threads);
+
 #ifdef __cplusplus
 }
 #endif

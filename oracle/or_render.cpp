@@ -696,7 +696,7 @@ struct Collect {
 // roulette and the NEE rate read the mean throughput.
 static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir, bool inside0,
                                 int* primary, const GuideState* G = nullptr, int frame = 0,
-                                ap_path_guiding_sample* slots = nullptr, int nl = 1) {
+                                ap_path_guiding_sample* slots = nullptr, int nl = 1, or_gsample* gs = nullptr) {
     const Waves wv = MakeWaves(rng, nl);
     const float* lb = wv.lb[0];
     const bool guiding = G != nullptr;
@@ -719,6 +719,10 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         float t;
         TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t);
         if (depth == 0 && primary) *primary = id;
+        if (depth == 0 && gs) {  // RecordScreenGBuffer's inputs (render.cuh:640-643)
+            *gs = or_gsample{};
+            gs->model = 4;  // MAT_MAX: a miss
+        }
         const float alpha = GuidingAlpha(frame);
         if (id < 0) {
             (void)rng.rand1();  // offset draw
@@ -735,6 +739,15 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         }
         Ctx c = HitContext(S, id, ro, rd, t);
         const int model = S.objs[id].material.shading_model;
+        if (depth == 0 && gs) {
+            gs->albedo[0] = c.gb.albedo.x; gs->albedo[1] = c.gb.albedo.y; gs->albedo[2] = c.gb.albedo.z;
+            gs->metallic = c.gb.metallic;
+            gs->normal[0] = c.normal.x; gs->normal[1] = c.normal.y; gs->normal[2] = c.normal.z;
+            gs->roughness = c.gb.roughness;
+            gs->depth = t;
+            gs->model = model;
+            gs->hit = 1;
+        }
         SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
         float brdf[MAXL];
         for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], model, rd, sc.next, c);
@@ -1243,6 +1256,42 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
                 sums[3 * p + 2] += static_cast<double>(res.z);
             }
             if (primary && s == prm->spp - 1) primary[p] = prim;
+        }
+        hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
+    }
+    return 0;
+}
+
+int or_render_screen(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                     const ap_render_params* prm, float cull_margin, float* hdr, double* sums,
+                     ap_screen_gbuffer* gbuffer, ap_screen_statistics* statistics, int threads) {
+    if (prm->use_guiding) return -1;
+    const Setup U = MakeSetup(bvh, objs, n, cam, prm, cull_margin);
+    const int64_t npix = static_cast<int64_t>(U.W) * U.H;
+    const int nl = prm->wavelengths == 4 ? 4 : 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+    for (int64_t p = 0; p < npix; ++p) {
+        float3 acc{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
+        for (int s = 0; s < prm->spp; ++s) {
+            const int frame = prm->frame_index + s;
+            Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
+            const float3 rd = CameraRay(U, rng, p);
+            or_gsample gs{};
+            float3 res = CalculateRadiance(U.S, U.P, rng, U.camO, rd, U.inside0, nullptr, nullptr, 0, nullptr, nl,
+                                           &gs);
+            res = fmax3(float3{0, 0, 0}, res);
+            acc = lerp(acc, res, 1.0f / static_cast<float>(1 + frame));
+            if (sums) {
+                sums[3 * p] += static_cast<double>(res.x);
+                sums[3 * p + 1] += static_cast<double>(res.y);
+                sums[3 * p + 2] += static_cast<double>(res.z);
+            }
+            // Frame order per pixel: the G-buffer at depth 0, the statistics of the sample.
+            if (gbuffer) or_record_gbuffer(&gbuffer[p], &gs, 1);
+            if (statistics) {
+                const float x[3] = {res.x, res.y, res.z};
+                or_record_statistics(&statistics[p], x, 1);
+            }
         }
         hdr[3 * p] = acc.x; hdr[3 * p + 1] = acc.y; hdr[3 * p + 2] = acc.z;
     }

@@ -48,6 +48,12 @@ def lib():
         L.or_xyz_to_srgb.argtypes = [vp, vp]
         L.or_render.restype = i32
         L.or_render.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, i32]
+        L.or_render_screen.restype = i32
+        L.or_render_screen.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, vp, i32]
+        L.or_half_round.argtypes = [vp, sz, vp]
+        L.or_record_gbuffer.argtypes = [vp, vp, sz]
+        L.or_record_statistics.argtypes = [vp, vp, sz]
+        L.or_tonemap.argtypes = [vp, vp, vp, vp, sz, i32, i32, ctypes.c_float]
         L.or_render_nrc.restype = i32
         L.or_render_nrc.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, ctypes.c_float, ctypes.c_uint32,
                                     ctypes.c_uint32, vp, vp, vp, vp, vp, i32]
@@ -151,6 +157,24 @@ class OracleBVH:
         if rc != 0:
             raise RuntimeError("or_render_guided failed (guiding needs a Guide)")
         return hdr, sums, prim
+
+    def render_screen(self, camera, params, cull_margin=None, threads=None, gbuffer=True, statistics=True):
+        """or_render with the screen G-buffer and statistics records of every
+        pixel updated in frame order: (hdr, sums, gbuffer, statistics)."""
+        import ctypes as C
+
+        import aperture as ap
+        w, h = camera.width, camera.height
+        hdr = np.zeros((h, w, 3), np.float32)
+        sums = np.zeros((h, w, 3), np.float64)
+        gb = np.zeros(w * h, ap.SCREEN_GBUFFER_DTYPE) if gbuffer is True else gbuffer
+        st = np.zeros(w * h, ap.SCREEN_STATISTICS_DTYPE) if statistics is True else statistics
+        rc = lib().or_render_screen(self.h, _p(self.objs), len(self.objs), C.addressof(camera), C.addressof(params),
+                                    float(cull_margin or 0.0), _p(hdr), _p(sums), _p(gb) if gb is not None else None,
+                                    _p(st) if st is not None else None, threads or os.cpu_count())
+        if rc != 0:
+            raise RuntimeError("or_render_screen failed")
+        return hdr, sums, gb, st
 
     def render_nrc(self, camera, params, nrc_params, bmin, bmax, spread=0.01, train_paths=0, unbiased_every=0,
                    cull_margin=None, threads=None, hdr=None, sums=None):
@@ -454,4 +478,49 @@ def lambda_to_xyz(lam):
 def xyz_to_srgb(xyz):
     out = np.zeros(3, np.float32)
     lib().or_xyz_to_srgb(_p(_f32(xyz)), _p(out))
+    return out
+
+
+# Display and statistics framebuffers (or_display.cpp).
+GSAMPLE_DTYPE = np.dtype({"names": ["albedo", "metallic", "normal", "roughness", "depth", "model", "hit", "pad"],
+                          "formats": [("<f4", 3), "<f4", ("<f4", 3), "<f4", "<f4", "<i4", "<i4", "<i4"],
+                          "offsets": [0, 12, 16, 28, 32, 36, 40, 44], "itemsize": 48})
+
+
+def half_round(x):
+    """binary16 bits of float32 values (round to nearest even)."""
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    out = np.zeros(len(x), np.uint16)
+    lib().or_half_round(_p(x), len(x), _p(out))
+    return out
+
+
+def record_gbuffer(samples, gbuffer=None):
+    """RecordScreenGBuffer over a sequence of GSAMPLE_DTYPE samples into one record."""
+    import aperture as ap
+    g = np.zeros(1, ap.SCREEN_GBUFFER_DTYPE) if gbuffer is None else \
+        np.array(gbuffer, ap.SCREEN_GBUFFER_DTYPE).reshape(1)
+    s = np.ascontiguousarray(samples, GSAMPLE_DTYPE)
+    lib().or_record_gbuffer(_p(g), _p(s), len(s))
+    return g[0]
+
+
+def record_statistics(xyz, stats=None):
+    """RecordScreenStatisticBuffer over (n, 3) samples into one record."""
+    import aperture as ap
+    st = np.zeros(1, ap.SCREEN_STATISTICS_DTYPE) if stats is None else \
+        np.array(stats, ap.SCREEN_STATISTICS_DTYPE).reshape(1)
+    x = _f32(xyz, (-1, 3))
+    lib().or_record_statistics(_p(st), _p(x), len(x))
+    return st[0]
+
+
+def tonemap(hdr=None, gbuffer=None, statistics=None, n=None, tone=0, display=0, exposure=1.0):
+    """The Tonemap kernel restated: uint32 ARGB per pixel."""
+    if n is None:
+        n = (np.asarray(hdr).size // 3) if hdr is not None else len(gbuffer if gbuffer is not None else statistics)
+    out = np.zeros(n, np.uint32)
+    h = _f32(hdr, (-1, 3)) if hdr is not None else None
+    lib().or_tonemap(_p(h) if h is not None else None, _p(gbuffer) if gbuffer is not None else None,
+                     _p(statistics) if statistics is not None else None, _p(out), n, tone, display, exposure)
     return out
