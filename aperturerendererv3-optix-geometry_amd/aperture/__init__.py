@@ -190,7 +190,8 @@ EXPORTED_SYMBOLS = (
     "ap_nrc_set_group_bounds", "ap_nrc_infer_grouped", "ap_nrc_gradient_grouped", "ap_nrc_train_step_grouped",
     "ap_batch_create", "ap_batch_destroy", "ap_batch_render", "ap_batch_set_launch", "ap_batch_get_launch",
     "ap_batch_set_graph", "ap_batch_autotune", "ap_render_nrc", "ap_nrc_last_frame", "ap_batch_render_nrc",
-    "ap_batch_nrc_last_frame", "ap_tonemap",
+    "ap_batch_nrc_last_frame", "ap_tonemap", "ap_sdf_cache_create", "ap_sdf_cache_destroy",
+    "ap_sdf_cache_add_functor", "ap_sdf_cache_add_volume", "ap_sdf_cache_get_volume", "ap_scene_set_sdf_cache",
 )
 
 
@@ -240,6 +241,12 @@ def lib():
         L.ap_batch_render_nrc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.ap_batch_nrc_last_frame.argtypes = [vp, vp, sz]
         L.ap_tonemap.argtypes = [vp, vp, vp, vp, i32, i32, i32, i32, ctypes.c_float, vp]
+        L.ap_sdf_cache_create.argtypes = [vp]
+        L.ap_sdf_cache_destroy.argtypes = [vp]
+        L.ap_sdf_cache_add_functor.argtypes = [vp, i32, vp, vp, vp]
+        L.ap_sdf_cache_add_volume.argtypes = [vp, vp, vp, vp]
+        L.ap_sdf_cache_get_volume.argtypes = [vp, ctypes.c_uint32, vp, vp]
+        L.ap_scene_set_sdf_cache.argtypes = [vp, vp]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
@@ -518,6 +525,57 @@ def xyz_to_srgb(xyz):
     return d_out.to_numpy(np.float32, a.size).reshape(a.shape)
 
 
+SDF_TORUS, SDF_VOLUME = 0, 1
+SDF_FUNCTOR_TORUS, SDF_FUNCTOR_DIAMOND = 0, 1
+
+
+class SdfCache:
+    """Volumes of signed distances for SDF_VOLUME objects (ap_sdf_cache_*)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().ap_sdf_cache_create(ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def add_functor(self, functor, info, dims):
+        """A volume of the torus or diamond functor on a dims grid; returns its index."""
+        inf = np.ascontiguousarray(info, np.float32).reshape(4)
+        dm = np.ascontiguousarray(dims, np.int32).reshape(3)
+        idx = ctypes.c_uint32()
+        check(lib().ap_sdf_cache_add_functor(self.handle, functor, _ptr(inf), _ptr(dm), ctypes.byref(idx)))
+        return idx.value
+
+    def add_volume(self, values):
+        """A volume from a host (nz, ny, nx) array; returns its index."""
+        v = np.ascontiguousarray(values, np.float32)
+        dm = np.array([v.shape[2], v.shape[1], v.shape[0]], np.int32)
+        idx = ctypes.c_uint32()
+        check(lib().ap_sdf_cache_add_volume(self.handle, _ptr(v), _ptr(dm), ctypes.byref(idx)))
+        return idx.value
+
+    def volume(self, index):
+        """Volume `index` as a host (nz, ny, nx) array."""
+        dm = np.zeros(3, np.int32)
+        check(lib().ap_sdf_cache_get_volume(self.handle, index, _ptr(dm), None))
+        out = np.zeros((dm[2], dm[1], dm[0]), np.float32)
+        check(lib().ap_sdf_cache_get_volume(self.handle, index, _ptr(dm), _ptr(out)))
+        return out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().ap_sdf_cache_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def tonemap(hdr, width, height, tone=TONE_LINEAR, display=DISPLAY_IMAGE, exposure=1.0, gbuffer=None,
             statistics=None, ldr=None, stream=None):
     """ap_tonemap over device buffers; returns the device ldr buffer (uint32 ARGB)."""
@@ -752,6 +810,11 @@ class SceneRenderer:
         out = BVHInfo()
         check(lib().ap_scene_get_info(self.handle, ctypes.byref(out)))
         return out
+
+    def set_sdf_cache(self, cache):
+        """The SdfCache the scene's SDF_VOLUME objects read (None: none)."""
+        self._sdf_cache = cache
+        check(lib().ap_scene_set_sdf_cache(self.handle, cache.handle if cache is not None else None))
 
     def set_traversal_mode(self, mode):
         check(lib().ap_scene_set_traversal_mode(self.handle, mode))

@@ -30,9 +30,28 @@ constexpr int kShadingModels = 5;  // EShadingModel::MAT_MAX + 1 (object.cuh:45-
 __device__ __forceinline__ float half_to_float(uint16_t h) {
     return static_cast<float>(__builtin_bit_cast(_Float16, h));
 }
-// FP32 -> binary16, round to nearest even (as CUDA's __float2half).
+// FP32 -> binary16, round to nearest even (as CUDA's __float2half), with
+// integer operations on the FP32 bits: the FP32 value is rounded first. (A
+// plain conversion lets the compiler fuse it with the multiply or add that
+// produced the value into one v_fma_mix, which rounds only once.)
 __device__ __forceinline__ uint16_t float_to_half(float f) {
-    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(f));
+    uint32_t u = __float_as_uint(f);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    u &= 0x7fffffffu;
+    if (u > 0x7f800000u) return static_cast<uint16_t>(sign | 0x7e00u);   // NaN
+    if (u >= 0x477ff000u) return static_cast<uint16_t>(sign | 0x7c00u);  // rounds past 65504
+    if (u >= 0x38800000u) {                                              // normal binary16
+        uint32_t r = u - (112u << 23);
+        r += 0xfffu + ((r >> 13) & 1u);
+        return static_cast<uint16_t>(sign | (r >> 13));
+    }
+    if (u <= 0x33000000u) return static_cast<uint16_t>(sign);            // at most 2^-25: zero
+    const uint32_t e = u >> 23, m = (u & 0x7fffffu) | 0x800000u;
+    const uint32_t sh = 126u - e;
+    uint32_t q = m >> sh;
+    const uint32_t rem = m & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return static_cast<uint16_t>(sign | q);
 }
 
 // RecordShadingModel (denoise.cuh:84-101).

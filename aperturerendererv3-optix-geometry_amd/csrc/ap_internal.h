@@ -43,23 +43,32 @@ enum : uint32_t {
     PRIM_KIND_NEVER = 1u,       // invisible or unsupported type: always a miss
     PRIM_KIND_CUBE = 2u,        // analytic OBB (core/object/aabb.cuh:28-89)
     PRIM_KIND_SPHERE = 3u,      // analytic sphere (core/object/sphere.cuh:37-70)
+    PRIM_KIND_SDF = 4u,         // signed-distance object (core/object/sdf/sdf_base.cuh:115-163)
 };
 
 struct alignas(16) PrimRec {
     float p0[3]; int32_t obj;     // obj = SceneObject index (the hit ID)
     float e1[3]; uint32_t flags;  // PRIM_LAST | kind << PRIM_KIND_SHIFT
-    float e2[3]; int32_t aux;     // analytic record index for CUBE/SPHERE
+    float e2[3]; int32_t aux;     // analytic record index for CUBE/SPHERE/SDF
 };
 static_assert(sizeof(PrimRec) == 48, "PrimRec is 48 B");
 
-// Transform data of an analytic (cube or sphere) primitive.
+// Transform data of an analytic (cube, sphere or SDF) primitive.
 struct alignas(16) AnalyticRec {
-    float center[3]; float pad0;
-    float extent[3]; float pad1;
-    float w2o[9];    float pad2[3];   // worldToObject_ rows
-    float o2w[9];    float pad3[3];   // objectToWorld_ rows
+    float center[3]; int32_t sdf_type;  // SDF objects: ESDFType; others -1
+    float extent[3]; int32_t volume;    // SDF_VOLUME: cachedVolumeIndex_
+    float w2o[9];    float pad2[3];     // worldToObject_ rows
+    float o2w[9];    float pad3[3];     // objectToWorld_ rows
+    float sdf_info[4];                  // SDFInfo::sdfInfo0_
 };
-static_assert(sizeof(AnalyticRec) == 128, "AnalyticRec is 128 B");
+static_assert(sizeof(AnalyticRec) == 144, "AnalyticRec is 144 B");
+
+struct SdfVolume;  // ap_sdf.h
+// The SDF cache's device volume table and its capacity (ap_sdf.hip), and a
+// host view of volume k (samples in host memory).
+const SdfVolume* sdf_cache_table(const ap_sdf_cache* c);
+int32_t sdf_cache_capacity(const ap_sdf_cache* c);
+bool sdf_cache_host_volume(const ap_sdf_cache* c, int32_t k, SdfVolume& v);
 
 struct RootInfo {
     float bmin[3];
@@ -72,6 +81,10 @@ struct DeviceScene {
     const NodeRec* nodes;
     const PrimRec* prims;
     const AnalyticRec* analytic;
+    const SdfVolume* volumes;   // the SDF volume table (ap_scene_set_sdf_cache), or none
+    int32_t n_volumes;
+    int32_t has_sdf;            // the scene holds SDF objects (traversal runs the marching variant)
+    float sdf_eps_any;          // SDF marching epsilon of any-hit launches (closest hit: tmin / 2)
     RootInfo root;
     int32_t stack_cap;
     float line_margin;   // world-space growth of boxes for the CULL line test
@@ -114,6 +127,7 @@ struct ap_scene {
     int64_t device_bytes = 0;
     ap::RenderState* render = nullptr;
     ap::GuideDevice* guide = nullptr;
+    const ap_sdf_cache* sdf_cache = nullptr;  // volumes of SDF_VOLUME objects (ap_scene_set_sdf_cache)
 };
 
 namespace ap {
@@ -128,15 +142,17 @@ int spectrum_get(const f4** rgb_lut, const f4** lambda_lut, const float** tables
 // Kernel launchers (ap_trace.hip).
 // dyn_n (optional, device): the actual ray count, read by the kernel; n is
 // then the capacity the grid is sized for.
+// sdf_eps_any: the SDF marching epsilon of an any-hit launch (SDF_EPS of the
+// shadow rays' depth); closest-hit rays march with tmin / 2.
 hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n, ap_hit* hits,
                         ap_trace_stats* stats, bool any_hit, hipStream_t stream,
-                        const uint32_t* dyn_n = nullptr);
+                        const uint32_t* dyn_n = nullptr, float sdf_eps_any = 0.5f * 1.0e-3f);
 // Rays of several scenes in one queue: ray j traverses scenes[ray_scene[j]]
 // (a device array); `shape` gives the launch shape, stack capacity (the
 // largest of the scenes') and traversal mode.
 hipError_t launch_trace_multi(const DeviceScene& shape, const DeviceScene* scenes, const uint16_t* ray_scene,
                               const ap_ray* rays, size_t n_total, ap_hit* hits, bool any_hit, hipStream_t stream,
-                              const uint32_t* dyn_n);
+                              const uint32_t* dyn_n, float sdf_eps_any = 0.5f * 1.0e-3f);
 
 // Radiance cache entry points for the integrator (ap_nrc.hip).
 struct NrcState;

@@ -35,6 +35,7 @@
 #include "ap_spectrum.h"
 #include "ap_guide.h"
 #include "ap_display.h"
+#include "ap_sdf.h"
 
 namespace ap {
 
@@ -129,6 +130,9 @@ struct alignas(16) BatchScene {
     uint32_t n_pix;
     float cam[14];     // origin, dir, right, up (f3 each), tan_x, tan_y
     int32_t width, height;
+    const SdfVolume* volumes;  // the scene's SDF volume table
+    int32_t n_volumes;
+    int32_t pad;
 };
 
 // What the batch kernels see of a batch.
@@ -150,7 +154,7 @@ struct RenderState {
     LightRec* lights = nullptr;
     int32_t n_lights = 0;
     int32_t n_mats = 0;
-    std::vector<int32_t> analytic_objs;   // cube and sphere objects (the camera-inside test)
+    std::vector<int32_t> analytic_objs;   // cube, sphere and SDF objects (the camera-inside test)
     // path buffers
     size_t capacity = 0;
     void* paths = nullptr;            // PathT<NL>, kPathBytes per entry
@@ -446,6 +450,8 @@ struct ShadeParams {
     int32_t out_q;
     int32_t n_lights;
     GSample* gsample = nullptr;   // depth 0: each path's first-hit G-buffer sample, or none
+    const SdfVolume* volumes = nullptr;  // the scene's SDF volume table
+    int32_t n_volumes = 0;
 };
 
 // Hit attributes as the reference intersectors report them.
@@ -454,7 +460,8 @@ struct HitInfo {
     int32_t mat, light_index;
 };
 
-__device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict__ an, f3 o, f3 d, float t) {
+__device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict__ an, f3 o, f3 d, float t,
+                                  const SdfVolume* __restrict__ vols, int32_t n_vols, float sdf_eps) {
     HitInfo h;
     h.mat = g.mat;
     h.light_index = g.light_index;
@@ -477,6 +484,28 @@ __device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict
     const m3 o2w{{a.o2w[0], a.o2w[1], a.o2w[2]}, {a.o2w[3], a.o2w[4], a.o2w[5]}, {a.o2w[6], a.o2w[7], a.o2w[8]}};
     const f3 lo = mul(w2o, o - c);
     const f3 ld = normalize(mul(w2o, d));
+    if (g.type == AP_OBJ_SDF) {
+        // IntersectSDFObject (sdf_base.cuh:115-163): the march again for the
+        // local hit point, the central-difference normal there.
+        SdfField F;
+        if (!sdf_field(a.sdf_type, a.volume, a.sdf_info, vols, n_vols, F)) {  // not reached: traversal checks it
+            h.pos = o + d * t;
+            h.normal = -d;
+            return h;
+        }
+        const f3 div = inv_safe(ld);
+        const f3 t1 = (-ext - lo) * div, t2 = (ext - lo) * div;
+        const f3 mn = fmin3(t1, t2), mx = fmax3(t1, t2);
+        const float te = fmaxf(fmaxf(mn.x, mn.y), mn.z);
+        const float tx = fminf(fminf(mx.x, mx.y), mx.z);
+        float th = t;
+        f3 lp = lo + ld * t;
+        (void)sdf_march(F, lo, ld, te, tx, ext, sdf_eps, th, lp);
+        const f3 ln = sdf_local_normal(F, lp, ext, sdf_eps);
+        h.pos = mul(o2w, lp) + c;
+        h.normal = normalize(mul(o2w, ln));
+        return h;
+    }
     if (g.type == AP_OBJ_CUBE) {
         // IntersectAABBObject (aabb.cuh:28-89).
         const f3 div = inv_safe(ld);
@@ -625,6 +654,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     // The path's scene: the batch's record of it, or the kernel arguments.
     uint16_t scene_id = 0;
     int32_t n_lights = sp.n_lights;
+    const SdfVolume* volumes = sp.volumes;
+    int32_t n_volumes = sp.n_volumes;
     if constexpr (MULTI) {
         scene_id = static_cast<uint16_t>(S.flags >> 16);
         const BatchScene& B = bv.scenes[scene_id];
@@ -633,6 +664,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         lights = B.lights;
         analytic = B.analytic;
         n_lights = B.n_lights;
+        volumes = B.volumes;
+        n_volumes = B.n_volumes;
     }
     const bool inside = (S.flags & PATH_INSIDE) != 0u;
     if (sp.depth == 0 && primary_id && S.frame == primary_frame) primary_id[S.pixel] = H.id;
@@ -665,7 +698,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         S.flags &= ~PATH_PREV_NEE;
     } else {
         const GeoRec g = geo[H.id];
-        const HitInfo hi = hit_attributes(g, analytic, o, d, H.t);
+        // SDF_EPS of the extension ray's depth: INTERSECT_EPS(ndd) / 2.
+        const HitInfo hi = hit_attributes(g, analytic, o, d, H.t, volumes, n_volumes, eps[S.ndd] * 0.5f);
         const MatRec M = mats[hi.mat];
         const f3 N = hi.normal;
         f3 hp = hi.pos;
@@ -1472,7 +1506,8 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
         int32_t k = 0;
         for (size_t slot = 0; slot < s->bvh.indices.size(); ++slot) {
             const ap_scene_object& o = s->objects[s->bvh.indices[slot]];
-            if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SPHERE) analytic_of[s->bvh.indices[slot]] = k++;
+            if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SPHERE || o.type == AP_OBJ_SDF)
+                analytic_of[s->bvh.indices[slot]] = k++;
         }
     }
     for (size_t i = 0; i < n; ++i) {
@@ -1553,7 +1588,8 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
     r->n_mats = static_cast<int32_t>(mats.size());
     r->analytic_objs.clear();
     for (size_t i = 0; i < n; ++i)
-        if (s->objects[i].type == AP_OBJ_CUBE || s->objects[i].type == AP_OBJ_SPHERE)
+        if (s->objects[i].type == AP_OBJ_CUBE || s->objects[i].type == AP_OBJ_SPHERE ||
+            s->objects[i].type == AP_OBJ_SDF)
             r->analytic_objs.push_back(static_cast<int32_t>(i));
     r->tri_mode = tri_mode;
     return AP_OK;
@@ -1581,13 +1617,27 @@ int ensure_paths(RenderState* r, size_t cap) {
 }
 
 // OverlapBase at the camera origin over every object (intersection.cuh:399-414,
-// aabb.cuh:91-106, sphere.cuh:179-192): whether the camera starts inside one.
+// aabb.cuh:91-106, sphere.cuh:179-192, sdf_base.cuh:165-177): whether the
+// camera starts inside one.
 bool camera_inside(const ap_scene* s, const std::vector<int32_t>& analytic, f3 o) {
     for (const int32_t k : analytic) {
         const ap_scene_object& ob = s->objects[k];
         const f3 lo = mul(M3(ob.world_to_object), o - F(ob.center));
         const f3 e = F(ob.extent);
-        if (ob.type == AP_OBJ_CUBE) {
+        if (ob.type == AP_OBJ_SDF) {
+            const ap_float4& q = ob.additional.sdf_info.sdf_info0;
+            const float info[4] = {q.x, q.y, q.z, q.w};
+            SdfVolume host{};
+            SdfField field;
+            bool ok = false;
+            if (ob.additional.sdf_type == SDF_TORUS) {
+                ok = sdf_field(SDF_TORUS, -1, info, nullptr, 0, field);
+            } else if (ob.additional.sdf_type == SDF_VOLUME &&
+                       sdf_cache_host_volume(s->sdf_cache, ob.additional.sdf_info.cached_volume_index, host)) {
+                ok = sdf_field(SDF_VOLUME, 0, info, &host, 1, field);
+            }
+            if (ok && field(to_unit(lo, e)) <= 0.0f) return true;
+        } else if (ob.type == AP_OBJ_CUBE) {
             if (-e.x < lo.x && -e.y < lo.y && -e.z < lo.z && e.x > lo.x && e.y > lo.y && e.z > lo.z) return true;
         } else {
             const float rr = max3(e);
@@ -1629,6 +1679,12 @@ int ensure_nrc(RenderState* r, size_t cap, size_t train_cap) {
         r->nrc_train_cap = train_cap;
     }
     return AP_OK;
+}
+
+// SDF_EPS(depth) = INTERSECT_EPS(depth) / 2 (config.cuh:74), INTERSECT_EPS as
+// the device table computes it: the marching epsilon of a bounce's shadow rays.
+float sdf_eps_of_depth(int depth) {
+    return 0.5f * (1.0e-3f * std::exp(fminf(static_cast<float>(depth) * 0.5f, 4.0f)));
 }
 
 // Frame hash of the training-path offset and the training draws.
@@ -1704,6 +1760,8 @@ int run_frames(const RunCtx& c) {
             ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                            prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
             if (fb->gbuffer && !train_only) sp.gsample = r->gsamples;
+            sp.volumes = s->dev.volumes;
+            sp.n_volumes = s->dev.n_volumes;
             if (guiding) {
                 hipLaunchKernelGGL((shade_kernel<NL, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
                                    r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
@@ -1721,7 +1779,8 @@ int run_frames(const RunCtx& c) {
                                    r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
                                    r->shadow_rays, pays, r->counters, prim, last_frame, rc_ptr, gv, nv);
             }
-            e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2]);
+            e = launch_trace(s->dev, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2],
+                             sdf_eps_of_depth(depth));
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
             hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->counters, pays,
                                r->shadow_hits, paths, rc_ptr);
@@ -1996,7 +2055,10 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
         B.n_lights = r->n_lights;
         const f3 o{b->cams[k].origin[0], b->cams[k].origin[1], b->cams[k].origin[2]};
         B.inside = camera_inside(s, r->analytic_objs, o) ? 1 : 0;
+        B.volumes = s->dev.volumes;
+        B.n_volumes = s->dev.n_volumes;
         b->h_devs[k] = s->dev;
+        if (s->dev.has_sdf) b->shape.has_sdf = 1;
     }
     hipError_t e = hipMemcpy(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
                              hipMemcpyHostToDevice);
@@ -2032,7 +2094,7 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
                            b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, NrcView{}, bv);
         e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
-                               stream, &b->counters[2]);
+                               stream, &b->counters[2], sdf_eps_of_depth(depth));
         if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
         hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
                            b->shadow_hits, paths, b->rays_traced);
@@ -2209,7 +2271,7 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
                            b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, nv, bv);
         e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
-                               stream, &b->counters[2]);
+                               stream, &b->counters[2], sdf_eps_of_depth(depth));
         if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
         hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
                            b->shadow_hits, paths, b->rays_traced);

@@ -124,7 +124,7 @@ static int build_device(ap_scene* s) {
             p.e1[0] = e1.x; p.e1[1] = e1.y; p.e1[2] = e1.z;
             p.e2[0] = e2.x; p.e2[1] = e2.y; p.e2[2] = e2.z;
             kind = PRIM_KIND_TRIANGLE;
-        } else if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SPHERE) {
+        } else if (o.type == AP_OBJ_CUBE || o.type == AP_OBJ_SPHERE || o.type == AP_OBJ_SDF) {
             AnalyticRec a;
             std::memset(&a, 0, sizeof(a));
             a.center[0] = o.center.x; a.center[1] = o.center.y; a.center[2] = o.center.z;
@@ -132,9 +132,18 @@ static int build_device(ap_scene* s) {
             const float* w = &o.world_to_object.x.x;
             const float* v = &o.object_to_world.x.x;
             for (int k = 0; k < 9; ++k) { a.w2o[k] = w[k]; a.o2w[k] = v[k]; }
+            a.sdf_type = -1;
+            a.volume = -1;
+            if (o.type == AP_OBJ_SDF) {
+                a.sdf_type = o.additional.sdf_type;
+                a.volume = o.additional.sdf_info.cached_volume_index;
+                const ap_float4& q = o.additional.sdf_info.sdf_info0;
+                a.sdf_info[0] = q.x; a.sdf_info[1] = q.y; a.sdf_info[2] = q.z; a.sdf_info[3] = q.w;
+                s->dev.has_sdf = 1;
+            }
             p.aux = static_cast<int32_t>(analytic.size());
             analytic.push_back(a);
-            kind = o.type == AP_OBJ_CUBE ? PRIM_KIND_CUBE : PRIM_KIND_SPHERE;
+            kind = o.type == AP_OBJ_CUBE ? PRIM_KIND_CUBE : (o.type == AP_OBJ_SPHERE ? PRIM_KIND_SPHERE : PRIM_KIND_SDF);
         }
         p.flags = kind << PRIM_KIND_SHIFT;
     }
@@ -190,9 +199,6 @@ int ap_scene_create(const ap_scene_object* objects, size_t count, const ap_build
         return fail(AP_ERR_INVALID_ARG, "ap_scene_create: too many objects");
     for (size_t i = 0; i < count; ++i) {
         const int t = objects[i].type;
-        if (t == AP_OBJ_SDF)
-            return fail(AP_ERR_UNSUPPORTED, "ap_scene_create: SDF objects are not supported (object " +
-                                                std::to_string(i) + ")");
         if (t < AP_OBJ_CUBE || t > AP_OBJ_TRIANGLE)
             return fail(AP_ERR_INVALID_ARG, "ap_scene_create: unknown object type at " + std::to_string(i));
     }
@@ -384,6 +390,14 @@ static int trace_common(const ap_scene* s, const ap_ray* rays, size_t n, ap_hit*
         return fail(AP_ERR_INVALID_ARG, std::string(name) + ": too many rays");
     const hipError_t e = launch_trace(s->dev, rays, n, hits, stats, any, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, name);
+    return AP_OK;
+}
+
+int ap_scene_set_sdf_cache(ap_scene* s, const ap_sdf_cache* cache) {
+    if (!s) return fail(AP_ERR_INVALID_ARG, "ap_scene_set_sdf_cache: NULL scene");
+    s->sdf_cache = cache;
+    s->dev.volumes = sdf_cache_table(cache);
+    s->dev.n_volumes = sdf_cache_capacity(cache);
     return AP_OK;
 }
 

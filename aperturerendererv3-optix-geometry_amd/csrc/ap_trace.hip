@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "ap_internal.h"
+#include "ap_sdf.h"
 
 namespace ap {
 namespace {
@@ -72,6 +73,28 @@ __device__ bool sphere_t(f3 o, f3 d, const AnalyticRec& a, float& t) {
     return true;
 }
 
+// IntersectSDFObject (core/object/sdf/sdf_base.cuh:115-163), distance only:
+// the local ray through the object's box, then RaymarchLocal with marching
+// epsilon eps.
+__device__ bool sdf_t(f3 o, f3 d, const AnalyticRec& a, const SdfVolume* vols, int32_t n_vols, float eps, float& t) {
+    SdfField F;
+    if (!sdf_field(a.sdf_type, a.volume, a.sdf_info, vols, n_vols, F)) return false;
+    const f3 c = ld3(a.center), ext = ld3(a.extent);
+    const m3 w2o = ld33(a.w2o);
+    const f3 lo = mul(w2o, o - c);
+    const f3 ld = normalize(mul(w2o, d));
+    const f3 div = inv_safe(ld);
+    const f3 t1 = (-ext - lo) * div;
+    const f3 t2 = (ext - lo) * div;
+    const f3 mn = fmin3(t1, t2), mx = fmax3(t1, t2);
+    const float te = fmaxf(fmaxf(mn.x, mn.y), mn.z);
+    const float tx = fminf(fminf(mx.x, mx.y), mx.z);
+    if (te > tx) return false;
+    if (tx <= 0.0f) return false;
+    f3 lp;
+    return sdf_march(F, lo, ld, te, tx, ext, eps, t, lp);
+}
+
 constexpr uint32_t kXcds = 8;
 constexpr int RING = 8;  // LDS stack entries per lane (a power of two below every CAP)  // gfx950: workgroups are dealt round-robin to 8 XCDs
 
@@ -93,7 +116,7 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) {
 // MULTI: rays of several scenes in one queue; ray j belongs to scene
 // ray_scene[j] and traverses scenes[ray_scene[j]] (`s` then only carries the
 // launch shape). Otherwise every ray traverses `s`.
-template <bool ANY, bool STATS, bool CULL, int CAP, bool MULTI = false>
+template <bool ANY, bool STATS, bool CULL, int CAP, bool MULTI = false, bool SDF = false>
 __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray* __restrict__ rays,
                                                      uint32_t n_cap, const uint32_t* __restrict__ dyn_n,
                                                      ap_hit* __restrict__ hits,
@@ -134,6 +157,8 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
     const NodeRec* nodes = s.nodes;
     const PrimRec* prims = s.prims;
     const AnalyticRec* analytic = s.analytic;
+    const SdfVolume* volumes = s.volumes;
+    int32_t n_volumes = s.n_volumes;
 
     for (;;) {
         const uint64_t idle = __ballot(!live);
@@ -162,6 +187,10 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                         nodes = D.nodes;
                         prims = D.prims;
                         analytic = D.analytic;
+                        if constexpr (SDF) {
+                            volumes = D.volumes;
+                            n_volumes = D.n_volumes;
+                        }
                         root = D.root;
                         margin = D.line_margin;
                     }
@@ -228,6 +257,11 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                         ok = cube_t(o, d, analytic[__float_as_int(C.w)], t);
                     } else if (kind == PRIM_KIND_SPHERE) {
                         ok = sphere_t(o, d, analytic[__float_as_int(C.w)], t);
+                    } else if (SDF && kind == PRIM_KIND_SDF) {
+                        // SDF_EPS of the ray's depth: tmin / 2 for closest hit (tmin is
+                        // INTERSECT_EPS of the depth), the launch's value for any hit.
+                        const float eps = ANY ? s.sdf_eps_any : (tmin > 0.0f ? tmin * 0.5f : kSdfEps0);
+                        ok = sdf_t(o, d, analytic[__float_as_int(C.w)], volumes, n_volumes, eps, t);
                     } else {
                         ok = false;
                     }
@@ -297,8 +331,12 @@ hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, uint32_t n, cons
     if (cap <= C) {                                                                                     \
         const uint32_t chunk = static_cast<uint32_t>(s.wave_rays);                                      \
         const uint32_t waves = std::max((n + chunk - 1) / chunk, std::min((n + WAVE - 1) / WAVE, kMinWaves)); \
-        hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C, MULTI>), dim3(waves), block, 0, stream, s, rays, \
-                           n, dyn_n, hits, stats, scenes, ray_scene);                                   \
+        if (s.has_sdf)                                                                                  \
+            hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C, MULTI, true>), dim3(waves), block, 0, stream, s, \
+                               rays, n, dyn_n, hits, stats, scenes, ray_scene);                         \
+        else                                                                                            \
+            hipLaunchKernelGGL((trace_kernel<ANY, STATS, CULL, C, MULTI>), dim3(waves), block, 0, stream, s, \
+                               rays, n, dyn_n, hits, stats, scenes, ray_scene);                         \
         return hipGetLastError();                                                                       \
     }
     AP_CAP_CASE(16)
@@ -314,9 +352,12 @@ hipError_t launch_cap(const DeviceScene& s, const ap_ray* rays, uint32_t n, cons
 
 }  // namespace
 
-hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n_total, ap_hit* hits,
-                        ap_trace_stats* stats, bool any_hit, hipStream_t stream, const uint32_t* dyn_n) {
+hipError_t launch_trace(const DeviceScene& scene, const ap_ray* rays, size_t n_total, ap_hit* hits,
+                        ap_trace_stats* stats, bool any_hit, hipStream_t stream, const uint32_t* dyn_n,
+                        float sdf_eps_any) {
     if (n_total == 0) return hipSuccess;
+    DeviceScene s = scene;
+    s.sdf_eps_any = sdf_eps_any;
     // Ray indices are 32-bit inside the kernel; larger host batches go in slices.
     constexpr size_t kSlice = size_t{1} << 31;
     if (n_total > kSlice) {
@@ -324,7 +365,7 @@ hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n_total
         for (size_t off = 0; off < n_total; off += kSlice) {
             const size_t m = std::min(kSlice, n_total - off);
             const hipError_t e = launch_trace(s, rays + off, m, hits + off, stats ? stats + off : nullptr, any_hit,
-                                              stream, nullptr);
+                                              stream, nullptr, sdf_eps_any);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -345,10 +386,12 @@ hipError_t launch_trace(const DeviceScene& s, const ap_ray* rays, size_t n_total
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_trace_multi(const DeviceScene& shape, const DeviceScene* scenes, const uint16_t* ray_scene,
+hipError_t launch_trace_multi(const DeviceScene& batch_shape, const DeviceScene* scenes, const uint16_t* ray_scene,
                               const ap_ray* rays, size_t n_total, ap_hit* hits, bool any_hit, hipStream_t stream,
-                              const uint32_t* dyn_n) {
+                              const uint32_t* dyn_n, float sdf_eps_any) {
     if (n_total == 0) return hipSuccess;
+    DeviceScene shape = batch_shape;
+    shape.sdf_eps_any = sdf_eps_any;
     if (n_total > (size_t{1} << 31)) return hipErrorInvalidValue;
     const uint32_t n = static_cast<uint32_t>(n_total);
     if (shape.mode == AP_TRAVERSE_CULLED)

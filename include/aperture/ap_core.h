@@ -153,6 +153,41 @@ int ap_trace_any(const ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hi
 int ap_trace_closest_stats(const ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hits,
                            ap_trace_stats* stats, void* stream);
 
+/* Signed-distance objects. A SceneObject of type AP_OBJ_SDF with
+ * additionalObjectInfo_.sdfType_ SDF_TORUS (0) is the torus of
+ * sdfInfo0_.xy = (R, r) in the unit space of its box; SDF_VOLUME (1) samples
+ * the volume cachedVolumeIndex_ of the scene's SDF cache; other types are
+ * never hit (IntersectSDFObject, core/object/sdf/sdf_master.cuh:32-70). Rays
+ * march up to 512 steps (RaymarchLocal, sdf_base.cuh:54-110) with epsilon
+ * SDF_EPS = INTERSECT_EPS(depth) / 2: tmin / 2 for ap_trace_closest (tmin is
+ * INTERSECT_EPS of the ray's depth in the integrator; 5e-4 when tmin is 0),
+ * 5e-4 for ap_trace_any. SDF objects cannot be sampled as lights (as in
+ * NEESampleLight, nee.cuh:330-377). */
+#define AP_SDF_TORUS 0
+#define AP_SDF_VOLUME 1
+#define AP_SDF_FUNCTOR_TORUS 0    /* SdfTorus of info.xy (sdf_torus.cuh:15-26) */
+#define AP_SDF_FUNCTOR_DIAMOND 1  /* SdfDiamondUnit of info (sdf_diamond.cuh:21-73) */
+
+typedef struct ap_sdf_cache ap_sdf_cache;
+
+/* Replaces SDFCacheManager (core/object/sdf/sdf_cache_manager.cuh:36-114):
+ * up to 128 volumes on the current device. */
+int ap_sdf_cache_create(ap_sdf_cache** out);
+int ap_sdf_cache_destroy(ap_sdf_cache* cache);
+/* AddVolume with a built-in functor: dims[0] x dims[1] x dims[2] samples
+ * (x fastest) of the functor at (i / (dims - 1)) - 0.5, filled on the
+ * device (FillSDFVolume, sdf_cache_manager.cuh:15-34). Synchronous;
+ * *index receives the volume's cachedVolumeIndex_. */
+int ap_sdf_cache_add_functor(ap_sdf_cache* cache, int32_t functor, const float info[4], const int32_t dims[3],
+                             uint32_t* index);
+/* AddVolume with caller-computed host samples (same layout). Synchronous. */
+int ap_sdf_cache_add_volume(ap_sdf_cache* cache, const float* host_values, const int32_t dims[3], uint32_t* index);
+/* A volume's dims and (if host_values is not NULL) its samples. */
+int ap_sdf_cache_get_volume(const ap_sdf_cache* cache, uint32_t index, int32_t dims[3], float* host_values);
+/* The SDF cache the scene's SDF_VOLUME objects read (NULL: none, such
+ * objects are never hit). The cache must outlive the scene's use of it. */
+int ap_scene_set_sdf_cache(ap_scene* scene, const ap_sdf_cache* cache);
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char* ap_last_error_string(void);
 

@@ -16,6 +16,11 @@ extern "C" {
 typedef struct or_bvh or_bvh;
 
 or_bvh* or_bvh_build(const ap_scene_object* objs, size_t n, int leaf_size, int bin_count);
+/* The SDF cache's volumes for SDF_VOLUME objects: n volumes, dims[3k..3k+2],
+ * samples concatenated in volume order (or_bvh.cpp; process-wide). */
+void or_sdf_set_volumes(const float* values, const int32_t* dims, size_t n);
+void or_sdf_fill(int functor, const float* info, int nx, int ny, int nz, float* out);
+int or_sdf_hit(const ap_scene_object* objs, int idx, const ap_ray* r, float eps, float* pos, float* normal);
 void or_bvh_free(or_bvh* h);
 size_t or_bvh_node_count(const or_bvh* h);
 void or_bvh_export(const or_bvh* h, ap_bvh_node* nodes, int32_t* indices);

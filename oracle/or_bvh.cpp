@@ -20,6 +20,7 @@
 #include <limits>
 #include <vector>
 
+#include "ap_fmath.h"
 #include "aperture/ap_layout.h"
 #include "or_api.h"
 #include "or_math.h"
@@ -260,14 +261,203 @@ bool IntersectSphereT(float3 o, float3 d, const ap_scene_object& ob, float& t, b
     return true;
 }
 
-// IntersectBase + SceneObjectsIntersector acceptance (intersection.cuh:93-172).
-bool Intersector(const ap_scene_object* objs, int idx, float3 o, float3 d, float tMin, float tMax, float& tOut) {
+// ---- Signed-distance objects (core/object/sdf/) ----
+// Volumes of the SDF cache (SDFCacheManager), registered by or_sdf_set_volumes.
+struct OrVolume {
+    std::vector<float> v;
+    int dim[3];
+    float voxelSizeMax;
+};
+static std::vector<OrVolume> g_volumes;
+
+// SdfTorus (sdf_torus.cuh:15-19).
+static float SdfTorus(float3 u, float R, float r) {
+    const float qx = std::sqrt(u.x * u.x + u.z * u.z) - R;
+    return std::sqrt(qx * qx + u.y * u.y) - r;
+}
+
+// PlaneSDF / SdfDiamondUnit (sdf_diamond.cuh:15-66), sin, cos and atan2 the
+// shared portable ones.
+static float PlaneSDF(float3 p, float3 n, float d) {
+    const float len = std::fmax(std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z), 1e-8f);
+    return (p.x * n.x + p.y * n.y + p.z * n.z - d) / len;
+}
+static float SdfDiamondUnit(float3 u, const float* info) {
+    const float d = (info[0] != 0.0f) ? info[0] : 0.94f;
+    const float bias = (info[1] != 0.0f) ? info[1] : 0.5f;
+    const float tableZ = (info[2] != 0.0f) ? info[2] : 0.3f;
+    const float scale = (info[3] != 0.0f) ? info[3] : 1.0f;
+    const float scaleInv = 1.0f / scale;
+    const float3 p{u.x * scaleInv, u.y * scaleInv, u.z * scaleInv};
+    constexpr float af2 = 4.0f / 3.141592654f;
+    const float s = apfm::atan2(p.y, p.x);
+    const float sf = std::floor(s * af2 + bias) / af2;
+    const float sf2 = std::floor(s * af2) / af2;
+    auto cs = [](float a, float& c, float& sn) { apfm::sincos(a, sn, c); };
+    float csf, ssf, c0, s0, csf1, ssf1, csf2, ssf2, csf8, ssf8;
+    cs(sf, csf, ssf);
+    cs(s, c0, s0);
+    cs(sf + 0.21f, csf1, ssf1);
+    cs(sf - 0.21f, csf2, ssf2);
+    cs(sf2 + 0.393f, csf8, ssf8);
+    float dMax = -1e20f;
+    dMax = std::fmax(dMax, PlaneSDF(p, {csf, ssf, 1.444f}, d));
+    dMax = std::fmax(dMax, PlaneSDF(p, {csf, ssf, -1.072f}, d));
+    dMax = std::fmax(dMax, PlaneSDF(p, {0.0f, 0.0f, 1.0f}, tableZ));
+    dMax = std::fmax(dMax, PlaneSDF(p, {0.0f, 0.0f, -1.0f}, 0.865f));
+    dMax = std::fmax(dMax, PlaneSDF(p, {c0, s0, 0.0f}, 0.911f));
+    dMax = std::fmax(dMax, PlaneSDF(p, {csf1, ssf1, -1.02f}, 0.9193f));
+    dMax = std::fmax(dMax, PlaneSDF(p, {csf2, ssf2, -1.02f}, 0.9193f));
+    dMax = std::fmax(dMax, PlaneSDF(p, {csf2, ssf2, 1.03f}, 0.912f));
+    dMax = std::fmax(dMax, PlaneSDF(p, {csf1, ssf1, 1.03f}, 0.912f));
+    dMax = std::fmax(dMax, PlaneSDF(p, {csf8, ssf8, 2.21f}, 1.131f));
+    return dMax * scale;
+}
+
+static float VolAt(const OrVolume& V, int x, int y, int z) {  // Index3D (sdf_volume.cuh:47-50)
+    return V.v[static_cast<size_t>(z) * V.dim[0] * V.dim[1] + static_cast<size_t>(y) * V.dim[0] + x];
+}
+static int ClampI(int x, int lo, int hi) { return std::max(lo, std::min(hi, x)); }
+
+// SampleSDFAdaptive01 (sdf_volume.cuh:64-153): nearest sample, or the
+// trilinear one within voxelSizeMax of the surface.
+static float SampleSDFAdaptive01(const OrVolume& V, float3 p) {
+    if (V.v.empty()) return 1.0e20f;
+    const float3 g{p.x * static_cast<float>(V.dim[0] - 1), p.y * static_cast<float>(V.dim[1] - 1),
+                   p.z * static_cast<float>(V.dim[2] - 1)};
+    const float nearest = VolAt(V, ClampI(static_cast<int>(std::floor(g.x + 0.5f)), 0, V.dim[0] - 1),
+                                ClampI(static_cast<int>(std::floor(g.y + 0.5f)), 0, V.dim[1] - 1),
+                                ClampI(static_cast<int>(std::floor(g.z + 0.5f)), 0, V.dim[2] - 1));
+    if (std::fabs(nearest) > V.voxelSizeMax) return nearest;
+    int x0 = static_cast<int>(std::floor(g.x)), y0 = static_cast<int>(std::floor(g.y));
+    int z0 = static_cast<int>(std::floor(g.z));
+    const int x1 = ClampI(x0 + 1, 0, V.dim[0] - 1), y1 = ClampI(y0 + 1, 0, V.dim[1] - 1);
+    const int z1 = ClampI(z0 + 1, 0, V.dim[2] - 1);
+    x0 = ClampI(x0, 0, V.dim[0] - 1);
+    y0 = ClampI(y0, 0, V.dim[1] - 1);
+    z0 = ClampI(z0, 0, V.dim[2] - 1);
+    const float fx = std::fmin(std::fmax(g.x - static_cast<float>(x0), 0.0f), 1.0f);
+    const float fy = std::fmin(std::fmax(g.y - static_cast<float>(y0), 0.0f), 1.0f);
+    const float fz = std::fmin(std::fmax(g.z - static_cast<float>(z0), 0.0f), 1.0f);
+    const float c00 = VolAt(V, x0, y0, z0) * (1.0f - fx) + VolAt(V, x1, y0, z0) * fx;
+    const float c10 = VolAt(V, x0, y1, z0) * (1.0f - fx) + VolAt(V, x1, y1, z0) * fx;
+    const float c01 = VolAt(V, x0, y0, z1) * (1.0f - fx) + VolAt(V, x1, y0, z1) * fx;
+    const float c11 = VolAt(V, x0, y1, z1) * (1.0f - fx) + VolAt(V, x1, y1, z1) * fx;
+    const float c0 = c00 * (1.0f - fy) + c10 * fy;
+    const float c1 = c01 * (1.0f - fy) + c11 * fy;
+    return c0 * (1.0f - fz) + c1 * fz;
+}
+
+// The object's distance functor (sdf_master.cuh:32-70): nullptr when the
+// object is never hit (undefined type, missing volume).
+struct SDFFn {
+    int type;
+    const OrVolume* vol;
+    float R, r;
+    float operator()(float3 u) const {
+        if (type == 0) return SdfTorus(u, R, r);
+        return SampleSDFAdaptive01(*vol, u + 0.5f);  // SDFVolumeFunctor (sdf_volume.cuh:155-171)
+    }
+};
+static bool MakeSDFFn(const ap_scene_object& ob, SDFFn& f) {
+    f.type = ob.additional.sdf_type;
+    f.vol = nullptr;
+    f.R = ob.additional.sdf_info.sdf_info0.x;
+    f.r = ob.additional.sdf_info.sdf_info0.y;
+    if (f.type == 0) return true;
+    const int k = ob.additional.sdf_info.cached_volume_index;
+    if (f.type == 1 && k >= 0 && k < static_cast<int>(g_volumes.size()) && !g_volumes[k].v.empty()) {
+        f.vol = &g_volumes[k];
+        return true;
+    }
+    return false;
+}
+
+static float3 ToUnitSpace(float3 p, float3 e) {  // sdf_base.cuh:12-21
+    return {p.x / (2.0f * std::fmax(e.x, 1e-8f)), p.y / (2.0f * std::fmax(e.y, 1e-8f)),
+            p.z / (2.0f * std::fmax(e.z, 1e-8f))};
+}
+
+// RaymarchLocal (sdf_base.cuh:54-110).
+static bool RaymarchLocal(const SDFFn& sdf, float3 lo, float3 ld, float tEnter, float tExit, float3 extent,
+                          float currentEPS, float* outT, float3* outP) {
+    const float minStep = currentEPS * 0.5f;
+    const bool bStartInside = sdf(ToUnitSpace(lo, extent)) <= 0.0f;
+    float t = std::fmax(0.0f, tEnter);
+    float tLast = t;
+    const float sMin = 2.0f * std::fmin(extent.x, std::fmin(extent.y, extent.z));
+    for (int i = 0; i < 512; ++i) {
+        tLast = t;
+        const float3 pL = lo + ld * t;
+        const float dLocal = sdf(ToUnitSpace(pL, extent)) * sMin;
+        const float realDistance = std::fabs(dLocal);
+        if (bStartInside ? (dLocal > 0 && dLocal < currentEPS) : (dLocal < 0 && -dLocal < currentEPS)) {
+            *outT = tLast;
+            *outP = pL;
+            return true;
+        }
+        const float advance = std::fmax(sdf.type == 1 ? realDistance * 0.9f : realDistance, minStep);
+        t += advance;
+        if (t > tExit) break;
+    }
+    return false;
+}
+
+// IntersectSDFObject (sdf_base.cuh:115-163): distance and local hit point.
+bool IntersectSDFT(float3 o, float3 d, const ap_scene_object& ob, float eps, float& t, float3* localPos) {
+    SDFFn f;
+    if (!MakeSDFFn(ob, f)) return false;
+    const float3x3 w2o = M(ob.world_to_object);
+    const float3 lo = w2o * (o - F(ob.center));
+    const float3 ld = normalize(w2o * d);
+    const float3 div = invSafe(ld);
+    const float3 t1 = (-F(ob.extent) - lo) * div;
+    const float3 t2 = (F(ob.extent) - lo) * div;
+    const float3 mn = fmin3(t1, t2), mx = fmax3(t1, t2);
+    const float tE = std::fmax(std::fmax(mn.x, mn.y), mn.z);
+    const float tX = std::fmin(std::fmin(mx.x, mx.y), mx.z);
+    if (tE > tX) return false;
+    if (tX <= 0.0f) return false;
+    float3 lp{0, 0, 0};
+    if (!RaymarchLocal(f, lo, ld, tE, tX, F(ob.extent), eps, &t, &lp)) return false;
+    if (localPos) *localPos = lp;
+    return true;
+}
+
+// sdfNormalUnit + UnitNormalToLocal (sdf_base.cuh:23-52).
+float3 SDFLocalNormal(const ap_scene_object& ob, float3 lp, float eps) {
+    SDFFn f;
+    (void)MakeSDFFn(ob, f);
+    const float3 e = F(ob.extent);
+    const float sMin = 2.0f * std::fmin(e.x, std::fmin(e.y, e.z));
+    const float h = eps / std::fmax(sMin, 1e-8f);
+    const float3 u = ToUnitSpace(lp, e);
+    const float dx = f({u.x + h, u.y, u.z}) - f({u.x - h, u.y, u.z});
+    const float dy = f({u.x, u.y + h, u.z}) - f({u.x, u.y - h, u.z});
+    const float dz = f({u.x, u.y, u.z + h}) - f({u.x, u.y, u.z - h});
+    const float3 nu = normalize(float3{dx, dy, dz});
+    return normalize(float3{nu.x / (2.0f * std::fmax(e.x, 1e-8f)), nu.y / (2.0f * std::fmax(e.y, 1e-8f)),
+                            nu.z / (2.0f * std::fmax(e.z, 1e-8f))});
+}
+
+// OverlapSDFObject (sdf_base.cuh:165-177).
+bool OverlapSDF(const ap_scene_object& ob, float3 p) {
+    SDFFn f;
+    if (!MakeSDFFn(ob, f)) return false;
+    return f(ToUnitSpace(M(ob.world_to_object) * (p - F(ob.center)), F(ob.extent))) <= 0.0f;
+}
+
+// IntersectBase + SceneObjectsIntersector acceptance (intersection.cuh:93-172);
+// sdfEps = SDF_EPS of the ray's depth.
+bool Intersector(const ap_scene_object* objs, int idx, float3 o, float3 d, float tMin, float tMax, float& tOut,
+                 float sdfEps) {
     const ap_scene_object& ob = objs[idx];
     float t = 0.0f;
     bool hit = false, inside = false;
     float u, v;
     if (ob.type == AP_OBJ_CUBE) hit = IntersectAABBT(o, d, ob, t, inside);
     else if (ob.type == AP_OBJ_SPHERE) hit = IntersectSphereT(o, d, ob, t, inside);
+    else if (ob.type == AP_OBJ_SDF) hit = IntersectSDFT(o, d, ob, sdfEps, t, nullptr);
     else if (ob.type == AP_OBJ_TRIANGLE) hit = IntersectTriangleT(o, d, ob, t, u, v);
     if (!hit) return false;
     if (t < tMin || t >= tMax) return false;
@@ -282,7 +472,7 @@ struct TraceResult { int id; float t; unsigned nodes; unsigned prims; };
 // box tests (the core's CULLED mode), for its visit-order accounting.
 TraceResult TraverseBVH(bool anyHit, const ap_scene_object* objs, const std::vector<BVHNode>& nodes,
                         const std::vector<int>& objectIndices, float3 o, float3 d, float tMin, float tMax,
-                        bool cull, float margin) {
+                        bool cull, float margin, float sdfEps) {
     TraceResult r{-1, std::numeric_limits<float>::infinity(), 0u, 0u};
     if (nodes.empty()) return r;
     r.t = tMax;
@@ -303,7 +493,7 @@ TraceResult TraverseBVH(bool anyHit, const ap_scene_object* objs, const std::vec
                 const int objectIndex = objectIndices[node.first_ + i];
                 r.prims++;
                 float t;
-                if (Intersector(objs, objectIndex, o, d, tMin, r.t, t)) {
+                if (Intersector(objs, objectIndex, o, d, tMin, r.t, t, sdfEps)) {
                     r.t = t;
                     r.id = objectIndex;
                     if (anyHit) return r;
@@ -334,6 +524,11 @@ TraceResult TraverseBVH(bool anyHit, const ap_scene_object* objs, const std::vec
 
 using namespace orc;
 
+// The SDF marching epsilon of a standalone ray, as the core's traversal entry
+// points take it: tmin / 2 for closest hit (tmin = INTERSECT_EPS of the
+// depth), SDF_EPS(0) for any hit or tmin 0.
+float SdfEpsOfRay(float tmin, bool anyHit) { return (!anyHit && tmin > 0.0f) ? tmin * 0.5f : 0.5f * 1.0e-3f; }
+
 struct or_bvh {
     const ap_scene_object* objs;  // borrowed
     size_t count;
@@ -354,6 +549,41 @@ or_bvh* or_bvh_build(const ap_scene_object* objs, size_t n, int leaf_size, int b
 }
 
 void or_bvh_free(or_bvh* h) { delete h; }
+
+// FillSDFVolume (sdf_cache_manager.cuh:15-34) with the torus (functor 0) or
+// diamond (1) functor: out[z][y][x] at (i / (n - 1)) - 0.5.
+void or_sdf_fill(int functor, const float* info, int nx, int ny, int nz, float* out) {
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const float fx = nx > 1 ? static_cast<float>(x) / static_cast<float>(nx - 1) : 0.0f;
+                const float fy = ny > 1 ? static_cast<float>(y) / static_cast<float>(ny - 1) : 0.0f;
+                const float fz = nz > 1 ? static_cast<float>(z) / static_cast<float>(nz - 1) : 0.0f;
+                const float3 u{fx - 0.5f, fy - 0.5f, fz - 0.5f};
+                out[static_cast<size_t>(z) * nx * ny + static_cast<size_t>(y) * nx + x] =
+                    functor == 0 ? SdfTorus(u, info[0], info[1]) : SdfDiamondUnit(u, info);
+            }
+}
+
+// The SDF cache's volumes (n volumes, dims[3k..3k+2], samples concatenated in
+// volume order), replacing any registered before.
+void or_sdf_set_volumes(const float* values, const int32_t* dims, size_t n) {
+    g_volumes.clear();
+    size_t off = 0;
+    for (size_t k = 0; k < n; ++k) {
+        OrVolume V;
+        for (int a = 0; a < 3; ++a) V.dim[a] = dims[3 * k + a];
+        const size_t m = static_cast<size_t>(V.dim[0]) * V.dim[1] * V.dim[2];
+        V.v.assign(values + off, values + off + m);
+        off += m;
+        // voxelSizeMax_ = |1 / max(dim, 1)| x SDF_VOLUME_USE_NEAREST_SCALE (sdf_volume.cuh:24-44).
+        const float3 inv{1.0f / std::fmax(static_cast<float>(V.dim[0]), 1.0f),
+                         1.0f / std::fmax(static_cast<float>(V.dim[1]), 1.0f),
+                         1.0f / std::fmax(static_cast<float>(V.dim[2]), 1.0f)};
+        V.voxelSizeMax = length(inv) * 2.5f;
+        g_volumes.push_back(std::move(V));
+    }
+}
 
 size_t or_bvh_node_count(const or_bvh* h) { return h->bvh.nodes_.size(); }
 
@@ -377,7 +607,7 @@ void or_trace(const or_bvh* h, const ap_ray* rays, size_t n, int any_hit, int cu
         const ap_ray& r = rays[i];
         const TraceResult t = TraverseBVH(any_hit != 0, h->objs, h->bvh.nodes_, h->bvh.objectIndices_,
                                           {r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}, r.tmin, r.tmax, cull != 0,
-                                          margin);
+                                          margin, SdfEpsOfRay(r.tmin, any_hit != 0));
         ids[i] = t.id;
         ts[i] = t.t;
         if (nodes) nodes[i] = t.nodes;
@@ -397,7 +627,8 @@ void or_brute_closest(const ap_scene_object* objs, size_t count, const ap_ray* r
         float best = r.tmax;
         for (size_t k = 0; k < count; ++k) {
             float t;
-            if (Intersector(objs, static_cast<int>(k), {r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}, r.tmin, best, t)) {
+            if (Intersector(objs, static_cast<int>(k), {r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}, r.tmin, best, t,
+                            SdfEpsOfRay(r.tmin, false))) {
                 best = t;
                 id = static_cast<int>(k);
             }
@@ -407,11 +638,28 @@ void or_brute_closest(const ap_scene_object* objs, size_t count, const ap_ray* r
     }
 }
 
+// World hit point and normal of an SDF object along a ray (IntersectSDFObject,
+// sdf_base.cuh:138-162) with marching epsilon eps; false on a miss.
+int or_sdf_hit(const ap_scene_object* objs, int idx, const ap_ray* r, float eps, float* pos, float* normal) {
+    const ap_scene_object& ob = objs[idx];
+    float t;
+    float3 lp;
+    const float3 o{r->ox, r->oy, r->oz}, d{r->dx, r->dy, r->dz};
+    if (ob.type != AP_OBJ_SDF || !IntersectSDFT(o, d, ob, eps, t, &lp)) return 0;
+    const float3 p = M(ob.object_to_world) * lp + F(ob.center);
+    const float3 n = normalize(M(ob.object_to_world) * SDFLocalNormal(ob, lp, eps));
+    pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
+    normal[0] = n.x; normal[1] = n.y; normal[2] = n.z;
+    return 1;
+}
+
 // Distance of one object along one ray, or NaN when the object is missed or
 // outside [tmin, tmax).
 float or_object_t(const ap_scene_object* objs, int idx, const ap_ray* r) {
     float t;
-    if (Intersector(objs, idx, {r->ox, r->oy, r->oz}, {r->dx, r->dy, r->dz}, r->tmin, r->tmax, t)) return t;
+    if (Intersector(objs, idx, {r->ox, r->oy, r->oz}, {r->dx, r->dy, r->dz}, r->tmin, r->tmax, t,
+                    SdfEpsOfRay(r->tmin, false)))
+        return t;
     return std::numeric_limits<float>::quiet_NaN();
 }
 
@@ -420,9 +668,9 @@ float or_object_t(const ap_scene_object* objs, int idx, const ap_ray* r) {
 namespace orc {
 // One traversal for the integrator restatement (or_render.cpp).
 void TraceOne(const or_bvh* h, float3 o, float3 d, float tMin, float tMax, bool anyHit, bool cull, float margin,
-              int* id, float* t) {
+              int* id, float* t, float sdfEps) {
     const TraceResult r =
-        TraverseBVH(anyHit, h->objs, h->bvh.nodes_, h->bvh.objectIndices_, o, d, tMin, tMax, cull, margin);
+        TraverseBVH(anyHit, h->objs, h->bvh.nodes_, h->bvh.objectIndices_, o, d, tMin, tMax, cull, margin, sdfEps);
     *id = r.id;
     *t = r.t;
 }

@@ -346,7 +346,12 @@ static SampleCtx SampleBSDF(const float lb[4], Rng& rng, int model, float3 rayDi
 }
 
 // ---- intersection context (intersection.cuh, aabb.cuh, sphere.cuh) ----
-static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t) {
+extern bool IntersectSDFT(float3 o, float3 d, const ap_scene_object& ob, float eps, float& t, float3* localPos);
+extern float3 SDFLocalNormal(const ap_scene_object& ob, float3 lp, float eps);
+extern bool OverlapSDF(const ap_scene_object& ob, float3 p);
+
+// sdfEps: SDF_EPS of the ray's depth (IntersectSDFObject's normal step).
+static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t, float sdfEps) {
     Ctx c;
     const ap_scene_object& ob = S.objs[idx];
     c.hit = true;
@@ -373,6 +378,15 @@ static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t) {
     const float3 lo = w2o * (o - F3(ob.center));
     const float3 ld = normalize(w2o * d);
     const float3 ext = F3(ob.extent);
+    if (ob.type == AP_OBJ_SDF) {  // IntersectSDFObject (sdf_base.cuh:138-162)
+        float th = t;
+        float3 lp = lo + ld * t;
+        (void)IntersectSDFT(o, d, ob, sdfEps, th, &lp);
+        const float3 ln = SDFLocalNormal(ob, lp, sdfEps);
+        c.pos = o2w * lp + F3(ob.center);
+        c.normal = normalize(o2w * ln);
+        return c;
+    }
     if (ob.type == AP_OBJ_CUBE) {
         const float3 div = invSafe(ld);
         const float3 t1 = (-ext - lo) * div, t2 = (ext - lo) * div;
@@ -405,7 +419,7 @@ static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t) {
 }
 
 extern void TraceOne(const or_bvh* h, float3 o, float3 d, float tMin, float tMax, bool anyHit, bool cull,
-                     float margin, int* id, float* t);
+                     float margin, int* id, float* t, float sdfEps);
 
 // ---- NEE (nee.cuh) ----
 struct LightSample { float3 point, normal; float pFacet, pA; int object; bool valid; };
@@ -717,7 +731,8 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         const float eps = INTERSECT_EPS(ndd);
         int id;
         float t;
-        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t);
+        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t,
+                 eps * 0.5f);
         if (depth == 0 && primary) *primary = id;
         if (depth == 0 && gs) {  // RecordScreenGBuffer's inputs (render.cuh:640-643)
             *gs = or_gsample{};
@@ -737,7 +752,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             if (guiding) col.Endpoint(endpoint, ndd);
             break;
         }
-        Ctx c = HitContext(S, id, ro, rd, t);
+        Ctx c = HitContext(S, id, ro, rd, t, eps * 0.5f);
         const int model = S.objs[id].material.shading_model;
         if (depth == 0 && gs) {
             gs->albedo[0] = c.gb.albedo.x; gs->albedo[1] = c.gb.albedo.y; gs->albedo[2] = c.gb.albedo.z;
@@ -819,7 +834,8 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                         if (tMax > 0.0f) {
                             int sid;
                             float st;
-                            TraceOne(S.bvh, c.pos, sd, 0.0f, tMax, true, S.margin > 0.0f, S.margin, &sid, &st);
+                            TraceOne(S.bvh, c.pos, sd, 0.0f, tMax, true, S.margin > 0.0f, S.margin, &sid, &st,
+                                     INTERSECT_EPS(depth) * 0.5f);
                             occluded = sid >= 0;
                         }
                     }
@@ -944,7 +960,8 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         const float eps = INTERSECT_EPS(ndd);
         int id;
         float t;
-        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t);
+        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t,
+                 eps * 0.5f);
         if (id < 0) {
             (void)rng.rand1();
             if (!inside) {
@@ -955,7 +972,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             }
             break;
         }
-        Ctx c = HitContext(S, id, ro, rd, t);
+        Ctx c = HitContext(S, id, ro, rd, t, eps * 0.5f);
         const float3 pos0 = c.pos;
         const int model = S.objs[id].material.shading_model;
         SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
@@ -1030,7 +1047,8 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
                         if (tMax > 0.0f) {
                             int sid;
                             float st;
-                            TraceOne(S.bvh, c.pos, sd, 0.0f, tMax, true, S.margin > 0.0f, S.margin, &sid, &st);
+                            TraceOne(S.bvh, c.pos, sd, 0.0f, tMax, true, S.margin > 0.0f, S.margin, &sid, &st,
+                                     INTERSECT_EPS(depth) * 0.5f);
                             occluded = sid >= 0;
                         }
                     }
@@ -1165,6 +1183,10 @@ static Setup MakeSetup(const or_bvh* bvh, const ap_scene_object* objs, size_t n,
     U.up = normalize(cross(U.camD, U.right));
     for (size_t i = 0; i < n && !U.inside0; ++i) {
         const ap_scene_object& ob = objs[i];
+        if (ob.type == AP_OBJ_SDF) {
+            U.inside0 = OverlapSDF(ob, U.camO);
+            continue;
+        }
         if (ob.type != AP_OBJ_CUBE && ob.type != AP_OBJ_SPHERE) continue;
         const float3 lo = M3(ob.world_to_object) * (U.camO - F3(ob.center));
         const float3 e = F3(ob.extent);

@@ -20,6 +20,10 @@ def lib():
         vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
         L.or_bvh_build.restype = vp
         L.or_bvh_build.argtypes = [vp, sz, i32, i32]
+        L.or_sdf_set_volumes.argtypes = [vp, vp, sz]
+        L.or_sdf_fill.argtypes = [i32, vp, i32, i32, i32, vp]
+        L.or_sdf_hit.restype = i32
+        L.or_sdf_hit.argtypes = [vp, i32, vp, ctypes.c_float, vp, vp]
         L.or_bvh_free.argtypes = [vp]
         L.or_bvh_node_count.restype = sz
         L.or_bvh_node_count.argtypes = [vp]
@@ -524,3 +528,28 @@ def tonemap(hdr=None, gbuffer=None, statistics=None, n=None, tone=0, display=0, 
     lib().or_tonemap(_p(h) if h is not None else None, _p(gbuffer) if gbuffer is not None else None,
                      _p(statistics) if statistics is not None else None, _p(out), n, tone, display, exposure)
     return out
+
+
+def sdf_set_volumes(volumes):
+    """Registers the SDF cache's volumes with the oracle: a list of host (nz, ny, nx) arrays."""
+    vals = np.concatenate([np.ascontiguousarray(v, np.float32).reshape(-1) for v in volumes]) if volumes else \
+        np.zeros(1, np.float32)
+    dims = np.array([[v.shape[2], v.shape[1], v.shape[0]] for v in volumes], np.int32).reshape(-1) if volumes else \
+        np.zeros(3, np.int32)
+    lib().or_sdf_set_volumes(_p(vals), _p(dims), len(volumes))
+
+
+def sdf_fill(functor, info, dims):
+    """FillSDFVolume restated in numpy-free C: the oracle's torus/diamond functor grid (nz, ny, nx)."""
+    info = np.ascontiguousarray(info, np.float32).reshape(4)
+    out = np.zeros((dims[2], dims[1], dims[0]), np.float32)
+    lib().or_sdf_fill(functor, _p(info), int(dims[0]), int(dims[1]), int(dims[2]), _p(out))
+    return out
+
+
+def sdf_hit(objs, idx, ray, eps=5e-4):
+    """World hit point and normal of SDF object idx along one ray, or None."""
+    pos, nrm = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    r = _rays(np.asarray(ray).reshape(1))
+    ok = lib().or_sdf_hit(_p(_objs(objs)), int(idx), _p(r), eps, _p(pos), _p(nrm))
+    return (pos, nrm) if ok else None

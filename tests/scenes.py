@@ -90,6 +90,32 @@ def spheres(centers, radii):
     return update_proxy(o)
 
 
+def sdfs(centers, extents, sdf_type=0, info=(0.375, 0.125, 0.0, 0.0), volume=-1, rotations=None):
+    """SDF SceneObjects: torus (sdf_type 0, info.xy = R, r in unit space) or
+    volume (sdf_type 1, volume = cachedVolumeIndex_), optionally rotated
+    (rotations: object-to-world row matrices; world-to-object is the
+    transpose). The proxy is the box of the 8 transformed corners."""
+    centers = np.asarray(centers, np.float32).reshape(-1, 3)
+    n = len(centers)
+    o = blank(n)
+    o["type"] = ap.OBJ_SDF
+    o["center"] = centers
+    o["extent"] = np.abs(np.broadcast_to(np.asarray(extents, np.float32), centers.shape))
+    o["sdf_type"] = sdf_type
+    o["sdf_info0"] = np.broadcast_to(np.asarray(info, np.float32), (n, 4))
+    o["cached_volume_index"] = volume
+    rot = np.broadcast_to(np.eye(3, dtype=np.float32), (n, 3, 3)) if rotations is None else \
+        np.asarray(rotations, np.float32).reshape(n, 3, 3)
+    o["object_to_world"] = rot
+    o["world_to_object"] = np.transpose(rot, (0, 2, 1))
+    signs = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float32)
+    corners = centers[:, None, :] + np.einsum("nij,nkj->nki", rot, signs[None] * o["extent"][:, None, :])
+    o["proxy_min"] = corners.min(1)
+    o["proxy_max"] = corners.max(1)
+    o["proxy_center"] = centers
+    return o
+
+
 def quad_grid(n, z=0.0, size=1.0):
     """n x n axis-aligned quads (2 triangles each) on the plane z, spanning
     [0,size]^2: every interior edge and vertex is shared."""

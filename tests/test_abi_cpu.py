@@ -64,8 +64,8 @@ def test_errors_are_codes_not_aborts():
     assert lib.ap_scene_create(None, 0, None, None) == -1
     assert b"out_scene" in lib.ap_last_error_string()
     assert lib.ap_trace_closest(None, None, 1, None, None) == -1
-    sdf = np.zeros(1, dtype=ap.SCENE_OBJECT_DTYPE)
-    sdf["type"] = ap.OBJ_SDF
+    bad = np.zeros(1, dtype=ap.SCENE_OBJECT_DTYPE)
+    bad["type"] = 7
     h = ctypes.c_void_p()
-    assert lib.ap_scene_create(sdf.ctypes.data, 1, None, ctypes.byref(h)) == -4
-    assert b"SDF" in lib.ap_last_error_string()
+    assert lib.ap_scene_create(bad.ctypes.data, 1, None, ctypes.byref(h)) == -1
+    assert b"unknown object type" in lib.ap_last_error_string()
