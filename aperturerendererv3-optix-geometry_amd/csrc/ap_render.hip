@@ -36,6 +36,7 @@
 #include "ap_guide.h"
 #include "ap_display.h"
 #include "ap_sdf.h"
+#include "ap_prim.h"
 
 namespace ap {
 
@@ -55,6 +56,7 @@ struct MatIn {
     f3 albedo, emissive;
     float specular, metallic, roughness;
     int32_t model, light_index;
+    float ior, sigma_a, abbe, tilt;
 };
 
 // A path carries NL wavelengths: NL = 1 is the reference's one wavelength
@@ -79,7 +81,20 @@ static_assert(sizeof(PathT<1>) == 64, "PathT<1> is 64 B");
 static_assert(sizeof(PathT<4>) == 112, "PathT<4> is 112 B");
 constexpr size_t kPathBytes = sizeof(PathT<4>);
 
-enum : uint32_t { PATH_PREV_NEE = 1u, PATH_INSIDE = 2u };
+// flags: PATH_* bits, the MediaContext depth in bits 4..15 (hits since the
+// path last left a medium by a miss, saturating), a batch path's scene in
+// bits 16..31.
+enum : uint32_t { PATH_PREV_NEE = 1u, PATH_INSIDE = 2u, PATH_HERO_ONLY = 4u };
+constexpr uint32_t kMediaDepthShift = 4u, kMediaDepthMax = 0xfffu;
+
+// The rest of a path's MediaContext (material.cuh:12-18), read only while
+// the path is inside a medium: the vertex the ray left (Beer-Lambert
+// distance) and the medium's object, whose surface alone the next ray is
+// traced against (-1: every object, for a camera that starts inside).
+struct alignas(16) MediaRec {
+    float vertex[3];
+    int32_t object;
+};
 
 template <int NL>
 struct alignas(16) ShadowPay {
@@ -155,9 +170,11 @@ struct RenderState {
     int32_t n_lights = 0;
     int32_t n_mats = 0;
     std::vector<int32_t> analytic_objs;   // cube, sphere and SDF objects (the camera-inside test)
+    bool has_glass = false;               // some object's shading model is MAT_GLASS
     // path buffers
     size_t capacity = 0;
     void* paths = nullptr;            // PathT<NL>, kPathBytes per entry
+    MediaRec* media = nullptr;        // per path
     ap_ray* rays[2] = {nullptr, nullptr};
     int32_t* path_of[2] = {nullptr, nullptr};
     ap_hit* hits = nullptr;
@@ -191,7 +208,7 @@ constexpr int kEpsEntries = 1024;
 
 void render_state_free(RenderState* r) {
     if (!r) return;
-    void* ptrs[] = {r->geo, r->mats, r->lights, r->paths, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
+    void* ptrs[] = {r->geo, r->mats, r->lights, r->paths, r->media, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
                     r->hits, r->shadow_rays, r->shadow_payload, r->shadow_hits, r->counters, r->eps,
                     r->target_tmp, r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_qrgb, r->nrc_verts,
                     r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters, r->gsamples};
@@ -392,7 +409,8 @@ __device__ __forceinline__ float mean_mask(const float* m) {
 template <int NL>
 __global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FrameParams fp, const float* __restrict__ eps,
                                                           PathT<NL>* __restrict__ paths, ap_ray* __restrict__ rays,
-                                                          int32_t* __restrict__ path_of, GuideRec* __restrict__ recs) {
+                                                          int32_t* __restrict__ path_of, GuideRec* __restrict__ recs,
+                                                          MediaRec* __restrict__ media) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t total = static_cast<uint32_t>(fp.n_pixels) * static_cast<uint32_t>(fp.frames);
     if (i >= total) return;
@@ -424,6 +442,9 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FramePa
     s.last_pos[0] = s.last_pos[1] = s.last_pos[2] = 0.0f;
     s.ndd = 0;
     s.flags = fp.inside ? PATH_INSIDE : 0u;
+    // A camera inside an object starts inside a medium with no object of
+    // its own and the camera as the vertex (render.cuh:535-538).
+    if (fp.inside) media[i] = MediaRec{{cam.origin.x, cam.origin.y, cam.origin.z}, -1};
     s.pixel = p;
     s.frame = frame;
     s.rng_state = rng.state;
@@ -452,6 +473,7 @@ struct ShadeParams {
     GSample* gsample = nullptr;   // depth 0: each path's first-hit G-buffer sample, or none
     const SdfVolume* volumes = nullptr;  // the scene's SDF volume table
     int32_t n_volumes = 0;
+    MediaRec* media = nullptr;    // per path
 };
 
 // Hit attributes as the reference intersectors report them.
@@ -536,6 +558,33 @@ __device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict
     const f3 ln = normalize(lp) * (inside ? -1.0f : 1.0f);
     h.pos = mul(o2w, lp) + c;
     h.normal = normalize(mul(o2w, ln));
+    return h;
+}
+
+// Intersect with a solo object (intersection.cuh:176-193): the ray against
+// one object alone, accepted in [tmin, inf) as SceneObjectsIntersector does.
+__device__ ap_hit solo_hit(const GeoRec& g, int32_t object, const AnalyticRec* __restrict__ an,
+                           const SdfVolume* __restrict__ vols, int32_t n_vols, f3 o, f3 d, float tmin) {
+    float t = 0.0f;
+    bool ok = false;
+    if (g.type == AP_OBJ_TRIANGLE) {
+        float u, v;
+        ok = tri_mt(o, d, {g.p0[0], g.p0[1], g.p0[2]}, {g.e1[0], g.e1[1], g.e1[2]}, {g.e2[0], g.e2[1], g.e2[2]}, t,
+                    u, v);
+    } else if (g.type == AP_OBJ_CUBE) {
+        ok = cube_t(o, d, an[g.analytic], t);
+    } else if (g.type == AP_OBJ_SPHERE) {
+        ok = sphere_t(o, d, an[g.analytic], t);
+    } else if (g.type == AP_OBJ_SDF) {
+        ok = sdf_t(o, d, an[g.analytic], vols, n_vols, tmin > 0.0f ? tmin * 0.5f : kSdfEps0, t);
+    }
+    ap_hit h;
+    h.id = -1;
+    h.t = __int_as_float(0x7f800000);
+    if (ok && !(t < tmin || t >= __int_as_float(0x7f800000))) {
+        h.id = object;
+        h.t = t;
+    }
     return h;
 }
 
@@ -647,7 +696,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     PathT<NL> S = paths[pid];
     Pcg32 rng{S.rng_state, S.rng_inc};
     const ap_ray R = rays_in[i];
-    const ap_hit H = hits[i];
+    ap_hit H = hits[i];
     const f3 o{R.ox, R.oy, R.oz}, d{R.dx, R.dy, R.dz};
     const Spec<NL> sc = spec_of<NL>(S.lam, lam_lut, tables);
     const f4 lb = sc.lb[0];    // the hero's basis: every sampling decision uses it
@@ -667,7 +716,14 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         volumes = B.volumes;
         n_volumes = B.n_volumes;
     }
+    // The MediaContext the ray was traced in.
     const bool inside = (S.flags & PATH_INSIDE) != 0u;
+    const uint32_t media_depth = (S.flags >> kMediaDepthShift) & kMediaDepthMax;
+    MediaRec md{{0.0f, 0.0f, 0.0f}, -1};
+    if (inside) md = sp.media[pid];
+    // Trace (render.cuh:89-92): inside a medium with an object of its own the
+    // ray meets only that object; the traversal skipped such a ray.
+    if (inside && md.object >= 0) H = solo_hit(geo[md.object], md.object, analytic, volumes, n_volumes, o, d, R.tmin);
     if (sp.depth == 0 && primary_id && S.frame == primary_frame) primary_id[S.pixel] = H.id;
 
     bool emit_next = false, emit_shadow = false, emit_query = false;
@@ -685,17 +741,46 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             sp.gsample[pid] = g;
         }
         (void)rand1(rng);
-        float le_end[NL];
-        for (int j = 0; j < NL; ++j) le_end[j] = 0.0f;
         if (!inside) {
+            float le_end[NL];
             for (int j = 0; j < NL; ++j) {
                 const float le = sp.sky ? interact(rgb_lut, sc.lb[j], sky_lighting(d)) : 0.0f;
                 S.radiance[j] = S.radiance[j] + S.mask[j] * le * sc.d65[j];
                 le_end[j] = le * sc.d65[j];
             }
+            if (GUIDE) pg_endpoint<NL>(gv.recs[pid], le_end, S.ndd);
+            S.flags &= ~PATH_PREV_NEE;
+        } else {
+            // A miss inside a medium (bMissThinSurfaceInsideMedia_, render.cuh:109-155):
+            // no emission, NEE or MIS update, weight 1. The reference's hit
+            // record stays zero-initialised, so the path goes on from the
+            // world origin in the same direction, outside any medium; Trace's
+            // miss branch clears the delta flag, so the vertex counts.
+            float p_rr = 1.0f;
+            bool survived = true;
+            if (S.ndd >= sp.rr_min_depth) {
+                const float mm = mean_mask<NL>(S.mask);
+                const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
+                p_rr = lerpf(basic, 1.0f, 0.0f);
+                if (rand1(rng) > p_rr) survived = false;
+            }
+            if (survived) {
+                float contrib[NL];
+                for (int j = 0; j < NL; ++j) contrib[j] = 1.0f / fmaxf(1.0f, 1e-12f);
+                if (GUIDE) pg_advance<NL>(gv.recs[pid], d, contrib, p_rr, S.ndd, false, d, f3{0.0f, 0.0f, 0.0f},
+                                          f3{0.0f, 0.0f, 0.0f});
+                for (int j = 0; j < NL; ++j) {
+                    S.mask[j] = S.mask[j] * contrib[j];
+                    S.mask[j] = S.mask[j] / p_rr;
+                }
+                S.flags &= ~(PATH_INSIDE | (kMediaDepthMax << kMediaDepthShift));
+                S.ndd += 1;
+                if (sp.depth + 1 < sp.max_depth) {
+                    emit_next = true;
+                    next = ap_ray{0.0f, 0.0f, 0.0f, eps[S.ndd], d.x, d.y, d.z, __int_as_float(0x7f800000)};
+                }
+            }
         }
-        if (GUIDE) pg_endpoint<NL>(gv.recs[pid], le_end, S.ndd);
-        S.flags &= ~PATH_PREV_NEE;
     } else {
         const GeoRec g = geo[H.id];
         // SDF_EPS of the extension ray's depth: INTERSECT_EPS(ndd) / 2.
@@ -716,12 +801,38 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             g.pad = 0;
             sp.gsample[pid] = g;
         }
-        // SampleBSDF (material.cuh:177-330) for DIFFUSE / LIGHT / GGX.
+        // SampleBSDF (material.cuh:177-330).
         f3 next_dir{0.0f, 0.0f, 0.0f};
         float pdf = 1.0f;
         bool terminate = false;
         bool pure_light = false;
-        if (M.shading_model == SM_DIFFUSE) {
+        bool delta = false;
+        bool inside_next = inside;   // the MediaContext after this hit
+        if (M.shading_model == SM_GLASS) {
+            // MAT_GLASS (material.cuh:221-272): the hero's IOR decides.
+            delta = true;
+            const Interface f = glass_interface(ior_at(S.lam[0], M.ior, M.abbe), inside, d, N);
+            if (f.k <= 0.0f) {  // total internal reflection
+                pdf = 1.0f;
+                if (media_depth < static_cast<uint32_t>(kMaxDeltaDepth)) {
+                    next_dir = reflect(d, f.n_in);
+                } else {        // forced out
+                    next_dir = d;
+                    inside_next = !inside;
+                }
+            } else {
+                const float cosT = sqrtf(f.k);
+                const float Fr = fresnel_dielectric(f.cosI, cosT, f.eta);
+                if (rand1(rng) < Fr) {
+                    next_dir = reflect(d, f.n_in);
+                    pdf = Fr;
+                } else {
+                    next_dir = refract_dir(d, f.n_in, f.eta);
+                    pdf = 1.0f - Fr;
+                    inside_next = !inside;
+                }
+            }
+        } else if (M.shading_model == SM_DIFFUSE) {
             const float e0 = rand1(rng), e1 = rand1(rng), e2 = rand1(rng);
             const DirSample ds = cosine_hemisphere(f3{e0, e1, e2}, N);
             next_dir = ds.dir;
@@ -758,7 +869,12 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             }
         }
         float bsdf_cos[NL];
-        for (int j = 0; j < NL; ++j) bsdf_cos[j] = eval_bsdf_cos(M, sc.lb[j], d, next_dir, N);
+        if (delta) {
+            const float dist = inside ? length(f3{md.vertex[0], md.vertex[1], md.vertex[2]} - hp) : 0.0f;
+            for (int j = 0; j < NL; ++j) bsdf_cos[j] = eval_glass(M, S.lam[j], sc.lb[j], inside, dist, d, next_dir, N);
+        } else {
+            for (int j = 0; j < NL; ++j) bsdf_cos[j] = eval_bsdf_cos(M, sc.lb[j], d, next_dir, N);
+        }
         // Trace's offset of the shading point (render.cuh:156-161).
         {
             const float off = eps[S.ndd] * fmaxf(1.0f, length(hp));
@@ -779,7 +895,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                 for (int j = 1; j < NL; ++j) t = t + lambda_to_xyz(tables, S.lam[j]) * S.mask[j];
                 g_query = pg_query_xyz(t, sp.depth == 0);
             }
-            if (!pure_light && !inside && gv.probe_count > 0) {
+            if (!delta && !pure_light && !inside_next && gv.probe_count > 0) {
                 probe = pg_query_probe(gv, hp, pg_reflect(d, N));
                 if (probe) {
                     const float p_b = pdf;
@@ -823,7 +939,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                 const f3 e{M.emissive_rgb[0], M.emissive_rgb[1], M.emissive_rgb[2]};
                 for (int j = 0; j < NL; ++j) le_end[j] = interact(rgb_lut, sc.lb[j], e * mis) * sc.d65[j];
             }
-        } else if (GUIDE && !(M.emissive_rgb[0] == 0.0f && M.emissive_rgb[1] == 0.0f && M.emissive_rgb[2] == 0.0f)) {
+        } else if (GUIDE && !delta &&
+                   !(M.emissive_rgb[0] == 0.0f && M.emissive_rgb[1] == 0.0f && M.emissive_rgb[2] == 0.0f)) {
             for (int j = 0; j < NL; ++j)
                 le_end[j] = interact_basis(M.emissive_basis, M.emissive_scale, sc.lb[j]) * sc.d65[j];
         }
@@ -841,7 +958,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             }
             const bool scatter = !pure_light && !(M.shading_model == SM_GGX && !(fmaxf(0.0f, dot(N, -d)) > 0.0f));
             bool cache_end = false;
-            if (scatter && !inside && P.s * P.s > nv.spread * P.a0) {
+            if (scatter && !inside_next && P.s * P.s > nv.spread * P.a0) {
                 if (!(P.flags & NRC_TRAIN)) {
                     cache_end = true;
                 } else if (!(P.flags & NRC_UNBIASED)) {
@@ -879,7 +996,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             bool nee_deferred = false;
             const float mm = mean_mask<NL>(S.mask);
             const float rate = sp.nee_always ? 1.0f : sqrtf(saturatef(mm));
-            if (sp.use_nee && S.ndd < sp.nee_max_depth && !inside && rand1(rng) < rate) {
+            if (sp.use_nee && S.ndd < sp.nee_max_depth && !delta && !inside_next && rand1(rng) < rate) {
                 const LightSample ls = sample_light(lights, n_lights, hp, rng);
                 if (ls.valid) {
                     const float pat = fmaxf(1e-20f, ls.p_area * ls.p_facet);
@@ -942,25 +1059,43 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             S.last_pos[0] = hp.x; S.last_pos[1] = hp.y; S.last_pos[2] = hp.z;
             float contrib[NL];
             for (int j = 0; j < NL; ++j) contrib[j] = bsdf_cos[j] / fmaxf(pdf, 1e-12f);
+            // Hero wavelengths at a dispersive interface: the secondaries would
+            // refract elsewhere, so they end here and the hero goes on alone
+            // with weight psum / p(hero), an unbiased one-wavelength estimate
+            // under the pixel's division by psum.
+            if (NL > 1 && delta && !(S.flags & PATH_HERO_ONLY) && abbe_slope(M.ior, M.abbe) != 0.0f) {
+                contrib[0] = contrib[0] * (S.psum / lambda_pdf_mix(S.lam[0]));
+                for (int j = 1; j < NL; ++j) contrib[j] = 0.0f;
+                S.flags |= PATH_HERO_ONLY;
+            }
             // Russian roulette (render.cuh:838-859).
             float p_rr = 1.0f;
             bool survived = true;
             if (S.ndd >= sp.rr_min_depth) {
                 const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
-                p_rr = lerpf(basic, 1.0f, inside ? 0.99f : 0.0f);
+                p_rr = lerpf(basic, 1.0f, inside_next ? 0.99f : 0.0f);
                 if (rand1(rng) > p_rr) survived = false;
             }
             if (survived) {
-                if (GUIDE) pg_advance<NL>(gv.recs[pid], next_dir, contrib, p_rr, S.ndd, !inside, d, hp, N);
+                if (GUIDE) pg_advance<NL>(gv.recs[pid], next_dir, contrib, p_rr, S.ndd, !delta && !inside_next, d, hp, N);
                 for (int j = 0; j < NL; ++j) {
                     S.mask[j] = S.mask[j] * contrib[j];
                     S.mask[j] = S.mask[j] / p_rr;
                 }
-                S.ndd += 1;  // no delta surfaces in this core
+                if (!delta) S.ndd += 1;
+                // EvaluateMaterial's MediaContext (material.cuh:410-411): this
+                // object, one more hit, the hit point before the offset.
+                const uint32_t depth_next = media_depth < kMediaDepthMax ? media_depth + 1u : kMediaDepthMax;
+                S.flags = (S.flags & ~(PATH_INSIDE | (kMediaDepthMax << kMediaDepthShift))) |
+                          (inside_next ? PATH_INSIDE : 0u) | (depth_next << kMediaDepthShift);
                 if (sp.depth + 1 < sp.max_depth) {
                     emit_next = true;
+                    // A ray inside a medium meets only the medium's object: a
+                    // NaN tmax makes the traversal reject the root at once,
+                    // and the next shade intersects the object itself.
                     next = ap_ray{hp.x, hp.y, hp.z, eps[S.ndd], next_dir.x, next_dir.y, next_dir.z,
-                                  __int_as_float(0x7f800000)};
+                                  inside_next ? __int_as_float(0x7fc00000) : __int_as_float(0x7f800000)};
+                    if (inside_next) sp.media[pid] = MediaRec{{hi.pos.x, hi.pos.y, hi.pos.z}, H.id};
                 }
             }
         }
@@ -1239,7 +1374,8 @@ template <int NL>
 __global__ __launch_bounds__(kBlock) void generate_batch_kernel(BatchView bv, uint32_t total, uint64_t seed,
                                                                 const float* __restrict__ eps,
                                                                 PathT<NL>* __restrict__ paths, ap_ray* __restrict__ rays,
-                                                                int32_t* __restrict__ path_of) {
+                                                                int32_t* __restrict__ path_of,
+                                                                MediaRec* __restrict__ media) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= total) return;
     const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
@@ -1272,6 +1408,7 @@ __global__ __launch_bounds__(kBlock) void generate_batch_kernel(BatchView bv, ui
     st.last_pos[0] = st.last_pos[1] = st.last_pos[2] = 0.0f;
     st.ndd = 0;
     st.flags = (B.inside ? PATH_INSIDE : 0u) | (static_cast<uint32_t>(sc) << 16);
+    if (B.inside) media[i] = MediaRec{{origin.x, origin.y, origin.z}, -1};
     st.pixel = p;
     st.frame = frame;
     st.rng_state = rng.state;
@@ -1450,6 +1587,10 @@ __global__ void build_mats_kernel(const MatIn* __restrict__ in, int n, const f4*
     r.albedo_rgb[1] = m.albedo.y;
     r.albedo_rgb[2] = m.albedo.z;
     r.pad3 = 0;
+    r.ior = m.ior;
+    r.sigma_a = m.sigma_a;
+    r.abbe = m.abbe;
+    r.tilt = m.tilt;
     out[i] = r;
 }
 
@@ -1512,8 +1653,6 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
     }
     for (size_t i = 0; i < n; ++i) {
         const ap_scene_object& o = s->objects[i];
-        if (o.material.shading_model == AP_MAT_GLASS)
-            return fail(AP_ERR_UNSUPPORTED, "ap_render: glass materials are not supported (object " + std::to_string(i) + ")");
         const ap_gbuffer& gb = o.material.texture.base;
         if (gb.albedo_tex || gb.normal_tex || gb.metallic_tex || gb.roughness_tex)
             return fail(AP_ERR_UNSUPPORTED, "ap_render: textured materials are not supported (object " + std::to_string(i) + ")");
@@ -1523,10 +1662,12 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
             m.albedo = {1.0f, 1.0f, 1.0f};
             m.emissive = {0.0f, 0.0f, 0.0f};
             m.specular = 0.0f; m.metallic = 0.0f; m.roughness = 0.0f;
+            m.ior = 1.5f; m.sigma_a = 100.0f; m.abbe = 20.0f; m.tilt = 0.0f;
         } else {
             m.albedo = F(gb.albedo);
             m.emissive = F(gb.emissive);
             m.specular = gb.specular; m.metallic = gb.metallic; m.roughness = gb.roughness;
+            m.ior = gb.ior; m.sigma_a = gb.sigma_a; m.abbe = gb.abbe_number; m.tilt = gb.tilt_number;
         }
         m.model = o.material.shading_model;
         m.light_index = o.light_index;
@@ -1591,19 +1732,22 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
         if (s->objects[i].type == AP_OBJ_CUBE || s->objects[i].type == AP_OBJ_SPHERE ||
             s->objects[i].type == AP_OBJ_SDF)
             r->analytic_objs.push_back(static_cast<int32_t>(i));
+    r->has_glass = false;
+    for (size_t i = 0; i < n; ++i) r->has_glass = r->has_glass || s->objects[i].material.shading_model == AP_MAT_GLASS;
     r->tri_mode = tri_mode;
     return AP_OK;
 }
 
 int ensure_paths(RenderState* r, size_t cap) {
     if (r->capacity >= cap) return AP_OK;
-    void* old[] = {r->paths, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1], r->hits, r->shadow_rays,
+    void* old[] = {r->paths, r->media, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1], r->hits, r->shadow_rays,
                    r->shadow_payload, r->shadow_hits};
     for (void* p : old)
         if (p) (void)hipFree(p);
     r->capacity = 0;
     int rc;
     if ((rc = dev_alloc(reinterpret_cast<uint8_t**>(&r->paths), cap * kPathBytes)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->media, cap)) != AP_OK) return rc;
     for (int q = 0; q < 2; ++q) {
         if ((rc = dev_alloc(&r->rays[q], cap)) != AP_OK) return rc;
         if ((rc = dev_alloc(&r->path_of[q], cap)) != AP_OK) return rc;
@@ -1741,7 +1885,7 @@ int run_frames(const RunCtx& c) {
         hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, r->counters, total, 0u, 0u);
         const GuideView gv = (guiding && !train_only) ? guide_view(s->guide) : GuideView{};
         hipLaunchKernelGGL(generate_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, cp, fp, r->eps, paths,
-                           r->rays[0], r->path_of[0], gv.recs);
+                           r->rays[0], r->path_of[0], gv.recs, r->media);
         NrcView nv{};
         if (nrc) {
             hipLaunchKernelGGL(nrc_init_kernel, grid_of(static_cast<size_t>(total)), dim3(kBlock), 0, stream,
@@ -1762,6 +1906,7 @@ int run_frames(const RunCtx& c) {
             if (fb->gbuffer && !train_only) sp.gsample = r->gsamples;
             sp.volumes = s->dev.volumes;
             sp.n_volumes = s->dev.n_volumes;
+            sp.media = r->media;
             if (guiding) {
                 hipLaunchKernelGGL((shade_kernel<NL, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
                                    r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
@@ -1892,6 +2037,8 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     if (r->tri_mode != prm->triangle_material) {
         if ((rc = build_shading(s, r, prm->triangle_material, rgb_lut)) != AP_OK) return rc;
     }
+    if (cache && r->has_glass)
+        return fail(AP_ERR_UNSUPPORTED, "ap_render_nrc: glass materials are not supported with the radiance cache");
     const int32_t n_pix = cam->width * cam->height;
     // Frames per batch: about 4M paths in flight. With guiding, one frame at
     // a time: each frame samples the probes built from the frames before it
@@ -1968,6 +2115,7 @@ struct BatchState {
     uint32_t* d_pix_begin = nullptr;
     DeviceScene shape{};                // launch shape, largest stack, traversal mode
     uint8_t* paths = nullptr;
+    MediaRec* media = nullptr;
     ap_ray* rays[2] = {nullptr, nullptr};
     int32_t* path_of[2] = {nullptr, nullptr};
     ap_hit* hits = nullptr;
@@ -2020,7 +2168,7 @@ struct BatchState {
     }
     ~BatchState() {
         drop_graph();
-        void* ptrs[] = {d_scenes, d_devs, d_pix_begin, paths, rays[0], rays[1], path_of[0], path_of[1], hits,
+        void* ptrs[] = {d_scenes, d_devs, d_pix_begin, paths, media, rays[0], rays[1], path_of[0], path_of[1], hits,
                         shadow_rays, shadow_payload, shadow_hits, ray_scene[0], ray_scene[1], shadow_scene, counters,
                         frame, rays_traced, eps, nrc_table, nrc_paths, nrc_queries, nrc_qpath, nrc_qrgb,
                         nrc_counters, nrc_qcount, nrc_tile0, nrc_pool_begin, nrc_pool_n,
@@ -2079,7 +2227,7 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
     const uint32_t total = b->total;
     hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
     hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
-                       b->eps, paths, b->rays[0], b->path_of[0]);
+                       b->eps, paths, b->rays[0], b->path_of[0], b->media);
     int in_q = 0;
     for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
         const int out_q = 1 - in_q;
@@ -2087,8 +2235,9 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
         hipError_t e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[in_q], b->rays[in_q], total, b->hits,
                                           false, stream, &b->counters[in_q]);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
-        const ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
-                             prm->nee_always, prm->sky, in_q, out_q, 0};
+        ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
+                       prm->nee_always, prm->sky, in_q, out_q, 0};
+        sp.media = b->media;
         hipLaunchKernelGGL((shade_kernel<NL, false, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
                            nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
@@ -2252,7 +2401,7 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
     b->nrc_last_hash = h;
     hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
     hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
-                       b->eps, paths, b->rays[0], b->path_of[0]);
+                       b->eps, paths, b->rays[0], b->path_of[0], b->media);
     hipLaunchKernelGGL(nrc_init_batch_kernel, grid_of(total), dim3(kBlock), 0, stream, bv, total, b->nrc_table, h,
                        np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount);
     const NrcView nv{b->nrc_paths, b->nrc_queries, b->nrc_qpath, b->nrc_verts, b->nrc_counters, np->spread,
@@ -2264,8 +2413,9 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
         hipError_t e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[in_q], b->rays[in_q], total, b->hits,
                                           false, stream, &b->counters[in_q]);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
-        const ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
-                             prm->nee_always, prm->sky, in_q, out_q, 0};
+        ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
+                       prm->nee_always, prm->sky, in_q, out_q, 0};
+        sp.media = b->media;
         hipLaunchKernelGGL((shade_kernel<NL, false, true, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
                            nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
@@ -2489,6 +2639,7 @@ int ap_batch_create(ap_scene* const* scenes, const ap_camera* cameras, size_t n,
     alloc(&B.d_devs, n * sizeof(DeviceScene));
     alloc(&B.d_pix_begin, (n + 1) * sizeof(uint32_t));
     alloc(&B.paths, cap * kPathBytes);
+    alloc(&B.media, cap * sizeof(MediaRec));
     for (int q = 0; q < 2; ++q) {
         alloc(&B.rays[q], cap * sizeof(ap_ray));
         alloc(&B.path_of[q], cap * sizeof(int32_t));
@@ -2575,6 +2726,10 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
     const float* tables = nullptr;
     if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
     if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut)) != AP_OK) return rc;
+    for (ap_scene* sc : B.scenes)
+        if (sc->render->has_glass)
+            return fail(AP_ERR_UNSUPPORTED,
+                        "ap_batch_render_nrc: glass materials are not supported with the radiance cache");
     if ((rc = batch_nrc_prepare(&B, nrc_params, ns)) != AP_OK) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);

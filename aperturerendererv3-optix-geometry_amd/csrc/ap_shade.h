@@ -9,7 +9,10 @@
 //   SkyLighting                           core/render/render.cuh:58-73
 //   AreaToSolidAnglePDF, box-facet and sphere-cap light sampling and their
 //   paired solid-angle pdfs               core/nee/nee.cuh:24-327
-// Glass (MAT_GLASS) is not part of this core; scenes using it are rejected.
+//   Glass: IORFromAbbeNM, SigmaAFromTiltNM  core/spectrum/spectrum.cuh:7-57
+//          BeerLambert_Spectrum, FresnelDielectricExact  core/render/brdf.cuh:20-32
+//          refract                         core/vector.cuh:267-275
+//          SampleBSDF / EvaluateBSDFWithCos (MAT_GLASS)  core/render/material.cuh:122-169, 221-272
 #pragma once
 
 #include "ap_sample.h"
@@ -38,6 +41,8 @@ struct alignas(16) MatRec {
     int32_t pad2;
     float albedo_rgb[3];     // the GBuffer albedo (radiance-cache query records)
     int32_t pad3;
+    float ior, sigma_a;      // glass: IOR at the d line, absorption coefficient
+    float abbe, tilt;        // glass: Abbe number, absorption tilt over wavelength
 };
 
 AP_HD float interact_basis(f4 rgb_basis, float scale, f4 lambda_basis) {
@@ -159,6 +164,94 @@ AP_HD float bsdf_pdf(const MatRec& m, f4 lb, f3 N, f3 ray_dir, f3 L) {
         return fmaxf(0.0f, pSpec * pdfSpec + (1.0f - pSpec) * pdfDiff);
     }
     return 0.0f;
+}
+
+// ---- glass ----
+// Fraunhofer F, C and d lines (nm) as reciprocals.
+constexpr float kInvF = 1.0f / 486.1327f;
+constexpr float kInvC = 1.0f / 656.2725f;
+constexpr float kInvD = 1.0f / 587.5618f;
+constexpr float kPerfectDelta = 0.99f;       // PERFECT_DELTA_SURFACE
+constexpr int kMaxDeltaDepth = 1024;         // MAX_DELTA_SURFACE_DEPTH
+
+// IORLinearKFromAbbe: the slope of n over 1/lambda from the Abbe number.
+AP_HD float abbe_slope(float nd, float vd) {
+    if (!(vd > 0.0f)) return 0.0f;
+    return (nd - 1.0f) / (vd * (kInvF - kInvC));
+}
+
+// IORFromAbbeNM: n(lambda) = nd + K (1/lambda - 1/lambda_d).
+AP_HD float ior_at(float lambda, float nd, float vd) {
+    if (!(lambda > 0.0f)) return nd;
+    const float K = abbe_slope(nd, vd);
+    const float invL = 1.0f / lambda;
+    return nd + K * (invL - kInvD);
+}
+
+// SigmaAFromTiltNM with a tilt gain of 1.
+AP_HD float sigma_a_at(float sigma_a, float lambda, float tilt) {
+    const float u = 1.0f / fmaxf(lambda, 1.0e-6f);
+    const float du = (u - kInvD) / (kInvF - kInvC);
+    const float sA = sigma_a * (1.0f + 1.0f * tilt * du);
+    return fmaxf(0.0f, sA);
+}
+
+// BeerLambert_Spectrum: albedo_l^(distance sigma_a).
+AP_HD float beer_lambert(float distance, float albedo_l, float sigma_a) {
+    return apfm::exp(distance * sigma_a * apfm::log(fminf(fmaxf(albedo_l, 1.0e-12f), 1.0f)));
+}
+
+// FresnelDielectricExact, eta = n_in / n_out.
+AP_HD float fresnel_dielectric(float cosI, float cosT, float eta) {
+    const float invE = 1.0f / eta;
+    const float rPar = (invE * cosI - cosT) / (invE * cosI + cosT);
+    const float rPerp = (cosI - invE * cosT) / (cosI + invE * cosT);
+    return 0.5f * (rPar * rPar + rPerp * rPerp);
+}
+
+AP_HD f3 refract_dir(f3 in, f3 normal, float eta) {
+    const f3 n = (dot(normal, in) > 0.0f) ? -normal : normal;
+    const float cosI = -dot(n, in);
+    const float k = 1.0f - (eta * eta) * fmaxf(0.0f, 1.0f - cosI * cosI);
+    const float cosT = sqrtf(fmaxf(0.0f, k));
+    return eta * in + (eta * cosI - cosT) * n;
+}
+
+// The interface a ray meets: the normal on its side, n_in / n_out, the
+// incident cosine and 1 - sin^2(theta_t) (<= 0: total internal reflection).
+struct Interface {
+    f3 n_in;
+    float eta, cosI, k;
+};
+
+AP_HD Interface glass_interface(float ior, bool inside, f3 ray_dir, f3 N) {
+    Interface f;
+    f.n_in = (dot(N, ray_dir) > 0.0f) ? -N : N;
+    f.eta = inside ? ior : 1.0f / ior;
+    f.cosI = -dot(f.n_in, ray_dir);
+    const float sin2I = fmaxf(0.0f, 1.0f - f.cosI * f.cosI);
+    f.k = 1.0f - sin2I * (f.eta * f.eta);
+    return f;
+}
+
+// EvaluateBSDFWithCos for MAT_GLASS at one wavelength: the absorption over
+// `distance` when the ray ran inside the medium, times the Fresnel weight
+// of the branch (reflection or refraction) that L follows.
+AP_HD float eval_glass(const MatRec& m, float lambda, f4 lb, bool inside, float distance, f3 ray_dir, f3 L, f3 N) {
+    const float ior = ior_at(lambda, m.ior, m.abbe);
+    float transmittance = 1.0f;
+    if (inside) transmittance = beer_lambert(distance, albedo_response(m, lb), sigma_a_at(m.sigma_a, lambda, m.tilt));
+    const Interface f = glass_interface(ior, inside, ray_dir, N);
+    if (f.k <= 0.0f) return transmittance * kPerfectDelta;
+    const float cosT = sqrtf(f.k);
+    const float Fr = fresnel_dielectric(f.cosI, cosT, f.eta);
+    const f3 R = reflect(ray_dir, f.n_in);
+    const f3 T = refract_dir(ray_dir, f.n_in, f.eta);
+    const float dR = dot(L, R);
+    const float dT = dot(L, T);
+    if (dR >= dT) return transmittance * Fr;
+    const float weight = (f.eta * f.eta) * (fabsf(cosT) / fmaxf(1e-6f, f.cosI));
+    return transmittance * (1.0f - Fr) * weight;
 }
 
 AP_HD float power_heuristic(float pa, float pb) {

@@ -117,6 +117,19 @@ int or_render_screen(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
                      const ap_render_params* prm, float cull_margin, float* hdr, double* sums,
                      ap_screen_gbuffer* gbuffer, ap_screen_statistics* statistics, int threads);
 
+/* Glass at one wavelength (or_render.cpp): IORFromAbbeNM, SigmaAFromTiltNM,
+ * FresnelDielectricExact, BeerLambert_Spectrum, and SampleBSDF followed by
+ * EvaluateBSDFWithCos of MAT_GLASS. mat = ior, abbe, sigma_a, tilt, albedo
+ * rgb; the MediaContext is (inside, depth, vertex); the Fresnel draw comes
+ * from PCG32 seeded with seed; out = next direction (3), pdf, brdf, inside
+ * after, delta flag. */
+float or_glass_ior(float lambda, float nd, float vd);
+float or_glass_sigma(float sigma_a, float lambda, float tilt);
+float or_glass_fresnel(float cosI, float cosT, float eta);
+float or_glass_beer(float distance, float resp, float sigma_a);
+void or_glass_bsdf(const float* d, const float* n, const float* mat, float lambda, int inside, int depth,
+                   const float* vertex, const float* hit, uint64_t seed, float* out);
+
 #ifdef __cplusplus
 }
 #endif

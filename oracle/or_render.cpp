@@ -16,8 +16,13 @@
 // restriction: PCG32 replaces curand (stream per pixel, start state hashed
 // from seed, frame and pixel with SplitMix64); rsqrtf is 1/sqrtf; triangles may carry their object's
 // material (param) instead of IntersectTriangleObject's default GBuffer;
-// glass and textures are not restated; a path that misses while "inside a
-// medium" ends (the reference continues from an undefined point).
+// textures are not restated.
+// Glass and media (material.cuh:12-18, 122-169, 221-272, render.cuh:84-160):
+// MediaContext, the solo trace inside a medium, the miss inside a medium
+// (which restarts at the world origin, as the reference's zero-initialised
+// hit record makes it), the exact dielectric with Abbe dispersion and
+// tilted Beer-Lambert absorption; with hero wavelengths the secondaries end
+// at a dispersive interface (HeroOnlyAtDispersion).
 // Directional path guiding (or_guiding.h) follows render.cuh:546-639 and
 // scene.cu:308-386.
 // The learned radiance cache has no reference counterpart: CalculateRadianceNRC
@@ -92,6 +97,15 @@ inline float INTERSECT_EPS(int depth) { return 1.0e-3f * std::exp(std::fmin(stat
 struct GB {
     float3 albedo, emissive;
     float specular, metallic, roughness;
+    float ior, sigmaA, abbe, tilt;
+};
+
+// MediaContext (material.cuh:12-18).
+struct MediaCtx {
+    bool inside = false;
+    float3 vertex{0, 0, 0};
+    int object = -1;
+    int depth = 0;
 };
 
 struct Ctx {  // IntersectionContext subset
@@ -255,11 +269,81 @@ static float GGX_G_Correlated(float alpha, float NoL, float NoV) {
 }
 static float F0_UEStyle(float s) { return 0.08f * clampf(s, 0.0f, 1.0f); }
 
+// ---- glass: dispersion and absorption (spectrum.cuh:7-57), Beer-Lambert
+// and the exact dielectric Fresnel term (brdf.cuh:20-32), refract
+// (vector.cuh:267-275) ----
+static constexpr float kWaveF = 486.1327f, kWaveD = 587.5618f, kWaveC = 656.2725f;
+static constexpr float kInvF = 1.0f / kWaveF, kInvC = 1.0f / kWaveC, kInvD = 1.0f / kWaveD;
+
+static float IORLinearKFromAbbe(float nd, float Vd) {
+    if (!(Vd > 0.0f)) return 0.0f;
+    return (nd - 1.0f) / (Vd * (kInvF - kInvC));
+}
+
+static float IORFromAbbeNM(float lambda, float nd, float Vd) {
+    if (!(lambda > 0.0f)) return nd;
+    const float K = IORLinearKFromAbbe(nd, Vd);
+    const float invL = 1.0f / lambda;
+    return nd + K * (invL - kInvD);
+}
+
+static float SigmaAFromTiltNM(float sigmaA, float lambda, float tilt) {
+    const float u = 1.0f / std::fmax(lambda, 1.0e-6f);
+    const float du = (u - kInvD) / (kInvF - kInvC);
+    const float sA = sigmaA * (1.0f + 1.0f * tilt * du);
+    return std::fmax(0.0f, sA);
+}
+
+static float BeerLambert_Spectrum(float distance, float resp, float sigmaA) {
+    return apfm::exp(distance * sigmaA * apfm::log(std::fmin(std::fmax(resp, 1.0e-12f), 1.0f)));
+}
+
+static float FresnelDielectricExact(float cosI, float cosT, float eta) {
+    const float invE = 1.0f / eta;
+    const float rPar = (invE * cosI - cosT) / (invE * cosI + cosT);
+    const float rPerp = (cosI - invE * cosT) / (cosI + invE * cosT);
+    return 0.5f * (rPar * rPar + rPerp * rPerp);
+}
+
+static float3 Refract(float3 inDir, float3 normal, float eta) {
+    const float3 n = (dot(normal, inDir) > 0.0f) ? -normal : normal;
+    const float cosI = -dot(n, inDir);
+    const float k = 1.0f - (eta * eta) * std::fmax(0.0f, 1.0f - cosI * cosI);
+    const float cosT = std::sqrt(std::fmax(0.0f, k));
+    return eta * inDir + (eta * cosI - cosT) * n;
+}
+
 // ---- materials (material.cuh) ----
-static float EvaluateBSDFWithCos(const float lb[4], int model, float3 rayDir, float3 L, const Ctx& c) {
+// m: the MediaContext the ray was traced in (glass only).
+static float EvaluateBSDFWithCos(const float lb[4], float lambda, int model, float3 rayDir, float3 L, const Ctx& c,
+                                 const MediaCtx& m) {
     const float3 N = c.normal;
     const float NoL = std::fmax(0.0f, dot(N, L));
     if (model == AP_MAT_DIFFUSE) return InteractC(lb, c.gb.albedo) * (NoL / PI_F);
+    if (model == AP_MAT_GLASS) {
+        const float IOR = IORFromAbbeNM(lambda, c.gb.ior, c.gb.abbe);
+        float transmittance = 1.0f;
+        if (m.inside) {
+            const float rayDistance = length(m.vertex - c.pos);
+            transmittance = BeerLambert_Spectrum(rayDistance, InteractC(lb, c.gb.albedo),
+                                                 SigmaAFromTiltNM(c.gb.sigmaA, lambda, c.gb.tilt));
+        }
+        const float3 nIn = (dot(N, rayDir) > 0.0f) ? -N : N;
+        const float eta = m.inside ? IOR : 1.0f / IOR;
+        const float cosI = -dot(nIn, rayDir);
+        const float sin2I = std::fmax(0.0f, 1.0f - cosI * cosI);
+        const float k = 1.0f - sin2I * (eta * eta);
+        if (k <= 0.0f) return transmittance * 0.99f;  // PERFECT_DELTA_SURFACE
+        const float cosT = std::sqrt(k);
+        const float Fr = FresnelDielectricExact(cosI, cosT, eta);
+        const float3 R = reflect(rayDir, nIn);
+        const float3 T = Refract(rayDir, nIn, eta);
+        const float dR = dot(L, R);
+        const float dT = dot(L, T);
+        if (dR >= dT) return transmittance * Fr;
+        const float weight = (eta * eta) * (std::fabs(cosT) / std::fmax(1e-6f, cosI));
+        return transmittance * (1.0f - Fr) * weight;
+    }
     if (model == AP_MAT_GGX) {
         const float3 V = normalize(-rayDir);
         const float NoV = std::fmax(0.0f, dot(N, V));
@@ -306,10 +390,45 @@ struct SampleCtx {
     bool inside = false;
 };
 
-static SampleCtx SampleBSDF(const float lb[4], Rng& rng, int model, float3 rayDir, const Ctx& c, bool inside) {
+static SampleCtx SampleBSDF(const float lb[4], float lambda, Rng& rng, int model, float3 rayDir, const Ctx& c,
+                            const MediaCtx& m) {
     SampleCtx r;
     const float3 N = c.normal;
     const float3 V = -rayDir;
+    const bool inside = m.inside;
+    if (model == AP_MAT_GLASS) {
+        const float IOR = IORFromAbbeNM(lambda, c.gb.ior, c.gb.abbe);
+        r.delta = true;
+        const float eta = inside ? IOR : 1.0f / IOR;
+        const float3 nIn = (dot(N, rayDir) > 0.0f) ? -N : N;
+        const float cosI = -dot(nIn, rayDir);
+        const float sin2I = std::fmax(0.0f, 1.0f - cosI * cosI);
+        const float k = 1.0f - sin2I * (eta * eta);
+        r.emissive = {0, 0, 0};
+        if (k <= 0.0f) {  // total internal reflection
+            if (m.depth < 1024) {  // MAX_DELTA_SURFACE_DEPTH
+                r.next = reflect(rayDir, nIn);
+                r.inside = inside;
+            } else {  // forced out
+                r.next = rayDir;
+                r.inside = !inside;
+            }
+            r.pdf = 1.0f;
+            return r;
+        }
+        const float cosT = std::sqrt(k);
+        const float Fr = FresnelDielectricExact(cosI, cosT, eta);
+        if (rng.rand1() < Fr) {
+            r.next = reflect(rayDir, nIn);
+            r.pdf = Fr;
+            r.inside = inside;
+        } else {
+            r.next = Refract(rayDir, nIn, eta);
+            r.pdf = 1.0f - Fr;
+            r.inside = !inside;
+        }
+        return r;
+    }
     if (model == AP_MAT_DIFFUSE) {
         const float e0 = rng.rand1(), e1 = rng.rand1(), e2 = rng.rand1();
         const Sample s = SampleCosine({e0, e1, e2}, N);
@@ -359,7 +478,8 @@ static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t, floa
     c.lightIndex = S.light_index[idx];
     c.distance = t;
     const ap_gbuffer& g = ob.material.texture.base;
-    c.gb = {F3(g.albedo), F3(g.emissive), g.specular, g.metallic, g.roughness};
+    c.gb = {F3(g.albedo), F3(g.emissive), g.specular, g.metallic, g.roughness,
+            g.ior, g.sigma_a, g.abbe_number, g.tilt_number};
     if (ob.type == AP_OBJ_TRIANGLE) {
         float tt, u, v;
         IntersectTriangleT(o, d, ob, tt, u, v);
@@ -371,7 +491,8 @@ static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t, floa
         const float3 n = normalize(n0 * w + n1 * u + n2 * v);
         c.pos = o + d * t;
         c.normal = dot(d, n) > 0.0f ? -n : n;
-        if (S.tri_mode == AP_TRIANGLE_MATERIAL_REFERENCE) c.gb = {{1, 1, 1}, {0, 0, 0}, 0.0f, 0.0f, 0.0f};
+        if (S.tri_mode == AP_TRIANGLE_MATERIAL_REFERENCE)  // a default GBuffer (object.cuh:60-72)
+            c.gb = {{1, 1, 1}, {0, 0, 0}, 0.0f, 0.0f, 0.0f, 1.5f, 100.0f, 20.0f, 0.0f};
         return c;
     }
     const float3x3 w2o = M3(ob.world_to_object), o2w = M3(ob.object_to_world);
@@ -420,6 +541,8 @@ static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t, floa
 
 extern void TraceOne(const or_bvh* h, float3 o, float3 d, float tMin, float tMax, bool anyHit, bool cull,
                      float margin, int* id, float* t, float sdfEps);
+extern bool Intersector(const ap_scene_object* objs, int idx, float3 o, float3 d, float tMin, float tMax,
+                        float& tOut, float sdfEps);
 
 // ---- NEE (nee.cuh) ----
 struct LightSample { float3 point, normal; float pFacet, pA; int object; bool valid; };
@@ -582,6 +705,36 @@ struct RenderOpts {
     int max_depth, rr_min_depth, nee_max_depth, use_nee, nee_always, sky;
 };
 
+// Trace's intersection (render.cuh:88-107): inside a medium with an object
+// of its own the ray meets only that object's surface (Intersect with a solo
+// object, intersection.cuh:185-193), otherwise the BVH.
+static void TraceMedia(const Scene& S, const MediaCtx& m, float3 ro, float3 rd, float eps, int* id, float* t) {
+    const float inf = std::numeric_limits<float>::infinity();
+    if (m.inside && m.object >= 0) {
+        float tt;
+        if (Intersector(S.objs, m.object, ro, rd, eps, inf, tt, eps * 0.5f)) {
+            *id = m.object;
+            *t = tt;
+        } else {
+            *id = -1;
+            *t = inf;
+        }
+        return;
+    }
+    TraceOne(S.bvh, ro, rd, eps, inf, false, S.margin > 0.0f, S.margin, id, t, eps * 0.5f);
+}
+
+// MediaContext after a hit (EvaluateMaterial, material.cuh:410-411; SampleBSDF
+// sets the vertex to the hit point before Trace's offset).
+static MediaCtx MediaAfterHit(const MediaCtx& m, const SampleCtx& sc, int id, float3 hitPos) {
+    MediaCtx n;
+    n.inside = sc.inside;
+    n.vertex = hitPos;
+    n.object = id;
+    n.depth = m.depth + 1;
+    return n;
+}
+
 // The wavelengths of a path (the core's hero-wavelength mode when n = 4):
 // the hero and its rotations by 105 nm over [360, 780], with their bases,
 // D65 weights, CMF values and the sum of their pdfs.
@@ -622,6 +775,19 @@ static float3 WavesXYZ(const Waves& w, const float* L) {
     float3 acc = w.xyz[0] * L[0];
     for (int j = 1; j < w.n; ++j) acc = acc + w.xyz[j] * L[j];
     return acc / w.psum;
+}
+
+// Hero wavelengths at a dispersive interface (the core's extension; one
+// wavelength has nothing to end): the secondary wavelengths would refract
+// elsewhere, so they end there and the hero continues alone with weight
+// psum / p(hero), which keeps the rest of the path an unbiased
+// one-wavelength estimate under the pixel's division by psum.
+static void HeroOnlyAtDispersion(const Waves& wv, int nl, const SampleCtx& sc, const GB& gb, bool& heroOnly,
+                                 float* contrib) {
+    if (nl == 1 || !sc.delta || heroOnly || IORLinearKFromAbbe(gb.ior, gb.abbe) == 0.0f) return;
+    contrib[0] = contrib[0] * (wv.psum / PdfCIE1931Mix(wv.lam[0]));
+    for (int j = 1; j < nl; ++j) contrib[j] = 0.0f;
+    heroOnly = true;
 }
 
 static float MeanMask(const float* m, int n) {
@@ -722,17 +888,27 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
     for (int j = 0; j < nl; ++j) { radiance[j] = 0.0f; mask[j] = 1.0f; }
     float3 query{1.0f, 1.0f, 1.0f};  // maskForProbeQuery
     float3 ro = origin, rd = dir;
-    bool inside = inside0;
+    MediaCtx media;
+    media.inside = inside0;
+    media.vertex = origin;
+    bool heroOnly = false;  // secondary wavelengths ended at a dispersive interface
     bool prevNEE = false;
     float lastPdf = 0.0f;
     float3 lastPos{0, 0, 0};
+    auto updateQuery = [&]() {
+        // maskForProbeQuery: the throughput's XYZ (one wavelength: xyz * mask).
+        float3 t = wv.xyz[0] * mask[0];
+        for (int j = 1; j < nl; ++j) t = t + wv.xyz[j] * mask[j];
+        query = fmax3(t, float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
+        // sum3 + PROBES_QUERY_EPS is a double sum (the constant is a double literal).
+        query = query / static_cast<float>(static_cast<double>(query.x + query.y + query.z) + 1.0e-6);
+    };
     for (int depth = 0; depth < P.max_depth; ++depth) {
         // Trace (render.cuh:75-163).
         const float eps = INTERSECT_EPS(ndd);
         int id;
         float t;
-        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t,
-                 eps * 0.5f);
+        TraceMedia(S, media, ro, rd, eps, &id, &t);
         if (depth == 0 && primary) *primary = id;
         if (depth == 0 && gs) {  // RecordScreenGBuffer's inputs (render.cuh:640-643)
             *gs = or_gsample{};
@@ -741,16 +917,41 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         const float alpha = GuidingAlpha(frame);
         if (id < 0) {
             (void)rng.rand1();  // offset draw
-            float endpoint[MAXL] = {};
-            if (!inside) {
+            if (!media.inside) {
+                float endpoint[MAXL] = {};
                 for (int j = 0; j < nl; ++j) {
                     const float le = P.sky ? InteractC(wv.lb[j], SkyLighting(rd)) : 0.0f;
                     radiance[j] = radiance[j] + mask[j] * le * wv.d65[j];
                     endpoint[j] = le * wv.d65[j];
                 }
+                if (guiding) col.Endpoint(endpoint, ndd);
+                break;
             }
-            if (guiding) col.Endpoint(endpoint, ndd);
-            break;
+            // A miss inside a medium (bMissThinSurfaceInsideMedia_, render.cuh:109-126,
+            // 143-155): no emission, no NEE and no MIS update; weight 1; the
+            // zero-initialised hit record makes the next ray start at the
+            // world origin in the same direction, outside any medium; Trace's
+            // miss branch clears bDeltaSurface_, so the vertex counts.
+            float p = 1.0f;
+            if (ndd >= P.rr_min_depth) {
+                const float mm = MeanMask(mask, nl);
+                const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
+                p = lerp(basic, 1.0f, 0.0f);
+                if (rng.rand1() > p) break;
+            }
+            float contrib[MAXL];
+            for (int j = 0; j < nl; ++j) contrib[j] = 1.0f / std::fmax(1.0f, 1e-12f);
+            if (guiding) col.Advance(rd, contrib, p, ndd, false, rd, float3{0, 0, 0}, float3{0, 0, 0});
+            for (int j = 0; j < nl; ++j) {
+                mask[j] = mask[j] * contrib[j];
+                mask[j] = mask[j] / p;
+            }
+            if (guiding) updateQuery();
+            media = MediaCtx{};
+            media.vertex = ro;
+            ro = float3{0, 0, 0};
+            ndd++;
+            continue;
         }
         Ctx c = HitContext(S, id, ro, rd, t, eps * 0.5f);
         const int model = S.objs[id].material.shading_model;
@@ -763,12 +964,14 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             gs->model = model;
             gs->hit = 1;
         }
-        SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
+        SampleCtx sc = SampleBSDF(lb, wv.lam[0], rng, model, rd, c, media);
         float brdf[MAXL];
-        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], model, rd, sc.next, c);
+        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, sc.next, c, media);
+        const MediaCtx mediaIn = media;
+        media = MediaAfterHit(media, sc, id, c.pos);
+        const bool inside = media.inside;
         const float offEps = eps * std::fmax(1.0f, length(c.pos));
         c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
-        inside = sc.inside;
         // Directional guiding of the sampled direction (render.cuh:546-639).
         const ap_octahedron* probe = nullptr;
         if (guiding && !sc.delta && !sc.pureLight && !inside && !G->probes.empty()) {
@@ -782,7 +985,8 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                         if (dot(gs.dir, c.normal) > 1.0e-12f) {
                             const float pBS = BSDF_PDF(lb, model, c.gb, c.normal, rd, gs.dir);
                             sc.next = gs.dir;
-                            for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], model, rd, gs.dir, c);
+                            for (int j = 0; j < nl; ++j)
+                                brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, gs.dir, c, mediaIn);
                             sc.pdf = (1.0f - alpha) * std::fmax(pBS, 1.0e-12f) + alpha * gs.pdf;
                         } else {
                             probe = nullptr;
@@ -857,7 +1061,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                         }
                         const float w = PowerHeuristic(pLo, pBs);
                         for (int j = 0; j < nl; ++j) {
-                            const float bc = EvaluateBSDFWithCos(wv.lb[j], model, rd, L, c);
+                            const float bc = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, L, c, mediaIn);
                             const float resp = InteractC(wv.lb[j], F3(lg.emissive));
                             const float contribution = bc * resp * (G2 / pAT);
                             radiance[j] = radiance[j] + mask[j] * (contribution * w) * wv.d65[j];
@@ -871,6 +1075,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         lastPos = c.pos;
         float contrib[MAXL];
         for (int j = 0; j < nl; ++j) contrib[j] = brdf[j] / std::fmax(sc.pdf, 1e-12f);
+        HeroOnlyAtDispersion(wv, nl, sc, c.gb, heroOnly, contrib);
         float p = 1.0f;
         if (ndd >= P.rr_min_depth) {
             const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
@@ -885,14 +1090,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             mask[j] = mask[j] * contrib[j];
             mask[j] = mask[j] / p;
         }
-        if (guiding) {
-            // maskForProbeQuery: the throughput's XYZ (one wavelength: xyz * mask).
-            float3 t = wv.xyz[0] * mask[0];
-            for (int j = 1; j < nl; ++j) t = t + wv.xyz[j] * mask[j];
-            query = fmax3(t, float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
-            // sum3 + PROBES_QUERY_EPS is a double sum (the constant is a double literal).
-            query = query / static_cast<float>(static_cast<double>(query.x + query.y + query.z) + 1.0e-6);
-        }
+        if (guiding) updateQuery();
         ro = c.pos;
         rd = sc.next;
         if (!sc.delta) ndd++;
@@ -936,7 +1134,9 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
     float radiance[MAXL], mask[MAXL];
     for (int j = 0; j < nl; ++j) { radiance[j] = 0.0f; mask[j] = 1.0f; }
     float3 ro = origin, rd = dir;
-    bool inside = inside0;
+    MediaCtx media;
+    media.inside = inside0;
+    media.vertex = origin;
     bool prevNEE = false;
     float lastPdf = 0.0f;
     float3 lastPos{0, 0, 0};
@@ -960,27 +1160,51 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         const float eps = INTERSECT_EPS(ndd);
         int id;
         float t;
-        TraceOne(S.bvh, ro, rd, eps, std::numeric_limits<float>::infinity(), false, S.margin > 0.0f, S.margin, &id, &t,
-                 eps * 0.5f);
+        TraceMedia(S, media, ro, rd, eps, &id, &t);
         if (id < 0) {
             (void)rng.rand1();
-            if (!inside) {
+            if (!media.inside) {
                 for (int j = 0; j < nl; ++j) {
                     const float le = P.sky ? InteractC(wv.lb[j], SkyLighting(rd)) : 0.0f;
                     radiance[j] = radiance[j] + mask[j] * le * wv.d65[j];
                 }
+                break;
             }
-            break;
+            // A miss inside a medium (bMissThinSurfaceInsideMedia_, render.cuh:109-126,
+            // 143-155): no emission, no NEE and no MIS update; weight 1; the
+            // zero-initialised hit record makes the next ray start at the
+            // world origin in the same direction, outside any medium; Trace's
+            // miss branch clears bDeltaSurface_, so the vertex counts.
+            float p = 1.0f;
+            if (ndd >= P.rr_min_depth) {
+                const float mm = MeanMask(mask, nl);
+                const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);
+                p = lerp(basic, 1.0f, 0.0f);
+                if (rng.rand1() > p) break;
+            }
+            float contrib[MAXL];
+            for (int j = 0; j < nl; ++j) contrib[j] = 1.0f / std::fmax(1.0f, 1e-12f);
+            for (int j = 0; j < nl; ++j) {
+                mask[j] = mask[j] * contrib[j];
+                mask[j] = mask[j] / p;
+            }
+            media = MediaCtx{};
+            media.vertex = ro;
+            ro = float3{0, 0, 0};
+            ndd++;
+            continue;
         }
         Ctx c = HitContext(S, id, ro, rd, t, eps * 0.5f);
         const float3 pos0 = c.pos;
         const int model = S.objs[id].material.shading_model;
-        SampleCtx sc = SampleBSDF(lb, rng, model, rd, c, inside);
+        SampleCtx sc = SampleBSDF(lb, wv.lam[0], rng, model, rd, c, media);
         float brdf[MAXL];
-        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], model, rd, sc.next, c);
+        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, sc.next, c, media);
+        const MediaCtx mediaIn = media;
+        media = MediaAfterHit(media, sc, id, c.pos);
+        const bool inside = media.inside;
         const float offEps = eps * std::fmax(1.0f, length(c.pos));
         c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
-        inside = sc.inside;
         if (c.lightIndex >= 0) {
             float misW = 1.0f;
             if (P.use_nee && depth > 0 && prevNEE) {
@@ -1066,7 +1290,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
                         const float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
                         const float w = PowerHeuristic(pLo, pBs);
                         for (int j = 0; j < nl; ++j) {
-                            const float bc = EvaluateBSDFWithCos(wv.lb[j], model, rd, L, c);
+                            const float bc = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, L, c, mediaIn);
                             const float resp = InteractC(wv.lb[j], F3(lg.emissive));
                             const float contribution = bc * resp * (G2 / pAT);
                             radiance[j] = radiance[j] + mask[j] * (contribution * w) * wv.d65[j];
@@ -1452,6 +1676,37 @@ void or_guide_export(const or_guide* g, ap_octahedron* probes, int32_t* volume, 
         const int32_t b[9] = {G.start.x, G.start.y, G.start.z, G.end.x, G.end.y, G.end.z, G.size.x, G.size.y, G.size.z};
         std::memcpy(bounds, b, sizeof(b));
     }
+}
+
+
+// Glass restatement at one wavelength, for its tests.
+float or_glass_ior(float lambda, float nd, float vd) { return IORFromAbbeNM(lambda, nd, vd); }
+float or_glass_sigma(float sigma_a, float lambda, float tilt) { return SigmaAFromTiltNM(sigma_a, lambda, tilt); }
+float or_glass_fresnel(float cosI, float cosT, float eta) { return FresnelDielectricExact(cosI, cosT, eta); }
+float or_glass_beer(float distance, float resp, float sigma_a) { return BeerLambert_Spectrum(distance, resp, sigma_a); }
+
+void or_glass_bsdf(const float* d, const float* n, const float* mat, float lambda, int inside, int depth,
+                   const float* vertex, const float* hit, uint64_t seed, float* out) {
+    Ctx c;
+    c.hit = true;
+    c.pos = {hit[0], hit[1], hit[2]};
+    c.normal = {n[0], n[1], n[2]};
+    c.gb = {{mat[4], mat[5], mat[6]}, {0, 0, 0}, 0.0f, 0.0f, 0.0f, mat[0], mat[2], mat[1], mat[3]};
+    MediaCtx m;
+    m.inside = inside != 0;
+    m.depth = depth;
+    m.vertex = {vertex[0], vertex[1], vertex[2]};
+    float lb[4];
+    QueryLambda(lambda, lb);
+    Rng rng = Rng::seeded(seed, 7u);
+    const float3 rd{d[0], d[1], d[2]};
+    const SampleCtx sc = SampleBSDF(lb, lambda, rng, AP_MAT_GLASS, rd, c, m);
+    const float brdf = EvaluateBSDFWithCos(lb, lambda, AP_MAT_GLASS, rd, sc.next, c, m);
+    out[0] = sc.next.x; out[1] = sc.next.y; out[2] = sc.next.z;
+    out[3] = sc.pdf;
+    out[4] = brdf;
+    out[5] = sc.inside ? 1.0f : 0.0f;
+    out[6] = sc.delta ? 1.0f : 0.0f;
 }
 
 }  // extern "C"

@@ -25,6 +25,11 @@ def lib():
         L.or_sdf_hit.restype = i32
         L.or_sdf_hit.argtypes = [vp, i32, vp, ctypes.c_float, vp, vp]
         L.or_bvh_free.argtypes = [vp]
+        f32 = ctypes.c_float
+        for nm in ("or_glass_ior", "or_glass_sigma", "or_glass_fresnel", "or_glass_beer"):
+            getattr(L, nm).restype = f32
+            getattr(L, nm).argtypes = [f32, f32, f32]
+        L.or_glass_bsdf.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp, ctypes.c_uint64, vp]
         L.or_bvh_node_count.restype = sz
         L.or_bvh_node_count.argtypes = [vp]
         L.or_bvh_export.argtypes = [vp, vp, vp]
@@ -553,3 +558,34 @@ def sdf_hit(objs, idx, ray, eps=5e-4):
     r = _rays(np.asarray(ray).reshape(1))
     ok = lib().or_sdf_hit(_p(_objs(objs)), int(idx), _p(r), eps, _p(pos), _p(nrm))
     return (pos, nrm) if ok else None
+
+
+def glass_ior(lam, nd, vd):
+    """IORFromAbbeNM(lambda, nd, Vd)."""
+    return lib().or_glass_ior(float(lam), float(nd), float(vd))
+
+
+def glass_sigma(sigma_a, lam, tilt):
+    """SigmaAFromTiltNM(sigma_a, lambda, tilt) with a tilt gain of 1."""
+    return lib().or_glass_sigma(float(sigma_a), float(lam), float(tilt))
+
+
+def glass_fresnel(cos_i, cos_t, eta):
+    """FresnelDielectricExact(cosI, cosT, eta = n_in / n_out)."""
+    return lib().or_glass_fresnel(float(cos_i), float(cos_t), float(eta))
+
+
+def glass_beer(distance, resp, sigma_a):
+    """BeerLambert_Spectrum(distance, albedo response, sigma_a)."""
+    return lib().or_glass_beer(float(distance), float(resp), float(sigma_a))
+
+
+def glass_bsdf(d, n, lam, ior=1.5, abbe=0.0, sigma_a=0.0, tilt=0.0, albedo=(1, 1, 1), inside=False, depth=0,
+               vertex=(0, 0, 0), hit=(0, 0, 0), seed=1):
+    """SampleBSDF then EvaluateBSDFWithCos of MAT_GLASS at one wavelength:
+    (next direction, pdf, brdf * cos, inside after, delta)."""
+    mat = _f32([ior, abbe, sigma_a, tilt, *albedo])
+    out = np.zeros(7, np.float32)
+    lib().or_glass_bsdf(_p(_f32(d)), _p(_f32(n)), _p(mat), float(lam), int(bool(inside)), int(depth),
+                        _p(_f32(vertex)), _p(_f32(hit)), int(seed), _p(out))
+    return out[:3].copy(), float(out[3]), float(out[4]), bool(out[5]), bool(out[6])

@@ -174,9 +174,9 @@ def test_render_rejects_unsupported_and_bad_arguments(gpu):
     bad = ap.Camera.make(BOX_CAM["origin"], BOX_CAM["direction"], BOX_CAM["fov_deg"], 0, 8)
     with pytest.raises(ap.ApertureError):
         scene.render(bad, ap.RenderParams.defaults(), hdr)
-    glass = objs.copy()
-    glass["shading_model"][0] = ap.MAT_GLASS
-    gscene = ap.SceneRenderer(glass)
+    textured = objs.copy()
+    textured["textures"][0] = (1, 0, 0, 0)   # an albedo texture handle: textures are not supported
+    gscene = ap.SceneRenderer(textured)
     with pytest.raises(ap.ApertureError) as e:
         gscene.render(cam, ap.RenderParams.defaults(), hdr)
     assert e.value.code == ap.ERR_UNSUPPORTED
