@@ -73,7 +73,11 @@ void ap_render_params_default(ap_render_params* params);
 /* Replaces SceneRenderer::Render(hdr, ldr, size, origin, dir, frameIndex, ...)'s
  * RenderCamera dispatch (core/scene.cu:308-346, 257-274): adds params->spp
  * frames (frame_index, frame_index+1, ...) to the framebuffers. Asynchronous
- * on `stream`; concurrent renders of one scene race on its path buffers. */
+ * on `stream`; concurrent renders of one scene race on its path buffers.
+ * Materials: diffuse, light, GGX and glass (the exact dielectric with
+ * n(lambda) from the Abbe number and Beer-Lambert absorption with the tilted
+ * sigmaA, material.cuh:122-169, 221-272; media as CalculateRadiance tracks
+ * them, render.cuh:84-160); textured materials return AP_ERR_UNSUPPORTED. */
 int ap_render(ap_scene* scene, const ap_camera* camera, const ap_render_params* params,
               const ap_framebuffers* fb, void* stream);
 
