@@ -192,6 +192,8 @@ EXPORTED_SYMBOLS = (
     "ap_batch_set_graph", "ap_batch_autotune", "ap_render_nrc", "ap_nrc_last_frame", "ap_batch_render_nrc",
     "ap_batch_nrc_last_frame", "ap_tonemap", "ap_sdf_cache_create", "ap_sdf_cache_destroy",
     "ap_sdf_cache_add_functor", "ap_sdf_cache_add_volume", "ap_sdf_cache_get_volume", "ap_scene_set_sdf_cache",
+    "ap_texture_cache_create", "ap_texture_cache_destroy", "ap_texture_cache_add_rgba8", "ap_texture_cache_count",
+    "ap_texture_sample", "ap_scene_set_texture_cache", "ap_mesh_load_obj",
 )
 
 
@@ -247,6 +249,13 @@ def lib():
         L.ap_sdf_cache_add_volume.argtypes = [vp, vp, vp, vp]
         L.ap_sdf_cache_get_volume.argtypes = [vp, ctypes.c_uint32, vp, vp]
         L.ap_scene_set_sdf_cache.argtypes = [vp, vp]
+        L.ap_texture_cache_create.argtypes = [vp]
+        L.ap_texture_cache_destroy.argtypes = [vp]
+        L.ap_texture_cache_add_rgba8.argtypes = [vp, vp, i32, i32, vp]
+        L.ap_texture_cache_count.argtypes = [vp, vp]
+        L.ap_texture_sample.argtypes = [vp, ctypes.c_uint64, vp, sz, vp]
+        L.ap_scene_set_texture_cache.argtypes = [vp, vp]
+        L.ap_mesh_load_obj.argtypes = [ctypes.c_char_p, vp, ctypes.c_float, vp, vp, sz, vp]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
@@ -527,6 +536,72 @@ def xyz_to_srgb(xyz):
 
 SDF_TORUS, SDF_VOLUME = 0, 1
 SDF_FUNCTOR_TORUS, SDF_FUNCTOR_DIAMOND = 0, 1
+
+
+class TextureCache:
+    """RGBA8 images behind GBuffer texture handles (ap_texture_cache_*)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().ap_texture_cache_create(ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def add_rgba8(self, image):
+        """An (height, width, 4) uint8 image, rows from the top; returns its handle (>= 1)."""
+        img = np.ascontiguousarray(image, np.uint8)
+        if img.ndim != 3 or img.shape[2] != 4:
+            raise ValueError("image must be (height, width, 4) uint8")
+        h = ctypes.c_uint64()
+        check(lib().ap_texture_cache_add_rgba8(self.handle, _ptr(img), img.shape[1], img.shape[0], ctypes.byref(h)))
+        return h.value
+
+    def count(self):
+        n = ctypes.c_uint32()
+        check(lib().ap_texture_cache_count(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def sample(self, handle, uv):
+        """tex2D of image `handle` at (n, 2) UV pairs, as the integrator samples it: (n, 4) RGBA."""
+        u = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((len(u), 4), np.float32)
+        check(lib().ap_texture_sample(self.handle, handle, _ptr(u), len(u), _ptr(out)))
+        return out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().ap_texture_cache_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_obj(path, position=(0.0, 0.0, 0.0), scale=1.0, textures=(0, 0, 0, 0)):
+    """LoadTexturedMesh: the triangles of an OBJ file as a SCENE_OBJECT_DTYPE array
+    (ap_mesh_load_obj); textures = albedo, normal, metallic, roughness handles."""
+    pos = np.ascontiguousarray(position, np.float32).reshape(3)
+    tex = np.ascontiguousarray(textures, np.uint64).reshape(4)
+    n = ctypes.c_size_t()
+    p = os.fsencode(path)
+    check(lib().ap_mesh_load_obj(p, _ptr(pos), float(scale), _ptr(tex), None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, SCENE_OBJECT_DTYPE)
+    if n.value:
+        check(lib().ap_mesh_load_obj(p, _ptr(pos), float(scale), _ptr(tex), _ptr(out), n.value, ctypes.byref(n)))
+    return out
+
+
+def load_image_rgba8(path):
+    """stbi_load(path, ..., 4): an image file as (height, width, 4) uint8 (needs Pillow)."""
+    from PIL import Image
+
+    with Image.open(path) as im:
+        return np.asarray(im.convert("RGBA"), np.uint8).copy()
 
 
 class SdfCache:
@@ -815,6 +890,11 @@ class SceneRenderer:
         """The SdfCache the scene's SDF_VOLUME objects read (None: none)."""
         self._sdf_cache = cache
         check(lib().ap_scene_set_sdf_cache(self.handle, cache.handle if cache is not None else None))
+
+    def set_texture_cache(self, cache):
+        """The TextureCache the scene's texture handles name (None: none)."""
+        self._tex_cache = cache
+        check(lib().ap_scene_set_texture_cache(self.handle, cache.handle if cache is not None else None))
 
     def set_traversal_mode(self, mode):
         check(lib().ap_scene_set_traversal_mode(self.handle, mode))

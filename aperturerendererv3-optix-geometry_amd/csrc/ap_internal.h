@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "aperture/ap_core.h"
+#include "aperture/ap_mesh.h"
 #include "ap_vec.h"
 
 namespace ap {
@@ -67,6 +68,10 @@ struct SdfVolume;  // ap_sdf.h
 // The SDF cache's device volume table and its capacity (ap_sdf.hip), and a
 // host view of volume k (samples in host memory).
 const SdfVolume* sdf_cache_table(const ap_sdf_cache* c);
+// Texture cache (ap_texture.hip): its device table of TexRec and image count.
+struct TexRec;
+const TexRec* texture_cache_table(const ap_texture_cache* c);
+uint32_t texture_cache_count(const ap_texture_cache* c);
 int32_t sdf_cache_capacity(const ap_sdf_cache* c);
 bool sdf_cache_host_volume(const ap_sdf_cache* c, int32_t k, SdfVolume& v);
 
@@ -128,6 +133,7 @@ struct ap_scene {
     ap::RenderState* render = nullptr;
     ap::GuideDevice* guide = nullptr;
     const ap_sdf_cache* sdf_cache = nullptr;  // volumes of SDF_VOLUME objects (ap_scene_set_sdf_cache)
+    const ap_texture_cache* tex_cache = nullptr;  // images of texture handles (ap_scene_set_texture_cache)
 };
 
 namespace ap {

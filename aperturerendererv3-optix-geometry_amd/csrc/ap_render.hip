@@ -37,6 +37,7 @@
 #include "ap_display.h"
 #include "ap_sdf.h"
 #include "ap_prim.h"
+#include "ap_texture.h"
 
 namespace ap {
 
@@ -57,6 +58,7 @@ struct MatIn {
     float specular, metallic, roughness;
     int32_t model, light_index;
     float ior, sigma_a, abbe, tilt;
+    int32_t albedo_tex;   // texture cache index of GBuffer.albedo_tex, or -1
 };
 
 // A path carries NL wavelengths: NL = 1 is the reference's one wavelength
@@ -148,6 +150,8 @@ struct alignas(16) BatchScene {
     const SdfVolume* volumes;  // the scene's SDF volume table
     int32_t n_volumes;
     int32_t pad;
+    const TexRec* textures;    // the scene's texture table
+    const float* uvs;          // per object: the triangle's three vertex UVs
 };
 
 // What the batch kernels see of a batch.
@@ -171,6 +175,8 @@ struct RenderState {
     int32_t n_mats = 0;
     std::vector<int32_t> analytic_objs;   // cube, sphere and SDF objects (the camera-inside test)
     bool has_glass = false;               // some object's shading model is MAT_GLASS
+    float* uvs = nullptr;                 // per object: six floats, the triangle's vertex UVs
+    int32_t max_tex = -1;                 // largest albedo texture index the materials name
     // path buffers
     size_t capacity = 0;
     void* paths = nullptr;            // PathT<NL>, kPathBytes per entry
@@ -208,7 +214,7 @@ constexpr int kEpsEntries = 1024;
 
 void render_state_free(RenderState* r) {
     if (!r) return;
-    void* ptrs[] = {r->geo, r->mats, r->lights, r->paths, r->media, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
+    void* ptrs[] = {r->geo, r->mats, r->lights, r->uvs, r->paths, r->media, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
                     r->hits, r->shadow_rays, r->shadow_payload, r->shadow_hits, r->counters, r->eps,
                     r->target_tmp, r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_qrgb, r->nrc_verts,
                     r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters, r->gsamples};
@@ -474,12 +480,15 @@ struct ShadeParams {
     const SdfVolume* volumes = nullptr;  // the scene's SDF volume table
     int32_t n_volumes = 0;
     MediaRec* media = nullptr;    // per path
+    const TexRec* textures = nullptr;  // the scene's texture table
+    const float* uvs = nullptr;        // per object: the triangle's three vertex UVs
 };
 
 // Hit attributes as the reference intersectors report them.
 struct HitInfo {
     f3 pos, normal;
     int32_t mat, light_index;
+    float bu, bv;    // a triangle's barycentrics of vertices 1 and 2
 };
 
 __device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict__ an, f3 o, f3 d, float t,
@@ -487,6 +496,8 @@ __device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict
     HitInfo h;
     h.mat = g.mat;
     h.light_index = g.light_index;
+    h.bu = 0.0f;
+    h.bv = 0.0f;
     if (g.type == AP_OBJ_TRIANGLE) {
         // IntersectTriangleObject (intersection.cuh:15-89): same MT, then the
         // interpolated normal turned toward the ray.
@@ -498,6 +509,8 @@ __device__ HitInfo hit_attributes(const GeoRec& g, const AnalyticRec* __restrict
         const f3 n = normalize(n0 * w + n1 * u + n2 * v);
         h.pos = o + d * t;
         h.normal = dot(d, n) > 0.0f ? -n : n;
+        h.bu = u;
+        h.bv = v;
         return h;
     }
     const AnalyticRec& a = an[g.analytic];
@@ -705,6 +718,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     int32_t n_lights = sp.n_lights;
     const SdfVolume* volumes = sp.volumes;
     int32_t n_volumes = sp.n_volumes;
+    const TexRec* textures = sp.textures;
+    const float* uvs = sp.uvs;
     if constexpr (MULTI) {
         scene_id = static_cast<uint16_t>(S.flags >> 16);
         const BatchScene& B = bv.scenes[scene_id];
@@ -715,6 +730,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         n_lights = B.n_lights;
         volumes = B.volumes;
         n_volumes = B.n_volumes;
+        textures = B.textures;
+        uvs = B.uvs;
     }
     // The MediaContext the ray was traced in.
     const bool inside = (S.flags & PATH_INSIDE) != 0u;
@@ -785,7 +802,26 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         const GeoRec g = geo[H.id];
         // SDF_EPS of the extension ray's depth: INTERSECT_EPS(ndd) / 2.
         const HitInfo hi = hit_attributes(g, analytic, o, d, H.t, volumes, n_volumes, eps[S.ndd] * 0.5f);
-        const MatRec M = mats[hi.mat];
+        // M0: the hit's GBuffer; M: with the albedo texture applied, as
+        // SampleBSDF and EvaluateBSDFWithCos apply it to their copy
+        // (material.cuh:69-74, 198-202). BSDF_PDF reads the GBuffer itself.
+        const MatRec M0 = mats[hi.mat];
+        MatRec M = M0;
+        if (M0.albedo_tex >= 0) {
+            float tu = 0.0f, tv = 0.0f;   // IntersectionContext::uv_ stays (0, 0) off triangles
+            if (g.type == AP_OBJ_TRIANGLE) {
+                const float* t = uvs + static_cast<size_t>(H.id) * 6;
+                const float w = 1.0f - hi.bu - hi.bv;
+                tu = t[0] * w + t[2] * hi.bu + t[4] * hi.bv;
+                tv = t[1] * w + t[3] * hi.bu + t[5] * hi.bv;
+            }
+            float c[4];
+            tex_sample(textures[M0.albedo_tex], tu, tv, c);
+            const f3 al{c[0] * c[0], c[1] * c[1], c[2] * c[2]};
+            const float la = max3(al);
+            M.albedo_basis = query_rgb_basis(rgb_lut, la > 1.0f ? al / la : al);
+            M.albedo_scale = la;
+        }
         const f3 N = hi.normal;
         f3 hp = hi.pos;
         if (sp.depth == 0 && sp.gsample) {
@@ -904,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                         if (rand1(rng) < g_alpha) {
                             const PgSample gs = pg_sample_hemisphere(*probe, rng, N, g_query);
                             if (dot(gs.dir, N) > 1.0e-12f) {
-                                const float p_bs = bsdf_pdf(M, lb, N, d, gs.dir);
+                                const float p_bs = bsdf_pdf(M0, lb, N, d, gs.dir);
                                 next_dir = gs.dir;
                                 for (int j = 0; j < NL; ++j) bsdf_cos[j] = eval_bsdf_cos(M, sc.lb[j], d, gs.dir, N);
                                 pdf = (1.0f - g_alpha) * fmaxf(p_bs, 1.0e-12f) + g_alpha * gs.pdf;
@@ -1012,7 +1048,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                         const float G = cosY / d2;
                         const MatRec& LM = mats[ls.mat];
                         const float pLo = area_to_solid_angle_pdf(hp, ls.point, ls.normal, ls.p_facet * ls.p_area);
-                        float pBs = fmaxf(bsdf_pdf(M, lb, N, d, L), 0.0f);
+                        float pBs = fmaxf(bsdf_pdf(M0, lb, N, d, L), 0.0f);
                         if (GUIDE && probe) {
                             const float p_pr = pg_hemisphere_pdf(*probe, L, N, g_query);
                             pBs = (1.0f - g_alpha) * pBs + g_alpha * p_pr;
@@ -1578,7 +1614,7 @@ __global__ void build_mats_kernel(const MatIn* __restrict__ in, int n, const f4*
     r.roughness = m.roughness;
     r.shading_model = m.model;
     r.light_index = m.light_index;
-    r.pad = 0;
+    r.albedo_tex = m.albedo_tex;
     r.emissive_rgb[0] = m.emissive.x;
     r.emissive_rgb[1] = m.emissive.y;
     r.emissive_rgb[2] = m.emissive.z;
@@ -1624,11 +1660,14 @@ inline m3 M3(const ap_float3x3& m) { return {F(m.x), F(m.y), F(m.z)}; }
 
 // Shading records of the scene for a triangle-material mode.
 int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
-    for (void* p : {static_cast<void*>(r->geo), static_cast<void*>(r->mats), static_cast<void*>(r->lights)})
+    for (void* p : {static_cast<void*>(r->geo), static_cast<void*>(r->mats), static_cast<void*>(r->lights),
+                    static_cast<void*>(r->uvs)})
         if (p) (void)hipFree(p);
-    r->geo = nullptr; r->mats = nullptr; r->lights = nullptr;
+    r->geo = nullptr; r->mats = nullptr; r->lights = nullptr; r->uvs = nullptr;
     const size_t n = s->objects.size();
     std::vector<GeoRec> geo(n);
+    std::vector<float> uvs(n * 6, 0.0f);
+    int32_t max_tex = -1;
     std::vector<MatIn> mats;
     std::map<std::vector<uint32_t>, int32_t> mat_index;
     auto intern = [&](const MatIn& m) {
@@ -1654,9 +1693,13 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
     for (size_t i = 0; i < n; ++i) {
         const ap_scene_object& o = s->objects[i];
         const ap_gbuffer& gb = o.material.texture.base;
-        if (gb.albedo_tex || gb.normal_tex || gb.metallic_tex || gb.roughness_tex)
-            return fail(AP_ERR_UNSUPPORTED, "ap_render: textured materials are not supported (object " + std::to_string(i) + ")");
+        // Only the albedo texture reaches the BSDFs (material.cuh:69-74, 198-202);
+        // the other handles are carried and not read, as in the reference.
+        if (gb.albedo_tex > static_cast<uint64_t>(AP_TEXTURE_CACHE_CAPACITY))
+            return fail(AP_ERR_INVALID_ARG, "ap_render: albedo texture handle out of range (object " + std::to_string(i) + ")");
         MatIn m;
+        m.albedo_tex = gb.albedo_tex ? static_cast<int32_t>(gb.albedo_tex - 1u) : -1;
+        max_tex = std::max(max_tex, m.albedo_tex);
         const bool ref_tri = o.type == AP_OBJ_TRIANGLE && tri_mode == AP_TRIANGLE_MATERIAL_REFERENCE;
         if (ref_tri) {  // default-constructed GBuffer (object.cuh:60-72)
             m.albedo = {1.0f, 1.0f, 1.0f};
@@ -1690,6 +1733,11 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
             const float* src[6] = {&p0.x, &e1.x, &e2.x, &n0.x, &n1.x, &n2.x};
             float* dst[6] = {g.p0, g.e1, g.e2, g.n0, g.n1, g.n2};
             for (int k = 0; k < 6; ++k) std::memcpy(dst[k], src[k], 12);
+            const ap_vertex* vs[3] = {&o.additional.v0, &o.additional.v1, &o.additional.v2};
+            for (int k = 0; k < 3; ++k) {
+                uvs[i * 6 + 2 * k] = vs[k]->uv.x;
+                uvs[i * 6 + 2 * k + 1] = vs[k]->uv.y;
+            }
         }
     }
     // Lights (scene.cu:1056-1080) with the OBB axes of GetOBBAxes (nee.cuh:24-31).
@@ -1714,6 +1762,8 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
     int rc;
     MatIn* d_in = nullptr;
     if ((rc = upload_vec(geo, &r->geo)) != AP_OK) return rc;
+    if ((rc = upload_vec(uvs, &r->uvs)) != AP_OK) return rc;
+    r->max_tex = max_tex;
     if ((rc = upload_vec(lights, &r->lights)) != AP_OK) return rc;
     if ((rc = upload_vec(mats, &d_in)) != AP_OK) return rc;
     if ((rc = dev_alloc(&r->mats, std::max<size_t>(mats.size(), 1))) != AP_OK) { (void)hipFree(d_in); return rc; }
@@ -1907,6 +1957,8 @@ int run_frames(const RunCtx& c) {
             sp.volumes = s->dev.volumes;
             sp.n_volumes = s->dev.n_volumes;
             sp.media = r->media;
+            sp.textures = texture_cache_table(s->tex_cache);
+            sp.uvs = r->uvs;
             if (guiding) {
                 hipLaunchKernelGGL((shade_kernel<NL, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
                                    r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
@@ -2039,6 +2091,8 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     }
     if (cache && r->has_glass)
         return fail(AP_ERR_UNSUPPORTED, "ap_render_nrc: glass materials are not supported with the radiance cache");
+    if (r->max_tex >= 0 && texture_cache_count(s->tex_cache) <= static_cast<uint32_t>(r->max_tex))
+        return fail(AP_ERR_INVALID_ARG, "ap_render: a material names a texture the scene's texture cache does not hold");
     const int32_t n_pix = cam->width * cam->height;
     // Frames per batch: about 4M paths in flight. With guiding, one frame at
     // a time: each frame samples the probes built from the frames before it
@@ -2205,6 +2259,10 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
         B.inside = camera_inside(s, r->analytic_objs, o) ? 1 : 0;
         B.volumes = s->dev.volumes;
         B.n_volumes = s->dev.n_volumes;
+        if (r->max_tex >= 0 && texture_cache_count(s->tex_cache) <= static_cast<uint32_t>(r->max_tex))
+            return fail(AP_ERR_INVALID_ARG, "ap_batch: a material names a texture the scene's texture cache does not hold");
+        B.textures = texture_cache_table(s->tex_cache);
+        B.uvs = r->uvs;
         b->h_devs[k] = s->dev;
         if (s->dev.has_sdf) b->shape.has_sdf = 1;
     }

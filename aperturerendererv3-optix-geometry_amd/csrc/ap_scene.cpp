@@ -401,6 +401,12 @@ int ap_scene_set_sdf_cache(ap_scene* s, const ap_sdf_cache* cache) {
     return AP_OK;
 }
 
+int ap_scene_set_texture_cache(ap_scene* s, const ap_texture_cache* cache) {
+    if (!s) return fail(AP_ERR_INVALID_ARG, "ap_scene_set_texture_cache: NULL scene");
+    s->tex_cache = cache;
+    return AP_OK;
+}
+
 int ap_trace_closest(const ap_scene* s, const ap_ray* rays, size_t n, ap_hit* hits, void* stream) {
     return trace_common(s, rays, n, hits, nullptr, false, stream, "ap_trace_closest");
 }

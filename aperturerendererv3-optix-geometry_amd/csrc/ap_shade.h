@@ -36,7 +36,7 @@ struct alignas(16) MatRec {
     float roughness;
     int32_t shading_model;
     int32_t light_index;
-    int32_t pad;
+    int32_t albedo_tex;      // texture cache index of the albedo texture, or -1
     float emissive_rgb[3];   // the GBuffer emissive (guiding endpoints query it with a MIS weight)
     int32_t pad2;
     float albedo_rgb[3];     // the GBuffer albedo (radiance-cache query records)

@@ -117,6 +117,13 @@ int or_render_screen(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
                      const ap_render_params* prm, float cull_margin, float* hdr, double* sums,
                      ap_screen_gbuffer* gbuffer, ap_screen_statistics* statistics, int threads);
 
+/* Albedo textures for or_render (or_render.cpp): n RGBA8 images of
+ * dims[2k] x dims[2k+1] texels (rows from the top) concatenated in rgba;
+ * texture handle k + 1 names image k. or_texture_sample: tex2D of image
+ * `index` at n UV pairs, RGBA out. */
+void or_texture_set(const uint8_t* rgba, const int32_t* dims, size_t n);
+void or_texture_sample(int32_t index, const float* uv, size_t n, float* out);
+
 /* Glass at one wavelength (or_render.cpp): IORFromAbbeNM, SigmaAFromTiltNM,
  * FresnelDielectricExact, BeerLambert_Spectrum, and SampleBSDF followed by
  * EvaluateBSDFWithCos of MAT_GLASS. mat = ior, abbe, sigma_a, tilt, albedo

@@ -98,6 +98,7 @@ struct GB {
     float3 albedo, emissive;
     float specular, metallic, roughness;
     float ior, sigmaA, abbe, tilt;
+    int albedoTex = -1;  // texture table index of albedoTex_, or -1
 };
 
 // MediaContext (material.cuh:12-18).
@@ -114,7 +115,57 @@ struct Ctx {  // IntersectionContext subset
     float distance = 0.0f;
     float3 pos{0, 0, 0}, normal{0, 0, 0};
     GB gb{};
+    float u = 0.0f, v = 0.0f;  // uv_
 };
+
+// ---- albedo textures (LoadCUDATexture's descriptor, texture_loader.cuh:36-46;
+// tex2D<float4> with the linear filtering the CUDA programming guide
+// documents: x_B = frac(u) W - 1/2, weights held with 8 fractional bits,
+// rounded to nearest here) ----
+struct OrTex {
+    std::vector<uint32_t> texels;  // RGBA8, R in the low byte, rows from the top
+    int w = 0, h = 0;
+};
+static std::vector<OrTex> g_textures;
+
+static float TexChannel(uint32_t t, int c) { return static_cast<float>((t >> (8 * c)) & 0xffu) / 255.0f; }
+
+static float TexWrap(float u) {
+    const float f = u - std::floor(u);
+    return (f >= 0.0f && f <= 1.0f) ? f : 0.0f;
+}
+
+static void TexSample(const OrTex& T, float u, float v, float out[4]) {
+    const float x = TexWrap(u) * static_cast<float>(T.w) - 0.5f;
+    const float y = TexWrap(v) * static_cast<float>(T.h) - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float a = std::floor((x - fx) * 256.0f + 0.5f) * (1.0f / 256.0f);
+    const float b = std::floor((y - fy) * 256.0f + 0.5f) * (1.0f / 256.0f);
+    int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+    if (i0 < 0) i0 += T.w;
+    if (j0 < 0) j0 += T.h;
+    const int i1 = (i0 + 1) % T.w, j1 = (j0 + 1) % T.h;
+    const uint32_t t00 = T.texels[static_cast<size_t>(j0) * T.w + i0];
+    const uint32_t t10 = T.texels[static_cast<size_t>(j0) * T.w + i1];
+    const uint32_t t01 = T.texels[static_cast<size_t>(j1) * T.w + i0];
+    const uint32_t t11 = T.texels[static_cast<size_t>(j1) * T.w + i1];
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    for (int c = 0; c < 4; ++c)
+        out[c] = w00 * TexChannel(t00, c) + w10 * TexChannel(t10, c) + w01 * TexChannel(t01, c) +
+                 w11 * TexChannel(t11, c);
+}
+
+// The GBuffer copy SampleBSDF and EvaluateBSDFWithCos shade with
+// (material.cuh:69-74, 198-202): albedo = texel.rgb^2 at uv_.
+static Ctx Textured(const Ctx& c) {
+    Ctx t = c;
+    if (c.gb.albedoTex >= 0 && c.gb.albedoTex < static_cast<int>(g_textures.size())) {
+        float s[4];
+        TexSample(g_textures[c.gb.albedoTex], c.u, c.v, s);
+        t.gb.albedo = {s[0] * s[0], s[1] * s[1], s[2] * s[2]};
+    }
+    return t;
+}
 
 struct Scene {
     const ap_scene_object* objs;
@@ -480,6 +531,8 @@ static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t, floa
     const ap_gbuffer& g = ob.material.texture.base;
     c.gb = {F3(g.albedo), F3(g.emissive), g.specular, g.metallic, g.roughness,
             g.ior, g.sigma_a, g.abbe_number, g.tilt_number};
+    const int tex = g.albedo_tex ? static_cast<int>(g.albedo_tex - 1u) : -1;
+    c.gb.albedoTex = tex;
     if (ob.type == AP_OBJ_TRIANGLE) {
         float tt, u, v;
         IntersectTriangleT(o, d, ob, tt, u, v);
@@ -491,8 +544,13 @@ static Ctx HitContext(const Scene& S, int idx, float3 o, float3 d, float t, floa
         const float3 n = normalize(n0 * w + n1 * u + n2 * v);
         c.pos = o + d * t;
         c.normal = dot(d, n) > 0.0f ? -n : n;
-        if (S.tri_mode == AP_TRIANGLE_MATERIAL_REFERENCE)  // a default GBuffer (object.cuh:60-72)
+        // uv_ (intersection.cuh:63-67).
+        c.u = ob.additional.v0.uv.x * w + ob.additional.v1.uv.x * u + ob.additional.v2.uv.x * v;
+        c.v = ob.additional.v0.uv.y * w + ob.additional.v1.uv.y * u + ob.additional.v2.uv.y * v;
+        if (S.tri_mode == AP_TRIANGLE_MATERIAL_REFERENCE) {  // a default GBuffer but the albedo texture (:58-61)
             c.gb = {{1, 1, 1}, {0, 0, 0}, 0.0f, 0.0f, 0.0f, 1.5f, 100.0f, 20.0f, 0.0f};
+            c.gb.albedoTex = tex;
+        }
         return c;
     }
     const float3x3 w2o = M3(ob.world_to_object), o2w = M3(ob.object_to_world);
@@ -964,9 +1022,10 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             gs->model = model;
             gs->hit = 1;
         }
-        SampleCtx sc = SampleBSDF(lb, wv.lam[0], rng, model, rd, c, media);
+        const Ctx ct = Textured(c);
+        SampleCtx sc = SampleBSDF(lb, wv.lam[0], rng, model, rd, ct, media);
         float brdf[MAXL];
-        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, sc.next, c, media);
+        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, sc.next, ct, media);
         const MediaCtx mediaIn = media;
         media = MediaAfterHit(media, sc, id, c.pos);
         const bool inside = media.inside;
@@ -986,7 +1045,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                             const float pBS = BSDF_PDF(lb, model, c.gb, c.normal, rd, gs.dir);
                             sc.next = gs.dir;
                             for (int j = 0; j < nl; ++j)
-                                brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, gs.dir, c, mediaIn);
+                                brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, gs.dir, ct, mediaIn);
                             sc.pdf = (1.0f - alpha) * std::fmax(pBS, 1.0e-12f) + alpha * gs.pdf;
                         } else {
                             probe = nullptr;
@@ -1061,7 +1120,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
                         }
                         const float w = PowerHeuristic(pLo, pBs);
                         for (int j = 0; j < nl; ++j) {
-                            const float bc = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, L, c, mediaIn);
+                            const float bc = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, L, ct, mediaIn);
                             const float resp = InteractC(wv.lb[j], F3(lg.emissive));
                             const float contribution = bc * resp * (G2 / pAT);
                             radiance[j] = radiance[j] + mask[j] * (contribution * w) * wv.d65[j];
@@ -1197,9 +1256,10 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         Ctx c = HitContext(S, id, ro, rd, t, eps * 0.5f);
         const float3 pos0 = c.pos;
         const int model = S.objs[id].material.shading_model;
-        SampleCtx sc = SampleBSDF(lb, wv.lam[0], rng, model, rd, c, media);
+        const Ctx ct = Textured(c);
+        SampleCtx sc = SampleBSDF(lb, wv.lam[0], rng, model, rd, ct, media);
         float brdf[MAXL];
-        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, sc.next, c, media);
+        for (int j = 0; j < nl; ++j) brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, sc.next, ct, media);
         const MediaCtx mediaIn = media;
         media = MediaAfterHit(media, sc, id, c.pos);
         const bool inside = media.inside;
@@ -1290,7 +1350,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
                         const float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
                         const float w = PowerHeuristic(pLo, pBs);
                         for (int j = 0; j < nl; ++j) {
-                            const float bc = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, L, c, mediaIn);
+                            const float bc = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, L, ct, mediaIn);
                             const float resp = InteractC(wv.lb[j], F3(lg.emissive));
                             const float contribution = bc * resp * (G2 / pAT);
                             radiance[j] = radiance[j] + mask[j] * (contribution * w) * wv.d65[j];
@@ -1707,6 +1767,26 @@ void or_glass_bsdf(const float* d, const float* n, const float* mat, float lambd
     out[4] = brdf;
     out[5] = sc.inside ? 1.0f : 0.0f;
     out[6] = sc.delta ? 1.0f : 0.0f;
+}
+
+// Albedo texture images for renders (index k = handle k + 1): n RGBA8
+// images of dims[2k] x dims[2k+1] texels concatenated in rgba.
+void or_texture_set(const uint8_t* rgba, const int32_t* dims, size_t n) {
+    g_textures.clear();
+    size_t off = 0;
+    for (size_t k = 0; k < n; ++k) {
+        OrTex t;
+        t.w = dims[2 * k];
+        t.h = dims[2 * k + 1];
+        t.texels.resize(static_cast<size_t>(t.w) * t.h);
+        std::memcpy(t.texels.data(), rgba + off, t.texels.size() * 4);
+        off += t.texels.size() * 4;
+        g_textures.push_back(std::move(t));
+    }
+}
+
+void or_texture_sample(int32_t index, const float* uv, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) TexSample(g_textures[index], uv[2 * i], uv[2 * i + 1], out + 4 * i);
 }
 
 }  // extern "C"

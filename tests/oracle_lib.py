@@ -30,6 +30,8 @@ def lib():
             getattr(L, nm).restype = f32
             getattr(L, nm).argtypes = [f32, f32, f32]
         L.or_glass_bsdf.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp, ctypes.c_uint64, vp]
+        L.or_texture_set.argtypes = [vp, vp, sz]
+        L.or_texture_sample.argtypes = [i32, vp, sz, vp]
         L.or_bvh_node_count.restype = sz
         L.or_bvh_node_count.argtypes = [vp]
         L.or_bvh_export.argtypes = [vp, vp, vp]
@@ -589,3 +591,20 @@ def glass_bsdf(d, n, lam, ior=1.5, abbe=0.0, sigma_a=0.0, tilt=0.0, albedo=(1, 1
     lib().or_glass_bsdf(_p(_f32(d)), _p(_f32(n)), _p(mat), float(lam), int(bool(inside)), int(depth),
                         _p(_f32(vertex)), _p(_f32(hit)), int(seed), _p(out))
     return out[:3].copy(), float(out[3]), float(out[4]), bool(out[5]), bool(out[6])
+
+
+def texture_set(images):
+    """Albedo textures for oracle renders: a list of (h, w, 4) uint8 images;
+    texture handle k + 1 names images[k]."""
+    imgs = [np.ascontiguousarray(i, np.uint8) for i in images]
+    dims = np.array([[i.shape[1], i.shape[0]] for i in imgs], np.int32).reshape(-1)
+    flat = np.concatenate([i.reshape(-1) for i in imgs]) if imgs else np.zeros(1, np.uint8)
+    lib().or_texture_set(_p(flat), _p(dims if len(dims) else np.zeros(2, np.int32)), len(imgs))
+
+
+def texture_sample(index, uv):
+    """tex2D of texture `index` (0-based) at (n, 2) UV pairs: (n, 4) RGBA."""
+    u = _f32(uv, (-1, 2))
+    out = np.zeros((len(u), 4), np.float32)
+    lib().or_texture_sample(int(index), _p(u), len(u), _p(out))
+    return out

@@ -175,11 +175,11 @@ def test_render_rejects_unsupported_and_bad_arguments(gpu):
     with pytest.raises(ap.ApertureError):
         scene.render(bad, ap.RenderParams.defaults(), hdr)
     textured = objs.copy()
-    textured["textures"][0] = (1, 0, 0, 0)   # an albedo texture handle: textures are not supported
+    textured["textures"][0] = (1, 0, 0, 0)   # an albedo texture handle, and no texture cache bound
     gscene = ap.SceneRenderer(textured)
     with pytest.raises(ap.ApertureError) as e:
         gscene.render(cam, ap.RenderParams.defaults(), hdr)
-    assert e.value.code == ap.ERR_UNSUPPORTED
+    assert e.value.code == ap.ERR_INVALID_ARG
     with pytest.raises(ap.ApertureError) as e:
         scene.render(cam, ap.RenderParams.defaults(wavelengths=3), hdr)
     assert e.value.code == ap.ERR_INVALID_ARG
