@@ -77,7 +77,8 @@ void ap_render_params_default(ap_render_params* params);
  * Materials: diffuse, light, GGX and glass (the exact dielectric with
  * n(lambda) from the Abbe number and Beer-Lambert absorption with the tilted
  * sigmaA, material.cuh:122-169, 221-272; media as CalculateRadiance tracks
- * them, render.cuh:84-160); textured materials return AP_ERR_UNSUPPORTED. */
+ * them, render.cuh:84-160); albedo textures through the scene's texture cache
+ * (ap_mesh.h). */
 int ap_render(ap_scene* scene, const ap_camera* camera, const ap_render_params* params,
               const ap_framebuffers* fb, void* stream);
 
