@@ -255,7 +255,7 @@ def lib():
         L.ap_texture_cache_count.argtypes = [vp, vp]
         L.ap_texture_sample.argtypes = [vp, ctypes.c_uint64, vp, sz, vp]
         L.ap_scene_set_texture_cache.argtypes = [vp, vp]
-        L.ap_mesh_load_obj.argtypes = [ctypes.c_char_p, vp, ctypes.c_float, vp, vp, sz, vp]
+        L.ap_mesh_load_obj.argtypes = [ctypes.c_char_p, vp, ctypes.c_float, vp, i32, vp, sz, vp]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
@@ -582,17 +582,24 @@ class TextureCache:
             pass
 
 
-def load_obj(path, position=(0.0, 0.0, 0.0), scale=1.0, textures=(0, 0, 0, 0)):
+MESH_REFERENCE_PROXY = 1
+
+
+def load_obj(path, position=(0.0, 0.0, 0.0), scale=1.0, textures=(0, 0, 0, 0), reference_proxy=False):
     """LoadTexturedMesh: the triangles of an OBJ file as a SCENE_OBJECT_DTYPE array
-    (ap_mesh_load_obj); textures = albedo, normal, metallic, roughness handles."""
+    (ap_mesh_load_obj); textures = albedo, normal, metallic, roughness handles.
+    Proxy centres are the triangles' centroids unless reference_proxy (the
+    reference's mesh position for every triangle, one BVH leaf per mesh)."""
     pos = np.ascontiguousarray(position, np.float32).reshape(3)
     tex = np.ascontiguousarray(textures, np.uint64).reshape(4)
+    flags = MESH_REFERENCE_PROXY if reference_proxy else 0
     n = ctypes.c_size_t()
     p = os.fsencode(path)
-    check(lib().ap_mesh_load_obj(p, _ptr(pos), float(scale), _ptr(tex), None, 0, ctypes.byref(n)))
+    check(lib().ap_mesh_load_obj(p, _ptr(pos), float(scale), _ptr(tex), flags, None, 0, ctypes.byref(n)))
     out = np.zeros(n.value, SCENE_OBJECT_DTYPE)
     if n.value:
-        check(lib().ap_mesh_load_obj(p, _ptr(pos), float(scale), _ptr(tex), _ptr(out), n.value, ctypes.byref(n)))
+        check(lib().ap_mesh_load_obj(p, _ptr(pos), float(scale), _ptr(tex), flags, _ptr(out), n.value,
+                                     ctypes.byref(n)))
     return out
 
 

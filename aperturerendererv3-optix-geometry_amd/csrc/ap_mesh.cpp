@@ -178,7 +178,7 @@ int read_obj(const char* path, Obj& o, std::string* err) {
 using namespace ap;
 
 extern "C" int ap_mesh_load_obj(const char* path, const float position[3], float scale, const uint64_t textures[4],
-                                ap_scene_object* out, size_t capacity, size_t* count) {
+                                int32_t flags, ap_scene_object* out, size_t capacity, size_t* count) {
     if (!path || !position || !count) return fail(AP_ERR_INVALID_ARG, "ap_mesh_load_obj: NULL argument");
     *count = 0;
     Obj o;
@@ -242,7 +242,13 @@ extern "C" int ap_mesh_load_obj(const char* path, const float position[3], float
         // UpdateProxy (object.cuh:309-342) for a triangle.
         f3 w[3];
         for (int k = 0; k < 3; ++k) w[k] = mul(I, p[k]) + c;
-        s.proxy.center = s.center;
+        // The reference's centre (the mesh position), or the centroid the builder can bin.
+        if (flags & AP_MESH_REFERENCE_PROXY) {
+            s.proxy.center = s.center;
+        } else {
+            const f3 m = ((w[0] + w[1]) + w[2]) / 3.0f;
+            s.proxy.center = {m.x, m.y, m.z};
+        }
         s.proxy.bound_min = {std::fmin(std::fmin(w[0].x, w[1].x), w[2].x), std::fmin(std::fmin(w[0].y, w[1].y), w[2].y),
                              std::fmin(std::fmin(w[0].z, w[1].z), w[2].z)};
         s.proxy.bound_max = {std::fmax(std::fmax(w[0].x, w[1].x), w[2].x), std::fmax(std::fmax(w[0].y, w[1].y), w[2].y),

@@ -18,6 +18,7 @@ reference traversal on a bounded random sample of the same rays.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -209,6 +210,101 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     return out
 
 
+def materials_bench(ap, spp, stream, W=1920, H=1080):
+    """Frame time at 1920x1080 of the C1 closed box holding glass and a
+    textured mesh: a dispersive glass sphere (IOR 1.5, Abbe 20), a rotated
+    IOR-2.4 glass box, a thin glass quad, and a 32x32-quad panel read from an
+    OBJ file by ap_mesh_load_obj with a 1024^2 RGBA8 albedo texture; max
+    depth 12, RR from depth 3, sky on; one and four wavelengths. The same box
+    without them is timed alongside."""
+    import tempfile
+
+    n = 32
+    lines = []
+    for j in range(n + 1):
+        for i in range(n + 1):
+            lines.append(f"v {-0.9 + 1.8 * i / n:.6f} {-0.9 + 1.8 * j / n:.6f} "
+                         f"{-0.93 + 0.03 * math.sin(i * 0.9 + j * 0.4):.6f}")
+            lines.append(f"vt {2.0 * i / n:.6f} {2.0 * j / n:.6f}")
+    lines.append("vn 0 0 1")
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i + 1
+            lines.append(f"f {a}/{a}/1 {a + n + 1}/{a + n + 1}/1 {a + n + 2}/{a + n + 2}/1 {a + 1}/{a + 1}/1")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "panel.obj")
+        with open(path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        yy, xx = np.mgrid[0:1024, 0:1024]
+        img = np.stack([(xx // 64 + yy // 64) % 2 * 200 + 40, (xx * 255) // 1023, (yy * 255) // 1023,
+                        np.full_like(xx, 255)], -1).astype(np.uint8)
+        tex = ap.TextureCache()
+        handle = tex.add_rgba8(img)
+        mesh = ap.load_obj(path, (0.0, 0.0, 0.0), 1.0, (handle, 0, 0, 0))
+    def glass(o, ior, sigma_a, albedo, abbe):
+        o["shading_model"] = ap.MAT_GLASS
+        o["ior"], o["sigma_a"], o["albedo"], o["abbe_number"] = ior, sigma_a, albedo, abbe
+        return o
+    box = ap.synth_box_scene(1, 3, 15.0)
+
+    def blank():
+        # A default SceneObject: GBuffer defaults, identity transforms, no light.
+        o = np.zeros(1, ap.SCENE_OBJECT_DTYPE)
+        o["albedo"], o["ior"], o["sigma_a"], o["normal"], o["abbe_number"] = 1.0, 1.5, 100.0, (0, 0, 1), 20.0
+        o["world_to_object"] = o["object_to_world"] = np.eye(3, dtype=np.float32)
+        o["cached_volume_index"] = o["object_index"] = o["light_index"] = -1
+        o["visible"] = 1
+        return o
+
+    sph, cube, quad = blank(), blank(), blank()
+    sph["type"], sph["center"], sph["extent"] = ap.OBJ_SPHERE, (-0.45, -0.55, 0.2), 0.3
+    c, s_ = math.cos(0.6), math.sin(0.6)
+    rot = np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]], np.float32)
+    cube["type"], cube["center"], cube["extent"] = ap.OBJ_CUBE, (0.45, -0.6, 0.0), (0.2, 0.3, 0.2)
+    cube["object_to_world"], cube["world_to_object"] = rot, rot.T
+    quad["type"], quad["center"], quad["extent"] = ap.OBJ_TRIANGLE, (0.0, 0.3, 0.4), 1.0
+    quad["v0_position"], quad["v1_position"], quad["v2_position"] = (-0.4, -0.3, 0), (0.4, -0.3, 0), (0.0, 0.35, 0)
+    for k in ("v0_normal", "v1_normal", "v2_normal"):
+        quad[k] = (0.0, 0.0, 1.0)
+    glass(sph, 1.5, 1.0, (0.9, 0.95, 1.0), 20.0)
+    glass(cube, 2.4, 2.0, (1.0, 0.8, 0.6), 0.0)
+    glass(quad, 1.45, 1.0, (1.0, 1.0, 1.0), 30.0)
+    # UpdateProxy: the triangle's vertex box; the rotated cube's box of its rotated extent; the sphere's radius.
+    ps = np.stack([quad[k][0] + quad["center"][0] for k in ("v0_position", "v1_position", "v2_position")])
+    quad["proxy_min"], quad["proxy_max"] = ps.min(0), ps.max(0)
+    half = np.abs(rot) @ cube["extent"][0]
+    cube["proxy_min"], cube["proxy_max"] = cube["center"][0] - half, cube["center"][0] + half
+    sph["proxy_min"], sph["proxy_max"] = sph["center"][0] - 0.3, sph["center"][0] + 0.3
+    for o in (sph, cube, quad):
+        o["proxy_center"] = o["center"]
+    objs = np.concatenate([box, sph, cube, quad, mesh]).astype(ap.SCENE_OBJECT_DTYPE)
+    cam = ap.Camera.make([0.0, 0.0, 3.4], [0.0, 0.0, -1.0], 40.0, W, H)
+    hdr = ap.DeviceBuffer(W * H * 12)
+    params = dict(max_depth=12, rr_min_depth=3, sky=1)
+    out = {"workload": f"C1 closed box + glass sphere (IOR 1.5, Abbe 20), rotated glass box (IOR 2.4), thin glass "
+                       f"triangle, {len(mesh)}-triangle OBJ panel with a 1024^2 albedo texture, 1920x1080, max depth "
+                       f"12, RR from depth 3, sky", "spp": spp}
+    for label, o in (("with_glass_and_texture", objs), ("plain_box", box)):
+        sc = ap.SceneRenderer(o)
+        sc.set_texture_cache(tex)
+        for wl in (1, 4):
+            sc.render(cam, ap.RenderParams.defaults(spp=1, wavelengths=wl, **params), hdr, stream=stream)
+            stream.synchronize()
+            e0, e1 = ap.Event(), ap.Event()
+            e0.record(stream)
+            sc.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, wavelengths=wl, **params), hdr,
+                      stream=stream)
+            e1.record(stream)
+            stream.synchronize()
+            ms = e0.elapsed_ms(e1) / spp
+            out[f"{label}_{wl}_wavelength{'s' if wl > 1 else ''}"] = {
+                "ms_per_spp": round(ms, 3), "samples_per_s": round(W * H / (ms / 1e3), 1)}
+        sc.close()
+    tex.close()
+    log(f"glass and textures: {out}")
+    return out
+
+
 def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc_warmup=0, nrc_passes=0,
                  nrc_boost=(8, 16)):
     """Samples/s of the wavefront path tracer at 1920x1080 on the heightfield
@@ -302,6 +398,7 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
         "guided_four_wavelengths": guided4,
         "four_wavelengths_with_gbuffer_and_statistics": with_records,
         "tonemap_1080p": tonemap,
+        "glass_and_textures_1080p": materials_bench(ap, spp, stream, W, H),
     }
     log(f"render: {out}")
     if nrc_spp > 0:

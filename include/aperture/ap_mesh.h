@@ -69,11 +69,18 @@ int ap_scene_set_texture_cache(ap_scene* scene, const ap_texture_cache* cache);
  * texture handles, vertex positions scaled by `scale`, normals as read
  * (else (0, 1, 0)), UV (u, 1 - v) (else (0, 0)); when the face's first
  * vertex has no normal, all three get normalize(cross(p1 - p0, p2 - p0)).
- * Proxies as UpdateProxy. With out == NULL only *count is set; otherwise up
- * to `capacity` objects are written and *count is the number of triangles.
+ * Proxy bounds as UpdateProxy. UpdateProxy leaves every triangle's proxy
+ * centre at the mesh position, so the reference's SAH builder, which bins
+ * centroids, puts the whole mesh in one leaf (bvh.cuh:232-234); by default
+ * the proxy centre here is the triangle's centroid ((p0 + p1 + p2) / 3 in
+ * world space), which gives the mesh a real tree and the same hits (ties at
+ * equal distance aside). AP_MESH_REFERENCE_PROXY keeps the reference's
+ * centres. With out == NULL only *count is set; otherwise up to `capacity`
+ * objects are written and *count is the number of triangles.
  * textures: albedo, normal, metallic, roughness handles (0 = none). */
+#define AP_MESH_REFERENCE_PROXY 1
 int ap_mesh_load_obj(const char* path, const float position[3], float scale, const uint64_t textures[4],
-                     ap_scene_object* out, size_t capacity, size_t* count);
+                     int32_t flags, ap_scene_object* out, size_t capacity, size_t* count);
 
 #ifdef __cplusplus
 }

@@ -135,8 +135,12 @@ def test_obj_triangles_match_an_independent_reading(tmp_path, crlf):
         w = [P[k] + np.float32(pos) for k in range(3)]
         assert np.array_equal(o["proxy_min"], np.minimum(np.minimum(w[0], w[1]), w[2]))
         assert np.array_equal(o["proxy_max"], np.maximum(np.maximum(w[0], w[1]), w[2]))
-        assert np.array_equal(o["proxy_center"], np.float32(pos))
+        assert np.array_equal(o["proxy_center"], ((w[0] + w[1]) + w[2]) / np.float32(3.0))
         assert o["light_index"] == -1 and o["object_index"] == -1 and o["visible"] == 1
+    ref = ap.load_obj(path, pos, scale, tex, reference_proxy=True)   # UpdateProxy's own centre
+    assert np.all(ref["proxy_center"] == np.float32(pos))
+    same = [f for f in ap.SCENE_OBJECT_DTYPE.names if f != "proxy_center"]
+    assert all(np.array_equal(ref[f], objs[f]) for f in same)
     # Quad 1 2 3 4: |v2 - v0|^2 = 2 < |v3 - v1|^2 = 2.0625, so (v0 v1 v2) (v0 v2 v3).
     assert np.array_equal(objs[1]["v2_position"], objs[2]["v1_position"])
     # Quad -3 -2 -1 -4 (vertices 6 7 8 5): 9.41 > 2, so (v0 v1 v3) (v1 v2 v3).
