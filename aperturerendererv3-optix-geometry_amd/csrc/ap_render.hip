@@ -83,10 +83,11 @@ static_assert(sizeof(PathT<1>) == 64, "PathT<1> is 64 B");
 static_assert(sizeof(PathT<4>) == 112, "PathT<4> is 112 B");
 constexpr size_t kPathBytes = sizeof(PathT<4>);
 
-// flags: PATH_* bits, the MediaContext depth in bits 4..15 (hits since the
+// flags: PATH_* bits (PATH_PREV_DELTA: the radiance cache's spread sum skips
+// the next segment), the MediaContext depth in bits 4..15 (hits since the
 // path last left a medium by a miss, saturating), a batch path's scene in
 // bits 16..31.
-enum : uint32_t { PATH_PREV_NEE = 1u, PATH_INSIDE = 2u, PATH_HERO_ONLY = 4u };
+enum : uint32_t { PATH_PREV_NEE = 1u, PATH_INSIDE = 2u, PATH_HERO_ONLY = 4u, PATH_PREV_DELTA = 8u };
 constexpr uint32_t kMediaDepthShift = 4u, kMediaDepthMax = 0xfffu;
 
 // The rest of a path's MediaContext (material.cuh:12-18), read only while
@@ -791,6 +792,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                     S.mask[j] = S.mask[j] / p_rr;
                 }
                 S.flags &= ~(PATH_INSIDE | (kMediaDepthMax << kMediaDepthShift));
+                S.flags |= PATH_PREV_DELTA;
                 S.ndd += 1;
                 if (sp.depth + 1 < sp.max_depth) {
                     emit_next = true;
@@ -989,10 +991,14 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             if (sp.depth == 0) {
                 P.a0 = dist2 / (12.566370614359172f * cos_i);
                 P.s = 0.0f;
-            } else {
+            } else if (!(S.flags & PATH_PREV_DELTA)) {
+                // A segment leaving a delta vertex (glass) or a miss inside a
+                // medium adds nothing: its pdf is a Dirac delta.
                 P.s = P.s + sqrtf(dist2 / (fmaxf(S.last_pdf, 1e-12f) * cos_i));
             }
-            const bool scatter = !pure_light && !(M.shading_model == SM_GGX && !(fmaxf(0.0f, dot(N, -d)) > 0.0f));
+            // Glass vertices neither end into the cache nor train it.
+            const bool scatter = !pure_light && !delta &&
+                                 !(M.shading_model == SM_GGX && !(fmaxf(0.0f, dot(N, -d)) > 0.0f));
             bool cache_end = false;
             if (scatter && !inside_next && P.s * P.s > nv.spread * P.a0) {
                 if (!(P.flags & NRC_TRAIN)) {
@@ -1119,6 +1125,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
                     S.mask[j] = S.mask[j] / p_rr;
                 }
                 if (!delta) S.ndd += 1;
+                S.flags = delta ? (S.flags | PATH_PREV_DELTA) : (S.flags & ~PATH_PREV_DELTA);
                 // EvaluateMaterial's MediaContext (material.cuh:410-411): this
                 // object, one more hit, the hit point before the offset.
                 const uint32_t depth_next = media_depth < kMediaDepthMax ? media_depth + 1u : kMediaDepthMax;
@@ -2089,8 +2096,6 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     if (r->tri_mode != prm->triangle_material) {
         if ((rc = build_shading(s, r, prm->triangle_material, rgb_lut)) != AP_OK) return rc;
     }
-    if (cache && r->has_glass)
-        return fail(AP_ERR_UNSUPPORTED, "ap_render_nrc: glass materials are not supported with the radiance cache");
     if (r->max_tex >= 0 && texture_cache_count(s->tex_cache) <= static_cast<uint32_t>(r->max_tex))
         return fail(AP_ERR_INVALID_ARG, "ap_render: a material names a texture the scene's texture cache does not hold");
     const int32_t n_pix = cam->width * cam->height;
@@ -2784,10 +2789,6 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
     const float* tables = nullptr;
     if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
     if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut)) != AP_OK) return rc;
-    for (ap_scene* sc : B.scenes)
-        if (sc->render->has_glass)
-            return fail(AP_ERR_UNSUPPORTED,
-                        "ap_batch_render_nrc: glass materials are not supported with the radiance cache");
     if ((rc = batch_nrc_prepare(&B, nrc_params, ns)) != AP_OK) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);

@@ -1200,6 +1200,8 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
     float lastPdf = 0.0f;
     float3 lastPos{0, 0, 0};
     float a0 = 0.0f, spreadSum = 0.0f;
+    bool prevDelta = false;  // the last vertex was glass, or a miss inside a medium
+    bool heroOnly = false;
     bool suffix = false;
     int nv = 0;
     float vrec[NRC_V][16], vR[NRC_V][MAXL], vM[NRC_V][MAXL];
@@ -1250,6 +1252,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             media = MediaCtx{};
             media.vertex = ro;
             ro = float3{0, 0, 0};
+            prevDelta = true;
             ndd++;
             continue;
         }
@@ -1282,10 +1285,11 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             if (depth == 0) {
                 a0 = dist2 / (12.566370614359172f * cosI);
                 spreadSum = 0.0f;
-            } else {
+            } else if (!prevDelta) {  // a segment leaving a delta vertex adds nothing
                 spreadSum = spreadSum + std::sqrt(dist2 / (std::fmax(lastPdf, 1e-12f) * cosI));
             }
-            const bool scatter = !sc.pureLight &&
+            prevDelta = sc.delta;
+            const bool scatter = !sc.pureLight && !sc.delta &&
                                  !(model == AP_MAT_GGX && !(std::fmax(0.0f, dot(c.normal, -rd)) > 0.0f));
             if (scatter && !inside && spreadSum * spreadSum > N.spread * a0) {
                 if (!train) {
@@ -1364,6 +1368,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         lastPos = c.pos;
         float contrib[MAXL];
         for (int j = 0; j < nl; ++j) contrib[j] = brdf[j] / std::fmax(sc.pdf, 1e-12f);
+        HeroOnlyAtDispersion(wv, nl, sc, c.gb, heroOnly, contrib);
         float p = 1.0f;
         if (ndd >= P.rr_min_depth) {
             const float basic = clampf(mm * 0.5f + mm * 0.5f, 0.05f, 0.95f);

@@ -9,7 +9,8 @@ a glass torus and a thin glass quad, in both triangle material modes, with
 one and four wavelengths; the same with the camera inside a glass sphere and
 inside a glass sphere that encloses the whole box; under directional guiding
 over several frames; in a batch next to other scenes (bit-identical to the
-scene-by-scene renders). The radiance cache refuses glass scenes. With four
+scene-by-scene renders); with the radiance cache (frozen), the oracle's
+cache renderer to the bars of its own tests. With four
 wavelengths, whose secondaries end at a dispersive interface, the image
 agrees in expectation with the one-wavelength estimator.
 """
@@ -21,6 +22,7 @@ import oracle_lib as ol
 import scenes
 import test_batch_gpu as bg
 import test_guiding_gpu as gg
+import test_nrc_render_gpu as nr
 import test_sdf_cpu as sc
 
 pytestmark = pytest.mark.gpu
@@ -130,15 +132,15 @@ def test_batch_with_glass_equals_single_scene_renders(gpu):
         s.close()
 
 
-def test_radiance_cache_refuses_glass(gpu):
-    scene = ap.SceneRenderer(glass_scene())
-    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 16, 16)
-    cache = ap.RadianceCache(seed=1)
-    hdr = ap.DeviceBuffer(16 * 16 * 12)
-    hdr.zero()
-    with pytest.raises(ap.ApertureError):
-        scene.render_nrc(cam, ap.RenderParams.defaults(spp=1, max_depth=6), cache, ap.NrcRenderParams.defaults(), hdr)
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_radiance_cache_with_glass_matches_the_oracle(gpu, wavelengths):
+    # Glass vertices neither end into the cache nor record training samples,
+    # and the spread sum skips the segment after one (its pdf is a delta).
+    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 48, 40)
+    prm = ap.RenderParams.defaults(spp=2, max_depth=12, rr_min_depth=4, seed=7, wavelengths=wavelengths)
+    scene, cache = nr.frozen_parity(glass_scene(), cam, prm, f"box with glass, {wavelengths} wavelengths")
     scene.close()
+    cache.close()
 
 
 def test_hero_wavelengths_agree_in_expectation(gpu):
