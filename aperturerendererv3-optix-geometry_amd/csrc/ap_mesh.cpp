@@ -11,6 +11,7 @@
 // there, but LoadTexturedMesh walks every shape in file order, so the
 // triangles come out in face order either way.
 #include <cerrno>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -96,8 +97,90 @@ bool valid(const Obj& o, const Idx& i) {
            (i.vn < 0 || static_cast<size_t>(i.vn) * 3 + 2 < o.vn.size());
 }
 
-// exportGroupsToShape's triangulation of one face (tiny_obj_loader.h:7157-7290
-// for three and four vertices): quads split along the shorter diagonal.
+// Even-odd point-in-polygon test of (tx, ty) (pnpoly, tiny_obj_loader.h:7113-7123).
+bool point_in_triangle(const float vx[3], const float vy[3], float tx, float ty) {
+    bool c = false;
+    for (int i = 0, j = 2; i < 3; j = i++)
+        if (((vy[i] > ty) != (vy[j] > ty)) && (tx < (vx[j] - vx[i]) * (ty - vy[i]) / (vy[j] - vy[i]) + vx[i])) c = !c;
+    return c;
+}
+
+// tinyobjloader's built-in ear clipping of a face of five or more vertices
+// (tiny_obj_loader.h:7431-7610): project on the two axes the first corner's
+// normal does not dominate, then walk the polygon from a guessed vertex,
+// clipping a corner whose turn agrees with its sign test and whose triangle
+// holds no other vertex, until three are left or a full pass clips nothing.
+void ear_clip(Obj& o, const std::vector<Idx>& face) {
+    const float* V = o.v.data();
+    auto P = [&](const Idx& i, int a) { return V[static_cast<size_t>(i.v) * 3 + a]; };
+    const size_t n0 = face.size();
+    int axes[2] = {1, 2};
+    for (size_t k = 0; k < n0; ++k) {
+        const Idx& a = face[k % n0];
+        const Idx& b = face[(k + 1) % n0];
+        const Idx& c = face[(k + 2) % n0];
+        const float e0x = P(b, 0) - P(a, 0), e0y = P(b, 1) - P(a, 1), e0z = P(b, 2) - P(a, 2);
+        const float e1x = P(c, 0) - P(b, 0), e1y = P(c, 1) - P(b, 1), e1z = P(c, 2) - P(b, 2);
+        const float cx = std::fabs(e0y * e1z - e0z * e1y);
+        const float cy = std::fabs(e0z * e1x - e0x * e1z);
+        const float cz = std::fabs(e0x * e1y - e0y * e1x);
+        if (cx > FLT_EPSILON || cy > FLT_EPSILON || cz > FLT_EPSILON) {
+            if (!(cx > cy && cx > cz)) {
+                axes[0] = 0;
+                if (cz > cx && cz > cy) axes[1] = 1;
+            }
+            break;
+        }
+    }
+    std::vector<Idx> rem = face;
+    size_t guess = 0;
+    size_t iterations = rem.size();
+    size_t previous = rem.size();
+    while (rem.size() > 3 && iterations > 0) {
+        const size_t n = rem.size();
+        if (guess >= n) guess -= n;
+        if (previous != n) {
+            previous = n;
+            iterations = n;
+        } else {
+            --iterations;
+        }
+        Idx ind[3];
+        float vx[3], vy[3];
+        for (int k = 0; k < 3; ++k) {
+            ind[k] = rem[(guess + k) % n];
+            vx[k] = P(ind[k], axes[0]);
+            vy[k] = P(ind[k], axes[1]);
+        }
+        const float e0x = vx[1] - vx[0], e0y = vy[1] - vy[0];
+        const float e1x = vx[2] - vx[1], e1y = vy[2] - vy[1];
+        const float cross = e0x * e1y - e0y * e1x;
+        const float area = (vx[0] * vy[1] - vy[0] * vx[1]) * 0.5f;
+        if (cross * area < 0.0f) {
+            guess += 1;
+            continue;
+        }
+        bool overlap = false;
+        for (size_t other = 3; other < n; ++other) {
+            const Idx& t = rem[(guess + other) % n];
+            if (point_in_triangle(vx, vy, P(t, axes[0]), P(t, axes[1]))) {
+                overlap = true;
+                break;
+            }
+        }
+        if (overlap) {
+            guess += 1;
+            continue;
+        }
+        o.tri.insert(o.tri.end(), ind, ind + 3);
+        rem.erase(rem.begin() + static_cast<std::ptrdiff_t>((guess + 1) % n));
+    }
+    if (rem.size() == 3) o.tri.insert(o.tri.end(), rem.begin(), rem.end());
+}
+
+// exportGroupsToShape's triangulation of one face (tiny_obj_loader.h:7157-7290):
+// triangles as they are, quads split along the shorter diagonal, larger
+// faces ear-clipped.
 int add_face(Obj& o, const std::vector<Idx>& f, std::string* err) {
     const size_t n = f.size();
     if (n < 3) return AP_OK;  // degenerate face: skipped
@@ -111,8 +194,8 @@ int add_face(Obj& o, const std::vector<Idx>& f, std::string* err) {
         return AP_OK;
     }
     if (n > 4) {
-        *err = "faces of more than four vertices are not supported";
-        return AP_ERR_UNSUPPORTED;
+        ear_clip(o, f);
+        return AP_OK;
     }
     const float* V = o.v.data();
     auto at = [&](int k, int c) { return V[static_cast<size_t>(f[k].v) * 3 + c]; };

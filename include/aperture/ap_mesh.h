@@ -61,9 +61,10 @@ int ap_scene_set_texture_cache(ap_scene* scene, const ap_texture_cache* cache);
  * the reference): v / vt / vn / f records (1-based or negative relative
  * indices; v, v/vt, v//vn, v/vt/vn), numbers parsed to double then float,
  * quads split along their shorter diagonal (v0 v1 v2 + v0 v2 v3 when
- * |v2 - v0|^2 < |v3 - v1|^2, else v0 v1 v3 + v1 v2 v3), faces of fewer than
- * three vertices skipped; faces of more than four vertices return
- * AP_ERR_UNSUPPORTED. Each triangle becomes one SceneObject: centre
+ * |v2 - v0|^2 < |v3 - v1|^2, else v0 v1 v3 + v1 v2 v3), larger faces
+ * ear-clipped as tinyobjloader's built-in triangulation clips them, faces of
+ * fewer than three vertices skipped, an index out of range an error
+ * (AP_ERR_INVALID_ARG). Each triangle becomes one SceneObject: centre
  * `position`, extent (scale, scale, scale), identity rotation, a GGX
  * material (white, roughness 0.4, metallic 0, specular 0) carrying the four
  * texture handles, vertex positions scaled by `scale`, normals as read

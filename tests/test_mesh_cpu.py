@@ -4,8 +4,10 @@ ap_mesh_load_obj (host code of the library, csrc/ap_mesh.cpp) against an
 independent reading of the same OBJ files here: the reference's
 LoadTexturedMesh (core/helper/scene_helper.h:453-564) over tinyobjloader
 2.0's reading (triangles as written, quads split along the shorter
-diagonal, negative indices, v / v/vt / v//vn / v/vt/vn vertices, faces of
-fewer than three vertices skipped), field by field and bit for bit.
+diagonal, larger faces ear-clipped, negative indices, v / v/vt / v//vn /
+v/vt/vn vertices, faces of fewer than three vertices skipped), field by
+field and bit for bit; ear-clipped convex and concave polygons in each
+projection plane are tiled by their triangles (areas add up).
 
 The oracle's texture sampler (oracle/or_render.cpp TexSample) against the
 CUDA-documented bilinear filter evaluated in FP64 with the same 8-bit
@@ -54,6 +56,67 @@ l 1 2
 """
 
 
+def ear_clip_here(face, V):
+    """tinyobjloader 2.0's built-in ear clipping, in float32 scalars."""
+    f32 = np.float32
+    P = lambda i, a: f32(V[i[0]][a])
+    n0 = len(face)
+    axes = [1, 2]
+    eps = f32(np.finfo(np.float32).eps)
+    for k in range(n0):
+        a, b, c = face[k % n0], face[(k + 1) % n0], face[(k + 2) % n0]
+        e0 = [P(b, i) - P(a, i) for i in range(3)]
+        e1 = [P(c, i) - P(b, i) for i in range(3)]
+        cx = abs(e0[1] * e1[2] - e0[2] * e1[1])
+        cy = abs(e0[2] * e1[0] - e0[0] * e1[2])
+        cz = abs(e0[0] * e1[1] - e0[1] * e1[0])
+        if cx > eps or cy > eps or cz > eps:
+            if not (cx > cy and cx > cz):
+                axes[0] = 0
+                if cz > cx and cz > cy:
+                    axes[1] = 1
+            break
+    rem, out = list(face), []
+    guess, iters, prev = 0, len(rem), len(rem)
+    while len(rem) > 3 and iters > 0:
+        n = len(rem)
+        if guess >= n:
+            guess -= n
+        if prev != n:
+            prev, iters = n, n
+        else:
+            iters -= 1
+        ind = [rem[(guess + k) % n] for k in range(3)]
+        vx = [P(i, axes[0]) for i in ind]
+        vy = [P(i, axes[1]) for i in ind]
+        cross = (vx[1] - vx[0]) * (vy[2] - vy[1]) - (vy[1] - vy[0]) * (vx[2] - vx[1])
+        area = (vx[0] * vy[1] - vy[0] * vx[1]) * f32(0.5)
+        if cross * area < 0:
+            guess += 1
+            continue
+        inside = False
+        for other in range(3, n):
+            t = rem[(guess + other) % n]
+            tx, ty = P(t, axes[0]), P(t, axes[1])
+            c = False
+            j = 2
+            for i in range(3):
+                if (vy[i] > ty) != (vy[j] > ty) and tx < (vx[j] - vx[i]) * (ty - vy[i]) / (vy[j] - vy[i]) + vx[i]:
+                    c = not c
+                j = i
+            if c:
+                inside = True
+                break
+        if inside:
+            guess += 1
+            continue
+        out.append(ind)
+        del rem[(guess + 1) % n]
+    if len(rem) == 3:
+        out.append(rem)
+    return out
+
+
 def read_obj_here(text, position, scale, textures):
     """The same reading, written independently of the library."""
     V, T, N, tris = [], [], [], []
@@ -86,7 +149,9 @@ def read_obj_here(text, position, scale, textures):
             if len(face) == 3:
                 tris.append(face)
                 continue
-            assert len(face) == 4
+            if len(face) > 4:
+                tris.extend(ear_clip_here(face, V))
+                continue
             P = [np.array(V[f[0]], np.float32) for f in face]
             e02, e13 = P[2] - P[0], P[3] - P[1]
             s02 = np.float32(np.float32(e02[0] * e02[0] + e02[1] * e02[1]) + e02[2] * e02[2])
@@ -148,10 +213,57 @@ def test_obj_triangles_match_an_independent_reading(tmp_path, crlf):
     assert np.array_equal(objs[5]["v2_position"], objs[6]["v2_position"])
 
 
+POLYGONS = """# a convex pentagon in the xy plane, a concave L hexagon in the xz plane,
+# a heptagon with two reflex corners in the yz plane
+v 0 0 0
+v 2 0 0
+v 2.5 1.5 0
+v 1 2.5 0
+v -0.5 1.5 0
+v 0 1 0
+v 3 1 0
+v 3 1 1
+v 1 1 1
+v 1 1 3
+v 0 1 3
+v 5 0 0
+v 5 2 0
+v 5 2 1
+v 5 1 1
+v 5 1.5 2.5
+v 5 0 2
+v 5 0.5 1
+f 1 2 3 4 5
+f 6 7 8 9 10 11
+f 12 13 14 15 16 17 18
+"""
+
+
+def shoelace(pts):
+    x, y = pts[:, 0], pts[:, 1]
+    return 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+
+
+def test_larger_faces_are_ear_clipped(tmp_path):
+    objs = ap.load_obj(write(tmp_path, POLYGONS, "poly.obj"))
+    want = read_obj_here(POLYGONS, (0, 0, 0), 1.0, (0, 0, 0, 0))
+    assert len(objs) == len(want) == 3 + 4 + 5
+    for o, (P, _, _) in zip(objs, want):
+        for k in range(3):
+            assert np.array_equal(o[f"v{k}_position"], P[k])
+    # Each polygon's triangles tile it: their areas add up to the polygon's.
+    V = np.array([[float(x) for x in l.split()[1:]] for l in POLYGONS.splitlines() if l.startswith("v ")])
+    for (lo, hi), idx, drop in (((0, 3), [0, 1, 2, 3, 4], 2), ((3, 7), [5, 6, 7, 8, 9, 10], 1),
+                                ((7, 12), [11, 12, 13, 14, 15, 16, 17], 0)):
+        keep = [a for a in range(3) if a != drop]
+        tri_area = 0.0
+        for o in objs[lo:hi]:
+            p = np.array([o[f"v{k}_position"] for k in range(3)], np.float64)[:, keep]
+            tri_area += shoelace(p)
+        assert abs(tri_area - shoelace(V[idx][:, keep])) < 1e-6
+
+
 def test_obj_errors(tmp_path):
-    with pytest.raises(ap.ApertureError) as e:
-        ap.load_obj(write(tmp_path, "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0 2 0\nf 1 2 3 4 5\n", "pent.obj"))
-    assert e.value.code == ap.ERR_UNSUPPORTED
     with pytest.raises(ap.ApertureError) as e:
         ap.load_obj(write(tmp_path, "v 0 0 0\nv 1 0 0\nv 1 1 0\nf 1 2 4\n", "range.obj"))
     assert e.value.code == ap.ERR_INVALID_ARG
