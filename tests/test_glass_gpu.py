@@ -109,6 +109,14 @@ def test_cameras_inside_a_medium_match_the_oracle(gpu, wavelengths):
     parity(glass_scene(), cam, prm, f"camera inside a glass sphere, {wavelengths} wavelengths", 200)
     cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 32, 24)              # inside the enclosing shell
     parity(enclosed_scene(), cam, prm, f"box inside a glass shell, {wavelengths} wavelengths")
+    # Inside a diffuse shell the path starts inside a medium: after the first
+    # hit the next ray meets that object alone and NEE is skipped
+    # (render.cuh:89-92, 782-785); a miss against it leaves the medium and
+    # restarts at the world origin (render.cuh:109-126).
+    shell = scenes.spheres([[0.0, 0.0, 0.0]], [5.0])
+    shell["albedo"] = [0.7, 0.7, 0.7]
+    parity(scenes.concat([ap.synth_box_scene(1, 2, 15.0), shell]), cam, prm,
+           f"box inside a diffuse shell, {wavelengths} wavelengths")
 
 
 def test_guided_glass_matches_the_oracle(gpu):
