@@ -53,7 +53,9 @@ int ap_texture_sample(const ap_texture_cache* cache, uint64_t handle, const floa
 
 /* Binds the cache whose images the scene's texture handles name (NULL
  * unbinds). Rendering a scene whose materials name a handle the bound cache
- * does not hold returns AP_ERR_INVALID_ARG. */
+ * does not hold returns AP_ERR_INVALID_ARG. The cache must outlive the
+ * scenes bound to it (unbind or destroy them first); images may be added
+ * while it is bound, between renders. */
 int ap_scene_set_texture_cache(ap_scene* scene, const ap_texture_cache* cache);
 
 /* LoadTexturedMesh (core/helper/scene_helper.h:453-564) over an OBJ file read
