@@ -10,7 +10,6 @@
 // p, comments) leaves the faces as they are: tinyobjloader splits shapes
 // there, but LoadTexturedMesh walks every shape in file order, so the
 // triangles come out in face order either way.
-#include <cerrno>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -41,7 +40,6 @@ bool is_space(char c) { return c == ' ' || c == '\t'; }
 float parse_real(const char*& p, float dflt, bool* got = nullptr) {
     while (is_space(*p)) ++p;
     char* end = nullptr;
-    errno = 0;
     const double d = std::strtod(p, &end);
     if (end == p) {
         if (got) *got = false;

@@ -675,8 +675,8 @@ def main():
                      help="also run the library's streaming copy kernel over 1 GiB (HBM counter scale reference)")
     args = ap_.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: the traversal core has no multi-GPU path in this build "
-              "(see DESIGN.md for how it would shard); run with --gpus 1.", file=sys.stderr)
+        print("bench.py runs on one GPU only: the traversal core has no multi-GPU path; "
+              "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import aperture as ap
