@@ -114,7 +114,10 @@ int ap_scene_set_traversal_mode(ap_scene* scene, int mode);
  * (1..64) at which a wave refills them with its next rays. A value of 0
  * keeps the current setting. Results are identical for every shape; only
  * speed changes. No reference counterpart (the reference launches one
- * thread per pixel). Defaults: 512 rays, refill at 16 idle lanes. */
+ * thread per pixel). Defaults: 512 rays, refill at 16 idle lanes. The
+ * renderer's own path queues (ap_render) use at most 128 rays per wave,
+ * dealt round-robin over the XCDs; host batches (ap_trace_*) use the shape
+ * as set, XCD by XCD. */
 int ap_scene_set_launch(ap_scene* scene, int32_t wave_rays, int32_t refill);
 int ap_scene_get_launch(const ap_scene* scene, int32_t* wave_rays, int32_t* refill);
 
