@@ -117,6 +117,18 @@ int or_render_screen(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
                      const ap_render_params* prm, float cull_margin, float* hdr, double* sums,
                      ap_screen_gbuffer* gbuffer, ap_screen_statistics* statistics, int threads);
 
+/* Pins of the BSDFs and light samplers (or_render.cpp). or_plant(k) turns on
+ * planted mistake k (0 = none) for the tests that show a pin fails. */
+void or_plant(int which);
+void or_ggx_vndf_sample(const float* V, const float* N, float alpha, const float* rnd, size_t n, float* out);
+void or_ggx_vndf_pdf(const float* V, const float* N, const float* L, size_t n, float alpha, float* out);
+void or_bsdf_eval(int model, const float* mat, float lambda, const float* rayDir, const float* N, const float* L,
+                  size_t n, float* f, float* pdf);
+void or_bsdf_sample(int model, const float* mat, float lambda, const float* rayDir, const float* N, uint64_t seed,
+                    size_t n, float* out);
+void or_nee_sample(const ap_scene_object* objs, size_t n, const float* p, size_t np, uint64_t seed, float* out);
+void or_nee_pdf(const ap_scene_object* objs, size_t n, int light, const float* q, size_t m, float* out);
+
 /* Albedo textures for or_render (or_render.cpp): n RGBA8 images of
  * dims[2k] x dims[2k+1] texels (rows from the top) concatenated in rgba;
  * texture handle k + 1 names image k. or_texture_sample: tex2D of image

@@ -47,6 +47,15 @@
 
 namespace orc {
 
+// Planted mistakes for the tests that show a pin fails on a wrong formula
+// (or_plant; 0 = none). Each one is a plausible transcription slip:
+//   1: the bounded-VNDF pdf loses its factor 2 (sampler and pdf alike);
+//   2: the sphere-cap area is pi r^2 (1 - cosC) (sampler and pdf alike);
+//   3: the GGX mixture pdf of a specular draw drops the Lambert term;
+//   4: the GGX NDF takes roughness where it takes alpha (D(alpha) -> D(sqrt alpha));
+//   5: the bounded-VNDF sampler skips normalising the stretched view vector.
+static int g_plant = 0;
+
 // ---- PCG32 (O'Neill 2014, pcg32_random_r / pcg32_srandom_r) ----
 struct Rng {
     uint64_t state, inc;
@@ -264,6 +273,7 @@ static Sample SampleCosine(float3 e, float3 n) {
     return {normalize(dir), z / PI_F};
 }
 static float GGX_D(float alpha, float NoH) {
+    if (g_plant == 4) alpha = std::sqrt(alpha);
     const float a2 = alpha * alpha;
     const float d = NoH * NoH * (a2 - 1.0f) + 1.0f;
     return a2 / std::fmax(PI_F * d * d, 1e-12f);
@@ -275,7 +285,7 @@ static Sample SampleGGXBoundedVNDF(float3 V, float3 N, float alpha, float3 rnd) 
     const float3 lv{XoV, YoV, NoV};
     const float NoV2 = std::fmin(1.0f, NoV * NoV);
     const float a2 = alpha * alpha;
-    const float3 vs = float3{XoV * alpha, YoV * alpha, NoV} / std::sqrt(a2 + (1.0f - a2) * NoV2);
+    const float3 vs = float3{XoV * alpha, YoV * alpha, NoV} / (g_plant == 5 ? 1.0f : std::sqrt(a2 + (1.0f - a2) * NoV2));
     const float phi = 2.0f * PI_F * rnd.x;
     const float s = 1.0f + std::sqrt(1.0f - NoV2);
     const float s2 = s * s;
@@ -290,7 +300,7 @@ static Sample SampleGGXBoundedVNDF(float3 V, float3 N, float alpha, float3 rnd) 
     const float3 dir = lL.x * X + lL.y * Y + lL.z * N;
     const float len2 = (1.0f - NoV2) * a2;
     const float t = std::sqrt(len2 + NoV2);
-    return {dir, GGX_D(alpha, H.z) / (2.0f * (k * NoV + t))};
+    return {dir, GGX_D(alpha, H.z) / ((g_plant == 1 ? 1.0f : 2.0f) * (k * NoV + t))};
 }
 static float GGXBoundedVNDF_PDF(float3 V, float3 N, float3 L, float alpha) {
     const float NoV = dot(N, V), NoL = dot(N, L);
@@ -304,7 +314,7 @@ static float GGXBoundedVNDF_PDF(float3 V, float3 N, float3 L, float alpha) {
     const float NoH = std::fmax(0.0f, dot(N, H));
     const float len2 = (1.0f - NoV2) * a2;
     const float t = std::sqrt(len2 + NoV2);
-    return std::fmax(GGX_D(alpha, NoH) / (2.0f * (k * NoV + t)), 1e-12f);
+    return std::fmax(GGX_D(alpha, NoH) / ((g_plant == 1 ? 1.0f : 2.0f) * (k * NoV + t)), 1e-12f);
 }
 static float FresnelSchlick(float F0, float cosX) {
     const float c = clampf(1.0f - cosX, 0.0f, 1.0f);
@@ -504,7 +514,7 @@ static SampleCtx SampleBSDF(const float lb[4], float lambda, Rng& rng, int model
         const float NoL = std::fmax(0.0f, dot(N, s.dir));
         if (NoL <= 0.0f) { r.terminate = true; return r; }
         r.next = s.dir;
-        r.pdf = pSpec * s.pdf + (1.0f - pSpec) * (NoL / PI_F);
+        r.pdf = pSpec * s.pdf + (g_plant == 3 ? 0.0f : (1.0f - pSpec) * (NoL / PI_F));
     } else {
         const float e0 = rng.rand1(), e1 = rng.rand1(), e2 = rng.rand1();
         const Sample s = SampleCosine({e0, e1, e2}, N);
@@ -669,7 +679,7 @@ static float NEEPDFOmegaFromHit(const ap_scene_object& L, const ap_light_info& l
         const float dist = length(sp - sc);
         if (dist <= r) return 0.0f;
         const float cosC = clampf(r / dist, 0.0f, 1.0f);
-        const float cap = 2.0f * PI_F * r * r * (1.0f - cosC);
+        const float cap = (g_plant == 2 ? 1.0f : 2.0f) * PI_F * r * r * (1.0f - cosC);
         if (cap <= 1e-20f) return 0.0f;
         return AreaToSolidAnglePDF(sp, lp, ln, li.select_prob / cap);
     }
@@ -725,7 +735,7 @@ static LightSample NEESampleLight(const Scene& S, float3 p, Rng& rng) {
         if (dist <= r) return ls;
         const float3 axis = pc / dist;
         const float cosC = clampf(r / dist, 0.0f, 1.0f);
-        const float cap = 2.0f * PI_F * r * r * (1.0f - cosC);
+        const float cap = (g_plant == 2 ? 1.0f : 2.0f) * PI_F * r * r * (1.0f - cosC);
         const float u = rng.rand1();
         const float vphi = rng.rand1();
         const float cosT = lerp(cosC, 1.0f, u);
@@ -1444,24 +1454,28 @@ struct Setup {
     int W, H;
 };
 
-static Setup MakeSetup(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
-                       const ap_render_params* prm, float cull_margin) {
-    Setup U{Scene{objs, n, bvh, {}, std::vector<int>(n, -1), prm->triangle_material, cull_margin},
-            RenderOpts{prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee, prm->nee_always, prm->sky},
-            {}, {}, {}, {}, 0.0f, 0.0f, false, cam->width, cam->height};
-    Scene& S = U.S;
+// The light table (scene.cu:1056-1080): area * luminance weights, linear CDF.
+static void BuildLights(Scene& S) {
     float total = 0.0f;
-    for (size_t i = 0; i < n; ++i) {
-        if (objs[i].material.shading_model == AP_MAT_LIGHT) {
+    for (size_t i = 0; i < S.n; ++i) {
+        if (S.objs[i].material.shading_model == AP_MAT_LIGHT) {
             S.light_index[i] = static_cast<int>(S.lights.size());
-            const float3 em = F3(objs[i].material.texture.base.emissive);
+            const float3 em = F3(S.objs[i].material.texture.base.emissive);
             ap_light_info li{static_cast<int32_t>(i),
-                             std::fmax(GetArea(objs[i]) * dot(em, float3{0.2126f, 0.7152f, 0.0722f}), 1.0e-20f), 0.0f};
+                             std::fmax(GetArea(S.objs[i]) * dot(em, float3{0.2126f, 0.7152f, 0.0722f}), 1.0e-20f), 0.0f};
             total += li.importance_weight;
             S.lights.push_back(li);
         }
     }
     for (auto& li : S.lights) li.select_prob = li.importance_weight / total;
+}
+
+static Setup MakeSetup(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                       const ap_render_params* prm, float cull_margin) {
+    Setup U{Scene{objs, n, bvh, {}, std::vector<int>(n, -1), prm->triangle_material, cull_margin},
+            RenderOpts{prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee, prm->nee_always, prm->sky},
+            {}, {}, {}, {}, 0.0f, 0.0f, false, cam->width, cam->height};
+    BuildLights(U.S);
     U.camO = float3{cam->origin[0], cam->origin[1], cam->origin[2]};
     U.camD = normalize(float3{cam->direction[0], cam->direction[1], cam->direction[2]});
     const float FOV = cam->fov_deg * 3.1415926f / 180.0f;
@@ -1772,6 +1786,108 @@ void or_glass_bsdf(const float* d, const float* n, const float* mat, float lambd
     out[4] = brdf;
     out[5] = sc.inside ? 1.0f : 0.0f;
     out[6] = sc.delta ? 1.0f : 0.0f;
+}
+
+// ---- pins of the BSDFs and light samplers (tests/test_bsdf_cpu.py) ----
+void or_plant(int which) { g_plant = which; }
+
+// SampleGGXBoundedVNDF (sample.cuh:179-227) for n (V, N, rnd) triples:
+// out[4i..4i+3] = direction, pdf.
+void or_ggx_vndf_sample(const float* V, const float* N, float alpha, const float* rnd, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const Sample s = SampleGGXBoundedVNDF(float3{V[3 * i], V[3 * i + 1], V[3 * i + 2]},
+                                              float3{N[3 * i], N[3 * i + 1], N[3 * i + 2]}, alpha,
+                                              float3{rnd[3 * i], rnd[3 * i + 1], rnd[3 * i + 2]});
+        out[4 * i] = s.dir.x; out[4 * i + 1] = s.dir.y; out[4 * i + 2] = s.dir.z; out[4 * i + 3] = s.pdf;
+    }
+}
+
+// GGXBoundedVNDF_PDF (sample.cuh:229-248) of n directions L for one (V, N).
+void or_ggx_vndf_pdf(const float* V, const float* N, const float* L, size_t n, float alpha, float* out) {
+    const float3 v{V[0], V[1], V[2]}, nn{N[0], N[1], N[2]};
+    for (size_t i = 0; i < n; ++i) out[i] = GGXBoundedVNDF_PDF(v, nn, float3{L[3 * i], L[3 * i + 1], L[3 * i + 2]}, alpha);
+}
+
+static Ctx PinCtx(const float* mat, const float* N) {
+    Ctx c;
+    c.hit = true;
+    c.normal = {N[0], N[1], N[2]};
+    c.gb = {{mat[0], mat[1], mat[2]}, {0, 0, 0}, mat[3], mat[4], mat[5], 1.5f, 0.0f, 0.0f, 0.0f};
+    return c;
+}
+
+// EvaluateBSDFWithCos and BSDF_PDF (material.cuh) of a Lambert or GGX
+// surface at wavelength lambda for n directions L; mat = albedo rgb,
+// specular, metallic, roughness.
+void or_bsdf_eval(int model, const float* mat, float lambda, const float* rayDir, const float* N, const float* L,
+                  size_t n, float* f, float* pdf) {
+    float lb[4];
+    QueryLambda(lambda, lb);
+    const Ctx c = PinCtx(mat, N);
+    const float3 rd{rayDir[0], rayDir[1], rayDir[2]};
+    const MediaCtx m;
+    for (size_t i = 0; i < n; ++i) {
+        const float3 l{L[3 * i], L[3 * i + 1], L[3 * i + 2]};
+        f[i] = EvaluateBSDFWithCos(lb, lambda, model, rd, l, c, m);
+        pdf[i] = BSDF_PDF(lb, model, c.gb, c.normal, rd, l);
+    }
+}
+
+// SampleBSDF (material.cuh) n times from PCG32(seed, 11): out[5i..5i+4] =
+// direction, pdf, terminate flag.
+void or_bsdf_sample(int model, const float* mat, float lambda, const float* rayDir, const float* N, uint64_t seed,
+                    size_t n, float* out) {
+    float lb[4];
+    QueryLambda(lambda, lb);
+    const Ctx c = PinCtx(mat, N);
+    const float3 rd{rayDir[0], rayDir[1], rayDir[2]};
+    const MediaCtx m;
+    Rng rng = Rng::seeded(seed, 11u);
+    for (size_t i = 0; i < n; ++i) {
+        const SampleCtx s = SampleBSDF(lb, lambda, rng, model, rd, c, m);
+        out[5 * i] = s.next.x; out[5 * i + 1] = s.next.y; out[5 * i + 2] = s.next.z;
+        out[5 * i + 3] = s.pdf;
+        out[5 * i + 4] = s.terminate ? 1.0f : 0.0f;
+    }
+}
+
+// NEESampleLight (nee.cuh:330-377) over the scene's light table from each of
+// np shading points, one draw each from PCG32(seed, i): out[10i..10i+9] =
+// light point, light normal, light object, the sampler's solid-angle pdf
+// (NEELightPDFOmegaFromArea, nee.cuh:48-52), NEEPDFOmegaFromHit at the
+// sampled point (nee.cuh:313-327), valid flag.
+void or_nee_sample(const ap_scene_object* objs, size_t n, const float* p, size_t np, uint64_t seed, float* out) {
+    Scene S{objs, n, nullptr, {}, std::vector<int>(n, -1), AP_TRIANGLE_MATERIAL_OBJECT, 0.0f};
+    BuildLights(S);
+    for (size_t i = 0; i < np; ++i) {
+        const float3 x{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+        Rng rng = Rng::seeded(seed, i);
+        const LightSample ls = NEESampleLight(S, x, rng);
+        float* o = out + 10 * i;
+        for (int k = 0; k < 10; ++k) o[k] = 0.0f;
+        o[6] = -1.0f;
+        if (!ls.valid) continue;
+        o[0] = ls.point.x; o[1] = ls.point.y; o[2] = ls.point.z;
+        o[3] = ls.normal.x; o[4] = ls.normal.y; o[5] = ls.normal.z;
+        o[6] = static_cast<float>(ls.object);
+        o[7] = AreaToSolidAnglePDF(x, ls.point, ls.normal, ls.pFacet * ls.pA);
+        o[8] = NEEPDFOmegaFromHit(objs[ls.object], S.lights[S.light_index[ls.object]], x, ls.point, ls.normal);
+        o[9] = 1.0f;
+    }
+}
+
+// NEEPDFOmegaFromHit of light object `light` for m (shading point, light
+// point, light normal) triples (9 floats each).
+void or_nee_pdf(const ap_scene_object* objs, size_t n, int light, const float* q, size_t m, float* out) {
+    Scene S{objs, n, nullptr, {}, std::vector<int>(n, -1), AP_TRIANGLE_MATERIAL_OBJECT, 0.0f};
+    BuildLights(S);
+    const int li = S.light_index[light];
+    for (size_t i = 0; i < m; ++i) {
+        const float* t = q + 9 * i;
+        out[i] = li < 0 ? 0.0f
+                        : NEEPDFOmegaFromHit(objs[light], S.lights[li], float3{t[0], t[1], t[2]},
+                                             float3{t[3], t[4], t[5]}, float3{t[6], t[7], t[8]});
+    }
 }
 
 // Albedo texture images for renders (index k = handle k + 1): n RGBA8

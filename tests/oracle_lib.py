@@ -30,6 +30,13 @@ def lib():
             getattr(L, nm).restype = f32
             getattr(L, nm).argtypes = [f32, f32, f32]
         L.or_glass_bsdf.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp, ctypes.c_uint64, vp]
+        L.or_plant.argtypes = [i32]
+        L.or_ggx_vndf_sample.argtypes = [vp, vp, f32, vp, sz, vp]
+        L.or_ggx_vndf_pdf.argtypes = [vp, vp, vp, sz, f32, vp]
+        L.or_bsdf_eval.argtypes = [i32, vp, f32, vp, vp, vp, sz, vp, vp]
+        L.or_bsdf_sample.argtypes = [i32, vp, f32, vp, vp, ctypes.c_uint64, sz, vp]
+        L.or_nee_sample.argtypes = [vp, sz, vp, sz, ctypes.c_uint64, vp]
+        L.or_nee_pdf.argtypes = [vp, sz, i32, vp, sz, vp]
         L.or_texture_set.argtypes = [vp, vp, sz]
         L.or_texture_sample.argtypes = [i32, vp, sz, vp]
         L.or_bvh_node_count.restype = sz
@@ -607,4 +614,82 @@ def texture_sample(index, uv):
     u = _f32(uv, (-1, 2))
     out = np.zeros((len(u), 4), np.float32)
     lib().or_texture_sample(int(index), _p(u), len(u), _p(out))
+    return out
+
+
+class planted:
+    """Context manager: the oracle runs with planted mistake `which` (or_plant)."""
+
+    def __init__(self, which):
+        self.which = int(which)
+
+    def __enter__(self):
+        lib().or_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_plant(0)
+        return False
+
+
+def ggx_vndf_sample(v, n, alpha, rnd):
+    """SampleGGXBoundedVNDF for arrays V (m,3), N (m,3), rnd (m,3): (dirs, pdfs)."""
+    rnd = _f32(rnd).reshape(-1, 3)
+    m = len(rnd)
+    v = np.ascontiguousarray(np.broadcast_to(_f32(v).reshape(-1, 3), (m, 3)))
+    n = np.ascontiguousarray(np.broadcast_to(_f32(n).reshape(-1, 3), (m, 3)))
+    out = np.zeros((m, 4), np.float32)
+    lib().or_ggx_vndf_sample(_p(v), _p(n), float(alpha), _p(rnd), m, _p(out))
+    return out[:, :3], out[:, 3]
+
+
+def ggx_vndf_pdf(v, n, dirs, alpha):
+    """GGXBoundedVNDF_PDF of directions (m,3) for one (V, N)."""
+    d = _f32(dirs).reshape(-1, 3)
+    out = np.zeros(len(d), np.float32)
+    lib().or_ggx_vndf_pdf(_p(_f32(v)), _p(_f32(n)), _p(d), len(d), float(alpha), _p(out))
+    return out
+
+
+def _mat(albedo, specular, metallic, roughness):
+    return np.array(list(albedo) + [specular, metallic, roughness], np.float32)
+
+
+def bsdf_eval(model, ray_dir, normal, dirs, lam=550.0, albedo=(1, 1, 1), specular=0.5, metallic=0.0, roughness=0.5):
+    """EvaluateBSDFWithCos and BSDF_PDF of directions (m,3): (f*cos, pdf)."""
+    d = _f32(dirs).reshape(-1, 3)
+    f = np.zeros(len(d), np.float32)
+    pdf = np.zeros(len(d), np.float32)
+    lib().or_bsdf_eval(int(model), _p(_mat(albedo, specular, metallic, roughness)), float(lam), _p(_f32(ray_dir)),
+                       _p(_f32(normal)), _p(d), len(d), _p(f), _p(pdf))
+    return f, pdf
+
+
+def bsdf_sample(model, ray_dir, normal, n, lam=550.0, albedo=(1, 1, 1), specular=0.5, metallic=0.0, roughness=0.5,
+                seed=1):
+    """SampleBSDF n times: (dirs (n,3), pdfs, terminated flags)."""
+    out = np.zeros((n, 5), np.float32)
+    lib().or_bsdf_sample(int(model), _p(_mat(albedo, specular, metallic, roughness)), float(lam), _p(_f32(ray_dir)),
+                         _p(_f32(normal)), int(seed), n, _p(out))
+    return out[:, :3], out[:, 3], out[:, 4] != 0.0
+
+
+def nee_sample(objs, points, seed=1):
+    """NEESampleLight from each shading point: dict of light point, normal,
+    object, sampler pdf (solid angle), NEEPDFOmegaFromHit, valid."""
+    o = _objs(objs)
+    p = _f32(points).reshape(-1, 3)
+    out = np.zeros((len(p), 10), np.float32)
+    lib().or_nee_sample(_p(o), len(o), _p(p), len(p), int(seed), _p(out))
+    return {"point": out[:, 0:3], "normal": out[:, 3:6], "object": out[:, 6].astype(np.int32),
+            "pdf_sampler": out[:, 7], "pdf_hit": out[:, 8], "valid": out[:, 9] != 0.0}
+
+
+def nee_pdf(objs, light, points, light_points, light_normals):
+    """NEEPDFOmegaFromHit of light object `light` for (shading point, light point, light normal) rows."""
+    o = _objs(objs)
+    q = np.ascontiguousarray(np.concatenate([_f32(points).reshape(-1, 3), _f32(light_points).reshape(-1, 3),
+                                             _f32(light_normals).reshape(-1, 3)], axis=1), np.float32)
+    out = np.zeros(len(q), np.float32)
+    lib().or_nee_pdf(_p(o), len(o), int(light), _p(q), len(q), _p(out))
     return out

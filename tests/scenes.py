@@ -81,13 +81,37 @@ def cubes(centers, extents, emissive=None):
     return update_proxy(o)
 
 
-def spheres(centers, radii):
+def spheres(centers, radii, emissive=None):
+    """Sphere SceneObjects; with `emissive`, lights (Material::CreateLight:
+    MAT_LIGHT, albedo 0), sampled by NEESampleSphereCapArea."""
     centers = np.asarray(centers, np.float32).reshape(-1, 3)
     o = blank(len(centers))
     o["type"] = ap.OBJ_SPHERE
     o["center"] = centers
     o["extent"] = np.asarray(radii, np.float32).reshape(-1, 1)
+    if emissive is not None:
+        o["shading_model"] = ap.MAT_LIGHT
+        o["albedo"] = 0.0
+        o["emissive"] = emissive
     return update_proxy(o)
+
+
+def sphere_lit_box(seed=1, subdiv=2, keep_cube_light=False, ggx=True):
+    """The closed-box workload lit by two emissive spheres (one under the
+    ceiling, one small and bright in a corner), optionally beside its ceiling
+    cube light, with every third mesh GGX: NEE picks sphere caps (and box
+    facets), BSDF rays that hit the spheres take the cap pdf for MIS."""
+    objs = ap.synth_box_scene(seed, subdiv, 15.0)
+    if not keep_cube_light:
+        objs = objs[objs["shading_model"] != ap.MAT_LIGHT]
+    if ggx:
+        g = (np.arange(len(objs)) % 3 == 0) & (objs["shading_model"] == ap.MAT_DIFFUSE)
+        objs["shading_model"][g] = ap.MAT_GGX
+        objs["roughness"][g] = np.linspace(0.1, 0.9, int(g.sum()), dtype=np.float32)
+        objs["metallic"][g] = 0.4
+        objs["specular"][g] = 0.5
+    lights = spheres([[0.0, 0.7, 0.0], [0.7, -0.75, -0.6]], [0.18, 0.06], emissive=[[12.0, 10.0, 8.0], [40.0, 60.0, 80.0]])
+    return concat([objs, lights])
 
 
 def sdfs(centers, extents, sdf_type=0, info=(0.375, 0.125, 0.0, 0.0), volume=-1, rotations=None):

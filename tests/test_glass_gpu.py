@@ -119,6 +119,35 @@ def test_cameras_inside_a_medium_match_the_oracle(gpu, wavelengths):
            f"box inside a diffuse shell, {wavelengths} wavelengths")
 
 
+def sphere_light_glass_scene():
+    """Glass balls around a small sphere light with GGX plates behind them,
+    inside the closed box without its ceiling light (the layout of the
+    reference's CreateGlassTestScene, scene_helper.h:176-262, scaled 2x)."""
+    box = ap.synth_box_scene(1, 2, 15.0)
+    box = box[box["shading_model"] != ap.MAT_LIGHT]
+    balls = scenes.spheres([[0.0, 0.5, 0.25], [-0.5, 0.0, -0.25], [0.5, 0.0, 0.25], [0.0, 0.0, 0.0],
+                            [0.0, -0.5, -0.25]], [0.25] * 5)
+    balls = make_glass(balls, 1.5, 1.0, [[1, 1, 1], [1, 1, 0], [0, 1, 1], [1, 0.02, 0.02], [0.02, 0.02, 1]],
+                       20.0, 0.0)
+    balls["ior"] = [1.6, 1.5, 1.55, 1.45, 1.5]
+    light = scenes.spheres([[0.25, 0.5, -0.25]], [0.1], emissive=[25.0, 25.0, 25.0])
+    plates = scenes.cubes([[0.0, 0.0, -0.5], [0.0, -0.5, -0.6], [0.0, 0.5, -0.6]], [0.25, 0.25, 0.02])
+    plates["shading_model"] = ap.MAT_GGX
+    plates["albedo"] = [[1.0, 0.5, 0.0], [0.9, 0.9, 0.9], [0.5, 0.5, 0.5]]
+    plates["roughness"] = [0.1, 0.1, 0.5]
+    plates["metallic"] = [1.0, 0.0, 1.0]
+    plates["specular"] = [1.0, 0.5, 1.0]
+    return scenes.concat([box, balls, light, plates])
+
+
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_sphere_light_among_glass_matches_the_oracle(gpu, wavelengths):
+    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 40, 40)
+    prm = ap.RenderParams.defaults(spp=3, max_depth=14, rr_min_depth=4, seed=8, wavelengths=wavelengths)
+    parity(sphere_light_glass_scene(), cam, prm, f"glass balls around a sphere light, {wavelengths} wavelengths",
+           200)
+
+
 def test_guided_glass_matches_the_oracle(gpu):
     cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 32, 24)
     gg.guided_parity(glass_scene(), cam, dict(spp=3, max_depth=12, rr_min_depth=4), "box with glass", calls=2)

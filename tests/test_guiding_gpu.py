@@ -76,3 +76,13 @@ def test_guided_four_wavelengths(gpu):
     objs = ap.synth_heightfield_scene(48, 8, 5, 20.0)
     cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 48, 27)
     guided_parity(objs, cam, dict(spp=3, max_depth=16, rr_min_depth=4, wavelengths=4), "heightfield, four wavelengths")
+
+
+def test_guided_sphere_lights(gpu):
+    """Guided sampling mixed with the sphere-cap light pdf in NEE's MIS."""
+    import scenes
+
+    objs = scenes.sphere_lit_box(4, keep_cube_light=True)
+    cam = ap.Camera.make(rg.BOX_CAM["origin"], rg.BOX_CAM["direction"], rg.BOX_CAM["fov_deg"], 40, 40)
+    guided_parity(objs, cam, dict(spp=3, max_depth=8), "box with sphere lights")
+    guided_parity(objs, cam, dict(spp=2, max_depth=8, wavelengths=4), "box with sphere lights, four wavelengths")

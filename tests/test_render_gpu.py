@@ -195,3 +195,16 @@ def test_hero_wavelengths_match_oracle(gpu):
     cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 48, 27)
     compare_with_oracle(hf, cam, ap.RenderParams.defaults(spp=4, max_depth=16, rr_min_depth=4, wavelengths=4),
                         "heightfield 64^2, four wavelengths per path")
+
+
+def test_sphere_lights_match_oracle(gpu):
+    """Emissive spheres as the only lights (NEESampleSphereCapArea and the
+    paired NEEPDFOmegaFromHit_Sphere, nee.cuh:254-310), with GGX surfaces; then
+    a table of the cube light and two spheres, one and four wavelengths."""
+    cam = ap.Camera.make(BOX_CAM["origin"], BOX_CAM["direction"], BOX_CAM["fov_deg"], 48, 48)
+    objs = scenes.sphere_lit_box(2)
+    compare_with_oracle(objs, cam, box_params(spp=8, depth=8), "sphere lights only")
+    mixed = scenes.sphere_lit_box(3, keep_cube_light=True)
+    compare_with_oracle(mixed, cam, box_params(spp=8, depth=8, seed=4), "cube and sphere lights")
+    compare_with_oracle(mixed, cam, box_params(spp=6, depth=8, wavelengths=4, nee_always=1),
+                        "cube and sphere lights, four wavelengths, NEE at every vertex")
