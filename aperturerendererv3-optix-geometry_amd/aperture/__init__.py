@@ -803,15 +803,17 @@ class Batch:
     def handle(self):
         return self._h.value
 
-    def render(self, params, hdrs, sums=None, stream=None):
+    def render(self, params, hdrs, sums=None, stream=None, count_rays=True):
         """Adds params.spp frames of every scene; hdrs[k] / sums[k] are device
-        buffers (DeviceBuffer or address, None to skip). Returns the rays traced."""
+        buffers (DeviceBuffer or address, None to skip). Returns the rays
+        traced; with count_rays=False the call does not wait for its frames
+        and returns None."""
         n = len(self._scenes)
         h = (ctypes.c_void_p * n)(*[_ptr(x) if x is not None else None for x in hdrs])
         sm = None if sums is None else (ctypes.c_void_p * n)(*[_ptr(x) if x is not None else None for x in sums])
-        rays = (ctypes.c_uint64 * 2)()
+        rays = (ctypes.c_uint64 * 2)() if count_rays else None
         check(lib().ap_batch_render(self.handle, ctypes.byref(params), h, sm, rays, _stream(stream)))
-        return int(rays[0]), int(rays[1])
+        return (int(rays[0]), int(rays[1])) if count_rays else None
 
     def render_nrc(self, params, cache, nrc_params, hdrs, sums=None, stream=None):
         """render() with per-scene radiance caches: `cache` is a RadianceCache

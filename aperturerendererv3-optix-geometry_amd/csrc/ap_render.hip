@@ -1496,6 +1496,12 @@ __global__ void advance_frame_kernel(int32_t* frame) {
     if (threadIdx.x == 0) *frame += 1;
 }
 
+// Sets a batch's frame counter in stream order (a host copy on another
+// stream would race with the frames still in flight on this one).
+__global__ void set_frame_kernel(int32_t* frame, int32_t value) {
+    if (threadIdx.x == 0) *frame = value;
+}
+
 // Radiance-cache layout of one scene of a batch: its training-path stride,
 // the most training paths one frame can give it, and its first training slot
 // (slots of scene s: [slot_base, slot_base + n_train_max)).
@@ -2194,12 +2200,21 @@ struct BatchState {
     unsigned long long* rays_traced = nullptr;
     float* eps = nullptr;
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    bool key_valid = false;
-    ap_render_params key{};
-    int32_t key_wave_rays = 0, key_refill = 0;
-    hipStream_t key_stream = nullptr;
+    // One instantiated frame graph per (render parameters, launch shape,
+    // stream), kept until the batch is destroyed: the autotuner cycles over
+    // launch shapes, and destroying an executable graph whose launches may
+    // still be in flight (or still be reported by a profiler) is avoided.
+    struct Graph {
+        ap_render_params key;
+        int32_t wave_rays, refill;
+        hipStream_t stream;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    static constexpr size_t kMaxGraphs = 16;
+    hipStream_t last_stream = nullptr;  // stream of the last call (its frames may still run on it)
+    bool stream_used = false;
     // Per-scene radiance caches (ap_batch_render_nrc).
     uint32_t nrc_slots = 0;                 // training slots of all scenes
     uint32_t nrc_table_paths = 0xffffffffu; // train_paths the table was built for
@@ -2224,15 +2239,18 @@ struct BatchState {
     uint64_t nrc_last_hash = 0;
     bool nrc_rendered = false;
 
-    void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-        key_valid = false;
+    // Waits for the frames in flight, then destroys every graph.
+    void drop_graphs() {
+        if (!graphs.empty()) (void)hipDeviceSynchronize();
+        for (Graph& g : graphs) {
+            (void)hipGraphExecDestroy(g.exec);
+            (void)hipGraphDestroy(g.graph);
+        }
+        graphs.clear();
     }
     ~BatchState() {
-        drop_graph();
+        if (stream_used) (void)hipDeviceSynchronize();
+        drop_graphs();
         void* ptrs[] = {d_scenes, d_devs, d_pix_begin, paths, media, rays[0], rays[1], path_of[0], path_of[1], hits,
                         shadow_rays, shadow_payload, shadow_hits, ray_scene[0], ray_scene[1], shadow_scene, counters,
                         frame, rays_traced, eps, nrc_table, nrc_paths, nrc_queries, nrc_qpath, nrc_qrgb,
@@ -2251,8 +2269,20 @@ namespace {
 
 // Shading records of every scene for the call's triangle-material mode and
 // the batch records with this call's framebuffers, uploaded.
-int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const* sums, const f4* rgb_lut) {
+int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const* sums, const f4* rgb_lut,
+                  hipStream_t stream) {
     int rc;
+    // The scene records are read by the frames of the previous call, which
+    // may still run on its stream: wait for them before rewriting the records.
+    // (A different stream: the previous one may have been destroyed since, so
+    // the whole device is waited for.)
+    hipError_t e = hipSuccess;
+    if (b->stream_used) {
+        e = stream == b->last_stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize();
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: wait for the previous call");
+    }
+    b->last_stream = stream;
+    b->stream_used = true;
     for (size_t k = 0; k < b->scenes.size(); ++k) {
         ap_scene* s = b->scenes[k];
         if (!s->render) s->render = new RenderState();
@@ -2277,8 +2307,7 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
         b->h_devs[k] = s->dev;
         if (s->dev.has_sdf) b->shape.has_sdf = 1;
     }
-    hipError_t e = hipMemcpy(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
-                             hipMemcpyHostToDevice);
+    e = hipMemcpy(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(b->d_devs, b->h_devs.data(), b->h_devs.size() * sizeof(DeviceScene), hipMemcpyHostToDevice);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: upload scene records");
@@ -2325,12 +2354,12 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: launch");
 }
 
-bool same_key(const BatchState* b, const ap_render_params* p, hipStream_t st) {
-    const ap_render_params& k = b->key;
-    return b->key_valid && b->key_stream == st && k.seed == p->seed && k.max_depth == p->max_depth &&
+bool same_key(const BatchState::Graph& g, const BatchState* b, const ap_render_params* p, hipStream_t st) {
+    const ap_render_params& k = g.key;
+    return g.stream == st && k.seed == p->seed && k.max_depth == p->max_depth &&
            k.rr_min_depth == p->rr_min_depth && k.nee_max_depth == p->nee_max_depth && k.use_nee == p->use_nee &&
            k.nee_always == p->nee_always && k.sky == p->sky && k.wavelengths == p->wavelengths &&
-           b->key_wave_rays == b->shape.wave_rays && b->key_refill == b->shape.refill;
+           g.wave_rays == b->shape.wave_rays && g.refill == b->shape.refill;
 }
 
 // spp batch frames from frame index frame0 (graph or direct launches).
@@ -2339,7 +2368,8 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
     const float* tables = nullptr;
     int rc = spectrum_get(&rgb_lut, &lam_lut, &tables);
     if (rc != AP_OK) return rc;
-    hipError_t e = hipMemcpy(b->frame, &frame0, sizeof(int32_t), hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(set_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame, frame0);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "ap_batch: frame counter");
     const bool four = prm->wavelengths == 4;
     auto frame = [&]() {
@@ -2351,8 +2381,18 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
             if ((rc = frame()) != AP_OK) return rc;
         return AP_OK;
     }
-    if (!same_key(b, prm, stream)) {
-        b->drop_graph();
+    hipGraphExec_t exec = nullptr;
+    for (const BatchState::Graph& g : b->graphs)
+        if (same_key(g, b, prm, stream)) exec = g.exec;
+    if (!exec) {
+        if (b->graphs.size() >= BatchState::kMaxGraphs) {
+            // The oldest graph goes, once nothing that may use it is in flight.
+            e = hipDeviceSynchronize();
+            if (e != hipSuccess) return hip_fail(e, "ap_batch: wait before dropping a graph");
+            (void)hipGraphExecDestroy(b->graphs.front().exec);
+            (void)hipGraphDestroy(b->graphs.front().graph);
+            b->graphs.erase(b->graphs.begin());
+        }
         e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: begin capture");
         rc = frame();
@@ -2363,20 +2403,15 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
             return rc;
         }
         if (e != hipSuccess) return hip_fail(e, "ap_batch: end capture");
-        b->graph = g;
-        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         if (e != hipSuccess) {
-            b->drop_graph();
+            (void)hipGraphDestroy(g);
             return hip_fail(e, "ap_batch: instantiate graph");
         }
-        b->key = *prm;
-        b->key_wave_rays = b->shape.wave_rays;
-        b->key_refill = b->shape.refill;
-        b->key_stream = stream;
-        b->key_valid = true;
+        b->graphs.push_back(BatchState::Graph{*prm, b->shape.wave_rays, b->shape.refill, stream, g, exec});
     }
     for (int32_t f = 0; f < frames; ++f) {
-        e = hipGraphLaunch(b->exec, stream);
+        e = hipGraphLaunch(exec, stream);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: graph launch");
     }
     return AP_OK;
@@ -2755,8 +2790,8 @@ int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* cons
     const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
     const float* tables = nullptr;
     if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
-    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut)) != AP_OK) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut, st)) != AP_OK) return rc;
     hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);
     if (e != hipSuccess) return hip_fail(e, "ap_batch_render: reset counters");
     if ((rc = batch_run(&B, params, params->frame_index, params->spp, st)) != AP_OK) return rc;
@@ -2794,11 +2829,14 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
     const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
     const float* tables = nullptr;
     if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
-    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut)) != AP_OK) return rc;
-    if ((rc = batch_nrc_prepare(&B, nrc_params, ns)) != AP_OK) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut, st)) != AP_OK) return rc;
+    if ((rc = batch_nrc_prepare(&B, nrc_params, ns)) != AP_OK) return rc;
     hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);
-    if (e == hipSuccess) e = hipMemcpy(B.frame, &params->frame_index, sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(set_frame_kernel, dim3(1), dim3(64), 0, st, B.frame, params->frame_index);
+        e = hipGetLastError();
+    }
     if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: reset counters");
     // Frame after frame, each rendered with the networks as the frames before
     // it trained them; launched directly (the Adam step count changes per frame).
@@ -2863,7 +2901,7 @@ int ap_batch_get_launch(const ap_batch* batch, int* wave_rays, int* refill) {
 int ap_batch_set_graph(ap_batch* batch, int enable) {
     if (!batch) return fail(AP_ERR_INVALID_ARG, "ap_batch_set_graph: NULL batch");
     batch->s.use_graph = enable != 0;
-    if (!batch->s.use_graph) batch->s.drop_graph();
+    if (!batch->s.use_graph) batch->s.drop_graphs();
     return AP_OK;
 }
 
@@ -2876,8 +2914,8 @@ int ap_batch_autotune(ap_batch* batch, const ap_render_params* params, int frame
     const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
     const float* tables = nullptr;
     if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
-    if ((rc = batch_prepare(&B, params->triangle_material, nullptr, nullptr, rgb_lut)) != AP_OK) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((rc = batch_prepare(&B, params->triangle_material, nullptr, nullptr, rgb_lut, st)) != AP_OK) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);

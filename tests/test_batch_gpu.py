@@ -7,7 +7,9 @@ shading are those of ap_render, so each scene's running mean and FP64 sums
 must equal, bit for bit, what ap_render gives for that scene alone - with the
 graph replayed over several frames (each replay checked through the final
 image and the sums of every frame), without the graph, with four wavelengths
-per path, and after the autotuner has picked a launch shape.
+per path, and after the autotuner has picked a launch shape. Calls issued
+back to back on a non-blocking stream, with no host wait between them
+(progressive frames, new framebuffers, an autotune), keep their order.
 """
 import numpy as np
 import pytest
@@ -91,4 +93,52 @@ def test_autotuner_keeps_images(gpu):
     with pytest.raises(ap.ApertureError) as e:
         batch.render(ap.RenderParams.defaults(use_guiding=1), [None] * len(cams), stream=stream)
     assert e.value.code == ap.ERR_UNSUPPORTED
+    batch.close()
+
+
+def _buffers(cams):
+    hdrs, sums = [], []
+    for c in cams:
+        n = c.width * c.height
+        h = ap.DeviceBuffer(n * 12)
+        h.zero()
+        hdrs.append(h)
+        sums.append(ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64)))
+    return hdrs, sums
+
+
+def _read(cams, hdrs, sums):
+    out = []
+    for c, h, s in zip(cams, hdrs, sums):
+        n = c.width * c.height
+        out.append((h.to_numpy(np.float32, n * 3).reshape(c.height, c.width, 3),
+                    s.to_numpy(np.float64, n * 3).reshape(c.height, c.width, 3)))
+    return out
+
+
+def test_back_to_back_calls_without_waits(gpu):
+    """The frame counter is set on the caller's stream and the scene records
+    are rewritten only after the previous call's frames are done: two
+    progressive calls, a call into new framebuffers, an autotune over every
+    launch shape (one graph each, none destroyed) and a last call, all queued
+    with no host wait, equal the single-scene renders."""
+    objs, cams = scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    kw = dict(max_depth=12, rr_min_depth=4, seed=11)
+    a_h, a_s = _buffers(cams)
+    b_h, b_s = _buffers(cams)
+    c_h, c_s = _buffers(cams)
+    ap.synchronize()
+    batch.render(ap.RenderParams.defaults(spp=2, frame_index=0, **kw), a_h, a_s, stream=stream, count_rays=False)
+    batch.render(ap.RenderParams.defaults(spp=2, frame_index=2, **kw), a_h, a_s, stream=stream, count_rays=False)
+    batch.render(ap.RenderParams.defaults(spp=4, frame_index=0, **kw), b_h, b_s, stream=stream, count_rays=False)
+    batch.autotune(ap.RenderParams.defaults(spp=1, **kw), frames=1, stream=stream)
+    batch.render(ap.RenderParams.defaults(spp=4, frame_index=0, **kw), c_h, c_s, stream=stream, count_rays=False)
+    stream.synchronize()
+    prm = ap.RenderParams.defaults(spp=4, **kw)
+    for label, (h, s) in (("two progressive calls", (a_h, a_s)), ("new framebuffers", (b_h, b_s)),
+                          ("after an autotune", (c_h, c_s))):
+        check_against_single(scenes, cams, prm, _read(cams, h, s), label)
     batch.close()
