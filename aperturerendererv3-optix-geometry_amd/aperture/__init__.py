@@ -194,6 +194,7 @@ EXPORTED_SYMBOLS = (
     "ap_sdf_cache_add_functor", "ap_sdf_cache_add_volume", "ap_sdf_cache_get_volume", "ap_scene_set_sdf_cache",
     "ap_texture_cache_create", "ap_texture_cache_destroy", "ap_texture_cache_add_rgba8", "ap_texture_cache_count",
     "ap_texture_sample", "ap_scene_set_texture_cache", "ap_mesh_load_obj",
+    "ap_mse_f64", "ap_mse_luminance_f64", "ap_relmse_f64",
 )
 
 
@@ -295,6 +296,9 @@ def lib():
                                 ctypes.POINTER(Framebuffers), vp]
         L.ap_render_offline.argtypes = [vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), vp]
         L.ap_xyz_to_srgb.argtypes = [vp, sz, vp, vp]
+        L.ap_mse_f64.argtypes = [vp, vp, sz, vp, vp]
+        L.ap_mse_luminance_f64.argtypes = [vp, vp, sz, vp, vp]
+        L.ap_relmse_f64.argtypes = [ctypes.POINTER(ImageView), ctypes.POINTER(ImageView), sz, ctypes.c_double, vp, vp]
         L.ap_synth_box_scene.argtypes = [ctypes.c_uint64, i32, ctypes.c_float, vp, sz, ctypes.POINTER(sz)]
         L.ap_synth_heightfield_scene.argtypes = [i32, i32, ctypes.c_uint64, ctypes.c_float, vp, sz,
                                                  ctypes.POINTER(sz)]
@@ -522,6 +526,65 @@ def spectrum_interact(lam, rgb):
     check(lib().ap_spectrum_interact(d_l.ptr, d_c.ptr, lam.size, d_o.ptr, None))
     synchronize()
     return d_o.to_numpy(np.float32, lam.size)
+
+
+IMAGE_XYZ_F32, IMAGE_XYZ_SUMS_F64 = 0, 1
+REDUCE_BLOCKS = 1024
+REDUCE_MAX_ELEMENTS = 1 << 34
+
+
+class ImageView(ctypes.Structure):
+    """ap_image_view (include/aperture/ap_measure.h)."""
+    _fields_ = [("data", ctypes.c_void_p), ("kind", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("scale", ctypes.c_double)]
+
+    @classmethod
+    def of(cls, buf, kind=IMAGE_XYZ_F32, scale=1.0):
+        p = _ptr(buf)
+        return cls(p.value if isinstance(p, ctypes.c_void_p) else p, kind, 0, scale)
+
+
+def _reduce_out(out, stream):
+    """A device double for a reduction's result: the caller's, or a fresh one
+    read back (synchronously) when out is None."""
+    return (DeviceBuffer(8), True) if out is None else (out, False)
+
+
+def _reduce_result(d_out, fresh, stream):
+    if not fresh:
+        return None
+    if stream is not None:
+        stream.synchronize()
+    synchronize()
+    return float(d_out.to_numpy(np.float64, 1)[0])
+
+
+def mse_f64(a, b, n, out=None, stream=None):
+    """ap_mse_f64 of n floats of device buffers a, b (FP64 mean of squared
+    differences); with out=None the value is returned."""
+    d_out, fresh = _reduce_out(out, stream)
+    check(lib().ap_mse_f64(_ptr(a), _ptr(b), n, _ptr(d_out), _stream(stream)))
+    return _reduce_result(d_out, fresh, stream)
+
+
+def mse_luminance_f64(a, b, n_pixels, out=None, stream=None):
+    """ap_mse_luminance_f64 of two float3 device images (DeviceMSELuminance)."""
+    d_out, fresh = _reduce_out(out, stream)
+    check(lib().ap_mse_luminance_f64(_ptr(a), _ptr(b), n_pixels, _ptr(d_out), _stream(stream)))
+    return _reduce_result(d_out, fresh, stream)
+
+
+def relmse_f64(image, reference, n_pixels, eps=1e-2, out=None, stream=None):
+    """ap_relmse_f64: image and reference are ImageView (or a float3 XYZ
+    DeviceBuffer)."""
+    if not isinstance(image, ImageView):
+        image = ImageView.of(image)
+    if not isinstance(reference, ImageView):
+        reference = ImageView.of(reference)
+    d_out, fresh = _reduce_out(out, stream)
+    check(lib().ap_relmse_f64(ctypes.byref(image), ctypes.byref(reference), n_pixels, eps, _ptr(d_out),
+                              _stream(stream)))
+    return _reduce_result(d_out, fresh, stream)
 
 
 def xyz_to_srgb(xyz):

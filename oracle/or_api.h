@@ -129,6 +129,13 @@ void or_bsdf_sample(int model, const float* mat, float lambda, const float* rayD
 void or_nee_sample(const ap_scene_object* objs, size_t n, const float* p, size_t np, uint64_t seed, float* out);
 void or_nee_pdf(const ap_scene_object* objs, size_t n, int light, const float* q, size_t m, float* out);
 
+/* FP64 image statistics (or_measure.cpp): DeviceMSE, DeviceMSELuminance and
+ * relMSE; image kinds as AP_IMAGE_XYZ_F32 (0) / AP_IMAGE_XYZ_SUMS_F64 (1). */
+double or_mse(const float* a, const float* b, size_t n);
+double or_mse_luminance(const float* a, const float* b, size_t n);
+double or_relmse(const void* img, int img_kind, double img_scale, const void* ref, int ref_kind, double ref_scale,
+                 size_t n, double eps);
+
 /* Albedo textures for or_render (or_render.cpp): n RGBA8 images of
  * dims[2k] x dims[2k+1] texels (rows from the top) concatenated in rgba;
  * texture handle k + 1 names image k. or_texture_sample: tex2D of image

@@ -31,6 +31,11 @@ def lib():
             getattr(L, nm).argtypes = [f32, f32, f32]
         L.or_glass_bsdf.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp, ctypes.c_uint64, vp]
         L.or_plant.argtypes = [i32]
+        for nm in ("or_mse", "or_mse_luminance"):
+            getattr(L, nm).restype = ctypes.c_double
+            getattr(L, nm).argtypes = [vp, vp, sz]
+        L.or_relmse.restype = ctypes.c_double
+        L.or_relmse.argtypes = [vp, i32, ctypes.c_double, vp, i32, ctypes.c_double, sz, ctypes.c_double]
         L.or_ggx_vndf_sample.argtypes = [vp, vp, f32, vp, sz, vp]
         L.or_ggx_vndf_pdf.argtypes = [vp, vp, vp, sz, f32, vp]
         L.or_bsdf_eval.argtypes = [i32, vp, f32, vp, vp, vp, sz, vp, vp]
@@ -693,3 +698,25 @@ def nee_pdf(objs, light, points, light_points, light_normals):
     out = np.zeros(len(q), np.float32)
     lib().or_nee_pdf(_p(o), len(o), int(light), _p(q), len(q), _p(out))
     return out
+
+
+def mse(a, b):
+    a, b = _f32(a).reshape(-1), _f32(b).reshape(-1)
+    return lib().or_mse(_p(a), _p(b), a.size)
+
+
+def mse_luminance(a, b):
+    """DeviceMSELuminance of two (..., 3) float images."""
+    a, b = _f32(a).reshape(-1, 3), _f32(b).reshape(-1, 3)
+    return lib().or_mse_luminance(_p(a), _p(b), len(a))
+
+
+def relmse(img, ref, eps=1e-2, img_scale=1.0, ref_scale=1.0):
+    """relMSE on linear sRGB of two XYZ images, each float32 (a mean) or
+    float64 (sums, with its scale)."""
+    def arg(x):
+        x = np.ascontiguousarray(x)
+        return x, (0 if x.dtype == np.float32 else 1)
+    (a, ka), (b, kb) = arg(img), arg(ref)
+    assert a.dtype in (np.float32, np.float64) and b.dtype in (np.float32, np.float64)
+    return lib().or_relmse(_p(a), ka, float(img_scale), _p(b), kb, float(ref_scale), a.size // 3, float(eps))
