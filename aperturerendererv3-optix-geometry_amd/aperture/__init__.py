@@ -194,7 +194,7 @@ EXPORTED_SYMBOLS = (
     "ap_sdf_cache_add_functor", "ap_sdf_cache_add_volume", "ap_sdf_cache_get_volume", "ap_scene_set_sdf_cache",
     "ap_texture_cache_create", "ap_texture_cache_destroy", "ap_texture_cache_add_rgba8", "ap_texture_cache_count",
     "ap_texture_sample", "ap_scene_set_texture_cache", "ap_mesh_load_obj",
-    "ap_mse_f64", "ap_mse_luminance_f64", "ap_relmse_f64",
+    "ap_mse_f64", "ap_mse_luminance_f64", "ap_relmse_f64", "ap_relmse_f64_noalloc",
 )
 
 
@@ -299,6 +299,7 @@ def lib():
         L.ap_mse_f64.argtypes = [vp, vp, sz, vp, vp]
         L.ap_mse_luminance_f64.argtypes = [vp, vp, sz, vp, vp]
         L.ap_relmse_f64.argtypes = [ctypes.POINTER(ImageView), ctypes.POINTER(ImageView), sz, ctypes.c_double, vp, vp]
+        L.ap_relmse_f64_noalloc.argtypes = L.ap_relmse_f64.argtypes
         L.ap_synth_box_scene.argtypes = [ctypes.c_uint64, i32, ctypes.c_float, vp, sz, ctypes.POINTER(sz)]
         L.ap_synth_heightfield_scene.argtypes = [i32, i32, ctypes.c_uint64, ctypes.c_float, vp, sz,
                                                  ctypes.POINTER(sz)]

@@ -171,6 +171,31 @@ int device_partials(double** out) {
     return AP_OK;
 }
 
+// The partials of ap_relmse_f64_noalloc: a file-scope device array, one copy
+// per GPU at its own address. Host code never passes `scratch` itself to a
+// kernel (on the host the name is a stand-in, not a device address); it uses
+// the address hipGetSymbolAddress returns with the device current, taken on
+// the first call on each device and kept.
+__device__ __align__(256) double scratch[AP_REDUCE_BLOCKS];
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxDevices] = {};
+
+int device_scratch(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(AP_ERR_UNSUPPORTED, "reduction: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(reduction scratch)");
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
+    return AP_OK;
+}
+
 int check_image(const ap_image_view* v, const char* what) {
     if (!v || !v->data) return fail(AP_ERR_INVALID_ARG, std::string(what) + ": NULL image");
     if (v->kind != AP_IMAGE_XYZ_F32 && v->kind != AP_IMAGE_XYZ_SUMS_F64)
@@ -206,21 +231,34 @@ int ap_mse_luminance_f64(const float* a, const float* b, size_t n_pixels, double
                          "ap_mse_luminance_f64");
 }
 
-int ap_relmse_f64(const ap_image_view* image, const ap_image_view* reference, size_t n_pixels, double eps,
-                  double* out, void* stream) {
-    if (!out) return fail(AP_ERR_INVALID_ARG, "ap_relmse_f64: NULL out");
-    if (n_pixels > AP_REDUCE_MAX_ELEMENTS) return fail(AP_ERR_INVALID_ARG, "ap_relmse_f64: more than 2^34 pixels");
-    if (!(eps > 0.0) || !std::isfinite(eps)) return fail(AP_ERR_INVALID_ARG, "ap_relmse_f64: eps must be > 0");
+namespace {
+
+int relmse(const ap_image_view* image, const ap_image_view* reference, size_t n_pixels, double eps, double* out,
+           void* stream, bool no_alloc, const char* what) {
+    if (!out) return fail(AP_ERR_INVALID_ARG, std::string(what) + ": NULL out");
+    if (n_pixels > AP_REDUCE_MAX_ELEMENTS)
+        return fail(AP_ERR_INVALID_ARG, std::string(what) + ": more than 2^34 pixels");
+    if (!(eps > 0.0) || !std::isfinite(eps)) return fail(AP_ERR_INVALID_ARG, std::string(what) + ": eps must be > 0");
     int rc;
-    if (n_pixels > 0 && ((rc = check_image(image, "ap_relmse_f64")) != AP_OK ||
-                         (rc = check_image(reference, "ap_relmse_f64")) != AP_OK))
+    if (n_pixels > 0 && ((rc = check_image(image, what)) != AP_OK || (rc = check_image(reference, what)) != AP_OK))
         return rc;
     double* partials = nullptr;
-    if ((rc = device_partials(&partials)) != AP_OK) return rc;
+    if ((rc = no_alloc ? device_scratch(&partials) : device_partials(&partials)) != AP_OK) return rc;
     const ImageF64 a = n_pixels ? ImageF64{image->data, image->kind, image->scale} : ImageF64{};
     const ImageF64 b = n_pixels ? ImageF64{reference->data, reference->kind, reference->scale} : ImageF64{};
-    return launch_reduce(RelTerm{a, b, eps}, n_pixels, 3.0, partials, out, static_cast<hipStream_t>(stream),
-                         "ap_relmse_f64");
+    return launch_reduce(RelTerm{a, b, eps}, n_pixels, 3.0, partials, out, static_cast<hipStream_t>(stream), what);
+}
+
+}  // namespace
+
+int ap_relmse_f64(const ap_image_view* image, const ap_image_view* reference, size_t n_pixels, double eps,
+                  double* out, void* stream) {
+    return relmse(image, reference, n_pixels, eps, out, stream, false, "ap_relmse_f64");
+}
+
+int ap_relmse_f64_noalloc(const ap_image_view* image, const ap_image_view* reference, size_t n_pixels, double eps,
+                          double* out, void* stream) {
+    return relmse(image, reference, n_pixels, eps, out, stream, true, "ap_relmse_f64_noalloc");
 }
 
 }  // extern "C"

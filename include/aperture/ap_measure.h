@@ -56,6 +56,19 @@ int ap_mse_luminance_f64(const float* a, const float* b, size_t n_pixels, double
 int ap_relmse_f64(const ap_image_view* image, const ap_image_view* reference, size_t n_pixels, double eps,
                   double* out, void* stream);
 
+/* ap_relmse_f64 without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It calls no
+ * hipMalloc, hipFree or allocating library routine: its AP_REDUCE_BLOCKS FP64
+ * partials (8 KiB) live in a static device array of the library, one per GPU,
+ * held until the process ends. The same limit as ap_relmse_f64
+ * (AP_REDUCE_MAX_ELEMENTS pixels; larger inputs are rejected). Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams, or from several threads) race and must be serialised by
+ * the caller. Nothing in the array needs clearing between calls. */
+int ap_relmse_f64_noalloc(const ap_image_view* image, const ap_image_view* reference, size_t n_pixels, double eps,
+                          double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
