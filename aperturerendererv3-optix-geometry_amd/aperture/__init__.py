@@ -481,6 +481,25 @@ def synth_heightfield_scene(grid=1024, n_spheres=256, seed=4, light=20.0):
     return objs
 
 
+def synth_batch_scenes(n_scenes=48, seed=6):
+    """The batched heterogeneous workload: n_scenes heightfield + icosphere
+    scenes with triangle counts log-uniform in [1e4, 5e5] (4% of them in
+    320-triangle spheres, the rest a square heightfield grid; scene k seeded
+    100 + k, 4 thin-cube lights) and square resolutions uniform in
+    {128, 256, 512, 1024}, all viewed by the same camera. Returns (objects
+    per scene, Camera per scene)."""
+    g = np.random.default_rng(seed)
+    objs, cams = [], []
+    for k in range(n_scenes):
+        tris = float(np.exp(g.uniform(np.log(1e4), np.log(5e5))))
+        spheres = max(1, int(tris * 0.04 / 320))
+        grid = max(8, int(round(np.sqrt(max(tris - 320 * spheres, 128) / 2))))
+        res = int(g.choice([128, 256, 512, 1024]))
+        objs.append(synth_heightfield_scene(grid, spheres, 100 + k, 20.0))
+        cams.append(Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, res, res))
+    return objs, cams
+
+
 def spectrum_setup():
     check(lib().ap_spectrum_setup())
 

@@ -448,16 +448,8 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
     launch per bounce over the rays of all scenes, each frame a replayed HIP
     graph, launch shape autotuned. Timed against the same frames rendered
     scene by scene with ap_render, and against the batch without the graph."""
-    g = np.random.default_rng(seed)
-    objs, cams = [], []
     t0 = time.perf_counter()
-    for k in range(n_scenes):
-        tris = float(np.exp(g.uniform(np.log(1e4), np.log(5e5))))
-        spheres = max(1, int(tris * 0.04 / 320))
-        grid = max(8, int(round(np.sqrt(max(tris - 320 * spheres, 128) / 2))))
-        res = int(g.choice([128, 256, 512, 1024]))
-        objs.append(ap.synth_heightfield_scene(grid, spheres, 100 + k, 20.0))
-        cams.append(ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, res, res))
+    objs, cams = ap.synth_batch_scenes(n_scenes, seed)
     scenes = [ap.SceneRenderer(o) for o in objs]
     log(f"batch: {n_scenes} scenes, {sum(len(o) for o in objs)} objects, "
         f"{sum(c.width * c.height for c in cams)} pixels per frame ({time.perf_counter() - t0:.1f} s)")

@@ -1179,6 +1179,8 @@ struct NrcOpts {
     float spread;
     const float* params;
     float bmin[3], bmax[3];
+    int width = 64;  // hidden width of the cache network (64 or 32)
+    int mode = 1;    // its rounding points: 1 BF16, 2 FP16
 };
 constexpr int NRC_V = 8;
 
@@ -1397,7 +1399,8 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         // The cache's prediction, clamped at 0, at the path's wavelengths.
         float feat[64], y[3];
         or_nrc_encode(query, 1, N.bmin, N.bmax, feat);
-        or_nrc_forward(N.params, feat, 1, 1, y);
+        if (N.width == 64 && N.mode == 1) or_nrc_forward(N.params, feat, 1, 1, y);
+        else or_nrc_forward_w(N.params, N.width, feat, 1, N.mode, y);
         const float3 cv = fmax3(float3{y[0], y[1], y[2]}, float3{0.0f, 0.0f, 0.0f});
         for (int j = 0; j < nl; ++j) radiance[j] = radiance[j] + mask[j] * InteractC(wv.lb[j], cv) * wv.d65[j];
         if (queried) *queried = true;
@@ -1638,10 +1641,22 @@ int or_render_nrc(const or_bvh* bvh, const ap_scene_object* objs, size_t n, cons
                   const ap_render_params* prm, float cull_margin, const float* params, const float* bmin,
                   const float* bmax, float spread, uint32_t train_paths, uint32_t unbiased_every, float* hdr,
                   double* sums, float* samples, uint8_t* valid, uint32_t* n_queries, int threads) {
+    return or_render_nrc_w(bvh, objs, n, cam, prm, cull_margin, params, 64, 1, bmin, bmax, spread, train_paths,
+                           unbiased_every, hdr, sums, samples, valid, n_queries, threads);
+}
+
+// or_render_nrc with a network of hidden width `width` (64 or 32) and
+// rounding points `mode` (1 BF16, 2 FP16): a per-scene cache of a batch.
+int or_render_nrc_w(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                    const ap_render_params* prm, float cull_margin, const float* params, int width, int mode,
+                    const float* bmin, const float* bmax, float spread, uint32_t train_paths,
+                    uint32_t unbiased_every, float* hdr, double* sums, float* samples, uint8_t* valid,
+                    uint32_t* n_queries, int threads) {
+    if ((width != 64 && width != 32) || (mode != 1 && mode != 2)) return -1;
     const Setup U = MakeSetup(bvh, objs, n, cam, prm, cull_margin);
     const int64_t npix = static_cast<int64_t>(U.W) * U.H;
     const int nl = prm->wavelengths == 4 ? 4 : 1;
-    NrcOpts N{spread, params, {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]}};
+    NrcOpts N{spread, params, {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]}, width, mode};
     uint32_t stride = 1, n_train_max = 0;
     if (train_paths > 0) {
         stride = std::max<uint32_t>(1u, static_cast<uint32_t>(npix) / train_paths);

@@ -10,7 +10,13 @@ image and the sums of every frame), without the graph, with four wavelengths
 per path, and after the autotuner has picked a launch shape. Calls issued
 back to back on a non-blocking stream, with no host wait between them
 (progressive frames, new framebuffers, an autotune), keep their order.
+The batch is also compared directly with the oracle's CalculateRadiance per
+scene (bit-identical images and FP64 sums), and one frame of the full
+48-scene workload (ap.synth_batch_scenes: 7.4 M triangles, 13.7 M pixels)
+equals the scene-by-scene renders for every scene, its smallest scene the
+oracle too.
 """
+import oracle_lib as ol
 import numpy as np
 import pytest
 
@@ -142,3 +148,52 @@ def test_back_to_back_calls_without_waits(gpu):
                           ("after an autotune", (c_h, c_s))):
         check_against_single(scenes, cams, prm, _read(cams, h, s), label)
     batch.close()
+
+
+def oracle_render(objs, scene, cam, prm):
+    margin = np.float32(scene.info().line_margin)
+    hdr, sums, _ = ol.OracleBVH(objs).render(cam, prm, cull_margin=margin)
+    return hdr, sums
+
+
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_batch_matches_the_oracle(gpu, wavelengths):
+    objs, cams = scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    prm = ap.RenderParams.defaults(spp=2, max_depth=12, rr_min_depth=4, seed=13, wavelengths=wavelengths)
+    got, _ = batch_render(batch, cams, prm, stream)
+    for k, (o, sc, cam) in enumerate(zip(objs, scenes, cams)):
+        hdr, sums = oracle_render(o, sc, cam, prm)
+        ident = float(np.all(got[k][0] == hdr, axis=-1).mean())
+        print(f"[batch] vs oracle, {wavelengths} wavelengths: scene {k} {cam.width}x{cam.height}: "
+              f"pixels bit-identical {ident:.4f}", flush=True)
+        assert np.array_equal(got[k][0], hdr) and np.array_equal(got[k][1], sums), k
+    batch.close()
+
+
+def test_full_size_batch_workload_one_frame(gpu):
+    """One frame of the 48-scene workload as the bench renders it (depth 12,
+    roulette from depth 4, four wavelengths, no sky), autotuned, through the
+    captured graph."""
+    objs, cams = ap.synth_batch_scenes(48, 6)
+    print(f"[batch] full workload: {len(objs)} scenes, {sum(len(o) for o in objs)} objects, "
+          f"{sum(c.width * c.height for c in cams)} pixels", flush=True)
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    P = dict(max_depth=12, rr_min_depth=4, sky=0, wavelengths=4, seed=6)
+    w, r, ms = batch.autotune(ap.RenderParams.defaults(spp=1, **P), frames=1, stream=stream)
+    print(f"[batch] full workload: autotuned {w} rays per wave, refill {r}, {ms:.1f} ms per frame", flush=True)
+    prm = ap.RenderParams.defaults(spp=1, **P)
+    got, rays = batch_render(batch, cams, prm, stream)
+    print(f"[batch] full workload: rays {rays}", flush=True)
+    check_against_single(scenes, cams, prm, got, "48-scene workload")
+    k = int(np.argmin([c.width * c.height for c in cams]))
+    hdr, sums = oracle_render(objs[k], scenes[k], cams[k], prm)
+    print(f"[batch] full workload: scene {k} ({cams[k].width}x{cams[k].height}) against the oracle", flush=True)
+    assert np.array_equal(got[k][0], hdr) and np.array_equal(got[k][1], sums)
+    batch.close()
+    for sc in scenes:
+        sc.close()

@@ -14,8 +14,15 @@ Bars:
     sample counts equal the BF16 run's), training is deterministic and moves
     every network;
   * trained per-scene FP16 caches lower each scene's relMSE against a
-    4096-spp plain reference well below the plain batch's at equal spp.
+    4096-spp plain reference well below the plain batch's at equal spp;
+  * frozen width-32 FP16 caches against the oracle's cache renderer with the
+    same network (oracle_lib render_nrc width 32, FP16 rounding points), scene
+    by scene: equal cache-query counts, and pixels within the frozen-cache bar
+    of tests/test_nrc_render_gpu.py (1e-4 relative for 99.5%, 2e-2 for all:
+    the MFMA and the FP64 network round at the same points but sum in other
+    orders).
 """
+import oracle_lib as ol
 import numpy as np
 import pytest
 
@@ -130,5 +137,42 @@ def test_per_scene_fp16_caches_gain_over_the_plain_batch(gpu):
         print(f"[batch-nrc] scene {k}: relMSE plain {e_plain:.4g}, per-scene FP16 cache {e_nrc:.4g} "
               f"({e_plain / e_nrc:.2f}x lower)", flush=True)
         assert e_nrc < 0.5 * e_plain
+    batch.close()
+    cache.close()
+
+
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_frozen_fp16_caches_match_the_oracle(gpu, wavelengths):
+    objs, cams = bg.scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    prm = ap.RenderParams.defaults(spp=2, max_depth=10, rr_min_depth=3, seed=23, wavelengths=wavelengths)
+    nprm = ap.NrcRenderParams.defaults(train_paths=256, batch=512, train_steps=0, unbiased_every=4)
+    w = np.stack([ncpu.init_params_w(32, 70 + k) for k in range(len(scenes))])
+    cache = ap.RadianceCache(seed=1, groups=len(scenes), width=32, precision="fp16")
+    cache.set_params(w)
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    got = batch_render_nrc(batch, cams, prm, cache, nprm, stream)
+    stats = batch.nrc_last_frame()
+    for k, (o, sc, cam) in enumerate(zip(objs, scenes, cams)):
+        # The scene's cache bounds, as ap_render_nrc and the batch set them.
+        single = ap.RadianceCache(seed=1, width=32, precision="fp16")
+        single.set_params(w[k])
+        nr.render_nrc(sc, cam, ap.RenderParams.defaults(spp=1, max_depth=1), single, nprm)
+        lo, hi = single.bounds()
+        single.close()
+        margin = np.float32(sc.info().line_margin)
+        o_hdr, o_sums, _, o_nq = ol.OracleBVH(o).render_nrc(cam, prm, w[k], lo, hi, spread=nprm.spread,
+                                                            train_paths=nprm.train_paths,
+                                                            unbiased_every=nprm.unbiased_every, cull_margin=margin,
+                                                            width=32, mode=2)
+        rel = np.abs(got[k][1] - o_sums) / (np.abs(o_sums) + 1e-6)
+        ident = float(np.all(got[k][1] == o_sums, axis=-1).mean())
+        print(f"[batch-nrc] vs oracle, FP16 width 32, {wavelengths} wavelengths: scene {k}: queries "
+              f"{stats[k].queries} vs {o_nq}, pixels bit-identical {ident:.4f}, rel p99.5 "
+              f"{np.quantile(rel, 0.995):.3g} max {rel.max():.3g}", flush=True)
+        assert stats[k].queries == o_nq
+        assert np.quantile(rel, 0.995) < 1e-4 and rel.max() < 2e-2
+    assert sum(s.queries for s in stats) > 0
     batch.close()
     cache.close()

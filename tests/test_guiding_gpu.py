@@ -86,3 +86,14 @@ def test_guided_sphere_lights(gpu):
     cam = ap.Camera.make(rg.BOX_CAM["origin"], rg.BOX_CAM["direction"], rg.BOX_CAM["fov_deg"], 40, 40)
     guided_parity(objs, cam, dict(spp=3, max_depth=8), "box with sphere lights")
     guided_parity(objs, cam, dict(spp=2, max_depth=8, wavelengths=4), "box with sphere lights, four wavelengths")
+
+
+def test_guided_full_heightfield_workload_crop(gpu):
+    """The full 1080p heightfield workload (2,097,152 + 81,920 triangles, 4
+    lights, depth 16, roulette from depth 4) seen through a 48x48 centre crop
+    of its camera, guided over two frames with one and four wavelengths."""
+    objs = ap.synth_heightfield_scene(1024, 256, 4, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0 * 48 / 1080, 48, 48)
+    guided_parity(objs, cam, dict(spp=2, max_depth=16, rr_min_depth=4), "full heightfield, centre crop")
+    guided_parity(objs, cam, dict(spp=2, max_depth=16, rr_min_depth=4, wavelengths=4),
+                  "full heightfield, centre crop, four wavelengths")
