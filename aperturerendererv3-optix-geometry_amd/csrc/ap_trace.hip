@@ -13,12 +13,24 @@
 //    the re-test reduces to `tEnter <= closest`, with no second node fetch.
 //  * A 64-B NodeRec holds both children's boxes; leaves are runs of 48-B
 //    PrimRecs in slot order, ended by PRIM_LAST.
-// The top RING entries of the per-lane stack live in LDS as [entry][lane]
-// columns (conflict-free), older ones in the lane's private memory; 4 KB of
-// LDS per wave leaves occupancy to the register budget. The capacity is
-// picked on the host from the BVH depth; pushes beyond it are dropped exactly
-// as the reference drops pushes beyond its 128 entries
-// (intersection.cuh:258-267), which with capacity >= depth + 2 never happens.
+// One unit of work per lane and loop iteration: a pop (then the node's
+// expansion) or one primitive of the leaf the lane is in. A lane that pops a
+// leaf tests its first primitive in the same iteration, and the leaf's
+// primitives follow one per iteration in slot order, so each ray still visits
+// nodes and primitives in the reference's order; but lanes expanding nodes no
+// longer wait masked off while another lane loops over a leaf (measured: 1.6x
+// primaries, 2x secondaries on the 1M-triangle soup; the loop is VALU-issue
+// bound, and this lifts SIMD lane utilisation).
+// The near child of an expansion is popped at once by the reference, with the
+// same closest distance it was tested against, so it stays in registers as
+// the next node (its re-test cannot fail) and only far children go on the
+// stack. The top RING stack entries live in LDS as [entry][lane] columns
+// (conflict-free), older ones in the lane's private (scratch) memory, written
+// with ordinary cached stores; 4 KB of LDS per wave leaves occupancy to the
+// register budget. The capacity is picked on the host from the BVH depth;
+// pushes beyond it are dropped exactly as the reference drops pushes beyond
+// its 128 entries (intersection.cuh:258-267), which with capacity >= depth +
+// 2 never happens.
 #include <algorithm>
 
 #include "ap_internal.h"
@@ -60,12 +72,17 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                                                      const uint16_t* __restrict__ ray_scene = nullptr) {
     // Stack entries [lo, sp) live in an LDS ring of RING entries per lane
     // (position i % RING); entries [0, lo) that a deeper push evicted live in
-    // the lane's private array (scratch memory, rarely touched).
+    // the lane's private array (scratch memory: this file is built with
+    // alloca promotion to registers off, so the array stays in scratch).
     __shared__ int32_t st_link[RING * WAVE];
     __shared__ float st_te[RING * WAVE];
-    // volatile keeps the compiler from promoting these to registers.
-    volatile int32_t sp_link[CAP - RING];
-    volatile float sp_te[CAP - RING];
+    int32_t sp_link[CAP - RING];
+    float sp_te[CAP - RING];
+    // The next node to visit when has_cur (the near child, or the root).
+    bool has_cur = false;
+    int32_t cur_link = 0;
+    float cur_te = 0.0f;
+    int32_t leaf_slot = -1;  // >= 0: the next primitive of the leaf being tested
     int lo = 0;
     const int lane = threadIdx.x;
     // A device-side count (queues of the integrator) bounds a grid sized for capacity n_cap.
@@ -119,6 +136,8 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                     nprim = 0;
                     sp = 0;
                     lo = 0;
+                    has_cur = false;
+                    leaf_slot = -1;
                     bool finished = true;
                     RootInfo root = s.root;
                     float margin = s.line_margin;
@@ -141,9 +160,9 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
                         w = absf3(inv) * margin;
                         float te;
                         if (slab_enter<CULL>(o, inv, w, ld3(root.bmin), ld3(root.bmax), tmin, dist, te)) {
-                            st_link[lane] = root.link;
-                            st_te[lane] = te;
-                            sp = 1;
+                            cur_link = root.link;
+                            cur_te = te;
+                            has_cur = true;
                             finished = false;
                         } else if (STATS) {
                             nnode = 1;  // the reference pops and rejects the root
@@ -165,92 +184,107 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
         }
         if (!live) continue;
 
-        --sp;
-        int32_t link;
-        float te;
-        if (sp < lo) {
-            link = sp_link[sp];
-            te = sp_te[sp];
-            lo = sp;
-        } else {
-            link = st_link[(sp & (RING - 1)) * WAVE + lane];
-            te = st_te[(sp & (RING - 1)) * WAVE + lane];
-        }
-        if (STATS) ++nnode;
         bool any_done = false;
-        if (te <= dist) {
-            if (link < 0) {
-                int32_t slot = ~link;
-                for (;;) {
-                    const float4* pr = reinterpret_cast<const float4*>(prims + slot);
-                    const float4 A = pr[0];
-                    const float4 B = pr[1];
-                    const float4 C = pr[2];
-                    const uint32_t flags = __float_as_uint(B.w);
-                    const uint32_t kind = flags >> PRIM_KIND_SHIFT;
-                    float t = 0.0f;
-                    bool ok;
-                    if (kind == PRIM_KIND_TRIANGLE) {
-                        float u, v;
-                        ok = tri_mt(o, d, {A.x, A.y, A.z}, {B.x, B.y, B.z}, {C.x, C.y, C.z}, t, u, v);
-                    } else if (kind == PRIM_KIND_CUBE) {
-                        ok = cube_t(o, d, analytic[__float_as_int(C.w)], t);
-                    } else if (kind == PRIM_KIND_SPHERE) {
-                        ok = sphere_t(o, d, analytic[__float_as_int(C.w)], t);
-                    } else if (SDF && kind == PRIM_KIND_SDF) {
-                        // SDF_EPS of the ray's depth: tmin / 2 for closest hit (tmin is
-                        // INTERSECT_EPS of the depth), the launch's value for any hit.
-                        const float eps = ANY ? s.sdf_eps_any : (tmin > 0.0f ? tmin * 0.5f : kSdfEps0);
-                        ok = sdf_t(o, d, analytic[__float_as_int(C.w)], volumes, n_volumes, eps, t);
-                    } else {
-                        ok = false;
-                    }
-                    if (STATS) ++nprim;
-                    // SceneObjectsIntersector acceptance (intersection.cuh:160).
-                    if (ok && !(t < tmin || t >= dist)) {
-                        dist = t;
-                        hit = __float_as_int(A.w);
-                        if (ANY) { any_done = true; break; }
-                    }
-                    if (flags & PRIM_LAST) break;
-                    ++slot;
-                }
+        if (leaf_slot < 0) {
+            int32_t link;
+            float te;
+            if (has_cur) {
+                link = cur_link;
+                te = cur_te;
+                has_cur = false;
             } else {
-                const float4* nr = reinterpret_cast<const float4*>(nodes + link);
-                const float4 a = nr[0];
-                const float4 bb = nr[1];
-                const float4 c = nr[2];
-                const float4 e = nr[3];
-                if (STATS) nnode += 2;
-                float tl = 0.0f, tr = 0.0f;
-                const bool hl = slab_enter<CULL>(o, inv, w, {a.x, a.y, a.z}, {bb.x, bb.y, bb.z}, tmin, dist, tl);
-                const bool hr = slab_enter<CULL>(o, inv, w, {c.x, c.y, c.z}, {e.x, e.y, e.z}, tmin, dist, tr);
-                const int32_t ll = __float_as_int(a.w);
-                const int32_t rl = __float_as_int(bb.w);
-                auto push = [&](int32_t l, float t) {
-                    if (sp >= CAP) return;
-                    if (sp - lo == RING) {
-                        const int k = (lo & (RING - 1)) * WAVE + lane;
-                        sp_link[lo] = st_link[k];
-                        sp_te[lo] = st_te[k];
-                        ++lo;
+                --sp;
+                if (sp < lo) {
+                    link = sp_link[sp];
+                    te = sp_te[sp];
+                    lo = sp;
+                } else {
+                    link = st_link[(sp & (RING - 1)) * WAVE + lane];
+                    te = st_te[(sp & (RING - 1)) * WAVE + lane];
+                }
+            }
+            if (STATS) ++nnode;
+            if (te <= dist) {
+                if (link < 0) {
+                    leaf_slot = ~link;   // its first primitive is tested below, in this iteration
+                } else {
+                    const float4* nr = reinterpret_cast<const float4*>(nodes + link);
+                    const float4 a = nr[0];
+                    const float4 bb = nr[1];
+                    const float4 c = nr[2];
+                    const float4 e = nr[3];
+                    if (STATS) nnode += 2;
+                    float tl = 0.0f, tr = 0.0f;
+                    const bool hl = slab_enter<CULL>(o, inv, w, {a.x, a.y, a.z}, {bb.x, bb.y, bb.z}, tmin, dist, tl);
+                    const bool hr = slab_enter<CULL>(o, inv, w, {c.x, c.y, c.z}, {e.x, e.y, e.z}, tmin, dist, tr);
+                    const int32_t ll = __float_as_int(a.w);
+                    const int32_t rl = __float_as_int(bb.w);
+                    if (hl && hr) {
+                        // Far child on the stack, near child next; ties keep left near.
+                        const bool left_far = tl > tr;
+                        if (sp < CAP) {
+                            if (sp - lo == RING) {
+                                const int k = (lo & (RING - 1)) * WAVE + lane;
+                                sp_link[lo] = st_link[k];
+                                sp_te[lo] = st_te[k];
+                                ++lo;
+                            }
+                            const int k = (sp & (RING - 1)) * WAVE + lane;
+                            st_link[k] = left_far ? ll : rl;
+                            st_te[k] = left_far ? tl : tr;
+                            ++sp;
+                        }
+                        // The reference pushes the near child above the far one
+                        // and drops a push at capacity, so the near child is kept
+                        // only while a slot is left for it.
+                        if (sp < CAP) {
+                            cur_link = left_far ? rl : ll;
+                            cur_te = left_far ? tr : tl;
+                            has_cur = true;
+                        }
+                    } else if ((hl || hr) && sp < CAP) {
+                        cur_link = hl ? ll : rl;
+                        cur_te = hl ? tl : tr;
+                        has_cur = true;
                     }
-                    const int k = (sp & (RING - 1)) * WAVE + lane;
-                    st_link[k] = l;
-                    st_te[k] = t;
-                    ++sp;
-                };
-                if (hl && hr) {
-                    // Far child first, near child on top; ties keep left on top.
-                    const bool left_far = tl > tr;
-                    push(left_far ? ll : rl, left_far ? tl : tr);
-                    push(left_far ? rl : ll, left_far ? tr : tl);
-                } else if (hl || hr) {
-                    push(hl ? ll : rl, hl ? tl : tr);
                 }
             }
         }
-        if (sp == 0 || any_done) {
+        if (leaf_slot >= 0) {
+            // One primitive of the current leaf per iteration, in slot order.
+            const float4* pr = reinterpret_cast<const float4*>(prims + leaf_slot);
+            const float4 A = pr[0];
+            const float4 B = pr[1];
+            const float4 C = pr[2];
+            const uint32_t flags = __float_as_uint(B.w);
+            const uint32_t kind = flags >> PRIM_KIND_SHIFT;
+            float t = 0.0f;
+            bool ok;
+            if (kind == PRIM_KIND_TRIANGLE) {
+                float u, v;
+                ok = tri_mt(o, d, {A.x, A.y, A.z}, {B.x, B.y, B.z}, {C.x, C.y, C.z}, t, u, v);
+            } else if (kind == PRIM_KIND_CUBE) {
+                ok = cube_t(o, d, analytic[__float_as_int(C.w)], t);
+            } else if (kind == PRIM_KIND_SPHERE) {
+                ok = sphere_t(o, d, analytic[__float_as_int(C.w)], t);
+            } else if (SDF && kind == PRIM_KIND_SDF) {
+                // SDF_EPS of the ray's depth: tmin / 2 for closest hit (tmin is
+                // INTERSECT_EPS of the depth), the launch's value for any hit.
+                const float eps = ANY ? s.sdf_eps_any : (tmin > 0.0f ? tmin * 0.5f : kSdfEps0);
+                ok = sdf_t(o, d, analytic[__float_as_int(C.w)], volumes, n_volumes, eps, t);
+            } else {
+                ok = false;
+            }
+            if (STATS) ++nprim;
+            // SceneObjectsIntersector acceptance (intersection.cuh:160).
+            if (ok && !(t < tmin || t >= dist)) {
+                dist = t;
+                hit = __float_as_int(A.w);
+                if (ANY) any_done = true;
+            }
+            leaf_slot = (flags & PRIM_LAST) || any_done ? -1 : leaf_slot + 1;
+        }
+        if ((leaf_slot < 0 && !has_cur && sp == 0) || any_done) {
             *reinterpret_cast<int2*>(hits + ri) = int2{hit, __float_as_int(dist)};
             if (STATS) stats[ri] = ap_trace_stats{nnode, nprim};
             live = false;
