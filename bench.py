@@ -84,20 +84,76 @@ def cpu_baseline(objs, rays_a, rays_b, seconds, threads):
         "cores": threads,
         "kind": "port",
         "sample": f"{n} rays drawn at random from the primary and secondary sets (equal shares), oracle "
-                  f"restatement of the reference TraverseBVH (its own slab test), OpenMP, {dt:.1f} s; the same "
+                  f"restatement of the reference TraverseBVH (its own slab test), OpenMP on {threads} threads "
+                  f"(the host's CPU share per GPU; nproc {os.cpu_count()}), {dt:.1f} s; the same "
                   f"sample with the core's line culling ran at {n / dt_cull / 1e6:.4f} Mrays/s",
     }
 
 
-SRGB = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
-                 [0.0556434, -0.2040259, 1.0572252]])
+def rel_mse(ap, img, ref, n, stream):
+    """relMSE of two device XYZ images (mean over pixels and linear-sRGB
+    channels of (I - I_ref)^2 / (I_ref^2 + 1e-2)), reduced in FP64 on the GPU
+    by ap_relmse_f64."""
+    return ap.relmse_f64(img, ref, n, eps=1e-2, stream=stream)
 
 
-def rel_mse(xyz, ref_xyz):
-    """mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2) on linear sRGB, FP64."""
-    a = np.maximum(xyz.reshape(-1, 3).astype(np.float64) @ SRGB.T, 0.0)
-    b = np.maximum(ref_xyz.reshape(-1, 3).astype(np.float64) @ SRGB.T, 0.0)
-    return float((((a - b) ** 2) / (b ** 2 + 1e-2)).mean())
+def c1_bench(ap, threads, cpu=True):
+    """The closed-box workload as specified for the CPU baseline: 5 walls (10
+    triangles), a 0.5x0.5 emissive ceiling quad (thin cube, radiance 15),
+    two 1280-triangle icospheres, Lambert albedos from seed 1, 256x256
+    pinhole (40 deg), 16 spp, max depth 8, one NEE sample at every vertex, no
+    sky. The GPU renders it (HIP events, after one untimed frame); the
+    oracle's restatement of the reference integrator renders the same 16
+    frames with the reference TraverseBVH (its own slab test, no culling) on
+    `threads` host threads. Both trace the same paths, so the CPU's rays are
+    the GPU's ray count, and the two images and primary-hit ID buffers must
+    be bit-identical."""
+    objs = ap.synth_box_scene(1, 3, 15.0)
+    W = H = 256
+    cam = ap.Camera.make([0.0, 0.0, 3.4], [0.0, 0.0, -1.0], 40.0, W, H)
+    prm = ap.RenderParams.defaults(spp=16, max_depth=8, nee_always=1, sky=0, seed=1)
+    scene = ap.SceneRenderer(objs)
+    stream = ap.Stream()
+    n = W * H
+    hdr, prim, counts = ap.DeviceBuffer(n * 12), ap.DeviceBuffer(n * 4), ap.DeviceBuffer(16)
+    hdr.zero(stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=1, max_depth=8, nee_always=1, sky=0, seed=99), hdr,
+                 stream=stream)
+    hdr.zero(stream)
+    counts.zero(stream)
+    e0, e1 = ap.Event(), ap.Event()
+    e0.record(stream)
+    scene.render(cam, prm, hdr, stream=stream, primary_id=prim, ray_count=counts)
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1)
+    rays = int(counts.to_numpy(np.uint64, 2).sum())
+    out = {"workload": "closed box: 5 walls (10 triangles), 0.5x0.5 ceiling light (thin cube, radiance 15), two "
+                       "1280-triangle icospheres, Lambert albedos from seed 1, 256x256 pinhole 40 deg, 16 spp, max "
+                       "depth 8, NEE at every vertex, no sky",
+           "rays": rays,
+           "gpu": {"ms": round(ms, 3), "samples_per_s": round(n * 16 / (ms / 1e3), 1),
+                   "mrays_per_s": round(rays / (ms / 1e3) / 1e6, 1)}}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib as ol
+
+        bvh = ol.OracleBVH(objs)
+        t0 = time.perf_counter()
+        o_hdr, _, o_prim = bvh.render(cam, prm, cull_margin=0.0, threads=threads)
+        dt = time.perf_counter() - t0
+        g_hdr = hdr.to_numpy(np.float32, n * 3).reshape(H, W, 3)
+        g_prim = prim.to_numpy(np.int32, n).reshape(H, W)
+        out["cpu"] = {"s": round(dt, 2), "samples_per_s": round(n * 16 / dt, 1), "mrays_per_s": round(rays / dt / 1e6, 4),
+                      "threads": threads, "nproc": os.cpu_count(), "kind": "port",
+                      "what": "oracle restatement of the reference integrator and its TraverseBVH (no culling), "
+                              "OpenMP over pixels, the full 16 spp"}
+        out["image_bit_identical"] = bool(np.array_equal(g_hdr, o_hdr))
+        out["primary_ids_bit_identical"] = bool(np.array_equal(g_prim, o_prim))
+        out["gpu_over_cpu"] = round(dt / (ms / 1e3), 1)
+    scene.close()
+    log(f"c1: {out}")
+    return out
 
 
 def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_passes, boost_frames=0,
@@ -121,7 +177,6 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     scene.render(cam, ap.RenderParams.defaults(spp=ref_spp, seed=1000, wavelengths=4, **params), ref, stream=stream)
     stream.synchronize()
     log(f"nrc: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
-    ref_img = ref.to_numpy(np.float32, n * 3)
     img = ap.DeviceBuffer(n * 12)
     img.zero(stream)
     e0, e1 = ap.Event(), ap.Event()
@@ -130,7 +185,7 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     e1.record(stream)
     stream.synchronize()
     budget = e0.elapsed_ms(e1)
-    plain = rel_mse(img.to_numpy(np.float32, n * 3), ref_img)
+    plain = rel_mse(ap, img, ref, n, stream)
     cache = ap.RadianceCache(seed=1, width=width, precision=precision)
     nprm = ap.NrcRenderParams.defaults(train_paths=65536, train_passes=train_passes, spread=spread)
     scratch = ap.DeviceBuffer(n * 12)
@@ -181,7 +236,7 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     used = e0.elapsed_ms(e1)
     frames = total
     st, _ = scene.nrc_last_frame(samples=False)
-    nrc = rel_mse(img.to_numpy(np.float32, n * 3), ref_img) if frames > warmup else float("nan")
+    nrc = rel_mse(ap, img, ref, n, stream) if frames > warmup else float("nan")
     cache.close()
     out = {
         "equal_time_ms": round(budget, 1),
@@ -548,14 +603,13 @@ def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_fr
             out.append(h)
         return out
 
-    def images(bufs):
-        return [b.to_numpy(np.float32, c.width * c.height * 3) for b, c in zip(bufs, cams)]
+    def rel(bufs, refs):
+        return [rel_mse(ap, b, r, c.width * c.height, stream) for b, r, c in zip(bufs, refs, cams)]
 
     t0 = time.perf_counter()
     ref = buffers()
     batch.render(ap.RenderParams.defaults(spp=ref_spp, seed=1000, **params), ref, stream=stream)
     stream.synchronize()
-    ref_imgs = images(ref)
     log(f"batch nrc: {n_scenes} scenes, {pix} pixels per frame, reference {ref_spp} spp in "
         f"{time.perf_counter() - t0:.1f} s")
     e0, e1 = ap.Event(), ap.Event()
@@ -568,7 +622,7 @@ def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_fr
     e1.record(stream)
     stream.synchronize()
     budget = e0.elapsed_ms(e1)
-    plain = [rel_mse(a, b) for a, b in zip(images(img), ref_imgs)]
+    plain = rel(img, ref)
     cache = ap.RadianceCache(seed=1, groups=n_scenes, width=32, precision="fp16")
     w0 = cache.params()
     nprm = ap.NrcRenderParams.defaults(train_paths=8192, batch=4096)
@@ -605,7 +659,7 @@ def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_fr
     stream.synchronize()
     used = e0.elapsed_ms(e1)
     st = batch.nrc_last_frame()
-    nrc = [rel_mse(a, b) for a, b in zip(images(img), ref_imgs)]
+    nrc = rel(img, ref)
     cache.close()
     batch.close()
     for sc in scenes:
@@ -661,6 +715,7 @@ def main():
                      help="plain spp that sets the equal-time budget of the per-scene cache batch comparison")
     ap_.add_argument("--batch-nrc-ref-spp", type=int, default=1024, help="spp of its plain reference images")
     ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
+    ap_.add_argument("--c1", type=int, default=1, help="run the 256x256 closed-box workload on GPU and CPU (0 skips it)")
     ap_.add_argument("--sweep", action="store_true",
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
     ap_.add_argument("--copy-probe", action="store_true",
@@ -832,6 +887,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(objs, prim, sec, args.cpu_seconds, threads)
     scene.close()
     del objs, prim, sec
+    if args.c1:
+        line["c1"] = c1_bench(ap, threads, cpu=not args.no_cpu)
     if args.render_spp > 0:
         line["render"] = render_bench(ap, args.render_spp, 0.0 if args.no_cpu else 10.0, threads,
                                       nrc_spp=args.nrc_spp, nrc_ref_spp=args.nrc_ref_spp,
