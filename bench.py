@@ -36,15 +36,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/), if committed."""
+def read_profile():
+    """Figures measured by rocprofv3 --pmc (profiles/trace_closest_c2_traffic.txt), if
+    committed: HBM bytes per launch, VALU issue fraction and SIMD lane use."""
     path = os.path.join(REPO, "profiles", "trace_closest_c2_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            key, _, val = line.partition(":")
+            if key in ("traffic_bytes_per_launch", "valu_issue_primary", "valu_issue_secondary", "lane_use_primary",
+                       "lane_use_secondary"):
+                out[key] = float(val)
+    return out
 
 
 def cpu_baseline(objs, rays_a, rays_b, seconds, threads):
@@ -841,7 +844,8 @@ def main():
     bytes_step = algo["primary"]["bytes"] + algo["secondary"]["bytes"]
     kernel_ms = t_prim + t_sec
     achieved = bytes_step / (kernel_ms / 1e3) / 1e9
-    traffic = read_traffic()
+    prof = read_profile()
+    traffic = prof.get("traffic_bytes_per_launch")
     log(f"primary {t_prim:.3f} ms ({n_prim / t_prim / 1e3:.0f} Mrays/s), secondary {t_sec:.3f} ms "
         f"({n_sec / t_sec / 1e3:.0f} Mrays/s)")
 
@@ -879,6 +883,10 @@ def main():
             "avg_launch_ms": round(kernel_ms / 2, 4),
             "nodes_per_ray": {k: round(v["nodes_per_ray"], 2) for k, v in algo.items()},
             "tests_per_ray": {k: round(v["tests_per_ray"], 2) for k, v in algo.items()},
+            "note": "algorithmic bytes over HBM peak; the records (69 MB) are served mostly from L2 and MALL, so "
+                    "frac can exceed 1: the kernel is bound by VALU issue (valu_issue, from the committed counters)",
+            "valu_issue": {k: prof.get(f"valu_issue_{k}") for k in ("primary", "secondary")},
+            "lane_use": {k: prof.get(f"lane_use_{k}") for k in ("primary", "secondary")},
         },
         "cpu_baseline": None,
     }
