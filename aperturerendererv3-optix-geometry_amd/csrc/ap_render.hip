@@ -1191,18 +1191,26 @@ __global__ __launch_bounds__(kBlock) void nrc_init_kernel(int32_t n_pix, uint32_
 
 // Radiance of a cache vertex: throughput x the prediction c (clamped at 0)
 // at the path's wavelengths.
+// With a guiding record, the prediction is also the path's endpoint radiance
+// for CollectRadiance: it is what the traced tail would have brought to the
+// open collection slots, weighted by their throughputs as an emitter hit is.
 template <int NL>
 __device__ __forceinline__ void nrc_apply_one(PathT<NL>& S, f3 c, const f4* __restrict__ rgb_lut,
-                                              const f4* __restrict__ lam_lut, const float* __restrict__ tables) {
+                                              const f4* __restrict__ lam_lut, const float* __restrict__ tables,
+                                              GuideRec* guide = nullptr) {
     c = fmax3(c, f3{0.0f, 0.0f, 0.0f});
     // interact() with the RGB basis looked up once for all wavelengths.
     const float lum = max3(c);
     const f4 cb = query_rgb_basis(rgb_lut, lum > 1.0f ? c / lum : c);
+    float le_end[NL];
     for (int j = 0; j < NL; ++j) {
         const f4 lb = query_lambda(lam_lut, S.lam[j]);
         const float le = lum > 1.0f ? spectral_interact(cb, lb) * lum : spectral_interact(cb, lb);
-        S.radiance[j] = S.radiance[j] + S.mask[j] * le * d65_norm(tables, S.lam[j]);
+        const float d65 = d65_norm(tables, S.lam[j]);
+        S.radiance[j] = S.radiance[j] + S.mask[j] * le * d65;
+        le_end[j] = le * d65;
     }
+    if (guide) pg_endpoint<NL>(*guide, le_end, S.ndd);
 }
 
 template <int NL>
@@ -1211,10 +1219,12 @@ __global__ __launch_bounds__(kBlock) void nrc_apply_kernel(const uint32_t* __res
                                                            const float* __restrict__ rgb, const f4* __restrict__ rgb_lut,
                                                            const f4* __restrict__ lam_lut,
                                                            const float* __restrict__ tables,
-                                                           PathT<NL>* __restrict__ paths) {
+                                                           PathT<NL>* __restrict__ paths, GuideRec* __restrict__ recs) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= counters[0]) return;
-    nrc_apply_one<NL>(paths[qpath[q]], f3{rgb[3 * q], rgb[3 * q + 1], rgb[3 * q + 2]}, rgb_lut, lam_lut, tables);
+    const int32_t pid = qpath[q];
+    nrc_apply_one<NL>(paths[pid], f3{rgb[3 * q], rgb[3 * q + 1], rgb[3 * q + 2]}, rgb_lut, lam_lut, tables,
+                      recs ? recs + pid : nullptr);
 }
 
 __device__ __forceinline__ bool finite3(f3 v) {
@@ -1978,7 +1988,13 @@ int run_frames(const RunCtx& c) {
             sp.media = r->media;
             sp.textures = texture_cache_table(s->tex_cache);
             sp.uvs = r->uvs;
-            if (guiding) {
+            if (guiding && nrc && !train_only) {
+                hipLaunchKernelGGL((shade_kernel<NL, true, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
+                                   r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
+                                   paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
+                                   r->path_of[out_q], r->shadow_rays, pays, r->counters, prim, last_frame,
+                                   rc_ptr, gv, nv);
+            } else if (guiding && !nrc) {
                 hipLaunchKernelGGL((shade_kernel<NL, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
                                    r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
                                    paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
@@ -2017,7 +2033,7 @@ int run_frames(const RunCtx& c) {
                                      stream)) != AP_OK)
                 return rc2;
             hipLaunchKernelGGL(nrc_apply_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->nrc_counters,
-                               r->nrc_qpath, r->nrc_qrgb, rgb_lut, lam_lut, tables, paths);
+                               r->nrc_qpath, r->nrc_qrgb, rgb_lut, lam_lut, tables, paths, gv.recs);
             if (n_train > 0) {
                 hipLaunchKernelGGL(nrc_targets_kernel<NL>, grid_of(n_train), dim3(kBlock), 0, stream, n_train,
                                    n_stride, n_offset, train_only ? 1u : 0u, r->nrc_paths, paths, tables,
@@ -2647,8 +2663,6 @@ int ap_render_nrc(ap_scene* scene, const ap_camera* camera, const ap_render_para
     if (!cache || !nrc_params) return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: NULL cache or parameters");
     if (nrc_groups(nrc_state(cache)) != 1)
         return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: the cache must hold one network");
-    if (params && params->use_guiding)
-        return fail(AP_ERR_UNSUPPORTED, "ap_render_nrc: guiding is not supported with the radiance cache");
     if (!(nrc_params->spread > 0.0f) || !(nrc_params->learning_rate >= 0.0f))
         return fail(AP_ERR_INVALID_ARG, "ap_render_nrc: spread must be > 0 and learning_rate >= 0");
     if (nrc_params->train_steps > 0 &&

@@ -80,6 +80,12 @@ int or_render_nrc_w(const or_bvh* bvh, const ap_scene_object* objs, size_t n, co
                     const float* bmin, const float* bmax, float spread, uint32_t train_paths,
                     uint32_t unbiased_every, float* hdr, double* sums, float* samples, uint8_t* valid,
                     uint32_t* n_queries, int threads);
+typedef struct or_guide or_guide;
+int or_render_nrc_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                         const ap_render_params* prm, float cull_margin, const float* params, int width, int mode,
+                         const float* bmin, const float* bmax, float spread, uint32_t train_paths,
+                         uint32_t unbiased_every, float* hdr, double* sums, float* samples, uint8_t* valid,
+                         uint32_t* n_queries, int threads, or_guide* guide);
 
 /* Spectral restatement (or_spectrum.cpp). Tables come from the core's data files. */
 int or_spectrum_load(const char* mlp_path, const char* cie_path);

@@ -1190,16 +1190,23 @@ struct NrcTrain {
     bool valid[NRC_V];
 };
 
-// CalculateRadiance without guiding, with the core's cache termination: a
-// path ends into the cache at the first scattering vertex where the squared
-// spread sum exceeds spread * a0; a training path restarts the sum at its
-// first crossing and ends at the second (an unbiased one never ends there).
-// Training paths record each scattering vertex before the cache vertex.
+// CalculateRadiance with the core's cache termination: a path ends into the
+// cache at the first scattering vertex where the squared spread sum exceeds
+// spread * a0; a training path restarts the sum at its first crossing and
+// ends at the second (an unbiased one never ends there). Training paths
+// record each scattering vertex before the cache vertex. With a guiding
+// state G the sampling is guided as in CalculateRadiance, and a path that
+// ends into the cache collects the prediction as its endpoint radiance.
 static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir,
                                    bool inside0, const NrcOpts& N, bool train, bool unbiased, int pixel,
-                                   NrcTrain* out, bool* queried, int nl = 1) {
+                                   NrcTrain* out, bool* queried, int nl = 1, const GuideState* G = nullptr,
+                                   int frame = 0, ap_path_guiding_sample* slots = nullptr) {
     const Waves wv = MakeWaves(rng, nl);
     const float* lb = wv.lb[0];
+    const bool guiding = G != nullptr;
+    Collect col;
+    if (guiding) col.Clear(nl);
+    float3 gquery{1.0f, 1.0f, 1.0f};  // maskForProbeQuery
     int ndd = 0;
     dir = normalize(dir);
     float radiance[MAXL], mask[MAXL];
@@ -1219,6 +1226,13 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
     float vrec[NRC_V][16], vR[NRC_V][MAXL], vM[NRC_V][MAXL];
     bool cacheEnd = false;
     float query[16];
+    int endNdd = 0;
+    auto updateQuery = [&]() {
+        float3 t = wv.xyz[0] * mask[0];
+        for (int j = 1; j < nl; ++j) t = t + wv.xyz[j] * mask[j];
+        gquery = fmax3(t, float3{PG_QUERY_EPS, PG_QUERY_EPS, PG_QUERY_EPS});
+        gquery = gquery / static_cast<float>(static_cast<double>(gquery.x + gquery.y + gquery.z) + 1.0e-6);
+    };
     if (out) for (int k = 0; k < NRC_V; ++k) out->valid[k] = false;
     if (queried) *queried = false;
     auto record = [&](float* q, float3 pos, const Ctx& c) {
@@ -1234,13 +1248,17 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         int id;
         float t;
         TraceMedia(S, media, ro, rd, eps, &id, &t);
+        const float alpha = GuidingAlpha(frame);
         if (id < 0) {
             (void)rng.rand1();
             if (!media.inside) {
+                float endpoint[MAXL] = {};
                 for (int j = 0; j < nl; ++j) {
                     const float le = P.sky ? InteractC(wv.lb[j], SkyLighting(rd)) : 0.0f;
                     radiance[j] = radiance[j] + mask[j] * le * wv.d65[j];
+                    endpoint[j] = le * wv.d65[j];
                 }
+                if (guiding) col.Endpoint(endpoint, ndd);
                 break;
             }
             // A miss inside a medium (bMissThinSurfaceInsideMedia_, render.cuh:109-126,
@@ -1257,10 +1275,12 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             }
             float contrib[MAXL];
             for (int j = 0; j < nl; ++j) contrib[j] = 1.0f / std::fmax(1.0f, 1e-12f);
+            if (guiding) col.Advance(rd, contrib, p, ndd, false, rd, float3{0, 0, 0}, float3{0, 0, 0});
             for (int j = 0; j < nl; ++j) {
                 mask[j] = mask[j] * contrib[j];
                 mask[j] = mask[j] / p;
             }
+            if (guiding) updateQuery();
             media = MediaCtx{};
             media.vertex = ro;
             ro = float3{0, 0, 0};
@@ -1280,6 +1300,34 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         const bool inside = media.inside;
         const float offEps = eps * std::fmax(1.0f, length(c.pos));
         c.pos = c.pos + sign(dot(c.normal, sc.next)) * c.normal * offEps * (rng.rand1() + 0.5f);
+        // Directional guiding of the sampled direction (render.cuh:546-639).
+        const ap_octahedron* probe = nullptr;
+        if (guiding && !sc.delta && !sc.pureLight && !inside && !G->probes.empty()) {
+            probe = G->Query(c.pos, Reflect(rd, c.normal));
+            if (probe) {
+                const float pB = sc.pdf;
+                const float pP = OctahedronHemispherePDF(*probe, sc.next, c.normal, gquery);
+                if (pB > 1.0e-12f && pP > 1.0e-12f) {
+                    if (rng.rand1() < alpha) {
+                        const GuideSample gs = SampleOctahedronHemisphere(*probe, rng, c.normal, gquery);
+                        if (dot(gs.dir, c.normal) > 1.0e-12f) {
+                            const float pBS = BSDF_PDF(lb, model, c.gb, c.normal, rd, gs.dir);
+                            sc.next = gs.dir;
+                            for (int j = 0; j < nl; ++j)
+                                brdf[j] = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, gs.dir, ct, mediaIn);
+                            sc.pdf = (1.0f - alpha) * std::fmax(pBS, 1.0e-12f) + alpha * gs.pdf;
+                        } else {
+                            probe = nullptr;
+                        }
+                    } else {
+                        sc.pdf = (1.0f - alpha) * pB + alpha * pP;
+                    }
+                } else {
+                    probe = nullptr;
+                }
+            }
+        }
+        float endpoint[MAXL] = {};
         if (c.lightIndex >= 0) {
             float misW = 1.0f;
             if (P.use_nee && depth > 0 && prevNEE) {
@@ -1289,7 +1337,12 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             }
             for (int j = 0; j < nl; ++j)
                 radiance[j] = radiance[j] + mask[j] * InteractC(wv.lb[j], sc.emissive) * wv.d65[j] * misW;
+            if (guiding)
+                for (int j = 0; j < nl; ++j) endpoint[j] = InteractC(wv.lb[j], sc.emissive * misW) * wv.d65[j];
+        } else if (guiding && !(sc.emissive.x == 0.0f && sc.emissive.y == 0.0f && sc.emissive.z == 0.0f)) {
+            for (int j = 0; j < nl; ++j) endpoint[j] = InteractC(wv.lb[j], sc.emissive) * wv.d65[j];
         }
+        if (guiding) col.Endpoint(endpoint, ndd);
         // Spread heuristic (the core's shade kernel, NRC branch).
         {
             const float cosI = std::fabs(dot(rd, c.normal));
@@ -1317,6 +1370,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             }
             if (cacheEnd) {
                 record(query, pos0, c);
+                endNdd = ndd;
                 break;
             }
             if (train && scatter && nv < NRC_V) {
@@ -1363,7 +1417,11 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
                         const ap_gbuffer& lg = S.objs[ls.object].material.texture.base;
                         neeValid = true;
                         const float pLo = AreaToSolidAnglePDF(c.pos, ls.point, ls.normal, ls.pFacet * ls.pA);
-                        const float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
+                        float pBs = std::fmax(BSDF_PDF(lb, model, c.gb, c.normal, rd, L), 0.0f);
+                        if (probe) {
+                            const float pPr = OctahedronHemispherePDF(*probe, L, c.normal, gquery);
+                            pBs = (1.0f - alpha) * pBs + alpha * pPr;
+                        }
                         const float w = PowerHeuristic(pLo, pBs);
                         for (int j = 0; j < nl; ++j) {
                             const float bc = EvaluateBSDFWithCos(wv.lb[j], wv.lam[j], model, rd, L, ct, mediaIn);
@@ -1387,10 +1445,15 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             p = lerp(basic, 1.0f, inside ? 0.99f : 0.0f);
             if (rng.rand1() > p) break;
         }
+        if (guiding) {
+            const bool allowStart = !sc.delta && !sc.pureLight && !inside;
+            col.Advance(sc.next, contrib, p, ndd, allowStart, rd, c.pos, c.normal);
+        }
         for (int j = 0; j < nl; ++j) {
             mask[j] = mask[j] * contrib[j];
             mask[j] = mask[j] / p;
         }
+        if (guiding) updateQuery();
         ro = c.pos;
         rd = sc.next;
         if (!sc.delta) ndd++;
@@ -1402,9 +1465,16 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         if (N.width == 64 && N.mode == 1) or_nrc_forward(N.params, feat, 1, 1, y);
         else or_nrc_forward_w(N.params, N.width, feat, 1, N.mode, y);
         const float3 cv = fmax3(float3{y[0], y[1], y[2]}, float3{0.0f, 0.0f, 0.0f});
-        for (int j = 0; j < nl; ++j) radiance[j] = radiance[j] + mask[j] * InteractC(wv.lb[j], cv) * wv.d65[j];
+        float endpoint[MAXL] = {};
+        for (int j = 0; j < nl; ++j) {
+            const float le = InteractC(wv.lb[j], cv);
+            radiance[j] = radiance[j] + mask[j] * le * wv.d65[j];
+            endpoint[j] = le * wv.d65[j];
+        }
+        if (guiding) col.Endpoint(endpoint, endNdd);
         if (queried) *queried = true;
     }
+    if (guiding && slots) col.Emit(wv, slots);
     if (out) {
         for (int k = 0; k < nv; ++k) {
             float L[MAXL] = {};
@@ -1652,7 +1722,21 @@ int or_render_nrc_w(const or_bvh* bvh, const ap_scene_object* objs, size_t n, co
                     const float* bmin, const float* bmax, float spread, uint32_t train_paths,
                     uint32_t unbiased_every, float* hdr, double* sums, float* samples, uint8_t* valid,
                     uint32_t* n_queries, int threads) {
+    return or_render_nrc_guided(bvh, objs, n, cam, prm, cull_margin, params, width, mode, bmin, bmax, spread,
+                                train_paths, unbiased_every, hdr, sums, samples, valid, n_queries, threads, nullptr);
+}
+
+// or_render_nrc_w with directional guiding (prm->use_guiding and a guide):
+// frames in order, each sampling the probes the frames before it built and
+// then folding its own guiding samples in, as or_render_guided.
+int or_render_nrc_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, const ap_camera* cam,
+                         const ap_render_params* prm, float cull_margin, const float* params, int width, int mode,
+                         const float* bmin, const float* bmax, float spread, uint32_t train_paths,
+                         uint32_t unbiased_every, float* hdr, double* sums, float* samples, uint8_t* valid,
+                         uint32_t* n_queries, int threads, or_guide* guide) {
     if ((width != 64 && width != 32) || (mode != 1 && mode != 2)) return -1;
+    if (prm->use_guiding && !guide) return -1;
+    GuideState* G = prm->use_guiding ? &guide->state : nullptr;
     const Setup U = MakeSetup(bvh, objs, n, cam, prm, cull_margin);
     const int64_t npix = static_cast<int64_t>(U.W) * U.H;
     const int nl = prm->wavelengths == 4 ? 4 : 1;
@@ -1663,6 +1747,7 @@ int or_render_nrc_w(const or_bvh* bvh, const ap_scene_object* objs, size_t n, co
         n_train_max = (static_cast<uint32_t>(npix) + stride - 1) / stride;
     }
     uint32_t queries = 0;
+    std::vector<ap_path_guiding_sample> gslots(G ? static_cast<size_t>(npix) * PG_COLLECT : 0);
     for (int s = 0; s < prm->spp; ++s) {
         const int frame = prm->frame_index + s;
         const bool last = s == prm->spp - 1;
@@ -1689,7 +1774,8 @@ int or_render_nrc_w(const or_bvh* bvh, const ap_scene_object* objs, size_t n, co
             NrcTrain tr;
             bool q = false;
             float3 res = CalculateRadianceNRC(U.S, U.P, rng, U.camO, rd, U.inside0, N, train, unbiased,
-                                              static_cast<int>(p), &tr, &q, nl);
+                                              static_cast<int>(p), &tr, &q, nl, G, frame,
+                                              G ? &gslots[static_cast<size_t>(p) * PG_COLLECT] : nullptr);
             frame_queries += q ? 1u : 0u;
             res = fmax3(float3{0, 0, 0}, res);
             const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
@@ -1709,6 +1795,7 @@ int or_render_nrc_w(const or_bvh* bvh, const ap_scene_object* objs, size_t n, co
             }
         }
         if (last) queries = frame_queries;
+        if (G) G->Update(gslots);
     }
     if (n_queries) *n_queries = queries;
     return 0;

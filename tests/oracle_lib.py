@@ -80,6 +80,9 @@ def lib():
         L.or_render_nrc.restype = i32
         L.or_render_nrc.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, vp, vp, ctypes.c_float, ctypes.c_uint32,
                                     ctypes.c_uint32, vp, vp, vp, vp, vp, i32]
+        L.or_render_nrc_guided.restype = i32
+        L.or_render_nrc_guided.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, i32, i32, vp, vp, ctypes.c_float,
+                                           ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp, vp, i32, vp]
         L.or_render_nrc_w.restype = i32
         L.or_render_nrc_w.argtypes = [vp, vp, sz, vp, vp, ctypes.c_float, vp, i32, i32, vp, vp, ctypes.c_float,
                                       ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp, vp, i32]
@@ -203,9 +206,10 @@ class OracleBVH:
         return hdr, sums, gb, st
 
     def render_nrc(self, camera, params, nrc_params, bmin, bmax, spread=0.01, train_paths=0, unbiased_every=0,
-                   cull_margin=None, threads=None, hdr=None, sums=None, width=64, mode=1):
+                   cull_margin=None, threads=None, hdr=None, sums=None, width=64, mode=1, guide=None):
         """The core's cache rendering with the cache frozen at nrc_params (a
-        network of hidden width `width`, rounding points `mode`: 1 BF16, 2 FP16):
+        network of hidden width `width`, rounding points `mode`: 1 BF16, 2 FP16),
+        guided by `guide` (a Guide) when params.use_guiding:
         (hdr, sums, samples of the last frame keyed pixel*8+vertex, cache queries)."""
         import ctypes as C
 
@@ -215,10 +219,11 @@ class OracleBVH:
         samples = np.zeros((w * h * 8, 20), np.float32)
         valid = np.zeros(w * h * 8, np.uint8)
         nq = C.c_uint32(0)
-        rc = lib().or_render_nrc_w(self.h, _p(self.objs), len(self.objs), C.addressof(camera), C.addressof(params),
-                                   float(cull_margin or 0.0), _p(_f32(nrc_params)), int(width), int(mode),
-                                   _p(_f32(bmin)), _p(_f32(bmax)), spread, train_paths, unbiased_every, _p(hdr),
-                                   _p(sums), _p(samples), _p(valid), C.byref(nq), threads or os.cpu_count())
+        rc = lib().or_render_nrc_guided(self.h, _p(self.objs), len(self.objs), C.addressof(camera),
+                                        C.addressof(params), float(cull_margin or 0.0), _p(_f32(nrc_params)),
+                                        int(width), int(mode), _p(_f32(bmin)), _p(_f32(bmax)), spread, train_paths,
+                                        unbiased_every, _p(hdr), _p(sums), _p(samples), _p(valid), C.byref(nq),
+                                        threads or os.cpu_count(), guide.h if guide is not None else None)
         if rc != 0:
             raise RuntimeError("or_render_nrc failed")
         return hdr, sums, samples[valid != 0], nq.value

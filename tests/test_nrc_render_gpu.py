@@ -15,7 +15,13 @@ Bars:
     two accumulations);
   * a cache that is never reached leaves ap_render's image bit for bit;
   * training is deterministic: two runs from the same weights give the same
-    image and the same weights, and the weights move.
+    image and the same weights, and the weights move;
+  * with directional guiding (probes built by a guided frame, bit-identical on
+    both sides), a frozen-cache guided frame follows the same paths: query
+    counts, training keys and query records bit-exact, pixels within the
+    bar above; the probes rebuilt from its guiding samples (which take the
+    cache prediction as a path's endpoint radiance) have the same count and
+    index volume, and texels within 1e-3.
 """
 import numpy as np
 import pytest
@@ -180,3 +186,50 @@ def test_trained_cache_gains_over_the_plain_tracer(gpu):
           f"({e_plain / e_nrc:.2f}x lower), untrained cache {e_raw:.4g}", flush=True)
     assert e_nrc < 0.5 * e_plain
     assert e_raw > 2.0 * e_nrc
+
+
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_guided_frozen_cache_matches_oracle(gpu, wavelengths):
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 40, 40)
+    scene = ap.SceneRenderer(objs)
+    margin = np.float32(scene.info().line_margin)
+    bvh = ol.OracleBVH(objs)
+    guide = ol.Guide()
+    kw = dict(max_depth=10, rr_min_depth=3, seed=13, wavelengths=wavelengths, use_guiding=1)
+    n = cam.width * cam.height
+    # A guided frame without the cache builds the probes on both sides.
+    prm0 = ap.RenderParams.defaults(spp=1, frame_index=48, **kw)
+    hdr0 = ap.DeviceBuffer(n * 12)
+    hdr0.zero()
+    scene.render(cam, prm0, hdr0)
+    ap.synchronize()
+    bvh.render(cam, prm0, cull_margin=margin, guide=guide)
+    assert np.array_equal(scene.guiding_export()[0].view(np.uint32), guide.export()[0].view(np.uint32))
+    # A guided frame with the frozen cache.
+    cache = ap.RadianceCache(seed=1)
+    w = ncpu.init_params(47, scale=0.7)
+    cache.set_params(w)
+    nprm = ap.NrcRenderParams.defaults(train_paths=256, train_steps=0, unbiased_every=4, spread=0.01)
+    prm1 = ap.RenderParams.defaults(spp=1, frame_index=49, **kw)
+    g_hdr, g_sums = render_nrc(scene, cam, prm1, cache, nprm)
+    st, g_s = scene.nrc_last_frame()
+    lo, hi = cache.bounds()
+    o_hdr, o_sums, o_s, o_nq = bvh.render_nrc(cam, prm1, w, lo, hi, spread=nprm.spread, train_paths=nprm.train_paths,
+                                              unbiased_every=nprm.unbiased_every, cull_margin=margin, guide=guide)
+    rel = np.abs(g_sums - o_sums) / (np.abs(o_sums) + 1e-6)
+    gd, gc, gv, gb = scene.guiding_export()
+    od, oc, ov, ob = guide.export()
+    trel = np.abs(gd - od) / (np.abs(od) + 1e-6)
+    print(f"[nrc-render] guided, {wavelengths} wavelengths: queries {st.queries} vs {o_nq}, samples {st.samples} vs "
+          f"{len(o_s)}, pixel rel p99.5 {np.quantile(rel, 0.995):.3g} max {rel.max():.3g}; probes {len(gc)} vs "
+          f"{len(oc)}, texel rel p99.9 {np.quantile(trel, 0.999):.3g}", flush=True)
+    assert st.queries == o_nq and st.queries > 0
+    assert st.samples == len(o_s) and st.samples > 0
+    assert np.array_equal(g_s[:, 19].view(np.uint32), o_s[:, 19].view(np.uint32))
+    assert np.array_equal(g_s[:, :13], o_s[:, :13])
+    assert np.quantile(rel, 0.995) < 1e-4 and rel.max() < 2e-2
+    assert np.array_equal(gb, ob) and np.array_equal(gv, ov) and len(gc) == len(oc)
+    assert np.quantile(trel, 0.999) < 1e-3
+    cache.close()
+    scene.close()
