@@ -3,6 +3,8 @@
 //
 // Input: the frame's guiding samples, PG_COLLECT slots per path in pixel
 // order (written by the accumulate kernel). Steps, all on the render stream:
+//   0. flag the valid slots and compact them in slot order (an exclusive
+//      scan of the flags);
 //   1. key every valid slot by its voxel (x, y, z, face) packed so that the
 //      unsigned order is the reference's Int4Less order; radix-sort
 //      (key, slot) pairs; mark run heads and scan them: the unique voxels in
@@ -17,16 +19,16 @@
 //      its sums are not reproducible; this order is, and the oracle uses it);
 //   4. one workgroup per probe with samples merges the sums into the
 //      running mean in FP64 and rebuilds the mips (MergeProbeKernel).
+// The scans and the stable radix sorts are the project's own wave64
+// primitives (ap_sort.hip).
 // The host reads back three small counts per frame (unique voxels with their
 // bounds, new probes), as the reference does.
 #include <cstring>
 
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
-
 #include "aperture/ap_guiding.h"
 #include "ap_guide.h"
 #include "ap_internal.h"
+#include "ap_sort.h"
 
 namespace ap {
 
@@ -308,14 +310,8 @@ int ensure_sort(GuideDevice* g, size_t n) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->ranks), n * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->list), n * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->uniq), n * sizeof(uint64_t));
-    size_t sort_bytes = 0, scan_bytes = 0;
-    if (e == hipSuccess)
-        e = rocprim::radix_sort_pairs(nullptr, sort_bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1],
-                                      static_cast<unsigned>(n), 0, 64, nullptr);
-    if (e == hipSuccess)
-        e = rocprim::exclusive_scan(nullptr, scan_bytes, g->flags, g->ranks, 0u, n, rocprim::plus<uint32_t>(), nullptr);
     if (e == hipSuccess) {
-        g->tmp_bytes = std::max(sort_bytes, scan_bytes);
+        g->tmp_bytes = std::max(sort_workspace_bytes(n), scan_workspace_bytes(n));
         e = hipMalloc(&g->tmp, g->tmp_bytes);
     }
     if (e != hipSuccess) return hip_fail(e, "guiding: sort buffers");
@@ -364,7 +360,7 @@ VolumeView view_of(const GuideDevice* g) {
 void guide_free(GuideDevice* g) {
     if (!g) return;
     void* ptrs[] = {g->probes, g->temp, g->volume, g->recs, g->slots, g->keys[0], g->keys[1], g->vals[0],
-                    g->vals[1], g->flags, g->ranks, g->uniq, g->tmp, g->info};
+                    g->vals[1], g->flags, g->ranks, g->list, g->uniq, g->tmp, g->info};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (g->info_host) (void)hipHostFree(g->info_host);
@@ -422,8 +418,7 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
     if (e != hipSuccess) return hip_fail(e, "guiding: info reset");
     // 0. the valid slots, in slot order (most of a frame's slots are empty).
     hipLaunchKernelGGL(valid_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, g->flags, g->info);
-    size_t bytes = g->tmp_bytes;
-    e = rocprim::exclusive_scan(g->tmp, bytes, g->flags, g->ranks, 0u, n, rocprim::plus<uint32_t>(), st);
+    e = scan_exclusive_u32(g->flags, g->ranks, n, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: slot scan");
     hipLaunchKernelGGL(compact_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->flags, g->ranks, n, g->list, g->info);
     e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
@@ -435,12 +430,10 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
     // 1. unique voxels in sorted order.
     hipLaunchKernelGGL(voxel_keys_kernel, grid_for(m), dim3(kBlock), 0, st, g->slots, g->list, m, g->keys[0],
                        g->vals[0]);
-    bytes = g->tmp_bytes;
-    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, 64, st);
+    e = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, 64, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
     hipLaunchKernelGGL(run_heads_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], m, g->flags);
-    bytes = g->tmp_bytes;
-    e = rocprim::exclusive_scan(g->tmp, bytes, g->flags, g->ranks, 0u, m, rocprim::plus<uint32_t>(), st);
+    e = scan_exclusive_u32(g->flags, g->ranks, m, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: voxel scan");
     hipLaunchKernelGGL(unique_voxels_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, m,
                        g->uniq, g->info);
@@ -491,8 +484,7 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
     }
     // Probes for the voxels without one, numbered in sorted order.
     hipLaunchKernelGGL(unindexed_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, n_uniq, view_of(g), g->flags);
-    bytes = g->tmp_bytes;
-    e = rocprim::exclusive_scan(g->tmp, bytes, g->flags, g->ranks, 0u, n_uniq, rocprim::plus<uint32_t>(), st);
+    e = scan_exclusive_u32(g->flags, g->ranks, n_uniq, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: probe scan");
     hipLaunchKernelGGL(assign_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, g->flags, g->ranks, n_uniq,
                        view_of(g), g->count, g->info);
@@ -512,8 +504,7 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
     int probe_bits = 1;
     while ((1ll << probe_bits) <= g->count) ++probe_bits;
     const int end_bit = std::min(64, 6 + probe_bits + 1);
-    bytes = g->tmp_bytes;
-    e = rocprim::radix_sort_pairs(g->tmp, bytes, g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, end_bit, st);
+    e = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, end_bit, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: texel sort");
     hipLaunchKernelGGL(texel_sums_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], g->vals[1], m, g->slots,
                        g->temp);

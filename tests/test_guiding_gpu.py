@@ -7,7 +7,9 @@ into the probes (unique voxels in (x, y, z, face) order, per-texel sums in
 sample order, FP64 merge, mips), and the next frame samples with them. Bar:
 after several frames the images, FP64 sums, probe texels and counts, and
 the index volume are bit-identical to the oracle's. Frames start at index
-48 so that the guiding probability is already near its 0.5 cap.
+48 so that the guiding probability is already near its 0.5 cap. The build's
+compaction, sorts and unique are the project's own scan and stable radix sort;
+a 256x160 frame gives them tens of 4096-key tiles and long runs of equal keys.
 """
 import numpy as np
 import pytest
@@ -97,3 +99,12 @@ def test_guided_full_heightfield_workload_crop(gpu):
     guided_parity(objs, cam, dict(spp=2, max_depth=16, rr_min_depth=4), "full heightfield, centre crop")
     guided_parity(objs, cam, dict(spp=2, max_depth=16, rr_min_depth=4, wavelengths=4),
                   "full heightfield, centre crop, four wavelengths")
+
+
+def test_guided_multi_tile_sorts(gpu):
+    """About 200 K guiding samples per frame: the voxel and texel sorts span
+    many tiles, and the texel sums depend on the sort keeping equal keys in
+    slot order."""
+    objs = ap.synth_box_scene(2, 2, 15.0)
+    cam = ap.Camera.make(rg.BOX_CAM["origin"], rg.BOX_CAM["direction"], rg.BOX_CAM["fov_deg"], 256, 160)
+    guided_parity(objs, cam, dict(spp=1, max_depth=8), "box 256x160", calls=3)
