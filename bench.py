@@ -159,38 +159,51 @@ def c1_bench(ap, threads, cpu=True):
     return out
 
 
-def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_passes, boost_frames=0,
-              boost_steps=16, spread=0.01, width=64, precision="bf16"):
-    """Equal-time relMSE of the radiance-cache renderer against the plain
-    wavefront tracer, both with four hero wavelengths per path. The plain
-    tracer renders plain_spp frames (timed with HIP events); the cache
-    renderer starts from freshly initialised weights and renders frame after
-    frame, training after each (and on train_passes training-only passes),
-    for as many frames as its calibrated per-frame time fits in the same GPU
-    time: its first `warmup` frames go to a scratch image, the rest are
-    averaged; the first `boost_frames` averaged frames train with
-    `boost_steps` Adam steps instead of 4, their extra cost calibrated and
-    charged to the budget. Both images are compared with a plain render of
-    ref_spp frames on another seed."""
-    W, H = cam.width, cam.height
-    n = W * H
-    ref = ap.DeviceBuffer(n * 12)
+def reference_image(ap, scene, cam, stream, params, ref_spp):
+    """A plain render of ref_spp frames (four wavelengths, seed 1000): the
+    reference the equal-time comparisons measure relMSE against."""
+    ref = ap.DeviceBuffer(cam.width * cam.height * 12)
     ref.zero(stream)
     t0 = time.perf_counter()
     scene.render(cam, ap.RenderParams.defaults(spp=ref_spp, seed=1000, wavelengths=4, **params), ref, stream=stream)
     stream.synchronize()
     log(f"nrc: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    return ref
+
+
+def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref, ref_spp, warmup, train_passes, boost_frames=0,
+              boost_steps=16, spread=0.01, width=64, precision="bf16", guided=False, lr=0.005):
+    """Equal-time relMSE of the radiance-cache renderer against the plain
+    wavefront tracer, both with four hero wavelengths per path and, with
+    `guided`, both with directional guiding (probes reset before each timed
+    run and rebuilt after every frame, inside the timed region). The plain
+    tracer renders plain_spp frames (timed with HIP events); the cache
+    renderer starts from freshly initialised weights (Adam learning rate
+    `lr`) and renders frame after frame, training after each (and on
+    train_passes training-only passes), for as many frames as its calibrated
+    per-frame time fits in the same GPU time: its first `warmup` frames go to
+    a scratch image, the rest are averaged; the first `boost_frames` averaged
+    frames train with `boost_steps` Adam steps instead of 4, their extra cost
+    calibrated and charged to the budget. Both images are compared with the
+    plain reference image `ref`; relMSE is reduced in FP64 on the GPU."""
+    W, H = cam.width, cam.height
+    n = W * H
+    g = 1 if guided else 0
     img = ap.DeviceBuffer(n * 12)
     img.zero(stream)
+    if guided:
+        scene.guiding_reset()
     e0, e1 = ap.Event(), ap.Event()
     e0.record(stream)
-    scene.render(cam, ap.RenderParams.defaults(spp=plain_spp, seed=5, wavelengths=4, **params), img, stream=stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=plain_spp, seed=5, wavelengths=4, use_guiding=g, **params), img,
+                 stream=stream)
     e1.record(stream)
     stream.synchronize()
     budget = e0.elapsed_ms(e1)
     plain = rel_mse(ap, img, ref, n, stream)
     cache = ap.RadianceCache(seed=1, width=width, precision=precision)
     nprm = ap.NrcRenderParams.defaults(train_paths=65536, train_passes=train_passes, spread=spread)
+    nprm.learning_rate = lr
     scratch = ap.DeviceBuffer(n * 12)
     scratch.zero(stream)
     w0 = cache.params()
@@ -201,7 +214,7 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     def nrc_frames(frames, buf, seed, first=0, p=nprm):
         if frames > 0:
             scene.render_nrc(cam, ap.RenderParams.defaults(spp=frames, frame_index=first, seed=seed, wavelengths=4,
-                                                           **params), cache, p, buf, stream=stream)
+                                                           use_guiding=g, **params), cache, p, buf, stream=stream)
 
     # Calibration (untimed; it also allocates the path, query and training
     # buffers): GPU time per frame of the cache renderer, frames submitted
@@ -209,12 +222,12 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     nrc_frames(2, scratch, 3)
     stream.synchronize()
     e0.record(stream)
-    nrc_frames(8, scratch, 4)
+    nrc_frames(8, scratch, 4, 32)
     e1.record(stream)
     stream.synchronize()
     per_frame = e0.elapsed_ms(e1) / 8
     e0.record(stream)
-    nrc_frames(4, scratch, 4, 0, boost)
+    nrc_frames(4, scratch, 4, 40, boost)
     e1.record(stream)
     stream.synchronize()
     per_boost = e0.elapsed_ms(e1) / 4
@@ -222,9 +235,12 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     nb_fit = boost_frames
     rest = budget - warmup * per_frame - nb_fit * per_boost
     total = max(warmup + 1, warmup + nb_fit + int(max(rest, 0.0) / per_frame))
-    # Timed run from the initial weights with a fresh optimiser: `warmup`
-    # frames into a scratch image, the rest averaged into the compared image.
+    # Timed run from the initial weights with a fresh optimiser (and empty
+    # probes): `warmup` frames into a scratch image, the rest averaged into
+    # the compared image.
     cache.set_params(w0)
+    if guided:
+        scene.guiding_reset()
     scratch.zero(stream)
     img.zero(stream)
     stream.synchronize()
@@ -241,7 +257,10 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
     st, _ = scene.nrc_last_frame(samples=False)
     nrc = rel_mse(ap, img, ref, n, stream) if frames > warmup else float("nan")
     cache.close()
+    if guided:
+        scene.guiding_reset()
     out = {
+        "guided": guided,
         "equal_time_ms": round(budget, 1),
         "plain_spp": plain_spp,
         "plain_relmse": round(plain, 5),
@@ -259,7 +278,7 @@ def nrc_bench(ap, scene, cam, stream, params, plain_spp, ref_spp, warmup, train_
         "cache_queries_per_frame": st.queries,
         "training_paths_per_frame": st.train_paths,
         "training_samples_per_frame": st.samples,
-        "cache": f"64-{width}x5-3 ReLU, {precision.upper()} MFMA / FP32 accumulate, relative L2, Adam lr 0.005, "
+        "cache": f"64-{width}x5-3 ReLU, {precision.upper()} MFMA / FP32 accumulate, relative L2, Adam lr {lr:g}, "
                  "4 steps x 16384 per frame from 65,536 training paths (the first boost frames take more steps, at "
                  f"their own cost), spread threshold {spread}",
         "spectral": "four hero wavelengths per path in the reference, plain and cache images",
@@ -460,8 +479,16 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     }
     log(f"render: {out}")
     if nrc_spp > 0:
-        out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes,
-                                        *nrc_boost)
+        # The heightfield workload with the cache (guided on both sides, Adam
+        # lr 1e-3), its learning-rate ablation, and the unguided comparison.
+        ref = reference_image(ap, scene, cam, stream, params, nrc_ref_spp)
+        out["nrc_vs_plain"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, ref, nrc_ref_spp, nrc_warmup,
+                                        nrc_passes, *nrc_boost, guided=True, lr=1e-3)
+        out["nrc_vs_plain_guided_lr_5e-3"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, ref, nrc_ref_spp,
+                                                       nrc_warmup, nrc_passes, *nrc_boost, guided=True, lr=5e-3)
+        out["nrc_vs_plain_unguided"] = nrc_bench(ap, scene, cam, stream, params, nrc_spp, ref, nrc_ref_spp,
+                                                 nrc_warmup, nrc_passes, *nrc_boost, guided=False, lr=5e-3)
+        ref.free()
         # The same comparison in a closed interior: the C1 box (coloured walls,
         # two spheres, one ceiling light) at 1920x1080, where long path tails
         # carry most of the light.
@@ -470,8 +497,10 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
         box_params = dict(max_depth=8, rr_min_depth=3, sky=1)
         box.render(box_cam, ap.RenderParams.defaults(spp=1, wavelengths=4, **box_params), hdr, stream=stream)
         stream.synchronize()
-        res = nrc_bench(ap, box, box_cam, stream, box_params, nrc_spp, nrc_ref_spp, nrc_warmup, nrc_passes,
-                        *nrc_boost)
+        box_ref = reference_image(ap, box, box_cam, stream, box_params, nrc_ref_spp)
+        res = nrc_bench(ap, box, box_cam, stream, box_params, nrc_spp, box_ref, nrc_ref_spp, nrc_warmup, nrc_passes,
+                        *nrc_boost, guided=True, lr=1e-3)
+        box_ref.free()
         res["workload"] = ("closed box of the C1 workload (5 walls, two 1280-triangle spheres, 0.5x0.5 ceiling "
                            "light) at 1920x1080, max depth 8, RR from depth 3")
         out["nrc_vs_plain_closed_box"] = res
