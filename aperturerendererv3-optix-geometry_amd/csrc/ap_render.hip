@@ -313,20 +313,20 @@ __device__ __forceinline__ void pg_advance(GuideRec& r, f3 next, const float* co
 template <int NL>
 __device__ __forceinline__ void pg_emit(const GuideRec& r, const float* lam, float psum,
                                         const float* __restrict__ tables, ap_path_guiding_sample* __restrict__ out) {
+    // Valid records are packed to the front of the path's PG_COLLECT slots
+    // in collection order and the rest are cleared; records are stored
+    // straight to `out`, so no local array is indexed at run time.
     int w = 0;
-    ap_path_guiding_sample rec[PG_COLLECT];
-    for (int k = 0; k < PG_COLLECT; ++k) {
-        __builtin_memset(&rec[k], 0, sizeof(ap_path_guiding_sample));
-    }
+#pragma unroll
     for (int i = 0; i < PG_COLLECT; ++i) {
-        if (w >= PG_COLLECT) break;
         if (!((r.valid >> i) & 1u)) continue;
         float smp[NL];
         for (int j = 0; j < NL; ++j) smp[j] = r.sample[i][j];
         const f3 rx = path_xyz<NL>(lam, smp, psum, tables);
         const float strength = fminf(max3(rx), PG_MAX_BRIGHT);
         if (strength < PG_MIN_BRIGHT) continue;
-        ap_path_guiding_sample& o = rec[w++];
+        ap_path_guiding_sample o;
+        __builtin_memset(&o, 0, sizeof(o));
         o.direction = ap_float3{r.dir[i][0], r.dir[i][1], r.dir[i][2]};
         const f3 c = fmax3(rx, f3{0.0f, 0.0f, 0.0f});
         o.radiance = ap_float3{c.x, c.y, c.z};
@@ -334,8 +334,11 @@ __device__ __forceinline__ void pg_emit(const GuideRec& r, const float* lam, flo
                                    f3{r.vdir[i][0], r.vdir[i][1], r.vdir[i][2]});
         o.voxel_grid = ap_int4{v.x, v.y, v.z, v.f};
         o.valid = 1;
+        out[w++] = o;
     }
-    for (int k = 0; k < PG_COLLECT; ++k) out[k] = rec[k];
+    ap_path_guiding_sample zero;
+    __builtin_memset(&zero, 0, sizeof(zero));
+    for (int k = w; k < PG_COLLECT; ++k) out[k] = zero;
 }
 
 struct CamParams {
@@ -620,37 +623,47 @@ __device__ LightSample sample_light(const LightRec* __restrict__ lights, int n, 
     }
     const LightRec& L = lights[pick];
     if (L.type == AP_OBJ_CUBE) {
-        f3 fn[6], fc[6];
-        float s1[6], s2[6], area[6];
-        box_facets(L, fn, fc, s1, s2);
-        const f3 ux{L.ux[0], L.ux[1], L.ux[2]}, uy{L.uy[0], L.uy[1], L.uy[2]}, uz{L.uz[0], L.uz[1], L.uz[2]};
-        const f3 tang[6] = {uy, uy, ux, ux, ux, ux};
-        const f3 bitan[6] = {uz, uz, uz, uz, uy, uy};
-        for (int f = 0; f < 6; ++f) area[f] = 4.0f * s1[f] * s2[f];
-        int vidx[6];
-        float vw[6];
-        int vc = 0;
+        // Facing weights in facet order; a facet is a candidate when its
+        // weight is positive, and the candidates are walked in facet order.
+        float w[6];
+        int vc = 0, first = 0;
         float vsum = 0.0f;
+#pragma unroll
         for (int f = 0; f < 6; ++f) {
-            const float w = facing_weight(fn[f], fc[f], pos, area[f]);
-            if (w > 0.0f) { vidx[vc] = f; vw[vc] = w; vsum += w; ++vc; }
+            const BoxFacet F = box_facet(L, f);
+            w[f] = facing_weight(F.n, F.c, pos, 4.0f * F.s1 * F.s2);
+            if (w[f] > 0.0f) {
+                if (vc == 0) first = f;
+                vsum += w[f];
+                ++vc;
+            }
         }
         if (vc == 0) return ls;
-        int slot = 0, face = vidx[0];
+        int face = -1;
+        float wsel = 0.0f;
         {
             const float r = rand1(rng) * vsum;
             float acc = 0.0f;
-            for (int k = 0; k < vc; ++k) {
-                acc += vw[k];
-                if (r <= acc) { slot = k; face = vidx[k]; break; }
-            }
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                if (w[f] > 0.0f && face < 0) {
+                    acc += w[f];
+                    if (r <= acc) { face = f; wsel = w[f]; }
+                }
         }
+        if (face < 0) {  // rounding left r above the last partial sum
+            face = first;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                if (f == first) wsel = w[f];
+        }
+        const BoxFacet F = box_facet(L, face);
         const float u1 = 2.0f * rand1(rng) - 1.0f;
         const float u2 = 2.0f * rand1(rng) - 1.0f;
-        ls.point = fc[face] + tang[face] * (u1 * s1[face]) + bitan[face] * (u2 * s2[face]);
-        ls.normal = fn[face];
-        ls.p_facet = 1.0f * (vw[slot] / vsum);
-        ls.p_area = 1.0f / area[face];
+        ls.point = F.c + F.t * (u1 * F.s1) + F.b * (u2 * F.s2);
+        ls.normal = F.n;
+        ls.p_facet = 1.0f * (wsel / vsum);
+        ls.p_area = 1.0f / (4.0f * F.s1 * F.s2);
         ls.valid = true;
     } else if (L.type == AP_OBJ_SPHERE) {
         const f3 sc{L.center[0], L.center[1], L.center[2]};

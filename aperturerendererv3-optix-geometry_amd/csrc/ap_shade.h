@@ -294,16 +294,38 @@ struct LightSample {
     bool valid;
 };
 
-AP_HD void box_facets(const LightRec& L, f3 n[6], f3 c[6], float s1[6], float s2[6]) {
+// One facet of a box light: outward normal n, centre c, in-plane axes t and
+// b with half extents s1 and s2. Facets 0..5 are +x, -x, +y, -y, +z, -z of
+// the OBB frame. Built by value from a facet index so that a loop over the
+// six facets, or a facet picked at run time, needs no indexed local arrays.
+struct BoxFacet {
+    f3 n, c, t, b;
+    float s1, s2;
+};
+
+AP_HD BoxFacet box_facet(const LightRec& L, int f) {
     const f3 ux{L.ux[0], L.ux[1], L.ux[2]}, uy{L.uy[0], L.uy[1], L.uy[2]}, uz{L.uz[0], L.uz[1], L.uz[2]};
     const f3 bc{L.center[0], L.center[1], L.center[2]};
     const f3 e{L.extent[0], L.extent[1], L.extent[2]};
-    n[0] = ux; n[1] = -ux; n[2] = uy; n[3] = -uy; n[4] = uz; n[5] = -uz;
-    c[0] = bc + ux * e.x; c[1] = bc - ux * e.x;
-    c[2] = bc + uy * e.y; c[3] = bc - uy * e.y;
-    c[4] = bc + uz * e.z; c[5] = bc - uz * e.z;
-    s1[0] = e.y; s1[1] = e.y; s1[2] = e.x; s1[3] = e.x; s1[4] = e.x; s1[5] = e.x;
-    s2[0] = e.z; s2[1] = e.z; s2[2] = e.z; s2[3] = e.z; s2[4] = e.y; s2[5] = e.y;
+    BoxFacet F;
+    f3 axis;
+    float ea;
+    const int a = f >> 1;
+    if (a == 0) {
+        axis = ux; ea = e.x; F.t = uy; F.b = uz; F.s1 = e.y; F.s2 = e.z;
+    } else if (a == 1) {
+        axis = uy; ea = e.y; F.t = ux; F.b = uz; F.s1 = e.x; F.s2 = e.z;
+    } else {
+        axis = uz; ea = e.z; F.t = ux; F.b = uy; F.s1 = e.x; F.s2 = e.y;
+    }
+    if (f & 1) {
+        F.n = -axis;
+        F.c = bc - axis * ea;
+    } else {
+        F.n = axis;
+        F.c = bc + axis * ea;
+    }
+    return F;
 }
 
 AP_HD float facing_weight(f3 fn, f3 fc, f3 p, float area) {
@@ -314,25 +336,27 @@ AP_HD float facing_weight(f3 fn, f3 fc, f3 p, float area) {
 // NEEPDFOmegaFromHit (nee.cuh:64-151, 254-327) for a light hit by a BSDF ray.
 AP_HD float nee_pdf_from_hit(const LightRec& L, f3 surf, f3 lp, f3 ln) {
     if (L.type == 0) {  // cube
-        f3 n[6], c[6];
-        float s1[6], s2[6], area[6];
-        box_facets(L, n, c, s1, s2);
-        for (int f = 0; f < 6; ++f) area[f] = 4.0f * s1[f] * s2[f];
         int hit = -1;
+#pragma unroll
         for (int f = 0; f < 6; ++f)
-            if (dot(n[f], ln) > 0.999f) { hit = f; break; }
+            if (hit < 0 && dot(box_facet(L, f).n, ln) > 0.999f) hit = f;
         if (hit < 0) {
             float best = 1e30f;
+#pragma unroll
             for (int f = 0; f < 6; ++f) {
-                const float d = fabsf(dot(n[f], lp - c[f]));
+                const BoxFacet F = box_facet(L, f);
+                const float d = fabsf(dot(F.n, lp - F.c));
                 if (d < best) { best = d; hit = f; }
             }
         }
         float sum = 0.0f, whit = 0.0f, ahit = 0.0f;
+#pragma unroll
         for (int f = 0; f < 6; ++f) {
-            const float w = facing_weight(n[f], c[f], surf, area[f]);
+            const BoxFacet F = box_facet(L, f);
+            const float area = 4.0f * F.s1 * F.s2;
+            const float w = facing_weight(F.n, F.c, surf, area);
             sum += w;
-            if (f == hit) { whit = w; ahit = area[f]; }
+            if (f == hit) { whit = w; ahit = area; }
         }
         if (sum <= 0.0f || whit <= 0.0f) return 0.0f;
         const float pa = L.select_prob * (1.0f / ahit) * (whit / sum);
