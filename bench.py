@@ -382,6 +382,81 @@ def materials_bench(ap, spp, stream, W=1920, H=1080):
     return out
 
 
+COUNTS_FIXTURE = os.path.join(REPO, "tests", "golden", "integrator_counts_heightfield.json")
+
+
+def integrator_roofline(res, key, samples):
+    """Algorithmic bytes of one frame of the heightfield render, priced per
+    path vertex: each closest-hit and shadow ray 32 (ray) + 8 (hit) + 32 per
+    node record + 36 per triangle test; each shaded vertex 128 (material and
+    type fetch) + 128 per spectral Interact (eight 16-byte LUT cells); each
+    probe query 8 (index) + 64 when it finds a probe (four texels); each
+    sample 24 (framebuffer read-modify-write). Per-ray node and test counts
+    and per-vertex Interact and probe counts are the oracle's accounting of
+    the same frame at 128x72 (the fixture, whose ray counts the GPU matches
+    exactly at that size); ray counts are the GPU's own at 1920x1080. The
+    result over the frame time on the render stream, against HBM peak."""
+    with open(COUNTS_FIXTURE) as f:
+        c = json.load(f)["views"]["view128x72"][key]
+    ext, shadow = res["rays_per_spp"]
+    per_ext = 40.0 + (32.0 * c["ext_nodes"] + 36.0 * c["ext_prims"]) / c["ext_rays"]
+    per_shadow = 40.0 + (32.0 * c["shadow_nodes"] + 36.0 * c["shadow_prims"]) / c["shadow_rays"]
+    hits = ext * c["hits"] / c["ext_rays"]
+    interacts = hits * c["interacts"] / c["hits"]
+    probes = hits * c["probe_queries"] / c["hits"]
+    found = hits * c["probe_found"] / c["hits"]
+    parts = {
+        "closest_hit_traversal": ext * per_ext,
+        "shadow_traversal": shadow * per_shadow,
+        "material": 128.0 * hits,
+        "spectral_interact": 128.0 * interacts,
+        "probes": 8.0 * probes + 64.0 * found,
+        "framebuffer": 24.0 * samples,
+    }
+    total = sum(parts.values())
+    gbs = total / (res["ms_per_spp"] / 1e3) / 1e9
+    return {
+        "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_spp": round(total),
+        "bytes_per_vertex": round(total / hits, 1),
+        "share": {k: round(v / total, 4) for k, v in parts.items()},
+        "nodes_per_ray": [round(c["ext_nodes"] / c["ext_rays"], 1), round(c["shadow_nodes"] / c["shadow_rays"], 1)],
+        "interacts_per_vertex": round(c["interacts"] / c["hits"], 2),
+        "counts": "oracle accounting of frame 49 at 128x72 (tests/golden/integrator_counts_heightfield.json)",
+    }
+
+
+def render_only(ap, mode, spp):
+    """One mode of the 1080p heightfield render alone, for counter runs: a
+    warm-up frame (for guiding, the frame that builds the first probes),
+    then `spp` frames timed with HIP events. Every frame starts with one
+    generate_kernel launch, which delimits frames in a kernel trace."""
+    objs = ap.synth_heightfield_scene(1024, 256, 4, 20.0)
+    scene = ap.SceneRenderer(objs)
+    W, H = 1920, 1080
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, W, H)
+    stream = ap.Stream()
+    hdr = ap.DeviceBuffer(W * H * 12)
+    hdr.zero(stream)
+    counts = ap.DeviceBuffer(16)
+    kw = dict(max_depth=16, rr_min_depth=4, sky=0, wavelengths=int(mode[-1]),
+              use_guiding=1 if mode.startswith("guided") else 0)
+    scene.render(cam, ap.RenderParams.defaults(spp=1, **kw), hdr, stream=stream)
+    counts.zero(stream)
+    e0, e1 = ap.Event(), ap.Event()
+    e0.record(stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, **kw), hdr, stream=stream, ray_count=counts)
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1)
+    rays = counts.to_numpy(np.uint64, 2)
+    res = {"ms_per_spp": round(ms / spp, 3), "rays_per_spp": [int(rays[0]) // spp, int(rays[1]) // spp]}
+    res["integrator_roofline"] = integrator_roofline(res, mode, W * H)
+    scene.close()
+    return {"render_only": mode, "workload": "1080p heightfield render, max depth 16, RR from depth 4, no sky",
+            "frames": spp, **res}
+
+
 def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc_warmup=0, nrc_passes=0,
                  nrc_boost=(8, 16)):
     """Samples/s of the wavefront path tracer at 1920x1080 on the heightfield
@@ -426,6 +501,7 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
             "samples_per_s": round(W * H * spp / (ms / 1e3), 1),
             "mrays_per_s": round(float(rays[0] + rays[1]) / (ms / 1e3) / 1e6, 1),
             "rays_per_sample": round(float(rays[0] + rays[1]) / (W * H * spp), 3),
+            "rays_per_spp": [int(rays[0]) // spp, int(rays[1]) // spp],
         }
 
     one = timed(wavelengths=1)
@@ -465,10 +541,14 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     tm_ms = e0.elapsed_ms(e1) / reps
     tonemap = {"ms": round(tm_ms, 4), "gb_per_s": round(W * H * 16 / (tm_ms / 1e3) / 1e9, 1),
                "curve": "ACES", "algorithmic_bytes": W * H * 16}
+    roof = {k: integrator_roofline(r, key, W * H) for k, r, key in
+            (("one_wavelength", one, "plain_1"), ("four_wavelengths", four, "plain_4"),
+             ("guided_one_wavelength", guided, "guided_1"), ("guided_four_wavelengths", guided4, "guided_4"))}
     out = {
         "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 thin-cube lights, "
                     "no sky, 1920x1080, max depth 16, RR from depth 4",
         "spp": spp,
+        "integrator_roofline": roof,
         "one_wavelength": one,
         "four_wavelengths": four,
         "guided_one_wavelength": guided,
@@ -752,6 +832,9 @@ def main():
                      help="time each traversal launch shape (rays per wave x refill) and print the table")
     ap_.add_argument("--copy-probe", action="store_true",
                      help="also run the library's streaming copy kernel over 1 GiB (HBM counter scale reference)")
+    ap_.add_argument("--render-only", choices=["plain_1", "plain_4", "guided_1", "guided_4"], default=None,
+                     help="profiling workload: only the 1080p heightfield render in one mode (one warm-up frame, "
+                          "then --render-spp frames), one JSON line of its own, no headline metric")
     args = ap_.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the traversal core has no multi-GPU path; "
@@ -764,6 +847,9 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(2)
     ap.check(ap.lib().ap_device_set(0))
+    if args.render_only:
+        print(json.dumps(render_only(ap, args.render_only, max(1, args.render_spp))), flush=True)
+        return
     stream = ap.Stream()
 
     t0 = time.perf_counter()

@@ -140,6 +140,15 @@ void or_bsdf_sample(int model, const float* mat, float lambda, const float* rayD
 void or_nee_sample(const ap_scene_object* objs, size_t n, const float* p, size_t np, uint64_t seed, float* out);
 void or_nee_pdf(const ap_scene_object* objs, size_t n, int light, const float* q, size_t m, float* out);
 
+/* Visit accounting (or_count.h): or_counts_begin zeroes the totals and
+ * turns counting on; or_counts_end turns it off and copies the OR_COUNT_N
+ * totals: closest-hit rays, their node records and primitive tests, any-hit
+ * rays, node records and tests, surface vertices, Interact evaluations,
+ * probe queries, probes found, camera paths. Not for concurrent renders. */
+#define OR_COUNT_N 11
+void or_counts_begin(void);
+void or_counts_end(uint64_t* out);
+
 /* FP64 image statistics (or_measure.cpp): DeviceMSE, DeviceMSELuminance and
  * relMSE; image kinds as AP_IMAGE_XYZ_F32 (0) / AP_IMAGE_XYZ_SUMS_F64 (1). */
 double or_mse(const float* a, const float* b, size_t n);

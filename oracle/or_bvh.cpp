@@ -24,6 +24,7 @@
 #include "aperture/ap_layout.h"
 #include "or_api.h"
 #include "or_math.h"
+#include "or_count.h"
 
 namespace orc {
 
@@ -673,5 +674,25 @@ void TraceOne(const or_bvh* h, float3 o, float3 d, float tMin, float tMax, bool 
         TraverseBVH(anyHit, h->objs, h->bvh.nodes_, h->bvh.objectIndices_, o, d, tMin, tMax, cull, margin, sdfEps);
     *id = r.id;
     *t = r.t;
+    Count(anyHit ? CNT_SHADOW_RAYS : CNT_EXT_RAYS);
+    Count(anyHit ? CNT_SHADOW_NODES : CNT_EXT_NODES, r.nodes);
+    Count(anyHit ? CNT_SHADOW_PRIMS : CNT_EXT_PRIMS, r.prims);
 }
+
+std::atomic<bool> g_counting{false};
+std::atomic<uint64_t> g_counts[CNT_N];
 }  // namespace orc
+
+extern "C" {
+
+void or_counts_begin(void) {
+    for (auto& c : orc::g_counts) c.store(0, std::memory_order_relaxed);
+    orc::g_counting.store(true);
+}
+
+void or_counts_end(uint64_t* out) {
+    orc::g_counting.store(false);
+    for (int k = 0; k < orc::CNT_N; ++k) out[k] = orc::g_counts[k].load();
+}
+
+}  // extern "C"

@@ -44,6 +44,7 @@
 #include "or_math.h"
 #include "or_spectrum.h"
 #include "or_guiding.h"
+#include "or_count.h"
 
 namespace orc {
 
@@ -231,6 +232,7 @@ static void QueryRGBBasisCached(float3 rgb, float out[4]) {
 }
 
 static float InteractC(const float lb[4], float3 rgb) {  // lut::Interact
+    Count(CNT_INTERACTS);
     const float lum = max3(rgb);
     float rb[4];
     if (lum > 1.0f) {
@@ -780,6 +782,8 @@ static void TraceMedia(const Scene& S, const MediaCtx& m, float3 ro, float3 rd, 
     const float inf = std::numeric_limits<float>::infinity();
     if (m.inside && m.object >= 0) {
         float tt;
+        Count(CNT_EXT_RAYS);
+        Count(CNT_EXT_PRIMS);
         if (Intersector(S.objs, m.object, ro, rd, eps, inf, tt, eps * 0.5f)) {
             *id = m.object;
             *t = tt;
@@ -945,6 +949,7 @@ struct Collect {
 static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, float3 origin, float3 dir, bool inside0,
                                 int* primary, const GuideState* G = nullptr, int frame = 0,
                                 ap_path_guiding_sample* slots = nullptr, int nl = 1, or_gsample* gs = nullptr) {
+    Count(CNT_PATHS);
     const Waves wv = MakeWaves(rng, nl);
     const float* lb = wv.lb[0];
     const bool guiding = G != nullptr;
@@ -1022,6 +1027,7 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
             continue;
         }
         Ctx c = HitContext(S, id, ro, rd, t, eps * 0.5f);
+        Count(CNT_HITS);
         const int model = S.objs[id].material.shading_model;
         if (depth == 0 && gs) {
             gs->albedo[0] = c.gb.albedo.x; gs->albedo[1] = c.gb.albedo.y; gs->albedo[2] = c.gb.albedo.z;
@@ -1045,6 +1051,8 @@ static float3 CalculateRadiance(const Scene& S, const RenderOpts& P, Rng& rng, f
         const ap_octahedron* probe = nullptr;
         if (guiding && !sc.delta && !sc.pureLight && !inside && !G->probes.empty()) {
             probe = G->Query(c.pos, Reflect(rd, c.normal));
+            Count(CNT_PROBE_QUERIES);
+            Count(CNT_PROBE_FOUND, probe ? 1 : 0);
             if (probe) {
                 const float pB = sc.pdf;
                 const float pP = OctahedronHemispherePDF(*probe, sc.next, c.normal, query);
@@ -1201,6 +1209,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
                                    bool inside0, const NrcOpts& N, bool train, bool unbiased, int pixel,
                                    NrcTrain* out, bool* queried, int nl = 1, const GuideState* G = nullptr,
                                    int frame = 0, ap_path_guiding_sample* slots = nullptr) {
+    Count(CNT_PATHS);
     const Waves wv = MakeWaves(rng, nl);
     const float* lb = wv.lb[0];
     const bool guiding = G != nullptr;
@@ -1289,6 +1298,7 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
             continue;
         }
         Ctx c = HitContext(S, id, ro, rd, t, eps * 0.5f);
+        Count(CNT_HITS);
         const float3 pos0 = c.pos;
         const int model = S.objs[id].material.shading_model;
         const Ctx ct = Textured(c);
@@ -1304,6 +1314,8 @@ static float3 CalculateRadianceNRC(const Scene& S, const RenderOpts& P, Rng& rng
         const ap_octahedron* probe = nullptr;
         if (guiding && !sc.delta && !sc.pureLight && !inside && !G->probes.empty()) {
             probe = G->Query(c.pos, Reflect(rd, c.normal));
+            Count(CNT_PROBE_QUERIES);
+            Count(CNT_PROBE_FOUND, probe ? 1 : 0);
             if (probe) {
                 const float pB = sc.pdf;
                 const float pP = OctahedronHemispherePDF(*probe, sc.next, c.normal, gquery);

@@ -31,6 +31,8 @@ def lib():
             getattr(L, nm).argtypes = [f32, f32, f32]
         L.or_glass_bsdf.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp, ctypes.c_uint64, vp]
         L.or_plant.argtypes = [i32]
+        L.or_counts_begin.argtypes = []
+        L.or_counts_end.argtypes = [vp]
         for nm in ("or_mse", "or_mse_luminance"):
             getattr(L, nm).restype = ctypes.c_double
             getattr(L, nm).argtypes = [vp, vp, sz]
@@ -157,6 +159,18 @@ class OracleBVH:
         idx = np.zeros(len(self.objs), dtype=np.int32)
         lib().or_bvh_export(self.h, _p(nodes), _p(idx))
         return nodes, idx
+
+    def line_margin(self):
+        """The core's CULLED-mode box growth for this tree: 2^-12 of the
+        largest root-bound coordinate magnitude (at least 1), in FP32."""
+        nodes, _ = self.export()
+        if len(nodes) == 0:
+            return np.float32(0.0)
+        r = nodes[0]
+        mag = np.float32(1.0)
+        for v in np.concatenate([r["bound_min"], r["bound_max"]]):
+            mag = max(mag, np.float32(abs(np.float32(v))))
+        return np.float32(mag * np.float32(2.0 ** -12))
 
     def trace(self, rays, any_hit=False, threads=None, stats=False, cull_margin=None):
         """Reference traversal; with cull_margin set, the core's CULLED visit order."""
@@ -643,6 +657,27 @@ class planted:
 
     def __exit__(self, *exc):
         lib().or_plant(0)
+        return False
+
+
+COUNT_NAMES = ("ext_rays", "ext_nodes", "ext_prims", "shadow_rays", "shadow_nodes", "shadow_prims", "hits",
+               "interacts", "probe_queries", "probe_found", "paths")
+
+
+class counting:
+    """Context manager: the oracle's visit accounting (or_counts_begin/_end)
+    over the renders inside it; `totals` holds the counts by COUNT_NAMES
+    afterwards."""
+
+    def __enter__(self):
+        self.totals = None
+        lib().or_counts_begin()
+        return self
+
+    def __exit__(self, *exc):
+        out = np.zeros(len(COUNT_NAMES), np.uint64)
+        lib().or_counts_end(_p(out))
+        self.totals = {k: int(v) for k, v in zip(COUNT_NAMES, out)}
         return False
 
 
