@@ -209,6 +209,10 @@ struct RenderState {
     // screen G-buffer samples of the paths in flight (framebuffers with a G-buffer)
     GSample* gsamples = nullptr;
     size_t gsample_cap = 0;
+    // A bounce's shadow rays are traced on `side` while the render stream
+    // traces the next bounce's extension rays; the events fork and join it.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_shaded = nullptr, ev_shadowed = nullptr;
 };
 
 constexpr int kEpsEntries = 1024;
@@ -221,6 +225,9 @@ void render_state_free(RenderState* r) {
                     r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters, r->gsamples};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (r->ev_shaded) (void)hipEventDestroy(r->ev_shaded);
+    if (r->ev_shadowed) (void)hipEventDestroy(r->ev_shadowed);
+    if (r->side) (void)hipStreamDestroy(r->side);
     delete r;
 }
 
@@ -1877,6 +1884,15 @@ bool camera_inside(const ap_scene* s, const std::vector<int32_t>& analytic, f3 o
     return false;
 }
 
+// The side stream (default priority, non-blocking) and its two events.
+int ensure_side(RenderState* r) {
+    if (r->side) return AP_OK;
+    hipError_t e = hipEventCreateWithFlags(&r->ev_shaded, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_shadowed, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_render: side stream");
+}
+
 int ensure_nrc(RenderState* r, size_t cap, size_t train_cap) {
     if (!r->nrc_counters) {
         const hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->nrc_counters), 4 * sizeof(uint32_t));
@@ -1987,12 +2003,19 @@ int run_frames(const RunCtx& c) {
         }
         auto* rc_ptr = train_only ? nullptr : reinterpret_cast<unsigned long long*>(fb->ray_count);
         int32_t* prim = train_only ? nullptr : fb->primary_id;
+        // Bounce k: shade, then the shadow rays on the side stream while the
+        // render stream traces bounce k + 1's extension rays (independent
+        // queues and outputs), joined before resolve. The shadow counter is
+        // cleared for the next shade only after resolve has read it.
         int in_q = 0;
+        hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, 1);
+        hipError_t e = prm->max_depth > 0
+                           ? launch_trace(qs, r->rays[0], total, r->hits, nullptr, false, stream, &r->counters[0])
+                           : hipSuccess;
+        if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
         for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
             const int out_q = 1 - in_q;
-            hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, out_q);
-            hipError_t e = launch_trace(qs, r->rays[in_q], total, r->hits, nullptr, false, stream, &r->counters[in_q]);
-            if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
+            const bool more = depth + 1 < prm->max_depth;
             ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                            prm->nee_always, prm->sky, in_q, out_q, r->n_lights};
             if (fb->gbuffer && !train_only) sp.gsample = r->gsamples;
@@ -2024,12 +2047,23 @@ int run_frames(const RunCtx& c) {
                                    r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
                                    r->shadow_rays, pays, r->counters, prim, last_frame, rc_ptr, gv, nv);
             }
-            e = launch_trace(qs, r->shadow_rays, total, r->shadow_hits, nullptr, true, stream, &r->counters[2],
-                             sdf_eps_of_depth(depth));
+            e = hipEventRecord(r->ev_shaded, stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(r->side, r->ev_shaded, 0);
+            if (e == hipSuccess)
+                e = launch_trace(qs, r->shadow_rays, total, r->shadow_hits, nullptr, true, r->side, &r->counters[2],
+                                 sdf_eps_of_depth(depth));
+            if (e == hipSuccess) e = hipEventRecord(r->ev_shadowed, r->side);
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
+            if (more) {
+                e = launch_trace(qs, r->rays[out_q], total, r->hits, nullptr, false, stream, &r->counters[out_q]);
+                if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
+            }
+            e = hipStreamWaitEvent(stream, r->ev_shadowed, 0);
+            if (e != hipSuccess) return hip_fail(e, "ap_render: join");
             hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->counters, pays,
                                r->shadow_hits, paths, rc_ptr);
             in_q = out_q;
+            if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, 1 - in_q);
             // Deep limits: stop once every path has ended.
             if (prm->max_depth > 32 && (depth % 16) == 15) {
                 uint32_t live = 0;
@@ -2064,7 +2098,7 @@ int run_frames(const RunCtx& c) {
                                fb->hdr, fb->odd, fb->even, fb->sum_xyz, gv.recs,
                                guiding ? guide_slots(s->guide) : nullptr, r->gsamples, fb->gbuffer,
                                fb->statistics);
-        const hipError_t e = hipGetLastError();
+        e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
         if (guiding && !train_only && (rc2 = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK)
             return rc2;
@@ -2150,6 +2184,7 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     const int32_t per_batch = (guiding || nrc)
         ? 1 : std::max(1, std::min(prm->spp, static_cast<int32_t>((4u << 20) / static_cast<uint32_t>(n_pix))));
     if ((rc = ensure_paths(r, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
+    if ((rc = ensure_side(r)) != AP_OK) return rc;
     if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(n_pix))) != AP_OK) return rc;
     if (fb->gbuffer && r->gsample_cap < static_cast<size_t>(per_batch) * n_pix) {
         if (r->gsamples) (void)hipFree(r->gsamples);
