@@ -2302,6 +2302,9 @@ struct BatchState {
     float* nrc_pool = nullptr;
     uint64_t nrc_last_hash = 0;
     bool nrc_rendered = false;
+    // Side stream of the overlapped shadow traces (as RenderState's).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_shaded = nullptr, ev_shadowed = nullptr;
 
     // Waits for the frames in flight, then destroys every graph.
     void drop_graphs() {
@@ -2322,6 +2325,9 @@ struct BatchState {
                         nrc_verts, nrc_vmask, nrc_offsets, nrc_slot_pix, nrc_pool};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
+        if (ev_shaded) (void)hipEventDestroy(ev_shaded);
+        if (ev_shadowed) (void)hipEventDestroy(ev_shadowed);
+        if (side) (void)hipStreamDestroy(side);
     }
     BatchView view() const {
         return BatchView{d_scenes, d_pix_begin, static_cast<int32_t>(scenes.size()), {ray_scene[0], ray_scene[1]},
@@ -2347,6 +2353,12 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
     }
     b->last_stream = stream;
     b->stream_used = true;
+    if (!b->side) {
+        e = hipEventCreateWithFlags(&b->ev_shaded, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_shadowed, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: side stream");
+    }
     for (size_t k = 0; k < b->scenes.size(); ++k) {
         ap_scene* s = b->scenes[k];
         if (!s->render) s->render = new RenderState();
@@ -2390,13 +2402,17 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
     hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
     hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
                        b->eps, paths, b->rays[0], b->path_of[0], b->media);
+    // Shadow rays of bounce k on the side stream, overlapped with bounce
+    // k + 1's extension rays; joined before resolve (as in run_frames).
     int in_q = 0;
+    hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, 1);
+    hipError_t e = prm->max_depth > 0 ? launch_trace_multi(b->shape, b->d_devs, b->ray_scene[0], b->rays[0], total,
+                                                           b->hits, false, stream, &b->counters[0])
+                                      : hipSuccess;
+    if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
     for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
         const int out_q = 1 - in_q;
-        hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, out_q);
-        hipError_t e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[in_q], b->rays[in_q], total, b->hits,
-                                          false, stream, &b->counters[in_q]);
-        if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
+        const bool more = depth + 1 < prm->max_depth;
         ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                        prm->nee_always, prm->sky, in_q, out_q, 0};
         sp.media = b->media;
@@ -2404,17 +2420,29 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
                            nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
                            b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, NrcView{}, bv);
-        e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
-                               stream, &b->counters[2], sdf_eps_of_depth(depth));
+        e = hipEventRecord(b->ev_shaded, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_shaded, 0);
+        if (e == hipSuccess)
+            e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
+                                   b->side, &b->counters[2], sdf_eps_of_depth(depth));
+        if (e == hipSuccess) e = hipEventRecord(b->ev_shadowed, b->side);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
+        if (more) {
+            e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[out_q], b->rays[out_q], total, b->hits, false,
+                                   stream, &b->counters[out_q]);
+            if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
+        }
+        e = hipStreamWaitEvent(stream, b->ev_shadowed, 0);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: join");
         hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
                            b->shadow_hits, paths, b->rays_traced);
         in_q = out_q;
+        if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, 1 - in_q);
     }
     hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
                        tables);
     hipLaunchKernelGGL(advance_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame);
-    const hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: launch");
 }
 
@@ -2574,13 +2602,17 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
                        np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount);
     const NrcView nv{b->nrc_paths, b->nrc_queries, b->nrc_qpath, b->nrc_verts, b->nrc_counters, np->spread,
                      b->nrc_qcount};
+    // Shadow rays of bounce k on the side stream, overlapped with bounce
+    // k + 1's extension rays; joined before resolve (as in run_frames).
     int in_q = 0;
+    hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, 1);
+    hipError_t e = prm->max_depth > 0 ? launch_trace_multi(b->shape, b->d_devs, b->ray_scene[0], b->rays[0], total,
+                                                           b->hits, false, stream, &b->counters[0])
+                                      : hipSuccess;
+    if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
     for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
         const int out_q = 1 - in_q;
-        hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, out_q);
-        hipError_t e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[in_q], b->rays[in_q], total, b->hits,
-                                          false, stream, &b->counters[in_q]);
-        if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
+        const bool more = depth + 1 < prm->max_depth;
         ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                        prm->nee_always, prm->sky, in_q, out_q, 0};
         sp.media = b->media;
@@ -2588,12 +2620,24 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
                            nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
                            b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, nv, bv);
-        e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
-                               stream, &b->counters[2], sdf_eps_of_depth(depth));
+        e = hipEventRecord(b->ev_shaded, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_shaded, 0);
+        if (e == hipSuccess)
+            e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
+                                   b->side, &b->counters[2], sdf_eps_of_depth(depth));
+        if (e == hipSuccess) e = hipEventRecord(b->ev_shadowed, b->side);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
+        if (more) {
+            e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[out_q], b->rays[out_q], total, b->hits, false,
+                                   stream, &b->counters[out_q]);
+            if (e != hipSuccess) return hip_fail(e, "ap_batch: extend");
+        }
+        e = hipStreamWaitEvent(stream, b->ev_shadowed, 0);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: join");
         hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
                            b->shadow_hits, paths, b->rays_traced);
         in_q = out_q;
+        if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, 1 - in_q);
     }
     // Cache vertices: every scene's queries to its own network, one inference for all.
     int rc = nrc_infer_groups(ns, b->nrc_queries, b->nrc_qpath, b->d_pix_begin, b->nrc_tile0, b->nrc_qcount,
@@ -2616,7 +2660,7 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
     hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
                        tables);
     hipLaunchKernelGGL(advance_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame);
-    const hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "ap_batch: launch");
     b->nrc_rendered = true;
     if (slots > 0 && np->train_steps > 0)
