@@ -336,10 +336,10 @@ AP_HD const ap_octahedron* pg_query_probe(const GuideView& g, f3 pos, f3 dir) {
 // Per-frame build (ap_guiding.hip).
 struct GuideDevice;
 int guide_prepare(GuideDevice** g, size_t n_paths);
-GuideView guide_view(const GuideDevice* g);
+GuideView guide_view(GuideDevice* g);
 ap_path_guiding_sample* guide_slots(GuideDevice* g);
 int guide_update(GuideDevice* g, size_t n_paths, hipStream_t stream);
 int guide_reset(GuideDevice* g);
-int guide_export(const GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count);
+int guide_export(GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count);
 
 }  // namespace ap

@@ -21,8 +21,10 @@
 //      running mean in FP64 and rebuilds the mips (MergeProbeKernel).
 // The scans and the stable radix sorts are the project's own wave64
 // primitives (ap_sort.hip).
-// The host reads back three small counts per frame (unique voxels with their
-// bounds, new probes), as the reference does.
+// The host waits once per frame, for the unique voxels' count and bounds
+// (the index volume is reallocated on the host when they grow it); the
+// valid-slot count stays on the device, and the new-probe count is read
+// back without a wait and taken in at the next use of the probes.
 #include <cstring>
 
 #include "aperture/ap_guiding.h"
@@ -59,7 +61,16 @@ struct GuideDevice {
     int32_t* info = nullptr;      // device: [0] unique count, [1..3] min xyz, [4..6] max xyz, [7] new probes, [8] error,
                                   // [9] valid slots
     int32_t* info_host = nullptr; // pinned
+    // Launch size of the voxel sort: a bound on the frame's valid slots from
+    // the frames before it; a frame with more sorts again at its real count.
+    uint32_t sort_hint = 0;
+    // The new-probe count of the last update is read back without a wait;
+    // `count` takes it in at the next use (settle).
+    bool pending = false;
+    hipEvent_t ev_count = nullptr;
 };
+
+int settle(GuideDevice* g);
 
 namespace {
 
@@ -119,20 +130,26 @@ __global__ void compact_slots_kernel(const uint32_t* __restrict__ flags, const u
     if (i == n - 1) info[9] = static_cast<int32_t>(ranks[i] + flags[i]);
     if (flags[i]) list[ranks[i]] = i;
 }
+// Item count of a launch sized for m: min(*dyn, m) (the valid slots).
+__device__ __forceinline__ uint32_t live_count(uint32_t m, const int32_t* dyn) {
+    return min(static_cast<uint32_t>(*dyn), m);
+}
+
 // (voxel, face) key of each valid slot, the slot index as value.
 __global__ void voxel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots,
-                                  const uint32_t* __restrict__ list, uint32_t m, uint64_t* __restrict__ keys,
-                                  uint32_t* __restrict__ vals) {
+                                  const uint32_t* __restrict__ list, uint32_t m, const int32_t* __restrict__ dyn,
+                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= m) return;
+    if (i >= live_count(m, dyn)) return;
     const uint32_t j = list[i];
     keys[i] = pack_voxel(slots[j].voxel_grid);
     vals[i] = j;
 }
 
-__global__ void run_heads_kernel(const uint64_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flags) {
+__global__ void run_heads_kernel(const uint64_t* __restrict__ keys, uint32_t n, const int32_t* __restrict__ dyn,
+                                 uint32_t* __restrict__ flags) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= live_count(n, dyn)) return;
     const uint64_t k = keys[i];
     flags[i] = (k != kNoKey && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
 }
@@ -141,6 +158,7 @@ __global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const ui
                                      const uint32_t* __restrict__ ranks, uint32_t n, uint64_t* __restrict__ uniq,
                                      int32_t* __restrict__ info) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    n = live_count(n, &info[9]);
     if (i >= n) return;
     if (i == n - 1) info[0] = static_cast<int32_t>(ranks[i] + flags[i]);
     if (!flags[i]) return;
@@ -152,6 +170,15 @@ __global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const ui
     atomicMax(&info[4], v.x);
     atomicMax(&info[5], v.y);
     atomicMax(&info[6], v.z);
+}
+
+// Restarts the unique count and voxel bounds before a second voxel sort.
+__global__ void reset_bounds_kernel(int32_t* __restrict__ info) {
+    if (threadIdx.x == 0) {
+        info[0] = 0;
+        info[1] = info[2] = info[3] = INT32_MAX;
+        info[4] = info[5] = info[6] = INT32_MIN;
+    }
 }
 
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, size_t n, int32_t v) {
@@ -191,6 +218,14 @@ __global__ void assign_kernel(const uint64_t* __restrict__ uniq, const uint32_t*
     if (!flags[i]) return;
     const int64_t s = vv.slot(unpack_voxel(uniq[i]));
     vv.volume[s] = base + static_cast<int32_t>(ranks[i]);
+}
+
+// Zeroes probes [base, base + info[7]) (the probes assign_kernel created).
+__global__ void zero_new_probes_kernel(ap_octahedron* __restrict__ p, size_t n_max, const int32_t* __restrict__ info) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const size_t n = min(n_max, static_cast<size_t>(info[7]));
+    if (i >= n * (sizeof(ap_octahedron) / 16)) return;
+    reinterpret_cast<uint4*>(p)[i] = uint4{0u, 0u, 0u, 0u};
 }
 
 __global__ void zero_probes_kernel(ap_octahedron* __restrict__ p, size_t n) {
@@ -321,7 +356,12 @@ int ensure_sort(GuideDevice* g, size_t n) {
 
 int grow_probes(GuideDevice* g, int32_t need, hipStream_t st) {
     if (need <= g->cap) return AP_OK;
-    const int32_t cap = std::max(need, std::max(1024, 2 * g->cap));
+    // Room for four times the need at the first growth, then doubling: a
+    // frame asks for room for all its unique voxels (count + n_uniq), which
+    // at 1080p is about twice the probe count, and a reallocation costs a
+    // wait and about a millisecond.
+    const int64_t grown = std::max<int64_t>(g->cap > 0 ? 2ll * g->cap : 4ll * need, 1024);
+    const int32_t cap = static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(need, grown), INT32_MAX));
     ap_octahedron *np = nullptr, *nt = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&np), static_cast<size_t>(cap) * sizeof(ap_octahedron));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&nt), static_cast<size_t>(cap) * sizeof(ap_octahedron));
@@ -364,6 +404,7 @@ void guide_free(GuideDevice* g) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (g->info_host) (void)hipHostFree(g->info_host);
+    if (g->ev_count) (void)hipEventDestroy(g->ev_count);
     delete g;
 }
 
@@ -381,9 +422,9 @@ int guide_prepare(GuideDevice** pg, size_t n_paths) {
     return AP_OK;
 }
 
-GuideView guide_view(const GuideDevice* g) {
+GuideView guide_view(GuideDevice* g) {
     GuideView v{};
-    if (!g) return v;
+    if (!g || settle(g) != AP_OK) return v;
     v.probes = g->probes;
     v.probe_count = g->count;
     v.volume = g->volume;
@@ -399,10 +440,23 @@ ap_path_guiding_sample* guide_slots(GuideDevice* g) { return g ? g->slots : null
 
 int guide_reset(GuideDevice* g) {
     if (!g) return AP_OK;
+    const int rc = settle(g);
+    if (rc != AP_OK) return rc;
     if (g->volume) (void)hipFree(g->volume);
     g->volume = nullptr;
     for (int a = 0; a < 3; ++a) g->start[a] = g->end[a] = g->size[a] = 0;
     g->count = 0;
+    return AP_OK;
+}
+
+// Takes in the new-probe count of the last update (a wait on its readback,
+// which the stream has normally long passed).
+int settle(GuideDevice* g) {
+    if (!g || !g->pending) return AP_OK;
+    const hipError_t e = hipEventSynchronize(g->ev_count);
+    if (e != hipSuccess) return hip_fail(e, "guiding: read new probes");
+    g->count += g->info_host[7];
+    g->pending = false;
     return AP_OK;
 }
 
@@ -411,35 +465,50 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
     if (n_slots == 0) return AP_OK;
     if (n_slots > 0xffffffffull) return fail(AP_ERR_UNSUPPORTED, "guiding: too many sample slots in one frame");
     const uint32_t n = static_cast<uint32_t>(n_slots);
-    int rc = ensure_sort(g, n_slots);
+    int rc = settle(g);
     if (rc != AP_OK) return rc;
+    if ((rc = ensure_sort(g, n_slots)) != AP_OK) return rc;
+    if (!g->ev_count) {
+        const hipError_t e = hipEventCreateWithFlags(&g->ev_count, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "guiding: event");
+    }
     const int32_t init[16] = {0, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN, 0, 0};
     hipError_t e = hipMemcpyAsync(g->info, init, sizeof(init), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: info reset");
-    // 0. the valid slots, in slot order (most of a frame's slots are empty).
+    // 0. the valid slots, in slot order (most of a frame's slots are empty);
+    // their count m stays on the device (info[9]).
     hipLaunchKernelGGL(valid_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, g->flags, g->info);
     e = scan_exclusive_u32(g->flags, g->ranks, n, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: slot scan");
     hipLaunchKernelGGL(compact_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->flags, g->ranks, n, g->list, g->info);
-    e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: read valid count");
+    const uint32_t* m_dev = reinterpret_cast<const uint32_t*>(g->info + 9);
+    // 1. unique voxels in sorted order, launched for `cap` slots and run
+    // over the first min(m, cap).
+    auto unique_voxels = [&](uint32_t cap) -> hipError_t {
+        hipLaunchKernelGGL(voxel_keys_kernel, grid_for(cap), dim3(kBlock), 0, st, g->slots, g->list, cap, g->info + 9,
+                           g->keys[0], g->vals[0]);
+        hipError_t err = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], cap, 0, 64, g->tmp, st,
+                                              m_dev);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(run_heads_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], cap, g->info + 9,
+                           g->flags);
+        err = scan_exclusive_u32(g->flags, g->ranks, cap, nullptr, g->tmp, st, m_dev);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(unique_voxels_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks,
+                           cap, g->uniq, g->info);
+        err = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        return err == hipSuccess ? hipStreamSynchronize(st) : err;
+    };
+    const uint32_t cap = std::min(n, std::max(g->sort_hint, 1u << 16));
+    if ((e = unique_voxels(cap)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
     if (g->info_host[8]) return fail(AP_ERR_UNSUPPORTED, "guiding: a voxel coordinate is outside [-2^18, 2^18)");
     const uint32_t m = static_cast<uint32_t>(g->info_host[9]);
+    g->sort_hint = static_cast<uint32_t>(std::min<uint64_t>(n, 2ull * m));
     if (m == 0) return AP_OK;
-    // 1. unique voxels in sorted order.
-    hipLaunchKernelGGL(voxel_keys_kernel, grid_for(m), dim3(kBlock), 0, st, g->slots, g->list, m, g->keys[0],
-                       g->vals[0]);
-    e = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, 64, g->tmp, st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
-    hipLaunchKernelGGL(run_heads_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], m, g->flags);
-    e = scan_exclusive_u32(g->flags, g->ranks, m, nullptr, g->tmp, st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: voxel scan");
-    hipLaunchKernelGGL(unique_voxels_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, m,
-                       g->uniq, g->info);
-    e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
+    if (m > cap) {  // more valid slots than the launch covered: sort them all
+        hipLaunchKernelGGL(reset_bounds_kernel, dim3(1), dim3(64), 0, st, g->info);
+        if ((e = unique_voxels(m)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+    }
     const uint32_t n_uniq = static_cast<uint32_t>(g->info_host[0]);
     if (n_uniq == 0) return AP_OK;
     // 2. grow the index volume to the union of the bounds.
@@ -482,40 +551,43 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
             g->size[a] = nd[a];
         }
     }
-    // Probes for the voxels without one, numbered in sorted order.
+    // Probes for the voxels without one, numbered in sorted order: at most
+    // n_uniq new ones, so room for them is made before their number is known.
+    if ((rc = grow_probes(g, g->count + static_cast<int32_t>(n_uniq), st)) != AP_OK) return rc;
     hipLaunchKernelGGL(unindexed_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, n_uniq, view_of(g), g->flags);
     e = scan_exclusive_u32(g->flags, g->ranks, n_uniq, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: probe scan");
     hipLaunchKernelGGL(assign_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, g->flags, g->ranks, n_uniq,
                        view_of(g), g->count, g->info);
+    hipLaunchKernelGGL(zero_new_probes_kernel,
+                       grid_for(static_cast<size_t>(n_uniq) * (sizeof(ap_octahedron) / 16)), dim3(kBlock), 0, st,
+                       g->probes + g->count, static_cast<size_t>(n_uniq), g->info);
     e = hipMemcpyAsync(g->info_host + 7, g->info + 7, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: read new probes");
-    const int32_t n_new = g->info_host[7];
-    if (n_new > 0) {
-        if ((rc = grow_probes(g, g->count + n_new, st)) != AP_OK) return rc;
-        hipLaunchKernelGGL(zero_probes_kernel, grid_for(static_cast<size_t>(n_new) * (sizeof(ap_octahedron) / 16)),
-                           dim3(kBlock), 0, st, g->probes + g->count, static_cast<size_t>(n_new));
-        g->count += n_new;
-    }
+    if (e == hipSuccess) e = hipEventRecord(g->ev_count, st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: new probes");
+    // Probes [count, count + n_uniq) cover the new ones; those past them have
+    // no samples and no texel keys.
+    const int32_t bound = g->count + static_cast<int32_t>(n_uniq);
     // 3. per (probe, texel) sums in slot order.
     hipLaunchKernelGGL(texel_keys_kernel, grid_for(m), dim3(kBlock), 0, st, g->slots, g->list, m, view_of(g),
                        g->keys[0], g->vals[0]);
     int probe_bits = 1;
-    while ((1ll << probe_bits) <= g->count) ++probe_bits;
+    while ((1ll << probe_bits) <= bound) ++probe_bits;
     const int end_bit = std::min(64, 6 + probe_bits + 1);
     e = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, end_bit, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: texel sort");
     hipLaunchKernelGGL(texel_sums_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], g->vals[1], m, g->slots,
                        g->temp);
-    // 4. FP64 merge and mips.
-    if (g->count > 0)
-        hipLaunchKernelGGL(merge_kernel, dim3(static_cast<unsigned>(g->count)), dim3(64), 0, st, g->probes, g->temp);
+    // 4. FP64 merge and mips (a probe without samples this frame is left as it is).
+    hipLaunchKernelGGL(merge_kernel, dim3(static_cast<unsigned>(bound)), dim3(64), 0, st, g->probes, g->temp);
+    g->pending = true;
     e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "guiding: update");
 }
 
-int guide_export(const GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count) {
+int guide_export(GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count) {
+    const int rc = settle(g);
+    if (rc != AP_OK) return rc;
     if (count) *count = g ? g->count : 0;
     if (bounds) {
         for (int a = 0; a < 3; ++a) {

@@ -61,8 +61,14 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sum) {
     return before + inc - v;
 }
 
+__device__ __forceinline__ uint32_t count_of(uint32_t n, const uint32_t* dyn_n) {
+    return dyn_n ? min(*dyn_n, n) : n;
+}
+
 __global__ __launch_bounds__(kScanThreads) void scan_totals_kernel(const uint32_t* __restrict__ in, uint32_t n,
+                                                                   const uint32_t* __restrict__ dyn_n,
                                                                    uint32_t* __restrict__ totals) {
+    n = count_of(n, dyn_n);
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
     uint32_t s = 0;
     for (int k = 0; k < kScanItems; ++k)
@@ -88,8 +94,10 @@ __global__ __launch_bounds__(1024) void scan_offsets_kernel(uint32_t* __restrict
 }
 
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in, uint32_t n,
+                                                                  const uint32_t* __restrict__ dyn_n,
                                                                   const uint32_t* __restrict__ offsets,
                                                                   uint32_t* __restrict__ out) {
+    n = count_of(n, dyn_n);
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
@@ -106,8 +114,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
 }
 
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t* __restrict__ keys, uint32_t n,
-                                                                  int shift, uint32_t* __restrict__ counts) {
+                                                                  const uint32_t* __restrict__ dyn_n, int shift,
+                                                                  uint32_t* __restrict__ counts) {
     __shared__ uint32_t hist[256];
+    n = count_of(n, dyn_n);
     hist[threadIdx.x] = 0u;
     __syncthreads();
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
@@ -121,8 +131,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t
 
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint64_t* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out, uint32_t n, int shift, const uint32_t* __restrict__ offsets) {
+    uint32_t* __restrict__ vals_out, uint32_t n, const uint32_t* __restrict__ dyn_n, int shift,
+    const uint32_t* __restrict__ offsets) {
     __shared__ uint32_t cnt[kWaves][256];
+    n = count_of(n, dyn_n);
     __shared__ uint32_t base_of[kWaves][256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * 256; i += kSortThreads) (&cnt[0][0])[i] = 0u;
@@ -183,22 +195,22 @@ size_t sort_workspace_bytes(size_t n) {
 }
 
 hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, void* work,
-                              hipStream_t stream) {
+                              hipStream_t stream, const uint32_t* dyn_n) {
     if (n == 0) {
         if (total) hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, stream, total, 0u);
         return hipGetLastError();
     }
     const uint32_t nb = div_up(n, kScanTile);
     auto* totals = static_cast<uint32_t*>(work);
-    hipLaunchKernelGGL(scan_totals_kernel, dim3(nb), dim3(kScanThreads), 0, stream, in, n, totals);
+    hipLaunchKernelGGL(scan_totals_kernel, dim3(nb), dim3(kScanThreads), 0, stream, in, n, dyn_n, totals);
     hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, stream, totals, nb, total);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, stream, in, n, totals, out);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, stream, in, n, dyn_n, totals, out);
     return hipGetLastError();
 }
 
 hipError_t radix_sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in,
                                 uint32_t* vals_out, uint32_t n, int begin_bit, int end_bit, void* work,
-                                hipStream_t stream) {
+                                hipStream_t stream, const uint32_t* dyn_n) {
     if (n == 0) return hipSuccess;
     if (begin_bit < 0 || end_bit > 64 || end_bit <= begin_bit) return hipErrorInvalidValue;
     const uint32_t nb = div_up(n, kSortTile);
@@ -220,11 +232,11 @@ hipError_t radix_sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, con
         uint64_t* dst_k = to_out ? keys_out : tmp_keys;
         uint32_t* dst_v = to_out ? vals_out : tmp_vals;
         const int shift = begin_bit + 8 * pass;
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, stream, src_k, n, shift, counts);
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, stream, src_k, n, dyn_n, shift, counts);
         hipError_t e = scan_exclusive_u32(counts, counts, static_cast<uint32_t>(n_counts), nullptr, scan_work, stream);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, stream, src_k, src_v, dst_k, dst_v,
-                           n, shift, counts);
+                           n, dyn_n, shift, counts);
         src_k = dst_k;
         src_v = dst_v;
     }
