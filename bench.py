@@ -672,7 +672,8 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
     return out
 
 
-def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_frames=8, boost_steps=16):
+def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_frames=8, boost_steps=16,
+                    c5=False):
     """Equal-time relMSE of batched rendering with per-scene learned radiance
     caches (one width-32 FP16 network per scene in a grouped cache: the
     scenes' cache queries answered by one grouped inference per frame, every
@@ -686,10 +687,15 @@ def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_fr
     calibrated time per frame fits in the same GPU time, its first
     boost_frames frames training with boost_steps Adam steps (their cost
     calibrated and charged). relMSE per scene against a plain batch of
-    ref_spp frames on another seed; reported as the mean over scenes."""
+    ref_spp frames on another seed; reported as the mean over scenes.
+    With c5, the scenes are the batched workload's own (synth_batch_scenes:
+    48 heightfield scenes, triangles log-uniform in [1e4, 5e5], resolutions
+    in {128^2 .. 1024^2}, seed 6)."""
     g = np.random.default_rng(seed)
     objs, cams = [], []
-    for k in range(n_scenes):
+    if c5:
+        objs, cams = ap.synth_batch_scenes(n_scenes, seed=6)
+    for k in range(0 if c5 else n_scenes):
         res = int(g.choice([256, 512]))
         if k % 2 == 0:
             tris = float(np.exp(g.uniform(np.log(1e4), np.log(2e5))))
@@ -776,11 +782,14 @@ def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_fr
     batch.close()
     for sc in scenes:
         sc.close()
-    kinds = ["heightfield" if k % 2 == 0 else "box" for k in range(n_scenes)]
+    kinds = ["heightfield" if (c5 or k % 2 == 0) else "box" for k in range(n_scenes)]
     mean = lambda v, kind=None: float(np.mean([x for x, t in zip(v, kinds) if kind in (None, t)]))
+    workload = (f"{n_scenes} heightfield+icosphere scenes of the batched workload (triangles log-uniform in "
+                "[1e4, 5e5], resolutions uniform in {128^2, 256^2, 512^2, 1024^2}, seed 6)" if c5 else
+                f"{n_scenes} scenes (heightfield+icosphere and closed boxes, alternating), resolutions uniform in "
+                "{256^2, 512^2}")
     out = {
-        "workload": f"{n_scenes} scenes (heightfield+icosphere and closed boxes, alternating), resolutions uniform in "
-                    f"{{256^2, 512^2}}, max depth {depth}, RR from depth 4, four hero wavelengths, no sky; one "
+        "workload": f"{workload}, max depth {depth}, RR from depth 4, four hero wavelengths, no sky; one "
                     "width-32 FP16 radiance-cache network per scene",
         "pixels_per_frame": int(pix),
         "equal_time_ms": round(budget, 1),
@@ -797,7 +806,7 @@ def batch_nrc_bench(ap, n_scenes, plain_spp, ref_spp, depth=12, seed=9, boost_fr
         "relmse_nrc_mean": round(mean(nrc), 5),
         "relmse_ratio_plain_over_nrc": round(mean(plain) / mean(nrc), 3),
         "ratio_heightfield_scenes": round(mean(plain, "heightfield") / mean(nrc, "heightfield"), 3),
-        "ratio_box_scenes": round(mean(plain, "box") / mean(nrc, "box"), 3),
+        "ratio_box_scenes": None if c5 else round(mean(plain, "box") / mean(nrc, "box"), 3),
         "per_scene_ratio_min_max": [round(min(a / b for a, b in zip(plain, nrc)), 3),
                                     round(max(a / b for a, b in zip(plain, nrc)), 3)],
     }
@@ -826,6 +835,9 @@ def main():
     ap_.add_argument("--batch-nrc-spp", type=int, default=64,
                      help="plain spp that sets the equal-time budget of the per-scene cache batch comparison")
     ap_.add_argument("--batch-nrc-ref-spp", type=int, default=1024, help="spp of its plain reference images")
+    ap_.add_argument("--batch-nrc-c5-spp", type=int, default=16,
+                     help="plain spp of the per-scene cache comparison on the 48-scene batched workload (0 skips it)")
+    ap_.add_argument("--batch-nrc-c5-ref-spp", type=int, default=512, help="spp of its plain reference images")
     ap_.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap_.add_argument("--c1", type=int, default=1, help="run the 256x256 closed-box workload on GPU and CPU (0 skips it)")
     ap_.add_argument("--sweep", action="store_true",
@@ -1020,6 +1032,8 @@ def main():
         line["batch"] = batch_bench(ap, args.batch_spp, 12)
     if args.batch_nrc_scenes > 0:
         line["batch_nrc"] = batch_nrc_bench(ap, args.batch_nrc_scenes, args.batch_nrc_spp, args.batch_nrc_ref_spp)
+    if args.batch_nrc_c5_spp > 0:
+        line["batch_nrc_c5"] = batch_nrc_bench(ap, 48, args.batch_nrc_c5_spp, args.batch_nrc_c5_ref_spp, c5=True)
     print(json.dumps(line), flush=True)
 
 
