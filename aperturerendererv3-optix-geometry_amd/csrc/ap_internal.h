@@ -173,9 +173,18 @@ int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const 
 // `steps` Adam steps on batches of `batch` rows per network: network g's
 // rows are drawn from the pool_n[g] samples (20 floats each) at
 // pool_begin[g] of `pool` (from 0 without pool_begin); nothing changes for a
-// network whose pool_n is 0.
+// network whose pool_n is 0. The draws follow `seed`; with frame_dev they
+// follow nrc_frame_hash(frame_seed, *frame_dev) ^ kNrcTrainSalt, read on the
+// device, so that a captured frame replays with the frame counter.
 int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_begin, const uint32_t* pool_n,
-                   uint32_t batch, uint32_t steps, float lr, uint64_t seed, hipStream_t st);
+                   uint32_t batch, uint32_t steps, float lr, uint64_t seed, hipStream_t st,
+                   const int32_t* frame_dev = nullptr, uint64_t frame_seed = 0);
+// Adds `steps` to the cache's host count of Adam steps (steps replayed from
+// a captured frame; negative to take back the steps a capture counted).
+void nrc_count_steps(NrcState* s, int32_t steps);
+// A process-unique number of the cache and its training buffers, renewed
+// when one is reallocated (a captured training step names its buffers).
+uint64_t nrc_generation(const NrcState* s);
 // Inference of every network of a grouped cache: region g = rows [seg[g],
 // seg[g+1]) (device), of which the first count[g] are live (all without
 // count), goes to network g, cut into 64-row tiles numbered from tile0[g];

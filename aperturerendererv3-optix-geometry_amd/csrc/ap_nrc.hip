@@ -28,6 +28,7 @@
 // networks scale the loss gradient by a power of two (exactly undone on the
 // FP32 gradient) so that it stays clear of FP16's subnormal range, and
 // saturate conversions at +-65504.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -499,7 +500,20 @@ __global__ void reduce_grad_kernel(const float* __restrict__ part, int chunks_pe
     grad[p] = s * inv_scale;
 }
 
-// Adam (the oracle's or_nrc_adam), then the 16-bit copies W[o][i] and
+// The bias-correction powers of the next Adam step, kept on the device so
+// that captured training steps replay with the right step: bias[0..2] are
+// the FP32 running products beta1^t, beta2^t and ema_decay^t (one factor
+// per step since the parameters were set).
+__global__ void bias_step_kernel(float* __restrict__ bias, float beta1, float beta2, float ema_decay) {
+    if (threadIdx.x == 0) {
+        bias[0] = bias[0] * beta1;
+        bias[1] = bias[1] * beta2;
+        bias[2] = bias[2] * ema_decay;
+    }
+}
+
+// Adam (the oracle's or_nrc_adam; c1 = 1 - beta1^t, c2 = 1 - beta2^t from
+// the device's running products), then the 16-bit copies W[o][i] and
 // W^T[i][o] of the training weights, and the exponential moving average of
 // the parameters (decay ema_decay, debiased by ema_c = 1 - decay^t) whose
 // 16-bit copy answers inference. A gate that reads 0 (gate[g] with
@@ -507,12 +521,13 @@ __global__ void reduce_grad_kernel(const float* __restrict__ part, int chunks_pe
 template <int W, bool F16>
 __global__ void adam_kernel(uint32_t groups, float* __restrict__ params, float* __restrict__ m,
                             float* __restrict__ v, const float* __restrict__ grad, float lr, float beta1, float beta2,
-                            float eps, float c1, float c2, uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
-                            const uint32_t* __restrict__ gate, int gate_per_group, float ema_decay, float ema_c,
-                            float* __restrict__ ema, uint16_t* __restrict__ w_inf) {
+                            float eps, const float* __restrict__ bias, uint16_t* __restrict__ w,
+                            uint16_t* __restrict__ wt, const uint32_t* __restrict__ gate, int gate_per_group,
+                            float ema_decay, float* __restrict__ ema, uint16_t* __restrict__ w_inf) {
     using G = Geo<W>;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= groups * G::NPAR) return;
+    const float c1 = 1.0f - bias[0], c2 = 1.0f - bias[1], ema_c = 1.0f - bias[2];
     const uint32_t g = p / G::NPAR;
     if (gate && gate[gate_per_group ? g : 0u] == 0u) return;
     const float gr = grad[p];
@@ -538,14 +553,17 @@ __global__ void adam_kernel(uint32_t groups, float* __restrict__ params, float* 
 // mod N_g of the network's pool (pool_n[g] samples from pool_begin[g], or
 // from 0 without pool_begin), P = 2^31 - 1 a prime above any pool size, so
 // that the rows of the steps of a frame walk the pool without repetition
-// while they last. Rows are laid out [network][batch].
+// while they last. Rows are laid out [network][batch]. With frame_dev, off
+// is derived from the device's frame counter (a captured frame's draws).
 __global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ pool,
                                                      const uint32_t* __restrict__ pool_begin,
                                                      const uint32_t* __restrict__ pool_n, uint32_t batch,
                                                      uint32_t groups, uint32_t step, uint64_t off,
+                                                     const int32_t* __restrict__ frame_dev, uint64_t frame_seed,
                                                      float* __restrict__ out) {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
     if (r >= batch * groups) return;
+    if (frame_dev) off = splitmix64(nrc_frame_hash(frame_seed, *frame_dev) ^ kNrcTrainSalt);
     const uint32_t g = r / batch, j = r % batch;
     const uint32_t n = pool_n[g];
     float4* dst = reinterpret_cast<float4*>(out + static_cast<size_t>(r) * SREC);
@@ -570,18 +588,20 @@ struct NrcState {
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     uint16_t *w = nullptr, *wt = nullptr;
     float* ema = nullptr;          // moving average of the parameters (zero-started, debiased)
+    float* bias = nullptr;         // device: beta1^t, beta2^t, ema_decay^t (running FP32 products)
     uint16_t* w_inf = nullptr;     // 16-bit weights inference reads: the debiased average, or w
     float ema_decay = 0.0f;
     Bounds bounds{{-1.0f, -1.0f, -1.0f}, {1.0f, 1.0f, 1.0f}};  // network 0's (all networks' with one)
     std::vector<Bounds> host_bounds;                            // per network
     Bounds* dev_bounds = nullptr;                               // per network, device copy
     uint32_t* map = nullptr;       // seg[groups + 1] and tile0[groups + 1] of ap_nrc_infer_grouped
-    int step = 0;
+    int step = 0;                  // Adam steps since the parameters were set (host count)
     uint32_t cap = 0;
     uint16_t *acts = nullptr, *deltas = nullptr, *dout = nullptr;
     float *y = nullptr, *part = nullptr, *loss_part = nullptr;
     float* batch = nullptr;   // gathered training rows of nrc_train_pool
     uint32_t batch_cap = 0;
+    uint64_t generation = 0;  // a process-unique number, renewed when a training buffer is reallocated
 };
 
 namespace {
@@ -641,12 +661,22 @@ int upload_params(NrcState* s, const std::vector<float>& p) {
     if (e == hipSuccess) e = hipMemcpy(s->w, w.data(), G * np * sizeof(uint16_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->w_inf, w.data(), G * np * sizeof(uint16_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(s->ema, 0, G * np * sizeof(float));
+    const float ones[4] = {1.0f, 1.0f, 1.0f, 0.0f};
+    if (e == hipSuccess) e = hipMemcpy(s->bias, ones, sizeof(ones), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->wt, wt.data(), G * nt * sizeof(uint16_t), hipMemcpyHostToDevice);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: upload parameters");
 }
 
+// Process-unique numbers for NrcState::generation: a captured frame is keyed
+// by it, and a new cache may reuse a destroyed one's address.
+uint64_t next_generation() {
+    static std::atomic<uint64_t> counter{0};
+    return ++counter;
+}
+
 int ensure_train(NrcState* s, uint32_t n) {
     if (s->cap >= n) return AP_OK;
+    s->generation = next_generation();
     void* ptrs[] = {s->acts, s->deltas, s->dout, s->y, s->part, s->loss_part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -723,18 +753,17 @@ int nrc_gradient(NrcState* s, const float* samples, uint32_t rows, hipStream_t s
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc: gradient");
 }
 
+constexpr float kBeta1 = 0.9f, kBeta2 = 0.99f, kAdamEps = 1e-8f;
+
 int nrc_adam(NrcState* s, float lr, hipStream_t st, const uint32_t* gate = nullptr, int gate_per_group = 0) {
     s->step += 1;
-    const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
-    const float c1 = 1.0f - std::pow(b1, static_cast<float>(s->step));
-    const float c2 = 1.0f - std::pow(b2, static_cast<float>(s->step));
-    const float ec = 1.0f - std::pow(s->ema_decay, static_cast<float>(s->step));
+    hipLaunchKernelGGL(bias_step_kernel, dim3(1), dim3(64), 0, st, s->bias, kBeta1, kBeta2, s->ema_decay);
     with_net(s, [&](auto net) {
         using N = decltype(net);
         constexpr int NPAR = Geo<N::W>::NPAR;
         hipLaunchKernelGGL((adam_kernel<N::W, N::F16>), dim3((s->groups * NPAR + 255) / 256), dim3(256), 0, st,
-                           s->groups, s->params, s->m, s->v, s->grad, lr, b1, b2, eps, c1, c2, s->w, s->wt, gate,
-                           gate_per_group, s->ema_decay, ec, s->ema, s->w_inf);
+                           s->groups, s->params, s->m, s->v, s->grad, lr, kBeta1, kBeta2, kAdamEps, s->bias, s->w,
+                           s->wt, gate, gate_per_group, s->ema_decay, s->ema, s->w_inf);
         return 0;
     });
     const hipError_t e = hipGetLastError();
@@ -748,12 +777,14 @@ int nrc_infer_dyn(NrcState* s, const float* queries, const int32_t* rows, const 
 }
 
 int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_begin, const uint32_t* pool_n,
-                   uint32_t batch, uint32_t steps, float lr, uint64_t seed, hipStream_t st) {
+                   uint32_t batch, uint32_t steps, float lr, uint64_t seed, hipStream_t st, const int32_t* frame_dev,
+                   uint64_t frame_seed) {
     if (steps == 0) return AP_OK;
     if (batch == 0 || batch % kChunkRows != 0 || batch * static_cast<uint64_t>(s->groups) > (1u << 24))
         return fail(AP_ERR_INVALID_ARG, "ap_nrc: batch must be a positive multiple of 256, at most 2^24 rows in all");
     const uint32_t rows = batch * s->groups;
     if (s->batch_cap < rows) {
+        s->generation = next_generation();
         if (s->batch) (void)hipFree(s->batch);
         s->batch = nullptr;
         s->batch_cap = 0;
@@ -765,7 +796,7 @@ int nrc_train_pool(NrcState* s, const float* pool, const uint32_t* pool_begin, c
     const int per_group = s->groups > 1 ? 1 : 0;
     for (uint32_t k = 0; k < steps; ++k) {
         hipLaunchKernelGGL(gather_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, pool, pool_begin, pool_n,
-                           batch, s->groups, k, off, s->batch);
+                           batch, s->groups, k, off, frame_dev, frame_seed, s->batch);
         int rc = nrc_gradient(s, s->batch, batch, st, pool_n, per_group);
         if (rc == AP_OK) rc = nrc_adam(s, lr, st, pool_n, per_group);
         if (rc != AP_OK) return rc;
@@ -797,6 +828,9 @@ int nrc_set_group_bounds(NrcState* s, uint32_t g, const float lo[3], const float
     return upload_bounds(s);
 }
 
+void nrc_count_steps(NrcState* s, int32_t steps) { s->step += steps; }
+uint64_t nrc_generation(const NrcState* s) { return s->generation; }
+
 void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]) {
     for (int a = 0; a < 3; ++a) {
         s->bounds.lo[a] = lo[a];
@@ -807,7 +841,7 @@ void nrc_set_bounds(NrcState* s, const float lo[3], const float hi[3]) {
 void nrc_free(NrcState* s) {
     if (!s) return;
     void* ptrs[] = {s->params, s->m, s->v, s->grad, s->w, s->wt, s->acts, s->deltas, s->dout, s->y, s->part,
-                    s->loss_part, s->batch, s->ema, s->w_inf, s->dev_bounds, s->map};
+                    s->loss_part, s->batch, s->ema, s->w_inf, s->dev_bounds, s->map, s->bias};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -859,6 +893,7 @@ int ap_nrc_create_grouped(uint32_t groups, uint32_t width, int32_t precision, ui
     if (precision != AP_NRC_BF16 && precision != AP_NRC_FP16)
         return fail(AP_ERR_INVALID_ARG, "ap_nrc_create_grouped: precision must be AP_NRC_BF16 or AP_NRC_FP16");
     NrcState* s = new NrcState();
+    s->generation = next_generation();
     s->width = static_cast<int>(width);
     s->f16 = precision == AP_NRC_FP16;
     s->groups = groups;
@@ -874,6 +909,7 @@ int ap_nrc_create_grouped(uint32_t groups, uint32_t width, int32_t precision, ui
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->wt), groups * s->npar_t * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->dev_bounds), groups * sizeof(Bounds));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->map), 2 * (groups + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->bias), 4 * sizeof(float));
     if (e == hipSuccess) e = hipMemset(s->m, 0, np * sizeof(float));
     if (e == hipSuccess) e = hipMemset(s->v, 0, np * sizeof(float));
     if (e == hipSuccess) e = hipMemset(s->grad, 0, np * sizeof(float));
@@ -1054,8 +1090,14 @@ int ap_nrc_train_step(ap_nrc* nrc, const float* samples, size_t n, float lr, voi
 int ap_nrc_set_ema(ap_nrc* nrc, float decay) {
     if (!nrc) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_ema: NULL cache");
     if (!(decay >= 0.0f && decay < 1.0f)) return fail(AP_ERR_INVALID_ARG, "ap_nrc_set_ema: decay must be in [0, 1)");
-    nrc->s->ema_decay = decay;
-    return AP_OK;
+    NrcState* s = nrc->s;
+    s->ema_decay = decay;
+    // The debiasing power of the new decay after the steps taken so far.
+    float pw = 1.0f;
+    for (int t = 0; t < s->step && pw != 0.0f; ++t) pw = pw * decay;
+    hipError_t e = hipDeviceSynchronize();  // no training step may be in flight
+    if (e == hipSuccess) e = hipMemcpy(s->bias + 2, &pw, sizeof(float), hipMemcpyHostToDevice);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_nrc_set_ema");
 }
 
 int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]) {

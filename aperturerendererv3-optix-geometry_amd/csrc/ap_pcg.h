@@ -41,6 +41,13 @@ AP_HD uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
+// Hash of a radiance-cache frame (seed, frame index): its training-path
+// offset, and, with kNrcTrainSalt, the seed of its training draws.
+AP_HD uint64_t nrc_frame_hash(uint64_t seed, int32_t frame) {
+    return splitmix64(splitmix64(static_cast<uint64_t>(frame)) ^ seed ^ 0x6e7263ull);
+}
+constexpr uint64_t kNrcTrainSalt = 0x747261696eull;
+
 // The stream of one path sample: stream `pixel`, start state hashed from
 // (seed, frame, pixel). Seeding consecutive frames with the raw frame number
 // (pcg32_seed(pixel, frame)) leaves their start states on an arithmetic

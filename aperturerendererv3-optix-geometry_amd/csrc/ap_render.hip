@@ -1551,12 +1551,14 @@ __device__ __forceinline__ void nrc_scene_train(const NrcBatchScene& T, uint32_t
 }
 
 // nrc_init_kernel over every scene of a batch; also clears the per-scene
-// query counts.
+// query counts. The frame hash comes from the batch's frame counter (so a
+// captured frame replays with the right one).
 __global__ __launch_bounds__(kBlock) void nrc_init_batch_kernel(BatchView bv, uint32_t total,
-                                                                const NrcBatchScene* __restrict__ T, uint64_t h,
+                                                                const NrcBatchScene* __restrict__ T, uint64_t seed,
                                                                 uint32_t unbiased_every, NrcPath* __restrict__ paths,
                                                                 uint32_t* __restrict__ counters,
                                                                 uint32_t* __restrict__ qcount) {
+    const uint64_t h = nrc_frame_hash(seed, *bv.frame);
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i == 0) { counters[0] = 0; counters[1] = 0; }
     if (i < static_cast<uint32_t>(bv.n_scenes)) qcount[i] = 0;
@@ -1599,7 +1601,7 @@ __global__ __launch_bounds__(kBlock) void nrc_apply_batch_kernel(BatchView bv, u
 // slot_pix[t] = the slot's path.
 template <int NL>
 __global__ __launch_bounds__(kBlock) void nrc_targets_batch_kernel(uint32_t slots, BatchView bv,
-                                                                   const NrcBatchScene* __restrict__ T, uint64_t h,
+                                                                   const NrcBatchScene* __restrict__ T, uint64_t seed,
                                                                    const NrcPath* __restrict__ npaths,
                                                                    const PathT<NL>* __restrict__ paths,
                                                                    const float* __restrict__ tables,
@@ -1616,7 +1618,7 @@ __global__ __launch_bounds__(kBlock) void nrc_targets_batch_kernel(uint32_t slot
     }
     const NrcBatchScene Ts = T[lo];
     uint32_t offset, n_train;
-    nrc_scene_train(Ts, bv.pix_begin[lo + 1] - bv.pix_begin[lo], h, offset, n_train);
+    nrc_scene_train(Ts, bv.pix_begin[lo + 1] - bv.pix_begin[lo], nrc_frame_hash(seed, *bv.frame), offset, n_train);
     const uint32_t k = t - Ts.slot_base;
     if (k >= n_train) {
         vmask[t] = 0u;
@@ -1933,10 +1935,6 @@ float sdf_eps_of_depth(int depth) {
     return 0.5f * (1.0e-3f * std::exp(fminf(static_cast<float>(depth) * 0.5f, 4.0f)));
 }
 
-// Frame hash of the training-path offset and the training draws.
-uint64_t nrc_frame_hash(uint64_t seed, int32_t frame) {
-    return splitmix64(splitmix64(static_cast<uint64_t>(frame)) ^ seed ^ 0x6e7263ull);
-}
 
 }  // namespace
 
@@ -2127,7 +2125,7 @@ int run_frames(const RunCtx& c) {
             h = nrc_frame_hash(prm->seed, fp.frame0);
             train_pixels(h, n_offset, n_train);
         }
-        if ((rc = run_batch(fp, false, n_offset, n_train, h ^ 0x747261696eull)) != AP_OK) return rc;
+        if ((rc = run_batch(fp, false, n_offset, n_train, h ^ kNrcTrainSalt)) != AP_OK) return rc;
         // Training-only passes of the cache: the training paths alone, on
         // their own pixels and random streams.
         const uint32_t passes = (nrc && n_train_max > 0) ? np->train_passes : 0u;
@@ -2138,7 +2136,7 @@ int run_frames(const RunCtx& c) {
             if (t2 == 0) continue;
             const FrameParams tp{fp.frame0, 1, static_cast<int32_t>(t2), inside, splitmix64(prm->seed ^ hk), o2,
                                  n_stride};
-            if ((rc = run_batch(tp, true, o2, t2, hk ^ 0x747261696eull)) != AP_OK) return rc;
+            if ((rc = run_batch(tp, true, o2, t2, hk ^ kNrcTrainSalt)) != AP_OK) return rc;
         }
     }
     return AP_OK;
@@ -2274,6 +2272,13 @@ struct BatchState {
         hipStream_t stream;
         hipGraph_t graph;
         hipGraphExec_t exec;
+        // Frames with per-scene caches (ap_batch_render_nrc): the cache, its
+        // parameters and the buffers the frame names; null for plain frames.
+        const NrcState* ns;
+        ap_nrc_render_params np;
+        uint64_t nrc_generation;
+        uint32_t nrc_slots;
+        size_t nrc_slot_cap;
     };
     std::vector<Graph> graphs;
     static constexpr size_t kMaxGraphs = 16;
@@ -2446,8 +2451,13 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: launch");
 }
 
-bool same_key(const BatchState::Graph& g, const BatchState* b, const ap_render_params* p, hipStream_t st) {
+bool same_key(const BatchState::Graph& g, const BatchState* b, const ap_render_params* p, hipStream_t st,
+              const NrcState* ns = nullptr, const ap_nrc_render_params* np = nullptr) {
     const ap_render_params& k = g.key;
+    if (g.ns != ns) return false;
+    if (ns && (std::memcmp(&g.np, np, sizeof(*np)) != 0 || g.nrc_generation != nrc_generation(ns) ||
+               g.nrc_slots != b->nrc_slots || g.nrc_slot_cap != b->nrc_slot_cap))
+        return false;
     return g.stream == st && k.seed == p->seed && k.max_depth == p->max_depth &&
            k.rr_min_depth == p->rr_min_depth && k.nee_max_depth == p->nee_max_depth && k.use_nee == p->use_nee &&
            k.nee_always == p->nee_always && k.sky == p->sky && k.wavelengths == p->wavelengths &&
@@ -2500,7 +2510,8 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
             (void)hipGraphDestroy(g);
             return hip_fail(e, "ap_batch: instantiate graph");
         }
-        b->graphs.push_back(BatchState::Graph{*prm, b->shape.wave_rays, b->shape.refill, stream, g, exec});
+        b->graphs.push_back(BatchState::Graph{*prm, b->shape.wave_rays, b->shape.refill, stream, g, exec, nullptr,
+                                              ap_nrc_render_params{}, 0ull, 0u, 0});
     }
     for (int32_t f = 0; f < frames; ++f) {
         e = hipGraphLaunch(exec, stream);
@@ -2583,23 +2594,24 @@ int batch_nrc_prepare(BatchState* b, const ap_nrc_render_params* np, NrcState* n
 
 // One batch frame with per-scene caches: batch_frame's bounces with paths
 // ending into their scene's network, the grouped inference of all networks,
-// the scenes' training samples, the running means, then the networks' Adam
-// steps on their own scenes' samples (one set of launches for all).
+// the scenes' training samples, the running means, the networks' Adam steps
+// on their own scenes' samples (one set of launches for all), then the frame
+// counter. Everything that changes from frame to frame (the frame hash, the
+// training draws, Adam's bias corrections) is read on the device, so the
+// sequence can be captured once and replayed.
 template <int NL>
 int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_render_params* np, NrcState* ns,
-                    int32_t frame, hipStream_t stream, const f4* rgb_lut, const f4* lam_lut, const float* tables) {
+                    hipStream_t stream, const f4* rgb_lut, const f4* lam_lut, const float* tables) {
     const BatchView bv = b->view();
     auto* paths = reinterpret_cast<PathT<NL>*>(b->paths);
     auto* pays = reinterpret_cast<ShadowPay<NL>*>(b->shadow_payload);
     const uint32_t total = b->total;
     const int32_t n = static_cast<int32_t>(b->scenes.size());
-    const uint64_t h = nrc_frame_hash(prm->seed, frame);
-    b->nrc_last_hash = h;
     hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
     hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
                        b->eps, paths, b->rays[0], b->path_of[0], b->media);
-    hipLaunchKernelGGL(nrc_init_batch_kernel, grid_of(total), dim3(kBlock), 0, stream, bv, total, b->nrc_table, h,
-                       np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount);
+    hipLaunchKernelGGL(nrc_init_batch_kernel, grid_of(total), dim3(kBlock), 0, stream, bv, total, b->nrc_table,
+                       prm->seed, np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount);
     const NrcView nv{b->nrc_paths, b->nrc_queries, b->nrc_qpath, b->nrc_verts, b->nrc_counters, np->spread,
                      b->nrc_qcount};
     // Shadow rays of bounce k on the side stream, overlapped with bounce
@@ -2648,7 +2660,7 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
     const uint32_t slots = b->nrc_slots;
     if (slots > 0) {
         hipLaunchKernelGGL(nrc_targets_batch_kernel<NL>, grid_of(slots), dim3(kBlock), 0, stream, slots, bv,
-                           b->nrc_table, h, b->nrc_paths, paths, tables, b->nrc_verts, b->nrc_vmask,
+                           b->nrc_table, prm->seed, b->nrc_paths, paths, tables, b->nrc_verts, b->nrc_vmask,
                            b->nrc_slot_pix);
         hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, slots, b->nrc_vmask, b->nrc_offsets,
                            b->nrc_counters);
@@ -2659,14 +2671,15 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
     }
     hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
                        tables);
-    hipLaunchKernelGGL(advance_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "ap_batch: launch");
-    b->nrc_rendered = true;
-    if (slots > 0 && np->train_steps > 0)
-        return nrc_train_pool(ns, b->nrc_pool, b->nrc_pool_begin, b->nrc_pool_n, np->batch, np->train_steps,
-                              np->learning_rate, h ^ 0x747261696eull, stream);
-    return AP_OK;
+    if (slots > 0 && np->train_steps > 0 &&
+        (rc = nrc_train_pool(ns, b->nrc_pool, b->nrc_pool_begin, b->nrc_pool_n, np->batch, np->train_steps,
+                             np->learning_rate, 0, stream, b->frame, prm->seed)) != AP_OK)
+        return rc;
+    hipLaunchKernelGGL(advance_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame);
+    e = hipGetLastError();
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: launch");
 }
 
 int batch_check_params(const ap_render_params* prm) {
@@ -2945,14 +2958,56 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
     }
     if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: reset counters");
     // Frame after frame, each rendered with the networks as the frames before
-    // it trained them; launched directly (the Adam step count changes per frame).
+    // it trained them. The call's first frame runs directly (it sizes every
+    // buffer it needs); the others replay one captured frame, cached per
+    // render parameters, cache parameters, buffers, launch shape and stream.
+    auto frame = [&]() {
+        return params->wavelengths == 4 ? batch_frame_nrc<4>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables)
+                                        : batch_frame_nrc<1>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables);
+    };
+    const int32_t steps = (B.nrc_slots > 0) ? static_cast<int32_t>(nrc_params->train_steps) : 0;
+    hipGraphExec_t exec = nullptr;
     for (int32_t f = 0; f < params->spp; ++f) {
-        const int32_t frame = params->frame_index + f;
-        rc = params->wavelengths == 4
-            ? batch_frame_nrc<4>(&B, params, nrc_params, ns, frame, st, rgb_lut, lam_lut, tables)
-            : batch_frame_nrc<1>(&B, params, nrc_params, ns, frame, st, rgb_lut, lam_lut, tables);
-        if (rc != AP_OK) return rc;
+        if (f == 0 || !B.use_graph || st == nullptr) {
+            if ((rc = frame()) != AP_OK) return rc;
+            continue;
+        }
+        if (!exec)
+            for (const BatchState::Graph& g : B.graphs)
+                if (same_key(g, &B, params, st, ns, nrc_params)) exec = g.exec;
+        if (!exec) {
+            if (B.graphs.size() >= BatchState::kMaxGraphs) {
+                e = hipDeviceSynchronize();
+                if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: wait before dropping a graph");
+                (void)hipGraphExecDestroy(B.graphs.front().exec);
+                (void)hipGraphDestroy(B.graphs.front().graph);
+                B.graphs.erase(B.graphs.begin());
+            }
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: begin capture");
+            rc = frame();
+            nrc_count_steps(ns, -steps);  // captured, not run
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(st, &g);
+            if (rc != AP_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: end capture");
+            e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return hip_fail(e, "ap_batch_render_nrc: instantiate graph");
+            }
+            B.graphs.push_back(BatchState::Graph{*params, B.shape.wave_rays, B.shape.refill, st, g, exec, ns,
+                                                 *nrc_params, nrc_generation(ns), B.nrc_slots, B.nrc_slot_cap});
+        }
+        e = hipGraphLaunch(exec, st);
+        if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: graph launch");
+        nrc_count_steps(ns, steps);
     }
+    B.nrc_last_hash = nrc_frame_hash(params->seed, params->frame_index + params->spp - 1);
+    B.nrc_rendered = true;
     if (ray_count) {
         unsigned long long c[2] = {0, 0};
         e = hipMemcpyAsync(c, B.rays_traced, sizeof(c), hipMemcpyDeviceToHost, st);

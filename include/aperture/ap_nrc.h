@@ -3,7 +3,9 @@
  * reference integrator has no cache. It follows neural radiance caching
  * (Mueller, Rousselle, Novak, Keller 2021): a 64 -> 64 x 5 -> 3 ReLU network
  * without biases, BF16 weights and activations with FP32 accumulation on
- * MFMA, relative L2 loss, Adam (beta1 0.9, beta2 0.99, eps 1e-8).
+ * MFMA, relative L2 loss, Adam (beta1 0.9, beta2 0.99, eps 1e-8; the bias
+ * corrections 1 - beta^t take beta^t as the FP32 product of t factors,
+ * kept on the device, so captured training steps replay correctly).
  *
  * Query record (16 floats): position xyz, outgoing direction xyz, shading
  * normal xyz, roughness, albedo rgb, 3 unused. Training sample (20 floats):
@@ -66,8 +68,9 @@ int ap_nrc_last_gradient(const ap_nrc* nrc, float* host_grad);
 /* Inference weights: with decay > 0, the exponential moving average of the
  * parameters over the Adam steps since the last ap_nrc_set_params/create,
  * ema_t = decay ema_{t-1} + (1 - decay) w_t from ema_0 = 0, divided by
- * 1 - decay^t; with 0 (the default) the current parameters. Training always
- * uses the current parameters. */
+ * 1 - decay^t (decay^t an FP32 product of t factors); with 0 (the default)
+ * the current parameters. Training always uses the current parameters.
+ * Waits for the device (no training step may be in flight). */
 int ap_nrc_set_ema(ap_nrc* nrc, float decay);
 /* Bounds the position encoding uses (set by ap_nrc_set_bounds or ap_render_nrc). */
 int ap_nrc_get_bounds(const ap_nrc* nrc, float bmin[3], float bmax[3]);

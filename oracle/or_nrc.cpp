@@ -371,11 +371,18 @@ double or_nrc_loss_fixed(const float* params, const float* feat, const float* ta
     return loss;
 }
 
-// One Adam step (t = step number from 1) on FP32 master parameters.
+// One Adam step (t = step number from 1) on FP32 master parameters; the
+// bias corrections 1 - beta^t take beta^t as the FP32 product of t factors
+// (the core keeps that product on the device).
 void or_nrc_adam_n(float* params, float* m, float* v, const float* grad, size_t n, int t, float lr, float beta1,
                    float beta2, float eps) {
-    const float c1 = 1.0f - std::pow(beta1, static_cast<float>(t));
-    const float c2 = 1.0f - std::pow(beta2, static_cast<float>(t));
+    float p1 = 1.0f, p2 = 1.0f;
+    for (int k = 0; k < t; ++k) {
+        p1 = p1 * beta1;
+        p2 = p2 * beta2;
+    }
+    const float c1 = 1.0f - p1;
+    const float c2 = 1.0f - p2;
     for (size_t k = 0; k < n; ++k) {
         const float g = grad[k];
         m[k] = beta1 * m[k] + (1.0f - beta1) * g;

@@ -116,6 +116,49 @@ def test_fp16_width32_caches_train_deterministically(gpu):
     batch.close()
 
 
+def test_replayed_cache_frames_equal_direct_frames(gpu):
+    """A call of several frames runs its first frame directly and replays a
+    captured frame for the rest (frame hash, training draws and Adam's bias
+    corrections read on the device). Four frames in one call must equal four
+    calls of one frame each (all direct), bit for bit: images, FP64 sums,
+    trained weights; and a second cache created after the first is destroyed
+    (possibly at its address) must not replay the first one's frame."""
+    objs, cams = bg.scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    nprm = ap.NrcRenderParams.defaults(train_paths=512, batch=512, train_steps=3)
+    stream = ap.Stream()
+    outs = []
+    batch = ap.Batch(scenes, cams)
+    for calls in ((4,), (1, 1, 1, 1), (4,)):
+        cache = ap.RadianceCache(seed=5, groups=len(scenes), width=32, precision="fp16")
+        hdrs, sums = [], []
+        for c in cams:
+            n = c.width * c.height
+            h = ap.DeviceBuffer(n * 12)
+            h.zero()
+            hdrs.append(h)
+            sums.append(ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64)))
+        frame = 0
+        for spp in calls:
+            prm = ap.RenderParams.defaults(spp=spp, frame_index=frame, max_depth=10, rr_min_depth=3, seed=31,
+                                           wavelengths=4)
+            batch.render_nrc(prm, cache, nprm, hdrs, sums, stream=stream)
+            frame += spp
+        stream.synchronize()
+        imgs = [(h.to_numpy(np.float32, c.width * c.height * 3), s.to_numpy(np.float64, c.width * c.height * 3))
+                for c, h, s in zip(cams, hdrs, sums)]
+        outs.append((imgs, cache.params(), [(x.queries, x.samples) for x in batch.nrc_last_frame()]))
+        print(f"[batch-nrc] calls {calls}: per-scene (queries, samples) {outs[-1][2]}", flush=True)
+        cache.close()
+    batch.close()
+    for imgs, w, st in outs[1:]:
+        assert st == outs[0][2]
+        assert np.array_equal(w, outs[0][1])
+        for (h0, s0), (h1, s1) in zip(outs[0][0], imgs):
+            assert np.array_equal(h0, h1) and np.array_equal(s0, s1)
+    assert np.isfinite(outs[0][1]).all()
+
+
 def test_per_scene_fp16_caches_gain_over_the_plain_batch(gpu):
     objs = [ap.synth_box_scene(1, 2, 15.0), ap.synth_box_scene(4, 1, 10.0)]
     cams = [ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 96, 96), ap.Camera.make([0.2, 0.1, 3.0], [0, 0, -1], 50.0,

@@ -95,8 +95,12 @@ def test_adam_matches_numpy():
     p2, m2, v2 = ol.nrc_adam(p, m, v, g, t, lr, b1, b2, eps)
     mm = b1 * m + (np.float32(1) - b1) * g
     vv = b2 * v + (np.float32(1) - b2) * (g * g)
-    c1 = np.float32(1) - np.float32(b1 ** t)
-    c2 = np.float32(1) - np.float32(b2 ** t)
+    p1 = p2_ = np.float32(1)
+    for _ in range(t):  # beta^t as the FP32 product of t factors
+        p1, p2_ = np.float32(p1 * b1), np.float32(p2_ * b2)
+    c1 = np.float32(1) - p1
+    c2 = np.float32(1) - p2_
+    assert abs(float(c1) - (1 - 0.9 ** 3)) < 1e-7 and abs(float(c2) - (1 - 0.99 ** 3)) < 1e-7
     pp = p - lr * (mm / c1) / (np.sqrt(vv / c2) + eps)
     assert np.array_equal(m2, mm) and np.array_equal(v2, vv)
     assert np.allclose(p2, pp, rtol=1e-6, atol=1e-9)
