@@ -154,22 +154,47 @@ __global__ void run_heads_kernel(const uint64_t* __restrict__ keys, uint32_t n, 
     flags[i] = (k != kNoKey && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
 }
 
+// Wave-wide minimum / maximum (every lane of the wave takes part).
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// The unique voxels in sorted order, their count (info[0]) and bounds
+// (info[1..6]); the bounds are reduced over each wave first, so the six
+// atomics are taken once per wave rather than once per voxel.
 __global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
                                      const uint32_t* __restrict__ ranks, uint32_t n, uint64_t* __restrict__ uniq,
                                      int32_t* __restrict__ info) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     n = live_count(n, &info[9]);
-    if (i >= n) return;
-    if (i == n - 1) info[0] = static_cast<int32_t>(ranks[i] + flags[i]);
-    if (!flags[i]) return;
-    uniq[ranks[i]] = keys[i];
-    const ap_int4 v = unpack_voxel(keys[i]);
-    atomicMin(&info[1], v.x);
-    atomicMin(&info[2], v.y);
-    atomicMin(&info[3], v.z);
-    atomicMax(&info[4], v.x);
-    atomicMax(&info[5], v.y);
-    atomicMax(&info[6], v.z);
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    if (i < n) {
+        if (i == n - 1) info[0] = static_cast<int32_t>(ranks[i] + flags[i]);
+        if (flags[i]) {
+            uniq[ranks[i]] = keys[i];
+            const ap_int4 v = unpack_voxel(keys[i]);
+            lo[0] = hi[0] = v.x;
+            lo[1] = hi[1] = v.y;
+            lo[2] = hi[2] = v.z;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min(lo[a]);
+        hi[a] = wave_max(hi[a]);
+    }
+    if ((threadIdx.x & 63) == 0 && lo[0] != INT32_MAX) {
+        atomicMin(&info[1], lo[0]);
+        atomicMin(&info[2], lo[1]);
+        atomicMin(&info[3], lo[2]);
+        atomicMax(&info[4], hi[0]);
+        atomicMax(&info[5], hi[1]);
+        atomicMax(&info[6], hi[2]);
+    }
 }
 
 // Restarts the unique count and voxel bounds before a second voxel sort.

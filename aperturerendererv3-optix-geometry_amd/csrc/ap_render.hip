@@ -30,6 +30,7 @@
 #include "aperture/ap_nrc.h"
 #include "aperture/ap_render.h"
 #include "ap_internal.h"
+#include "ap_sort.h"
 #include "ap_pcg.h"
 #include "ap_shade.h"
 #include "ap_spectrum.h"
@@ -1292,30 +1293,25 @@ __global__ __launch_bounds__(kBlock) void nrc_targets_kernel(uint32_t n_train, u
     vmask[t] = nrc_slot_targets<NL>(t, pid, npaths, paths, tables, verts);
 }
 
-// Exclusive prefix sum of the per-slot sample counts in one block (slot
-// order, so the sample pool is the same on every run); the total goes to
-// counters[1].
-__global__ __launch_bounds__(1024) void nrc_scan_kernel(uint32_t n_train, const uint32_t* __restrict__ vmask,
-                                                        uint32_t* __restrict__ offsets, uint32_t* __restrict__ counters) {
-    __shared__ uint32_t part[1024];
-    const uint32_t per = (n_train + 1023u) / 1024u;
-    const uint32_t lo = threadIdx.x * per, hi = min(lo + per, n_train);
-    uint32_t sum = 0;
-    for (uint32_t t = lo; t < hi; ++t) sum += __popc(vmask[t]);
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {
-        const uint32_t add = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += add;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - sum;
-    for (uint32_t t = lo; t < hi; ++t) {
-        offsets[t] = run;
-        run += __popc(vmask[t]);
-    }
-    if (threadIdx.x == 1023u) counters[1] = part[1023];
+// Per-slot sample counts (the set bits of each slot's vertex mask); their
+// exclusive prefix sum (the project's device-wide scan, in slot order, so
+// the sample pool is the same on every run) places each slot's samples.
+__global__ void nrc_count_kernel(uint32_t n_train, const uint32_t* __restrict__ vmask,
+                                 uint32_t* __restrict__ counts) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < n_train) counts[t] = static_cast<uint32_t>(__popc(vmask[t]));
+}
+
+// The offsets array of `cap` training slots is followed by the scan's workspace.
+size_t nrc_offsets_words(size_t cap) { return ((cap + 63) & ~size_t{63}) + scan_workspace_bytes(cap) / 4 + 1; }
+void* nrc_scan_work(uint32_t* offsets, size_t cap) { return offsets + ((cap + 63) & ~size_t{63}); }
+
+// offsets[t] = samples of slots before t; the total to counters[1].
+hipError_t nrc_scan(uint32_t n_train, const uint32_t* vmask, uint32_t* offsets, size_t cap, uint32_t* counters,
+                    hipStream_t stream) {
+    hipLaunchKernelGGL(nrc_count_kernel, dim3((std::max<uint32_t>(n_train, 1u) + kBlock - 1) / kBlock), dim3(kBlock),
+                       0, stream, n_train, vmask, offsets);
+    return scan_exclusive_u32(offsets, offsets, n_train, &counters[1], nrc_scan_work(offsets, cap), stream);
 }
 
 // Training samples in slot, vertex order: query (13 floats), 3 zeros, target
@@ -1922,7 +1918,7 @@ int ensure_nrc(RenderState* r, size_t cap, size_t train_cap) {
         r->nrc_train_cap = 0;
         if ((rc = dev_alloc(&r->nrc_verts, train_cap * kNrcV * kNrcVert)) != AP_OK) return rc;
         if ((rc = dev_alloc(&r->nrc_vmask, train_cap)) != AP_OK) return rc;
-        if ((rc = dev_alloc(&r->nrc_offsets, train_cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&r->nrc_offsets, nrc_offsets_words(train_cap))) != AP_OK) return rc;
         if ((rc = dev_alloc(&r->nrc_pool, train_cap * kNrcV * kNrcSample)) != AP_OK) return rc;
         r->nrc_train_cap = train_cap;
     }
@@ -2083,8 +2079,9 @@ int run_frames(const RunCtx& c) {
                 hipLaunchKernelGGL(nrc_targets_kernel<NL>, grid_of(n_train), dim3(kBlock), 0, stream, n_train,
                                    n_stride, n_offset, train_only ? 1u : 0u, r->nrc_paths, paths, tables,
                                    r->nrc_verts, r->nrc_vmask);
-                hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, n_train, r->nrc_vmask,
-                                   r->nrc_offsets, r->nrc_counters);
+                const hipError_t se = nrc_scan(n_train, r->nrc_vmask, r->nrc_offsets, r->nrc_train_cap,
+                                               r->nrc_counters, stream);
+                if (se != hipSuccess) return hip_fail(se, "ap_render: training sample scan");
                 hipLaunchKernelGGL(nrc_compact_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
                                    n_offset, r->nrc_vmask, r->nrc_offsets, r->nrc_verts, r->nrc_pool);
             }
@@ -2570,7 +2567,7 @@ int batch_nrc_prepare(BatchState* b, const ap_nrc_render_params* np, NrcState* n
         b->nrc_slot_cap = 0;
         if ((rc = dev_alloc(&b->nrc_verts, slot_cap * kNrcV * kNrcVert)) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_vmask, slot_cap)) != AP_OK) return rc;
-        if ((rc = dev_alloc(&b->nrc_offsets, slot_cap)) != AP_OK) return rc;
+        if ((rc = dev_alloc(&b->nrc_offsets, nrc_offsets_words(slot_cap))) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_slot_pix, slot_cap)) != AP_OK) return rc;
         if ((rc = dev_alloc(&b->nrc_pool, slot_cap * kNrcV * kNrcSample)) != AP_OK) return rc;
         b->nrc_slot_cap = slot_cap;
@@ -2662,8 +2659,9 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
         hipLaunchKernelGGL(nrc_targets_batch_kernel<NL>, grid_of(slots), dim3(kBlock), 0, stream, slots, bv,
                            b->nrc_table, prm->seed, b->nrc_paths, paths, tables, b->nrc_verts, b->nrc_vmask,
                            b->nrc_slot_pix);
-        hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, stream, slots, b->nrc_vmask, b->nrc_offsets,
-                           b->nrc_counters);
+        if ((e = nrc_scan(slots, b->nrc_vmask, b->nrc_offsets, b->nrc_slot_cap, b->nrc_counters, stream)) !=
+            hipSuccess)
+            return hip_fail(e, "ap_batch: training sample scan");
         hipLaunchKernelGGL(nrc_pool_ranges_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, slots,
                            b->nrc_table, b->nrc_offsets, b->nrc_counters, b->nrc_pool_begin, b->nrc_pool_n);
         hipLaunchKernelGGL(nrc_compact_kernel, grid_of(slots), dim3(kBlock), 0, stream, slots, 1u, 0u, b->nrc_vmask,
