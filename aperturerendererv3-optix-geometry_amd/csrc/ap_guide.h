@@ -339,6 +339,12 @@ int guide_prepare(GuideDevice** g, size_t n_paths);
 GuideView guide_view(GuideDevice* g);
 ap_path_guiding_sample* guide_slots(GuideDevice* g);
 int guide_update(GuideDevice* g, size_t n_paths, hipStream_t stream);
+// guide_update in two halves around its one wait, so that several scenes'
+// builds can share a wait: begin enqueues up to the read-back of the counts
+// the host needs; after the stream has been synchronised, end enqueues the
+// rest.
+int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t stream);
+int guide_update_end(GuideDevice* g, hipStream_t stream);
 int guide_reset(GuideDevice* g);
 int guide_export(GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count);
 

@@ -68,6 +68,10 @@ struct GuideDevice {
     // `count` takes it in at the next use (settle).
     bool pending = false;
     hipEvent_t ev_count = nullptr;
+    // An update between guide_update_begin and guide_update_end: its slot
+    // count and voxel-sort launch size.
+    bool open = false;
+    uint32_t upd_n = 0, upd_cap = 0;
 };
 
 int settle(GuideDevice* g);
@@ -195,6 +199,13 @@ __global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const ui
         atomicMax(&info[5], hi[1]);
         atomicMax(&info[6], hi[2]);
     }
+}
+
+// Starts a build's counters: unique count 0, empty bounds, no new probes,
+// no error, no valid slots.
+__global__ void init_info_kernel(int32_t* __restrict__ info) {
+    const int k = threadIdx.x;
+    if (k < 16) info[k] = (k >= 1 && k <= 3) ? INT32_MAX : (k >= 4 && k <= 6) ? INT32_MIN : 0;
 }
 
 // Restarts the unique count and voxel bounds before a second voxel sort.
@@ -485,7 +496,25 @@ int settle(GuideDevice* g) {
     return AP_OK;
 }
 
-int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
+// Steps 0 and 1 of a build, then the read-back of the counts the host
+// needs (unique voxels and their bounds), all enqueued without a wait.
+hipError_t unique_voxels(GuideDevice* g, uint32_t cap, hipStream_t st) {
+    const uint32_t* m_dev = reinterpret_cast<const uint32_t*>(g->info + 9);
+    hipLaunchKernelGGL(voxel_keys_kernel, grid_for(cap), dim3(kBlock), 0, st, g->slots, g->list, cap, g->info + 9,
+                       g->keys[0], g->vals[0]);
+    hipError_t err = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], cap, 0, 64, g->tmp, st,
+                                          m_dev);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(run_heads_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], cap, g->info + 9, g->flags);
+    err = scan_exclusive_u32(g->flags, g->ranks, cap, nullptr, g->tmp, st, m_dev);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(unique_voxels_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, cap,
+                       g->uniq, g->info);
+    return hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+}
+
+int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
+    g->open = false;
     const size_t n_slots = n_paths * PG_COLLECT;
     if (n_slots == 0) return AP_OK;
     if (n_slots > 0xffffffffull) return fail(AP_ERR_UNSUPPORTED, "guiding: too many sample slots in one frame");
@@ -497,42 +526,36 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
         const hipError_t e = hipEventCreateWithFlags(&g->ev_count, hipEventDisableTiming);
         if (e != hipSuccess) return hip_fail(e, "guiding: event");
     }
-    const int32_t init[16] = {0, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN, 0, 0};
-    hipError_t e = hipMemcpyAsync(g->info, init, sizeof(init), hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: info reset");
+    hipLaunchKernelGGL(init_info_kernel, dim3(1), dim3(64), 0, st, g->info);
     // 0. the valid slots, in slot order (most of a frame's slots are empty);
     // their count m stays on the device (info[9]).
     hipLaunchKernelGGL(valid_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, g->flags, g->info);
-    e = scan_exclusive_u32(g->flags, g->ranks, n, nullptr, g->tmp, st);
+    hipError_t e = scan_exclusive_u32(g->flags, g->ranks, n, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: slot scan");
     hipLaunchKernelGGL(compact_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->flags, g->ranks, n, g->list, g->info);
-    const uint32_t* m_dev = reinterpret_cast<const uint32_t*>(g->info + 9);
     // 1. unique voxels in sorted order, launched for `cap` slots and run
     // over the first min(m, cap).
-    auto unique_voxels = [&](uint32_t cap) -> hipError_t {
-        hipLaunchKernelGGL(voxel_keys_kernel, grid_for(cap), dim3(kBlock), 0, st, g->slots, g->list, cap, g->info + 9,
-                           g->keys[0], g->vals[0]);
-        hipError_t err = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], cap, 0, 64, g->tmp, st,
-                                              m_dev);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL(run_heads_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], cap, g->info + 9,
-                           g->flags);
-        err = scan_exclusive_u32(g->flags, g->ranks, cap, nullptr, g->tmp, st, m_dev);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL(unique_voxels_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks,
-                           cap, g->uniq, g->info);
-        err = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
-        return err == hipSuccess ? hipStreamSynchronize(st) : err;
-    };
     const uint32_t cap = std::min(n, std::max(g->sort_hint, 1u << 16));
-    if ((e = unique_voxels(cap)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+    if ((e = unique_voxels(g, cap, st)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+    g->open = true;
+    g->upd_n = n;
+    g->upd_cap = cap;
+    return AP_OK;
+}
+
+int guide_update_end(GuideDevice* g, hipStream_t st) {
+    if (!g->open) return AP_OK;
+    g->open = false;
+    int rc;
+    hipError_t e;
     if (g->info_host[8]) return fail(AP_ERR_UNSUPPORTED, "guiding: a voxel coordinate is outside [-2^18, 2^18)");
     const uint32_t m = static_cast<uint32_t>(g->info_host[9]);
-    g->sort_hint = static_cast<uint32_t>(std::min<uint64_t>(n, 2ull * m));
+    g->sort_hint = static_cast<uint32_t>(std::min<uint64_t>(g->upd_n, 2ull * m));
     if (m == 0) return AP_OK;
-    if (m > cap) {  // more valid slots than the launch covered: sort them all
+    if (m > g->upd_cap) {  // more valid slots than the launch covered: sort them all
         hipLaunchKernelGGL(reset_bounds_kernel, dim3(1), dim3(64), 0, st, g->info);
-        if ((e = unique_voxels(m)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+        if ((e = unique_voxels(g, m, st)) == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
     }
     const uint32_t n_uniq = static_cast<uint32_t>(g->info_host[0]);
     if (n_uniq == 0) return AP_OK;
@@ -608,6 +631,14 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
     g->pending = true;
     e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "guiding: update");
+}
+
+int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
+    int rc = guide_update_begin(g, n_paths, st);
+    if (rc != AP_OK || !g->open) return rc;
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
+    return guide_update_end(g, st);
 }
 
 int guide_export(GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count) {

@@ -154,6 +154,8 @@ struct alignas(16) BatchScene {
     int32_t pad;
     const TexRec* textures;    // the scene's texture table
     const float* uvs;          // per object: the triangle's three vertex UVs
+    GuideView guide;                      // guided batches: the scene's probes and index volume
+    ap_path_guiding_sample* guide_slots;  // guided batches: the scene's sample slots, pixel-major
 };
 
 // What the batch kernels see of a batch.
@@ -722,7 +724,6 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     ap_ray* __restrict__ shadow_rays, ShadowPay<NL>* __restrict__ shadow_pay, uint32_t* __restrict__ counters,
     int32_t* __restrict__ primary_id, int32_t primary_frame, unsigned long long* __restrict__ ray_count, GuideView gv,
     NrcView nv, BatchView bv = BatchView{}) {
-    static_assert(!MULTI || !GUIDE, "a batch renders without guiding");
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
     if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
@@ -754,6 +755,15 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
         n_volumes = B.n_volumes;
         textures = B.textures;
         uvs = B.uvs;
+        if constexpr (GUIDE) {  // the scene's probes; the path records are the batch's (gv.recs)
+            gv.probes = B.guide.probes;
+            gv.probe_count = B.guide.probe_count;
+            gv.volume = B.guide.volume;
+            for (int a = 0; a < 3; ++a) {
+                gv.start[a] = B.guide.start[a];
+                gv.size[a] = B.guide.size[a];
+            }
+        }
     }
     // The MediaContext the ray was traced in.
     const bool inside = (S.flags & PATH_INSIDE) != 0u;
@@ -1444,7 +1454,8 @@ __global__ __launch_bounds__(kBlock) void generate_batch_kernel(BatchView bv, ui
                                                                 const float* __restrict__ eps,
                                                                 PathT<NL>* __restrict__ paths, ap_ray* __restrict__ rays,
                                                                 int32_t* __restrict__ path_of,
-                                                                MediaRec* __restrict__ media) {
+                                                                MediaRec* __restrict__ media,
+                                                                GuideRec* __restrict__ recs = nullptr) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= total) return;
     const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
@@ -1483,24 +1494,33 @@ __global__ __launch_bounds__(kBlock) void generate_batch_kernel(BatchView bv, ui
     st.rng_state = rng.state;
     st.rng_inc = rng.inc;
     paths[i] = st;
+    if (recs) {
+        GuideRec z;
+        __builtin_memset(&z, 0, sizeof(z));
+        recs[i] = z;
+    }
     rays[i] = ap_ray{origin.x, origin.y, origin.z, eps[0], d.x, d.y, d.z, __int_as_float(0x7f800000)};
     path_of[i] = static_cast<int32_t>(i);
     bv.ray_scene[0][i] = static_cast<uint16_t>(sc);
 }
 
 // accumulate_kernel for one frame of a batch, into each scene's framebuffers
-// (skipped for a scene whose hdr is NULL).
+// (skipped for a scene whose hdr is NULL); with the path records of a guided
+// batch, each path's guiding samples go to its scene's slots.
 template <int NL>
 __global__ __launch_bounds__(kBlock) void accumulate_batch_kernel(BatchView bv, uint32_t total,
                                                                   const PathT<NL>* __restrict__ paths,
-                                                                  const float* __restrict__ tables) {
+                                                                  const float* __restrict__ tables,
+                                                                  const GuideRec* __restrict__ recs = nullptr) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= total) return;
     const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
     const BatchScene& B = bv.scenes[sc];
-    if (!B.hdr) return;
     const uint32_t p = i - B.pix_begin;
     const PathT<NL>& S = paths[i];
+    if (recs && B.guide_slots)
+        pg_emit<NL>(recs[i], S.lam, S.psum, tables, B.guide_slots + static_cast<size_t>(p) * PG_COLLECT);
+    if (!B.hdr) return;
     float rad[NL];
     for (int j = 0; j < NL; ++j) rad[j] = fix_nan_inf(S.radiance[j]);
     const f3 pre = path_xyz<NL>(S.lam, rad, S.psum, tables);
@@ -2307,6 +2327,7 @@ struct BatchState {
     float* nrc_pool = nullptr;
     uint64_t nrc_last_hash = 0;
     bool nrc_rendered = false;
+    GuideRec* guide_recs = nullptr;     // guided batches: each path's collection record
     // Side stream of the overlapped shadow traces (as RenderState's).
     hipStream_t side = nullptr;
     hipEvent_t ev_shaded = nullptr, ev_shadowed = nullptr;
@@ -2327,7 +2348,7 @@ struct BatchState {
                         shadow_rays, shadow_payload, shadow_hits, ray_scene[0], ray_scene[1], shadow_scene, counters,
                         frame, rays_traced, eps, nrc_table, nrc_paths, nrc_queries, nrc_qpath, nrc_qrgb,
                         nrc_counters, nrc_qcount, nrc_tile0, nrc_pool_begin, nrc_pool_n,
-                        nrc_verts, nrc_vmask, nrc_offsets, nrc_slot_pix, nrc_pool};
+                        nrc_verts, nrc_vmask, nrc_offsets, nrc_slot_pix, nrc_pool, guide_recs};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         if (ev_shaded) (void)hipEventDestroy(ev_shaded);
@@ -2345,7 +2366,7 @@ namespace {
 // Shading records of every scene for the call's triangle-material mode and
 // the batch records with this call's framebuffers, uploaded.
 int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const* sums, const f4* rgb_lut,
-                  hipStream_t stream) {
+                  hipStream_t stream, bool guiding = false) {
     int rc;
     // The scene records are read by the frames of the previous call, which
     // may still run on its stream: wait for them before rewriting the records.
@@ -2385,28 +2406,61 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
             return fail(AP_ERR_INVALID_ARG, "ap_batch: a material names a texture the scene's texture cache does not hold");
         B.textures = texture_cache_table(s->tex_cache);
         B.uvs = r->uvs;
+        B.guide = GuideView{};
+        B.guide_slots = nullptr;
+        if (guiding) {
+            if ((rc = guide_prepare(&s->guide, b->pix_begin[k + 1] - b->pix_begin[k])) != AP_OK) return rc;
+            B.guide = guide_view(s->guide);
+            B.guide_slots = guide_slots(s->guide);
+        }
         b->h_devs[k] = s->dev;
         if (s->dev.has_sdf) b->shape.has_sdf = 1;
     }
+    if (guiding && !b->guide_recs && (rc = dev_alloc(&b->guide_recs, b->total)) != AP_OK) return rc;
     e = hipMemcpy(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(b->d_devs, b->h_devs.data(), b->h_devs.size() * sizeof(DeviceScene), hipMemcpyHostToDevice);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: upload scene records");
 }
 
+// After a guided batch frame: every scene's probe build from its own slots
+// (ap_render's per-frame update; the scenes' first halves share one wait),
+// then the scene records with the rebuilt probes and index volumes for the
+// next frame.
+int batch_guides_update(BatchState* b, hipStream_t stream) {
+    int rc;
+    for (size_t k = 0; k < b->scenes.size(); ++k)
+        if ((rc = guide_update_begin(b->scenes[k]->guide, b->pix_begin[k + 1] - b->pix_begin[k], stream)) != AP_OK)
+            return rc;
+    hipError_t e = hipStreamSynchronize(stream);  // one wait for every scene's counts
+    if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding counts");
+    for (size_t k = 0; k < b->scenes.size(); ++k)
+        if ((rc = guide_update_end(b->scenes[k]->guide, stream)) != AP_OK) return rc;
+    for (size_t k = 0; k < b->scenes.size(); ++k) {
+        b->h_scenes[k].guide = guide_view(b->scenes[k]->guide);
+        b->h_scenes[k].guide_slots = guide_slots(b->scenes[k]->guide);
+    }
+    e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
+                       hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: guiding records");
+}
+
 // One batch frame: every scene's paths through every bounce, then the
 // running means and the frame counter. Nothing here synchronises, so the
 // sequence can be captured into a graph.
-template <int NL>
+template <int NL, bool GUIDE = false>
 int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, const f4* rgb_lut,
                 const f4* lam_lut, const float* tables) {
     const BatchView bv = b->view();
     auto* paths = reinterpret_cast<PathT<NL>*>(b->paths);
     auto* pays = reinterpret_cast<ShadowPay<NL>*>(b->shadow_payload);
     const uint32_t total = b->total;
+    GuideView gv{};
+    if constexpr (GUIDE) gv.recs = b->guide_recs;
     hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
     hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
-                       b->eps, paths, b->rays[0], b->path_of[0], b->media);
+                       b->eps, paths, b->rays[0], b->path_of[0], b->media, gv.recs);
     // Shadow rays of bounce k on the side stream, overlapped with bounce
     // k + 1's extension rays; joined before resolve (as in run_frames).
     int in_q = 0;
@@ -2421,10 +2475,10 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
         ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                        prm->nee_always, prm->sky, in_q, out_q, 0};
         sp.media = b->media;
-        hipLaunchKernelGGL((shade_kernel<NL, false, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
+        hipLaunchKernelGGL((shade_kernel<NL, GUIDE, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
                            nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
-                           b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, NrcView{}, bv);
+                           b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, gv, NrcView{}, bv);
         e = hipEventRecord(b->ev_shaded, stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_shaded, 0);
         if (e == hipSuccess)
@@ -2445,7 +2499,7 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
         if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, 1 - in_q);
     }
     hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
-                       tables);
+                       tables, gv.recs);
     hipLaunchKernelGGL(advance_frame_kernel, dim3(1), dim3(64), 0, stream, b->frame);
     e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: launch");
@@ -2474,6 +2528,17 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "ap_batch: frame counter");
     const bool four = prm->wavelengths == 4;
+    if (prm->use_guiding) {
+        // One frame at a time, each sampling the probes the frames before it
+        // built: launched directly (each probe build waits once per scene).
+        for (int32_t f = 0; f < frames; ++f) {
+            rc = four ? batch_frame<4, true>(b, prm, stream, rgb_lut, lam_lut, tables)
+                      : batch_frame<1, true>(b, prm, stream, rgb_lut, lam_lut, tables);
+            if (rc == AP_OK) rc = batch_guides_update(b, stream);
+            if (rc != AP_OK) return rc;
+        }
+        return AP_OK;
+    }
     auto frame = [&]() {
         return four ? batch_frame<4>(b, prm, stream, rgb_lut, lam_lut, tables)
                     : batch_frame<1>(b, prm, stream, rgb_lut, lam_lut, tables);
@@ -2690,7 +2755,6 @@ int batch_check_params(const ap_render_params* prm) {
         return fail(AP_ERR_INVALID_ARG, "ap_batch: bad triangle_material");
     if (prm->wavelengths != 0 && prm->wavelengths != 1 && prm->wavelengths != 4)
         return fail(AP_ERR_INVALID_ARG, "ap_batch: wavelengths must be 1 or 4 (0 reads as 1)");
-    if (prm->use_guiding) return fail(AP_ERR_UNSUPPORTED, "ap_batch: guiding is not supported in a batch");
     return AP_OK;
 }
 
@@ -2911,7 +2975,8 @@ int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* cons
     const float* tables = nullptr;
     if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut, st)) != AP_OK) return rc;
+    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut, st, params->use_guiding != 0)) != AP_OK)
+        return rc;
     hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);
     if (e != hipSuccess) return hip_fail(e, "ap_batch_render: reset counters");
     if ((rc = batch_run(&B, params, params->frame_index, params->spp, st)) != AP_OK) return rc;
@@ -2940,6 +3005,8 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
         return fail(AP_ERR_INVALID_ARG, "ap_batch_render_nrc: spread must be > 0 and learning_rate >= 0");
     if (nrc_params->train_passes != 0)
         return fail(AP_ERR_UNSUPPORTED, "ap_batch_render_nrc: training-only passes are not supported in a batch");
+    if (params->use_guiding)
+        return fail(AP_ERR_UNSUPPORTED, "ap_batch_render_nrc: guiding with per-scene caches is not supported");
     if (nrc_params->train_steps > 0 &&
         (nrc_params->batch == 0 || nrc_params->batch % 256 != 0 ||
          static_cast<uint64_t>(nrc_params->batch) * B.scenes.size() > (1u << 24)))
@@ -3072,6 +3139,10 @@ int ap_batch_autotune(ap_batch* batch, const ap_render_params* params, int frame
     if (!batch || !params || frames <= 0) return fail(AP_ERR_INVALID_ARG, "ap_batch_autotune: bad argument");
     int rc = batch_check_params(params);
     if (rc != AP_OK) return rc;
+    // Launch shapes are timed on unguided frames (probes stay untouched).
+    ap_render_params unguided = *params;
+    unguided.use_guiding = 0;
+    params = &unguided;
     BatchState& B = batch->s;
     const f4 *rgb_lut = nullptr, *lam_lut = nullptr;
     const float* tables = nullptr;

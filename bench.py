@@ -633,11 +633,11 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
     pix = sum(c.width * c.height for c in cams)
     e0, e1 = ap.Event(), ap.Event()
 
-    def timed(fn):
-        fn(ap.RenderParams.defaults(spp=1, frame_index=0, **params))  # untimed (graph capture, allocation)
+    def timed(fn, **extra):
+        fn(ap.RenderParams.defaults(spp=1, frame_index=0, **params, **extra))  # untimed (graph capture, allocation)
         stream.synchronize()
         e0.record(stream)
-        out = fn(ap.RenderParams.defaults(spp=spp, frame_index=1, **params))
+        out = fn(ap.RenderParams.defaults(spp=spp, frame_index=1, **params, **extra))
         e1.record(stream)
         stream.synchronize()
         return e0.elapsed_ms(e1), out
@@ -651,6 +651,9 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
             sc.render(c, p, h, stream=stream)
 
     ms_seq, _ = timed(one_by_one)
+    # Guided: every scene sampling and rebuilding its own probes each frame
+    # (frames launched one by one, one probe build per scene and frame).
+    ms_guided, _ = timed(lambda p: batch.render(p, hdrs, stream=stream), use_guiding=1)
     batch.close()
     for sc in scenes:
         sc.close()
@@ -667,6 +670,7 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
         "direct_launches": {"ms": round(ms_direct, 2), "samples_per_s": round(pix * spp / (ms_direct / 1e3), 1)},
         "scene_by_scene": {"ms": round(ms_seq, 2), "samples_per_s": round(pix * spp / (ms_seq / 1e3), 1)},
         "speedup_over_scene_by_scene": round(ms_seq / ms_graph, 3),
+        "guided": {"ms": round(ms_guided, 2), "samples_per_s": round(pix * spp / (ms_guided / 1e3), 1)},
     }
     log(f"batch: {out}")
     return out

@@ -36,10 +36,15 @@ int ap_batch_destroy(ap_batch* batch);
  * scene: hdr[k] (W_k*H_k*3 linear XYZ running mean, required unless NULL to
  * skip the scene's image) and, if sums is not NULL and sums[k] is not NULL,
  * FP64 per-pixel sums. The shading, traversal and spectral parameters are
- * ap_render's; guiding is not supported (AP_ERR_UNSUPPORTED), odd/even and
- * primary-id buffers are not produced. ray_count (host, may be NULL) gets
- * the closest-hit and shadow rays traced. Asynchronous on `stream` except
- * for the first call with new parameters, which captures the graph. */
+ * ap_render's; odd/even and primary-id buffers are not produced. With
+ * use_guiding, each scene is guided by its own probes (the scene's guiding
+ * state, shared with ap_render of that scene) and every frame rebuilds every
+ * scene's probes from that scene's samples, so scene k's image and probes
+ * equal ap_render's of scene k alone over the same frames; guided frames are
+ * launched one by one and the call waits once per scene and frame (the
+ * probe build). ray_count (host, may be NULL) gets the closest-hit and
+ * shadow rays traced. Otherwise asynchronous on `stream` except for the
+ * first call with new parameters, which captures the graph. */
 int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* const* hdr, double* const* sums,
                     uint64_t ray_count[2], void* stream);
 
@@ -53,8 +58,10 @@ int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* cons
  * batch rows per network, batch x n at most 2^24) in the same launches. With
  * a width-64 BF16 cache, scene k's image and network k's weights equal, bit
  * for bit, ap_render_nrc of scene k alone with a one-network cache that
- * starts from network k. train_passes must be 0. Frames are launched one by
- * one (no graph). */
+ * starts from network k. train_passes must be 0 and use_guiding 0. The
+ * call's first frame is launched directly, the others replay one captured
+ * graph (the frame index, the frame's training draws and Adam's bias
+ * corrections are read on the device). */
 int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc* cache,
                         const ap_nrc_render_params* nrc_params, float* const* hdr, double* const* sums,
                         uint64_t ray_count[2], void* stream);
@@ -71,7 +78,8 @@ int ap_batch_set_graph(ap_batch* batch, int enable);
 
 /* Renders one untimed and `frames` timed batch frames (into no image) for
  * each launch shape of {128, 256, 512} rays per wave x {8, 16} refill, and
- * keeps the fastest; reports it and its time per frame. */
+ * keeps the fastest; reports it and its time per frame. Frames are timed
+ * unguided (use_guiding is ignored, so no scene's probes change). */
 int ap_batch_autotune(ap_batch* batch, const ap_render_params* params, int frames, int* best_wave_rays,
                       int* best_refill, float* best_ms_per_frame, void* stream);
 

@@ -11,7 +11,8 @@ per path, and after the autotuner has picked a launch shape. Calls issued
 back to back on a non-blocking stream, with no host wait between them
 (progressive frames, new framebuffers, an autotune), keep their order.
 The batch is also compared directly with the oracle's CalculateRadiance per
-scene (bit-identical images and FP64 sums), and one frame of the full
+scene (bit-identical images and FP64 sums); a guided batch equals each
+scene's guided render alone (images, sums and probes); and one frame of the full
 48-scene workload (ap.synth_batch_scenes: 7.4 M triangles, 13.7 M pixels)
 equals the scene-by-scene renders for every scene, its smallest scene the
 oracle too.
@@ -96,9 +97,14 @@ def test_autotuner_keeps_images(gpu):
     check_against_single(scenes, cams, prm, got, "autotuned")
     with pytest.raises(ap.ApertureError):
         batch.set_launch(100, 8)
+    # Guiding with per-scene caches is refused (a guided plain batch is
+    # tested below).
+    cache = ap.RadianceCache(seed=1, groups=len(cams), width=32, precision="fp16")
     with pytest.raises(ap.ApertureError) as e:
-        batch.render(ap.RenderParams.defaults(use_guiding=1), [None] * len(cams), stream=stream)
+        batch.render_nrc(ap.RenderParams.defaults(use_guiding=1), cache, ap.NrcRenderParams.defaults(),
+                         [None] * len(cams), stream=stream)
     assert e.value.code == ap.ERR_UNSUPPORTED
+    cache.close()
     batch.close()
 
 
@@ -171,6 +177,54 @@ def test_batch_matches_the_oracle(gpu, wavelengths):
               f"pixels bit-identical {ident:.4f}", flush=True)
         assert np.array_equal(got[k][0], hdr) and np.array_equal(got[k][1], sums), k
     batch.close()
+
+
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_guided_batch_equals_guided_scenes(gpu, wavelengths):
+    """Guiding in a batch: every scene's paths sample its own probes, each
+    frame's samples go to its own slots and rebuild its own probes. Three
+    calls of one frame each (the first from empty probes) must leave every
+    scene's image, FP64 sums, probes, probe counts and index volume equal,
+    bit for bit, to ap_render of the scene alone over the same frames (which
+    tests/test_guiding_gpu.py ties to the oracle)."""
+    objs, cams = scenes_and_cameras()
+    stream = ap.Stream()
+    kw = dict(spp=1, max_depth=10, rr_min_depth=3, seed=12, wavelengths=wavelengths, use_guiding=1)
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    batch = ap.Batch(scenes, cams)
+    hdrs, sums = [], []
+    for c in cams:
+        n = c.width * c.height
+        h = ap.DeviceBuffer(n * 12)
+        h.zero()
+        hdrs.append(h)
+        sums.append(ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64)))
+    for f in range(48, 51):
+        batch.render(ap.RenderParams.defaults(frame_index=f, **kw), hdrs, sums, stream=stream)
+    stream.synchronize()
+    for k, (o, c) in enumerate(zip(objs, cams)):
+        n = c.width * c.height
+        alone = ap.SceneRenderer(o)
+        h = ap.DeviceBuffer(n * 12)
+        h.zero()
+        sm = ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64))
+        for f in range(48, 51):
+            alone.render(c, ap.RenderParams.defaults(frame_index=f, **kw), h, sum_xyz=sm)
+        ap.synchronize()
+        got = hdrs[k].to_numpy(np.float32, n * 3), sums[k].to_numpy(np.float64, n * 3)
+        want = h.to_numpy(np.float32, n * 3), sm.to_numpy(np.float64, n * 3)
+        gdata, gcount, gvol, gbounds = scenes[k].guiding_export()
+        odata, ocount, ovol, obounds = alone.guiding_export()
+        print(f"[batch] guided scene {k} {c.width}x{c.height}: {len(gcount)} probes (alone {len(ocount)}), "
+              f"pixels bit-identical {float(np.mean(got[0] == want[0])):.4f}", flush=True)
+        assert len(gcount) > 0
+        assert np.array_equal(gbounds, obounds) and np.array_equal(gvol, ovol) and np.array_equal(gcount, ocount)
+        assert np.array_equal(gdata.view(np.uint32), odata.view(np.uint32))
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        alone.close()
+    batch.close()
+    for sc in scenes:
+        sc.close()
 
 
 def test_full_size_batch_workload_one_frame(gpu):
