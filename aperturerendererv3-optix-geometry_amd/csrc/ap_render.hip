@@ -1992,10 +1992,12 @@ int run_frames(const RunCtx& c) {
     auto* paths = static_cast<PathT<NL>*>(r->paths);
     auto* pays = static_cast<ShadowPay<NL>*>(r->shadow_payload);
     int rc = AP_OK;
-    // The path queues are traced at most kQueueWaveRays rays per wave: their
-    // chunks go round-robin over the XCDs, where 128-ray chunks measured
-    // best at 1080p (8.4 against 9.0 ms per sample at 512).
-    constexpr int32_t kQueueWaveRays = 128;
+    // The path queues are traced at most kQueueWaveRays rays per wave, their
+    // chunks dealt round-robin over the XCDs. Measured at 1080p with four
+    // wavelengths: with 16M paths per batch, 512-ray chunks give 4.65 ms per
+    // sample (128: 4.98, 1024: 4.65, 2048: 4.72) and guided frames (2M paths)
+    // 10.0 ms (128: 10.2); with 4M paths per batch 128 had been best.
+    constexpr int32_t kQueueWaveRays = 512;
     DeviceScene qs = s->dev;
     qs.wave_rays = std::min(qs.wave_rays, kQueueWaveRays);
     // One batch of paths: a rendered batch of frames, or (train_only) the
