@@ -147,6 +147,18 @@ __device__ __forceinline__ bool positive(uint16_t h) { return (h & 0x8000u) == 0
 
 __device__ __forceinline__ u16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const u16x8*>(p); }
 
+// Orders one wave's LDS accesses: a wave's LDS operations complete in issue
+// order, so a tile that only its own wave reads and writes needs no block
+// barrier, only this fence against compiler reordering.
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
+    return static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool F16>
 __device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 c) {
     if constexpr (F16)
@@ -203,10 +215,11 @@ __device__ __forceinline__ uint32_t tile_count(const GroupMap& gm, uint32_t n) {
     return gm.tile0[gm.groups];
 }
 
-// Features [16q, 16q + 16) of the encoding of one query record (the
-// oracle's or_nrc_encode): sin/cos pairs of the normalised position at
-// frequencies 2^k pi for features 0..35, then direction, normal, roughness,
-// albedo and a constant 1, zeros from 47.
+// The encoding of one query record (the oracle's or_nrc_encode), feature
+// pair (2m, 2m + 1): for m < 18 the sine and cosine of the normalised
+// position coordinate a = m / 6 at frequency 2^(m mod 6) pi; above, the
+// direction (36..38), normal (39..41), roughness (42), albedo (43..45), a
+// constant 1 (46) and zeros. p[a] = (s[a] - lo[a]) / (hi[a] - lo[a]).
 __device__ __forceinline__ float plain_feature(const float* s, int k) {
     if (k < 39) return s[3 + (k - 36)];
     if (k < 42) return s[6 + (k - 39)];
@@ -214,22 +227,14 @@ __device__ __forceinline__ float plain_feature(const float* s, int k) {
     if (k < 46) return s[10 + (k - 43)];
     return k == 46 ? 1.0f : 0.0f;
 }
-__device__ void encode_part(const float* s, const Bounds& b, int q, float f[16]) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int k = 16 * q + 2 * j;
-        float v0, v1;
-        if (k < 36) {
-            const int a = k / 12, fr = (k % 12) / 2;
-            const float p = (s[a] - b.lo[a]) / (b.hi[a] - b.lo[a]);
-            const float arg = p * (static_cast<float>(1 << fr) * kPiF);
-            apfm::sincos(arg, v0, v1);
-        } else {
-            v0 = plain_feature(s, k);
-            v1 = plain_feature(s, k + 1);
-        }
-        f[2 * j] = v0;
-        f[2 * j + 1] = v1;
+__device__ __forceinline__ void encode_pair(const float* s, const float p[3], int m, float& v0, float& v1) {
+    if (m < 18) {
+        const int a = m / 6, fr = m % 6;
+        const float pa = a == 0 ? p[0] : (a == 1 ? p[1] : p[2]);
+        apfm::sincos(pa * (static_cast<float>(1 << fr) * kPiF), v0, v1);
+    } else {
+        v0 = plain_feature(s, 2 * m);
+        v1 = plain_feature(s, 2 * m + 1);
     }
 }
 
@@ -278,20 +283,27 @@ __global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict
         const uint32_t r = begin + static_cast<uint32_t>(wave) * 16u + static_cast<uint32_t>(row);
         const bool live = r < end;
         {
-            float f[16];
-            if (live) {
-                const size_t src = rows ? static_cast<size_t>(rows[r]) : static_cast<size_t>(r);
-                encode_part(rec + src * stride, b, q, f);
-            } else {
-                for (int k = 0; k < 16; ++k) f[k] = 0.0f;
-            }
-            for (int k = 0; k < 16; ++k) {
-                const uint16_t h = cvt<F16>(f[k]);
-                act[row * LS + 16 * q + k] = h;
-                if (TRAIN && live) acts[static_cast<size_t>(16 * q + k) * n_cap + r] = h;
+            // The four lanes of a row share its 18 sine/cosine pairs: lane q
+            // encodes pairs q, q + 4, ..., q + 28.
+            const size_t src = !live ? 0u : rows ? static_cast<size_t>(rows[r]) : static_cast<size_t>(r);
+            const float* s = rec + src * stride;
+            float p[3] = {0.0f, 0.0f, 0.0f};
+            if (live)
+                for (int a = 0; a < 3; ++a) p[a] = (s[a] - b.lo[a]) / (b.hi[a] - b.lo[a]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int m = 4 * j + q;
+                float v0 = 0.0f, v1 = 0.0f;
+                if (live) encode_pair(s, p, m, v0, v1);
+                const uint16_t lo = cvt<F16>(v0), hi = cvt<F16>(v1);
+                *reinterpret_cast<uint32_t*>(&act[row * LS + 2 * m]) = pack2(lo, hi);
+                if (TRAIN && live) {
+                    acts[static_cast<size_t>(2 * m) * n_cap + r] = lo;
+                    acts[static_cast<size_t>(2 * m + 1) * n_cap + r] = hi;
+                }
             }
         }
-        __syncthreads();
+        wave_sync();
         for (int l = 0; l < NL; ++l) {
             f32x4 acc[W / 16];
             const int ks = (l == 0 ? NF : W) / 32;
@@ -303,16 +315,19 @@ __global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict
                     acc[t] = mfma<F16>(a, bb, acc[t]);
                 }
             }
-            __syncthreads();
-            for (int t = 0; t < W / 16; ++t)
+            wave_sync();
+#pragma unroll
+            for (int t = 0; t < W / 16; ++t) {
+                uint16_t h[4];
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const int o = 16 * t + 4 * q + k;
                     const float x = acc[t][k] > 0.0f ? acc[t][k] : 0.0f;
-                    const uint16_t h = cvt<F16>(x);
-                    act[row * LS + o] = h;
-                    if (TRAIN && live) acts[static_cast<size_t>(G::act(l + 1) + o) * n_cap + r] = h;
+                    h[k] = cvt<F16>(x);
+                    if (TRAIN && live) acts[static_cast<size_t>(G::act(l + 1) + 16 * t + 4 * q + k) * n_cap + r] = h[k];
                 }
-            __syncthreads();
+                *reinterpret_cast<uint2*>(&act[row * LS + 16 * t + 4 * q]) = make_uint2(pack2(h[0], h[1]), pack2(h[2], h[3]));
+            }
+            wave_sync();
         }
         f32x4 y{0.0f, 0.0f, 0.0f, 0.0f};
         for (int s = 0; s < W / 32; ++s) {
@@ -325,7 +340,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const uint16_t* __restrict
             out[static_cast<size_t>(r) * NOUT + 1] = y[1];
             out[static_cast<size_t>(r) * NOUT + 2] = y[2];
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -413,32 +428,42 @@ __global__ __launch_bounds__(256) void backward_kernel(const uint16_t* __restric
         const uint32_t r = base + static_cast<uint32_t>(wave) * 16u + static_cast<uint32_t>(row);
         const bool live = r < n;
         // Output deltas as [row][o < 32] (o >= 16 zero).
-        for (int k = 0; k < 8; ++k) {
-            const int o = 8 * q + k;
-            del[row * LS + o] = (o < NOP && live) ? dout[static_cast<size_t>(o) * n_cap + r] : uint16_t{0};
+        {
+            uint16_t d[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int o = 8 * q + k;
+                d[k] = (o < NOP && live) ? dout[static_cast<size_t>(o) * n_cap + r] : uint16_t{0};
+            }
+            *reinterpret_cast<uint4*>(&del[row * LS + 8 * q]) =
+                make_uint4(pack2(d[0], d[1]), pack2(d[2], d[3]), pack2(d[4], d[5]), pack2(d[6], d[7]));
         }
-        __syncthreads();
+        wave_sync();
         f32x4 acc[W / 16];
         for (int t = 0; t < W / 16; ++t) {
             const u16x8 a = ld8(&swt[OUT_ROW0 + (16 * t + row) * 40 + 8 * q]);
             const u16x8 b = ld8(&del[row * LS + 8 * q]);
             acc[t] = mfma<F16>(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
         }
-        __syncthreads();
+        wave_sync();
         for (int h = NL - 1; h >= 0; --h) {
             // acc holds W_{h+1}^T delta_{h+1} (W_out for h = 4) at i = 16t + 4q + k.
-            for (int t = 0; t < W / 16; ++t)
+#pragma unroll
+            for (int t = 0; t < W / 16; ++t) {
+                uint16_t hv[4] = {0, 0, 0, 0};
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int i = 16 * t + 4 * q + k;
-                    uint16_t hv = 0;
                     if (live) {
                         const uint16_t a = acts[static_cast<size_t>(G::act(h + 1) + i) * n_cap + r];
-                        hv = positive(a) ? cvt<F16>(acc[t][k]) : uint16_t{0};
-                        deltas[static_cast<size_t>(h * W + i) * n_cap + r] = hv;
+                        hv[k] = positive(a) ? cvt<F16>(acc[t][k]) : uint16_t{0};
+                        deltas[static_cast<size_t>(h * W + i) * n_cap + r] = hv[k];
                     }
-                    del[row * LS + i] = hv;
                 }
-            __syncthreads();
+                *reinterpret_cast<uint2*>(&del[row * LS + 16 * t + 4 * q]) =
+                    make_uint2(pack2(hv[0], hv[1]), pack2(hv[2], hv[3]));
+            }
+            wave_sync();
             if (h == 0) break;
             for (int t = 0; t < W / 16; ++t) {
                 acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -448,7 +473,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const uint16_t* __restric
                     acc[t] = mfma<F16>(a, b, acc[t]);
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 }
