@@ -335,9 +335,16 @@ AP_HD const ap_octahedron* pg_query_probe(const GuideView& g, f3 pos, f3 dir) {
 
 // Per-frame build (ap_guiding.hip).
 struct GuideDevice;
-int guide_prepare(GuideDevice** g, size_t n_paths);
+// Path records for `sets` frame sets (1, or 2 for frames in flight two at a time).
+int guide_prepare(GuideDevice** g, size_t n_paths, int sets = 1);
+// Probes and index volume, with frame set 0's path records.
 GuideView guide_view(GuideDevice* g);
+// The path records of frame set 0 or 1.
+GuideRec* guide_recs(GuideDevice* g, int set);
 ap_path_guiding_sample* guide_slots(GuideDevice* g);
+// Per path, the number of valid samples at the front of its slots (written
+// with the slots by the accumulate kernels).
+uint32_t* guide_nvalid(GuideDevice* g);
 int guide_update(GuideDevice* g, size_t n_paths, hipStream_t stream);
 // guide_update in two halves around its one wait, so that several scenes'
 // builds can share a wait: begin enqueues up to the read-back of the counts

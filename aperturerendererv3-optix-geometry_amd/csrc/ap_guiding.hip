@@ -2,13 +2,16 @@
 // core/scene.cu:583-744, over core/guiding/guiding.cuh:70-517).
 //
 // Input: the frame's guiding samples, PG_COLLECT slots per path in pixel
-// order (written by the accumulate kernel). Steps, all on the render stream:
-//   0. flag the valid slots and compact them in slot order (an exclusive
-//      scan of the flags);
-//   1. key every valid slot by its voxel (x, y, z, face) packed so that the
-//      unsigned order is the reference's Int4Less order; radix-sort
-//      (key, slot) pairs; mark run heads and scan them: the unique voxels in
-//      sorted order (thrust sort + unique in the reference);
+// order (written by the accumulate kernel, each path's valid samples packed
+// to the front of its slots) and each path's number of valid samples. Steps,
+// all on the render stream:
+//   0. an exclusive scan of the per-path counts gives the valid slots in
+//      slot order, and their voxel bounds;
+//   1. key every valid slot by its voxel (x, y, z, face) relative to those
+//      bounds, so that the unsigned order is the reference's Int4Less order
+//      and the key fits 32 bits (absolute 62-bit keys when it does not);
+//      radix-sort (key, slot) pairs; mark run heads and scan them: the unique
+//      voxels in sorted order (thrust sort + unique in the reference);
 //   2. grow the index volume to the union of its bounds and the voxels'
 //      (CopyOldIntoNew), then number the voxels without a probe in sorted
 //      order and append zeroed probes for them (FilterUnallocatedGrids +
@@ -47,8 +50,12 @@ struct GuideDevice {
     int32_t start[3] = {0, 0, 0}, end[3] = {0, 0, 0}, size[3] = {0, 0, 0};
     GuideRec* recs = nullptr;
     size_t rec_cap = 0;
+    GuideRec* recs2 = nullptr;    // the second frame set's path records (pipelined frames)
+    size_t rec2_cap = 0;
     ap_path_guiding_sample* slots = nullptr;
     size_t slot_cap = 0;
+    uint32_t* nvalid = nullptr;   // per path: its valid samples (packed to the front of its slots)
+    size_t nvalid_cap = 0;
     uint64_t* keys[2] = {nullptr, nullptr};
     uint32_t* vals[2] = {nullptr, nullptr};
     uint32_t* flags = nullptr;
@@ -59,7 +66,7 @@ struct GuideDevice {
     size_t tmp_bytes = 0;
     size_t sort_cap = 0;
     int32_t* info = nullptr;      // device: [0] unique count, [1..3] min xyz, [4..6] max xyz, [7] new probes, [8] error,
-                                  // [9] valid slots
+                                  // [9] valid slots, [10] voxel range too wide for 32-bit keys
     int32_t* info_host = nullptr; // pinned
     // Launch size of the voxel sort: a bound on the frame's valid slots from
     // the frames before it; a frame with more sorts again at its real count.
@@ -72,6 +79,7 @@ struct GuideDevice {
     // count and voxel-sort launch size.
     bool open = false;
     uint32_t upd_n = 0, upd_cap = 0;
+    bool wide = false;            // the update's voxel keys are absolute 62-bit keys
 };
 
 int settle(GuideDevice* g);
@@ -112,41 +120,108 @@ struct VolumeView {
     }
 };
 
-// Valid slots (FilterValidSamples, guiding.cuh:70-99); a valid slot whose
-// voxel is outside the packable range is reported in info[8].
-__global__ void valid_slots_kernel(const ap_path_guiding_sample* __restrict__ slots, uint32_t n,
-                                   uint32_t* __restrict__ flags, int32_t* __restrict__ info) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const ap_path_guiding_sample s = slots[i];
-    uint32_t f = 0;
-    if (s.valid) {
-        const ap_int4 v = s.voxel_grid;
-        if (in_range(v.x) && in_range(v.y) && in_range(v.z) && v.w >= 0 && v.w < PG_FACES) f = 1;
-        else atomicOr(&info[8], 1);
-    }
-    flags[i] = f;
+// Wave-wide minimum / maximum (every lane of the wave takes part).
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
 }
-__global__ void compact_slots_kernel(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ ranks,
-                                     uint32_t n, uint32_t* __restrict__ list, int32_t* __restrict__ info) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    if (i == n - 1) info[9] = static_cast<int32_t>(ranks[i] + flags[i]);
-    if (flags[i]) list[ranks[i]] = i;
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// The valid slots in slot order (FilterValidSamples, guiding.cuh:70-99):
+// path p's nvalid[p] samples sit at the front of its PG_COLLECT slots and
+// start at offs[p] in the list. Also their count (info[9]) and voxel bounds
+// (info[1..6]); a valid slot whose voxel is outside the packable range is
+// reported in info[8]. Grid-stride over the paths: the bounds are reduced
+// over the block, then six atomics per block.
+constexpr unsigned kCompactBlocks = 1024;
+__global__ __launch_bounds__(kBlock) void compact_valid_kernel(const ap_path_guiding_sample* __restrict__ slots,
+                                                               const uint32_t* __restrict__ nvalid,
+                                                               const uint32_t* __restrict__ offs, uint32_t n_paths,
+                                                               uint32_t* __restrict__ list,
+                                                               int32_t* __restrict__ info) {
+    __shared__ int32_t red[6][kBlock / 64];
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    bool bad = false;
+    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < n_paths; p += gridDim.x * kBlock) {
+        const uint32_t c = min(nvalid[p], static_cast<uint32_t>(PG_COLLECT));
+        const uint32_t o = offs[p];
+        if (p == n_paths - 1) info[9] = static_cast<int32_t>(o + c);
+        for (uint32_t k = 0; k < c; ++k) {
+            const uint32_t slot = p * PG_COLLECT + k;
+            const ap_int4 v = slots[slot].voxel_grid;
+            list[o + k] = slot;
+            if (in_range(v.x) && in_range(v.y) && in_range(v.z) && v.w >= 0 && v.w < PG_FACES) {
+                lo[0] = min(lo[0], v.x); lo[1] = min(lo[1], v.y); lo[2] = min(lo[2], v.z);
+                hi[0] = max(hi[0], v.x); hi[1] = max(hi[1], v.y); hi[2] = max(hi[2], v.z);
+            } else {
+                bad = true;
+            }
+        }
+    }
+    if (bad) atomicOr(&info[8], 1);
+    const int w = threadIdx.x >> 6;
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min(lo[a]);
+        hi[a] = wave_max(hi[a]);
+        if ((threadIdx.x & 63) == 0) {
+            red[a][w] = lo[a];
+            red[3 + a][w] = hi[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k)
+            for (int a = 0; a < 3; ++a) {
+                red[a][0] = min(red[a][0], red[a][k]);
+                red[3 + a][0] = max(red[3 + a][0], red[3 + a][k]);
+            }
+        if (red[0][0] != INT32_MAX) {
+            atomicMin(&info[1], red[0][0]);
+            atomicMin(&info[2], red[1][0]);
+            atomicMin(&info[3], red[2][0]);
+            atomicMax(&info[4], red[3][0]);
+            atomicMax(&info[5], red[4][0]);
+            atomicMax(&info[6], red[5][0]);
+        }
+    }
 }
 // Item count of a launch sized for m: min(*dyn, m) (the valid slots).
 __device__ __forceinline__ uint32_t live_count(uint32_t m, const int32_t* dyn) {
     return min(static_cast<uint32_t>(*dyn), m);
 }
 
-// (voxel, face) key of each valid slot, the slot index as value.
+// (voxel, face) key of each valid slot, the slot index as value. Narrow
+// keys are relative to the frame's voxel bounds (info[1..6]):
+// ((x - x0) * ny + (y - y0)) * nz + (z - z0)) * 32 + face, the Int4Less order
+// in 32 bits; a range too wide for that sets info[10] (the build then sorts
+// the absolute keys of pack_voxel). Wide keys are pack_voxel's.
+template <bool WIDE>
 __global__ void voxel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots,
-                                  const uint32_t* __restrict__ list, uint32_t m, const int32_t* __restrict__ dyn,
+                                  const uint32_t* __restrict__ list, uint32_t m, int32_t* __restrict__ info,
                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= live_count(m, dyn)) return;
+    if (i >= live_count(m, &info[9])) return;
     const uint32_t j = list[i];
-    keys[i] = pack_voxel(slots[j].voxel_grid);
+    const ap_int4 v = slots[j].voxel_grid;
+    uint64_t k;
+    if constexpr (WIDE) {
+        k = pack_voxel(v);
+    } else {
+        const uint64_t ny = static_cast<uint64_t>(static_cast<int64_t>(info[5]) - info[2] + 1);
+        const uint64_t nz = static_cast<uint64_t>(static_cast<int64_t>(info[6]) - info[3] + 1);
+        const uint64_t nx = static_cast<uint64_t>(static_cast<int64_t>(info[4]) - info[1] + 1);
+        if (nx * ny * nz * 32ull > 0xffffffffull) {
+            if (i == 0) info[10] = 1;
+            k = 0;
+        } else {
+            k = ((static_cast<uint64_t>(v.x - info[1]) * ny + static_cast<uint64_t>(v.y - info[2])) * nz +
+                 static_cast<uint64_t>(v.z - info[3])) * 32ull + static_cast<uint64_t>(v.w);
+        }
+    }
+    keys[i] = k;
     vals[i] = j;
 }
 
@@ -158,47 +233,16 @@ __global__ void run_heads_kernel(const uint64_t* __restrict__ keys, uint32_t n, 
     flags[i] = (k != kNoKey && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
 }
 
-// Wave-wide minimum / maximum (every lane of the wave takes part).
-__device__ __forceinline__ int32_t wave_min(int32_t v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int32_t wave_max(int32_t v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// The unique voxels in sorted order, their count (info[0]) and bounds
-// (info[1..6]); the bounds are reduced over each wave first, so the six
-// atomics are taken once per wave rather than once per voxel.
+// The unique voxel keys in sorted order and their count (info[0]).
 __global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
-                                     const uint32_t* __restrict__ ranks, uint32_t n, uint64_t* __restrict__ uniq,
+                                     const uint32_t* __restrict__ ranks, uint32_t n,
+                                     const int32_t* __restrict__ dyn, uint64_t* __restrict__ uniq,
                                      int32_t* __restrict__ info) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    n = live_count(n, &info[9]);
-    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-    if (i < n) {
-        if (i == n - 1) info[0] = static_cast<int32_t>(ranks[i] + flags[i]);
-        if (flags[i]) {
-            uniq[ranks[i]] = keys[i];
-            const ap_int4 v = unpack_voxel(keys[i]);
-            lo[0] = hi[0] = v.x;
-            lo[1] = hi[1] = v.y;
-            lo[2] = hi[2] = v.z;
-        }
-    }
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min(lo[a]);
-        hi[a] = wave_max(hi[a]);
-    }
-    if ((threadIdx.x & 63) == 0 && lo[0] != INT32_MAX) {
-        atomicMin(&info[1], lo[0]);
-        atomicMin(&info[2], lo[1]);
-        atomicMin(&info[3], lo[2]);
-        atomicMax(&info[4], hi[0]);
-        atomicMax(&info[5], hi[1]);
-        atomicMax(&info[6], hi[2]);
-    }
+    n = live_count(n, dyn);
+    if (i >= n) return;
+    if (i == n - 1) info[0] = static_cast<int32_t>(ranks[i] + flags[i]);
+    if (flags[i]) uniq[ranks[i]] = keys[i];
 }
 
 // Starts a build's counters: unique count 0, empty bounds, no new probes,
@@ -206,15 +250,6 @@ __global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const ui
 __global__ void init_info_kernel(int32_t* __restrict__ info) {
     const int k = threadIdx.x;
     if (k < 16) info[k] = (k >= 1 && k <= 3) ? INT32_MAX : (k >= 4 && k <= 6) ? INT32_MIN : 0;
-}
-
-// Restarts the unique count and voxel bounds before a second voxel sort.
-__global__ void reset_bounds_kernel(int32_t* __restrict__ info) {
-    if (threadIdx.x == 0) {
-        info[0] = 0;
-        info[1] = info[2] = info[3] = INT32_MAX;
-        info[4] = info[5] = info[6] = INT32_MIN;
-    }
 }
 
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, size_t n, int32_t v) {
@@ -237,22 +272,41 @@ __global__ void copy_volume_kernel(const int32_t* __restrict__ old_vol, VolumeVi
     for (int f = 0; f < PG_FACES; ++f) newv.volume[dst * PG_FACES + f] = old_vol[i * PG_FACES + f];
 }
 
-__global__ void unindexed_kernel(const uint64_t* __restrict__ uniq, uint32_t n, VolumeView vv,
+// Voxel of a unique key: narrow keys relative to start[] over ny x nz
+// voxels, or pack_voxel's absolute keys.
+struct KeyDecode {
+    int32_t wide;
+    int32_t start[3];
+    uint32_t ny, nz;
+    __device__ ap_int4 operator()(uint64_t k) const {
+        if (wide) return unpack_voxel(k);
+        ap_int4 v;
+        v.w = static_cast<int32_t>(k & 31u);
+        k >>= 5;
+        v.z = start[2] + static_cast<int32_t>(k % nz);
+        k /= nz;
+        v.y = start[1] + static_cast<int32_t>(k % ny);
+        v.x = start[0] + static_cast<int32_t>(k / ny);
+        return v;
+    }
+};
+
+__global__ void unindexed_kernel(const uint64_t* __restrict__ uniq, uint32_t n, KeyDecode dec, VolumeView vv,
                                  uint32_t* __restrict__ flags) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int64_t s = vv.slot(unpack_voxel(uniq[i]));
+    const int64_t s = vv.slot(dec(uniq[i]));
     flags[i] = (s >= 0 && vv.volume[s] < 0) ? 1u : 0u;
 }
 
 __global__ void assign_kernel(const uint64_t* __restrict__ uniq, const uint32_t* __restrict__ flags,
-                              const uint32_t* __restrict__ ranks, uint32_t n, VolumeView vv, int32_t base,
-                              int32_t* __restrict__ info) {
+                              const uint32_t* __restrict__ ranks, uint32_t n, KeyDecode dec, VolumeView vv,
+                              int32_t base, int32_t* __restrict__ info) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     if (i == n - 1) info[7] = static_cast<int32_t>(ranks[i] + flags[i]);
     if (!flags[i]) return;
-    const int64_t s = vv.slot(unpack_voxel(uniq[i]));
+    const int64_t s = vv.slot(dec(uniq[i]));
     vv.volume[s] = base + static_cast<int32_t>(ranks[i]);
 }
 
@@ -435,7 +489,7 @@ VolumeView view_of(const GuideDevice* g) {
 
 void guide_free(GuideDevice* g) {
     if (!g) return;
-    void* ptrs[] = {g->probes, g->temp, g->volume, g->recs, g->slots, g->keys[0], g->keys[1], g->vals[0],
+    void* ptrs[] = {g->probes, g->temp, g->volume, g->recs, g->recs2, g->slots, g->nvalid, g->keys[0], g->keys[1], g->vals[0],
                     g->vals[1], g->flags, g->ranks, g->list, g->uniq, g->tmp, g->info};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -444,12 +498,14 @@ void guide_free(GuideDevice* g) {
     delete g;
 }
 
-int guide_prepare(GuideDevice** pg, size_t n_paths) {
+int guide_prepare(GuideDevice** pg, size_t n_paths, int sets) {
     if (!*pg) *pg = new GuideDevice();
     GuideDevice* g = *pg;
     int rc;
     if ((rc = grow(&g->recs, &g->rec_cap, n_paths, "guiding: path records")) != AP_OK) return rc;
+    if (sets > 1 && (rc = grow(&g->recs2, &g->rec2_cap, n_paths, "guiding: path records")) != AP_OK) return rc;
     if ((rc = grow(&g->slots, &g->slot_cap, n_paths * PG_COLLECT, "guiding: sample slots")) != AP_OK) return rc;
+    if ((rc = grow(&g->nvalid, &g->nvalid_cap, n_paths, "guiding: sample counts")) != AP_OK) return rc;
     if (!g->info) {
         hipError_t e = hipMalloc(reinterpret_cast<void**>(&g->info), 16 * sizeof(int32_t));
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&g->info_host), 16 * sizeof(int32_t));
@@ -472,7 +528,10 @@ GuideView guide_view(GuideDevice* g) {
     return v;
 }
 
+GuideRec* guide_recs(GuideDevice* g, int set) { return g ? (set == 1 ? g->recs2 : g->recs) : nullptr; }
+
 ap_path_guiding_sample* guide_slots(GuideDevice* g) { return g ? g->slots : nullptr; }
+uint32_t* guide_nvalid(GuideDevice* g) { return g ? g->nvalid : nullptr; }
 
 int guide_reset(GuideDevice* g) {
     if (!g) return AP_OK;
@@ -496,20 +555,27 @@ int settle(GuideDevice* g) {
     return AP_OK;
 }
 
-// Steps 0 and 1 of a build, then the read-back of the counts the host
-// needs (unique voxels and their bounds), all enqueued without a wait.
-hipError_t unique_voxels(GuideDevice* g, uint32_t cap, hipStream_t st) {
+// Step 1 of a build over the first min(m, cap) valid slots (m on the
+// device), then the read-back of the counts the host needs (unique voxels
+// and bounds, the valid-slot count and the range flags), all enqueued
+// without a wait. Narrow keys take four 8-bit radix passes, wide ones eight.
+hipError_t unique_voxels(GuideDevice* g, uint32_t cap, bool wide, hipStream_t st) {
     const uint32_t* m_dev = reinterpret_cast<const uint32_t*>(g->info + 9);
-    hipLaunchKernelGGL(voxel_keys_kernel, grid_for(cap), dim3(kBlock), 0, st, g->slots, g->list, cap, g->info + 9,
-                       g->keys[0], g->vals[0]);
-    hipError_t err = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], cap, 0, 64, g->tmp, st,
-                                          m_dev);
+    g->wide = wide;
+    if (wide)
+        hipLaunchKernelGGL(voxel_keys_kernel<true>, grid_for(cap), dim3(kBlock), 0, st, g->slots, g->list, cap,
+                           g->info, g->keys[0], g->vals[0]);
+    else
+        hipLaunchKernelGGL(voxel_keys_kernel<false>, grid_for(cap), dim3(kBlock), 0, st, g->slots, g->list, cap,
+                           g->info, g->keys[0], g->vals[0]);
+    hipError_t err = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], cap, 0, wide ? 64 : 32,
+                                          g->tmp, st, m_dev);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(run_heads_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], cap, g->info + 9, g->flags);
     err = scan_exclusive_u32(g->flags, g->ranks, cap, nullptr, g->tmp, st, m_dev);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(unique_voxels_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, cap,
-                       g->uniq, g->info);
+                       g->info + 9, g->uniq, g->info);
     return hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
 }
 
@@ -527,16 +593,17 @@ int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
         if (e != hipSuccess) return hip_fail(e, "guiding: event");
     }
     hipLaunchKernelGGL(init_info_kernel, dim3(1), dim3(64), 0, st, g->info);
-    // 0. the valid slots, in slot order (most of a frame's slots are empty);
-    // their count m stays on the device (info[9]).
-    hipLaunchKernelGGL(valid_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->slots, n, g->flags, g->info);
-    hipError_t e = scan_exclusive_u32(g->flags, g->ranks, n, nullptr, g->tmp, st);
+    // 0. the valid slots in slot order from the per-path counts (most of a
+    // frame's slots are empty); their count m stays on the device (info[9]).
+    const uint32_t np = static_cast<uint32_t>(n_paths);
+    hipError_t e = scan_exclusive_u32(g->nvalid, g->ranks, np, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: slot scan");
-    hipLaunchKernelGGL(compact_slots_kernel, grid_for(n), dim3(kBlock), 0, st, g->flags, g->ranks, n, g->list, g->info);
+    hipLaunchKernelGGL(compact_valid_kernel, dim3(std::min(kCompactBlocks, grid_for(np).x)), dim3(kBlock), 0, st,
+                       g->slots, g->nvalid, g->ranks, np, g->list, g->info);
     // 1. unique voxels in sorted order, launched for `cap` slots and run
     // over the first min(m, cap).
     const uint32_t cap = std::min(n, std::max(g->sort_hint, 1u << 16));
-    if ((e = unique_voxels(g, cap, st)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+    if ((e = unique_voxels(g, cap, false, st)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
     g->open = true;
     g->upd_n = n;
     g->upd_cap = cap;
@@ -552,11 +619,17 @@ int guide_update_end(GuideDevice* g, hipStream_t st) {
     const uint32_t m = static_cast<uint32_t>(g->info_host[9]);
     g->sort_hint = static_cast<uint32_t>(std::min<uint64_t>(g->upd_n, 2ull * m));
     if (m == 0) return AP_OK;
-    if (m > g->upd_cap) {  // more valid slots than the launch covered: sort them all
-        hipLaunchKernelGGL(reset_bounds_kernel, dim3(1), dim3(64), 0, st, g->info);
-        if ((e = unique_voxels(g, m, st)) == hipSuccess) e = hipStreamSynchronize(st);
+    if (m > g->upd_cap || g->info_host[10]) {
+        // More valid slots than the launch covered, or a voxel range too
+        // wide for 32-bit keys: sort them all again (with absolute keys).
+        if ((e = unique_voxels(g, m, g->info_host[10] != 0, st)) == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
     }
+    KeyDecode dec{};
+    dec.wide = g->wide ? 1 : 0;
+    for (int a = 0; a < 3; ++a) dec.start[a] = g->info_host[1 + a];
+    dec.ny = static_cast<uint32_t>(g->info_host[5] - g->info_host[2] + 1);
+    dec.nz = static_cast<uint32_t>(g->info_host[6] - g->info_host[3] + 1);
     const uint32_t n_uniq = static_cast<uint32_t>(g->info_host[0]);
     if (n_uniq == 0) return AP_OK;
     // 2. grow the index volume to the union of the bounds.
@@ -602,11 +675,12 @@ int guide_update_end(GuideDevice* g, hipStream_t st) {
     // Probes for the voxels without one, numbered in sorted order: at most
     // n_uniq new ones, so room for them is made before their number is known.
     if ((rc = grow_probes(g, g->count + static_cast<int32_t>(n_uniq), st)) != AP_OK) return rc;
-    hipLaunchKernelGGL(unindexed_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, n_uniq, view_of(g), g->flags);
+    hipLaunchKernelGGL(unindexed_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, n_uniq, dec, view_of(g),
+                       g->flags);
     e = scan_exclusive_u32(g->flags, g->ranks, n_uniq, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: probe scan");
     hipLaunchKernelGGL(assign_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, g->flags, g->ranks, n_uniq,
-                       view_of(g), g->count, g->info);
+                       dec, view_of(g), g->count, g->info);
     hipLaunchKernelGGL(zero_new_probes_kernel,
                        grid_for(static_cast<size_t>(n_uniq) * (sizeof(ap_octahedron) / 16)), dim3(kBlock), 0, st,
                        g->probes + g->count, static_cast<size_t>(n_uniq), g->info);

@@ -20,6 +20,7 @@
 // popcount). Several frames are traced as one batch and accumulated in frame
 // order.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -156,6 +157,7 @@ struct alignas(16) BatchScene {
     const float* uvs;          // per object: the triangle's three vertex UVs
     GuideView guide;                      // guided batches: the scene's probes and index volume
     ap_path_guiding_sample* guide_slots;  // guided batches: the scene's sample slots, pixel-major
+    uint32_t* guide_nvalid;               // guided batches: the scene's per-pixel valid-sample counts
 };
 
 // What the batch kernels see of a batch.
@@ -216,6 +218,23 @@ struct RenderState {
     // traces the next bounce's extension rays; the events fork and join it.
     hipStream_t side = nullptr;
     hipEvent_t ev_shaded = nullptr, ev_shadowed = nullptr;
+    // Frames rendered one at a time (guiding, the radiance cache) are
+    // pipelined two deep: while a frame's last bounces, accumulation, probe
+    // build and cache training run, the next frame's camera rays and their
+    // closest hits are computed on `lead` into the other frame set (the
+    // buffers below; set 0 is the ones above), and the cache trains on
+    // `lead` beside the next frame's bounces.
+    size_t alt_capacity = 0;
+    void* alt_paths = nullptr;
+    MediaRec* alt_media = nullptr;
+    ap_ray* alt_rays0 = nullptr;
+    int32_t* alt_path_of0 = nullptr;
+    ap_hit* alt_hits = nullptr;
+    uint32_t* alt_counters = nullptr;
+    hipStream_t lead = nullptr;
+    hipEvent_t ev_start = nullptr, ev_tail = nullptr, ev_begun = nullptr, ev_inferred = nullptr,
+               ev_trained = nullptr;
+    int nrc_last_set = 0;             // the frame set (cache counters) of the last cache frame
 };
 
 constexpr int kEpsEntries = 1024;
@@ -225,12 +244,16 @@ void render_state_free(RenderState* r) {
     void* ptrs[] = {r->geo, r->mats, r->lights, r->uvs, r->paths, r->media, r->rays[0], r->rays[1], r->path_of[0], r->path_of[1],
                     r->hits, r->shadow_rays, r->shadow_payload, r->shadow_hits, r->counters, r->eps,
                     r->target_tmp, r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_qrgb, r->nrc_verts,
-                    r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters, r->gsamples};
+                    r->nrc_vmask, r->nrc_offsets, r->nrc_pool, r->nrc_counters, r->gsamples, r->alt_paths,
+                    r->alt_media, r->alt_rays0, r->alt_path_of0, r->alt_hits, r->alt_counters};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    if (r->ev_shaded) (void)hipEventDestroy(r->ev_shaded);
-    if (r->ev_shadowed) (void)hipEventDestroy(r->ev_shadowed);
+    hipEvent_t evs[] = {r->ev_shaded, r->ev_shadowed, r->ev_start, r->ev_tail, r->ev_begun, r->ev_inferred,
+                        r->ev_trained};
+    for (hipEvent_t ev : evs)
+        if (ev) (void)hipEventDestroy(ev);
     if (r->side) (void)hipStreamDestroy(r->side);
+    if (r->lead) (void)hipStreamDestroy(r->lead);
     delete r;
 }
 
@@ -319,14 +342,15 @@ __device__ __forceinline__ void pg_advance(GuideRec& r, f3 next, const float* co
 }
 
 // Path end (render.cuh:910-957): valid collection slots to guiding samples,
-// radiance sum_j xyz(l_j) sample_j / psum.
+// radiance sum_j xyz(l_j) sample_j / psum. Returns their number.
 template <int NL>
-__device__ __forceinline__ void pg_emit(const GuideRec& r, const float* lam, float psum,
-                                        const float* __restrict__ tables, ap_path_guiding_sample* __restrict__ out) {
+__device__ __forceinline__ uint32_t pg_emit(const GuideRec& r, const float* lam, float psum,
+                                            const float* __restrict__ tables,
+                                            ap_path_guiding_sample* __restrict__ out) {
     // Valid records are packed to the front of the path's PG_COLLECT slots
-    // in collection order and the rest are cleared; records are stored
-    // straight to `out`, so no local array is indexed at run time.
-    int w = 0;
+    // in collection order (the build reads only that many); records are
+    // stored straight to `out`, so no local array is indexed at run time.
+    uint32_t w = 0;
 #pragma unroll
     for (int i = 0; i < PG_COLLECT; ++i) {
         if (!((r.valid >> i) & 1u)) continue;
@@ -346,9 +370,7 @@ __device__ __forceinline__ void pg_emit(const GuideRec& r, const float* lam, flo
         o.valid = 1;
         out[w++] = o;
     }
-    ap_path_guiding_sample zero;
-    __builtin_memset(&zero, 0, sizeof(zero));
-    for (int k = w; k < PG_COLLECT; ++k) out[k] = zero;
+    return w;
 }
 
 struct CamParams {
@@ -1381,6 +1403,7 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
                                                             float* __restrict__ even, double* __restrict__ sums,
                                                             const GuideRec* __restrict__ recs,
                                                             ap_path_guiding_sample* __restrict__ slots,
+                                                            uint32_t* __restrict__ nvalid,
                                                             const GSample* __restrict__ gsamples,
                                                             ap_screen_gbuffer* __restrict__ gbuffer,
                                                             ap_screen_statistics* __restrict__ stats) {
@@ -1397,7 +1420,8 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int32_t f = 0; f < fp.frames; ++f) {
         const PathT<NL>& S = paths[static_cast<size_t>(f) * fp.n_pixels + p];
-        if (slots && f == 0) pg_emit<NL>(recs[p], S.lam, S.psum, tables, slots + static_cast<size_t>(p) * PG_COLLECT);
+        if (slots && f == 0)
+            nvalid[p] = pg_emit<NL>(recs[p], S.lam, S.psum, tables, slots + static_cast<size_t>(p) * PG_COLLECT);
         float rad[NL];
         for (int j = 0; j < NL; ++j) rad[j] = fix_nan_inf(S.radiance[j]);
         const f3 pre = path_xyz<NL>(S.lam, rad, S.psum, tables);
@@ -1519,7 +1543,8 @@ __global__ __launch_bounds__(kBlock) void accumulate_batch_kernel(BatchView bv, 
     const uint32_t p = i - B.pix_begin;
     const PathT<NL>& S = paths[i];
     if (recs && B.guide_slots)
-        pg_emit<NL>(recs[i], S.lam, S.psum, tables, B.guide_slots + static_cast<size_t>(p) * PG_COLLECT);
+        B.guide_nvalid[p] =
+            pg_emit<NL>(recs[i], S.lam, S.psum, tables, B.guide_slots + static_cast<size_t>(p) * PG_COLLECT);
     if (!B.hdr) return;
     float rad[NL];
     for (int j = 0; j < NL; ++j) rad[j] = fix_nan_inf(S.radiance[j]);
@@ -1911,9 +1936,36 @@ int ensure_side(RenderState* r) {
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_render: side stream");
 }
 
+// The second frame set and the pipeline's stream and events.
+int ensure_pipeline(RenderState* r, size_t cap) {
+    if (!r->lead) {
+        hipError_t e = hipStreamCreateWithFlags(&r->lead, hipStreamNonBlocking);
+        hipEvent_t* evs[] = {&r->ev_start, &r->ev_tail, &r->ev_begun, &r->ev_inferred, &r->ev_trained};
+        for (hipEvent_t* ev : evs)
+            if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "ap_render: pipeline stream");
+    }
+    if (r->alt_capacity >= cap) return AP_OK;
+    void* old[] = {r->alt_paths, r->alt_media, r->alt_rays0, r->alt_path_of0, r->alt_hits, r->alt_counters};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    r->alt_paths = nullptr; r->alt_media = nullptr; r->alt_rays0 = nullptr; r->alt_path_of0 = nullptr;
+    r->alt_hits = nullptr; r->alt_counters = nullptr;
+    r->alt_capacity = 0;
+    int rc;
+    if ((rc = dev_alloc(reinterpret_cast<uint8_t**>(&r->alt_paths), cap * kPathBytes)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->alt_media, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->alt_rays0, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->alt_path_of0, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->alt_hits, cap)) != AP_OK) return rc;
+    if ((rc = dev_alloc(&r->alt_counters, 4)) != AP_OK) return rc;
+    r->alt_capacity = cap;
+    return AP_OK;
+}
+
 int ensure_nrc(RenderState* r, size_t cap, size_t train_cap) {
-    if (!r->nrc_counters) {
-        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->nrc_counters), 4 * sizeof(uint32_t));
+    if (!r->nrc_counters) {  // four per frame set
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->nrc_counters), 8 * sizeof(uint32_t));
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(cache counters)");
     }
     int rc;
@@ -1967,10 +2019,26 @@ struct RunCtx {
     const f4* lam_lut;
     const float* tables;
     int32_t n_pix, per_batch;
-    bool guiding, nrc;
+    bool guiding, nrc, pipelined;
     uint32_t n_stride, n_train_max;
     CamParams cp;
     int32_t inside, last_frame;
+};
+
+// The buffers of one frame in flight: what the next frame's start (its
+// camera rays and their closest hits) writes while this frame finishes.
+// Queue 1 and the shadow queue are shared: the next frame first uses them
+// in its first shade, after this frame has ended.
+template <int NL>
+struct FrameSet {
+    PathT<NL>* paths;
+    MediaRec* media;
+    ap_ray* rays[2];
+    int32_t* path_of[2];
+    ap_hit* hits;
+    uint32_t* counters;      // [0],[1] path queues, [2] shadow queue
+    GuideRec* recs;          // guiding path records, or null
+    uint32_t* nrc_counters;  // cache counters, or null
 };
 
 template <int NL>
@@ -1985,11 +2053,10 @@ int run_frames(const RunCtx& c) {
     const f4 *rgb_lut = c.rgb_lut, *lam_lut = c.lam_lut;
     const float* tables = c.tables;
     const int32_t n_pix = c.n_pix, per_batch = c.per_batch;
-    const bool guiding = c.guiding, nrc = c.nrc;
+    const bool guiding = c.guiding, nrc = c.nrc, pipelined = c.pipelined;
     const uint32_t n_stride = c.n_stride, n_train_max = c.n_train_max;
     const CamParams cp = c.cp;
     const int32_t inside = c.inside, last_frame = c.last_frame;
-    auto* paths = static_cast<PathT<NL>*>(r->paths);
     auto* pays = static_cast<ShadowPay<NL>*>(r->shadow_payload);
     int rc = AP_OK;
     // The path queues are traced at most kQueueWaveRays rays per wave, their
@@ -1998,24 +2065,67 @@ int run_frames(const RunCtx& c) {
     // sample (128: 4.98, 1024: 4.65, 2048: 4.72) and guided frames (2M paths)
     // 10.0 ms (128: 10.2); with 4M paths per batch 128 had been best.
     constexpr int32_t kQueueWaveRays = 512;
+    // A pipelined frame starts the next one once its first kLeadBounces
+    // bounces are launched: from there on its queues are short and the GPU
+    // has room for the next frame's camera rays.
+    constexpr int32_t kLeadBounces = 2;
     DeviceScene qs = s->dev;
     qs.wave_rays = std::min(qs.wave_rays, kQueueWaveRays);
-    // One batch of paths: a rendered batch of frames, or (train_only) the
-    // training paths of one pass of the radiance cache, which renders
-    // nothing into the framebuffers.
-    auto run_batch = [&](const FrameParams& fp, bool train_only, uint32_t n_offset, uint32_t n_train,
-                         uint64_t train_seed) -> int {
+    auto frame_set = [&](int k) {
+        FrameSet<NL> f;
+        if (k == 0) {
+            f.paths = static_cast<PathT<NL>*>(r->paths);
+            f.media = r->media;
+            f.rays[0] = r->rays[0];
+            f.path_of[0] = r->path_of[0];
+            f.hits = r->hits;
+            f.counters = r->counters;
+        } else {
+            f.paths = static_cast<PathT<NL>*>(r->alt_paths);
+            f.media = r->alt_media;
+            f.rays[0] = r->alt_rays0;
+            f.path_of[0] = r->alt_path_of0;
+            f.hits = r->alt_hits;
+            f.counters = r->alt_counters;
+        }
+        f.rays[1] = r->rays[1];
+        f.path_of[1] = r->path_of[1];
+        f.recs = guiding ? guide_recs(s->guide, k) : nullptr;
+        f.nrc_counters = nrc ? r->nrc_counters + 4 * k : nullptr;
+        return f;
+    };
+    // The start of a batch of paths on `st`: path states and camera rays,
+    // then their closest hits (bounce 0's extension trace).
+    auto begin_batch = [&](const FrameParams& fp, const FrameSet<NL>& F, bool train_only, hipStream_t st) -> int {
         const uint32_t total = static_cast<uint32_t>(fp.frames) * static_cast<uint32_t>(fp.n_pixels);
-        hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, r->counters, total, 0u, 0u);
-        const GuideView gv = (guiding && !train_only) ? guide_view(s->guide) : GuideView{};
-        hipLaunchKernelGGL(generate_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, cp, fp, r->eps, paths,
-                           r->rays[0], r->path_of[0], gv.recs, r->media);
+        hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, st, F.counters, total, 0u, 0u);
+        hipLaunchKernelGGL(generate_kernel<NL>, grid_of(total), dim3(kBlock), 0, st, cp, fp, r->eps, F.paths,
+                           F.rays[0], F.path_of[0], (guiding && !train_only) ? F.recs : nullptr, F.media);
+        hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, st, F.counters, 1);
+        const hipError_t e =
+            prm->max_depth > 0 ? launch_trace(qs, F.rays[0], total, F.hits, nullptr, false, st, &F.counters[0])
+                               : hipSuccess;
+        return e == hipSuccess ? AP_OK : hip_fail(e, "ap_render: extend");
+    };
+    bool trained_on_lead = false;   // a cache training step runs on `lead`, not yet waited for
+    // The rest of a batch of paths on the render stream: a rendered batch of
+    // frames, or (train_only) the training paths of one pass of the radiance
+    // cache, which renders nothing into the framebuffers. at_tail runs once
+    // the first kLeadBounces bounces are launched.
+    auto finish_batch = [&](const FrameParams& fp, const FrameSet<NL>& F, bool train_only, uint32_t n_offset,
+                            uint32_t n_train, uint64_t train_seed, const std::function<int()>& at_tail) -> int {
+        const uint32_t total = static_cast<uint32_t>(fp.frames) * static_cast<uint32_t>(fp.n_pixels);
+        GuideView gv{};
+        if (guiding && !train_only) {
+            gv = guide_view(s->guide);   // the probes the frames before built
+            gv.recs = F.recs;
+        }
         NrcView nv{};
         if (nrc) {
             hipLaunchKernelGGL(nrc_init_kernel, grid_of(static_cast<size_t>(total)), dim3(kBlock), 0, stream,
                                static_cast<int32_t>(total), n_stride, n_offset, n_train, np->unbiased_every,
-                               train_only ? 1u : 0u, r->nrc_paths, r->nrc_counters);
-            nv = NrcView{r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_verts, r->nrc_counters, np->spread};
+                               train_only ? 1u : 0u, r->nrc_paths, F.nrc_counters);
+            nv = NrcView{r->nrc_paths, r->nrc_queries, r->nrc_qpath, r->nrc_verts, F.nrc_counters, np->spread};
         }
         auto* rc_ptr = train_only ? nullptr : reinterpret_cast<unsigned long long*>(fb->ray_count);
         int32_t* prim = train_only ? nullptr : fb->primary_id;
@@ -2024,11 +2134,9 @@ int run_frames(const RunCtx& c) {
         // queues and outputs), joined before resolve. The shadow counter is
         // cleared for the next shade only after resolve has read it.
         int in_q = 0;
-        hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, 1);
-        hipError_t e = prm->max_depth > 0
-                           ? launch_trace(qs, r->rays[0], total, r->hits, nullptr, false, stream, &r->counters[0])
-                           : hipSuccess;
-        if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
+        hipError_t e = hipSuccess;
+        bool tail_started = false;
+        int rc2 = AP_OK;
         for (int32_t depth = 0; depth < prm->max_depth; ++depth) {
             const int out_q = 1 - in_q;
             const bool more = depth + 1 < prm->max_depth;
@@ -2037,92 +2145,103 @@ int run_frames(const RunCtx& c) {
             if (fb->gbuffer && !train_only) sp.gsample = r->gsamples;
             sp.volumes = s->dev.volumes;
             sp.n_volumes = s->dev.n_volumes;
-            sp.media = r->media;
+            sp.media = F.media;
             sp.textures = texture_cache_table(s->tex_cache);
             sp.uvs = r->uvs;
-            if (guiding && nrc && !train_only) {
-                hipLaunchKernelGGL((shade_kernel<NL, true, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
-                                   r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
-                                   paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
-                                   r->path_of[out_q], r->shadow_rays, pays, r->counters, prim, last_frame,
-                                   rc_ptr, gv, nv);
-            } else if (guiding && !nrc) {
-                hipLaunchKernelGGL((shade_kernel<NL, true, false>), grid_of(total), dim3(kBlock), 0, stream, sp,
-                                   r->geo, r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps,
-                                   paths, r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q],
-                                   r->path_of[out_q], r->shadow_rays, pays, r->counters, prim, last_frame,
-                                   rc_ptr, gv, nv);
-            } else if (nrc) {
-                hipLaunchKernelGGL((shade_kernel<NL, false, true>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
-                                   r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, paths,
-                                   r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
-                                   r->shadow_rays, pays, r->counters, prim, last_frame, rc_ptr, gv, nv);
-            } else {
-                hipLaunchKernelGGL((shade_kernel<NL, false, false>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo,
-                                   r->mats, r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, paths,
-                                   r->rays[in_q], r->path_of[in_q], r->hits, r->rays[out_q], r->path_of[out_q],
-                                   r->shadow_rays, pays, r->counters, prim, last_frame, rc_ptr, gv, nv);
-            }
+#define AP_SHADE(G, N)                                                                                         \
+    hipLaunchKernelGGL((shade_kernel<NL, G, N>), grid_of(total), dim3(kBlock), 0, stream, sp, r->geo, r->mats, \
+                       r->lights, s->dev.analytic, rgb_lut, lam_lut, tables, r->eps, F.paths, F.rays[in_q],    \
+                       F.path_of[in_q], F.hits, F.rays[out_q], F.path_of[out_q], r->shadow_rays, pays,         \
+                       F.counters, prim, last_frame, rc_ptr, gv, nv)
+            if (guiding && nrc && !train_only) AP_SHADE(true, true);
+            else if (guiding && !nrc) AP_SHADE(true, false);
+            else if (nrc) AP_SHADE(false, true);
+            else AP_SHADE(false, false);
+#undef AP_SHADE
             e = hipEventRecord(r->ev_shaded, stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(r->side, r->ev_shaded, 0);
             if (e == hipSuccess)
-                e = launch_trace(qs, r->shadow_rays, total, r->shadow_hits, nullptr, true, r->side, &r->counters[2],
+                e = launch_trace(qs, r->shadow_rays, total, r->shadow_hits, nullptr, true, r->side, &F.counters[2],
                                  sdf_eps_of_depth(depth));
             if (e == hipSuccess) e = hipEventRecord(r->ev_shadowed, r->side);
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
             if (more) {
-                e = launch_trace(qs, r->rays[out_q], total, r->hits, nullptr, false, stream, &r->counters[out_q]);
+                e = launch_trace(qs, F.rays[out_q], total, F.hits, nullptr, false, stream, &F.counters[out_q]);
                 if (e != hipSuccess) return hip_fail(e, "ap_render: extend");
             }
             e = hipStreamWaitEvent(stream, r->ev_shadowed, 0);
             if (e != hipSuccess) return hip_fail(e, "ap_render: join");
-            hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->counters, pays,
-                               r->shadow_hits, paths, rc_ptr);
+            hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, F.counters, pays,
+                               r->shadow_hits, F.paths, rc_ptr);
             in_q = out_q;
-            if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, r->counters, 1 - in_q);
+            if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, F.counters, 1 - in_q);
+            if (depth + 1 == kLeadBounces) {
+                tail_started = true;
+                if ((rc2 = at_tail()) != AP_OK) return rc2;
+            }
             // Deep limits: stop once every path has ended.
             if (prm->max_depth > 32 && (depth % 16) == 15) {
                 uint32_t live = 0;
-                e = hipMemcpyAsync(&live, &r->counters[in_q], sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+                e = hipMemcpyAsync(&live, &F.counters[in_q], sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
                 if (e == hipSuccess) e = hipStreamSynchronize(stream);
                 if (e != hipSuccess) return hip_fail(e, "ap_render: live count");
                 if (live == 0) break;
             }
         }
-        int rc2 = AP_OK;
+        if (!tail_started && (rc2 = at_tail()) != AP_OK) return rc2;
         if (nrc) {
-            // Cache vertices, then the training samples of the batch.
-            if ((rc2 = nrc_infer_dyn(ns, r->nrc_queries, r->nrc_qpath, &r->nrc_counters[0], total, r->nrc_qrgb,
+            // Cache vertices (with the weights the frames before trained),
+            // then the training samples of the batch.
+            if (trained_on_lead) {
+                if ((e = hipStreamWaitEvent(stream, r->ev_trained, 0)) != hipSuccess)
+                    return hip_fail(e, "ap_render: wait for training");
+                trained_on_lead = false;
+            }
+            if ((rc2 = nrc_infer_dyn(ns, r->nrc_queries, r->nrc_qpath, &F.nrc_counters[0], total, r->nrc_qrgb,
                                      stream)) != AP_OK)
                 return rc2;
-            hipLaunchKernelGGL(nrc_apply_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, r->nrc_counters,
-                               r->nrc_qpath, r->nrc_qrgb, rgb_lut, lam_lut, tables, paths, gv.recs);
+            hipLaunchKernelGGL(nrc_apply_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, F.nrc_counters,
+                               r->nrc_qpath, r->nrc_qrgb, rgb_lut, lam_lut, tables, F.paths, gv.recs);
             if (n_train > 0) {
                 hipLaunchKernelGGL(nrc_targets_kernel<NL>, grid_of(n_train), dim3(kBlock), 0, stream, n_train,
-                                   n_stride, n_offset, train_only ? 1u : 0u, r->nrc_paths, paths, tables,
+                                   n_stride, n_offset, train_only ? 1u : 0u, r->nrc_paths, F.paths, tables,
                                    r->nrc_verts, r->nrc_vmask);
                 const hipError_t se = nrc_scan(n_train, r->nrc_vmask, r->nrc_offsets, r->nrc_train_cap,
-                                               r->nrc_counters, stream);
+                                               F.nrc_counters, stream);
                 if (se != hipSuccess) return hip_fail(se, "ap_render: training sample scan");
                 hipLaunchKernelGGL(nrc_compact_kernel, grid_of(n_train), dim3(kBlock), 0, stream, n_train, n_stride,
                                    n_offset, r->nrc_vmask, r->nrc_offsets, r->nrc_verts, r->nrc_pool);
             }
             r->nrc_last_train = n_train;
+            r->nrc_last_set = static_cast<int>(F.nrc_counters - r->nrc_counters) / 4;
             r->nrc_rendered = true;
         }
         if (!train_only)
-            hipLaunchKernelGGL(accumulate_kernel<NL>, grid_of(fp.n_pixels), dim3(kBlock), 0, stream, fp, paths, tables,
-                               fb->hdr, fb->odd, fb->even, fb->sum_xyz, gv.recs,
-                               guiding ? guide_slots(s->guide) : nullptr, r->gsamples, fb->gbuffer,
-                               fb->statistics);
+            hipLaunchKernelGGL(accumulate_kernel<NL>, grid_of(fp.n_pixels), dim3(kBlock), 0, stream, fp, F.paths,
+                               tables, fb->hdr, fb->odd, fb->even, fb->sum_xyz, gv.recs,
+                               guiding ? guide_slots(s->guide) : nullptr, guiding ? guide_nvalid(s->guide) : nullptr,
+                               r->gsamples, fb->gbuffer, fb->statistics);
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "ap_render: launch");
+        if (nrc && pipelined) {
+            // The cache trains on `lead` beside the next frame's bounces;
+            // the next frame's inference waits for it.
+            e = hipEventRecord(r->ev_inferred, stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(r->lead, r->ev_inferred, 0);
+            if (e != hipSuccess) return hip_fail(e, "ap_render: training fork");
+        }
         if (guiding && !train_only && (rc2 = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK)
             return rc2;
         if (nrc && n_train > 0 && np->train_steps > 0) {
-            rc2 = nrc_train_pool(ns, r->nrc_pool, nullptr, &r->nrc_counters[1], np->batch, np->train_steps, np->learning_rate,
-                                 train_seed, stream);
+            const hipStream_t ts = pipelined ? r->lead : stream;
+            rc2 = nrc_train_pool(ns, r->nrc_pool, nullptr, &F.nrc_counters[1], np->batch, np->train_steps,
+                                 np->learning_rate, train_seed, ts);
             if (rc2 != AP_OK) return rc2;
+            if (pipelined) {
+                if ((e = hipEventRecord(r->ev_trained, r->lead)) != hipSuccess)
+                    return hip_fail(e, "ap_render: training event");
+                trained_on_lead = true;
+            }
         }
         return AP_OK;
     };
@@ -2135,28 +2254,70 @@ int run_frames(const RunCtx& c) {
             n_train = std::min(n_train_max, (static_cast<uint32_t>(n_pix) - n_offset + n_stride - 1) / n_stride);
         }
     };
-    for (int32_t f0 = 0; f0 < prm->spp; f0 += per_batch) {
+    auto batch_params = [&](int32_t f0) {
         const int32_t nf = std::min(per_batch, prm->spp - f0);
-        const FrameParams fp{prm->frame_index + f0, nf, n_pix, inside, prm->seed, 0u, 1u};
+        return FrameParams{prm->frame_index + f0, nf, n_pix, inside, prm->seed, 0u, 1u};
+    };
+    const std::function<int()> no_tail = [] { return AP_OK; };
+    hipError_t e = hipSuccess;
+    if (pipelined) {
+        // The first frame starts on `lead` after the work before this call.
+        e = hipEventRecord(r->ev_start, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(r->lead, r->ev_start, 0);
+        if (e != hipSuccess) return hip_fail(e, "ap_render: pipeline start");
+        if ((rc = begin_batch(batch_params(0), frame_set(0), false, r->lead)) != AP_OK) return rc;
+        if ((e = hipEventRecord(r->ev_begun, r->lead)) != hipSuccess) return hip_fail(e, "ap_render: pipeline");
+    }
+    int k = 0;   // frame set of the current batch
+    for (int32_t f0 = 0; f0 < prm->spp; f0 += per_batch) {
+        const FrameParams fp = batch_params(f0);
+        const FrameSet<NL> F = frame_set(k);
         uint32_t n_offset = 0, n_train = 0;
         uint64_t h = 0;
         if (nrc) {
             h = nrc_frame_hash(prm->seed, fp.frame0);
             train_pixels(h, n_offset, n_train);
         }
-        if ((rc = run_batch(fp, false, n_offset, n_train, h ^ kNrcTrainSalt)) != AP_OK) return rc;
+        const bool has_next = f0 + per_batch < prm->spp;
+        std::function<int()> start_next = no_tail;
+        if (pipelined) {
+            if ((e = hipStreamWaitEvent(stream, r->ev_begun, 0)) != hipSuccess)
+                return hip_fail(e, "ap_render: pipeline");
+            if (has_next)
+                start_next = [&, f0, k]() -> int {
+                    hipError_t e2 = hipEventRecord(r->ev_tail, stream);
+                    if (e2 == hipSuccess) e2 = hipStreamWaitEvent(r->lead, r->ev_tail, 0);
+                    if (e2 != hipSuccess) return hip_fail(e2, "ap_render: pipeline");
+                    const int rc3 = begin_batch(batch_params(f0 + per_batch), frame_set(1 - k), false, r->lead);
+                    if (rc3 != AP_OK) return rc3;
+                    e2 = hipEventRecord(r->ev_begun, r->lead);
+                    return e2 == hipSuccess ? AP_OK : hip_fail(e2, "ap_render: pipeline");
+                };
+        } else if ((rc = begin_batch(fp, F, false, stream)) != AP_OK) {
+            return rc;
+        }
+        if ((rc = finish_batch(fp, F, false, n_offset, n_train, h ^ kNrcTrainSalt, start_next)) != AP_OK) return rc;
         // Training-only passes of the cache: the training paths alone, on
         // their own pixels and random streams.
         const uint32_t passes = (nrc && n_train_max > 0) ? np->train_passes : 0u;
-        for (uint32_t k = 0; k < passes; ++k) {
-            const uint64_t hk = splitmix64(h ^ (0x70617373ull + k));
+        for (uint32_t q = 0; q < passes; ++q) {
+            const uint64_t hk = splitmix64(h ^ (0x70617373ull + q));
             uint32_t o2 = 0, t2 = 0;
             train_pixels(hk, o2, t2);
             if (t2 == 0) continue;
             const FrameParams tp{fp.frame0, 1, static_cast<int32_t>(t2), inside, splitmix64(prm->seed ^ hk), o2,
                                  n_stride};
-            if ((rc = run_batch(tp, true, o2, t2, hk ^ kNrcTrainSalt)) != AP_OK) return rc;
+            const FrameSet<NL> T = frame_set(0);
+            if ((rc = begin_batch(tp, T, true, stream)) != AP_OK) return rc;
+            if ((rc = finish_batch(tp, T, true, o2, t2, hk ^ kNrcTrainSalt, no_tail)) != AP_OK) return rc;
         }
+        if (pipelined) k = 1 - k;
+    }
+    if (pipelined) {
+        // The caller's stream ends after everything on `lead` (the last training).
+        e = hipEventRecord(r->ev_start, r->lead);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, r->ev_start, 0);
+        if (e != hipSuccess) return hip_fail(e, "ap_render: pipeline end");
     }
     return AP_OK;
 }
@@ -2203,9 +2364,14 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     const bool nrc = cache != nullptr;
     const int32_t per_batch = (guiding || nrc)
         ? 1 : std::max(1, std::min(prm->spp, static_cast<int32_t>((16u << 20) / static_cast<uint32_t>(n_pix))));
+    // Frames rendered one at a time are pipelined two deep (no training-only
+    // passes of the cache, which reuse the first frame set).
+    const bool pipelined = per_batch == 1 && prm->spp > 1 && (guiding || nrc) && !(nrc && np->train_passes > 0);
     if ((rc = ensure_paths(r, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
     if ((rc = ensure_side(r)) != AP_OK) return rc;
-    if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(n_pix))) != AP_OK) return rc;
+    if (pipelined && (rc = ensure_pipeline(r, static_cast<size_t>(n_pix))) != AP_OK) return rc;
+    if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(n_pix), pipelined ? 2 : 1)) != AP_OK)
+        return rc;
     if (fb->gbuffer && r->gsample_cap < static_cast<size_t>(per_batch) * n_pix) {
         if (r->gsamples) (void)hipFree(r->gsamples);
         r->gsamples = nullptr;
@@ -2250,8 +2416,8 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     const int32_t inside = camera_inside(s, r->analytic_objs, cp.origin) ? 1 : 0;
     const int32_t last_frame = prm->frame_index + prm->spp - 1;
 
-    RunCtx c{s, r, prm, fb, stream, np, ns, rgb_lut, lam_lut, tables, n_pix, per_batch, guiding, nrc, n_stride,
-             n_train_max, cp, inside, last_frame};
+    RunCtx c{s, r, prm, fb, stream, np, ns, rgb_lut, lam_lut, tables, n_pix, per_batch, guiding, nrc, pipelined,
+             n_stride, n_train_max, cp, inside, last_frame};
     return prm->wavelengths == 4 ? run_frames<4>(c) : run_frames<1>(c);
 }
 
@@ -2410,10 +2576,12 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
         B.uvs = r->uvs;
         B.guide = GuideView{};
         B.guide_slots = nullptr;
+        B.guide_nvalid = nullptr;
         if (guiding) {
             if ((rc = guide_prepare(&s->guide, b->pix_begin[k + 1] - b->pix_begin[k])) != AP_OK) return rc;
             B.guide = guide_view(s->guide);
             B.guide_slots = guide_slots(s->guide);
+            B.guide_nvalid = guide_nvalid(s->guide);
         }
         b->h_devs[k] = s->dev;
         if (s->dev.has_sdf) b->shape.has_sdf = 1;
@@ -2441,6 +2609,7 @@ int batch_guides_update(BatchState* b, hipStream_t stream) {
     for (size_t k = 0; k < b->scenes.size(); ++k) {
         b->h_scenes[k].guide = guide_view(b->scenes[k]->guide);
         b->h_scenes[k].guide_slots = guide_slots(b->scenes[k]->guide);
+        b->h_scenes[k].guide_nvalid = guide_nvalid(b->scenes[k]->guide);
     }
     e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
                        hipMemcpyHostToDevice, stream);
@@ -2850,7 +3019,7 @@ int ap_nrc_last_frame(ap_scene* scene, ap_nrc_frame_stats* stats, float* host_sa
     if (!r || !r->nrc_rendered) return fail(AP_ERR_INVALID_ARG, "ap_nrc_last_frame: no ap_render_nrc frame yet");
     uint32_t c[2] = {0, 0};
     hipError_t e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(c, r->nrc_counters, sizeof(c), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(c, r->nrc_counters + 4 * r->nrc_last_set, sizeof(c), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "ap_nrc_last_frame");
     stats->queries = c[0];
     stats->train_paths = r->nrc_last_train;

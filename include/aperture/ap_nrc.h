@@ -144,9 +144,14 @@ void ap_nrc_render_params_default(ap_nrc_render_params* p);
 
 /* ap_render (include/aperture/ap_render.h) with the cache: frames one at a
  * time, each rendered with the cache as the frames before it trained it,
- * then trained on its own samples. The scene's root box, grown by 1e-3 of
- * its extent on each side, becomes the cache bounds. Guiding is not
- * supported with the cache (AP_ERR_UNSUPPORTED). Asynchronous on `stream`. */
+ * then trained on its own samples (with use_guiding, each also sampling the
+ * probes the frames before it built; a path that ends into the cache takes
+ * the prediction as its endpoint radiance for the probes). The scene's root
+ * box, grown by 1e-3 of its extent on each side, becomes the cache bounds.
+ * The frames of one call overlap on the device (the next frame's camera
+ * rays beside this frame's last bounces, the training beside the next
+ * frame's bounces) with the same results as one call per frame.
+ * Asynchronous on `stream`. */
 int ap_render_nrc(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, ap_nrc* cache,
                   const ap_nrc_render_params* nrc_params, const ap_framebuffers* fb, void* stream);
 

@@ -233,3 +233,44 @@ def test_guided_frozen_cache_matches_oracle(gpu, wavelengths):
     assert np.quantile(trel, 0.999) < 1e-3
     cache.close()
     scene.close()
+
+
+def test_frames_of_one_call_equal_frames_one_call_each(gpu):
+    """The frames of one ap_render_nrc call run two deep (the next frame's
+    camera rays and closest hits beside this frame's last bounces, the
+    cache's training beside the next frame's bounces, each frame set with
+    its own path state and counters); one frame per call runs none of that
+    overlap. Both give the same images, FP64 sums, trained weights and
+    last-frame samples, with and without guiding."""
+    objs = ap.synth_heightfield_scene(32, 3, 5, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 64, 48)
+    n = cam.width * cam.height
+    nprm = ap.NrcRenderParams.defaults(train_paths=1024, batch=1024, train_steps=2)
+    frames = 5
+    for guided in (0, 1):
+        kw = dict(max_depth=16, rr_min_depth=4, seed=3, wavelengths=4, use_guiding=guided)
+        out = []
+        for one_call_each in (False, True):
+            scene = ap.SceneRenderer(objs)
+            cache = ap.RadianceCache(seed=9)
+            d_hdr = ap.DeviceBuffer(n * 12)
+            d_hdr.zero()
+            d_sums = ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64))
+            if one_call_each:
+                for f in range(frames):
+                    scene.render_nrc(cam, ap.RenderParams.defaults(spp=1, frame_index=f, **kw), cache, nprm, d_hdr,
+                                     sum_xyz=d_sums)
+            else:
+                scene.render_nrc(cam, ap.RenderParams.defaults(spp=frames, **kw), cache, nprm, d_hdr, sum_xyz=d_sums)
+            ap.synchronize()
+            st, smp = scene.nrc_last_frame()
+            out.append((d_hdr.to_numpy(np.float32, n * 3), d_sums.to_numpy(np.float64, n * 3), cache.params(),
+                        st.queries, st.samples, smp))
+            cache.close()
+            scene.close()
+        (h1, s1, w1, q1, n1, p1), (h2, s2, w2, q2, n2, p2) = out
+        print(f"[nrc-render] pipelined frames, guided {guided}: queries {q1} / {q2}, samples {n1} / {n2}", flush=True)
+        assert q1 == q2 and n1 == n2 and n1 > 0
+        assert np.array_equal(p1.view(np.uint32), p2.view(np.uint32))
+        assert np.array_equal(h1, h2) and np.array_equal(s1, s2)
+        assert np.array_equal(w1, w2)
