@@ -108,7 +108,7 @@ class RenderParams(ctypes.Structure):
                 ("rr_min_depth", ctypes.c_int32), ("nee_max_depth", ctypes.c_int32), ("use_nee", ctypes.c_int32),
                 ("nee_always", ctypes.c_int32), ("sky", ctypes.c_int32), ("triangle_material", ctypes.c_int32),
                 ("use_guiding", ctypes.c_int32), ("seed", ctypes.c_uint64), ("wavelengths", ctypes.c_int32),
-                ("pad0", ctypes.c_int32)]
+                ("guide_frames", ctypes.c_int32)]
 
     @classmethod
     def defaults(cls, **kw):

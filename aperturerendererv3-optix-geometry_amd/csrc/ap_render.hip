@@ -1420,8 +1420,10 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(FrameParams fp, cons
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int32_t f = 0; f < fp.frames; ++f) {
         const PathT<NL>& S = paths[static_cast<size_t>(f) * fp.n_pixels + p];
-        if (slots && f == 0)
-            nvalid[p] = pg_emit<NL>(recs[p], S.lam, S.psum, tables, slots + static_cast<size_t>(p) * PG_COLLECT);
+        if (slots) {  // frame-major slots of the batch's frames (guided groups)
+            const size_t i = static_cast<size_t>(f) * fp.n_pixels + p;
+            nvalid[i] = pg_emit<NL>(recs[i], S.lam, S.psum, tables, slots + i * PG_COLLECT);
+        }
         float rad[NL];
         for (int j = 0; j < NL; ++j) rad[j] = fix_nan_inf(S.radiance[j]);
         const f3 pre = path_xyz<NL>(S.lam, rad, S.psum, tables);
@@ -2230,7 +2232,8 @@ int run_frames(const RunCtx& c) {
             if (e == hipSuccess) e = hipStreamWaitEvent(r->lead, r->ev_inferred, 0);
             if (e != hipSuccess) return hip_fail(e, "ap_render: training fork");
         }
-        if (guiding && !train_only && (rc2 = guide_update(s->guide, static_cast<size_t>(n_pix), stream)) != AP_OK)
+        if (guiding && !train_only &&
+            (rc2 = guide_update(s->guide, static_cast<size_t>(fp.frames) * n_pix, stream)) != AP_OK)
             return rc2;
         if (nrc && n_train > 0 && np->train_steps > 0) {
             const hipStream_t ts = pipelined ? r->lead : stream;
@@ -2360,17 +2363,25 @@ int render_frames(ap_scene* s, const ap_camera* cam, const ap_render_params* prm
     // (SceneRenderer::Render then UpdatePathGuidingSample, scene.cu:308-386).
     // With the radiance cache, one frame at a time too: each frame queries
     // the cache the frames before it trained.
+    // With guiding in groups of guide_frames frames, a group is one batch
+    // (within the same bound).
     const bool guiding = prm->use_guiding != 0;
     const bool nrc = cache != nullptr;
-    const int32_t per_batch = (guiding || nrc)
-        ? 1 : std::max(1, std::min(prm->spp, static_cast<int32_t>((16u << 20) / static_cast<uint32_t>(n_pix))));
+    if (prm->guide_frames < 0) return fail(AP_ERR_INVALID_ARG, "ap_render: guide_frames must be >= 0");
+    const int32_t group = guiding ? std::max(1, prm->guide_frames) : 1;
+    if (nrc && group > 1)
+        return fail(AP_ERR_UNSUPPORTED, "ap_render_nrc: guiding groups of several frames are not supported with the cache");
+    const int32_t max_batch = std::max(1, static_cast<int32_t>((16u << 20) / static_cast<uint32_t>(n_pix)));
+    if (group > max_batch)
+        return fail(AP_ERR_UNSUPPORTED, "ap_render: a guiding group exceeds 16M paths (guide_frames x pixels)");
+    const int32_t per_batch = nrc ? 1 : guiding ? std::min(group, prm->spp) : std::min(prm->spp, max_batch);
     // Frames rendered one at a time are pipelined two deep (no training-only
     // passes of the cache, which reuse the first frame set).
     const bool pipelined = per_batch == 1 && prm->spp > 1 && (guiding || nrc) && !(nrc && np->train_passes > 0);
     if ((rc = ensure_paths(r, static_cast<size_t>(per_batch) * n_pix)) != AP_OK) return rc;
     if ((rc = ensure_side(r)) != AP_OK) return rc;
     if (pipelined && (rc = ensure_pipeline(r, static_cast<size_t>(n_pix))) != AP_OK) return rc;
-    if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(n_pix), pipelined ? 2 : 1)) != AP_OK)
+    if (guiding && (rc = guide_prepare(&s->guide, static_cast<size_t>(per_batch) * n_pix, pipelined ? 2 : 1)) != AP_OK)
         return rc;
     if (fb->gbuffer && r->gsample_cap < static_cast<size_t>(per_batch) * n_pix) {
         if (r->gsamples) (void)hipFree(r->gsamples);
@@ -2926,6 +2937,8 @@ int batch_check_params(const ap_render_params* prm) {
         return fail(AP_ERR_INVALID_ARG, "ap_batch: bad triangle_material");
     if (prm->wavelengths != 0 && prm->wavelengths != 1 && prm->wavelengths != 4)
         return fail(AP_ERR_INVALID_ARG, "ap_batch: wavelengths must be 1 or 4 (0 reads as 1)");
+    if (prm->use_guiding && prm->guide_frames > 1)
+        return fail(AP_ERR_UNSUPPORTED, "ap_batch: guided batches update the probes after every frame (guide_frames 1)");
     return AP_OK;
 }
 
@@ -2956,6 +2969,7 @@ void ap_render_params_default(ap_render_params* p) {
     p->use_guiding = 0;
     p->seed = 0;
     p->wavelengths = 1;
+    p->guide_frames = 1;
 }
 
 int ap_render(ap_scene* scene, const ap_camera* camera, const ap_render_params* params, const ap_framebuffers* fb,

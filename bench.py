@@ -515,6 +515,15 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     stream.synchronize()
     guided4 = timed(use_guiding=1, wavelengths=4)
     guided4["probes"] = len(scene.guiding_export()[1])
+    # Guiding in groups of 8 frames (guide_frames = 8): the group's frames
+    # sample the probes built before it and are traced as one 16.6M-path
+    # batch, then one update folds the group's samples in.
+    scene.guiding_reset()
+    scene.render(cam, ap.RenderParams.defaults(spp=1, use_guiding=1, wavelengths=4, **params), hdr, stream=stream)
+    stream.synchronize()
+    guided4_groups = timed(use_guiding=1, wavelengths=4, guide_frames=8)
+    guided4_groups["probes"] = len(scene.guiding_export()[1])
+    guided4_groups["guide_frames"] = 8
     # Frames with the screen G-buffer and statistics recorded (the extra
     # per-path first-hit record and the per-pixel updates in accumulate).
     gbuf = ap.DeviceBuffer(W * H * ap.SCREEN_GBUFFER_DTYPE.itemsize)
@@ -543,7 +552,8 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
                "curve": "ACES", "algorithmic_bytes": W * H * 16}
     roof = {k: integrator_roofline(r, key, W * H) for k, r, key in
             (("one_wavelength", one, "plain_1"), ("four_wavelengths", four, "plain_4"),
-             ("guided_one_wavelength", guided, "guided_1"), ("guided_four_wavelengths", guided4, "guided_4"))}
+             ("guided_one_wavelength", guided, "guided_1"), ("guided_four_wavelengths", guided4, "guided_4"),
+             ("guided_four_wavelengths_groups_of_8", guided4_groups, "guided_4"))}
     out = {
         "workload": "heightfield 1024^2 (2,097,152 tris) + 256 icospheres (81,920 tris) + 4 thin-cube lights, "
                     "no sky, 1920x1080, max depth 16, RR from depth 4",
@@ -553,6 +563,7 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
         "four_wavelengths": four,
         "guided_one_wavelength": guided,
         "guided_four_wavelengths": guided4,
+        "guided_four_wavelengths_groups_of_8": guided4_groups,
         "four_wavelengths_with_gbuffer_and_statistics": with_records,
         "tonemap_1080p": tonemap,
         "glass_and_textures_1080p": materials_bench(ap, spp, stream, W, H),

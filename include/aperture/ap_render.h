@@ -49,8 +49,13 @@ typedef struct ap_render_params {
     int32_t wavelengths;       /* 1 (or 0): one wavelength per path (the reference's CalculateRadiance);
                                 * 4: hero-wavelength sampling: the hero drawn by SampleCIE1931 and three
                                 * rotations by 105 nm over [360, 780], one path for all four, the pixel
-                                * estimate sum_j xyz(l_j) L(l_j) / sum_k p(l_k). Not with guiding. */
-    int32_t pad0;
+                                * estimate sum_j xyz(l_j) L(l_j) / sum_k p(l_k). */
+    int32_t guide_frames;      /* with use_guiding: frames per probe update. 1 (or 0): after every frame,
+                                * as SceneRenderer::Render then UpdatePathGuidingSample (scene.cu:308-386);
+                                * F > 1: the frames of a call go in groups of F that sample the same
+                                * probes (built from the groups before) and are traced together, then
+                                * one update folds the group's samples in, frame after frame in slot
+                                * order (the last group of a call may be shorter). */
 } ap_render_params;
 
 /* Caller-owned device framebuffers (GUI.hpp:232-274 owns them in the

@@ -1621,19 +1621,27 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
     const int nl = prm->wavelengths == 4 ? 4 : 1;
     if (prm->use_guiding) {
         // Frames in order; each renders with the probes built from the frames
-        // before it, then folds its own samples in (scene.cu:308-386).
+        // before it, then folds its own samples in (scene.cu:308-386). With
+        // guide_frames F > 1 the frames go in groups of F that render with
+        // the probes built before the group; then one update folds in the
+        // group's samples, frame-major in slot order.
         if (!guide) return -1;
         GuideState& G = guide->state;
-        std::vector<ap_path_guiding_sample> slots(static_cast<size_t>(npix) * PG_COLLECT);
+        const int group = std::max(1, prm->guide_frames);
+        const size_t frame_slots = static_cast<size_t>(npix) * PG_COLLECT;
+        const size_t cap_frames = static_cast<size_t>(std::min(group, std::max(prm->spp, 1)));
+        std::vector<ap_path_guiding_sample> slots(frame_slots * cap_frames);
         for (int s = 0; s < prm->spp; ++s) {
             const int frame = prm->frame_index + s;
+            const size_t f_in_group = static_cast<size_t>(s % group);
+            ap_path_guiding_sample* fslots = &slots[f_in_group * frame_slots];
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
             for (int64_t p = 0; p < npix; ++p) {
                 Rng rng = PathRng(static_cast<uint64_t>(p), static_cast<uint64_t>(frame), prm->seed);
                 const float3 rd = CameraRay(U, rng, p);
                 int prim = -1;
                 float3 res = CalculateRadiance(S, P, rng, U.camO, rd, inside0, &prim, &G, frame,
-                                               &slots[static_cast<size_t>(p) * PG_COLLECT], nl);
+                                               &fslots[static_cast<size_t>(p) * PG_COLLECT], nl);
                 res = fmax3(float3{0, 0, 0}, res);
                 const float3 prev{hdr[3 * p], hdr[3 * p + 1], hdr[3 * p + 2]};
                 const float3 acc = lerp(prev, res, 1.0f / static_cast<float>(1 + frame));
@@ -1645,7 +1653,11 @@ int or_render_guided(const or_bvh* bvh, const ap_scene_object* objs, size_t n, c
                 }
                 if (primary && s == prm->spp - 1) primary[p] = prim;
             }
-            G.Update(slots);
+            if (f_in_group + 1 == static_cast<size_t>(group) || s == prm->spp - 1) {
+                slots.resize((f_in_group + 1) * frame_slots);
+                G.Update(slots);
+                slots.resize(frame_slots * cap_frames);
+            }
         }
         return 0;
     }

@@ -15,7 +15,11 @@ mathematics fixes, independent of the restated code:
     (scene.cu:583-744, guiding.cuh:349-506);
   * guiding only changes the sampling pdf, so the gray floor under a square
     light still converges to its closed form (one and four wavelengths per
-    path).
+    path);
+  * frames in a guiding group (guide_frames F) all sample the probes built
+    before the group: their FP64 sums equal those of the same frames each
+    rendered alone after the same earlier frames, and a group of F frames
+    over empty probes equals the unguided render of those frames.
 """
 import math
 
@@ -190,3 +194,44 @@ def test_guided_floor_matches_closed_form():
         assert g.probe_count() > 0
         y = float((sums[..., 1] / spp).mean())
         assert abs(y / expected - 1.0) < 0.05, (nl, y, expected)
+
+
+def test_guiding_group_frames_share_the_probes_before_the_group():
+    """guide_frames = 3: frames 2, 3, 4 of a group sample the probes built
+    from frames 0 and 1, so each equals that frame rendered alone (one frame,
+    its own call) after the same two frames; the FP64 sums of the group are
+    the three frames' sums added in frame order. A probe update inside the
+    group (a frame seeing the probes of the frame before it) would fail it."""
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make([0, 0, 3.4], [0, 0, -1], 40.0, 20, 16)
+    bvh = ol.OracleBVH(objs)
+    kw = dict(max_depth=8, use_guiding=1, seed=9)
+
+    def warm():
+        g = ol.Guide()
+        bvh.render(cam, ap.RenderParams.defaults(spp=2, frame_index=0, **kw), cull_margin=2 ** -12, guide=g)
+        assert g.probe_count() > 0
+        return g
+
+    g = warm()
+    _, group_sums, _ = bvh.render(cam, ap.RenderParams.defaults(spp=3, frame_index=2, guide_frames=3, **kw),
+                                  cull_margin=2 ** -12, guide=g)
+    alone = np.zeros_like(group_sums)
+    for f in (2, 3, 4):
+        _, s_f, _ = bvh.render(cam, ap.RenderParams.defaults(spp=1, frame_index=f, **kw), cull_margin=2 ** -12,
+                               guide=warm())
+        alone += s_f
+    assert np.array_equal(group_sums, alone)
+    # With guide_frames 1 the same frames differ: frames 3 and 4 see frame 2's samples.
+    g1 = warm()
+    _, seq_sums, _ = bvh.render(cam, ap.RenderParams.defaults(spp=3, frame_index=2, **kw), cull_margin=2 ** -12,
+                                guide=g1)
+    assert not np.array_equal(seq_sums, alone)
+    # Over empty probes a group is the unguided render of its frames.
+    g0 = ol.Guide()
+    _, empty_sums, _ = bvh.render(cam, ap.RenderParams.defaults(spp=3, frame_index=0, guide_frames=4, **kw),
+                                  cull_margin=2 ** -12, guide=g0)
+    plain = dict(kw, use_guiding=0)
+    _, plain_sums, _ = bvh.render(cam, ap.RenderParams.defaults(spp=3, frame_index=0, **plain), cull_margin=2 ** -12)
+    assert np.array_equal(empty_sums, plain_sums)
+    assert g0.probe_count() > 0

@@ -108,3 +108,35 @@ def test_guided_multi_tile_sorts(gpu):
     objs = ap.synth_box_scene(2, 2, 15.0)
     cam = ap.Camera.make(rg.BOX_CAM["origin"], rg.BOX_CAM["direction"], rg.BOX_CAM["fov_deg"], 256, 160)
     guided_parity(objs, cam, dict(spp=1, max_depth=8), "box 256x160", calls=3)
+
+
+def test_guided_groups_of_frames(gpu):
+    """guide_frames > 1: the frames of a group are traced together with the
+    probes built before the group, then one update folds the group's samples
+    in (frame-major slot order); groups of 2 over five frames (the last group
+    shorter) and of 4 with four wavelengths, bit-identical to the oracle's
+    groups (images, sums, probes, index volume)."""
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make(rg.BOX_CAM["origin"], rg.BOX_CAM["direction"], rg.BOX_CAM["fov_deg"], 48, 40)
+    guided_parity(objs, cam, dict(spp=5, max_depth=8, guide_frames=2), "box scene, groups of 2 frames")
+    objs = ap.synth_heightfield_scene(48, 8, 5, 20.0)
+    cam = ap.Camera.make([0.0, 0.8, 1.4], [0.0, -0.8, -1.4], 45.0, 48, 27)
+    guided_parity(objs, cam, dict(spp=4, max_depth=16, rr_min_depth=4, wavelengths=4, guide_frames=4),
+                  "heightfield, four wavelengths, groups of 4 frames")
+
+
+def test_guided_groups_are_refused_where_unsupported(gpu):
+    """Groups of frames with the cache or in a batch are refused, as is a
+    negative group size."""
+    objs = ap.synth_box_scene(1, 2, 15.0)
+    cam = ap.Camera.make(rg.BOX_CAM["origin"], rg.BOX_CAM["direction"], rg.BOX_CAM["fov_deg"], 16, 16)
+    scene = ap.SceneRenderer(objs)
+    hdr = ap.DeviceBuffer(16 * 16 * 12)
+    with pytest.raises(ap.ApertureError):
+        scene.render(cam, ap.RenderParams.defaults(spp=2, use_guiding=1, guide_frames=-1), hdr)
+    cache = ap.RadianceCache(seed=1)
+    with pytest.raises(ap.ApertureError):
+        scene.render_nrc(cam, ap.RenderParams.defaults(spp=2, use_guiding=1, guide_frames=2), cache,
+                         ap.NrcRenderParams.defaults(), hdr)
+    cache.close()
+    scene.close()
