@@ -41,6 +41,12 @@ namespace ap {
 constexpr int kCoordBits = 19;
 constexpr int32_t kCoordBias = 1 << (kCoordBits - 1);
 constexpr uint64_t kNoKey = ~0ull;
+// Build counters on the device (GuideDevice::info): [0] unique voxels,
+// [1..3] / [4..6] the frame's voxel bounds, [7] new probes, [8] error, [9]
+// valid slots, [10] a voxel range too wide for 32-bit keys, and:
+constexpr int kInfoCount = 11;  // probes on the device (carried across frames)
+constexpr int kInfoHost = 12;   // 1: the host finishes this build (the speculative kernels stand down)
+constexpr int kInfoBase = 13;   // probe count this frame's new probes are numbered from
 
 struct GuideDevice {
     ap_octahedron* probes = nullptr;
@@ -65,6 +71,7 @@ struct GuideDevice {
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
     size_t sort_cap = 0;
+    int32_t* bounds_part = nullptr;  // per compaction block: its voxel bounds (8 ints)
     int32_t* info = nullptr;      // device: [0] unique count, [1..3] min xyz, [4..6] max xyz, [7] new probes, [8] error,
                                   // [9] valid slots, [10] voxel range too wide for 32-bit keys
     int32_t* info_host = nullptr; // pinned
@@ -75,6 +82,9 @@ struct GuideDevice {
     // `count` takes it in at the next use (settle).
     bool pending = false;
     hipEvent_t ev_count = nullptr;
+    // The host's read-back of an update's counters and check (info[12]),
+    // recorded right after the check so the host waits only that long.
+    hipEvent_t ev_flag = nullptr;
     // An update between guide_update_begin and guide_update_end: its slot
     // count and voxel-sort launch size.
     bool open = false;
@@ -120,6 +130,11 @@ struct VolumeView {
     }
 };
 
+// The voxel bounds of an index volume (inclusive).
+struct VolumeBox {
+    int32_t start[3], end[3];
+};
+
 // Wave-wide minimum / maximum (every lane of the wave takes part).
 __device__ __forceinline__ int32_t wave_min(int32_t v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -132,16 +147,17 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
 
 // The valid slots in slot order (FilterValidSamples, guiding.cuh:70-99):
 // path p's nvalid[p] samples sit at the front of its PG_COLLECT slots and
-// start at offs[p] in the list. Also their count (info[9]) and voxel bounds
-// (info[1..6]); a valid slot whose voxel is outside the packable range is
-// reported in info[8]. Grid-stride over the paths: the bounds are reduced
-// over the block, then six atomics per block.
-constexpr unsigned kCompactBlocks = 1024;
+// start at offs[p] in the list. Also their count (info[9]) and each block's
+// voxel bounds (part[block][0..5], reduced by bounds_reduce_kernel); a valid
+// slot whose voxel is outside the packable range is reported in info[8].
+// Grid-stride over the paths.
+constexpr unsigned kCompactBlocks = 4096;
 __global__ __launch_bounds__(kBlock) void compact_valid_kernel(const ap_path_guiding_sample* __restrict__ slots,
                                                                const uint32_t* __restrict__ nvalid,
                                                                const uint32_t* __restrict__ offs, uint32_t n_paths,
                                                                uint32_t* __restrict__ list,
-                                                               int32_t* __restrict__ info) {
+                                                               int32_t* __restrict__ info,
+                                                               int32_t* __restrict__ part) {
     __shared__ int32_t red[6][kBlock / 64];
     int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
     bool bad = false;
@@ -178,14 +194,37 @@ __global__ __launch_bounds__(kBlock) void compact_valid_kernel(const ap_path_gui
                 red[a][0] = min(red[a][0], red[a][k]);
                 red[3 + a][0] = max(red[3 + a][0], red[3 + a][k]);
             }
-        if (red[0][0] != INT32_MAX) {
-            atomicMin(&info[1], red[0][0]);
-            atomicMin(&info[2], red[1][0]);
-            atomicMin(&info[3], red[2][0]);
-            atomicMax(&info[4], red[3][0]);
-            atomicMax(&info[5], red[4][0]);
-            atomicMax(&info[6], red[5][0]);
+        for (int a = 0; a < 6; ++a) part[blockIdx.x * 8 + a] = red[a][0];
+    }
+}
+
+// info[1..6]: the voxel bounds over the `blocks` partial bounds (one block).
+__global__ __launch_bounds__(kBlock) void bounds_reduce_kernel(const int32_t* __restrict__ part, uint32_t blocks,
+                                                               int32_t* __restrict__ info) {
+    __shared__ int32_t red[6][kBlock / 64];
+    int32_t v[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+    for (uint32_t b = threadIdx.x; b < blocks; b += kBlock)
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], part[b * 8 + a]);
+            v[3 + a] = max(v[3 + a], part[b * 8 + 3 + a]);
         }
+    const int w = threadIdx.x >> 6;
+    for (int a = 0; a < 3; ++a) {
+        v[a] = wave_min(v[a]);
+        v[3 + a] = wave_max(v[3 + a]);
+        if ((threadIdx.x & 63) == 0) {
+            red[a][w] = v[a];
+            red[3 + a][w] = v[3 + a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k)
+            for (int a = 0; a < 3; ++a) {
+                red[a][0] = min(red[a][0], red[a][k]);
+                red[3 + a][0] = max(red[3 + a][0], red[3 + a][k]);
+            }
+        for (int a = 0; a < 6; ++a) info[1 + a] = red[a][0];
     }
 }
 // Item count of a launch sized for m: min(*dyn, m) (the valid slots).
@@ -246,10 +285,55 @@ __global__ void unique_voxels_kernel(const uint64_t* __restrict__ keys, const ui
 }
 
 // Starts a build's counters: unique count 0, empty bounds, no new probes,
-// no error, no valid slots.
+// no error, no valid slots; the device's probe count (info[11]) carries over.
 __global__ void init_info_kernel(int32_t* __restrict__ info) {
     const int k = threadIdx.x;
-    if (k < 16) info[k] = (k >= 1 && k <= 3) ? INT32_MAX : (k >= 4 && k <= 6) ? INT32_MIN : 0;
+    if (k < 16 && k != kInfoCount) info[k] = (k >= 1 && k <= 3) ? INT32_MAX : (k >= 4 && k <= 6) ? INT32_MIN : 0;
+}
+
+// Sets the device's probe count (info[11]).
+__global__ void set_count_kernel(int32_t* __restrict__ info, int32_t count) {
+    if (threadIdx.x == 0) info[kInfoCount] = count;
+}
+
+// Whether the rest of the build can run without the host (info[12] = 0):
+// no error, 32-bit keys, every valid slot sorted (m <= cap), and, with
+// unique voxels, an index volume that already covers their bounds (so it is
+// not reallocated) and room for a probe per unique voxel. info[13] takes the
+// probe count the new probes are numbered from.
+__global__ void check_kernel(int32_t* __restrict__ info, uint32_t cap, int32_t has_volume, VolumeBox box,
+                             int32_t probe_cap) {
+    if (threadIdx.x != 0) return;
+    const int32_t n_uniq = info[0];
+    bool host = info[8] != 0 || info[10] != 0 || static_cast<uint32_t>(info[9]) > cap;
+    if (!host && n_uniq > 0) {
+        host = !has_volume || static_cast<int64_t>(info[kInfoCount]) + n_uniq > probe_cap;
+        for (int a = 0; a < 3 && !host; ++a)
+            host = info[1 + a] < box.start[a] || info[4 + a] > box.end[a];
+    }
+    info[kInfoHost] = host ? 1 : 0;
+    info[kInfoBase] = info[kInfoCount];
+}
+// A speculative kernel's work: none once the host finishes the build
+// (info[12]) or without unique voxels.
+__device__ __forceinline__ bool spec_off(const int32_t* __restrict__ info) {
+    return info[kInfoHost] != 0 || info[0] <= 0;
+}
+// The narrow-key decoder of this frame's voxel bounds (info[1..6]).
+__device__ __forceinline__ void narrow_decode(const int32_t* __restrict__ info, uint64_t k, ap_int4& v) {
+    const uint32_t ny = static_cast<uint32_t>(info[5] - info[2] + 1), nz = static_cast<uint32_t>(info[6] - info[3] + 1);
+    v.w = static_cast<int32_t>(k & 31u);
+    k >>= 5;
+    v.z = info[3] + static_cast<int32_t>(k % nz);
+    k /= nz;
+    v.y = info[2] + static_cast<int32_t>(k % ny);
+    v.x = info[1] + static_cast<int32_t>(k / ny);
+}
+// The device's probe count after this frame's new probes.
+__global__ void commit_kernel(int32_t* __restrict__ info, int32_t host_base) {
+    if (threadIdx.x != 0) return;
+    if (host_base >= 0) info[kInfoCount] = host_base + info[7];
+    else if (!spec_off(info)) info[kInfoCount] = info[kInfoBase] + info[7];
 }
 
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, size_t n, int32_t v) {
@@ -291,23 +375,53 @@ struct KeyDecode {
     }
 };
 
+// Flags the unique voxels without a probe. With spec (the speculative
+// build), the count and the key decoder come from the device's counters.
 __global__ void unindexed_kernel(const uint64_t* __restrict__ uniq, uint32_t n, KeyDecode dec, VolumeView vv,
-                                 uint32_t* __restrict__ flags) {
+                                 uint32_t* __restrict__ flags, const int32_t* __restrict__ spec) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int64_t s = vv.slot(dec(uniq[i]));
+    ap_int4 v;
+    if (spec) {
+        if (spec_off(spec) || i >= static_cast<uint32_t>(spec[0])) return;
+        narrow_decode(spec, uniq[i], v);
+    } else {
+        if (i >= n) return;
+        v = dec(uniq[i]);
+    }
+    const int64_t s = vv.slot(v);
     flags[i] = (s >= 0 && vv.volume[s] < 0) ? 1u : 0u;
 }
 
+// Numbers the unindexed voxels from base in sorted order (info[7] takes
+// their count). With spec, n and base come from the device's counters.
 __global__ void assign_kernel(const uint64_t* __restrict__ uniq, const uint32_t* __restrict__ flags,
                               const uint32_t* __restrict__ ranks, uint32_t n, KeyDecode dec, VolumeView vv,
-                              int32_t base, int32_t* __restrict__ info) {
+                              int32_t base, int32_t* __restrict__ info, bool spec) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (spec) {
+        if (spec_off(info)) return;
+        n = static_cast<uint32_t>(info[0]);
+        base = info[kInfoBase];
+    }
     if (i >= n) return;
     if (i == n - 1) info[7] = static_cast<int32_t>(ranks[i] + flags[i]);
     if (!flags[i]) return;
-    const int64_t s = vv.slot(dec(uniq[i]));
+    ap_int4 v;
+    if (spec) narrow_decode(info, uniq[i], v);
+    else v = dec(uniq[i]);
+    const int64_t s = vv.slot(v);
     vv.volume[s] = base + static_cast<int32_t>(ranks[i]);
+}
+
+// Zeroes the new probes [info[13], info[13] + info[7]) of a speculative
+// build (grid-stride).
+__global__ void zero_new_spec_kernel(ap_octahedron* __restrict__ probes, const int32_t* __restrict__ info) {
+    if (spec_off(info)) return;
+    const size_t words = static_cast<size_t>(info[7]) * (sizeof(ap_octahedron) / 16);
+    uint4* p = reinterpret_cast<uint4*>(probes + info[kInfoBase]);
+    for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < words;
+         i += static_cast<size_t>(gridDim.x) * kBlock)
+        p[i] = uint4{0u, 0u, 0u, 0u};
 }
 
 // Zeroes probes [base, base + info[7]) (the probes assign_kernel created).
@@ -325,9 +439,16 @@ __global__ void zero_probes_kernel(ap_octahedron* __restrict__ p, size_t n) {
     reinterpret_cast<uint4*>(p)[i] = uint4{0u, 0u, 0u, 0u};
 }
 
+// (probe, leaf texel) key of each valid slot; with spec, over the first
+// min(n, m) slots and only while the build is the device's.
 __global__ void texel_keys_kernel(const ap_path_guiding_sample* __restrict__ slots, const uint32_t* __restrict__ list,
-                                  uint32_t n, VolumeView vv, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                  uint32_t n, VolumeView vv, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                  const int32_t* __restrict__ spec) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (spec) {
+        if (spec_off(spec)) return;
+        n = live_count(n, &spec[9]);
+    }
     if (i >= n) return;
     const uint32_t slot = list[i];
     const ap_path_guiding_sample s = slots[slot];
@@ -349,8 +470,13 @@ __global__ void texel_keys_kernel(const ap_path_guiding_sample* __restrict__ slo
 
 // Head of each (probe, texel) run sums its samples in slot order.
 __global__ void texel_sums_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t n,
-                                  const ap_path_guiding_sample* __restrict__ slots, ap_octahedron* __restrict__ temp) {
+                                  const ap_path_guiding_sample* __restrict__ slots, ap_octahedron* __restrict__ temp,
+                                  const int32_t* __restrict__ spec) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (spec) {
+        if (spec_off(spec)) return;
+        n = live_count(n, &spec[9]);
+    }
     if (i >= n) return;
     const uint64_t k = keys[i];
     if (k == kNoKey || (i > 0 && keys[i - 1] == k)) return;
@@ -371,33 +497,72 @@ __global__ void texel_sums_kernel(const uint64_t* __restrict__ keys, const uint3
     atomicAdd(&t.count, cnt);
 }
 
-// MergeProbeKernel (guiding.cuh:436-506): one 64-lane workgroup per probe.
-__global__ __launch_bounds__(64) void merge_kernel(ap_octahedron* __restrict__ probes, ap_octahedron* __restrict__ temp) {
-    ap_octahedron& od = probes[blockIdx.x];
-    ap_octahedron& nw = temp[blockIdx.x];
+// Lane src's texel of a 64-lane probe block.
+__device__ __forceinline__ ap_float4 texel_from(const ap_float4& t, int src) {
+    return ap_float4{__shfl(t.x, src, 64), __shfl(t.y, src, 64), __shfl(t.z, src, 64), __shfl(t.w, src, 64)};
+}
+// (((a + b) + c) + d) * 0.25, as GenerateMipmaps averages four texels.
+__device__ __forceinline__ ap_float4 quad_mean(const ap_float4& a, const ap_float4& b, const ap_float4& c,
+                                               const ap_float4& d) {
+    return ap_float4{(((a.x + b.x) + c.x) + d.x) * 0.25f, (((a.y + b.y) + c.y) + d.y) * 0.25f,
+                     (((a.z + b.z) + c.z) + d.z) * 0.25f, (((a.w + b.w) + c.w) + d.w) * 0.25f};
+}
+
+// One probe's FP64 count-weighted merge of this frame's texel sums, then its
+// mips (pg_generate_mipmaps: leaves clamped to PG_PROBE_MIN, each mip texel
+// the mean of its four children), one leaf texel per lane and the mips
+// reduced across lanes with the same operations in the same order.
+__device__ void merge_probe(ap_octahedron& od, ap_octahedron& nw) {
     const unsigned long long cnew = nw.count;
     if (cnew == 0ull) return;
     const unsigned long long cold = od.count;
     const double inv_den = 1.0 / static_cast<double>(cold + cnew);
     const double alpha = static_cast<double>(cold) * inv_den;
     const int k = threadIdx.x;
-    {
-        ap_float4 o = od.data[k];
-        const ap_float4 n = nw.data[k];
-        o.x = static_cast<float>(static_cast<double>(o.x) * alpha + static_cast<double>(n.x) * inv_den);
-        o.y = static_cast<float>(static_cast<double>(o.y) * alpha + static_cast<double>(n.y) * inv_den);
-        o.z = static_cast<float>(static_cast<double>(o.z) * alpha + static_cast<double>(n.z) * inv_den);
-        od.data[k] = o;
-        nw.data[k] = ap_float4{0.0f, 0.0f, 0.0f, 0.0f};
-    }
-    __syncthreads();
+    ap_float4 o = od.data[k];
+    const ap_float4 n = nw.data[k];
+    o.x = static_cast<float>(static_cast<double>(o.x) * alpha + static_cast<double>(n.x) * inv_den);
+    o.y = static_cast<float>(static_cast<double>(o.y) * alpha + static_cast<double>(n.y) * inv_den);
+    o.z = static_cast<float>(static_cast<double>(o.z) * alpha + static_cast<double>(n.z) * inv_den);
+    nw.data[k] = ap_float4{0.0f, 0.0f, 0.0f, 0.0f};
+    o.x = fmaxf(o.x, PG_PROBE_MIN);
+    o.y = fmaxf(o.y, PG_PROBE_MIN);
+    o.z = fmaxf(o.z, PG_PROBE_MIN);
+    o.w = fmaxf(o.w, PG_PROBE_MIN);
+    od.data[k] = o;
+    // 8x8 -> 4x4 (lanes 0..15), 4x4 -> 2x2 (lanes 0..3), 2x2 -> 1x1 (lane 0).
+    const int x1 = k & 3, y1 = (k >> 2) & 3;
+    const int a1 = (2 * y1) * PG_RES + 2 * x1;
+    const ap_float4 m1 = quad_mean(texel_from(o, a1), texel_from(o, a1 + 1), texel_from(o, a1 + PG_RES),
+                                   texel_from(o, a1 + PG_RES + 1));
+    if (k < 16) od.data[PG_RES2 + k] = m1;
+    const int x2 = k & 1, y2 = (k >> 1) & 1;
+    const int a2 = (2 * y2) * 4 + 2 * x2;
+    const ap_float4 m2 = quad_mean(texel_from(m1, a2), texel_from(m1, a2 + 1), texel_from(m1, a2 + 4),
+                                   texel_from(m1, a2 + 5));
+    if (k < 4) od.data[PG_RES2 + 16 + k] = m2;
+    const ap_float4 m3 = quad_mean(texel_from(m2, 0), texel_from(m2, 1), texel_from(m2, 2), texel_from(m2, 3));
     if (k == 0) {
+        od.data[PG_RES2 + 20] = m3;
+        od.data[PG_TOTAL] = ap_float4{0.0f, 0.0f, 0.0f, 0.0f};
         const unsigned long long sum = cold + cnew;
         od.count = sum < PG_MAX_COUNT ? sum : PG_MAX_COUNT;
         nw.count = 0ull;
-        pg_generate_mipmaps(od);
     }
 }
+
+// MergeProbeKernel (guiding.cuh:436-506): one 64-lane workgroup per probe
+// of [0, n), grid-stride; with spec, n is the device's probe count and the
+// merge runs only while the build is the device's.
+__global__ __launch_bounds__(64) void merge_kernel(ap_octahedron* __restrict__ probes, ap_octahedron* __restrict__ temp,
+                                                   uint32_t n, const int32_t* __restrict__ spec) {
+    if (spec) {
+        if (spec_off(spec)) return;
+        n = static_cast<uint32_t>(spec[kInfoCount]);
+    }
+    for (uint32_t p = blockIdx.x; p < n; p += gridDim.x) merge_probe(probes[p], temp[p]);
+}
+
 
 template <class T>
 int grow(T** p, size_t* cap, size_t need, const char* what) {
@@ -489,12 +654,13 @@ VolumeView view_of(const GuideDevice* g) {
 
 void guide_free(GuideDevice* g) {
     if (!g) return;
-    void* ptrs[] = {g->probes, g->temp, g->volume, g->recs, g->recs2, g->slots, g->nvalid, g->keys[0], g->keys[1], g->vals[0],
+    void* ptrs[] = {g->probes, g->temp, g->volume, g->recs, g->recs2, g->slots, g->nvalid, g->bounds_part, g->keys[0], g->keys[1], g->vals[0],
                     g->vals[1], g->flags, g->ranks, g->list, g->uniq, g->tmp, g->info};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (g->info_host) (void)hipHostFree(g->info_host);
     if (g->ev_count) (void)hipEventDestroy(g->ev_count);
+    if (g->ev_flag) (void)hipEventDestroy(g->ev_flag);
     delete g;
 }
 
@@ -509,16 +675,24 @@ int guide_prepare(GuideDevice** pg, size_t n_paths, int sets) {
     if (!g->info) {
         hipError_t e = hipMalloc(reinterpret_cast<void**>(&g->info), 16 * sizeof(int32_t));
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&g->info_host), 16 * sizeof(int32_t));
+        if (e == hipSuccess) e = hipMemset(g->info, 0, 16 * sizeof(int32_t));   // no probes on the device yet
+        if (e == hipSuccess)
+            e = hipMalloc(reinterpret_cast<void**>(&g->bounds_part), kCompactBlocks * 8 * sizeof(int32_t));
         if (e != hipSuccess) return hip_fail(e, "guiding: info");
     }
     return AP_OK;
 }
 
+// Probes, index volume and frame set 0's records. Needs no wait: the
+// pointers and bounds change only on the host (a volume or probe growth),
+// and an index volume exists once a probe does, so the probe count the
+// kernels test for zero is max(count, 1) with a volume (the host's count
+// may lag the device's by the last update).
 GuideView guide_view(GuideDevice* g) {
     GuideView v{};
-    if (!g || settle(g) != AP_OK) return v;
+    if (!g) return v;
     v.probes = g->probes;
-    v.probe_count = g->count;
+    v.probe_count = g->volume ? std::max(g->count, 1) : 0;
     v.volume = g->volume;
     for (int a = 0; a < 3; ++a) {
         v.start[a] = g->start[a];
@@ -541,16 +715,21 @@ int guide_reset(GuideDevice* g) {
     g->volume = nullptr;
     for (int a = 0; a < 3; ++a) g->start[a] = g->end[a] = g->size[a] = 0;
     g->count = 0;
+    if (g->info) {
+        const int32_t zero = 0;
+        const hipError_t e = hipMemcpy(g->info + kInfoCount, &zero, sizeof(zero), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "guiding: reset");
+    }
     return AP_OK;
 }
 
-// Takes in the new-probe count of the last update (a wait on its readback,
-// which the stream has normally long passed).
+// Takes in the device's probe count after the last update (a wait on its
+// read-back, which the stream has normally long passed).
 int settle(GuideDevice* g) {
     if (!g || !g->pending) return AP_OK;
     const hipError_t e = hipEventSynchronize(g->ev_count);
     if (e != hipSuccess) return hip_fail(e, "guiding: read new probes");
-    g->count += g->info_host[7];
+    g->count = g->info_host[kInfoCount];
     g->pending = false;
     return AP_OK;
 }
@@ -576,7 +755,14 @@ hipError_t unique_voxels(GuideDevice* g, uint32_t cap, bool wide, hipStream_t st
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(unique_voxels_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], g->flags, g->ranks, cap,
                        g->info + 9, g->uniq, g->info);
-    return hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    return hipGetLastError();
+}
+
+// Probes the texel sort must number: up to `bound`.
+int texel_end_bit(int64_t bound) {
+    int probe_bits = 1;
+    while ((1ll << probe_bits) <= bound) ++probe_bits;
+    return std::min(64, 6 + probe_bits + 1);
 }
 
 int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
@@ -585,11 +771,11 @@ int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
     if (n_slots == 0) return AP_OK;
     if (n_slots > 0xffffffffull) return fail(AP_ERR_UNSUPPORTED, "guiding: too many sample slots in one frame");
     const uint32_t n = static_cast<uint32_t>(n_slots);
-    int rc = settle(g);
-    if (rc != AP_OK) return rc;
+    int rc;
     if ((rc = ensure_sort(g, n_slots)) != AP_OK) return rc;
     if (!g->ev_count) {
-        const hipError_t e = hipEventCreateWithFlags(&g->ev_count, hipEventDisableTiming);
+        hipError_t e = hipEventCreateWithFlags(&g->ev_count, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_flag, hipEventDisableTiming);
         if (e != hipSuccess) return hip_fail(e, "guiding: event");
     }
     hipLaunchKernelGGL(init_info_kernel, dim3(1), dim3(64), 0, st, g->info);
@@ -598,12 +784,54 @@ int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
     const uint32_t np = static_cast<uint32_t>(n_paths);
     hipError_t e = scan_exclusive_u32(g->nvalid, g->ranks, np, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: slot scan");
-    hipLaunchKernelGGL(compact_valid_kernel, dim3(std::min(kCompactBlocks, grid_for(np).x)), dim3(kBlock), 0, st,
-                       g->slots, g->nvalid, g->ranks, np, g->list, g->info);
+    const unsigned cblocks = std::min(kCompactBlocks, grid_for(np).x);
+    hipLaunchKernelGGL(compact_valid_kernel, dim3(cblocks), dim3(kBlock), 0, st, g->slots, g->nvalid, g->ranks, np,
+                       g->list, g->info, g->bounds_part);
+    hipLaunchKernelGGL(bounds_reduce_kernel, dim3(1), dim3(kBlock), 0, st, g->bounds_part, cblocks, g->info);
     // 1. unique voxels in sorted order, launched for `cap` slots and run
     // over the first min(m, cap).
     const uint32_t cap = std::min(n, std::max(g->sort_hint, 1u << 16));
     if ((e = unique_voxels(g, cap, false, st)) != hipSuccess) return hip_fail(e, "guiding: voxel sort");
+    // 2-4 on the device alone when the index volume needs no growth and the
+    // probes have room (check_kernel), with every count read on the device;
+    // otherwise these kernels stand down and guide_update_end finishes the
+    // build on the host's counts. The host reads the check's result back
+    // right after it, while the device goes on.
+    VolumeBox box;
+    for (int a = 0; a < 3; ++a) {
+        box.start[a] = g->start[a];
+        box.end[a] = g->end[a];
+    }
+    hipLaunchKernelGGL(check_kernel, dim3(1), dim3(64), 0, st, g->info, cap, g->volume ? 1 : 0, box, g->cap);
+    e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(g->ev_flag, st);
+    if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
+    int32_t* info = g->info;
+    const uint32_t* n_uniq_dev = reinterpret_cast<const uint32_t*>(info);
+    const VolumeView vv = view_of(g);
+    hipLaunchKernelGGL(unindexed_kernel, grid_for(cap), dim3(kBlock), 0, st, g->uniq, cap, KeyDecode{}, vv, g->flags,
+                       info);
+    if ((e = scan_exclusive_u32(g->flags, g->ranks, cap, nullptr, g->tmp, st, n_uniq_dev)) != hipSuccess)
+        return hip_fail(e, "guiding: probe scan");
+    hipLaunchKernelGGL(assign_kernel, grid_for(cap), dim3(kBlock), 0, st, g->uniq, g->flags, g->ranks, cap,
+                       KeyDecode{}, vv, 0, info, true);
+    hipLaunchKernelGGL(commit_kernel, dim3(1), dim3(64), 0, st, info, -1);
+    hipLaunchKernelGGL(zero_new_spec_kernel, dim3(1024), dim3(kBlock), 0, st, g->probes, info);
+    hipLaunchKernelGGL(texel_keys_kernel, grid_for(cap), dim3(kBlock), 0, st, g->slots, g->list, cap, vv, g->keys[0],
+                       g->vals[0], info);
+    e = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], cap, 0, texel_end_bit(g->cap), g->tmp,
+                             st, reinterpret_cast<const uint32_t*>(info + 9));
+    if (e != hipSuccess) return hip_fail(e, "guiding: texel sort");
+    hipLaunchKernelGGL(texel_sums_kernel, grid_for(cap), dim3(kBlock), 0, st, g->keys[1], g->vals[1], cap, g->slots,
+                       g->temp, info);
+    const uint32_t merge_grid = static_cast<uint32_t>(std::max<int64_t>(
+        1, std::min<int64_t>(g->cap, std::max<int64_t>(4096, 2ll * g->count + 1024))));
+    hipLaunchKernelGGL(merge_kernel, dim3(merge_grid), dim3(64), 0, st, g->probes, g->temp, 0u, info);
+    e = hipMemcpyAsync(g->info_host + kInfoCount, info + kInfoCount, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(g->ev_count, st);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "guiding: update");
+    g->pending = true;
     g->open = true;
     g->upd_n = n;
     g->upd_cap = cap;
@@ -614,15 +842,24 @@ int guide_update_end(GuideDevice* g, hipStream_t st) {
     if (!g->open) return AP_OK;
     g->open = false;
     int rc;
-    hipError_t e;
+    hipError_t e = hipEventSynchronize(g->ev_flag);
+    if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
     if (g->info_host[8]) return fail(AP_ERR_UNSUPPORTED, "guiding: a voxel coordinate is outside [-2^18, 2^18)");
     const uint32_t m = static_cast<uint32_t>(g->info_host[9]);
     g->sort_hint = static_cast<uint32_t>(std::min<uint64_t>(g->upd_n, 2ull * m));
+    if (g->info_host[kInfoHost] == 0) return AP_OK;  // the device finishes the build (or there is nothing to build)
+    // The host finishes it: wait for the stood-down kernels, take the
+    // device's probe count (unchanged), then grow what must grow.
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "guiding: read counts");
+    g->count = g->info_host[kInfoCount];
+    g->pending = false;
     if (m == 0) return AP_OK;
     if (m > g->upd_cap || g->info_host[10]) {
         // More valid slots than the launch covered, or a voxel range too
         // wide for 32-bit keys: sort them all again (with absolute keys).
-        if ((e = unique_voxels(g, m, g->info_host[10] != 0, st)) == hipSuccess) e = hipStreamSynchronize(st);
+        if ((e = unique_voxels(g, m, g->info_host[10] != 0, st)) == hipSuccess)
+            e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(e, "guiding: voxel sort");
     }
     KeyDecode dec{};
@@ -676,15 +913,16 @@ int guide_update_end(GuideDevice* g, hipStream_t st) {
     // n_uniq new ones, so room for them is made before their number is known.
     if ((rc = grow_probes(g, g->count + static_cast<int32_t>(n_uniq), st)) != AP_OK) return rc;
     hipLaunchKernelGGL(unindexed_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, n_uniq, dec, view_of(g),
-                       g->flags);
+                       g->flags, nullptr);
     e = scan_exclusive_u32(g->flags, g->ranks, n_uniq, nullptr, g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: probe scan");
     hipLaunchKernelGGL(assign_kernel, grid_for(n_uniq), dim3(kBlock), 0, st, g->uniq, g->flags, g->ranks, n_uniq,
-                       dec, view_of(g), g->count, g->info);
+                       dec, view_of(g), g->count, g->info, false);
+    hipLaunchKernelGGL(commit_kernel, dim3(1), dim3(64), 0, st, g->info, g->count);
     hipLaunchKernelGGL(zero_new_probes_kernel,
                        grid_for(static_cast<size_t>(n_uniq) * (sizeof(ap_octahedron) / 16)), dim3(kBlock), 0, st,
                        g->probes + g->count, static_cast<size_t>(n_uniq), g->info);
-    e = hipMemcpyAsync(g->info_host + 7, g->info + 7, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(g->info_host + kInfoCount, g->info + kInfoCount, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipEventRecord(g->ev_count, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: new probes");
     // Probes [count, count + n_uniq) cover the new ones; those past them have
@@ -692,26 +930,22 @@ int guide_update_end(GuideDevice* g, hipStream_t st) {
     const int32_t bound = g->count + static_cast<int32_t>(n_uniq);
     // 3. per (probe, texel) sums in slot order.
     hipLaunchKernelGGL(texel_keys_kernel, grid_for(m), dim3(kBlock), 0, st, g->slots, g->list, m, view_of(g),
-                       g->keys[0], g->vals[0]);
-    int probe_bits = 1;
-    while ((1ll << probe_bits) <= bound) ++probe_bits;
-    const int end_bit = std::min(64, 6 + probe_bits + 1);
-    e = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, end_bit, g->tmp, st);
+                       g->keys[0], g->vals[0], nullptr);
+    e = radix_sort_pairs_u64(g->keys[0], g->keys[1], g->vals[0], g->vals[1], m, 0, texel_end_bit(bound), g->tmp, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: texel sort");
     hipLaunchKernelGGL(texel_sums_kernel, grid_for(m), dim3(kBlock), 0, st, g->keys[1], g->vals[1], m, g->slots,
-                       g->temp);
+                       g->temp, nullptr);
     // 4. FP64 merge and mips (a probe without samples this frame is left as it is).
-    hipLaunchKernelGGL(merge_kernel, dim3(static_cast<unsigned>(bound)), dim3(64), 0, st, g->probes, g->temp);
+    hipLaunchKernelGGL(merge_kernel, dim3(static_cast<unsigned>(bound)), dim3(64), 0, st, g->probes, g->temp,
+                       static_cast<uint32_t>(bound), nullptr);
     g->pending = true;
     e = hipGetLastError();
     return e == hipSuccess ? AP_OK : hip_fail(e, "guiding: update");
 }
 
 int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
-    int rc = guide_update_begin(g, n_paths, st);
+    const int rc = guide_update_begin(g, n_paths, st);
     if (rc != AP_OK || !g->open) return rc;
-    const hipError_t e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
     return guide_update_end(g, st);
 }
 

@@ -2613,17 +2613,28 @@ int batch_guides_update(BatchState* b, hipStream_t stream) {
     for (size_t k = 0; k < b->scenes.size(); ++k)
         if ((rc = guide_update_begin(b->scenes[k]->guide, b->pix_begin[k + 1] - b->pix_begin[k], stream)) != AP_OK)
             return rc;
-    hipError_t e = hipStreamSynchronize(stream);  // one wait for every scene's counts
-    if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding counts");
+    // Each end waits for its scene's check only; a scene whose volume or
+    // probe storage had to grow is finished on the host there.
     for (size_t k = 0; k < b->scenes.size(); ++k)
         if ((rc = guide_update_end(b->scenes[k]->guide, stream)) != AP_OK) return rc;
+    // The scene records change only when a scene's probes, volume or bounds
+    // were reallocated, or its first probes appeared.
+    bool changed = false;
     for (size_t k = 0; k < b->scenes.size(); ++k) {
-        b->h_scenes[k].guide = guide_view(b->scenes[k]->guide);
+        const GuideView v = guide_view(b->scenes[k]->guide);
+        GuideView& o = b->h_scenes[k].guide;
+        bool same = v.probes == o.probes && v.volume == o.volume && (v.probe_count > 0) == (o.probe_count > 0);
+        for (int a = 0; a < 3; ++a) same = same && v.start[a] == o.start[a] && v.size[a] == o.size[a];
+        if (!same) {
+            o = v;
+            changed = true;
+        }
         b->h_scenes[k].guide_slots = guide_slots(b->scenes[k]->guide);
         b->h_scenes[k].guide_nvalid = guide_nvalid(b->scenes[k]->guide);
     }
-    e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
-                       hipMemcpyHostToDevice, stream);
+    if (!changed) return AP_OK;
+    hipError_t e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
+                                  hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: guiding records");
 }
