@@ -1632,12 +1632,15 @@ __global__ __launch_bounds__(kBlock) void nrc_apply_batch_kernel(BatchView bv, u
                                                                  const f4* __restrict__ rgb_lut,
                                                                  const f4* __restrict__ lam_lut,
                                                                  const float* __restrict__ tables,
-                                                                 PathT<NL>* __restrict__ paths) {
+                                                                 PathT<NL>* __restrict__ paths,
+                                                                 GuideRec* __restrict__ recs = nullptr) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= total) return;
     const int32_t sc = batch_scene_of(bv.pix_begin, bv.n_scenes, i);
     if (i - bv.pix_begin[sc] >= qcount[sc]) return;
-    nrc_apply_one<NL>(paths[qpath[i]], f3{rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]}, rgb_lut, lam_lut, tables);
+    const int32_t pid = qpath[i];
+    nrc_apply_one<NL>(paths[pid], f3{rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]}, rgb_lut, lam_lut, tables,
+                      recs ? recs + pid : nullptr);
 }
 
 // nrc_targets_kernel over the training slots of every scene of a batch;
@@ -2857,7 +2860,7 @@ int batch_nrc_prepare(BatchState* b, const ap_nrc_render_params* np, NrcState* n
 // counter. Everything that changes from frame to frame (the frame hash, the
 // training draws, Adam's bias corrections) is read on the device, so the
 // sequence can be captured once and replayed.
-template <int NL>
+template <int NL, bool GUIDE = false>
 int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_render_params* np, NrcState* ns,
                     hipStream_t stream, const f4* rgb_lut, const f4* lam_lut, const float* tables) {
     const BatchView bv = b->view();
@@ -2865,9 +2868,13 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
     auto* pays = reinterpret_cast<ShadowPay<NL>*>(b->shadow_payload);
     const uint32_t total = b->total;
     const int32_t n = static_cast<int32_t>(b->scenes.size());
+    // Guided: each path samples its scene's probes (the scene records carry
+    // them) and collects its guiding samples in the batch's records.
+    GuideView gv{};
+    if constexpr (GUIDE) gv.recs = b->guide_recs;
     hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, b->counters, total, 0u, 0u);
     hipLaunchKernelGGL(generate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, prm->seed,
-                       b->eps, paths, b->rays[0], b->path_of[0], b->media);
+                       b->eps, paths, b->rays[0], b->path_of[0], b->media, gv.recs);
     hipLaunchKernelGGL(nrc_init_batch_kernel, grid_of(total), dim3(kBlock), 0, stream, bv, total, b->nrc_table,
                        prm->seed, np->unbiased_every, b->nrc_paths, b->nrc_counters, b->nrc_qcount);
     const NrcView nv{b->nrc_paths, b->nrc_queries, b->nrc_qpath, b->nrc_verts, b->nrc_counters, np->spread,
@@ -2886,10 +2893,10 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
         ShadeParams sp{depth, prm->max_depth, prm->rr_min_depth, prm->nee_max_depth, prm->use_nee,
                        prm->nee_always, prm->sky, in_q, out_q, 0};
         sp.media = b->media;
-        hipLaunchKernelGGL((shade_kernel<NL, false, true, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
+        hipLaunchKernelGGL((shade_kernel<NL, GUIDE, true, true>), grid_of(total), dim3(kBlock), 0, stream, sp,
                            nullptr, nullptr, nullptr, nullptr, rgb_lut, lam_lut, tables, b->eps, paths,
                            b->rays[in_q], b->path_of[in_q], b->hits, b->rays[out_q], b->path_of[out_q],
-                           b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, GuideView{}, nv, bv);
+                           b->shadow_rays, pays, b->counters, nullptr, 0, b->rays_traced, gv, nv, bv);
         e = hipEventRecord(b->ev_shaded, stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_shaded, 0);
         if (e == hipSuccess)
@@ -2914,7 +2921,7 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
                               b->nrc_tiles * 64u, b->nrc_qrgb, stream);
     if (rc != AP_OK) return rc;
     hipLaunchKernelGGL(nrc_apply_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total,
-                       b->nrc_qcount, b->nrc_qpath, b->nrc_qrgb, rgb_lut, lam_lut, tables, paths);
+                       b->nrc_qcount, b->nrc_qpath, b->nrc_qrgb, rgb_lut, lam_lut, tables, paths, gv.recs);
     const uint32_t slots = b->nrc_slots;
     if (slots > 0) {
         hipLaunchKernelGGL(nrc_targets_batch_kernel<NL>, grid_of(slots), dim3(kBlock), 0, stream, slots, bv,
@@ -2929,7 +2936,7 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
                            b->nrc_offsets, b->nrc_verts, b->nrc_pool, b->nrc_slot_pix);
     }
     hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
-                       tables);
+                       tables, gv.recs);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "ap_batch: launch");
     if (slots > 0 && np->train_steps > 0 &&
@@ -3201,8 +3208,6 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
         return fail(AP_ERR_INVALID_ARG, "ap_batch_render_nrc: spread must be > 0 and learning_rate >= 0");
     if (nrc_params->train_passes != 0)
         return fail(AP_ERR_UNSUPPORTED, "ap_batch_render_nrc: training-only passes are not supported in a batch");
-    if (params->use_guiding)
-        return fail(AP_ERR_UNSUPPORTED, "ap_batch_render_nrc: guiding with per-scene caches is not supported");
     if (nrc_params->train_steps > 0 &&
         (nrc_params->batch == 0 || nrc_params->batch % 256 != 0 ||
          static_cast<uint64_t>(nrc_params->batch) * B.scenes.size() > (1u << 24)))
@@ -3213,7 +3218,8 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
     const float* tables = nullptr;
     if ((rc = spectrum_get(&rgb_lut, &lam_lut, &tables)) != AP_OK) return rc;
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut, st)) != AP_OK) return rc;
+    const bool guiding = params->use_guiding != 0;
+    if ((rc = batch_prepare(&B, params->triangle_material, hdr, sums, rgb_lut, st, guiding)) != AP_OK) return rc;
     if ((rc = batch_nrc_prepare(&B, nrc_params, ns)) != AP_OK) return rc;
     hipError_t e = hipMemsetAsync(B.rays_traced, 0, 2 * sizeof(unsigned long long), st);
     if (e == hipSuccess) {
@@ -3226,12 +3232,22 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
     // buffer it needs); the others replay one captured frame, cached per
     // render parameters, cache parameters, buffers, launch shape and stream.
     auto frame = [&]() {
-        return params->wavelengths == 4 ? batch_frame_nrc<4>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables)
-                                        : batch_frame_nrc<1>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables);
+        const bool four = params->wavelengths == 4;
+        if (guiding)
+            return four ? batch_frame_nrc<4, true>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables)
+                        : batch_frame_nrc<1, true>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables);
+        return four ? batch_frame_nrc<4>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables)
+                    : batch_frame_nrc<1>(&B, params, nrc_params, ns, st, rgb_lut, lam_lut, tables);
     };
     const int32_t steps = (B.nrc_slots > 0) ? static_cast<int32_t>(nrc_params->train_steps) : 0;
     hipGraphExec_t exec = nullptr;
     for (int32_t f = 0; f < params->spp; ++f) {
+        if (guiding) {
+            // Guided: every frame directly, each scene's probes rebuilt from
+            // its own samples after the frame.
+            if ((rc = frame()) != AP_OK || (rc = batch_guides_update(&B, st)) != AP_OK) return rc;
+            continue;
+        }
         if (f == 0 || !B.use_graph || st == nullptr) {
             if ((rc = frame()) != AP_OK) return rc;
             continue;

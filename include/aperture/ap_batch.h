@@ -58,10 +58,15 @@ int ap_batch_render(ap_batch* batch, const ap_render_params* params, float* cons
  * batch rows per network, batch x n at most 2^24) in the same launches. With
  * a width-64 BF16 cache, scene k's image and network k's weights equal, bit
  * for bit, ap_render_nrc of scene k alone with a one-network cache that
- * starts from network k. train_passes must be 0 and use_guiding 0. The
- * call's first frame is launched directly, the others replay one captured
- * graph (the frame index, the frame's training draws and Adam's bias
- * corrections are read on the device). */
+ * starts from network k. train_passes must be 0. With use_guiding, each
+ * scene is also guided by its own probes, a path that ends into its scene's
+ * cache gives the probes the prediction as its endpoint radiance, and every
+ * scene's probes are rebuilt from its own samples after every frame; scene k
+ * then equals ap_render_nrc of scene k alone, guided, probes included.
+ * Unguided, the call's first frame is launched directly and the others
+ * replay one captured graph (the frame index, the frame's training draws and
+ * Adam's bias corrections are read on the device); guided frames are all
+ * launched directly. */
 int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc* cache,
                         const ap_nrc_render_params* nrc_params, float* const* hdr, double* const* sums,
                         uint64_t ray_count[2], void* stream);

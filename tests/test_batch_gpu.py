@@ -97,14 +97,11 @@ def test_autotuner_keeps_images(gpu):
     check_against_single(scenes, cams, prm, got, "autotuned")
     with pytest.raises(ap.ApertureError):
         batch.set_launch(100, 8)
-    # Guiding with per-scene caches is refused (a guided plain batch is
-    # tested below).
-    cache = ap.RadianceCache(seed=1, groups=len(cams), width=32, precision="fp16")
+    # Guiding in groups of frames is refused in a batch (a guided batch
+    # updates every scene's probes after every frame, tested below).
     with pytest.raises(ap.ApertureError) as e:
-        batch.render_nrc(ap.RenderParams.defaults(use_guiding=1), cache, ap.NrcRenderParams.defaults(),
-                         [None] * len(cams), stream=stream)
+        batch.render(ap.RenderParams.defaults(use_guiding=1, guide_frames=2), [None] * len(cams), stream=stream)
     assert e.value.code == ap.ERR_UNSUPPORTED
-    cache.close()
     batch.close()
 
 

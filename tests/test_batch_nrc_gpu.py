@@ -90,6 +90,50 @@ def test_batch_caches_equal_single_scene_caches(gpu, steps):
     cache.close()
 
 
+@pytest.mark.parametrize("nl", [1, 4])
+def test_guided_batch_caches_equal_single_scene_caches(gpu, nl):
+    """Guided batches with per-scene caches: each scene samples its own
+    probes, a path that ends into its scene's cache gives the probes the
+    prediction as its endpoint radiance, and each scene's probes are rebuilt
+    from its own samples after every frame. Images, FP64 sums, trained
+    weights, per-scene counters and probes equal ap_render_nrc of each scene
+    alone, guided, with its own one-network cache."""
+    print(f"[batch-nrc] guided, width 64 BF16, {nl} wavelength(s)", flush=True)
+    objs, cams = bg.scenes_and_cameras()
+    scenes = [ap.SceneRenderer(o) for o in objs]
+    prm = ap.RenderParams.defaults(spp=3, max_depth=10, rr_min_depth=3, seed=23, use_guiding=1, wavelengths=nl)
+    nprm = ap.NrcRenderParams.defaults(train_paths=256, batch=512, train_steps=2, unbiased_every=4)
+    w = weights(len(scenes))
+    cache = ap.RadianceCache(seed=1, groups=len(scenes), width=64, precision="bf16")
+    cache.set_params(w)
+    batch = ap.Batch(scenes, cams)
+    stream = ap.Stream()
+    got = batch_render_nrc(batch, cams, prm, cache, nprm, stream)
+    stats = batch.nrc_last_frame()
+    after = cache.params()
+    probes = [sc.guiding_export() for sc in scenes]
+    for sc in scenes:
+        sc.guiding_reset()
+    for k, (sc, cam) in enumerate(zip(scenes, cams)):
+        single = ap.RadianceCache(seed=1)
+        single.set_params(w[k])
+        hdr, sums = nr.render_nrc(sc, cam, prm, single, nprm)
+        st, _ = sc.nrc_last_frame(samples=False)
+        gdata, gcount, gvol, gbounds = sc.guiding_export()
+        bdata, bcount, bvol, bbounds = probes[k]
+        print(f"[batch-nrc]   scene {k} {cam.width}x{cam.height}: queries {stats[k].queries} vs {st.queries}, "
+              f"probes {len(bcount)} vs {len(gcount)}", flush=True)
+        assert (stats[k].queries, stats[k].train_paths, stats[k].samples) == (st.queries, st.train_paths, st.samples)
+        assert np.array_equal(got[k][0], hdr) and np.array_equal(got[k][1], sums)
+        assert np.array_equal(after[k], single.params())
+        assert np.array_equal(bbounds, gbounds) and np.array_equal(bvol, gvol) and np.array_equal(bcount, gcount)
+        assert np.array_equal(bdata.view(np.uint32), gdata.view(np.uint32))
+        single.close()
+    assert sum(s.queries for s in stats) > 0 and sum(len(p[1]) for p in probes) > 0
+    batch.close()
+    cache.close()
+
+
 def test_fp16_width32_caches_train_deterministically(gpu):
     objs, cams = bg.scenes_and_cameras()
     scenes = [ap.SceneRenderer(o) for o in objs]
