@@ -478,6 +478,177 @@ __global__ __launch_bounds__(256) void backward_kernel(const uint16_t* __restric
     }
 }
 
+// Forward, relative-L2 loss and backward of 64-row tiles in one launch: the
+// forward_kernel<true>, loss_kernel and backward_kernel sequence with each
+// wave's activations kept in LDS for the backward's ReLU masks and its
+// output deltas computed where the forward ends. The MFMA operands and their
+// order are those of the three kernels, so acts, dout and deltas (stored
+// [feature][row] for the weight gradients) are bit-identical to theirs; the
+// loss is summed per 64-row tile (loss_part[tile]). Rows [g * rows_per_group,
+// ...) use network g; n is a multiple of 64.
+template <int W, bool F16>
+__global__ __launch_bounds__(256) void fwd_bwd_kernel(const uint16_t* __restrict__ w, const uint16_t* __restrict__ wt,
+                                                      Bounds bnd, const Bounds* __restrict__ bounds,
+                                                      const float* __restrict__ rec, uint32_t n,
+                                                      uint32_t rows_per_group, const uint32_t* __restrict__ gate,
+                                                      int gate_per_group, float scale, uint16_t* __restrict__ acts,
+                                                      uint16_t* __restrict__ dout, uint16_t* __restrict__ deltas,
+                                                      float* __restrict__ loss_part) {
+    using G = Geo<W>;
+    constexpr int LT = G::LT;
+    constexpr int OUT_ROW0 = 4 * W * LT;
+    __shared__ __attribute__((aligned(16))) uint16_t sw[G::ROWS * LS];
+    __shared__ __attribute__((aligned(16))) uint16_t swt[4 * W * LT + W * 40];
+    // Per wave: the input features and the five hidden activations of its 16
+    // rows; the input tile doubles as the backward's delta tile.
+    __shared__ __attribute__((aligned(16))) uint16_t sa[4][NL + 1][16 * LS];
+    __shared__ float red[256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = lane & 15, q = lane >> 4;
+    const uint32_t tiles = n / 64u;
+    uint32_t loaded = 0xffffffffu;
+    for (uint32_t v = blockIdx.x; v < tiles; v += gridDim.x) {
+        const uint32_t base = v * 64u;
+        const uint32_t g = base / rows_per_group;
+        if (g != loaded) {
+            __syncthreads();
+            load_weights<W>(w + static_cast<size_t>(g) * G::NPAR, sw);
+            const uint16_t* src = wt + static_cast<size_t>(g) * G::NPAR_T;
+            for (int e = threadIdx.x; e < (4 * W * W + W * WT_OUT_STRIDE) / 8; e += blockDim.x) {
+                const int x = e * 8;
+                if (x < 4 * W * W) {
+                    const int h = x / (W * W), rem = x % (W * W);
+                    *reinterpret_cast<u16x8*>(&swt[(h * W + rem / W) * LT + rem % W]) = ld8(&src[G::off(1) + x]);
+                } else {
+                    const int rem = x - 4 * W * W;
+                    *reinterpret_cast<u16x8*>(&swt[OUT_ROW0 + (rem / WT_OUT_STRIDE) * 40 + rem % WT_OUT_STRIDE]) =
+                        ld8(&src[G::off(NL) + rem]);
+                }
+            }
+            __syncthreads();
+            loaded = g;
+        }
+        const Bounds b = bounds ? bounds[g] : bnd;
+        const uint32_t r = base + static_cast<uint32_t>(wave) * 16u + static_cast<uint32_t>(row);
+        uint16_t* a0 = sa[wave][0];
+        {
+            const float* sp = rec + static_cast<size_t>(r) * SREC;
+            float p[3];
+            for (int a = 0; a < 3; ++a) p[a] = (sp[a] - b.lo[a]) / (b.hi[a] - b.lo[a]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int m = 4 * j + q;
+                float v0, v1;
+                encode_pair(sp, p, m, v0, v1);
+                const uint16_t lo = cvt<F16>(v0), hi = cvt<F16>(v1);
+                *reinterpret_cast<uint32_t*>(&a0[row * LS + 2 * m]) = pack2(lo, hi);
+                acts[static_cast<size_t>(2 * m) * n + r] = lo;
+                acts[static_cast<size_t>(2 * m + 1) * n + r] = hi;
+            }
+        }
+        wave_sync();
+        for (int l = 0; l < NL; ++l) {
+            const uint16_t* in = sa[wave][l];
+            uint16_t* out = sa[wave][l + 1];
+            f32x4 acc[W / 16];
+            const int ks = (l == 0 ? NF : W) / 32;
+            for (int t = 0; t < W / 16; ++t) {
+                acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int s2 = 0; s2 < ks; ++s2) {
+                    const u16x8 a = ld8(&sw[(l * W + 16 * t + row) * LS + 32 * s2 + 8 * q]);
+                    const u16x8 bb = ld8(&in[row * LS + 32 * s2 + 8 * q]);
+                    acc[t] = mfma<F16>(a, bb, acc[t]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < W / 16; ++t) {
+                uint16_t h[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float x = acc[t][k] > 0.0f ? acc[t][k] : 0.0f;
+                    h[k] = cvt<F16>(x);
+                    acts[static_cast<size_t>(G::act(l + 1) + 16 * t + 4 * q + k) * n + r] = h[k];
+                }
+                *reinterpret_cast<uint2*>(&out[row * LS + 16 * t + 4 * q]) =
+                    make_uint2(pack2(h[0], h[1]), pack2(h[2], h[3]));
+            }
+            wave_sync();
+        }
+        f32x4 y{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int s2 = 0; s2 < W / 32; ++s2) {
+            const u16x8 a = ld8(&sw[(NL * W + row) * LS + 32 * s2 + 8 * q]);
+            const u16x8 bb = ld8(&sa[wave][NL][row * LS + 32 * s2 + 8 * q]);
+            y = mfma<F16>(a, bb, y);
+        }
+        // loss_kernel's arithmetic on the row's three outputs (lanes q = 0
+        // hold them); the output deltas go to dout ([o][row], o < 16) and,
+        // as the backward's input tile, to the wave's [row][o < 32] tile.
+        float lrow = 0.0f;
+        uint16_t* del = a0;
+        {
+            const bool open = gate == nullptr || gate[gate_per_group ? g : 0u] != 0u;
+            uint16_t d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (q == 0 && open) {
+                const float norm = 1.0f / (3.0f * static_cast<float>(rows_per_group));
+                const float gnorm = norm * scale;
+                for (int o = 0; o < NOUT; ++o) {
+                    const float vv = y[o];
+                    const float tt = rec[static_cast<size_t>(r) * SREC + QREC + o];
+                    const float den = vv * vv + 0.01f;
+                    const float e = vv - tt;
+                    lrow += e * e / den * norm;
+                    d[o] = cvt<F16>(2.0f * e / den * gnorm);
+                }
+            }
+            if (q < 2)
+                for (int k = 0; k < 8; ++k) dout[static_cast<size_t>(8 * q + k) * n + r] = d[k];
+            *reinterpret_cast<uint4*>(&del[row * LS + 8 * q]) =
+                make_uint4(pack2(d[0], d[1]), pack2(d[2], d[3]), pack2(d[4], d[5]), pack2(d[6], d[7]));
+        }
+        red[threadIdx.x] = lrow;
+        __syncthreads();
+        for (int s2 = 128; s2 > 0; s2 >>= 1) {
+            if (static_cast<int>(threadIdx.x) < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) loss_part[v] = red[0];
+        // backward_kernel's delta chain, the ReLU masks from the LDS tiles.
+        f32x4 acc[W / 16];
+        for (int t = 0; t < W / 16; ++t) {
+            const u16x8 a = ld8(&swt[OUT_ROW0 + (16 * t + row) * 40 + 8 * q]);
+            const u16x8 bb = ld8(&del[row * LS + 8 * q]);
+            acc[t] = mfma<F16>(a, bb, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        }
+        wave_sync();
+        for (int h = NL - 1; h >= 0; --h) {
+            const uint16_t* act = sa[wave][h + 1];
+#pragma unroll
+            for (int t = 0; t < W / 16; ++t) {
+                uint16_t hv[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int i = 16 * t + 4 * q + k;
+                    hv[k] = positive(act[row * LS + i]) ? cvt<F16>(acc[t][k]) : uint16_t{0};
+                    deltas[static_cast<size_t>(h * W + i) * n + r] = hv[k];
+                }
+                *reinterpret_cast<uint2*>(&del[row * LS + 16 * t + 4 * q]) =
+                    make_uint2(pack2(hv[0], hv[1]), pack2(hv[2], hv[3]));
+            }
+            wave_sync();
+            if (h == 0) break;
+            for (int t = 0; t < W / 16; ++t) {
+                acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int s2 = 0; s2 < W / 32; ++s2) {
+                    const u16x8 a = ld8(&swt[((h - 1) * W + 16 * t + row) * LT + 32 * s2 + 8 * q]);
+                    const u16x8 bb = ld8(&del[row * LS + 32 * s2 + 8 * q]);
+                    acc[t] = mfma<F16>(a, bb, acc[t]);
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
 // Weight-gradient partials of one row chunk: block (chunk, layer); the
 // chunk's partial is in its group's parameter order.
 template <int W, bool F16>
@@ -715,7 +886,7 @@ int ensure_train(NrcState* s, uint32_t n) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->y), N * NOUT * sizeof(float));
     if (e == hipSuccess)
         e = hipMalloc(reinterpret_cast<void**>(&s->part), (N / kChunkRows) * s->npar * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->loss_part), ((N + 255) / 256) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->loss_part), ((N + 63) / 64) * sizeof(float));
     if (e != hipSuccess) return hip_fail(e, "ap_nrc: training buffers");
     s->cap = n;
     return AP_OK;
@@ -762,12 +933,9 @@ int nrc_gradient(NrcState* s, const float* samples, uint32_t rows, hipStream_t s
     with_net(s, [&](auto net) {
         using N = decltype(net);
         constexpr int NPAR = Geo<N::W>::NPAR;
-        hipLaunchKernelGGL((forward_kernel<true, N::W, N::F16>), dim3(blocks_for(n)), dim3(256), 0, st, s->w,
-                           s->bounds, gm, samples, SREC, n, nullptr, nullptr, s->y, s->acts, n);
-        hipLaunchKernelGGL(loss_kernel<N::F16>, dim3((n + 255) / 256), dim3(256), 0, st, s->y, samples, n, rows, n,
-                           gate, gate_per_group, scale, s->dout, s->loss_part);
-        hipLaunchKernelGGL((backward_kernel<N::W, N::F16>), dim3(blocks_for(n)), dim3(256), 0, st, s->wt, n, rows, n,
-                           s->acts, s->dout, s->deltas);
+        hipLaunchKernelGGL((fwd_bwd_kernel<N::W, N::F16>), dim3(blocks_for(n)), dim3(256), 0, st, s->w, s->wt,
+                           s->bounds, gm.bounds, samples, n, rows, gate, gate_per_group, scale, s->acts, s->dout,
+                           s->deltas, s->loss_part);
         hipLaunchKernelGGL((wgrad_kernel<N::W, N::F16>), dim3(n / kChunkRows, NL + 1), dim3(256), 0, st, n, s->acts,
                            s->deltas, s->dout, s->part);
         hipLaunchKernelGGL(reduce_grad_kernel<N::W>, dim3((s->groups * NPAR + 255) / 256), dim3(256), 0, st,
@@ -891,9 +1059,10 @@ int check_batch(const char* who, size_t n) {
     return AP_OK;
 }
 
-// Copies the loss partials of the last gradient into per-network losses.
+// Copies the loss partials (one per 64-row tile) of the last gradient into
+// per-network losses.
 hipError_t read_losses(const NrcState* s, uint32_t rows, float* loss) {
-    const uint32_t per = rows / 256, n = per * s->groups;
+    const uint32_t per = rows / 64, n = per * s->groups;
     std::vector<float> lp(n);
     const hipError_t e = hipMemcpy(lp.data(), s->loss_part, n * sizeof(float), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return e;
