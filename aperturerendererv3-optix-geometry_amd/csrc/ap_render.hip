@@ -2513,6 +2513,13 @@ struct BatchState {
     // Side stream of the overlapped shadow traces (as RenderState's).
     hipStream_t side = nullptr;
     hipEvent_t ev_shaded = nullptr, ev_shadowed = nullptr;
+    // Guided batches: the scenes' probe builds run on kGuideStreams streams
+    // (scene k on stream k mod kGuideStreams), forked from the render stream
+    // after the frame's accumulation and joined before the next frame.
+    static constexpr int kGuideStreams = 8;
+    hipStream_t gstreams[kGuideStreams] = {};
+    hipEvent_t gevents[kGuideStreams] = {};
+    hipEvent_t ev_frame = nullptr;
 
     // Waits for the frames in flight, then destroys every graph.
     void drop_graphs() {
@@ -2536,6 +2543,11 @@ struct BatchState {
         if (ev_shaded) (void)hipEventDestroy(ev_shaded);
         if (ev_shadowed) (void)hipEventDestroy(ev_shadowed);
         if (side) (void)hipStreamDestroy(side);
+        for (int k = 0; k < kGuideStreams; ++k) {
+            if (gevents[k]) (void)hipEventDestroy(gevents[k]);
+            if (gstreams[k]) (void)hipStreamDestroy(gstreams[k]);
+        }
+        if (ev_frame) (void)hipEventDestroy(ev_frame);
     }
     BatchView view() const {
         return BatchView{d_scenes, d_pix_begin, static_cast<int32_t>(scenes.size()), {ray_scene[0], ray_scene[1]},
@@ -2613,13 +2625,36 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
 // next frame.
 int batch_guides_update(BatchState* b, hipStream_t stream) {
     int rc;
-    for (size_t k = 0; k < b->scenes.size(); ++k)
-        if ((rc = guide_update_begin(b->scenes[k]->guide, b->pix_begin[k + 1] - b->pix_begin[k], stream)) != AP_OK)
+    constexpr int K = BatchState::kGuideStreams;
+    hipError_t e = hipSuccess;
+    if (!b->ev_frame) {
+        e = hipEventCreateWithFlags(&b->ev_frame, hipEventDisableTiming);
+        for (int k = 0; k < K && e == hipSuccess; ++k) {
+            e = hipStreamCreateWithFlags(&b->gstreams[k], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&b->gevents[k], hipEventDisableTiming);
+        }
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding streams");
+    }
+    const size_t n = b->scenes.size();
+    const int used = static_cast<int>(std::min<size_t>(n, K));
+    // The scenes' builds are independent (each its own buffers and
+    // counters): fork them over the guiding streams.
+    e = hipEventRecord(b->ev_frame, stream);
+    for (int k = 0; k < used && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->gstreams[k], b->ev_frame, 0);
+    if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding fork");
+    for (size_t k = 0; k < n; ++k)
+        if ((rc = guide_update_begin(b->scenes[k]->guide, b->pix_begin[k + 1] - b->pix_begin[k],
+                                     b->gstreams[k % K])) != AP_OK)
             return rc;
     // Each end waits for its scene's check only; a scene whose volume or
     // probe storage had to grow is finished on the host there.
-    for (size_t k = 0; k < b->scenes.size(); ++k)
-        if ((rc = guide_update_end(b->scenes[k]->guide, stream)) != AP_OK) return rc;
+    for (size_t k = 0; k < n; ++k)
+        if ((rc = guide_update_end(b->scenes[k]->guide, b->gstreams[k % K])) != AP_OK) return rc;
+    for (int k = 0; k < used && e == hipSuccess; ++k) {
+        e = hipEventRecord(b->gevents[k], b->gstreams[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, b->gevents[k], 0);
+    }
+    if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding join");
     // The scene records change only when a scene's probes, volume or bounds
     // were reallocated, or its first probes appeared.
     bool changed = false;
@@ -2636,8 +2671,8 @@ int batch_guides_update(BatchState* b, hipStream_t stream) {
         b->h_scenes[k].guide_nvalid = guide_nvalid(b->scenes[k]->guide);
     }
     if (!changed) return AP_OK;
-    hipError_t e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
-                                  hipMemcpyHostToDevice, stream);
+    e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
+                       hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: guiding records");
 }
