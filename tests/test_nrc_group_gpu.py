@@ -151,3 +151,28 @@ def test_grouped_cache_argument_checks(gpu):
     one = ap.RadianceCache(seed=1, groups=1, width=32, precision="fp16")
     assert one.infer(cpu.random_queries(64)).shape == (64, 3)
     one.close()
+
+
+def test_many_tiles_per_block_equal_single_networks(gpu):
+    """48 networks x 4096 rows (196,608 rows: more 64-row tiles than the
+    training kernel's 2048 blocks, so blocks walk several tiles and reload
+    weights between networks, as a 48-scene batch's training step does):
+    each network's gradient and loss equal, bit for bit, those of the same
+    network alone on its own rows."""
+    G, B = 48, 4096
+    c, p = make_cache(G, 32, "fp16")
+    rng = np.random.default_rng(61)
+    q = cpu.random_queries(G * B, seed=62).reshape(G, B, 16)
+    t = rng.uniform(0, 2, (G, B, 3)).astype(np.float32)
+    s = np.stack([samples_for(q[g], t[g]) for g in range(G)])
+    loss, grad = c.gradient(s)
+    for g in (0, 1, 31, 47):
+        one = ap.RadianceCache(seed=5, groups=1, width=32, precision="fp16")
+        one.set_bounds(*group_bounds(g))
+        one.set_params(p[g])
+        l1, g1 = one.gradient(s[g])
+        print(f"[nrc-group] 48 networks, network {g}: loss {loss[g]:.6g} vs alone {l1:.6g}", flush=True)
+        assert np.array_equal(g1, grad[g]) and l1 == loss[g]
+        assert np.abs(g1).max() > 0.0
+        one.close()
+    c.close()
