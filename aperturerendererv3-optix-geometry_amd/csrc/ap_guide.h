@@ -350,8 +350,15 @@ int guide_update(GuideDevice* g, size_t n_paths, hipStream_t stream);
 // builds can share a wait: begin enqueues up to the read-back of the counts
 // the host needs; after the stream has been synchronised, end enqueues the
 // rest.
-int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t stream);
-int guide_update_end(GuideDevice* g, hipStream_t stream);
+// With capture, begin records no events (its launches go into a graph that
+// is replayed later); after a replay the caller waits for the stream, calls
+// guide_replayed, then guide_update_end with synced.
+int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t stream, bool capture = false);
+int guide_update_end(GuideDevice* g, hipStream_t stream, bool synced = false);
+void guide_replayed(GuideDevice* g);
+// Whether the last update's check left the build to the host (a volume or
+// probe growth, or a re-sort).
+bool guide_needs_host(const GuideDevice* g);
 int guide_reset(GuideDevice* g);
 int guide_export(GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count);
 

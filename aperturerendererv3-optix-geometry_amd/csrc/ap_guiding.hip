@@ -765,7 +765,7 @@ int texel_end_bit(int64_t bound) {
     return std::min(64, 6 + probe_bits + 1);
 }
 
-int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
+int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st, bool capture) {
     g->open = false;
     const size_t n_slots = n_paths * PG_COLLECT;
     if (n_slots == 0) return AP_OK;
@@ -804,7 +804,7 @@ int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
     }
     hipLaunchKernelGGL(check_kernel, dim3(1), dim3(64), 0, st, g->info, cap, g->volume ? 1 : 0, box, g->cap);
     e = hipMemcpyAsync(g->info_host, g->info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipEventRecord(g->ev_flag, st);
+    if (e == hipSuccess && !capture) e = hipEventRecord(g->ev_flag, st);
     if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
     int32_t* info = g->info;
     const uint32_t* n_uniq_dev = reinterpret_cast<const uint32_t*>(info);
@@ -828,26 +828,35 @@ int guide_update_begin(GuideDevice* g, size_t n_paths, hipStream_t st) {
         1, std::min<int64_t>(g->cap, std::max<int64_t>(4096, 2ll * g->count + 1024))));
     hipLaunchKernelGGL(merge_kernel, dim3(merge_grid), dim3(64), 0, st, g->probes, g->temp, 0u, info);
     e = hipMemcpyAsync(g->info_host + kInfoCount, info + kInfoCount, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipEventRecord(g->ev_count, st);
+    if (e == hipSuccess && !capture) e = hipEventRecord(g->ev_count, st);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "guiding: update");
-    g->pending = true;
+    // Captured (a graph replays these launches): no events; the host reads
+    // the counters after it has waited for a replay (guide_update_end with
+    // synced).
+    g->pending = !capture;
     g->open = true;
     g->upd_n = n;
     g->upd_cap = cap;
     return AP_OK;
 }
 
-int guide_update_end(GuideDevice* g, hipStream_t st) {
+int guide_update_end(GuideDevice* g, hipStream_t st, bool synced) {
     if (!g->open) return AP_OK;
     g->open = false;
     int rc;
-    hipError_t e = hipEventSynchronize(g->ev_flag);
+    hipError_t e = synced ? hipSuccess : hipEventSynchronize(g->ev_flag);
     if (e != hipSuccess) return hip_fail(e, "guiding: read counts");
     if (g->info_host[8]) return fail(AP_ERR_UNSUPPORTED, "guiding: a voxel coordinate is outside [-2^18, 2^18)");
     const uint32_t m = static_cast<uint32_t>(g->info_host[9]);
     g->sort_hint = static_cast<uint32_t>(std::min<uint64_t>(g->upd_n, 2ull * m));
-    if (g->info_host[kInfoHost] == 0) return AP_OK;  // the device finishes the build (or there is nothing to build)
+    if (g->info_host[kInfoHost] == 0) {  // the device finished the build (or there was nothing to build)
+        if (synced) {
+            g->count = g->info_host[kInfoCount];
+            g->pending = false;
+        }
+        return AP_OK;
+    }
     // The host finishes it: wait for the stood-down kernels, take the
     // device's probe count (unchanged), then grow what must grow.
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "guiding: read counts");
@@ -944,10 +953,16 @@ int guide_update_end(GuideDevice* g, hipStream_t st) {
 }
 
 int guide_update(GuideDevice* g, size_t n_paths, hipStream_t st) {
-    const int rc = guide_update_begin(g, n_paths, st);
+    const int rc = guide_update_begin(g, n_paths, st, false);
     if (rc != AP_OK || !g->open) return rc;
-    return guide_update_end(g, st);
+    return guide_update_end(g, st, false);
 }
+
+// A captured update was replayed and waited for: its counters are on the host.
+void guide_replayed(GuideDevice* g) {
+    if (g) g->open = true;
+}
+bool guide_needs_host(const GuideDevice* g) { return g && g->info_host && g->info_host[kInfoHost] != 0; }
 
 int guide_export(GuideDevice* g, ap_octahedron* probes, int32_t* volume, int32_t* bounds, int32_t* count) {
     const int rc = settle(g);

@@ -2481,7 +2481,11 @@ struct BatchState {
         uint64_t nrc_generation;
         uint32_t nrc_slots;
         size_t nrc_slot_cap;
+        uint64_t guide_generation;  // guided frames: the guiding sizes the frame was captured with
     };
+    // Renewed whenever a scene's guiding storage or sort size changes, which
+    // the launches of a captured guided frame name.
+    uint64_t guide_generation = 0;
     std::vector<Graph> graphs;
     static constexpr size_t kMaxGraphs = 16;
     hipStream_t last_stream = nullptr;  // stream of the last call (its frames may still run on it)
@@ -2623,7 +2627,53 @@ int batch_prepare(BatchState* b, int tri_mode, float* const* hdr, double* const*
 // (ap_render's per-frame update; the scenes' first halves share one wait),
 // then the scene records with the rebuilt probes and index volumes for the
 // next frame.
-int batch_guides_update(BatchState* b, hipStream_t stream) {
+// Refreshes the scene records whose probes, volume or bounds were
+// reallocated (or whose first probes appeared).
+int batch_guides_views(BatchState* b, hipStream_t stream) {
+    bool changed = false;
+    for (size_t k = 0; k < b->scenes.size(); ++k) {
+        const GuideView v = guide_view(b->scenes[k]->guide);
+        GuideView& o = b->h_scenes[k].guide;
+        bool same = v.probes == o.probes && v.volume == o.volume && (v.probe_count > 0) == (o.probe_count > 0);
+        for (int a = 0; a < 3; ++a) same = same && v.start[a] == o.start[a] && v.size[a] == o.size[a];
+        if (!same) {
+            o = v;
+            changed = true;
+        }
+        b->h_scenes[k].guide_slots = guide_slots(b->scenes[k]->guide);
+        b->h_scenes[k].guide_nvalid = guide_nvalid(b->scenes[k]->guide);
+    }
+    if (!changed) return AP_OK;
+    hipError_t e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
+                                  hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: guiding records");
+}
+
+// After a replayed guided frame and a wait for its stream: the scenes'
+// builds that their device check left to the host are finished; `grew` is
+// set when one was (the captured frame then names stale sizes).
+int batch_guides_after_replay(BatchState* b, hipStream_t stream, bool* grew) {
+    int rc;
+    *grew = false;
+    for (size_t k = 0; k < b->scenes.size(); ++k) {
+        GuideDevice* g = b->scenes[k]->guide;
+        guide_replayed(g);
+        if (guide_needs_host(g)) *grew = true;
+        if ((rc = guide_update_end(g, b->gstreams[k % BatchState::kGuideStreams], true)) != AP_OK) return rc;
+    }
+    for (int k = 0; k < BatchState::kGuideStreams; ++k) {
+        const hipError_t e = b->gstreams[k] ? hipStreamSynchronize(b->gstreams[k]) : hipSuccess;
+        if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding finish");
+    }
+    return batch_guides_views(b, stream);
+}
+
+// Every scene's probe build after a guided batch frame, the builds spread
+// over the guiding streams. With capture (the frame is being captured into
+// a graph) the builds are only enqueued; batch_guides_after_replay finishes
+// them after each replay.
+int batch_guides_update(BatchState* b, hipStream_t stream, bool capture = false) {
     int rc;
     constexpr int K = BatchState::kGuideStreams;
     hipError_t e = hipSuccess;
@@ -2644,37 +2694,20 @@ int batch_guides_update(BatchState* b, hipStream_t stream) {
     if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding fork");
     for (size_t k = 0; k < n; ++k)
         if ((rc = guide_update_begin(b->scenes[k]->guide, b->pix_begin[k + 1] - b->pix_begin[k],
-                                     b->gstreams[k % K])) != AP_OK)
+                                     b->gstreams[k % K], capture)) != AP_OK)
             return rc;
     // Each end waits for its scene's check only; a scene whose volume or
     // probe storage had to grow is finished on the host there.
-    for (size_t k = 0; k < n; ++k)
-        if ((rc = guide_update_end(b->scenes[k]->guide, b->gstreams[k % K])) != AP_OK) return rc;
+    if (!capture)
+        for (size_t k = 0; k < n; ++k)
+            if ((rc = guide_update_end(b->scenes[k]->guide, b->gstreams[k % K])) != AP_OK) return rc;
     for (int k = 0; k < used && e == hipSuccess; ++k) {
         e = hipEventRecord(b->gevents[k], b->gstreams[k]);
         if (e == hipSuccess) e = hipStreamWaitEvent(stream, b->gevents[k], 0);
     }
     if (e != hipSuccess) return hip_fail(e, "ap_batch: guiding join");
-    // The scene records change only when a scene's probes, volume or bounds
-    // were reallocated, or its first probes appeared.
-    bool changed = false;
-    for (size_t k = 0; k < b->scenes.size(); ++k) {
-        const GuideView v = guide_view(b->scenes[k]->guide);
-        GuideView& o = b->h_scenes[k].guide;
-        bool same = v.probes == o.probes && v.volume == o.volume && (v.probe_count > 0) == (o.probe_count > 0);
-        for (int a = 0; a < 3; ++a) same = same && v.start[a] == o.start[a] && v.size[a] == o.size[a];
-        if (!same) {
-            o = v;
-            changed = true;
-        }
-        b->h_scenes[k].guide_slots = guide_slots(b->scenes[k]->guide);
-        b->h_scenes[k].guide_nvalid = guide_nvalid(b->scenes[k]->guide);
-    }
-    if (!changed) return AP_OK;
-    e = hipMemcpyAsync(b->d_scenes, b->h_scenes.data(), b->h_scenes.size() * sizeof(BatchScene),
-                       hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_batch: guiding records");
+    if (capture) return AP_OK;
+    return batch_guides_views(b, stream);
 }
 
 // One batch frame: every scene's paths through every bounce, then the
@@ -2743,6 +2776,8 @@ bool same_key(const BatchState::Graph& g, const BatchState* b, const ap_render_p
     if (ns && (std::memcmp(&g.np, np, sizeof(*np)) != 0 || g.nrc_generation != nrc_generation(ns) ||
                g.nrc_slots != b->nrc_slots || g.nrc_slot_cap != b->nrc_slot_cap))
         return false;
+    if ((k.use_guiding != 0) != (p->use_guiding != 0) || (p->use_guiding && g.guide_generation != b->guide_generation))
+        return false;
     return g.stream == st && k.seed == p->seed && k.max_depth == p->max_depth &&
            k.rr_min_depth == p->rr_min_depth && k.nee_max_depth == p->nee_max_depth && k.use_nee == p->use_nee &&
            k.nee_always == p->nee_always && k.sky == p->sky && k.wavelengths == p->wavelengths &&
@@ -2761,12 +2796,57 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
     const bool four = prm->wavelengths == 4;
     if (prm->use_guiding) {
         // One frame at a time, each sampling the probes the frames before it
-        // built: launched directly (each probe build waits once per scene).
+        // built. A call's first frame runs directly (it sizes the builds'
+        // buffers); the others replay one captured frame with its 48 probe
+        // builds, then the host waits for it and finishes any build whose
+        // device check found that storage must grow (after which the
+        // captured frame is stale and is captured again).
+        auto gframe = [&](bool capture) {
+            int r2 = four ? batch_frame<4, true>(b, prm, stream, rgb_lut, lam_lut, tables)
+                          : batch_frame<1, true>(b, prm, stream, rgb_lut, lam_lut, tables);
+            return r2 == AP_OK ? batch_guides_update(b, stream, capture) : r2;
+        };
         for (int32_t f = 0; f < frames; ++f) {
-            rc = four ? batch_frame<4, true>(b, prm, stream, rgb_lut, lam_lut, tables)
-                      : batch_frame<1, true>(b, prm, stream, rgb_lut, lam_lut, tables);
-            if (rc == AP_OK) rc = batch_guides_update(b, stream);
-            if (rc != AP_OK) return rc;
+            if (f == 0 || !b->use_graph || stream == nullptr) {
+                if ((rc = gframe(false)) != AP_OK) return rc;
+                continue;
+            }
+            hipGraphExec_t gexec = nullptr;
+            for (const BatchState::Graph& g : b->graphs)
+                if (same_key(g, b, prm, stream)) gexec = g.exec;
+            if (!gexec) {
+                if (b->graphs.size() >= BatchState::kMaxGraphs) {
+                    e = hipDeviceSynchronize();
+                    if (e != hipSuccess) return hip_fail(e, "ap_batch: wait before dropping a graph");
+                    (void)hipGraphExecDestroy(b->graphs.front().exec);
+                    (void)hipGraphDestroy(b->graphs.front().graph);
+                    b->graphs.erase(b->graphs.begin());
+                }
+                e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) return hip_fail(e, "ap_batch: begin capture");
+                rc = gframe(true);
+                hipGraph_t g = nullptr;
+                e = hipStreamEndCapture(stream, &g);
+                if (rc != AP_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (e != hipSuccess) return hip_fail(e, "ap_batch: end capture");
+                e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    (void)hipGraphDestroy(g);
+                    return hip_fail(e, "ap_batch: instantiate graph");
+                }
+                b->graphs.push_back(BatchState::Graph{*prm, b->shape.wave_rays, b->shape.refill, stream, g, gexec,
+                                                      nullptr, ap_nrc_render_params{}, 0ull, 0u, 0,
+                                                      b->guide_generation});
+            }
+            e = hipGraphLaunch(gexec, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return hip_fail(e, "ap_batch: guided graph launch");
+            bool grew = false;
+            if ((rc = batch_guides_after_replay(b, stream, &grew)) != AP_OK) return rc;
+            if (grew) ++b->guide_generation;
         }
         return AP_OK;
     }
@@ -2807,7 +2887,7 @@ int batch_run(BatchState* b, const ap_render_params* prm, int32_t frame0, int32_
             return hip_fail(e, "ap_batch: instantiate graph");
         }
         b->graphs.push_back(BatchState::Graph{*prm, b->shape.wave_rays, b->shape.refill, stream, g, exec, nullptr,
-                                              ap_nrc_render_params{}, 0ull, 0u, 0});
+                                              ap_nrc_render_params{}, 0ull, 0u, 0, 0ull});
     }
     for (int32_t f = 0; f < frames; ++f) {
         e = hipGraphLaunch(exec, stream);
@@ -3315,7 +3395,7 @@ int ap_batch_render_nrc(ap_batch* batch, const ap_render_params* params, ap_nrc*
                 return hip_fail(e, "ap_batch_render_nrc: instantiate graph");
             }
             B.graphs.push_back(BatchState::Graph{*params, B.shape.wave_rays, B.shape.refill, st, g, exec, ns,
-                                                 *nrc_params, nrc_generation(ns), B.nrc_slots, B.nrc_slot_cap});
+                                                 *nrc_params, nrc_generation(ns), B.nrc_slots, B.nrc_slot_cap, 0ull});
         }
         e = hipGraphLaunch(exec, st);
         if (e != hipSuccess) return hip_fail(e, "ap_batch_render_nrc: graph launch");

@@ -663,7 +663,11 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
 
     ms_seq, _ = timed(one_by_one)
     # Guided: every scene sampling and rebuilding its own probes each frame
-    # (frames launched one by one, one probe build per scene and frame).
+    # (one probe build per scene and frame, the builds spread over eight
+    # streams); a frame and its 48 builds replayed from one captured graph,
+    # and launched directly.
+    ms_guided_direct, _ = timed(lambda p: batch.render(p, hdrs, stream=stream), use_guiding=1)
+    batch.set_graph(True)
     ms_guided, _ = timed(lambda p: batch.render(p, hdrs, stream=stream), use_guiding=1)
     batch.close()
     for sc in scenes:
@@ -681,7 +685,8 @@ def batch_bench(ap, spp, depth, n_scenes=48, seed=6):
         "direct_launches": {"ms": round(ms_direct, 2), "samples_per_s": round(pix * spp / (ms_direct / 1e3), 1)},
         "scene_by_scene": {"ms": round(ms_seq, 2), "samples_per_s": round(pix * spp / (ms_seq / 1e3), 1)},
         "speedup_over_scene_by_scene": round(ms_seq / ms_graph, 3),
-        "guided": {"ms": round(ms_guided, 2), "samples_per_s": round(pix * spp / (ms_guided / 1e3), 1)},
+        "guided": {"ms": round(ms_guided, 2), "samples_per_s": round(pix * spp / (ms_guided / 1e3), 1),
+                   "direct_launches_ms": round(ms_guided_direct, 2)},
     }
     log(f"batch: {out}")
     return out

@@ -184,9 +184,23 @@ def test_guided_batch_equals_guided_scenes(gpu, wavelengths):
     scene's image, FP64 sums, probes, probe counts and index volume equal,
     bit for bit, to ap_render of the scene alone over the same frames (which
     tests/test_guiding_gpu.py ties to the oracle)."""
+    guided_batch_vs_alone(wavelengths, [(48, 1), (49, 1), (50, 1)])
+
+
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_guided_batch_replayed_frames_equal_guided_scenes(gpu, wavelengths):
+    """Calls of several guided frames: a call's first frame runs directly,
+    the others replay one captured frame (render launches and every scene's
+    probe build), including early frames whose builds must grow probe or
+    volume storage (finished on the host after the replay, the captured
+    frame then captured again); equal to each scene alone, probes included."""
+    guided_batch_vs_alone(wavelengths, [(48, 3), (51, 4)])
+
+
+def guided_batch_vs_alone(wavelengths, calls):
     objs, cams = scenes_and_cameras()
     stream = ap.Stream()
-    kw = dict(spp=1, max_depth=10, rr_min_depth=3, seed=12, wavelengths=wavelengths, use_guiding=1)
+    kw = dict(max_depth=10, rr_min_depth=3, seed=12, wavelengths=wavelengths, use_guiding=1)
     scenes = [ap.SceneRenderer(o) for o in objs]
     batch = ap.Batch(scenes, cams)
     hdrs, sums = [], []
@@ -196,17 +210,17 @@ def test_guided_batch_equals_guided_scenes(gpu, wavelengths):
         h.zero()
         hdrs.append(h)
         sums.append(ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64)))
-    for f in range(48, 51):
-        batch.render(ap.RenderParams.defaults(frame_index=f, **kw), hdrs, sums, stream=stream)
+    for f0, spp in calls:
+        batch.render(ap.RenderParams.defaults(frame_index=f0, spp=spp, **kw), hdrs, sums, stream=stream)
     stream.synchronize()
+    first, last = calls[0][0], calls[-1][0] + calls[-1][1]
     for k, (o, c) in enumerate(zip(objs, cams)):
         n = c.width * c.height
         alone = ap.SceneRenderer(o)
         h = ap.DeviceBuffer(n * 12)
         h.zero()
         sm = ap.DeviceBuffer.from_numpy(np.zeros(n * 3, np.float64))
-        for f in range(48, 51):
-            alone.render(c, ap.RenderParams.defaults(frame_index=f, **kw), h, sum_xyz=sm)
+        alone.render(c, ap.RenderParams.defaults(frame_index=first, spp=last - first, **kw), h, sum_xyz=sm)
         ap.synchronize()
         got = hdrs[k].to_numpy(np.float32, n * 3), sums[k].to_numpy(np.float64, n * 3)
         want = h.to_numpy(np.float32, n * 3), sm.to_numpy(np.float64, n * 3)
