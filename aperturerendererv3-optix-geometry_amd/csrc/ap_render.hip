@@ -260,6 +260,8 @@ void render_state_free(RenderState* r) {
 namespace {
 
 constexpr int kBlock = 256;
+// Lane groups of a shade block: miss, the four shading models, past the queue.
+constexpr uint32_t kShadeGroups = 6;
 
 __device__ __forceinline__ float rand1(Pcg32& r) { return pcg32_float(r); }
 
@@ -746,15 +748,61 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
     ap_ray* __restrict__ shadow_rays, ShadowPay<NL>* __restrict__ shadow_pay, uint32_t* __restrict__ counters,
     int32_t* __restrict__ primary_id, int32_t primary_frame, unsigned long long* __restrict__ ray_count, GuideView gv,
     NrcView nv, BatchView bv = BatchView{}) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[sp.in_q];
-    if (ray_count && i == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
+    const uint32_t first = blockIdx.x * kBlock;
+    if (ray_count && first == 0 && threadIdx.x == 0) atomicAdd(&ray_count[0], static_cast<unsigned long long>(n));
+    if (first >= n) return;   // the whole block
+    // The block's lanes take its queue entries grouped by what they shade:
+    // misses, then hits by shading model (entries past the queue last), in
+    // queue order within a group, so a wave runs one branch of the shading
+    // code instead of all of them. A path's result does not depend on the
+    // lane that shades it; only the order of the queues it appends to does.
+    __shared__ ap_hit s_hit[kBlock];
+    __shared__ uint16_t s_perm[kBlock];
+    __shared__ uint32_t s_count[kBlock / 64][kShadeGroups];
+    {
+        const uint32_t i = first + threadIdx.x;
+        ap_hit h{-1, 0.0f};
+        uint32_t key = kShadeGroups - 1;
+        if (i < n) {
+            h = hits[i];
+            key = 0;
+            if (h.id >= 0) {
+                if constexpr (MULTI) {
+                    key = 1;
+                } else {
+                    const uint32_t model = static_cast<uint32_t>(mats[geo[h.id].mat].shading_model);
+                    key = 1u + (model < 4u ? model : 3u);
+                }
+            }
+        }
+        s_hit[threadIdx.x] = h;
+        const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kShadeGroups; ++k) {
+            const uint64_t m = __ballot(key == k);
+            if (lane == 0) s_count[wave][k] = static_cast<uint32_t>(__popcll(m));
+            if (key == k) rank = static_cast<uint32_t>(__popcll(m & below));
+        }
+        __syncthreads();
+        uint32_t pos = rank;
+        for (uint32_t k = 0; k < key; ++k)
+#pragma unroll
+            for (uint32_t w = 0; w < kBlock / 64; ++w) pos += s_count[w][k];
+        for (uint32_t w = 0; w < wave; ++w) pos += s_count[w][key];
+        s_perm[pos] = static_cast<uint16_t>(threadIdx.x);
+        __syncthreads();
+    }
+    const uint32_t slot = s_perm[threadIdx.x];
+    const uint32_t i = first + slot;
     if (i >= n) return;
     const int32_t pid = path_in[i];
     PathT<NL> S = paths[pid];
     Pcg32 rng{S.rng_state, S.rng_inc};
     const ap_ray R = rays_in[i];
-    ap_hit H = hits[i];
+    ap_hit H = s_hit[slot];
     const f3 o{R.ox, R.oy, R.oz}, d{R.dx, R.dy, R.dz};
     const Spec<NL> sc = spec_of<NL>(S.lam, lam_lut, tables);
     const f4 lb = sc.lb[0];    // the hero's basis: every sampling decision uses it
