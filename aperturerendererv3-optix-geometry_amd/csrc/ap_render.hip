@@ -2118,10 +2118,13 @@ int run_frames(const RunCtx& c) {
     // sample (128: 4.98, 1024: 4.65, 2048: 4.72) and guided frames (2M paths)
     // 10.0 ms (128: 10.2); with 4M paths per batch 128 had been best.
     constexpr int32_t kQueueWaveRays = 512;
-    // A pipelined frame starts the next one once its first kLeadBounces
+    // A pipelined frame starts the next one once its first lead_bounces
     // bounces are launched: from there on its queues are short and the GPU
-    // has room for the next frame's camera rays.
-    constexpr int32_t kLeadBounces = 2;
+    // has room for the next frame's camera rays. Measured at 1080p with four
+    // wavelengths: guided frames 8.87 ms when the next frame starts after
+    // bounce 2, 8.80 after 3, 8.60 after 4, 8.62 after 5; cache frames, whose
+    // paths end sooner, are fastest after bounce 2 (5.17 ms; 5.35 after 4).
+    const int32_t lead_bounces = nrc ? 2 : 4;
     DeviceScene qs = s->dev;
     qs.wave_rays = std::min(qs.wave_rays, kQueueWaveRays);
     auto frame_set = [&](int k) {
@@ -2164,7 +2167,7 @@ int run_frames(const RunCtx& c) {
     // The rest of a batch of paths on the render stream: a rendered batch of
     // frames, or (train_only) the training paths of one pass of the radiance
     // cache, which renders nothing into the framebuffers. at_tail runs once
-    // the first kLeadBounces bounces are launched.
+    // the first lead_bounces bounces are launched.
     auto finish_batch = [&](const FrameParams& fp, const FrameSet<NL>& F, bool train_only, uint32_t n_offset,
                             uint32_t n_train, uint64_t train_seed, const std::function<int()>& at_tail) -> int {
         const uint32_t total = static_cast<uint32_t>(fp.frames) * static_cast<uint32_t>(fp.n_pixels);
@@ -2228,7 +2231,7 @@ int run_frames(const RunCtx& c) {
                                r->shadow_hits, F.paths, rc_ptr);
             in_q = out_q;
             if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, F.counters, 1 - in_q);
-            if (depth + 1 == kLeadBounces) {
+            if (depth + 1 == lead_bounces) {
                 tail_started = true;
                 if ((rc2 = at_tail()) != AP_OK) return rc2;
             }
