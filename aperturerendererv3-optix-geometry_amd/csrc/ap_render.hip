@@ -49,7 +49,7 @@ struct alignas(16) GeoRec {
     float e1[3]; int32_t mat;        // MatRec index
     float e2[3]; int32_t analytic;   // AnalyticRec index (cube / sphere), else -1
     float n0[3]; int32_t light_index;
-    float n1[3]; int32_t pad0;
+    float n1[3]; int32_t model;      // the material's shading model (groups shading lanes)
     float n2[3]; int32_t pad1;
 };
 static_assert(sizeof(GeoRec) == 96, "GeoRec is 96 B");
@@ -768,12 +768,10 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(
             h = hits[i];
             key = 0;
             if (h.id >= 0) {
-                if constexpr (MULTI) {
-                    key = 1;
-                } else {
-                    const uint32_t model = static_cast<uint32_t>(mats[geo[h.id].mat].shading_model);
-                    key = 1u + (model < 4u ? model : 3u);
-                }
+                const GeoRec* g = geo;
+                if constexpr (MULTI) g = bv.scenes[bv.ray_scene[sp.in_q][i]].geo;
+                const uint32_t model = static_cast<uint32_t>(g[h.id].model);
+                key = 1u + (model < 4u ? model : 3u);
             }
         }
         s_hit[threadIdx.x] = h;
@@ -1859,6 +1857,7 @@ int build_shading(ap_scene* s, RenderState* r, int tri_mode, const f4* lut) {
         g.mat = intern(m);
         g.analytic = analytic_of[i];
         g.light_index = o.light_index;
+        g.model = m.model;
         if (o.type == AP_OBJ_TRIANGLE) {
             const m3 o2w = M3(o.object_to_world);
             const f3 c = F(o.center);
