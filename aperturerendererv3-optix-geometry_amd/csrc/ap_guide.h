@@ -300,16 +300,20 @@ AP_HD void pg_generate_mipmaps(ap_octahedron& p) {
 }
 
 // Per-path radiance collection (CalculateRadiance's radianceSample* arrays),
-// with room for four wavelengths per path (index 0 alone with one).
+// with room for four wavelengths per path (index 0 alone with one). A slot's
+// radiance and throughput are 16-B aligned rows (one vector access each).
+// Only `mask` and `valid` must start at zero: a slot's other fields are
+// written when it opens, and only open or valid slots are read.
 struct alignas(16) GuideRec {
+    float sample[PG_COLLECT][4];
+    float mask[PG_COLLECT][4];
     float dir[PG_COLLECT][3];
     float loc[PG_COLLECT][3];
     float vdir[PG_COLLECT][3];
-    float sample[PG_COLLECT][4];
-    float mask[PG_COLLECT][4];
     uint32_t valid;   // bit i: slot i collected radiance
-    uint32_t pad[3];
+    uint32_t pad[2];
 };
+static_assert(sizeof(GuideRec) == 352, "GuideRec is 352 B");
 
 // Device view of the guiding state for the shading kernel.
 struct GuideView {

@@ -302,13 +302,38 @@ template <int NL>
 __device__ __forceinline__ f3 path_xyz(const float lam[NL], const float* L, float psum,
                                        const float* __restrict__ tables);
 
+// A slot row of a guiding record in registers: NL = 4 moves it as one
+// 16-B vector, fewer wavelengths element by element.
+template <int NL>
+__device__ __forceinline__ float4 pg_row(const float (&row)[4]) {
+    if constexpr (NL == 4) return *reinterpret_cast<const float4*>(row);
+    float4 v{0.0f, 0.0f, 0.0f, 0.0f};
+    v.x = row[0];
+    if constexpr (NL > 1) v.y = row[1];
+    if constexpr (NL > 2) v.z = row[2];
+    return v;
+}
+
+template <int NL>
+__device__ __forceinline__ void pg_set_row(float (&row)[4], float4 v) {
+    if constexpr (NL == 4) {
+        *reinterpret_cast<float4*>(row) = v;
+        return;
+    }
+    row[0] = v.x;
+    if constexpr (NL > 1) row[1] = v.y;
+    if constexpr (NL > 2) row[2] = v.z;
+}
+
 // CollectRadiance (render.cuh:353-432) on a path's guiding record. A slot
 // is open while any of its wavelengths' throughputs is non-zero (with one
 // wavelength, the reference's mask[i] != 0).
 template <int NL>
-__device__ __forceinline__ bool pg_open(const GuideRec& r, int i) {
-    bool open = false;
-    for (int j = 0; j < NL; ++j) open = open || r.mask[i][j] != 0.0f;
+__device__ __forceinline__ bool pg_open_row(float4 m) {
+    bool open = m.x != 0.0f;
+    if constexpr (NL > 1) open = open || m.y != 0.0f;
+    if constexpr (NL > 2) open = open || m.z != 0.0f;
+    if constexpr (NL > 3) open = open || m.w != 0.0f;
     return open;
 }
 
@@ -318,29 +343,55 @@ __device__ __forceinline__ void pg_endpoint(GuideRec& r, const float* le, int nd
     for (int j = 0; j < NL; ++j) any = any || le[j] != 0.0f;
     if (!any) return;
     const int hi = ndd < PG_COLLECT ? ndd : PG_COLLECT;
-    for (int i = 0; i < hi; ++i)
-        if (pg_open<NL>(r, i)) {
-            for (int j = 0; j < NL; ++j) r.sample[i][j] = r.sample[i][j] + r.mask[i][j] * le[j];
+    for (int i = 0; i < hi; ++i) {
+        const float4 m = pg_row<NL>(r.mask[i]);
+        if (pg_open_row<NL>(m)) {
+            float4 smp = pg_row<NL>(r.sample[i]);
+            smp.x = smp.x + m.x * le[0];
+            if constexpr (NL > 1) smp.y = smp.y + m.y * le[1];
+            if constexpr (NL > 2) smp.z = smp.z + m.z * le[2];
+            if constexpr (NL > 3) smp.w = smp.w + m.w * le[3];
+            pg_set_row<NL>(r.sample[i], smp);
             r.valid |= 1u << i;
         }
+    }
 }
 
 template <int NL>
 __device__ __forceinline__ void pg_advance(GuideRec& r, f3 next, const float* contrib, float rr_prob, int ndd,
                                            bool allow, f3 ray_dir, f3 pos, f3 nrm) {
-    if (allow && ndd < PG_COLLECT && !pg_open<NL>(r, ndd)) {
+    // A slot opens at most once (ndd grows past it), so its radiance starts
+    // at zero here rather than when the path starts.
+    const float4 ones{1.0f, 1.0f, 1.0f, 1.0f}, zeros{0.0f, 0.0f, 0.0f, 0.0f};
+    if (allow && ndd < PG_COLLECT && !pg_open_row<NL>(pg_row<NL>(r.mask[ndd]))) {
         r.dir[ndd][0] = next.x; r.dir[ndd][1] = next.y; r.dir[ndd][2] = next.z;
         r.loc[ndd][0] = pos.x; r.loc[ndd][1] = pos.y; r.loc[ndd][2] = pos.z;
         const f3 v = pg_reflect(ray_dir, nrm);
         r.vdir[ndd][0] = v.x; r.vdir[ndd][1] = v.y; r.vdir[ndd][2] = v.z;
-        for (int j = 0; j < NL; ++j) r.mask[ndd][j] = 1.0f;
+        pg_set_row<NL>(r.sample[ndd], zeros);
+        pg_set_row<NL>(r.mask[ndd], ones);
     }
     const float inv_q = static_cast<float>(1.0 / static_cast<double>(rr_prob));
     const int up = ndd + (allow ? 1 : 0);
     const int hi = up < PG_COLLECT ? up : PG_COLLECT;
-    for (int i = 0; i < hi; ++i)
-        if (pg_open<NL>(r, i))
-            for (int j = 0; j < NL; ++j) r.mask[i][j] = r.mask[i][j] * (contrib[j] * inv_q);
+    for (int i = 0; i < hi; ++i) {
+        float4 m = pg_row<NL>(r.mask[i]);
+        if (pg_open_row<NL>(m)) {
+            m.x = m.x * (contrib[0] * inv_q);
+            if constexpr (NL > 1) m.y = m.y * (contrib[1] * inv_q);
+            if constexpr (NL > 2) m.z = m.z * (contrib[2] * inv_q);
+            if constexpr (NL > 3) m.w = m.w * (contrib[3] * inv_q);
+            pg_set_row<NL>(r.mask[i], m);
+        }
+    }
+}
+
+// A path's guiding record at its start: no slot open, none valid.
+__device__ __forceinline__ void pg_clear(GuideRec& r) {
+    const float4 zeros{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < PG_COLLECT; ++i) *reinterpret_cast<float4*>(r.mask[i]) = zeros;
+    r.valid = 0u;
 }
 
 // Path end (render.cuh:910-957): valid collection slots to guiding samples,
@@ -494,11 +545,7 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(CamParams cam, FramePa
     s.rng_state = rng.state;
     s.rng_inc = rng.inc;
     paths[i] = s;
-    if (recs) {
-        GuideRec z;
-        __builtin_memset(&z, 0, sizeof(z));
-        recs[i] = z;
-    }
+    if (recs) pg_clear(recs[i]);
     rays[i] = ap_ray{cam.origin.x, cam.origin.y, cam.origin.z, eps[0], d.x, d.y, d.z, __int_as_float(0x7f800000)};
     path_of[i] = static_cast<int32_t>(i);
 }
@@ -1566,11 +1613,7 @@ __global__ __launch_bounds__(kBlock) void generate_batch_kernel(BatchView bv, ui
     st.rng_state = rng.state;
     st.rng_inc = rng.inc;
     paths[i] = st;
-    if (recs) {
-        GuideRec z;
-        __builtin_memset(&z, 0, sizeof(z));
-        recs[i] = z;
-    }
+    if (recs) pg_clear(recs[i]);
     rays[i] = ap_ray{origin.x, origin.y, origin.z, eps[0], d.x, d.y, d.z, __int_as_float(0x7f800000)};
     path_of[i] = static_cast<int32_t>(i);
     bv.ray_scene[0][i] = static_cast<uint16_t>(sc);
