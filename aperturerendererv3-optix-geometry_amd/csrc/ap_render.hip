@@ -2227,10 +2227,12 @@ int run_frames(const RunCtx& c) {
         }
         auto* rc_ptr = train_only ? nullptr : reinterpret_cast<unsigned long long*>(fb->ray_count);
         int32_t* prim = train_only ? nullptr : fb->primary_id;
-        // Bounce k: shade, then the shadow rays on the side stream while the
+        // Bounce k: shade, then on the side stream the shadow rays, their
+        // resolve and the counter resets for the next shade (the queue this
+        // shade read, the shadow queue after resolve has read it), while the
         // render stream traces bounce k + 1's extension rays (independent
-        // queues and outputs), joined before resolve. The shadow counter is
-        // cleared for the next shade only after resolve has read it.
+        // queues and outputs, the path states untouched); joined before the
+        // next shade.
         int in_q = 0;
         hipError_t e = hipSuccess;
         bool tail_started = false;
@@ -2261,7 +2263,12 @@ int run_frames(const RunCtx& c) {
             if (e == hipSuccess)
                 e = launch_trace(qs, r->shadow_rays, total, r->shadow_hits, nullptr, true, r->side, &F.counters[2],
                                  sdf_eps_of_depth(depth));
-            if (e == hipSuccess) e = hipEventRecord(r->ev_shadowed, r->side);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, r->side, F.counters, pays,
+                                   r->shadow_hits, F.paths, rc_ptr);
+                if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, r->side, F.counters, in_q);
+                e = hipEventRecord(r->ev_shadowed, r->side);
+            }
             if (e != hipSuccess) return hip_fail(e, "ap_render: shadow");
             if (more) {
                 e = launch_trace(qs, F.rays[out_q], total, F.hits, nullptr, false, stream, &F.counters[out_q]);
@@ -2269,10 +2276,7 @@ int run_frames(const RunCtx& c) {
             }
             e = hipStreamWaitEvent(stream, r->ev_shadowed, 0);
             if (e != hipSuccess) return hip_fail(e, "ap_render: join");
-            hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, F.counters, pays,
-                               r->shadow_hits, F.paths, rc_ptr);
             in_q = out_q;
-            if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, F.counters, 1 - in_q);
             if (depth + 1 == lead_bounces) {
                 tail_started = true;
                 if ((rc2 = at_tail()) != AP_OK) return rc2;
@@ -2841,7 +2845,12 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
         if (e == hipSuccess)
             e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
                                    b->side, &b->counters[2], sdf_eps_of_depth(depth));
-        if (e == hipSuccess) e = hipEventRecord(b->ev_shadowed, b->side);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, b->side, b->counters, pays,
+                               b->shadow_hits, paths, b->rays_traced);
+            if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, b->side, b->counters, in_q);
+            e = hipEventRecord(b->ev_shadowed, b->side);
+        }
         if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
         if (more) {
             e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[out_q], b->rays[out_q], total, b->hits, false,
@@ -2850,10 +2859,7 @@ int batch_frame(BatchState* b, const ap_render_params* prm, hipStream_t stream, 
         }
         e = hipStreamWaitEvent(stream, b->ev_shadowed, 0);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: join");
-        hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
-                           b->shadow_hits, paths, b->rays_traced);
         in_q = out_q;
-        if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, 1 - in_q);
     }
     hipLaunchKernelGGL(accumulate_batch_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, bv, total, paths,
                        tables, gv.recs);
@@ -3110,7 +3116,12 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
         if (e == hipSuccess)
             e = launch_trace_multi(b->shape, b->d_devs, b->shadow_scene, b->shadow_rays, total, b->shadow_hits, true,
                                    b->side, &b->counters[2], sdf_eps_of_depth(depth));
-        if (e == hipSuccess) e = hipEventRecord(b->ev_shadowed, b->side);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, b->side, b->counters, pays,
+                               b->shadow_hits, paths, b->rays_traced);
+            if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, b->side, b->counters, in_q);
+            e = hipEventRecord(b->ev_shadowed, b->side);
+        }
         if (e != hipSuccess) return hip_fail(e, "ap_batch: shadow");
         if (more) {
             e = launch_trace_multi(b->shape, b->d_devs, b->ray_scene[out_q], b->rays[out_q], total, b->hits, false,
@@ -3119,10 +3130,7 @@ int batch_frame_nrc(BatchState* b, const ap_render_params* prm, const ap_nrc_ren
         }
         e = hipStreamWaitEvent(stream, b->ev_shadowed, 0);
         if (e != hipSuccess) return hip_fail(e, "ap_batch: join");
-        hipLaunchKernelGGL(resolve_kernel<NL>, grid_of(total), dim3(kBlock), 0, stream, b->counters, pays,
-                           b->shadow_hits, paths, b->rays_traced);
         in_q = out_q;
-        if (more) hipLaunchKernelGGL(begin_bounce_kernel, dim3(1), dim3(64), 0, stream, b->counters, 1 - in_q);
     }
     // Cache vertices: every scene's queries to its own network, one inference for all.
     int rc = nrc_infer_groups(ns, b->nrc_queries, b->nrc_qpath, b->d_pix_begin, b->nrc_tile0, b->nrc_qcount,
