@@ -195,6 +195,7 @@ EXPORTED_SYMBOLS = (
     "ap_texture_cache_create", "ap_texture_cache_destroy", "ap_texture_cache_add_rgba8", "ap_texture_cache_count",
     "ap_texture_sample", "ap_scene_set_texture_cache", "ap_mesh_load_obj",
     "ap_mse_f64", "ap_mse_luminance_f64", "ap_relmse_f64", "ap_relmse_f64_noalloc",
+    "ap_guiding_scan_u32", "ap_guiding_sort_pairs_u64",
 )
 
 
@@ -258,6 +259,8 @@ def lib():
         L.ap_scene_set_texture_cache.argtypes = [vp, vp]
         L.ap_mesh_load_obj.argtypes = [ctypes.c_char_p, vp, ctypes.c_float, vp, i32, vp, sz, vp]
         L.ap_guiding_export.argtypes = [vp, vp, sz, vp, sz, vp, vp]
+        L.ap_guiding_scan_u32.argtypes = [vp, vp, sz, vp]
+        L.ap_guiding_sort_pairs_u64.argtypes = [vp, vp, vp, vp, sz, i32, i32]
         L.ap_scene_export_bvh.argtypes = [vp, vp, vp, vp]
         for name in ("ap_trace_closest", "ap_trace_any"):
             getattr(L, name).argtypes = [vp, vp, sz, vp, vp]
@@ -577,6 +580,31 @@ def _reduce_result(d_out, fresh, stream):
         stream.synchronize()
     synchronize()
     return float(d_out.to_numpy(np.float64, 1)[0])
+
+
+def guiding_scan_u32(values):
+    """ap_guiding_scan_u32 of a host uint32 array: (exclusive prefix sums,
+    total), computed on the device."""
+    a = np.ascontiguousarray(values, dtype=np.uint32)
+    n = a.size
+    d_in = DeviceBuffer.from_numpy(a) if n else DeviceBuffer(4)
+    d_out = DeviceBuffer(max(4 * n, 4))
+    d_tot = DeviceBuffer(4)
+    check(lib().ap_guiding_scan_u32(d_in.ptr, d_out.ptr, n, d_tot.ptr))
+    return d_out.to_numpy(np.uint32, n), int(d_tot.to_numpy(np.uint32, 1)[0])
+
+
+def guiding_sort_pairs_u64(keys, vals, begin_bit=0, end_bit=64):
+    """ap_guiding_sort_pairs_u64 of host uint64 keys and uint32 values:
+    (sorted keys, values in the same order), stable, on the device."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    v = np.ascontiguousarray(vals, dtype=np.uint32)
+    n = k.size
+    dk = DeviceBuffer.from_numpy(k) if n else DeviceBuffer(8)
+    dv = DeviceBuffer.from_numpy(v) if n else DeviceBuffer(4)
+    ok, ov = DeviceBuffer(max(8 * n, 8)), DeviceBuffer(max(4 * n, 4))
+    check(lib().ap_guiding_sort_pairs_u64(dk.ptr, ok.ptr, dv.ptr, ov.ptr, n, begin_bit, end_bit))
+    return ok.to_numpy(np.uint64, n), ov.to_numpy(np.uint32, n)
 
 
 def mse_f64(a, b, n, out=None, stream=None):

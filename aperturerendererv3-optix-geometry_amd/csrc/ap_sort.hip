@@ -4,7 +4,8 @@
 // per lane, contiguous): (1) each block's total; (2) one 1024-lane block scans
 // the totals in chunks, carrying the running sum; (3) each block scans its
 // items (lane totals by a wave64 shuffle scan, the four waves' totals through
-// LDS) and adds its offset.
+// LDS) and adds its offset. Up to 65,536 items (the radix sort's digit
+// counts, for instance) one 1024-lane block scans them all in one launch.
 //
 // Radix sort, per 8-bit digit: (1) each block of 256 lanes histograms a tile
 // of 4096 keys in LDS and writes its counts digit-major (count[d * blocks +
@@ -19,6 +20,9 @@
 // leave a block in input order and blocks follow each other, so every pass is
 // stable.
 #include "ap_sort.h"
+
+#include "aperture/ap_guiding.h"
+#include "ap_internal.h"
 
 namespace ap {
 namespace {
@@ -88,6 +92,36 @@ __global__ __launch_bounds__(1024) void scan_offsets_kernel(uint32_t* __restrict
         uint32_t chunk;
         const uint32_t ex = block_excl_scan<1024>(v, &chunk);
         if (i < nb) totals[i] = carry + ex;
+        carry += chunk;
+    }
+    if (total && threadIdx.x == 0) *total = carry;
+}
+
+// A scan of at most kSmallScan items in one launch: one 1024-lane block
+// walks the items in chunks of 1024 x kSmallItems (kSmallItems consecutive
+// items per lane), carrying the running sum; in place is allowed (a lane
+// reads its items before it writes them).
+constexpr int kSmallItems = 8;
+constexpr uint32_t kSmallScan = 65536;
+__global__ __launch_bounds__(1024) void scan_small_kernel(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                          const uint32_t* __restrict__ dyn_n,
+                                                          uint32_t* __restrict__ total) {
+    n = count_of(n, dyn_n);
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < n; c += 1024u * kSmallItems) {
+        const uint32_t base = c + threadIdx.x * kSmallItems;
+        uint32_t v[kSmallItems];
+        uint32_t s = 0;
+        for (int k = 0; k < kSmallItems; ++k) {
+            v[k] = base + k < n ? in[base + k] : 0u;
+            s += v[k];
+        }
+        uint32_t chunk;
+        uint32_t run = carry + block_excl_scan<1024>(s, &chunk);
+        for (int k = 0; k < kSmallItems; ++k) {
+            if (base + k < n) out[base + k] = run;
+            run += v[k];
+        }
         carry += chunk;
     }
     if (total && threadIdx.x == 0) *total = carry;
@@ -200,6 +234,10 @@ hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t n, uin
         if (total) hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, stream, total, 0u);
         return hipGetLastError();
     }
+    if (n <= kSmallScan) {
+        hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(1024), 0, stream, in, out, n, dyn_n, total);
+        return hipGetLastError();
+    }
     const uint32_t nb = div_up(n, kScanTile);
     auto* totals = static_cast<uint32_t*>(work);
     hipLaunchKernelGGL(scan_totals_kernel, dim3(nb), dim3(kScanThreads), 0, stream, in, n, dyn_n, totals);
@@ -244,3 +282,34 @@ hipError_t radix_sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, con
 }
 
 }  // namespace ap
+
+extern "C" {
+
+int ap_guiding_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total) {
+    using namespace ap;
+    if ((n > 0 && (!in || !out)) || n > 0xffffffffull) return fail(AP_ERR_INVALID_ARG, "ap_guiding_scan_u32: bad arguments");
+    void* work = nullptr;
+    hipError_t e = hipMalloc(&work, scan_workspace_bytes(n) + 256);
+    if (e == hipSuccess) e = scan_exclusive_u32(in, out, static_cast<uint32_t>(n), total, work, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (work) (void)hipFree(work);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_guiding_scan_u32");
+}
+
+int ap_guiding_sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                              size_t n, int32_t begin_bit, int32_t end_bit) {
+    using namespace ap;
+    if ((n > 0 && (!keys_in || !keys_out || !vals_in || !vals_out)) || n > 0xffffffffull || begin_bit < 0 ||
+        end_bit > 64 || end_bit <= begin_bit)
+        return fail(AP_ERR_INVALID_ARG, "ap_guiding_sort_pairs_u64: bad arguments");
+    void* work = nullptr;
+    hipError_t e = hipMalloc(&work, sort_workspace_bytes(n) + 256);
+    if (e == hipSuccess)
+        e = radix_sort_pairs_u64(keys_in, keys_out, vals_in, vals_out, static_cast<uint32_t>(n), begin_bit, end_bit,
+                                 work, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (work) (void)hipFree(work);
+    return e == hipSuccess ? AP_OK : hip_fail(e, "ap_guiding_sort_pairs_u64");
+}
+
+}  // extern "C"

@@ -32,6 +32,17 @@ int ap_guiding_reset(ap_scene* scene);
 int ap_guiding_export(const ap_scene* scene, ap_octahedron* probes, size_t probe_capacity, int32_t* volume,
                       size_t volume_capacity, int32_t* bounds, int32_t* probe_count);
 
+/* The guiding build's device primitives, for their tests (no reference
+ * counterpart: the reference calls thrust). Device pointers; the workspace
+ * is allocated inside and the device synchronised before returning.
+ * ap_guiding_scan_u32: out[i] = in[0] + ... + in[i-1] (out may equal in),
+ * *total (device, may be NULL) = the sum of all n; n < 2^32.
+ * ap_guiding_sort_pairs_u64: stable LSD radix sort of (key, value) pairs by
+ * key bits [begin_bit, end_bit), 8 bits per pass; inputs left unchanged. */
+int ap_guiding_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total);
+int ap_guiding_sort_pairs_u64(const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                              size_t n, int32_t begin_bit, int32_t end_bit);
+
 #ifdef __cplusplus
 }
 #endif
