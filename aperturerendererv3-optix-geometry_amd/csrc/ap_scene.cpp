@@ -312,7 +312,9 @@ int ap_scene_autotune(ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hit
                       int32_t* best_wave_rays, int32_t* best_refill, float* best_ms) {
     if (!scene) return fail(AP_ERR_INVALID_ARG, "ap_scene_autotune: NULL scene");
     if (n == 0 || !rays || !hits) return fail(AP_ERR_INVALID_ARG, "ap_scene_autotune: empty or NULL ray batch");
-    static const int32_t kWaveRays[] = {128, 256, 512, 1024};
+    // Steps of 64-128 rays: the best chunk differs by ray set (4096^2
+    // primaries of the 1M-triangle soup: 512 rays, 1513 Mrays/s; 256: 1386).
+    static const int32_t kWaveRays[] = {128, 192, 256, 320, 384, 448, 512, 640, 768, 1024};
     static const int32_t kRefill[] = {4, 8, 16};
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;

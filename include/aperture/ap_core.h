@@ -115,17 +115,18 @@ int ap_scene_set_traversal_mode(ap_scene* scene, int mode);
  * keeps the current setting. Results are identical for every shape; only
  * speed changes. No reference counterpart (the reference launches one
  * thread per pixel). Defaults: 512 rays, refill at 16 idle lanes. The
- * renderer's own path queues (ap_render) use at most 128 rays per wave,
- * dealt round-robin over the XCDs; host batches (ap_trace_*) use the shape
- * as set, XCD by XCD. */
+ * renderer's own path queues (ap_render) use at most 512 rays per wave.
+ * Consecutive chunks of rays go to consecutive waves, dealt round-robin
+ * over the XCDs. */
 int ap_scene_set_launch(ap_scene* scene, int32_t wave_rays, int32_t refill);
 int ap_scene_get_launch(const ap_scene* scene, int32_t* wave_rays, int32_t* refill);
 
 /* Autotuner of the launch shape: traces the caller's device ray batch
  * (closest hit, or any hit when any_hit != 0) into `hits` under each
- * candidate shape (rays per wave 128/256/512/1024 x refill 4/8/16; one
- * untimed launch, then the faster of two launches timed with HIP events on
- * `stream`), keeps the fastest on the scene and reports it. Synchronises
+ * candidate shape (rays per wave 128, 192, 256, 320, 384, 448, 512, 640,
+ * 768 or 1024 x refill 4/8/16; one untimed launch, then the faster of two
+ * launches timed with HIP events on `stream`), keeps the fastest on the
+ * scene and reports it. Synchronises
  * `stream`. The hits written are those of an ordinary trace. No reference
  * counterpart. */
 int ap_scene_autotune(ap_scene* scene, const ap_ray* rays, size_t n, ap_hit* hits, int any_hit, void* stream,
