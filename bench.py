@@ -365,11 +365,13 @@ def materials_bench(ap, spp, stream, W=1920, H=1080):
         sc = ap.SceneRenderer(o)
         sc.set_texture_cache(tex)
         for wl in (1, 4):
-            sc.render(cam, ap.RenderParams.defaults(spp=1, wavelengths=wl, **params), hdr, stream=stream)
+            # An untimed call of the same shape (frames 1..spp), then frames spp+1..2spp timed.
+            sc.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, wavelengths=wl, **params), hdr,
+                      stream=stream)
             stream.synchronize()
             e0, e1 = ap.Event(), ap.Event()
             e0.record(stream)
-            sc.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, wavelengths=wl, **params), hdr,
+            sc.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1 + spp, wavelengths=wl, **params), hdr,
                       stream=stream)
             e1.record(stream)
             stream.synchronize()
@@ -467,8 +469,9 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     wavelengths per path, and with directional guiding (one and four
     wavelengths; probes rebuilt after every frame, the rebuild inside the
     timed region).
-    Frames are timed with HIP events on the render stream after one untimed
-    frame."""
+    Each mode renders `spp` untimed frames in one call, then `spp` more in
+    one call timed with HIP events on the render stream (guided modes start
+    from a probe reset and one untimed frame before both)."""
     t0 = time.perf_counter()
     objs = ap.synth_heightfield_scene(1024, 256, 4, 20.0)
     scene = ap.SceneRenderer(objs)
@@ -484,11 +487,17 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     stream.synchronize()
 
     def timed(**kw):
+        # An untimed call of the same shape first (frames 1..spp): buffers
+        # sized for a batch of spp frames, and for guided frames the second
+        # frame set of the two in flight, are allocated outside the timed
+        # call, which renders frames spp+1..2spp.
+        scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, **params, **kw), hdr, stream=stream)
+        stream.synchronize()
         counts.zero(stream)
         e0, e1 = ap.Event(), ap.Event()
         t0 = time.perf_counter()
         e0.record(stream)
-        scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, **params, **kw), hdr, stream=stream,
+        scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1 + spp, **params, **kw), hdr, stream=stream,
                      ray_count=counts)
         e1.record(stream)
         stream.synchronize()
@@ -530,10 +539,13 @@ def render_bench(ap, spp, cpu_seconds, threads, nrc_spp=0, nrc_ref_spp=2048, nrc
     stats = ap.DeviceBuffer(W * H * ap.SCREEN_STATISTICS_DTYPE.itemsize)
     gbuf.zero(stream)
     stats.zero(stream)
-    e0, e1 = ap.Event(), ap.Event()
-    e0.record(stream)
     scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1, wavelengths=4, **params), hdr, stream=stream,
                  gbuffer=gbuf, statistics=stats)
+    stream.synchronize()
+    e0, e1 = ap.Event(), ap.Event()
+    e0.record(stream)
+    scene.render(cam, ap.RenderParams.defaults(spp=spp, frame_index=1 + spp, wavelengths=4, **params), hdr,
+                 stream=stream, gbuffer=gbuf, statistics=stats)
     e1.record(stream)
     stream.synchronize()
     with_records = {"ms_per_spp": round(e0.elapsed_ms(e1) / spp, 3)}
