@@ -92,7 +92,7 @@ __global__ __launch_bounds__(WAVE) void trace_kernel(DeviceScene s, const ap_ray
     // eighth instead (for its L2) leaves the XCDs unequally loaded wherever
     // cost varies across the batch: an integrator queue in pixel order (8.9
     // against 10.4 ms per sample at 1080p), and the 4096^2 primaries of the
-    // 1M-triangle soup (1484 against 1171 Mrays/s at 448 / 512 rays per wave).
+    // 1M-triangle soup (1513 against 1171 Mrays/s at 512 rays per wave).
     const uint32_t wave_rank = b;
     uint32_t chunk = static_cast<uint32_t>(s.wave_rays);
     {
